@@ -1,0 +1,517 @@
+/*
+ * Test infrastructure (see ce_oracle.h): CPU restatement of the reference
+ * cluster-expansion updater and of the samplers' Metropolis step.  The loop
+ * structure and floating-point operation order follow the reference so the
+ * correlation functions and energies reproduce it bit for bit:
+ *   update_cf                 cpp/src/ce_updater.cpp:313-406
+ *   spin_product_one_atom     cpp/src/ce_updater.cpp:244-285
+ *   sort_indices              cpp/src/ce_updater.cpp:981-994
+ *   get_energy / dot          cpp/src/ce_updater.cpp:236-242, cpp/src/named_array.cpp:25-33
+ *   calculate dispatch        cpp/src/ce_updater.cpp:471-550, 831-852, 996-1000
+ *   undo / clear              cpp/src/ce_updater.cpp:408-448, 552-555
+ *   history ring buffer       cpp/src/cf_history_tracker.cpp:46-149 (max_history 1000)
+ *   Metropolis test           cemc/mcmc/montecarlo.py:910-956
+ *   canonical proposal        cemc/mcmc/montecarlo.py:890-908,
+ *                             cemc/mcmc/swap_move_index_tracker.py:39-74
+ *   SGC proposal              cemc/mcmc/sgc_montecarlo.py:62-76
+ * Deliberate difference: a flip of a background site is rejected before the
+ * history slot is taken (the reference advances the ring first, ce_updater.cpp:323-332).
+ */
+#include "ce_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define CEO_MAX_HISTORY 1000
+#define CEO_MAX_CLUSTER 4
+#define CEO_KB 8.617330337217213e-05
+
+static _Thread_local char g_err[256];
+
+static int fail(int code, const char* msg) {
+  snprintf(g_err, sizeof g_err, "%s", msg);
+  return code;
+}
+
+const char* ceo_last_error(void) { return g_err; }
+
+struct ceo_model {
+  int n_sites, n_species, n_bfs, n_groups, n_cf, n_clusters, K;
+  double* basis;
+  int* site_group;
+  int* group_count;
+  int* cf_kind;
+  int* cf_dec;
+  int* cf_cluster;
+  int* cf_eq_off;
+  int* eq_decos;
+  int* cl_size;
+  int* cl_M;
+  int* cl_norm;
+  int* cl_mem_off;
+  int* members;
+  int* cl_ord_off;
+  int* order;
+  int* trans;
+};
+
+typedef struct {
+  int site, old_id, new_id;
+} ceo_change;
+
+struct ceo_chain {
+  const ceo_model* m;
+  int8_t* occ;
+  double* eci;
+  double* ring;  /* CEO_MAX_HISTORY x n_cf */
+  ceo_change* changes;
+  unsigned current, buffer_size;
+  int n_threads;
+  /* species index tracker of the canonical sampler (swap_move_index_tracker.py:22-59):
+   * per-species site lists, rebuilt in ascending site order when the
+   * occupation was changed outside a run */
+  int* trk;
+  int* trk_off;
+  int* trk_count;
+  int trk_valid;
+};
+
+static void* dup_bytes(const void* src, size_t n) {
+  void* p = malloc(n ? n : 1);
+  if (p && n) memcpy(p, src, n);
+  return p;
+}
+
+ceo_model* ceo_model_create(int n_sites, int n_species, int n_bfs, const double* basis,
+                            int n_groups, const int* site_group, int n_cf, const int* cf_kind,
+                            const int* cf_dec, const int* cf_cluster, const int* cf_eq_off,
+                            const int* eq_decos, int n_clusters, const int* cl_size,
+                            const int* cl_M, const int* cl_norm, const int* cl_mem_off,
+                            const int* members, const int* cl_ord_off, const int* order, int K,
+                            const int* trans) {
+  ceo_model* m = (ceo_model*)calloc(1, sizeof *m);
+  if (!m) return NULL;
+  m->n_sites = n_sites;
+  m->n_species = n_species;
+  m->n_bfs = n_bfs;
+  m->n_groups = n_groups;
+  m->n_cf = n_cf;
+  m->n_clusters = n_clusters;
+  m->K = K;
+  m->basis = (double*)dup_bytes(basis, sizeof(double) * (size_t)n_bfs * n_species);
+  m->site_group = (int*)dup_bytes(site_group, sizeof(int) * (size_t)n_sites);
+  m->group_count = (int*)calloc((size_t)n_groups + 1, sizeof(int));
+  for (int i = 0; i < n_sites; ++i)
+    if (site_group[i] >= 0) m->group_count[site_group[i]] += 1;
+  m->cf_kind = (int*)dup_bytes(cf_kind, sizeof(int) * (size_t)n_cf);
+  m->cf_dec = (int*)dup_bytes(cf_dec, sizeof(int) * (size_t)n_cf);
+  m->cf_cluster = (int*)dup_bytes(cf_cluster, sizeof(int) * (size_t)n_cf * n_groups);
+  m->cf_eq_off = (int*)dup_bytes(cf_eq_off, sizeof(int) * (size_t)(n_cf + 1));
+  m->eq_decos = (int*)dup_bytes(eq_decos, sizeof(int) * CEO_MAX_CLUSTER * (size_t)cf_eq_off[n_cf]);
+  m->cl_size = (int*)dup_bytes(cl_size, sizeof(int) * (size_t)n_clusters);
+  m->cl_M = (int*)dup_bytes(cl_M, sizeof(int) * (size_t)n_clusters);
+  m->cl_norm = (int*)dup_bytes(cl_norm, sizeof(int) * (size_t)n_clusters);
+  m->cl_mem_off = (int*)dup_bytes(cl_mem_off, sizeof(int) * (size_t)(n_clusters + 1));
+  m->members = (int*)dup_bytes(members, sizeof(int) * (size_t)cl_mem_off[n_clusters]);
+  m->cl_ord_off = (int*)dup_bytes(cl_ord_off, sizeof(int) * (size_t)(n_clusters + 1));
+  m->order = (int*)dup_bytes(order, sizeof(int) * (size_t)cl_ord_off[n_clusters]);
+  m->trans = (int*)dup_bytes(trans, sizeof(int) * (size_t)n_sites * K);
+  return m;
+}
+
+void ceo_model_destroy(ceo_model* m) {
+  if (!m) return;
+  free(m->basis);
+  free(m->site_group);
+  free(m->group_count);
+  free(m->cf_kind);
+  free(m->cf_dec);
+  free(m->cf_cluster);
+  free(m->cf_eq_off);
+  free(m->eq_decos);
+  free(m->cl_size);
+  free(m->cl_M);
+  free(m->cl_norm);
+  free(m->cl_mem_off);
+  free(m->members);
+  free(m->cl_ord_off);
+  free(m->order);
+  free(m->trans);
+  free(m);
+}
+
+/* ---------------------------------------------------------------- history */
+static double* ring_slot(const ceo_chain* c, unsigned slot) {
+  return c->ring + (size_t)slot * c->m->n_cf;
+}
+
+static unsigned current_slot(const ceo_chain* c) {
+  return c->current == 0 ? CEO_MAX_HISTORY - 1 : c->current - 1;
+}
+
+ceo_chain* ceo_chain_create(const ceo_model* m, const int8_t* occupation, const double* cf_init,
+                            const double* eci) {
+  ceo_chain* c = (ceo_chain*)calloc(1, sizeof *c);
+  if (!c) return NULL;
+  c->m = m;
+  c->occ = (int8_t*)dup_bytes(occupation, (size_t)m->n_sites);
+  c->eci = (double*)dup_bytes(eci, sizeof(double) * (size_t)m->n_cf);
+  c->ring = (double*)calloc((size_t)CEO_MAX_HISTORY * m->n_cf, sizeof(double));
+  c->changes = (ceo_change*)calloc(CEO_MAX_HISTORY, sizeof(ceo_change));
+  c->n_threads = 1;
+  c->trk = (int*)malloc(sizeof(int) * (size_t)(m->n_sites ? m->n_sites : 1));
+  c->trk_off = (int*)calloc((size_t)m->n_species + 1, sizeof(int));
+  c->trk_count = (int*)calloc((size_t)m->n_species, sizeof(int));
+  c->trk_valid = 0;
+  /* CFHistoryTracker::insert then clear (cf_history_tracker.cpp:82-108, 146-149) */
+  memcpy(ring_slot(c, 0), cf_init, sizeof(double) * (size_t)m->n_cf);
+  c->current = 1;
+  c->buffer_size = 1;
+  return c;
+}
+
+void ceo_chain_destroy(ceo_chain* c) {
+  if (!c) return;
+  free(c->occ);
+  free(c->eci);
+  free(c->ring);
+  free(c->changes);
+  free(c->trk);
+  free(c->trk_off);
+  free(c->trk_count);
+  free(c);
+}
+
+void ceo_chain_set_threads(ceo_chain* c, int n_threads) { c->n_threads = n_threads > 0 ? n_threads : 1; }
+
+int ceo_history_size(const ceo_chain* c) { return (int)c->buffer_size; }
+
+/* ------------------------------------------------------------- CF update */
+static double spin_product_one_atom(const ceo_chain* c, int ref, int cl, const int* dec, int ref_id) {
+  const ceo_model* m = c->m;
+  const int n = m->cl_size[cl];
+  const int n_memb = n - 1;
+  const int M = m->cl_M[cl];
+  const int* mem = m->members + m->cl_mem_off[cl];
+  const int* ord = m->order + m->cl_ord_off[cl];
+  const int* trow = m->trans + (size_t)ref * m->K;
+  double sp = 0.0;
+  for (int i = 0; i < M; ++i) {
+    double sp_temp = 1.0;
+    int indices[CEO_MAX_CLUSTER];
+    int sorted[CEO_MAX_CLUSTER];
+    indices[0] = ref;
+    for (int j = 0; j < n_memb; ++j) indices[j + 1] = trow[mem[i * n_memb + j]];
+    for (int j = 0; j < n; ++j) sorted[j] = indices[ord[i * n + j]];
+    for (int j = 0; j < n; ++j) {
+      int s = sorted[j] == ref ? ref_id : c->occ[sorted[j]];
+      sp_temp *= m->basis[dec[j] * m->n_species + s];
+    }
+    sp += sp_temp;
+  }
+  return sp;
+}
+
+static void update_one_cf(const ceo_chain* c, int k, int site, int old_sid, int new_sid,
+                          const double* cur, double* next) {
+  const ceo_model* m = c->m;
+  const int kind = m->cf_kind[k];
+  if (kind == CEO_CF_C0) {
+    next[k] = cur[k];
+    return;
+  }
+  if (kind == CEO_CF_SINGLET) {
+    const int d = m->cf_dec[k];
+    next[k] = cur[k] + (m->basis[d * m->n_species + new_sid] - m->basis[d * m->n_species + old_sid]) /
+                           (double)(unsigned)m->n_sites;
+    return;
+  }
+  const int g = m->site_group[site];
+  const int cl = m->cf_cluster[g * m->n_cf + k];
+  if (cl < 0) {
+    next[k] = cur[k];
+    return;
+  }
+  const int n_e = m->cf_eq_off[k + 1] - m->cf_eq_off[k];
+  double delta_sp = 0.0;
+  for (int e = 0; e < n_e; ++e) {
+    const int* deco = m->eq_decos + (size_t)(m->cf_eq_off[k] + e) * CEO_MAX_CLUSTER;
+    double sp_ref = spin_product_one_atom(c, site, cl, deco, old_sid);
+    double sp_new = spin_product_one_atom(c, site, cl, deco, new_sid);
+    delta_sp += sp_new - sp_ref;
+  }
+  delta_sp *= ((double)(unsigned)m->cl_size[cl] / (double)(size_t)n_e);
+  delta_sp /= (m->cl_norm[cl] * m->group_count[g]);
+  next[k] = cur[k] + delta_sp;
+}
+
+static int update_cf_impl(ceo_chain* c, int site, int old_id, int new_id) {
+  const ceo_model* m = c->m;
+  if (old_id == new_id) return CEO_OK;
+  if (site < 0 || site >= m->n_sites) return fail(CEO_EINVAL, "site index out of range");
+  if (new_id < 0 || new_id >= m->n_species) return fail(CEO_EINVAL, "unknown species");
+  if (m->site_group[site] < 0) return fail(CEO_ERUNTIME, "Attempting to move a background atom!");
+  const double* cur = ring_slot(c, current_slot(c));
+  /* get_next (cf_history_tracker.cpp:46-56) */
+  unsigned slot = c->current;
+  c->current = (c->current + 1) % CEO_MAX_HISTORY;
+  if (c->buffer_size < CEO_MAX_HISTORY) c->buffer_size += 1;
+  double* next = ring_slot(c, slot);
+  c->changes[slot].site = site;
+  c->changes[slot].old_id = old_id;
+  c->changes[slot].new_id = new_id;
+  const int old_sid = c->occ[site];
+  c->occ[site] = (int8_t)new_id;
+  const int new_sid = new_id;
+  if (c->n_threads > 1) {
+#pragma omp parallel for num_threads(c->n_threads) schedule(dynamic)
+    for (int k = 0; k < m->n_cf; ++k) update_one_cf(c, k, site, old_sid, new_sid, cur, next);
+  } else {
+    for (int k = 0; k < m->n_cf; ++k) update_one_cf(c, k, site, old_sid, new_sid, cur, next);
+  }
+  return CEO_OK;
+}
+
+double ceo_get_energy(const ceo_chain* c) {
+  const double* cf = ring_slot(c, current_slot(c));
+  double dot = 0.0;
+  for (int k = 0; k < c->m->n_cf; ++k) dot += c->eci[k] * cf[k];
+  return dot * (double)(unsigned)c->m->n_sites;
+}
+
+static int calculate_impl(ceo_chain* c, int n, const int* site, const int* old_id, const int* new_id,
+                          double* energy) {
+  int rc = CEO_OK;
+  if (n == 1) {
+    rc = update_cf_impl(c, site[0], old_id[0], new_id[0]);
+  } else if (n > 1) {
+    int all_swaps = (n % 2) == 0;
+    for (int p = 0; all_swaps && p < n / 2; ++p) {
+      /* is_swap_move (ce_updater.cpp:996-1000) */
+      if (!(old_id[2 * p] == new_id[2 * p + 1] && new_id[2 * p + 1] == old_id[2 * p + 1]))
+        all_swaps = 0;
+    }
+    if (all_swaps) {
+      if (n / 2 >= CEO_MAX_HISTORY / 2)
+        return fail(CEO_EINVAL,
+                    "The length of sequence of swap move exceeds the buffer size for the history tracker");
+      for (int p = 0; p < n / 2 && rc == CEO_OK; ++p) {
+        int a = site[2 * p], b = site[2 * p + 1];
+        if (c->occ[a] == c->occ[b])
+          return fail(CEO_ERUNTIME,
+                      "This version of the calculate function assumes that the provided update is "
+                      "swapping two atoms");
+        if (c->occ[a] != old_id[2 * p] || c->occ[b] != old_id[2 * p + 1])
+          return fail(CEO_ERUNTIME, "The atom position tracker does not match the current state");
+        rc = update_cf_impl(c, a, old_id[2 * p], new_id[2 * p]);
+        if (rc == CEO_OK) rc = update_cf_impl(c, b, old_id[2 * p + 1], new_id[2 * p + 1]);
+      }
+    } else {
+      for (int i = 0; i < n && rc == CEO_OK; ++i) rc = update_cf_impl(c, site[i], old_id[i], new_id[i]);
+    }
+  }
+  if (rc != CEO_OK) return rc;
+  if (energy) *energy = ceo_get_energy(c);
+  return CEO_OK;
+}
+
+static void undo_impl(ceo_chain* c) {
+  int steps = (int)c->buffer_size - 1;
+  for (int i = 0; i < steps; ++i) {
+    /* pop (cf_history_tracker.cpp:58-80) */
+    if (c->buffer_size == 0) break;
+    c->current = c->current == 0 ? CEO_MAX_HISTORY - 1 : c->current - 1;
+    const ceo_change* ch = &c->changes[c->current];
+    c->buffer_size -= 1;
+    c->occ[ch->site] = (int8_t)ch->old_id;
+  }
+}
+
+int ceo_update_cf(ceo_chain* c, int site, int old_id, int new_id) {
+  c->trk_valid = 0;
+  return update_cf_impl(c, site, old_id, new_id);
+}
+
+int ceo_calculate(ceo_chain* c, int n, const int* site, const int* old_id, const int* new_id,
+                  double* energy) {
+  c->trk_valid = 0;
+  return calculate_impl(c, n, site, old_id, new_id, energy);
+}
+
+int ceo_undo_changes(ceo_chain* c) {
+  c->trk_valid = 0;
+  undo_impl(c);
+  return CEO_OK;
+}
+
+void ceo_clear_history(ceo_chain* c) { c->buffer_size = 1; }
+
+void ceo_get_cf(const ceo_chain* c, double* out) {
+  memcpy(out, ring_slot(c, current_slot(c)), sizeof(double) * (size_t)c->m->n_cf);
+}
+
+void ceo_set_ecis(ceo_chain* c, const double* eci) {
+  memcpy(c->eci, eci, sizeof(double) * (size_t)c->m->n_cf);
+}
+
+void ceo_get_occupation(const ceo_chain* c, int8_t* out) { memcpy(out, c->occ, (size_t)c->m->n_sites); }
+
+/* ------------------------------------------------------------- Philox */
+#define PHILOX_M0 0xD2E7470EE14C6C93ULL
+#define PHILOX_M1 0xCA5A826395121157ULL
+#define PHILOX_W0 0x9E3779B97F4A7C15ULL
+#define PHILOX_W1 0xBB67AE8584CAA73BULL
+
+void ceo_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]) {
+  uint64_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+  uint64_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += PHILOX_W0;
+      k1 += PHILOX_W1;
+    }
+    __uint128_t p0 = (__uint128_t)PHILOX_M0 * x0;
+    __uint128_t p1 = (__uint128_t)PHILOX_M1 * x2;
+    uint64_t hi0 = (uint64_t)(p0 >> 64), lo0 = (uint64_t)p0;
+    uint64_t hi1 = (uint64_t)(p1 >> 64), lo1 = (uint64_t)p1;
+    x0 = hi1 ^ x1 ^ k0;
+    x1 = lo1;
+    x2 = hi0 ^ x3 ^ k1;
+    x3 = lo0;
+  }
+  out[0] = x0;
+  out[1] = x1;
+  out[2] = x2;
+  out[3] = x3;
+}
+
+static inline uint64_t bounded64(uint64_t w, uint64_t n) { return (uint64_t)(((__uint128_t)w * n) >> 64); }
+static inline uint32_t bounded32(uint32_t w, uint32_t n) { return (uint32_t)(((uint64_t)w * n) >> 32); }
+static inline double unit_double(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
+
+/* --------------------------------------------------------- Metropolis */
+int ceo_run(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,
+            int64_t n_moves, double temperature, uint8_t* accept_bits, double* e_trace,
+            int64_t trace_every, int64_t* n_accept) {
+  const ceo_model* m = c->m;
+  const int N = m->n_sites;
+  int present[64];
+  int n_present = 0;
+  int* movable = NULL;
+  int n_movable = 0;
+  int64_t acc = 0;
+  int rc = CEO_OK;
+  int* count = c->trk_count;
+  int* trk_off = c->trk_off;
+  int* tracker = c->trk;
+  if (kind == 0) {
+    if (!c->trk_valid) {
+      memset(count, 0, sizeof(int) * (size_t)m->n_species);
+      for (int i = 0; i < N; ++i)
+        if (m->site_group[i] >= 0) count[c->occ[i]] += 1;
+      trk_off[0] = 0;
+      for (int s = 0; s < m->n_species; ++s) trk_off[s + 1] = trk_off[s] + count[s];
+      int* fill = (int*)calloc((size_t)(unsigned)m->n_species + 1, sizeof(int));
+      for (int i = 0; i < N; ++i) {
+        if (m->site_group[i] < 0) continue;
+        int s = c->occ[i];
+        tracker[trk_off[s] + fill[s]++] = i;
+      }
+      free(fill);
+      c->trk_valid = 1;
+    }
+    for (int s = 0; s < m->n_species; ++s)
+      if (count[s] > 0 && n_present < 64) present[n_present++] = s;
+    if (n_present < 2) return fail(CEO_EINVAL, "canonical moves need at least two species present");
+  } else {
+    c->trk_valid = 0;
+    movable = (int*)malloc(sizeof(int) * (size_t)(N ? N : 1));
+    for (int i = 0; i < N; ++i)
+      if (m->site_group[i] >= 0) movable[n_movable++] = i;
+    if (m->n_species < 2 || n_movable == 0) {
+      free(movable);
+      return fail(CEO_EINVAL, "SGC moves need two species and a movable site");
+    }
+  }
+  const double kT = temperature * CEO_KB;
+  double e_cur = ceo_get_energy(c);
+  const uint64_t key[2] = {seed, chain_uid};
+  for (int64_t t = 0; t < n_moves; ++t) {
+    const uint64_t ctr[4] = {move_begin + (uint64_t)t, 0, 0, 0};
+    uint64_t w[4];
+    ceo_philox4x64(ctr, key, w);
+    int sites[2], olds[2], news[2], n_ch;
+    int sa = 0, sb = 0, a = 0, b = 0;
+    if (kind == 0) {
+      uint32_t ia = bounded32((uint32_t)(w[0] >> 32), (uint32_t)n_present);
+      uint32_t ib = (ia + 1 + bounded32((uint32_t)w[0], (uint32_t)(n_present - 1))) % (uint32_t)n_present;
+      a = present[ia];
+      b = present[ib];
+      sa = (int)bounded64(w[1], (uint64_t)count[a]);
+      sb = (int)bounded64(w[2], (uint64_t)count[b]);
+      sites[0] = tracker[trk_off[a] + sa];
+      sites[1] = tracker[trk_off[b] + sb];
+      olds[0] = a;
+      news[0] = b;
+      olds[1] = b;
+      news[1] = a;
+      n_ch = 2;
+    } else {
+      int site = movable[bounded64(w[1], (uint64_t)n_movable)];
+      int old = c->occ[site];
+      int nw = (int)((old + 1 + (int)bounded32((uint32_t)w[0], (uint32_t)(m->n_species - 1))) % m->n_species);
+      sites[0] = site;
+      olds[0] = old;
+      news[0] = nw;
+      n_ch = 1;
+    }
+    const double u = unit_double(w[3]);
+    double e_new = 0.0;
+    rc = calculate_impl(c, n_ch, sites, olds, news, &e_new);
+    if (rc != CEO_OK) break;
+    int accept;
+    if (e_new < e_cur) {
+      accept = 1;
+    } else {
+      const double energy_diff = e_new - e_cur;
+      const double probability = exp(-energy_diff / kT);
+      accept = u <= probability;
+    }
+    if (accept) {
+      ceo_clear_history(c);
+      e_cur = e_new;
+      acc += 1;
+      if (kind == 0) {
+        tracker[trk_off[a] + sa] = sites[1];
+        tracker[trk_off[b] + sb] = sites[0];
+      }
+      if (accept_bits) accept_bits[t >> 3] |= (uint8_t)(1u << (t & 7));
+    } else {
+      undo_impl(c);
+    }
+    if (e_trace && trace_every > 0 && ((t + 1) % trace_every) == 0) e_trace[(t + 1) / trace_every - 1] = e_cur;
+  }
+  free(movable);
+  if (n_accept) *n_accept = acc;
+  return rc;
+}
+
+int ceo_run_many(ceo_chain** chains, int n_chains, int kind, uint64_t seed,
+                 const uint64_t* chain_uids, uint64_t move_begin, int64_t n_moves,
+                 const double* temperatures, int n_threads, int64_t* n_accept) {
+  int rc = CEO_OK;
+  if (n_threads < 1) n_threads = 1;
+#pragma omp parallel for num_threads(n_threads) schedule(dynamic, 1) reduction(| : rc)
+  for (int i = 0; i < n_chains; ++i) {
+    int64_t acc = 0;
+    rc |= ceo_run(chains[i], kind, seed, chain_uids[i], move_begin, n_moves, temperatures[i], NULL, NULL,
+                  0, &acc);
+    if (n_accept) n_accept[i] = acc;
+  }
+  return rc;
+}

@@ -1,0 +1,90 @@
+/*
+ * Test infrastructure: CPU oracle for the cluster-expansion Metropolis path.
+ *
+ * A plain-C restatement of the reference updater's arithmetic
+ * (cpp/src/ce_updater.cpp, cpp/src/cf_history_tracker.cpp,
+ * cpp/src/named_array.cpp) and of the Metropolis step of the reference
+ * samplers (cemc/mcmc/montecarlo.py:890-1038, cemc/mcmc/sgc_montecarlo.py:62-76),
+ * driven by a fed Philox4x64-10 move stream.  Only tests, the smoke check and
+ * the CPU-baseline leg of the benchmark use it; the product never does.
+ */
+#ifndef CE_ORACLE_H
+#define CE_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { CEO_OK = 0, CEO_EINVAL = 1, CEO_ERUNTIME = 2 };
+
+enum { CEO_CF_C0 = 0, CEO_CF_SINGLET = 1, CEO_CF_CLUSTER = 2 };
+
+typedef struct ceo_model ceo_model;
+typedef struct ceo_chain ceo_chain;
+
+/* Model tables (shared, read only).
+ *  basis[d*n_species + s]              phi_d(s)          (basis_function.cpp:34-42)
+ *  site_group[i]                       translation group, -1 = background
+ *  cf_kind[k], cf_dec[k]               per correlation function in sorted-name order
+ *  cf_cluster[g*n_cf + k]              cluster id in group g (-1: not present)
+ *  cf_eq_off[k]..cf_eq_off[k+1]        equivalent decorations of CF k (index into eq_decos,
+ *                                      each entry is 4 ints, unused tail = 0)
+ *  cl_size[c], cl_M[c], cl_norm[c]     size n, sub-clusters M, C_p = sum over groups of M
+ *  cl_mem_off[c]                       offset into members (M*(n-1) trans slots)
+ *  cl_ord_off[c]                       offset into order (M*n)
+ *  trans[i*K + slot]                   translation table restricted to used columns
+ */
+ceo_model* ceo_model_create(int n_sites, int n_species, int n_bfs, const double* basis,
+                            int n_groups, const int* site_group, int n_cf, const int* cf_kind,
+                            const int* cf_dec, const int* cf_cluster, const int* cf_eq_off,
+                            const int* eq_decos, int n_clusters, const int* cl_size,
+                            const int* cl_M, const int* cl_norm, const int* cl_mem_off,
+                            const int* members, const int* cl_ord_off, const int* order, int K,
+                            const int* trans);
+void ceo_model_destroy(ceo_model* m);
+
+ceo_chain* ceo_chain_create(const ceo_model* m, const int8_t* occupation, const double* cf_init,
+                            const double* eci);
+void ceo_chain_destroy(ceo_chain* c);
+void ceo_chain_set_threads(ceo_chain* c, int n_threads);
+
+/* Updater operations (ce_updater.cpp:313-406, 408-448, 471-519, 552-555, 236-242).
+ * A change is (site, old species, new species); old/new ids only decide the
+ * no-op test the reference makes on symbol strings (ce_updater.cpp:315-318). */
+int ceo_update_cf(ceo_chain* c, int site, int old_id, int new_id);
+int ceo_calculate(ceo_chain* c, int n, const int* site, const int* old_id, const int* new_id,
+                  double* energy);
+int ceo_undo_changes(ceo_chain* c);
+void ceo_clear_history(ceo_chain* c);
+double ceo_get_energy(const ceo_chain* c);
+void ceo_get_cf(const ceo_chain* c, double* out);
+void ceo_set_ecis(ceo_chain* c, const double* eci);
+void ceo_get_occupation(const ceo_chain* c, int8_t* out);
+int ceo_history_size(const ceo_chain* c);
+const char* ceo_last_error(void);
+
+/* Philox4x64-10 block (Salmon et al., SC'11), counter used as given. */
+void ceo_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
+
+/* Fed-stream Metropolis runs.  Move t of chain `chain_uid` uses the Philox block
+ * with key (seed, chain_uid) and counter (t, 0, 0, 0).
+ *  kind 0: canonical swaps with the species index tracker
+ *  kind 1: semi-grand-canonical flips
+ * accept_bits: one bit per move (LSB first), may be NULL.
+ * e_trace: energy after every `trace_every` moves, may be NULL.
+ * Returns CEO_OK or an error code. */
+int ceo_run(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,
+            int64_t n_moves, double temperature, uint8_t* accept_bits, double* e_trace,
+            int64_t trace_every, int64_t* n_accept);
+
+/* Run many independent chains on OpenMP threads (one chain per thread at a
+ * time), each over the same number of moves. */
+int ceo_run_many(ceo_chain** chains, int n_chains, int kind, uint64_t seed,
+                 const uint64_t* chain_uids, uint64_t move_begin, int64_t n_moves,
+                 const double* temperatures, int n_threads, int64_t* n_accept);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
