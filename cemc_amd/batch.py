@@ -1,0 +1,159 @@
+"""Batched Markov chains resident on the GPU.
+
+Each chain is one lattice of a ``CEModel`` with its own temperature, move
+kind (canonical swaps or semi-grand-canonical flips), chemical potentials
+and Philox stream key.  ``run`` advances every chain by its own number of
+trial moves in one persistent launch (one wavefront per chain).  Per move the
+kernel restates the reference sampler step (cemc/mcmc/montecarlo.py:970-1038)
+with the proposal distributions of montecarlo.py:890-908 and
+sgc_montecarlo.py:62-76 drawn from the fed stream: move t of a chain uses
+Philox4x64-10 with key (seed, uid) and counter (t, 0, 0, 0).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _capi
+
+CANONICAL = _capi.MOVE_CANONICAL
+SGC = _capi.MOVE_SGC
+
+
+def chem_pot_vector(model, chem_pot):
+    """Chemical potentials keyed by singlet name -> array indexed by decoration."""
+    vec = np.zeros(max(model.n_species - 1, 1))
+    if chem_pot is None:
+        return vec
+    if isinstance(chem_pot, dict):
+        for name, val in chem_pot.items():
+            if not name.startswith("c1_") or name not in model.names:
+                raise ValueError("A chemical potential that is currently not tracked is added: %s" % name)
+            vec[int(name.rsplit("_", 1)[1])] = float(val)
+        return vec
+    arr = np.asarray(chem_pot, dtype=np.float64)
+    vec[:len(arr)] = arr
+    return vec
+
+
+class ChainBatch:
+    def __init__(self, models, chains):
+        """models: one CEModel per chain; chains: dicts with keys kind,
+        temperature, occupation, seed, uid and optionally chem_pot."""
+        lib = _capi.load()
+        n = len(chains)
+        if n == 0 or len(models) != n:
+            raise ValueError("one model per chain is required")
+        self.models = list(models)
+        self._lib = lib
+        descs = (_capi.ChainDesc * n)()
+        handles = (ctypes.c_void_p * n)()
+        keep = []
+        self.n_sites = np.zeros(n, dtype=np.int64)
+        self.kinds = np.zeros(n, dtype=np.int32)
+        self.temperatures = np.zeros(n)
+        for c, (m, ch) in enumerate(zip(models, chains)):
+            occ, op = _capi.i8(ch["occupation"])
+            if occ.shape[0] != m.n_sites:
+                raise ValueError("occupation length does not match the model")
+            mu, mp = _capi.f64(chem_pot_vector(m, ch.get("chem_pot")))
+            keep += [occ, mu]
+            d = descs[c]
+            d.kind = int(ch.get("kind", CANONICAL))
+            d.temperature = float(ch["temperature"])
+            d.chem_pot = mp
+            d.seed = int(ch.get("seed", 0)) & 0xFFFFFFFFFFFFFFFF
+            d.uid = int(ch.get("uid", c)) & 0xFFFFFFFFFFFFFFFF
+            d.occupation = op
+            handles[c] = m.handle
+            self.n_sites[c] = m.n_sites
+            self.kinds[c] = d.kind
+            self.temperatures[c] = d.temperature
+        h = ctypes.c_void_p()
+        _capi.check(lib.cemc_batch_create(n, handles, descs, ctypes.byref(h)))
+        self.handle = h
+        self.n_chains = n
+
+    def _moves(self, n_moves):
+        if np.isscalar(n_moves):
+            return None, int(n_moves)
+        a = np.ascontiguousarray(n_moves, dtype=np.int64)
+        if a.shape[0] != self.n_chains:
+            raise ValueError("one move count per chain is required")
+        return a, 0
+
+    def run(self, n_moves):
+        """Advance each chain by n_moves (int or one value per chain)."""
+        a, scalar = self._moves(n_moves)
+        ptr = a.ctypes.data_as(_capi._i64p) if a is not None else None
+        _capi.check(self._lib.cemc_batch_run(self.handle, ptr, scalar))
+
+    def run_timed(self, n_moves):
+        """Like run; returns the launch's device time in milliseconds (HIP events)."""
+        a, scalar = self._moves(n_moves)
+        ptr = a.ctypes.data_as(_capi._i64p) if a is not None else None
+        ms = ctypes.c_double(0.0)
+        _capi.check(self._lib.cemc_batch_run_timed(self.handle, ptr, scalar, ctypes.byref(ms)))
+        return ms.value
+
+    def run_logged(self, n_moves):
+        """Like run; returns per-chain accept bits (uint8, LSB first, one bit per move)."""
+        a, scalar = self._moves(n_moves)
+        ptr = a.ctypes.data_as(_capi._i64p) if a is not None else None
+        most = int(a.max()) if a is not None else scalar
+        words = max((most + 31) // 32, 1)
+        log = np.zeros((self.n_chains, words), dtype=np.uint32)
+        _capi.check(self._lib.cemc_batch_run_logged(self.handle, ptr, scalar,
+                                                    log.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), words))
+        return log.view(np.uint8).reshape(self.n_chains, -1)
+
+    def run_sweeps(self, n_sweeps):
+        """One sweep = n_sites trial moves of that chain."""
+        return self.run(np.asarray(n_sweeps, dtype=np.int64) * self.n_sites)
+
+    def energies(self):
+        out = np.zeros(self.n_chains)
+        _capi.check(self._lib.cemc_batch_get_energy(self.handle, out.ctypes.data_as(_capi._dp)))
+        return out
+
+    def occupation(self, chain):
+        out = np.zeros(int(self.n_sites[chain]), dtype=np.int8)
+        _capi.check(self._lib.cemc_batch_get_occupation(self.handle, int(chain), out.ctypes.data_as(_capi._i8p)))
+        return out
+
+    def stats(self):
+        n = self.n_chains
+        moves = np.zeros(n, dtype=np.int64)
+        acc = np.zeros(n, dtype=np.int64)
+        se = np.zeros(n)
+        se2 = np.zeros(n)
+        _capi.check(self._lib.cemc_batch_get_stats(self.handle, moves.ctypes.data_as(_capi._i64p),
+                                                   acc.ctypes.data_as(_capi._i64p), se.ctypes.data_as(_capi._dp),
+                                                   se2.ctypes.data_as(_capi._dp)))
+        return {"moves": moves, "accepted": acc, "sum_e": se, "sum_e2": se2}
+
+    def cf(self, chain):
+        """Correlation functions of one chain, computed from scratch on the GPU,
+        and the energy they give."""
+        m = self.models[chain]
+        out = np.zeros(m.n_cf)
+        e = ctypes.c_double(0.0)
+        _capi.check(self._lib.cemc_batch_get_cf(self.handle, int(chain), out.ctypes.data_as(_capi._dp),
+                                                ctypes.byref(e)))
+        return out, e.value
+
+    def set_temperature(self, chain, temperature):
+        _capi.check(self._lib.cemc_batch_set_temperature(self.handle, int(chain), float(temperature)))
+        self.temperatures[chain] = float(temperature)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.cemc_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

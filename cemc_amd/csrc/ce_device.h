@@ -1,0 +1,219 @@
+// Device-side building blocks of the cluster-expansion Metropolis engine for
+// CDNA4 (gfx950): data layouts shared by host and kernels, the Philox4x64-10
+// counter RNG, 64-lane DPP reductions and lattice-neighbour evaluation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cemc {
+
+constexpr int kMaxSpecies = 8;           // species per model
+constexpr int kMaxBasis = kMaxSpecies - 1;
+constexpr int kMaxGroups = 8;            // translation-symmetry groups per model
+constexpr int kMaxClusterSize = 4;       // reference limit (cpp/src/cluster.cpp:131-170)
+constexpr double kBoltzmannEV = 8.617330337217213e-05;
+
+// ---------------------------------------------------------------------------
+// Sub-cluster record: one cluster containing the reference site, as listed in
+// the reference's `indices`/`order` for that site (cpp/src/cluster.cpp:185-232).
+//   w0: slot of member 0..2 (10 bits each, canonical order with the reference
+//       site removed) | (size-2) << 30
+//   w1: energy-table base (24 bits) | canonical position of the reference
+//       site << 24 | cluster entry id << 26 (6 bits, for CF kernels use the
+//       separate per-entry range table)
+// Slot = column index into the translation table (used columns only).
+struct SubRec {
+  uint32_t w0;
+  uint32_t w1;
+};
+
+__host__ __device__ inline uint32_t rec_slot(SubRec r, int q) { return (r.w0 >> (10 * q)) & 1023u; }
+__host__ __device__ inline int rec_size(SubRec r) { return int(r.w0 >> 30) + 2; }
+__host__ __device__ inline uint32_t rec_gbase(SubRec r) { return r.w1 & 0xFFFFFFu; }
+__host__ __device__ inline int rec_refpos(SubRec r) { return int((r.w1 >> 24) & 3u); }
+
+// Cluster entry = one (translation group, orbit) pair with its record range.
+struct ClusterEntry {
+  int32_t rec_begin;   // into the model's record array
+  int32_t rec_count;   // M (sub-clusters for the group's reference site)
+  int32_t size;        // n
+  int32_t norm;        // C_p = sum over groups of M (ce_updater.cpp:137-144)
+};
+
+// Device view of one model (lattice + clusters + tables), in global memory.
+struct ModelDev {
+  int32_t n_sites, n_species, n_bfs, n_groups;
+  int32_t n_cf, n_entries, K, periodic;
+  int32_t dims[3];
+  int32_t has_background;
+  int32_t n_movable;
+  int32_t gtab_size;      // doubles in the energy table
+  int32_t n_rec;          // records over all groups
+  int32_t pow_s[kMaxClusterSize];  // n_species^q
+  int32_t grp_rec_off[kMaxGroups];  // energy-path records of group g: [off, off+cnt)
+  int32_t grp_rec_cnt[kMaxGroups];
+  int32_t grp_count[kMaxGroups];    // T_g
+  double basis[kMaxBasis * kMaxSpecies];  // phi[d*n_species + s]
+  const SubRec* recs;
+  const double* gtab;
+  const int32_t* trans;       // n_sites * K
+  const uint32_t* col_cells;  // K packed cell offsets (x | y<<8 | z<<16)
+  const int8_t* site_group;   // n_sites (-1 background)
+  const int32_t* movable;     // non-background sites (only read with background)
+  // correlation-function tables
+  const int32_t* cf_kind;     // n_cf: 0 c0, 1 singlet, 2 cluster
+  const int32_t* cf_dec;      // n_cf
+  const int32_t* cf_entry;    // n_groups * n_cf (cluster entry or -1)
+  const int32_t* cf_eq_off;   // n_cf + 1
+  const uint32_t* eq_decos;   // packed 4 x 8-bit decoration per equivalent decoration
+  const ClusterEntry* entries;
+  const double* eci;          // n_cf
+};
+
+// Per-chain parameters (host written).
+struct ChainParams {
+  int32_t model;
+  int32_t kind;         // 0 canonical swaps, 1 semi-grand-canonical flips
+  int32_t n_sites;
+  int32_t n_present;
+  int64_t occ_off;      // byte offset into the occupation pool
+  int64_t trk_off;      // element offset into the tracker pool
+  int32_t present[kMaxSpecies];
+  int32_t count[kMaxSpecies];
+  int32_t seg_off[kMaxSpecies];
+  double kT;
+  double j1[kMaxBasis];  // singlet ECIs (chemical potential folded in)
+  uint64_t seed;
+  uint64_t uid;
+};
+
+// Per-chain mutable state (kernel written).
+struct ChainState {
+  double energy;
+  uint64_t moves_done;
+  uint64_t n_accept;
+  double sum_e;
+  double sum_e2;
+  double pad[3];
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+constexpr uint64_t kPhiloxM0 = 0xD2E7470EE14C6C93ull;
+constexpr uint64_t kPhiloxM1 = 0xCA5A826395121157ull;
+constexpr uint64_t kPhiloxW0 = 0x9E3779B97F4A7C15ull;
+constexpr uint64_t kPhiloxW1 = 0xBB67AE8584CAA73Bull;
+
+struct Philox4 {
+  uint64_t w0, w1, w2, w3;
+};
+
+__device__ inline Philox4 philox4x64_10(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3, uint64_t k0,
+                                        uint64_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += kPhiloxW0;
+      k1 += kPhiloxW1;
+    }
+    const uint64_t lo0 = kPhiloxM0 * c0;
+    const uint64_t hi0 = __umul64hi(kPhiloxM0, c0);
+    const uint64_t lo1 = kPhiloxM1 * c2;
+    const uint64_t hi1 = __umul64hi(kPhiloxM1, c2);
+    const uint64_t n0 = hi1 ^ c1 ^ k0;
+    const uint64_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+  }
+  return Philox4{c0, c1, c2, c3};
+}
+
+__device__ inline uint64_t bounded64(uint64_t w, uint64_t n) { return __umul64hi(w, n); }
+__device__ inline uint32_t bounded32(uint32_t w, uint32_t n) { return __umulhi(w, n); }
+__device__ inline double unit_double(uint64_t w) { return double(w >> 11) * 0x1.0p-53; }
+
+// ---------------------------------------------------------------------------
+// Cross-lane helpers (wave64).
+__device__ inline uint64_t readlane_u64(uint64_t v, int lane) {
+  const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(v >> 32), lane);
+  return (uint64_t(hi) << 32) | lo;
+}
+
+__device__ inline double readlane_f64(double v, int lane) {
+  return __longlong_as_double(int64_t(readlane_u64(uint64_t(__double_as_longlong(v)), lane)));
+}
+
+template <int kCtrl, int kRowMask = 0xF, int kBankMask = 0xF>
+__device__ inline double dpp_f64(double v) {
+  const int64_t bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(uint32_t(bits)), kCtrl, kRowMask, kBankMask, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(uint32_t(uint64_t(bits) >> 32)), kCtrl, kRowMask,
+                                             kBankMask, false);
+  return __longlong_as_double(int64_t((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo)));
+}
+
+// Sum over the 16 lanes of each DPP row; every lane of the row ends with the
+// row total.  Order of additions is fixed, so the result is deterministic.
+__device__ inline double row_sum_f64(double v) {
+  v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141>(v);  // row_half_mirror
+  v += dpp_f64<0x140>(v);  // row_mirror
+  return v;
+}
+
+// Sum over all 64 lanes, returned as a wave-uniform value.
+__device__ inline double wave_sum_f64(double v) {
+  v = row_sum_f64(v);
+  const double r0 = readlane_f64(v, 0);
+  const double r1 = readlane_f64(v, 16);
+  const double r2 = readlane_f64(v, 32);
+  const double r3 = readlane_f64(v, 48);
+  return (r0 + r1) + (r2 + r3);
+}
+
+// ---------------------------------------------------------------------------
+// Lattice neighbours: site playing the role of column `slot` when `ref` is the
+// reference site (the translation matrix, ce_updater.cpp:264).
+struct RefSite {
+  int32_t site;
+  int32_t x, y, z;  // cell coordinates (closed-form periodic lattices only)
+};
+
+__device__ inline RefSite make_ref(const ModelDev& m, int32_t site) {
+  RefSite r;
+  r.site = site;
+  if (m.periodic) {
+    const int32_t l23 = m.dims[1] * m.dims[2];
+    r.x = site / l23;
+    const int32_t rem = site - r.x * l23;
+    r.y = rem / m.dims[2];
+    r.z = rem - r.y * m.dims[2];
+  } else {
+    r.x = r.y = r.z = 0;
+  }
+  return r;
+}
+
+template <bool kPeriodic>
+__device__ inline int32_t neighbour(const ModelDev& m, const RefSite& r, uint32_t slot,
+                                    const uint32_t* cells) {
+  if constexpr (kPeriodic) {
+    const uint32_t c = cells[slot];
+    int32_t x = r.x + int32_t(c & 0xFFu);
+    int32_t y = r.y + int32_t((c >> 8) & 0xFFu);
+    int32_t z = r.z + int32_t((c >> 16) & 0xFFu);
+    x -= (x >= m.dims[0]) ? m.dims[0] : 0;
+    y -= (y >= m.dims[1]) ? m.dims[1] : 0;
+    z -= (z >= m.dims[2]) ? m.dims[2] : 0;
+    return (x * m.dims[1] + y) * m.dims[2] + z;
+  } else {
+    return m.trans[int64_t(r.site) * m.K + slot];
+  }
+}
+
+}  // namespace cemc

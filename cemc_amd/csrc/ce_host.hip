@@ -1,0 +1,1074 @@
+// Host side of libcemcgpu: the C ABI declared in include/cemc_gpu.h.
+//
+// cemc_model    parses the flattened settings (what CEUpdater::init reads,
+//               cpp/src/ce_updater.cpp:32-234) into sub-cluster records,
+//               per-cluster energy tables with the ECIs folded in, and the
+//               correlation-function tables; uploads them to HBM.
+// cemc_updater  one lattice with the reference's history ring buffer
+//               (cpp/src/cf_history_tracker.cpp:46-149) and dispatch
+//               (cpp/src/ce_updater.cpp:471-550, 831-852); the arithmetic
+//               runs in updater_kernel.
+// cemc_batch    device-resident Markov chains advanced by sweep_kernel.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cemc_gpu.h"
+#include "ce_kernels.h"
+
+using namespace cemc;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIP_CHECK(expr)                                                                         \
+  do {                                                                                          \
+    hipError_t err_ = (expr);                                                                   \
+    if (err_ != hipSuccess) throw DeviceError(std::string(#expr) + ": " + hipGetErrorString(err_)); \
+  } while (0)
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return CEMC_OK;
+  } catch (const std::invalid_argument& e) {
+    g_last_error = e.what();
+    return CEMC_EINVAL;
+  } catch (const DeviceError& e) {
+    g_last_error = e.what();
+    return CEMC_EDEVICE;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return CEMC_ERUNTIME;
+  }
+}
+
+void require(bool ok, const std::string& msg) {
+  if (!ok) throw std::invalid_argument(msg);
+}
+
+template <class T>
+T* upload(const std::vector<T>& v) {
+  T* p = nullptr;
+  HIP_CHECK(hipMalloc(&p, std::max<size_t>(v.size(), 1) * sizeof(T)));
+  if (!v.empty()) HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+template <class T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+constexpr int kMaxHistory = 1000;  // CFHistoryTracker::max_history (cf_history_tracker.hpp:56)
+constexpr uint32_t kGtabLdsBudget = 96u * 1024u;
+
+// Dynamic LDS a workgroup may use on the current device (160 KB on gfx950).
+uint32_t lds_limit() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  size_t v = prop.sharedMemPerBlock;
+  return uint32_t(std::min<size_t>(v, 160u * 1024u));
+}
+
+}  // namespace
+
+// ============================================================================
+struct cemc_model {
+  int n_sites = 0, n_species = 0, n_bfs = 0, n_groups = 0, n_cf = 0, n_entries = 0, K = 0;
+  bool periodic = false;
+  int dims[3] = {0, 0, 0};
+  std::vector<double> basis, eci, gtab;
+  std::vector<int32_t> site_group, cf_kind, cf_dec, cf_entry, cf_eq_off, movable;
+  std::vector<uint32_t> eq_decos, col_cells;
+  std::vector<int32_t> entry_group, entry_size, entry_M, entry_norm, entry_goff;
+  std::vector<int32_t> trans;
+  std::vector<SubRec> recs;
+  std::vector<ClusterEntry> entries;
+  std::vector<int32_t> grp_count;
+  std::vector<int32_t> grp_rec_off, grp_rec_cnt;
+  // device
+  SubRec* d_recs = nullptr;
+  double* d_gtab = nullptr;
+  int32_t* d_trans = nullptr;
+  uint32_t* d_cells = nullptr;
+  int8_t* d_site_group = nullptr;
+  int32_t* d_movable = nullptr;
+  int32_t *d_cf_kind = nullptr, *d_cf_dec = nullptr, *d_cf_entry = nullptr, *d_cf_eq_off = nullptr;
+  uint32_t* d_eq_decos = nullptr;
+  ClusterEntry* d_entries = nullptr;
+  double* d_eci = nullptr;
+  ModelDev view{};
+  ModelDev* d_view = nullptr;
+
+  int pow_s(int q) const {
+    int p = 1;
+    for (int i = 0; i < q; ++i) p *= n_species;
+    return p;
+  }
+
+  void build_gtab(const std::vector<double>& J);
+  void refresh_view();
+  ~cemc_model();
+};
+
+cemc_model::~cemc_model() {
+  dev_free(d_recs);
+  dev_free(d_gtab);
+  dev_free(d_trans);
+  dev_free(d_cells);
+  dev_free(d_site_group);
+  dev_free(d_movable);
+  dev_free(d_cf_kind);
+  dev_free(d_cf_dec);
+  dev_free(d_cf_entry);
+  dev_free(d_cf_eq_off);
+  dev_free(d_eq_decos);
+  dev_free(d_entries);
+  dev_free(d_eci);
+  dev_free(d_view);
+}
+
+// Energy tables.  For cluster entry c (size n, group g) and decoration e over
+// canonical positions, the weight is
+//   W_c[e] = sum_{k on c} mult_k(e) * N * J_k * n / (|E_k| * C_c * T_g)
+// where mult_k(e) counts e in the equivalent decorations of CF k.  This is
+// the energy form of the per-CF increment (ce_updater.cpp:393-404) times
+// N * J_k (ce_updater.cpp:236-242).  With the reference site at canonical
+// position r and decoration digit d there, the table stores
+//   G_c[r][d][code] = sum_{e: e_r = d} W_c[e] * prod_{q != r} phi[e_q](s_q)
+// with code = sum_q s_q * S^q over the other members in canonical order, so
+// a flip a->b of the reference site changes the energy by
+//   sum_d (phi_d(b) - phi_d(a)) * G_c[r][d][code].
+void cemc_model::build_gtab(const std::vector<double>& J) {
+  gtab.assign(gtab.size(), 0.0);
+  const int S = n_species, B = n_bfs;
+  for (int c = 0; c < n_entries; ++c) {
+    const int n = entry_size[c];
+    const int g = entry_group[c];
+    int n_deco = 1;
+    for (int i = 0; i < n; ++i) n_deco *= B;
+    std::vector<double> W(n_deco, 0.0);
+    for (int k = 0; k < n_cf; ++k) {
+      if (cf_kind[k] != CEMC_CF_CLUSTER || cf_entry[g * n_cf + k] != c) continue;
+      const int e0 = cf_eq_off[k], e1 = cf_eq_off[k + 1];
+      const double w = double(n_sites) * J[k] * double(n) /
+                       (double(e1 - e0) * double(entry_norm[c]) * double(grp_count[g]));
+      for (int e = e0; e < e1; ++e) {
+        int idx = 0;
+        for (int pos = n - 1; pos >= 0; --pos) idx = idx * B + int((eq_decos[e] >> (8 * pos)) & 0xFFu);
+        W[idx] += w;
+      }
+    }
+    const int P = pow_s(n - 1);
+    for (int r = 0; r < n; ++r) {
+      for (int idx = 0; idx < n_deco; ++idx) {
+        if (W[idx] == 0.0) continue;
+        int deco[kMaxClusterSize];
+        int t = idx;
+        for (int pos = 0; pos < n; ++pos) {
+          deco[pos] = t % B;
+          t /= B;
+        }
+        const int d = deco[r];
+        double* G = gtab.data() + entry_goff[c] + (r * B + d) * P;
+        for (int code = 0; code < P; ++code) {
+          double prod = W[idx];
+          int cc = code;
+          for (int pos = 0; pos < n; ++pos) {
+            if (pos == r) continue;
+            const int s = cc % S;
+            cc /= S;
+            prod *= basis[deco[pos] * S + s];
+          }
+          G[code] += prod;
+        }
+      }
+    }
+  }
+}
+
+void cemc_model::refresh_view() {
+  ModelDev& v = view;
+  std::memset(&v, 0, sizeof v);
+  v.n_sites = n_sites;
+  v.n_species = n_species;
+  v.n_bfs = n_bfs;
+  v.n_groups = n_groups;
+  v.n_cf = n_cf;
+  v.n_entries = n_entries;
+  v.K = K;
+  v.periodic = periodic ? 1 : 0;
+  for (int i = 0; i < 3; ++i) v.dims[i] = dims[i];
+  v.has_background = int(movable.size()) != n_sites;
+  v.n_movable = int(movable.size());
+  v.gtab_size = int(gtab.size());
+  v.n_rec = int(recs.size());
+  for (int q = 0; q < kMaxClusterSize; ++q) v.pow_s[q] = pow_s(q);
+  for (int g = 0; g < n_groups; ++g) {
+    v.grp_rec_off[g] = grp_rec_off[g];
+    v.grp_rec_cnt[g] = grp_rec_cnt[g];
+    v.grp_count[g] = grp_count[g];
+  }
+  for (size_t i = 0; i < basis.size(); ++i) v.basis[i] = basis[i];
+  v.recs = d_recs;
+  v.gtab = d_gtab;
+  v.trans = d_trans;
+  v.col_cells = d_cells;
+  v.site_group = d_site_group;
+  v.movable = d_movable;
+  v.cf_kind = d_cf_kind;
+  v.cf_dec = d_cf_dec;
+  v.cf_entry = d_cf_entry;
+  v.cf_eq_off = d_cf_eq_off;
+  v.eq_decos = d_eq_decos;
+  v.entries = d_entries;
+  v.eci = d_eci;
+  if (!d_view) HIP_CHECK(hipMalloc(&d_view, sizeof(ModelDev)));
+  HIP_CHECK(hipMemcpy(d_view, &v, sizeof(ModelDev), hipMemcpyHostToDevice));
+}
+
+static void build_model(cemc_model* m, const cemc_model_desc* d) {
+  require(d != nullptr, "model description is NULL");
+  require(d->n_sites > 0, "n_sites must be positive");
+  require(d->n_species >= 2 && d->n_species <= kMaxSpecies, "n_species must be in [2, 8]");
+  require(d->n_groups >= 1 && d->n_groups <= kMaxGroups, "n_groups must be in [1, 8]");
+  require(d->n_cf >= 1, "at least one correlation function is required");
+  require(d->n_trans_cols >= 0 && d->n_trans_cols <= 1024, "at most 1024 translation columns are supported");
+  require(d->n_entries >= 0 && d->n_entries < 64, "at most 63 cluster entries are supported");
+  m->n_sites = d->n_sites;
+  m->n_species = d->n_species;
+  m->n_bfs = d->n_species - 1;
+  m->n_groups = d->n_groups;
+  m->n_cf = d->n_cf;
+  m->n_entries = d->n_entries;
+  m->K = d->n_trans_cols;
+  const int N = m->n_sites, S = m->n_species, B = m->n_bfs, G = m->n_groups;
+  m->basis.assign(d->basis, d->basis + B * S);
+  m->site_group.assign(d->site_group, d->site_group + N);
+  m->grp_count.assign(G, 0);
+  for (int i = 0; i < N; ++i) {
+    const int g = m->site_group[i];
+    require(g >= -1 && g < G, "site group out of range");
+    if (g >= 0) {
+      m->grp_count[g] += 1;
+      m->movable.push_back(i);
+    }
+  }
+  require(!m->movable.empty(), "no movable site");
+  m->cf_kind.assign(d->cf_kind, d->cf_kind + m->n_cf);
+  m->cf_dec.assign(d->cf_dec, d->cf_dec + m->n_cf);
+  m->cf_entry.assign(d->cf_entry, d->cf_entry + G * m->n_cf);
+  m->cf_eq_off.assign(d->cf_eq_off, d->cf_eq_off + m->n_cf + 1);
+  const int n_eq = m->cf_eq_off[m->n_cf];
+  require(n_eq >= 0, "bad equivalent decoration offsets");
+  m->eq_decos.resize(n_eq);
+  for (int e = 0; e < n_eq; ++e) {
+    uint32_t packed = 0;
+    for (int p = 0; p < 4; ++p) {
+      const int v = d->eq_decos[4 * e + p];
+      require(v >= 0 && v < std::max(B, 1), "decoration digit out of range");
+      packed |= uint32_t(v) << (8 * p);
+    }
+    m->eq_decos[e] = packed;
+  }
+  for (int k = 0; k < m->n_cf; ++k) {
+    const int kind = m->cf_kind[k];
+    require(kind >= 0 && kind <= 2, "unknown correlation-function kind");
+    if (kind == CEMC_CF_SINGLET) require(m->cf_dec[k] >= 0 && m->cf_dec[k] < B, "singlet decoration out of range");
+    require(m->cf_eq_off[k + 1] >= m->cf_eq_off[k], "bad equivalent decoration offsets");
+    if (kind == CEMC_CF_CLUSTER) {
+      for (int g = 0; g < G; ++g) {
+        const int c = m->cf_entry[g * m->n_cf + k];
+        require(c >= -1 && c < m->n_entries, "cluster entry out of range");
+        if (c >= 0) {
+          require(d->entry_group[c] == g, "cluster entry belongs to another group");
+          require(m->cf_eq_off[k + 1] > m->cf_eq_off[k], "cluster correlation function without decoration");
+        }
+      }
+    }
+  }
+  m->eci.assign(d->eci, d->eci + m->n_cf);
+  // translation table
+  m->trans.assign(d->trans, d->trans + int64_t(N) * m->K);
+  for (int64_t i = 0; i < int64_t(N) * m->K; ++i)
+    require(m->trans[i] >= 0 && m->trans[i] < N, "translation table entry out of range");
+  // sub-cluster records, grouped by translation group then entry
+  m->entry_group.assign(d->entry_group, d->entry_group + m->n_entries);
+  m->entry_size.assign(d->entry_size, d->entry_size + m->n_entries);
+  m->entry_M.assign(d->entry_M, d->entry_M + m->n_entries);
+  m->entry_norm.assign(d->entry_norm, d->entry_norm + m->n_entries);
+  m->entries.assign(m->n_entries, ClusterEntry{0, 0, 0, 0});
+  m->entry_goff.assign(m->n_entries, 0);
+  int goff = 0;
+  for (int c = 0; c < m->n_entries; ++c) {
+    const int n = m->entry_size[c];
+    require(n >= 2 && n <= kMaxClusterSize, "only cluster sizes 2, 3 and 4 are supported");
+    require(m->entry_M[c] >= 1, "cluster without sub-clusters");
+    require(m->entry_norm[c] >= m->entry_M[c], "bad cluster normalisation");
+    m->entry_goff[c] = goff;
+    goff += n * B * m->pow_s(n - 1);
+  }
+  require(goff < (1 << 24), "energy tables too large");
+  m->gtab.assign(goff, 0.0);
+  m->grp_rec_off.assign(G, 0);
+  m->grp_rec_cnt.assign(G, 0);
+  for (int g = 0; g < G; ++g) {
+    m->grp_rec_off[g] = int(m->recs.size());
+    for (int c = 0; c < m->n_entries; ++c) {
+      if (m->entry_group[c] != g) continue;
+      const int n = m->entry_size[c];
+      const int M = m->entry_M[c];
+      const int32_t* mem = d->members + d->entry_mem_off[c];
+      const int32_t* ord = d->order + d->entry_ord_off[c];
+      require(d->entry_mem_off[c + 1] - d->entry_mem_off[c] == M * (n - 1), "members size mismatch");
+      require(d->entry_ord_off[c + 1] - d->entry_ord_off[c] == M * n, "order size mismatch");
+      m->entries[c] = ClusterEntry{int32_t(m->recs.size()), M, n, m->entry_norm[c]};
+      for (int s = 0; s < M; ++s) {
+        int r = -1;
+        uint32_t seen = 0;
+        for (int pos = 0; pos < n; ++pos) {
+          const int o = ord[s * n + pos];
+          require(o >= 0 && o < n && !(seen & (1u << o)), "order is not a permutation");
+          seen |= 1u << o;
+          if (o == 0) r = pos;
+        }
+        uint32_t w0 = uint32_t(n - 2) << 30;
+        int q = 0;
+        for (int pos = 0; pos < n; ++pos) {
+          if (pos == r) continue;
+          const int slot = mem[s * (n - 1) + ord[s * n + pos] - 1];
+          require(slot >= 0 && slot < m->K, "member column out of range");
+          w0 |= uint32_t(slot) << (10 * q);
+          ++q;
+        }
+        const uint32_t gbase = uint32_t(m->entry_goff[c] + r * B * m->pow_s(n - 1));
+        const uint32_t w1 = gbase | (uint32_t(r) << 24) | (uint32_t(c) << 26);
+        m->recs.push_back(SubRec{w0, w1});
+      }
+    }
+    m->grp_rec_cnt[g] = int(m->recs.size()) - m->grp_rec_off[g];
+  }
+  // closed-form periodic neighbours, accepted only if they reproduce `trans`
+  m->col_cells.assign(std::max(m->K, 1), 0u);
+  if (d->periodic_dims && d->col_cells) {
+    const int L1 = d->periodic_dims[0], L2 = d->periodic_dims[1], L3 = d->periodic_dims[2];
+    bool ok = L1 > 0 && L2 > 0 && L3 > 0 && L1 < 256 && L2 < 256 && L3 < 256 && int64_t(L1) * L2 * L3 == N;
+    for (int j = 0; ok && j < m->K; ++j) {
+      const int cx = d->col_cells[3 * j], cy = d->col_cells[3 * j + 1], cz = d->col_cells[3 * j + 2];
+      ok = cx >= 0 && cx < L1 && cy >= 0 && cy < L2 && cz >= 0 && cz < L3;
+      m->col_cells[j] = uint32_t(cx) | (uint32_t(cy) << 8) | (uint32_t(cz) << 16);
+    }
+    for (int i = 0; ok && i < N; ++i) {
+      const int x = i / (L2 * L3), y = (i / L3) % L2, z = i % L3;
+      for (int j = 0; ok && j < m->K; ++j) {
+        const int xx = (x + d->col_cells[3 * j]) % L1, yy = (y + d->col_cells[3 * j + 1]) % L2,
+                  zz = (z + d->col_cells[3 * j + 2]) % L3;
+        ok = m->trans[int64_t(i) * m->K + j] == (xx * L2 + yy) * L3 + zz;
+      }
+    }
+    if (ok) {
+      m->periodic = true;
+      m->dims[0] = L1;
+      m->dims[1] = L2;
+      m->dims[2] = L3;
+    }
+  }
+  m->build_gtab(m->eci);
+  // upload
+  std::vector<int8_t> sg(N);
+  for (int i = 0; i < N; ++i) sg[i] = int8_t(m->site_group[i]);
+  m->d_recs = upload(m->recs);
+  m->d_gtab = upload(m->gtab);
+  m->d_trans = upload(m->trans);
+  m->d_cells = upload(m->col_cells);
+  m->d_site_group = upload(sg);
+  m->d_movable = upload(m->movable);
+  m->d_cf_kind = upload(m->cf_kind);
+  m->d_cf_dec = upload(m->cf_dec);
+  m->d_cf_entry = upload(m->cf_entry);
+  m->d_cf_eq_off = upload(m->cf_eq_off);
+  m->d_eq_decos = upload(m->eq_decos);
+  m->d_entries = upload(m->entries);
+  m->d_eci = upload(m->eci);
+  m->refresh_view();
+}
+
+// Correlation functions of occupations already on the device (one launch for
+// several chains).  `models` is a device array of model views.
+static void compute_cf(const ModelDev* d_models, const std::vector<ChainParams>& params_host,
+                       const ChainParams* d_params, const int8_t* d_occ_pool, const std::vector<int32_t>& chains,
+                       int n_cf_max, std::vector<double>* cf_out, std::vector<double>* energy_out,
+                       hipStream_t stream) {
+  int max_sites = 1;
+  for (int c : chains) max_sites = std::max(max_sites, params_host[c].n_sites);
+  const int n_blocks = (max_sites + 255) / 256;
+  const int nc = int(chains.size());
+  int32_t* d_ids = upload(chains);
+  double *d_partial = nullptr, *d_cf = nullptr, *d_e = nullptr;
+  try {
+    HIP_CHECK(hipMalloc(&d_partial, sizeof(double) * size_t(nc) * n_cf_max * n_blocks));
+    HIP_CHECK(hipMalloc(&d_cf, sizeof(double) * size_t(nc) * n_cf_max));
+    HIP_CHECK(hipMalloc(&d_e, sizeof(double) * size_t(nc)));
+    HIP_CHECK(launch_cf(d_models, d_params, d_occ_pool, d_ids, nc, max_sites, d_partial, d_cf, d_e, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    if (cf_out) {
+      cf_out->resize(size_t(nc) * n_cf_max);
+      HIP_CHECK(hipMemcpy(cf_out->data(), d_cf, sizeof(double) * cf_out->size(), hipMemcpyDeviceToHost));
+    }
+    if (energy_out) {
+      energy_out->resize(nc);
+      HIP_CHECK(hipMemcpy(energy_out->data(), d_e, sizeof(double) * nc, hipMemcpyDeviceToHost));
+    }
+  } catch (...) {
+    dev_free(d_ids);
+    dev_free(d_partial);
+    dev_free(d_cf);
+    dev_free(d_e);
+    throw;
+  }
+  dev_free(d_ids);
+  dev_free(d_partial);
+  dev_free(d_cf);
+  dev_free(d_e);
+}
+
+// ============================================================================
+struct cemc_updater {
+  cemc_model* model = nullptr;
+  hipStream_t stream = nullptr;
+  int8_t* d_occ = nullptr;
+  double* d_ring = nullptr;
+  double* d_slot_energy = nullptr;
+  double* d_eci = nullptr;
+  double* d_scratch = nullptr;
+  int32_t* d_changes = nullptr;
+  int32_t* d_restore = nullptr;
+  int change_cap = 0;
+  std::vector<int8_t> occ;  // host mirror of the device occupation
+  std::vector<double> eci;
+  std::vector<int64_t> slot_epoch;
+  int64_t eci_epoch = 0;
+  unsigned current = 0, buffer_size = 0;
+  struct Change {
+    int site, old_id, new_id;
+  };
+  std::vector<Change> changes;
+  ~cemc_updater() {
+    dev_free(d_occ);
+    dev_free(d_ring);
+    dev_free(d_slot_energy);
+    dev_free(d_eci);
+    dev_free(d_scratch);
+    dev_free(d_changes);
+    dev_free(d_restore);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  unsigned current_slot() const { return current == 0 ? kMaxHistory - 1 : current - 1; }
+};
+
+namespace {
+
+void updater_flush(cemc_updater* u, const std::vector<int32_t>& packed) {
+  const int n = int(packed.size() / 4);
+  if (n == 0) return;
+  if (n > u->change_cap) {
+    dev_free(u->d_changes);
+    u->change_cap = std::max(n, 2 * u->change_cap);
+    HIP_CHECK(hipMalloc(&u->d_changes, sizeof(int32_t) * 4 * u->change_cap));
+  }
+  HIP_CHECK(hipMemcpyAsync(u->d_changes, packed.data(), sizeof(int32_t) * packed.size(), hipMemcpyHostToDevice,
+                           u->stream));
+  HIP_CHECK(launch_updater(u->model->d_view, u->d_occ, u->d_ring, u->d_slot_energy, u->d_changes, n, u->d_eci,
+                           u->d_scratch, u->stream));
+  HIP_CHECK(hipStreamSynchronize(u->stream));
+  u->slot_epoch[packed[packed.size() - 1]] = u->eci_epoch;
+}
+
+// One change (ce_updater.cpp:313-347): no-op if the symbols are equal, else
+// takes the next history slot and records the change.  Appends the kernel
+// work item; the flip itself happens on the device.
+int updater_stage(cemc_updater* u, int site, int old_id, int new_id, std::vector<int32_t>& packed) {
+  const cemc_model& m = *u->model;
+  if (old_id == new_id) return CEMC_OK;
+  if (site < 0 || site >= m.n_sites) {
+    g_last_error = "site index out of range";
+    return CEMC_EINVAL;
+  }
+  if (new_id < 0 || new_id >= m.n_species || old_id < 0 || old_id >= m.n_species) {
+    g_last_error = "species id out of range";
+    return CEMC_EINVAL;
+  }
+  if (m.site_group[site] < 0) {
+    g_last_error = "Attempting to move a background atom!";
+    return CEMC_ERUNTIME;
+  }
+  const unsigned src = u->current_slot();
+  const unsigned dst = u->current;
+  u->current = (u->current + 1) % kMaxHistory;
+  if (u->buffer_size < unsigned(kMaxHistory)) u->buffer_size += 1;
+  u->changes[dst] = cemc_updater::Change{site, old_id, new_id};
+  u->occ[site] = int8_t(new_id);
+  packed.push_back(site);
+  packed.push_back(new_id);
+  packed.push_back(int32_t(src));
+  packed.push_back(int32_t(dst));
+  return CEMC_OK;
+}
+
+double updater_energy(cemc_updater* u) {
+  const unsigned slot = u->current_slot();
+  if (u->slot_epoch[slot] != u->eci_epoch) {
+    HIP_CHECK(launch_slot_energy(u->model->d_view, u->d_ring, u->d_slot_energy, u->d_eci, int(slot), u->stream));
+    HIP_CHECK(hipStreamSynchronize(u->stream));
+    u->slot_epoch[slot] = u->eci_epoch;
+  }
+  double e = 0.0;
+  HIP_CHECK(hipMemcpy(&e, u->d_slot_energy + slot, sizeof(double), hipMemcpyDeviceToHost));
+  return e;
+}
+
+}  // namespace
+
+// ============================================================================
+struct cemc_batch {
+  int n_chains = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<cemc_model*> models;  // distinct models, index = ChainParams::model
+  std::vector<ChainParams> params;
+  std::vector<int> n_cf_of_chain;
+  ModelDev* d_models = nullptr;
+  ChainParams* d_params = nullptr;
+  ChainState* d_states = nullptr;
+  int8_t* d_occ = nullptr;
+  uint16_t* d_trk = nullptr;
+  int64_t* d_nmoves = nullptr;
+  int32_t* d_order = nullptr;
+  struct Class {
+    bool periodic, gtab_lds;
+    int begin, count;
+    uint32_t lds;
+  };
+  std::vector<Class> classes;
+  int n_cf_max = 1;
+  ~cemc_batch() {
+    dev_free(d_models);
+    dev_free(d_params);
+    dev_free(d_states);
+    dev_free(d_occ);
+    dev_free(d_trk);
+    dev_free(d_nmoves);
+    dev_free(d_order);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void batch_launch(cemc_batch* b, const int64_t* n_moves, int64_t moves, bool timed, double* ms,
+                  uint32_t* host_log = nullptr, int64_t log_words = 0) {
+  std::vector<int64_t> nm(b->n_chains);
+  for (int c = 0; c < b->n_chains; ++c) {
+    nm[c] = n_moves ? n_moves[c] : moves;
+    require(nm[c] >= 0, "number of moves must be non-negative");
+    if (host_log) require(nm[c] <= 32 * log_words, "accept log too small for the number of moves");
+  }
+  uint32_t* d_log = nullptr;
+  if (host_log) {
+    HIP_CHECK(hipMalloc(&d_log, sizeof(uint32_t) * size_t(b->n_chains) * log_words));
+    HIP_CHECK(hipMemsetAsync(d_log, 0, sizeof(uint32_t) * size_t(b->n_chains) * log_words, b->stream));
+  }
+  HIP_CHECK(hipMemcpyAsync(b->d_nmoves, nm.data(), sizeof(int64_t) * nm.size(), hipMemcpyHostToDevice, b->stream));
+  if (timed) HIP_CHECK(hipEventRecord(b->ev0, b->stream));
+  for (const auto& cl : b->classes) {
+    LaunchSweep a;
+    a.params = b->d_params;
+    a.states = b->d_states;
+    a.models = b->d_models;
+    a.occ_pool = b->d_occ;
+    a.trk_pool = b->d_trk;
+    a.chain_order = b->d_order + cl.begin;
+    a.n_moves = b->d_nmoves;
+    a.n_chains = cl.count;
+    a.lds_bytes = cl.lds;
+    a.periodic = cl.periodic;
+    a.gtab_lds = cl.gtab_lds;
+    a.accept_log = d_log;
+    a.log_words = log_words;
+    HIP_CHECK(launch_sweep(a, b->stream));
+  }
+  if (timed) HIP_CHECK(hipEventRecord(b->ev1, b->stream));
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  if (d_log) {
+    const hipError_t e = hipMemcpy(host_log, d_log, sizeof(uint32_t) * size_t(b->n_chains) * log_words,
+                                   hipMemcpyDeviceToHost);
+    (void)hipFree(d_log);
+    HIP_CHECK(e);
+  }
+  if (timed) {
+    float t = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&t, b->ev0, b->ev1));
+    *ms = double(t);
+  }
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+const char* cemc_last_error(void) { return g_last_error.c_str(); }
+
+int cemc_device_count(int* count) {
+  return guarded([&] {
+    require(count != nullptr, "count is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+  });
+}
+
+int cemc_model_create(const cemc_model_desc* desc, cemc_model** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    *out = nullptr;
+    auto* m = new cemc_model();
+    try {
+      build_model(m, desc);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+int cemc_model_set_ecis(cemc_model* m, const double* eci) {
+  return guarded([&] {
+    require(m && eci, "NULL argument");
+    m->eci.assign(eci, eci + m->n_cf);
+    m->build_gtab(m->eci);
+    HIP_CHECK(hipMemcpy(m->d_gtab, m->gtab.data(), sizeof(double) * std::max<size_t>(m->gtab.size(), 1),
+                        hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(m->d_eci, m->eci.data(), sizeof(double) * m->eci.size(), hipMemcpyHostToDevice));
+  });
+}
+
+int cemc_model_is_periodic(const cemc_model* m, int* periodic) {
+  return guarded([&] {
+    require(m && periodic, "NULL argument");
+    *periodic = m->periodic ? 1 : 0;
+  });
+}
+
+int cemc_model_destroy(cemc_model* m) {
+  return guarded([&] { delete m; });
+}
+
+// ---------------------------------------------------------------- updater
+int cemc_updater_create(cemc_model* model, const int8_t* occupation, const double* cf_init, const double* eci,
+                        cemc_updater** out) {
+  return guarded([&] {
+    require(model && occupation && eci && out, "NULL argument");
+    *out = nullptr;
+    const int N = model->n_sites, ncf = model->n_cf;
+    for (int i = 0; i < N; ++i)
+      require(occupation[i] >= 0 && occupation[i] < model->n_species, "occupation species id out of range");
+    auto* u = new cemc_updater();
+    try {
+      u->model = model;
+      HIP_CHECK(hipStreamCreate(&u->stream));
+      u->occ.assign(occupation, occupation + N);
+      u->eci.assign(eci, eci + ncf);
+      u->changes.assign(kMaxHistory, cemc_updater::Change{0, 0, 0});
+      u->slot_epoch.assign(kMaxHistory, -1);
+      u->d_occ = upload(u->occ);
+      u->d_eci = upload(u->eci);
+      HIP_CHECK(hipMalloc(&u->d_ring, sizeof(double) * size_t(kMaxHistory) * ncf));
+      HIP_CHECK(hipMalloc(&u->d_slot_energy, sizeof(double) * kMaxHistory));
+      HIP_CHECK(hipMalloc(&u->d_scratch, sizeof(double) * 4 * size_t(ncf)));
+      HIP_CHECK(hipMalloc(&u->d_restore, sizeof(int32_t) * 2 * kMaxHistory));
+      std::vector<double> cf0(ncf);
+      if (cf_init) {
+        cf0.assign(cf_init, cf_init + ncf);
+      } else {
+        ChainParams p;
+        std::memset(&p, 0, sizeof p);
+        p.model = 0;
+        p.n_sites = N;
+        for (int k = 0; k < ncf; ++k)
+          if (model->cf_kind[k] == CEMC_CF_SINGLET) p.j1[model->cf_dec[k]] = u->eci[k];
+        ChainParams* d_p = upload(std::vector<ChainParams>{p});
+        try {
+          std::vector<double> cf;
+          compute_cf(model->d_view, {p}, d_p, u->d_occ, {0}, ncf, &cf, nullptr, u->stream);
+          cf0 = cf;
+        } catch (...) {
+          dev_free(d_p);
+          throw;
+        }
+        dev_free(d_p);
+      }
+      // CFHistoryTracker::insert + clear (cf_history_tracker.cpp:82-108, 146-149)
+      HIP_CHECK(hipMemcpy(u->d_ring, cf0.data(), sizeof(double) * ncf, hipMemcpyHostToDevice));
+      u->current = 1;
+      u->buffer_size = 1;
+    } catch (...) {
+      delete u;
+      throw;
+    }
+    *out = u;
+  });
+}
+
+int cemc_updater_destroy(cemc_updater* u) {
+  return guarded([&] { delete u; });
+}
+
+int cemc_updater_update_cf(cemc_updater* u, int32_t site, int32_t old_id, int32_t new_id) {
+  int rc = CEMC_OK;
+  const int g = guarded([&] {
+    require(u != nullptr, "NULL updater");
+    std::vector<int32_t> packed;
+    rc = updater_stage(u, site, old_id, new_id, packed);
+    updater_flush(u, packed);
+  });
+  return g != CEMC_OK ? g : rc;
+}
+
+int cemc_updater_calculate(cemc_updater* u, int32_t n, const int32_t* site, const int32_t* old_id,
+                           const int32_t* new_id, double* energy) {
+  int rc = CEMC_OK;
+  const int g = guarded([&] {
+    require(u != nullptr && n >= 0, "bad arguments");
+    require(n == 0 || (site && old_id && new_id), "NULL change arrays");
+    std::vector<int32_t> packed;
+    bool swaps = n > 1 && (n % 2) == 0;
+    for (int p = 0; swaps && p < n / 2; ++p)  // is_swap_move (ce_updater.cpp:996-1000)
+      swaps = old_id[2 * p] == new_id[2 * p + 1] && new_id[2 * p + 1] == old_id[2 * p + 1];
+    if (swaps) {
+      if (n / 2 >= kMaxHistory / 2) {
+        g_last_error = "The length of sequence of swap move exceeds the buffer size for the history tracker";
+        rc = CEMC_EINVAL;
+        return;
+      }
+      for (int p = 0; p < n / 2 && rc == CEMC_OK; ++p) {
+        const int a = site[2 * p], b = site[2 * p + 1];
+        require(a >= 0 && a < u->model->n_sites && b >= 0 && b < u->model->n_sites, "site index out of range");
+        if (u->occ[a] == u->occ[b]) {
+          g_last_error =
+              "This version of the calculate function assumes that the provided update is swapping two atoms";
+          rc = CEMC_ERUNTIME;
+          break;
+        }
+        if (u->occ[a] != old_id[2 * p] || u->occ[b] != old_id[2 * p + 1]) {
+          g_last_error = "The atom position tracker does not match the current state";
+          rc = CEMC_ERUNTIME;
+          break;
+        }
+        rc = updater_stage(u, a, old_id[2 * p], new_id[2 * p], packed);
+        if (rc == CEMC_OK) rc = updater_stage(u, b, old_id[2 * p + 1], new_id[2 * p + 1], packed);
+      }
+    } else {
+      for (int i = 0; i < n && rc == CEMC_OK; ++i) rc = updater_stage(u, site[i], old_id[i], new_id[i], packed);
+    }
+    updater_flush(u, packed);
+    if (rc == CEMC_OK && energy) *energy = updater_energy(u);
+  });
+  return g != CEMC_OK ? g : rc;
+}
+
+int cemc_updater_undo_changes(cemc_updater* u) {
+  return guarded([&] {
+    require(u != nullptr, "NULL updater");
+    const int steps = int(u->buffer_size) - 1;
+    std::vector<int32_t> sites, species;
+    for (int i = 0; i < steps && u->buffer_size > 0; ++i) {  // pop (cf_history_tracker.cpp:58-80)
+      u->current = u->current == 0 ? kMaxHistory - 1 : u->current - 1;
+      const auto& ch = u->changes[u->current];
+      u->buffer_size -= 1;
+      u->occ[ch.site] = int8_t(ch.old_id);
+      sites.push_back(ch.site);
+      species.push_back(ch.old_id);
+    }
+    if (!sites.empty()) {
+      std::vector<int32_t> buf(sites);
+      buf.insert(buf.end(), species.begin(), species.end());
+      HIP_CHECK(hipMemcpyAsync(u->d_restore, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice,
+                               u->stream));
+      HIP_CHECK(launch_restore(u->d_occ, u->d_restore, u->d_restore + sites.size(), int(sites.size()), u->stream));
+      HIP_CHECK(hipStreamSynchronize(u->stream));
+    }
+  });
+}
+
+int cemc_updater_clear_history(cemc_updater* u) {
+  return guarded([&] {
+    require(u != nullptr, "NULL updater");
+    u->buffer_size = 1;
+  });
+}
+
+int cemc_updater_get_energy(cemc_updater* u, double* energy) {
+  return guarded([&] {
+    require(u && energy, "NULL argument");
+    *energy = updater_energy(u);
+  });
+}
+
+int cemc_updater_get_cf(cemc_updater* u, double* cf) {
+  return guarded([&] {
+    require(u && cf, "NULL argument");
+    HIP_CHECK(hipMemcpy(cf, u->d_ring + size_t(u->current_slot()) * u->model->n_cf,
+                        sizeof(double) * u->model->n_cf, hipMemcpyDeviceToHost));
+  });
+}
+
+int cemc_updater_set_ecis(cemc_updater* u, const double* eci) {
+  return guarded([&] {
+    require(u && eci, "NULL argument");
+    u->eci.assign(eci, eci + u->model->n_cf);
+    HIP_CHECK(hipMemcpy(u->d_eci, u->eci.data(), sizeof(double) * u->eci.size(), hipMemcpyHostToDevice));
+    u->eci_epoch += 1;
+  });
+}
+
+int cemc_updater_get_occupation(cemc_updater* u, int8_t* occupation) {
+  return guarded([&] {
+    require(u && occupation, "NULL argument");
+    HIP_CHECK(hipMemcpy(occupation, u->d_occ, u->occ.size(), hipMemcpyDeviceToHost));
+  });
+}
+
+int cemc_updater_history_size(cemc_updater* u, int32_t* size) {
+  return guarded([&] {
+    require(u && size, "NULL argument");
+    *size = int32_t(u->buffer_size);
+  });
+}
+
+// ------------------------------------------------------------------ batch
+int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const cemc_chain_desc* chains,
+                      cemc_batch** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    *out = nullptr;
+    require(n_chains > 0 && chain_models && chains, "bad arguments");
+    auto* b = new cemc_batch();
+    try {
+      b->n_chains = n_chains;
+      HIP_CHECK(hipStreamCreate(&b->stream));
+      HIP_CHECK(hipEventCreate(&b->ev0));
+      HIP_CHECK(hipEventCreate(&b->ev1));
+      std::vector<int8_t> occ_pool;
+      std::vector<uint16_t> trk_pool;
+      b->params.resize(n_chains);
+      b->n_cf_of_chain.resize(n_chains);
+      for (int c = 0; c < n_chains; ++c) {
+        cemc_model* m = chain_models[c];
+        require(m != nullptr, "NULL model");
+        const cemc_chain_desc& d = chains[c];
+        require(d.occupation != nullptr, "NULL occupation");
+        require(d.kind == CEMC_MOVE_CANONICAL || d.kind == CEMC_MOVE_SGC, "unknown move kind");
+        require(d.temperature > 0.0 && std::isfinite(d.temperature), "temperature must be positive");
+        auto it = std::find(b->models.begin(), b->models.end(), m);
+        const int mid = int(it - b->models.begin());
+        if (it == b->models.end()) b->models.push_back(m);
+        ChainParams& p = b->params[c];
+        std::memset(&p, 0, sizeof p);
+        p.model = mid;
+        p.kind = d.kind;
+        p.n_sites = m->n_sites;
+        p.kT = d.temperature * kBoltzmannEV;
+        p.seed = d.seed;
+        p.uid = d.uid;
+        b->n_cf_of_chain[c] = m->n_cf;
+        b->n_cf_max = std::max(b->n_cf_max, m->n_cf);
+        for (int k = 0; k < m->n_cf; ++k)
+          if (m->cf_kind[k] == CEMC_CF_SINGLET) p.j1[m->cf_dec[k]] = m->eci[k];
+        if (d.chem_pot)
+          for (int dd = 0; dd < m->n_bfs; ++dd) p.j1[dd] -= d.chem_pot[dd];
+        const int N = m->n_sites;
+        for (int i = 0; i < N; ++i)
+          require(d.occupation[i] >= 0 && d.occupation[i] < m->n_species, "occupation species id out of range");
+        p.occ_off = int64_t(occ_pool.size());
+        occ_pool.insert(occ_pool.end(), d.occupation, d.occupation + N);
+        occ_pool.resize((occ_pool.size() + 15) & ~size_t(15), 0);
+        p.trk_off = int64_t(trk_pool.size());
+        if (d.kind == CEMC_MOVE_CANONICAL) {
+          require(N <= 65535, "canonical chains support at most 65535 sites");
+          // species index tracker in ascending site order (swap_move_index_tracker.py:22-33)
+          int off = 0;
+          for (int s = 0; s < m->n_species; ++s) {
+            int cnt = 0;
+            for (int i = 0; i < N; ++i)
+              if (m->site_group[i] >= 0 && d.occupation[i] == s) {
+                trk_pool.push_back(uint16_t(i));
+                ++cnt;
+              }
+            p.seg_off[s] = off;
+            p.count[s] = cnt;
+            off += cnt;
+            if (cnt > 0) p.present[p.n_present++] = s;
+          }
+          require(p.n_present >= 2, "canonical moves need at least two species present");
+          trk_pool.resize(p.trk_off + N, 0);
+        }
+      }
+      std::vector<ModelDev> views;
+      for (auto* m : b->models) views.push_back(m->view);
+      b->d_models = upload(views);
+      b->d_params = upload(b->params);
+      b->d_occ = upload(occ_pool);
+      b->d_trk = upload(trk_pool);
+      HIP_CHECK(hipMalloc(&b->d_states, sizeof(ChainState) * n_chains));
+      HIP_CHECK(hipMalloc(&b->d_nmoves, sizeof(int64_t) * n_chains));
+      // launch classes and longest-first order inside each class
+      std::vector<int32_t> order;
+      const uint32_t limit = lds_limit();
+      const uint32_t gtab_budget = std::min(kGtabLdsBudget, limit);
+      for (int periodic = 1; periodic >= 0; --periodic) {
+        for (int glds = 1; glds >= 0; --glds) {
+          std::vector<int32_t> ids;
+          uint32_t lds = 0;
+          for (int c = 0; c < n_chains; ++c) {
+            const cemc_model* m = b->models[b->params[c].model];
+            if (int(m->periodic) != periodic) continue;
+            const int canon = b->params[c].kind == CEMC_MOVE_CANONICAL;
+            const uint32_t with_g = sweep_lds_bytes(m->n_sites, canon, int(m->recs.size()), m->K, m->periodic,
+                                                    int(m->gtab.size()));
+            const uint32_t no_g = sweep_lds_bytes(m->n_sites, canon, int(m->recs.size()), m->K, m->periodic, 0);
+            const bool fits = with_g <= gtab_budget;
+            if (int(fits) != glds) continue;
+            require(no_g <= limit, "chain state does not fit in the workgroup's LDS");
+            ids.push_back(c);
+            lds = std::max(lds, fits ? with_g : no_g);
+          }
+          if (ids.empty()) continue;
+          std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) {
+            const auto* mx = b->models[b->params[x].model];
+            const auto* my = b->models[b->params[y].model];
+            return int64_t(mx->n_sites) * mx->recs.size() > int64_t(my->n_sites) * my->recs.size();
+          });
+          b->classes.push_back(cemc_batch::Class{periodic != 0, glds != 0, int(order.size()), int(ids.size()), lds});
+          order.insert(order.end(), ids.begin(), ids.end());
+        }
+      }
+      b->d_order = upload(order);
+      uint32_t need = 0;
+      for (const auto& cl : b->classes) need = std::max(need, cl.lds);
+      if (need > 64u * 1024u) HIP_CHECK(sweep_set_max_lds(limit));
+      // initial energies from correlation functions computed on the GPU
+      std::vector<int32_t> all(n_chains);
+      std::iota(all.begin(), all.end(), 0);
+      std::vector<double> energies;
+      compute_cf(b->d_models, b->params, b->d_params, b->d_occ, all, b->n_cf_max, nullptr, &energies, b->stream);
+      std::vector<ChainState> st(n_chains);
+      std::memset(st.data(), 0, sizeof(ChainState) * st.size());
+      for (int c = 0; c < n_chains; ++c) st[c].energy = energies[c];
+      HIP_CHECK(hipMemcpy(b->d_states, st.data(), sizeof(ChainState) * st.size(), hipMemcpyHostToDevice));
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+int cemc_batch_destroy(cemc_batch* b) {
+  return guarded([&] { delete b; });
+}
+
+int cemc_batch_run(cemc_batch* b, const int64_t* n_moves, int64_t moves) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    batch_launch(b, n_moves, moves, false, nullptr);
+  });
+}
+
+int cemc_batch_run_timed(cemc_batch* b, const int64_t* n_moves, int64_t moves, double* kernel_ms) {
+  return guarded([&] {
+    require(b && kernel_ms, "NULL argument");
+    batch_launch(b, n_moves, moves, true, kernel_ms);
+  });
+}
+
+int cemc_batch_run_logged(cemc_batch* b, const int64_t* n_moves, int64_t moves, uint32_t* accept_bits,
+                          int64_t words_per_chain) {
+  return guarded([&] {
+    require(b && accept_bits, "NULL argument");
+    require(words_per_chain > 0, "words_per_chain must be positive");
+    batch_launch(b, n_moves, moves, false, nullptr, accept_bits, words_per_chain);
+  });
+}
+
+int cemc_batch_get_energy(cemc_batch* b, double* energy) {
+  return guarded([&] {
+    require(b && energy, "NULL argument");
+    std::vector<ChainState> st(b->n_chains);
+    HIP_CHECK(hipMemcpy(st.data(), b->d_states, sizeof(ChainState) * st.size(), hipMemcpyDeviceToHost));
+    for (int c = 0; c < b->n_chains; ++c) energy[c] = st[c].energy;
+  });
+}
+
+int cemc_batch_get_occupation(cemc_batch* b, int32_t chain, int8_t* occupation) {
+  return guarded([&] {
+    require(b && occupation, "NULL argument");
+    require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    const ChainParams& p = b->params[chain];
+    HIP_CHECK(hipMemcpy(occupation, b->d_occ + p.occ_off, size_t(p.n_sites), hipMemcpyDeviceToHost));
+  });
+}
+
+int cemc_batch_get_stats(cemc_batch* b, int64_t* moves_done, int64_t* n_accept, double* sum_e, double* sum_e2) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    std::vector<ChainState> st(b->n_chains);
+    HIP_CHECK(hipMemcpy(st.data(), b->d_states, sizeof(ChainState) * st.size(), hipMemcpyDeviceToHost));
+    for (int c = 0; c < b->n_chains; ++c) {
+      if (moves_done) moves_done[c] = int64_t(st[c].moves_done);
+      if (n_accept) n_accept[c] = int64_t(st[c].n_accept);
+      if (sum_e) sum_e[c] = st[c].sum_e;
+      if (sum_e2) sum_e2[c] = st[c].sum_e2;
+    }
+  });
+}
+
+int cemc_batch_get_cf(cemc_batch* b, int32_t chain, double* cf, double* energy) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    std::vector<double> cfv, ev;
+    compute_cf(b->d_models, b->params, b->d_params, b->d_occ, {chain}, b->n_cf_max, &cfv, &ev, b->stream);
+    if (cf) std::copy(cfv.begin(), cfv.begin() + b->n_cf_of_chain[chain], cf);
+    if (energy) *energy = ev[0];
+  });
+}
+
+int cemc_batch_set_temperature(cemc_batch* b, int32_t chain, double temperature) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    require(temperature > 0.0 && std::isfinite(temperature), "temperature must be positive");
+    b->params[chain].kT = temperature * kBoltzmannEV;
+    HIP_CHECK(hipMemcpy(&b->d_params[chain].kT, &b->params[chain].kT, sizeof(double), hipMemcpyHostToDevice));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,39 @@
+// Launch interface between the host library (ce_host.hip) and the kernels
+// (ce_kernels.hip).
+#pragma once
+
+#include "ce_device.h"
+
+namespace cemc {
+
+struct LaunchSweep {
+  const ChainParams* params;
+  ChainState* states;
+  const ModelDev* models;
+  int8_t* occ_pool;
+  uint16_t* trk_pool;
+  const int32_t* chain_order;  // blockIdx -> chain
+  const int64_t* n_moves;      // per chain
+  int n_chains;                // blocks in this launch
+  uint32_t lds_bytes;
+  bool periodic;
+  bool gtab_lds;
+  uint32_t* accept_log;  // optional: bit t of chain c at accept_log[c * log_words + t / 32]
+  int64_t log_words;
+};
+
+uint32_t sweep_lds_bytes(int n_sites, int canonical, int n_rec, int K, int periodic, int gtab_doubles);
+// Raises the dynamic-LDS limit of every sweep kernel instantiation.
+hipError_t sweep_set_max_lds(uint32_t bytes);
+
+hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream);
+hipError_t launch_cf(const ModelDev* models, const ChainParams* params, const int8_t* occ_pool,
+                     const int32_t* chain_ids, int n_chains, int max_sites, double* partial, double* cf_out,
+                     double* energy_out, hipStream_t stream);
+hipError_t launch_updater(const ModelDev* model, int8_t* occ, double* ring, double* slot_energy,
+                          const int32_t* changes, int n, const double* eci, double* scratch, hipStream_t stream);
+hipError_t launch_restore(int8_t* occ, const int32_t* sites, const int32_t* species, int n, hipStream_t stream);
+hipError_t launch_slot_energy(const ModelDev* model, const double* ring, double* slot_energy, const double* eci,
+                              int slot, hipStream_t stream);
+
+}  // namespace cemc

@@ -1,0 +1,172 @@
+"""GPU counterpart of the reference's updater binding ``PyCEUpdater``
+(cemc/cpp_ext/pyce_updater.pyx:5-60 over class CEUpdater,
+cpp/include/ce_updater.hpp:47-213).
+
+Same constructor and methods, same exceptions (ValueError for
+std::invalid_argument, RuntimeError otherwise).  The correlation-function
+arithmetic, history slots and occupation live on the GPU; this class converts
+symbols to species ids at the boundary and keeps the caller's atoms object in
+step with the lattice, as the reference does on every flip and undo
+(cpp/src/ce_updater.cpp:338-347, 435-446).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from .model import CEModel
+
+# Boltzmann constant of the reference's vibrational correction
+# (cpp/src/linear_vib_correction.cpp:3).
+_KB_VIB = 8.6173303e-5
+
+
+class PyCEUpdater:
+    def __init__(self, atoms, bc, corr_func, eci, model=None):
+        lib = _capi.load()
+        if bc is None:
+            raise ValueError("CEBulk object is nullptr!")
+        symbols = [a.symbol for a in atoms]
+        unknown = sorted(set(symbols) - set(bc.unique_elements))
+        if unknown:
+            raise ValueError("symbols %s are not among the settings' unique_elements" % unknown)
+        self.model = model if model is not None else CEModel(bc, eci)
+        if sorted(eci.keys()) != self.model.names:
+            raise ValueError("All ECIs does not correspond to a correlation function!")
+        self.atoms = atoms
+        self.bc = bc
+        self.corr_func = corr_func
+        self.eci = dict(eci)
+        self._num_threads = 1
+        self._vib = None
+        occ = self.model.species_ids(symbols)
+        # CFHistoryTracker::insert keeps only names that have an ECI and
+        # leaves the others at zero (cpp/src/cf_history_tracker.cpp:82-95)
+        cf0 = np.array([float(corr_func.get(n, 0.0)) for n in self.model.names])
+        eci_arr = np.array([float(eci[n]) for n in self.model.names])
+        o, op = _capi.i8(occ)
+        c, cp = _capi.f64(cf0)
+        e, ep = _capi.f64(eci_arr)
+        h = ctypes.c_void_p()
+        _capi.check(lib.cemc_updater_create(self.model.handle, op, cp, ep, ctypes.byref(h)))
+        self.handle = h
+        self._lib = lib
+        self._occ = occ.copy()
+
+    # --------------------------------------------------------------- helpers
+    def _ids(self, changes):
+        sid = self.model.sid
+        try:
+            site = np.array([int(c[0]) for c in changes], dtype=np.int32)
+            old = np.array([sid[c[1]] for c in changes], dtype=np.int32)
+            new = np.array([sid[c[2]] for c in changes], dtype=np.int32)
+        except KeyError as exc:
+            raise ValueError("unknown symbol %s" % exc)
+        return site, old, new
+
+    def _sync_atoms(self):
+        """Write species that changed on the device back into the atoms object."""
+        occ = np.zeros(self.model.n_sites, dtype=np.int8)
+        _capi.check(self._lib.cemc_updater_get_occupation(self.handle, occ.ctypes.data_as(_capi._i8p)))
+        diff = np.nonzero(occ != self._occ)[0]
+        for i in diff:
+            self.atoms[int(i)].symbol = self.model.species[int(occ[i])]
+        self._occ = occ
+
+    # ------------------------------------------------------------- interface
+    def clear_history(self):
+        _capi.check(self._lib.cemc_updater_clear_history(self.handle))
+
+    def undo_changes(self):
+        _capi.check(self._lib.cemc_updater_undo_changes(self.handle))
+        self._sync_atoms()
+
+    def update_cf(self, system_changes):
+        site, old, new = self._ids([system_changes])
+        rc = self._lib.cemc_updater_update_cf(self.handle, int(site[0]), int(old[0]), int(new[0]))
+        self._sync_atoms()
+        _capi.check(rc)
+
+    def calculate(self, system_changes):
+        site, old, new = self._ids(list(system_changes))
+        e = ctypes.c_double(0.0)
+        s, sp = _capi.i32(site)
+        o, opp = _capi.i32(old)
+        n, np_ = _capi.i32(new)
+        rc = self._lib.cemc_updater_calculate(self.handle, len(site), sp, opp, np_, ctypes.byref(e))
+        self._sync_atoms()
+        _capi.check(rc)
+        return e.value
+
+    def get_energy(self):
+        e = ctypes.c_double(0.0)
+        _capi.check(self._lib.cemc_updater_get_energy(self.handle, ctypes.byref(e)))
+        return e.value
+
+    def get_cf_array(self):
+        out = np.zeros(self.model.n_cf)
+        _capi.check(self._lib.cemc_updater_get_cf(self.handle, out.ctypes.data_as(_capi._dp)))
+        return out
+
+    def get_cf(self):
+        return dict(zip(self.model.names, self.get_cf_array().tolist()))
+
+    def set_ecis(self, ecis):
+        """ECI update by name (cpp/src/ce_updater.cpp:615-629)."""
+        for name in ecis:
+            if name not in self.eci:
+                raise ValueError("No name corresponding to " + name)
+        self.eci.update({k: float(v) for k, v in ecis.items()})
+        a, p = _capi.f64(np.array([self.eci[n] for n in self.model.names]))
+        _capi.check(self._lib.cemc_updater_set_ecis(self.handle, p))
+
+    def get_singlets(self):
+        cf = self.get_cf_array()
+        return np.array([cf[k] for k, n in enumerate(self.model.names) if n.startswith("c1")])
+
+    def get_symbols(self):
+        occ = np.zeros(self.model.n_sites, dtype=np.int8)
+        _capi.check(self._lib.cemc_updater_get_occupation(self.handle, occ.ctypes.data_as(_capi._i8p)))
+        return self.model.symbols(occ)
+
+    def get_occupation(self):
+        occ = np.zeros(self.model.n_sites, dtype=np.int8)
+        _capi.check(self._lib.cemc_updater_get_occupation(self.handle, occ.ctypes.data_as(_capi._i8p)))
+        return occ
+
+    def history_size(self):
+        v = ctypes.c_int32(0)
+        _capi.check(self._lib.cemc_updater_history_size(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def set_num_threads(self, num_threads):
+        """Kept for interface compatibility; the work is spread over GPU lanes."""
+        if int(num_threads) <= 0:
+            raise ValueError("num_threads must be positive")
+        self._num_threads = int(num_threads)
+
+    def add_linear_vib_correction(self, eci_per_kbT):
+        """LinearVibCorrection (cpp/src/linear_vib_correction.cpp:6-16)."""
+        self._vib = dict(eci_per_kbT)
+
+    def vib_energy(self, T):
+        if self._vib is None:
+            return 0.0
+        cf = self.get_cf()
+        e = 0.0
+        for name in sorted(self._vib):
+            e += cf[name] * self._vib[name] * _KB_VIB * T
+        return e
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.cemc_updater_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

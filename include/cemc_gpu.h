@@ -1,0 +1,177 @@
+/*
+ * cemc_gpu.h — C ABI of the MI355X cluster-expansion Metropolis engine
+ * (libcemcgpu.so, HIP kernels for gfx950).
+ *
+ * Plain pointers and sizes only.  Every call returns CEMC_OK (0) or an error
+ * code; cemc_last_error() gives the message of the calling thread's last
+ * failure.  Species are integer ids in sorted-name order, as the reference
+ * assigns them (cpp/src/symbols_with_numbers.cpp:11-21).  Correlation
+ * functions and ECIs are arrays in sorted-name order, the order of the
+ * reference's NamedArray (cpp/src/named_array.cpp:18-23,
+ * cpp/src/cf_history_tracker.cpp:7-16).
+ */
+#ifndef CEMC_GPU_H
+#define CEMC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CEMC_OK = 0,
+  CEMC_EINVAL = 1,   /* maps to ValueError   (std::invalid_argument in the reference) */
+  CEMC_ERUNTIME = 2, /* maps to RuntimeError (std::runtime_error in the reference)    */
+  CEMC_EDEVICE = 3   /* HIP runtime failure                                         */
+};
+
+enum { CEMC_CF_C0 = 0, CEMC_CF_SINGLET = 1, CEMC_CF_CLUSTER = 2 };
+enum { CEMC_MOVE_CANONICAL = 0, CEMC_MOVE_SGC = 1 };
+
+typedef struct cemc_model cemc_model;
+typedef struct cemc_updater cemc_updater;
+typedef struct cemc_batch cemc_batch;
+
+/* Message of the calling thread's last failed call. */
+const char* cemc_last_error(void);
+
+/* Number of visible GPUs (0 when none). */
+int cemc_device_count(int* count);
+
+/* ------------------------------------------------------------------ model
+ * Lattice, clusters, basis and ECIs: the data CEUpdater::init reads from the
+ * settings object (cpp/src/ce_updater.cpp:32-234), flattened.
+ *  entries    one per (translation group, cluster name) of size >= 2
+ *             (ce_updater.cpp:115-147); sub-cluster m of entry c lists its
+ *             n-1 member columns in members[mem_off[c] + m*(n-1) ...] as
+ *             indices into trans columns, and its canonical order in
+ *             order[ord_off[c] + m*n ...] with the semantics of
+ *             CEUpdater::sort_indices (ce_updater.cpp:981-994).
+ *  trans      n_sites x n_trans_cols translation table restricted to the used
+ *             columns (RowSparseStructMatrix, row_sparse_struct_matrix.cpp:10-122).
+ *  periodic_dims / col_cells   optional closed form of trans for a periodic
+ *             supercell with one site per cell, site = (x*L2 + y)*L3 + z:
+ *             trans(i, col) = site(cell(i) + col_cells[col]).  It is used
+ *             only after the library has checked it against `trans` for
+ *             every row.
+ */
+typedef struct cemc_model_desc {
+  int32_t n_sites;
+  int32_t n_species;
+  int32_t n_groups;
+  int32_t n_cf;
+  int32_t n_entries;
+  int32_t n_trans_cols;
+  const double* basis;          /* (n_species-1) x n_species                  */
+  const int32_t* site_group;    /* n_sites, -1 = background site               */
+  const int32_t* cf_kind;       /* n_cf, CEMC_CF_*                             */
+  const int32_t* cf_dec;        /* n_cf, decoration of singlets                */
+  const int32_t* cf_entry;      /* n_groups x n_cf, entry id or -1             */
+  const int32_t* cf_eq_off;     /* n_cf + 1                                    */
+  const int32_t* eq_decos;      /* 4 ints per equivalent decoration            */
+  const int32_t* entry_group;   /* n_entries                                   */
+  const int32_t* entry_size;    /* n_entries                                   */
+  const int32_t* entry_M;       /* n_entries                                   */
+  const int32_t* entry_norm;    /* n_entries, sub-clusters summed over groups  */
+  const int32_t* entry_mem_off; /* n_entries + 1                               */
+  const int32_t* members;
+  const int32_t* entry_ord_off; /* n_entries + 1                               */
+  const int32_t* order;
+  const int32_t* trans;         /* n_sites x n_trans_cols                      */
+  const int32_t* periodic_dims; /* 3 values or NULL                            */
+  const int32_t* col_cells;     /* n_trans_cols x 3 or NULL                    */
+  const double* eci;            /* n_cf                                        */
+} cemc_model_desc;
+
+/* Builds device tables; replaces CEUpdater::init's parsing
+ * (cpp/src/ce_updater.cpp:32-234, cpp/src/cluster.cpp:41-232). */
+int cemc_model_create(const cemc_model_desc* desc, cemc_model** out);
+/* ECIs of the batched path (energy tables rebuilt); CEUpdater::set_ecis
+ * (cpp/src/ce_updater.cpp:615-629). */
+int cemc_model_set_ecis(cemc_model* model, const double* eci);
+/* 1 when the closed-form periodic neighbour path is active, else 0. */
+int cemc_model_is_periodic(const cemc_model* model, int* periodic);
+int cemc_model_destroy(cemc_model* model);
+
+/* --------------------------------------------------------------- updater
+ * One lattice with the reference's history/undo semantics
+ * (class CEUpdater, cpp/include/ce_updater.hpp:47-213; Python binding
+ * PyCEUpdater, cemc/cpp_ext/pyce_updater.pyx:5-60). */
+
+/* PyCEUpdater.__init__ / CEUpdater::init (ce_updater.cpp:32-234).
+ * cf_init may be NULL: the correlation functions are then computed on the
+ * GPU from the occupation. */
+int cemc_updater_create(cemc_model* model, const int8_t* occupation, const double* cf_init,
+                        const double* eci, cemc_updater** out);
+int cemc_updater_destroy(cemc_updater* upd);
+/* PyCEUpdater.update_cf / CEUpdater::update_cf (ce_updater.cpp:287-406). */
+int cemc_updater_update_cf(cemc_updater* upd, int32_t site, int32_t old_id, int32_t new_id);
+/* PyCEUpdater.calculate / CEUpdater::calculate (ce_updater.cpp:471-550, 831-852).
+ * Returns the total energy after the changes. */
+int cemc_updater_calculate(cemc_updater* upd, int32_t n, const int32_t* site, const int32_t* old_id,
+                           const int32_t* new_id, double* energy);
+/* PyCEUpdater.undo_changes / CEUpdater::undo_changes (ce_updater.cpp:408-448). */
+int cemc_updater_undo_changes(cemc_updater* upd);
+/* PyCEUpdater.clear_history / CEUpdater::clear_history (ce_updater.cpp:552-555). */
+int cemc_updater_clear_history(cemc_updater* upd);
+/* PyCEUpdater.get_energy / CEUpdater::get_energy (ce_updater.cpp:236-242). */
+int cemc_updater_get_energy(cemc_updater* upd, double* energy);
+/* PyCEUpdater.get_cf / CEUpdater::get_cf (ce_updater.cpp:570-583); n_cf values. */
+int cemc_updater_get_cf(cemc_updater* upd, double* cf);
+/* PyCEUpdater.set_ecis / CEUpdater::set_ecis (ce_updater.cpp:615-629); n_cf values. */
+int cemc_updater_set_ecis(cemc_updater* upd, const double* eci);
+/* PyCEUpdater.get_symbols (ce_updater.hpp:111) as species ids; n_sites values. */
+int cemc_updater_get_occupation(cemc_updater* upd, int8_t* occupation);
+/* Number of history entries (CFHistoryTracker::history_size, cf_history_tracker.hpp:48). */
+int cemc_updater_history_size(cemc_updater* upd, int32_t* size);
+
+/* ----------------------------------------------------------------- batch
+ * Many independent Markov chains resident on the GPU, advanced by one
+ * persistent launch per call.  Each trial move restates
+ * Montecarlo._mc_step (cemc/mcmc/montecarlo.py:970-1038) with the
+ * proposals of _get_trial_move (montecarlo.py:890-908) or
+ * SGCMonteCarlo._get_trial_move (cemc/mcmc/sgc_montecarlo.py:62-76) drawn
+ * from a fed Philox4x64-10 stream: move t of a chain uses the block with key
+ * (seed, uid) and counter (t, 0, 0, 0). */
+typedef struct cemc_chain_desc {
+  int32_t kind;             /* CEMC_MOVE_CANONICAL or CEMC_MOVE_SGC             */
+  double temperature;       /* kelvin                                            */
+  const double* chem_pot;   /* n_species-1 values subtracted from the singlet
+                               ECIs (sgc_montecarlo.py:239-261), or NULL        */
+  uint64_t seed;
+  uint64_t uid;
+  const int8_t* occupation; /* n_sites species ids                              */
+} cemc_chain_desc;
+
+int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const cemc_chain_desc* chains,
+                      cemc_batch** out);
+int cemc_batch_destroy(cemc_batch* batch);
+/* Advance chain c by n_moves[c] trial moves (n_moves NULL: `moves` each).
+ * Montecarlo.runMC main loop (montecarlo.py:732-844) without equilibration. */
+int cemc_batch_run(cemc_batch* batch, const int64_t* n_moves, int64_t moves);
+/* Same as cemc_batch_run, also returning the launch's device time in ms
+ * measured with HIP events on the batch's stream. */
+int cemc_batch_run_timed(cemc_batch* batch, const int64_t* n_moves, int64_t moves, double* kernel_ms);
+/* Same as cemc_batch_run, also recording accept (1) / reject (0) of every
+ * move: bit t of chain c at accept_bits[c * words_per_chain + t / 32], bit
+ * t % 32.  Used to compare acceptance sequences with the reference. */
+int cemc_batch_run_logged(cemc_batch* batch, const int64_t* n_moves, int64_t moves, uint32_t* accept_bits,
+                          int64_t words_per_chain);
+/* Current energies (n_chains). */
+int cemc_batch_get_energy(cemc_batch* batch, double* energy);
+/* Occupation of one chain (n_sites of that chain). */
+int cemc_batch_get_occupation(cemc_batch* batch, int32_t chain, int8_t* occupation);
+/* Per chain: moves done, accepted moves, sum of E, sum of E^2 over moves. */
+int cemc_batch_get_stats(cemc_batch* batch, int64_t* moves_done, int64_t* n_accept, double* sum_e,
+                         double* sum_e2);
+/* Correlation functions of chain c computed from scratch on the GPU (n_cf of
+ * that chain's model) and the energy they give. */
+int cemc_batch_get_cf(cemc_batch* batch, int32_t chain, double* cf, double* energy);
+/* Temperature of one chain (parallel tempering uses it to exchange replicas). */
+int cemc_batch_set_temperature(cemc_batch* batch, int32_t chain, double temperature);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

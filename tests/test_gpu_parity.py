@@ -1,0 +1,179 @@
+"""GPU parity against the reference fixtures and the oracle.
+
+Acceptance sequences must be bit-identical; energies within 1e-10 eV/site and
+correlation functions within 1e-10 of the reference's (the tolerance of the
+target: |E_gpu - E_ref| / N <= 1e-10 eV/site).
+"""
+import numpy as np
+import pytest
+
+from golden_io import CHAIN_FIXTURES, ChainFixture, load_api_fixture
+
+E_TOL = 1e-10  # eV per site
+CF_TOL = 1e-10
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(packed, n):
+    return np.unpackbits(np.asarray(packed, dtype=np.uint8), bitorder="little")[:n]
+
+
+def _batch_for(fx, settings=None):
+    from cemc_amd.batch import ChainBatch
+    from cemc_amd.model import CEModel
+    model = CEModel(settings if settings is not None else fx.settings, fx.eci)
+    chain = dict(kind=fx.kind, temperature=fx.temperature, occupation=fx.occ0, seed=fx.seed,
+                 uid=fx.chain_uid, chem_pot=fx.meta["mu"] or None)
+    return model, ChainBatch([model], [chain])
+
+
+@pytest.mark.parametrize("fname", CHAIN_FIXTURES)
+def test_batch_reproduces_reference_chain(gpu, fname):
+    fx = ChainFixture(fname)
+    print("fixture", fname, "moves", fx.n_moves, flush=True)
+    model, batch = _batch_for(fx)
+    n = fx.n_sites
+    bits = []
+    for seg in range(fx.n_moves // fx.cf_every):
+        for sub in range(fx.cf_every // fx.trace_every):
+            log = batch.run_logged(fx.trace_every)
+            bits.append(_bits(log[0], fx.trace_every))
+            idx = seg * (fx.cf_every // fx.trace_every) + sub
+            e = batch.energies()[0]
+            assert abs(e - fx.e_trace[idx]) / n <= E_TOL, (idx, e, fx.e_trace[idx])
+        cf, e_cf = batch.cf(0)
+        assert np.max(np.abs(cf - fx.cf_checkpoints[seg])) <= CF_TOL
+    got = np.concatenate(bits)
+    ref = _bits(fx.accept_bits, fx.n_moves)
+    mism = np.nonzero(got != ref)[0]
+    assert mism.size == 0, "first mismatching move %d" % (mism[0] if mism.size else -1)
+    assert np.array_equal(batch.occupation(0), fx.occ_final)
+    st = batch.stats()
+    assert st["accepted"][0] == fx.n_accept
+    assert st["moves"][0] == fx.n_moves
+
+
+def test_periodic_and_table_paths_agree(gpu):
+    """The closed-form neighbour path reproduces the translation-table path."""
+    fx = ChainFixture("binary_canonical_plain.npz")
+    model_p, batch_p = _batch_for(fx)
+    assert model_p.periodic
+
+    class NoDims:
+        pass
+
+    plain = NoDims()
+    for key, val in vars(fx.settings).items():
+        if key not in ("dims", "cells"):
+            setattr(plain, key, val)
+    model_t, batch_t = _batch_for(fx, plain)
+    assert not model_t.periodic
+    a = batch_p.run_logged(20000)
+    b = batch_t.run_logged(20000)
+    assert np.array_equal(a, b)
+    assert np.array_equal(batch_p.occupation(0), batch_t.occupation(0))
+    assert abs(batch_p.energies()[0] - batch_t.energies()[0]) <= 1e-12 * fx.n_sites
+
+
+def test_initial_cf_matches_bruteforce(gpu):
+    """GPU correlation functions from scratch vs the oracle's brute force, with
+    symmetrised decorations (equivalent sites) on a ternary and a quaternary cell."""
+    from cemc_amd.batch import ChainBatch
+    from cemc_amd.lattice import FCCSettings, random_occupation, seeded_ecis
+    from cemc_amd.model import CEModel
+    from oracle.brute_force import brute_force_cf
+    for species, shells, L in ((["A", "B", "C"], {2: 4, 3: 3, 4: 2}, 6), (["A", "B", "C", "D"], {2: 3, 3: 3, 4: 3}, 5)):
+        s = FCCSettings(L, species, shells)
+        eci = seeded_ecis(s, 5, 0.01)
+        occ = random_occupation(s.n_sites, len(species), 17)
+        model = CEModel(s, eci)
+        batch = ChainBatch([model], [dict(kind=0, temperature=1000.0, occupation=occ, seed=1, uid=0)])
+        cf, e = batch.cf(0)
+        ref = brute_force_cf(s, occ, model.names)
+        ref_arr = np.array([ref[n] for n in model.names])
+        assert np.max(np.abs(cf - ref_arr)) <= 1e-12
+        e_ref = s.n_sites * float(np.dot(model.eci, ref_arr))
+        assert abs(e - e_ref) / s.n_sites <= 1e-12
+        assert abs(batch.energies()[0] - e) <= 1e-12 * s.n_sites
+
+
+def test_updater_replays_reference_api_log(gpu):
+    """PyCEUpdater on the GPU against the reference's recorded call sequence."""
+    from cemc_amd.lattice import SimpleAtoms, random_occupation, seeded_ecis
+    from cemc_amd.updater import PyCEUpdater
+    from oracle.brute_force import brute_force_cf
+    meta, bc = load_api_fixture()
+    names = meta["names"]
+    eci = seeded_ecis(bc, meta["eci_seed"], meta["eci_sigma"])
+    occ = random_occupation(bc.n_sites, 3, meta["occ_seed"])
+    cf0 = brute_force_cf(bc, occ, names)
+    atoms = SimpleAtoms([bc.species[s] for s in occ])
+    upd = PyCEUpdater(atoms, bc, cf0, eci)
+    n = bc.n_sites
+
+    def check(entry):
+        cf = upd.get_cf()
+        assert max(abs(cf[k] - v) for k, v in zip(names, entry["cf"])) <= CF_TOL, entry["op"]
+        assert abs(upd.get_energy() - entry["energy"]) / n <= E_TOL, entry["op"]
+        assert upd.get_symbols() == entry["symbols"], entry["op"]
+        assert [a.symbol for a in atoms] == entry["symbols"], entry["op"]
+        assert np.allclose(upd.get_singlets(), entry["singlets"], atol=CF_TOL, rtol=0)
+
+    for entry in meta["log"]:
+        op, val = entry["op"], entry["value"]
+        if op == "update_cf":
+            upd.update_cf(tuple(val))
+        elif op == "calculate":
+            e = upd.calculate([tuple(c) for c in val[:-1]])
+            assert abs(e - val[-1]) / n <= E_TOL
+        elif op == "undo_changes":
+            upd.undo_changes()
+        elif op == "clear_history":
+            upd.clear_history()
+        elif op == "set_ecis":
+            upd.set_ecis(val)
+        check(entry)
+    kind, _ = meta["errors"]["background"]
+    cur = upd.get_symbols()
+    sp = bc.species
+    with pytest.raises(RuntimeError):
+        upd.calculate([(5, cur[5], sp[(sp.index(cur[5]) + 1) % 3])])
+    assert kind == "RuntimeError"
+
+
+def test_updater_matches_oracle_with_equivalent_sites(gpu):
+    """Random updater call sequences on a ternary cell whose clusters have
+    equivalent sites (symmetrised decorations): GPU vs the C oracle."""
+    from cemc_amd.lattice import FCCSettings, SimpleAtoms, random_occupation, seeded_ecis
+    from cemc_amd.updater import PyCEUpdater
+    from oracle.brute_force import brute_force_cf
+    from oracle.oracle import OracleModel, OracleUpdater
+    s = FCCSettings(5, ["Al", "Mg", "Si"], {2: 3, 3: 2, 4: 1})
+    names = s.cf_names()
+    eci = seeded_ecis(s, 9, 0.01)
+    occ = random_occupation(s.n_sites, 3, 21)
+    cf0 = brute_force_cf(s, occ, names)
+    upd = PyCEUpdater(SimpleAtoms([s.species[v] for v in occ]), s, cf0, eci)
+    ora = OracleUpdater(OracleModel(s, names), occ, cf0, eci)
+    rng = np.random.Generator(np.random.Philox(3))
+    cur = list(occ)
+    for step in range(60):
+        i = int(rng.integers(0, s.n_sites))
+        j = int(rng.integers(0, s.n_sites))
+        if cur[i] != cur[j] and step % 3 == 0:
+            chs = [(i, s.species[cur[i]], s.species[cur[j]]), (j, s.species[cur[j]], s.species[cur[i]])]
+        else:
+            chs = [(i, s.species[cur[i]], s.species[(cur[i] + 1 + step % 2) % 3])]
+        e_gpu = upd.calculate(chs)
+        e_ora = ora.calculate(chs)
+        assert abs(e_gpu - e_ora) / s.n_sites <= 1e-12
+        if step % 4 == 1:
+            upd.undo_changes()
+            ora.undo_changes()
+        else:
+            upd.clear_history()
+            ora.clear_history()
+        cur = [s.species.index(v) for v in upd.get_symbols()]
+        assert cur == list(ora.get_occupation())
+        assert np.max(np.abs(upd.get_cf_array() - ora.get_cf_array())) <= 1e-12
