@@ -31,7 +31,7 @@ class ModelDesc(ctypes.Structure):
         ("cf_entry", _i32p), ("cf_eq_off", _i32p), ("eq_decos", _i32p), ("entry_group", _i32p),
         ("entry_size", _i32p), ("entry_M", _i32p), ("entry_norm", _i32p), ("entry_mem_off", _i32p),
         ("members", _i32p), ("entry_ord_off", _i32p), ("order", _i32p), ("trans", _i32p),
-        ("periodic_dims", _i32p), ("col_cells", _i32p), ("eci", _dp),
+        ("periodic_dims", _i32p), ("col_cells", _i32p), ("eci", _dp), ("background", _i32p),
     ]
 
 

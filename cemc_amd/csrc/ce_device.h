@@ -59,7 +59,8 @@ struct ModelDev {
   const double* gtab;
   const int32_t* trans;       // n_sites * K
   const uint32_t* col_cells;  // K packed cell offsets (x | y<<8 | z<<16)
-  const int8_t* site_group;   // n_sites (-1 background)
+  const int8_t* site_group;   // n_sites translation group (-1: none, background only)
+  const uint8_t* background;  // n_sites, 1 = background site (never moved)
   const int32_t* movable;     // non-background sites (only read with background)
   // correlation-function tables
   const int32_t* cf_kind;     // n_cf: 0 c0, 1 singlet, 2 cluster

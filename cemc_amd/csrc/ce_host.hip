@@ -93,6 +93,7 @@ struct cemc_model {
   int dims[3] = {0, 0, 0};
   std::vector<double> basis, eci, gtab;
   std::vector<int32_t> site_group, cf_kind, cf_dec, cf_entry, cf_eq_off, movable;
+  std::vector<uint8_t> background;
   std::vector<uint32_t> eq_decos, col_cells;
   std::vector<int32_t> entry_group, entry_size, entry_M, entry_norm, entry_goff;
   std::vector<int32_t> trans;
@@ -106,6 +107,7 @@ struct cemc_model {
   int32_t* d_trans = nullptr;
   uint32_t* d_cells = nullptr;
   int8_t* d_site_group = nullptr;
+  uint8_t* d_background = nullptr;
   int32_t* d_movable = nullptr;
   int32_t *d_cf_kind = nullptr, *d_cf_dec = nullptr, *d_cf_entry = nullptr, *d_cf_eq_off = nullptr;
   uint32_t* d_eq_decos = nullptr;
@@ -131,6 +133,7 @@ cemc_model::~cemc_model() {
   dev_free(d_trans);
   dev_free(d_cells);
   dev_free(d_site_group);
+  dev_free(d_background);
   dev_free(d_movable);
   dev_free(d_cf_kind);
   dev_free(d_cf_dec);
@@ -229,6 +232,7 @@ void cemc_model::refresh_view() {
   v.trans = d_trans;
   v.col_cells = d_cells;
   v.site_group = d_site_group;
+  v.background = d_background;
   v.movable = d_movable;
   v.cf_kind = d_cf_kind;
   v.cf_dec = d_cf_dec;
@@ -259,14 +263,17 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
   const int N = m->n_sites, S = m->n_species, B = m->n_bfs, G = m->n_groups;
   m->basis.assign(d->basis, d->basis + B * S);
   m->site_group.assign(d->site_group, d->site_group + N);
+  m->background.assign(N, 0);
+  if (d->background)
+    for (int i = 0; i < N; ++i) m->background[i] = d->background[i] != 0;
   m->grp_count.assign(G, 0);
   for (int i = 0; i < N; ++i) {
     const int g = m->site_group[i];
     require(g >= -1 && g < G, "site group out of range");
-    if (g >= 0) {
-      m->grp_count[g] += 1;
-      m->movable.push_back(i);
-    }
+    require(g >= 0 || m->background[i], "a site has not been assigned to any translational symmetry group");
+    // trans_symm_group_count counts background sites too (ce_updater.cpp:768-775)
+    if (g >= 0) m->grp_count[g] += 1;
+    if (!m->background[i]) m->movable.push_back(i);
   }
   require(!m->movable.empty(), "no movable site");
   m->cf_kind.assign(d->cf_kind, d->cf_kind + m->n_cf);
@@ -396,6 +403,7 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
   m->d_trans = upload(m->trans);
   m->d_cells = upload(m->col_cells);
   m->d_site_group = upload(sg);
+  m->d_background = upload(m->background);
   m->d_movable = upload(m->movable);
   m->d_cf_kind = upload(m->cf_kind);
   m->d_cf_dec = upload(m->cf_dec);
@@ -512,7 +520,7 @@ int updater_stage(cemc_updater* u, int site, int old_id, int new_id, std::vector
     g_last_error = "species id out of range";
     return CEMC_EINVAL;
   }
-  if (m.site_group[site] < 0) {
+  if (m.background[site]) {
     g_last_error = "Attempting to move a background atom!";
     return CEMC_ERUNTIME;
   }
@@ -918,7 +926,7 @@ int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const c
           for (int s = 0; s < m->n_species; ++s) {
             int cnt = 0;
             for (int i = 0; i < N; ++i)
-              if (m->site_group[i] >= 0 && d.occupation[i] == s) {
+              if (!m->background[i] && d.occupation[i] == s) {
                 trk_pool.push_back(uint16_t(i));
                 ++cnt;
               }

@@ -253,15 +253,16 @@ __global__ __launch_bounds__(256) void cf_partial_kernel(const ModelDev* __restr
   const int tid = int(threadIdx.x);
   const int i = int(blockIdx.x) * 256 + tid;
   const int g = (i < N) ? site_group(m, i) : -1;
+  const bool movable = i < N && !(m.has_background && m.background[i]);
   const int S = m.n_species;
   const RefSite ref = make_ref(m, i < N ? i : 0);
   for (int k = 0; k < m.n_cf; ++k) {
     double v = 0.0;
     const int kind = m.cf_kind[k];
-    if (g >= 0) {
-      if (kind == 1) {
-        v = m.basis[m.cf_dec[k] * S + occ[i]] / double(N);
-      } else if (kind == 2) {
+    if (kind == 1) {
+      if (movable) v = m.basis[m.cf_dec[k] * S + occ[i]] / double(N);
+    } else if (g >= 0) {
+      if (kind == 2) {
         const int ent = m.cf_entry[g * m.n_cf + k];
         if (ent >= 0) {
           const ClusterEntry E = m.entries[ent];

@@ -58,7 +58,8 @@ class CEModel:
         background[[int(i) for i in settings.background_indices]] = True
         if np.any((site_group < 0) & ~background):
             raise RuntimeError("a site has not been assigned to any translational symmetry group")
-        site_group[background] = -1
+        # background sites keep their group: trans_symm_group_count includes
+        # them (cpp/src/ce_updater.cpp:768-775)
         self.site_group = site_group
         self.background = background
         n_groups = len(settings.cluster_info)
@@ -140,13 +141,13 @@ class CEModel:
                 trans[:] = tm[:, cols]
             else:
                 for i in range(n_sites):
-                    if background[i]:
-                        continue
                     row = tm[i]
                     for j, c in enumerate(cols):
-                        if c not in row:
+                        if c in row:
+                            trans[i, j] = row[c]
+                        elif not background[i]:
+                            # background rows may be incomplete (ce_updater.cpp:924-927)
                             raise ValueError("Requested value %d is not a key in the dictionary!" % c)
-                        trans[i, j] = row[c]
         dims = cells = None
         if hasattr(settings, "dims") and hasattr(settings, "cells"):
             dims = np.array(settings.dims, dtype=np.int32)
@@ -187,6 +188,7 @@ class CEModel:
         desc.periodic_dims = arr(dims, "i32") if dims is not None else None
         desc.col_cells = arr(cells.ravel(), "i32") if cells is not None else None
         desc.eci = arr(self.eci, "f64")
+        desc.background = arr(background.astype(np.int32), "i32")
         handle = ctypes.c_void_p()
         _capi.check(lib.cemc_model_create(ctypes.byref(desc), ctypes.byref(handle)))
         self._keep = []

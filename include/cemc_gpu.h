@@ -64,7 +64,8 @@ typedef struct cemc_model_desc {
   int32_t n_entries;
   int32_t n_trans_cols;
   const double* basis;          /* (n_species-1) x n_species                  */
-  const int32_t* site_group;    /* n_sites, -1 = background site               */
+  const int32_t* site_group;    /* n_sites translation group; -1 (no group) is
+                                   allowed for background sites only           */
   const int32_t* cf_kind;       /* n_cf, CEMC_CF_*                             */
   const int32_t* cf_dec;        /* n_cf, decoration of singlets                */
   const int32_t* cf_entry;      /* n_groups x n_cf, entry id or -1             */
@@ -82,6 +83,8 @@ typedef struct cemc_model_desc {
   const int32_t* periodic_dims; /* 3 values or NULL                            */
   const int32_t* col_cells;     /* n_trans_cols x 3 or NULL                    */
   const double* eci;            /* n_cf                                        */
+  const int32_t* background;    /* n_sites or NULL; nonzero = background site,
+                                   never moved (ce_updater.cpp:330-332)         */
 } cemc_model_desc;
 
 /* Builds device tables; replaces CEUpdater::init's parsing

@@ -44,6 +44,7 @@ struct ceo_model {
   int n_sites, n_species, n_bfs, n_groups, n_cf, n_clusters, K;
   double* basis;
   int* site_group;
+  int* background;
   int* group_count;
   int* cf_kind;
   int* cf_dec;
@@ -88,7 +89,7 @@ static void* dup_bytes(const void* src, size_t n) {
 }
 
 ceo_model* ceo_model_create(int n_sites, int n_species, int n_bfs, const double* basis,
-                            int n_groups, const int* site_group, int n_cf, const int* cf_kind,
+                            int n_groups, const int* site_group, const int* background, int n_cf, const int* cf_kind,
                             const int* cf_dec, const int* cf_cluster, const int* cf_eq_off,
                             const int* eq_decos, int n_clusters, const int* cl_size,
                             const int* cl_M, const int* cl_norm, const int* cl_mem_off,
@@ -105,6 +106,9 @@ ceo_model* ceo_model_create(int n_sites, int n_species, int n_bfs, const double*
   m->K = K;
   m->basis = (double*)dup_bytes(basis, sizeof(double) * (size_t)n_bfs * n_species);
   m->site_group = (int*)dup_bytes(site_group, sizeof(int) * (size_t)n_sites);
+  m->background = (int*)dup_bytes(background, sizeof(int) * (size_t)n_sites);
+  /* trans_symm_group_count counts every site of the group, background sites
+   * included (ce_updater.cpp:768-775) */
   m->group_count = (int*)calloc((size_t)n_groups + 1, sizeof(int));
   for (int i = 0; i < n_sites; ++i)
     if (site_group[i] >= 0) m->group_count[site_group[i]] += 1;
@@ -128,6 +132,7 @@ void ceo_model_destroy(ceo_model* m) {
   if (!m) return;
   free(m->basis);
   free(m->site_group);
+  free(m->background);
   free(m->group_count);
   free(m->cf_kind);
   free(m->cf_dec);
@@ -255,7 +260,7 @@ static int update_cf_impl(ceo_chain* c, int site, int old_id, int new_id) {
   if (old_id == new_id) return CEO_OK;
   if (site < 0 || site >= m->n_sites) return fail(CEO_EINVAL, "site index out of range");
   if (new_id < 0 || new_id >= m->n_species) return fail(CEO_EINVAL, "unknown species");
-  if (m->site_group[site] < 0) return fail(CEO_ERUNTIME, "Attempting to move a background atom!");
+  if (m->background[site]) return fail(CEO_ERUNTIME, "Attempting to move a background atom!");
   const double* cur = ring_slot(c, current_slot(c));
   /* get_next (cf_history_tracker.cpp:46-56) */
   unsigned slot = c->current;
@@ -413,12 +418,12 @@ int ceo_run(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t 
     if (!c->trk_valid) {
       memset(count, 0, sizeof(int) * (size_t)m->n_species);
       for (int i = 0; i < N; ++i)
-        if (m->site_group[i] >= 0) count[c->occ[i]] += 1;
+        if (!m->background[i]) count[c->occ[i]] += 1;
       trk_off[0] = 0;
       for (int s = 0; s < m->n_species; ++s) trk_off[s + 1] = trk_off[s] + count[s];
       int* fill = (int*)calloc((size_t)(unsigned)m->n_species + 1, sizeof(int));
       for (int i = 0; i < N; ++i) {
-        if (m->site_group[i] < 0) continue;
+        if (m->background[i]) continue;
         int s = c->occ[i];
         tracker[trk_off[s] + fill[s]++] = i;
       }
@@ -432,7 +437,7 @@ int ceo_run(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t 
     c->trk_valid = 0;
     movable = (int*)malloc(sizeof(int) * (size_t)(N ? N : 1));
     for (int i = 0; i < N; ++i)
-      if (m->site_group[i] >= 0) movable[n_movable++] = i;
+      if (!m->background[i]) movable[n_movable++] = i;
     if (m->n_species < 2 || n_movable == 0) {
       free(movable);
       return fail(CEO_EINVAL, "SGC moves need two species and a movable site");

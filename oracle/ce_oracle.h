@@ -25,7 +25,8 @@ typedef struct ceo_chain ceo_chain;
 
 /* Model tables (shared, read only).
  *  basis[d*n_species + s]              phi_d(s)          (basis_function.cpp:34-42)
- *  site_group[i]                       translation group, -1 = background
+ *  site_group[i]                       translation group (-1: none; only allowed for background)
+ *  background[i]                       nonzero: background site, never moved
  *  cf_kind[k], cf_dec[k]               per correlation function in sorted-name order
  *  cf_cluster[g*n_cf + k]              cluster id in group g (-1: not present)
  *  cf_eq_off[k]..cf_eq_off[k+1]        equivalent decorations of CF k (index into eq_decos,
@@ -36,7 +37,7 @@ typedef struct ceo_chain ceo_chain;
  *  trans[i*K + slot]                   translation table restricted to used columns
  */
 ceo_model* ceo_model_create(int n_sites, int n_species, int n_bfs, const double* basis,
-                            int n_groups, const int* site_group, int n_cf, const int* cf_kind,
+                            int n_groups, const int* site_group, const int* background, int n_cf, const int* cf_kind,
                             const int* cf_dec, const int* cf_cluster, const int* cf_eq_off,
                             const int* eq_decos, int n_clusters, const int* cl_size,
                             const int* cl_M, const int* cl_norm, const int* cl_mem_off,

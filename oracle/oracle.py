@@ -36,7 +36,7 @@ def load_library():
     ip = ctypes.POINTER(ctypes.c_int)
     dp = ctypes.POINTER(ctypes.c_double)
     lib.ceo_model_create.restype = P
-    lib.ceo_model_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ip,
+    lib.ceo_model_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, ip, ip,
                                      ctypes.c_int, ip, ip, ip, ip, ip, ctypes.c_int, ip, ip, ip, ip, ip,
                                      ip, ip, ctypes.c_int, ip]
     lib.ceo_model_destroy.argtypes = [P]
@@ -115,9 +115,10 @@ class OracleModel:
         for g, sites in enumerate(bc.index_by_trans_symm):
             for i in sites:
                 site_group[int(i)] = g
-        for i in bkg:
-            site_group[i] = -1
+        background = np.zeros(n_sites, dtype=np.int32)
+        background[sorted(bkg)] = 1
         self.site_group = site_group
+        self.background = background
         n_groups = len(bc.cluster_info)
         cols = sorted({int(c) for info in bc.cluster_info for cl in info.values()
                        for sub in cl["indices"] for c in sub})
@@ -129,7 +130,7 @@ class OracleModel:
             trans[:, :K] = tm[:, cols]
         else:
             for i in range(n_sites):
-                if site_group[i] < 0:
+                if background[i]:
                     continue
                 row = tm[i]
                 for j, c in enumerate(cols):
@@ -183,7 +184,7 @@ class OracleModel:
                     eq_decos.extend(list(e) + [0] * (4 - len(e)))
             eq_off.append(len(eq_decos) // 4)
         arrs = dict(
-            basis=np.ascontiguousarray(basis.ravel()), site_group=site_group,
+            basis=np.ascontiguousarray(basis.ravel()), site_group=site_group, background=background,
             kind=kind, dec=dec, cf_cluster=np.ascontiguousarray(cf_cluster.ravel()),
             eq_off=np.array(eq_off, dtype=np.int32), eq_decos=np.array(eq_decos + [0] * 4, dtype=np.int32),
             cl_size=np.array(cl_size + [0], dtype=np.int32), cl_M=np.array(cl_M + [0], dtype=np.int32),
@@ -194,7 +195,7 @@ class OracleModel:
         self.n_cf = n_cf
         self.n_species = n_s
         self.handle = lib.ceo_model_create(
-            n_sites, n_s, n_bfs, _dp(arrs["basis"]), n_groups, _ip(site_group), n_cf, _ip(kind),
+            n_sites, n_s, n_bfs, _dp(arrs["basis"]), n_groups, _ip(site_group), _ip(background), n_cf, _ip(kind),
             _ip(dec), _ip(arrs["cf_cluster"]), _ip(arrs["eq_off"]), _ip(arrs["eq_decos"]), len(cl_size),
             _ip(arrs["cl_size"]), _ip(arrs["cl_M"]), _ip(arrs["cl_norm"]), _ip(arrs["mem_off"]),
             _ip(arrs["members"]), _ip(arrs["ord_off"]), _ip(arrs["order"]), K, _ip(arrs["trans"]))
