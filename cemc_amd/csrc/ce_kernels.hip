@@ -51,36 +51,93 @@ uint32_t sweep_lds_bytes(int n_sites, int canonical, int n_rec, int K, int perio
 }
 
 // ---------------------------------------------------------------------------
-// Energy change of flipping site `ref` from species `old` to `nw`, restricted
-// to the sub-cluster records [it_begin, it_end) handled by this lane.  When
-// `patch_site` >= 0 the species of that site reads as `patch_species` (the
-// second flip of a swap sees the first one applied, ce_updater.cpp:845-852).
+// Sweep kernel.
+//
+// Everything the move loop reads is staged before the loop: per-model scalars
+// in SGPRs (SweepCtx), small per-chain tables (species lists, basis
+// differences, singlet energy changes) and the model tables in LDS, and for
+// single-group models each lane's sub-cluster records in VGPRs.  One trial
+// move then costs: uniform decode of a pre-generated Philox block, one round
+// of LDS gathers of member species, one LDS read of the energy table per
+// sub-cluster, a DPP reduction and the Metropolis test.
+
+constexpr int kRegRecs = 4;  // sub-cluster records held in VGPRs per lane (<= 256 per group)
+
+struct SweepCtx {
+  int S, B, K;
+  int L1, L2, L3, L23;
+  uint32_t mag23, mag3;    // floor((2^32-1)/d) for the divisors L2*L3 and L3
+  const int32_t* trans;
+};
+
+__device__ inline uint32_t fast_div(uint32_t n, uint32_t d, uint32_t magic) {
+  // exact for n, d < 2^16: the estimate is the quotient or one less
+  uint32_t q = __umulhi(n, magic);
+  if (n - q * d >= d) ++q;
+  return q;
+}
+
+struct Ref {
+  int site, x, y, z;
+};
+
 template <bool kPeriodic>
-__device__ inline double flip_part(const ModelDev& m, const RefSite& ref, const SubRec* recs, int rec_off,
-                                   int rec_cnt, int lane, const int8_t* occ, const uint32_t* cells,
-                                   const double* gtab, const double* dphi, int patch_site,
-                                   int patch_species) {
-  double part = 0.0;
-  const int S = m.n_species;
-  const int B = m.n_bfs;
-  for (int it = lane; it < rec_cnt; it += 64) {
-    const SubRec r = recs[rec_off + it];
-    const int nm = rec_size(r) - 1;
-    int code = 0;
-    for (int q = nm - 1; q >= 0; --q) {
-      const int32_t nb = neighbour<kPeriodic>(m, ref, rec_slot(r, q), cells);
-      int s = occ[nb];
-      s = (nb == patch_site) ? patch_species : s;
-      code = code * S + s;
-    }
-    const int p = m.pow_s[nm];
-    const double* g = gtab + rec_gbase(r) + code;
-#pragma unroll
-    for (int d = 0; d < kMaxBasis; ++d) {
-      if (d < B) part += dphi[d] * g[d * p];
-    }
+__device__ __attribute__((always_inline)) inline Ref ref_of(const SweepCtx& c, int site) {
+  Ref r;
+  r.site = site;
+  if constexpr (kPeriodic) {
+    const uint32_t x = fast_div(uint32_t(site), uint32_t(c.L23), c.mag23);
+    const uint32_t rem = uint32_t(site) - x * uint32_t(c.L23);
+    const uint32_t y = fast_div(rem, uint32_t(c.L3), c.mag3);
+    r.x = int(x);
+    r.y = int(y);
+    r.z = int(rem - y * uint32_t(c.L3));
+  } else {
+    r.x = r.y = r.z = 0;
   }
-  return part;
+  return r;
+}
+
+template <bool kPeriodic>
+__device__ __attribute__((always_inline)) inline int nbr(const SweepCtx& c, const Ref& r, uint32_t slot, const uint32_t* cells) {
+  if constexpr (kPeriodic) {
+    const uint32_t cc = cells[slot];
+    int x = r.x + int(cc & 0xFFu);
+    int y = r.y + int((cc >> 8) & 0xFFu);
+    int z = r.z + int((cc >> 16) & 0xFFu);
+    x -= (x >= c.L1) ? c.L1 : 0;
+    y -= (y >= c.L2) ? c.L2 : 0;
+    z -= (z >= c.L3) ? c.L3 : 0;
+    return (x * c.L2 + y) * c.L3 + z;
+  } else {
+    return c.trans[int64_t(r.site) * c.K + slot];
+  }
+}
+
+// Energy-change contribution of one sub-cluster record when the reference
+// site's basis values change by dphi (decoration-indexed).
+template <bool kPeriodic>
+__device__ __attribute__((always_inline)) inline double rec_de(const SweepCtx& c, SubRec r, const Ref& ref, const int8_t* occ,
+                                const uint32_t* cells, const double* gtab, const double* dphi, int patch_site,
+                                int patch_species) {
+  const int nm = rec_size(r) - 1;
+  int code = 0;
+  for (int q = nm - 1; q >= 0; --q) {
+    const int nb = nbr<kPeriodic>(c, ref, rec_slot(r, q), cells);
+    int sp = occ[nb];
+    sp = (nb == patch_site) ? patch_species : sp;
+    code = code * c.S + sp;
+  }
+  // S^nm without indexing (a select over struct fields can end up in scratch)
+  int p = c.S;
+  p *= (nm >= 2) ? c.S : 1;
+  p *= (nm >= 3) ? c.S : 1;
+  const double* g = gtab + rec_gbase(r) + code;
+  double v = 0.0;
+#pragma unroll
+  for (int d = 0; d < kMaxBasis; ++d)
+    if (d < c.B) v += dphi[d] * g[d * p];
+  return v;
 }
 
 template <bool kPeriodic, bool kGtabLds>
@@ -93,14 +150,16 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
                                                    const int64_t* __restrict__ n_moves,
                                                    uint32_t* __restrict__ accept_log, int64_t log_words) {
   extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ int4 sp_info[kMaxSpecies];                          // (species, count, seg_off, -)
+  __shared__ double dphi_tab[kMaxSpecies * kMaxSpecies * kMaxBasis];  // [(old*S+new)*B + d]
+  __shared__ double dsingle_tab[kMaxSpecies * kMaxSpecies];      // singlet energy change per (old,new)
+  __shared__ int grp_tab[2 * kMaxGroups];
   const int chain = chain_order[blockIdx.x];
   const int lane = int(threadIdx.x);
   const ChainParams* __restrict__ pp = params + chain;
-  const int model_id = pp->model;
-  const ModelDev& m = models[model_id];
+  const ModelDev& m = models[pp->model];
   const int N = pp->n_sites;
-  const int kind = pp->kind;
-  const int canonical = kind == 0;
+  const bool canonical = pp->kind == 0;
   const LdsLayout L = lds_layout(N, canonical, m.n_rec, m.K, m.periodic, kGtabLds ? m.gtab_size : 0);
   int8_t* occ = reinterpret_cast<int8_t*>(lds + L.occ);
   uint16_t* trk = reinterpret_cast<uint16_t*>(lds + L.trk);
@@ -108,19 +167,59 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
   uint32_t* cells = reinterpret_cast<uint32_t*>(lds + L.cells);
   double* gtab_l = reinterpret_cast<double*>(lds + L.gtab);
 
-  // ---- stage chain state and model tables into LDS
+  SweepCtx c;
+  c.S = m.n_species;
+  c.B = m.n_bfs;
+  c.K = m.K;
+  c.L1 = m.dims[0];
+  c.L2 = m.dims[1];
+  c.L3 = m.dims[2];
+  c.L23 = c.L2 * c.L3;
+  c.mag23 = kPeriodic ? 0xFFFFFFFFu / uint32_t(c.L23) : 0u;
+  c.mag3 = kPeriodic ? 0xFFFFFFFFu / uint32_t(c.L3) : 0u;
+  c.trans = m.trans;
+  const bool multi_group = m.n_groups > 1 || m.has_background;
+  const bool has_bkg = m.has_background != 0;
+  const int n_pick = has_bkg ? m.n_movable : N;
+
+  // ---- stage chain state, per-chain tables and model tables into LDS
   const int8_t* gocc = occ_pool + pp->occ_off;
   for (int i = lane; i < N; i += 64) occ[i] = gocc[i];
   uint16_t* gtrk = trk_pool + pp->trk_off;
   if (canonical)
     for (int i = lane; i < N; i += 64) trk[i] = gtrk[i];
   for (int i = lane; i < m.n_rec; i += 64) recs[i] = m.recs[i];
-  if (m.periodic)
+  if (kPeriodic)
     for (int i = lane; i < m.K; i += 64) cells[i] = m.col_cells[i];
   if constexpr (kGtabLds)
     for (int i = lane; i < m.gtab_size; i += 64) gtab_l[i] = m.gtab[i];
   const double* gtab = kGtabLds ? gtab_l : m.gtab;
+  if (lane < kMaxSpecies) {
+    const int a = lane < pp->n_present ? pp->present[lane] : 0;
+    sp_info[lane] = make_int4(a, pp->count[a], pp->seg_off[a], 0);
+  }
+  if (lane < 2 * kMaxGroups) grp_tab[lane] = lane < kMaxGroups ? m.grp_rec_off[lane] : m.grp_rec_cnt[lane - kMaxGroups];
+  for (int i = lane; i < c.S * c.S; i += 64) {
+    const int o = i / c.S, nw = i - o * c.S;
+    double ds = 0.0;
+    for (int d = 0; d < c.B; ++d) {
+      const double dp = m.basis[d * c.S + nw] - m.basis[d * c.S + o];
+      dphi_tab[i * c.B + d] = dp;
+      ds += pp->j1[d] * dp;
+    }
+    dsingle_tab[i] = ds;
+  }
   __syncthreads();
+
+  // records of a single-group model stay in VGPRs for the whole launch
+  const int rcnt0 = grp_tab[kMaxGroups];
+  const bool reg_recs = !multi_group && rcnt0 <= 64 * kRegRecs;
+  SubRec rr[kRegRecs];
+#pragma unroll
+  for (int k = 0; k < kRegRecs; ++k) {
+    const int it = lane + 64 * k;
+    rr[k] = (reg_recs && it < rcnt0) ? recs[it] : SubRec{0, 0};
+  }
 
   double energy = states[chain].energy;
   const uint64_t t0 = states[chain].moves_done;
@@ -128,91 +227,115 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
   uint64_t n_acc = 0;
   double sum_e = 0.0, sum_e2 = 0.0;
   const double kT = pp->kT;
+  const double inv_kT = 1.0 / kT;
   const uint64_t seed = pp->seed, uid = pp->uid;
-  const int S = m.n_species;
-  const int B = m.n_bfs;
-  double j1[kMaxBasis];
-#pragma unroll
-  for (int d = 0; d < kMaxBasis; ++d) j1[d] = d < B ? pp->j1[d] : 0.0;
+  const int np = pp->n_present;
 
-  uint32_t* log = accept_log ? accept_log + int64_t(chain) * log_words : nullptr;
+  uint32_t* acc_log = accept_log ? accept_log + int64_t(chain) * log_words : nullptr;
   uint32_t log_word = 0;
   Philox4 blk{0, 0, 0, 0};
+  double log_u = 0.0;
   for (int64_t t = 0; t < nmov; ++t) {
     const int l = int(t & 63);
-    // lane l holds the Philox block of move (t - l) + l, computed once per 64 moves
-    if (l == 0) blk = philox4x64_10(t0 + uint64_t(t) + uint64_t(lane), 0, 0, 0, seed, uid);
+    // lane l holds the Philox block (and log u) of move t - (t mod 64) + l
+    if (l == 0) {
+      blk = philox4x64_10(t0 + uint64_t(t) + uint64_t(lane), 0, 0, 0, seed, uid);
+      log_u = log(unit_double(blk.w3));
+    }
     const uint64_t w0 = readlane_u64(blk.w0, l);
     const uint64_t w1 = readlane_u64(blk.w1, l);
-    const uint64_t w2 = readlane_u64(blk.w2, l);
     const uint64_t w3 = readlane_u64(blk.w3, l);
-    const double u = unit_double(w3);
+    const double lu = readlane_f64(log_u, l);
 
-    int si, sj = -1, old_i, new_i, sa = 0, sb = 0, a = 0, b = 0;
+    int si, sj = -1, old_i, new_i, sa = 0, sb = 0, off_a = 0, off_b = 0;
     if (canonical) {
-      const int np = pp->n_present;
+      const uint64_t w2 = readlane_u64(blk.w2, l);
       const uint32_t ia = bounded32(uint32_t(w0 >> 32), uint32_t(np));
       const uint32_t ib = (ia + 1u + bounded32(uint32_t(w0), uint32_t(np - 1))) % uint32_t(np);
-      a = pp->present[ia];
-      b = pp->present[ib];
-      sa = int(bounded64(w1, uint64_t(pp->count[a])));
-      sb = int(bounded64(w2, uint64_t(pp->count[b])));
-      si = trk[pp->seg_off[a] + sa];
-      sj = trk[pp->seg_off[b] + sb];
-      old_i = a;
-      new_i = b;
+      const int4 A = sp_info[ia];
+      const int4 Bq = sp_info[ib];
+      sa = int(bounded64(w1, uint64_t(A.y)));
+      sb = int(bounded64(w2, uint64_t(Bq.y)));
+      off_a = A.z;
+      off_b = Bq.z;
+      si = trk[off_a + sa];
+      sj = trk[off_b + sb];
+      old_i = A.x;
+      new_i = Bq.x;
     } else {
-      const uint64_t pick = bounded64(w1, uint64_t(m.has_background ? m.n_movable : N));
-      si = m.has_background ? m.movable[pick] : int(pick);
+      const uint64_t pick = bounded64(w1, uint64_t(n_pick));
+      si = has_bkg ? m.movable[pick] : int(pick);
       old_i = occ[si];
-      new_i = int((uint32_t(old_i) + 1u + bounded32(uint32_t(w0), uint32_t(S - 1))) % uint32_t(S));
+      new_i = int((uint32_t(old_i) + 1u + bounded32(uint32_t(w0), uint32_t(c.S - 1))) % uint32_t(c.S));
     }
-
+    const int pair_i = old_i * c.S + new_i;
     double dphi[kMaxBasis];
-    double de_single = 0.0;
 #pragma unroll
-    for (int d = 0; d < kMaxBasis; ++d) {
-      dphi[d] = d < B ? m.basis[d * S + new_i] - m.basis[d * S + old_i] : 0.0;
-      de_single += j1[d] * dphi[d];
+    for (int d = 0; d < kMaxBasis; ++d) dphi[d] = d < c.B ? dphi_tab[pair_i * c.B + d] : 0.0;
+    double de_single = dsingle_tab[pair_i];
+
+    const Ref ri = ref_of<kPeriodic>(c, si);
+    double part = 0.0;
+    if (reg_recs) {
+#pragma unroll
+      for (int k = 0; k < kRegRecs; ++k)
+        if (lane + 64 * k < rcnt0) part += rec_de<kPeriodic>(c, rr[k], ri, occ, cells, gtab, dphi, -1, 0);
+    } else {
+      const int gi = multi_group ? int(m.site_group[si]) : 0;
+      const int off = grp_tab[gi], cnt = grp_tab[kMaxGroups + gi];
+      for (int it = lane; it < cnt; it += 64) part += rec_de<kPeriodic>(c, recs[off + it], ri, occ, cells, gtab, dphi, -1, 0);
     }
-    const RefSite ri = make_ref(m, si);
-    const int gi = site_group(m, si);
-    double part = flip_part<kPeriodic>(m, ri, recs, m.grp_rec_off[gi], m.grp_rec_cnt[gi], lane, occ, cells,
-                                       gtab, dphi, -1, 0);
     if (canonical) {
+      // second flip of the swap: j goes b -> a and sees site i already at b
       double dphj[kMaxBasis];
 #pragma unroll
-      for (int d = 0; d < kMaxBasis; ++d) {
-        dphj[d] = -dphi[d];
-        de_single += j1[d] * dphj[d];
+      for (int d = 0; d < kMaxBasis; ++d) dphj[d] = -dphi[d];
+      de_single += dsingle_tab[new_i * c.S + old_i];
+      const Ref rj = ref_of<kPeriodic>(c, sj);
+      if (reg_recs) {
+#pragma unroll
+        for (int k = 0; k < kRegRecs; ++k)
+          if (lane + 64 * k < rcnt0) part += rec_de<kPeriodic>(c, rr[k], rj, occ, cells, gtab, dphj, si, new_i);
+      } else {
+        const int gj = multi_group ? int(m.site_group[sj]) : 0;
+        const int off = grp_tab[gj], cnt = grp_tab[kMaxGroups + gj];
+        for (int it = lane; it < cnt; it += 64)
+          part += rec_de<kPeriodic>(c, recs[off + it], rj, occ, cells, gtab, dphj, si, new_i);
       }
-      const RefSite rj = make_ref(m, sj);
-      const int gj = site_group(m, sj);
-      part += flip_part<kPeriodic>(m, rj, recs, m.grp_rec_off[gj], m.grp_rec_cnt[gj], lane, occ, cells, gtab,
-                                   dphj, si, new_i);
     }
     const double dE = wave_sum_f64(part) + de_single;
+    // Metropolis test of the reference: accept if E_new < E_cur, else if
+    // u <= exp(-(E_new - E_cur) / kT) (cemc/mcmc/montecarlo.py:950-956).  The
+    // comparison is first made against log(u) with a margin far above the
+    // rounding of log/exp; only moves inside the margin evaluate exp.
     bool accept;
     if (dE < 0.0) {
       accept = true;
     } else {
-      accept = u <= exp(-dE / kT);
+      const double x = -dE * inv_kT;
+      const double margin = 1e-12 * (1.0 + fabs(x));
+      if (lu < x - margin)
+        accept = true;
+      else if (lu > x + margin)
+        accept = false;
+      else
+        accept = unit_double(w3) <= exp(-dE / kT);
     }
     if (accept) {
       if (lane == 0) {
         occ[si] = int8_t(new_i);
         if (canonical) {
           occ[sj] = int8_t(old_i);
-          trk[pp->seg_off[a] + sa] = uint16_t(sj);
-          trk[pp->seg_off[b] + sb] = uint16_t(si);
+          trk[off_a + sa] = uint16_t(sj);
+          trk[off_b + sb] = uint16_t(si);
         }
       }
       energy += dE;
       ++n_acc;
       log_word |= 1u << (t & 31);
     }
-    if (log != nullptr && ((t & 31) == 31 || t + 1 == nmov)) {
-      if (lane == 0) log[t >> 5] = log_word;
+    if (acc_log != nullptr && ((t & 31) == 31 || t + 1 == nmov)) {
+      if (lane == 0) acc_log[t >> 5] = log_word;
       log_word = 0;
     }
     sum_e += energy;
