@@ -33,6 +33,20 @@ __host__ __device__ inline int rec_size(SubRec r) { return int(r.w0 >> 30) + 2; 
 __host__ __device__ inline uint32_t rec_gbase(SubRec r) { return r.w1 & 0xFFFFFFu; }
 __host__ __device__ inline int rec_refpos(SubRec r) { return int((r.w1 >> 24) & 3u); }
 
+// Sub-cluster record of the sweep kernel (16 bytes, held in VGPRs or LDS):
+//   m0..m2  members in canonical order with the reference site removed:
+//           closed-form periodic lattices: packed cell offset x | y<<8 | z<<16;
+//           otherwise: translation-table column.  Unused members are
+//           kUnusedMember: they address the reference site and their species
+//           is masked to zero.
+//   meta    energy-table base (24 bits) | number of members << 24 | valid << 31
+// Energy table of a cluster entry: G[r][code][d], index base_r + code*B + d,
+// with base_r stored in meta for the record's reference position r.
+struct FatRec {
+  uint32_t m0, m1, m2, meta;
+};
+constexpr uint32_t kUnusedMember = 0x80000000u;  // member slot of a smaller cluster (offset 0, flag bit 31)
+
 // Cluster entry = one (translation group, orbit) pair with its record range.
 struct ClusterEntry {
   int32_t rec_begin;   // into the model's record array
@@ -56,6 +70,7 @@ struct ModelDev {
   int32_t grp_count[kMaxGroups];    // T_g
   double basis[kMaxBasis * kMaxSpecies];  // phi[d*n_species + s]
   const SubRec* recs;
+  const FatRec* frecs;        // same order as recs
   const double* gtab;
   const int32_t* trans;       // n_sites * K
   const uint32_t* col_cells;  // K packed cell offsets (x | y<<8 | z<<16)
@@ -83,6 +98,13 @@ struct ChainParams {
   int32_t present[kMaxSpecies];
   int32_t count[kMaxSpecies];
   int32_t seg_off[kMaxSpecies];
+  // the same species tables packed for register use: present ids (4 bits
+  // each), counts and tracker segment offsets (16 bits each, entries 0-3 in
+  // word 0, 4-7 in word 1)
+  uint32_t present_pk;
+  uint32_t pad0;
+  uint64_t count_pk[2];
+  uint64_t segoff_pk[2];
   double kT;
   double j1[kMaxBasis];  // singlet ECIs (chemical potential folded in)
   uint64_t seed;

@@ -98,11 +98,13 @@ struct cemc_model {
   std::vector<int32_t> entry_group, entry_size, entry_M, entry_norm, entry_goff;
   std::vector<int32_t> trans;
   std::vector<SubRec> recs;
+  std::vector<FatRec> rec_slots, frecs;
   std::vector<ClusterEntry> entries;
   std::vector<int32_t> grp_count;
   std::vector<int32_t> grp_rec_off, grp_rec_cnt;
   // device
   SubRec* d_recs = nullptr;
+  FatRec* d_frecs = nullptr;
   double* d_gtab = nullptr;
   int32_t* d_trans = nullptr;
   uint32_t* d_cells = nullptr;
@@ -129,6 +131,7 @@ struct cemc_model {
 
 cemc_model::~cemc_model() {
   dev_free(d_recs);
+  dev_free(d_frecs);
   dev_free(d_gtab);
   dev_free(d_trans);
   dev_free(d_cells);
@@ -152,7 +155,7 @@ cemc_model::~cemc_model() {
 // the energy form of the per-CF increment (ce_updater.cpp:393-404) times
 // N * J_k (ce_updater.cpp:236-242).  With the reference site at canonical
 // position r and decoration digit d there, the table stores
-//   G_c[r][d][code] = sum_{e: e_r = d} W_c[e] * prod_{q != r} phi[e_q](s_q)
+//   G_c[r][code][d] = sum_{e: e_r = d} W_c[e] * prod_{q != r} phi[e_q](s_q)
 // with code = sum_q s_q * S^q over the other members in canonical order, so
 // a flip a->b of the reference site changes the energy by
 //   sum_d (phi_d(b) - phi_d(a)) * G_c[r][d][code].
@@ -187,7 +190,7 @@ void cemc_model::build_gtab(const std::vector<double>& J) {
           t /= B;
         }
         const int d = deco[r];
-        double* G = gtab.data() + entry_goff[c] + (r * B + d) * P;
+        double* G = gtab.data() + entry_goff[c] + r * P * B + d;
         for (int code = 0; code < P; ++code) {
           double prod = W[idx];
           int cc = code;
@@ -197,7 +200,7 @@ void cemc_model::build_gtab(const std::vector<double>& J) {
             cc /= S;
             prod *= basis[deco[pos] * S + s];
           }
-          G[code] += prod;
+          G[code * B] += prod;
         }
       }
     }
@@ -228,6 +231,7 @@ void cemc_model::refresh_view() {
   }
   for (size_t i = 0; i < basis.size(); ++i) v.basis[i] = basis[i];
   v.recs = d_recs;
+  v.frecs = d_frecs;
   v.gtab = d_gtab;
   v.trans = d_trans;
   v.col_cells = d_cells;
@@ -329,8 +333,9 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
     m->entry_goff[c] = goff;
     goff += n * B * m->pow_s(n - 1);
   }
-  require(goff < (1 << 24), "energy tables too large");
-  m->gtab.assign(goff, 0.0);
+  require(goff < (1 << 24) - 8, "energy tables too large");
+  // padded so fixed-width reads of decorations past the last one stay inside
+  m->gtab.assign(goff + 8, 0.0);
   m->grp_rec_off.assign(G, 0);
   m->grp_rec_cnt.assign(G, 0);
   for (int g = 0; g < G; ++g) {
@@ -365,6 +370,8 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
         const uint32_t gbase = uint32_t(m->entry_goff[c] + r * B * m->pow_s(n - 1));
         const uint32_t w1 = gbase | (uint32_t(r) << 24) | (uint32_t(c) << 26);
         m->recs.push_back(SubRec{w0, w1});
+        m->rec_slots.push_back({rec_slot(SubRec{w0, w1}, 0), rec_slot(SubRec{w0, w1}, 1),
+                                rec_slot(SubRec{w0, w1}, 2), gbase | (uint32_t(n - 1) << 24) | (1u << 31)});
       }
     }
     m->grp_rec_cnt[g] = int(m->recs.size()) - m->grp_rec_off[g];
@@ -395,10 +402,25 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
     }
   }
   m->build_gtab(m->eci);
+  // sweep records: member cell offsets on closed-form lattices, else columns
+  m->frecs.clear();
+  for (const auto& fr : m->rec_slots) {
+    FatRec f = fr;
+    const int nm = int((fr.meta >> 24) & 3u);
+    if (m->periodic) {
+      f.m0 = m->col_cells[fr.m0];
+      f.m1 = m->col_cells[fr.m1];
+      f.m2 = m->col_cells[fr.m2];
+    }
+    if (nm < 2) f.m1 = kUnusedMember;
+    if (nm < 3) f.m2 = kUnusedMember;
+    m->frecs.push_back(f);
+  }
   // upload
   std::vector<int8_t> sg(N);
   for (int i = 0; i < N; ++i) sg[i] = int8_t(m->site_group[i]);
   m->d_recs = upload(m->recs);
+  m->d_frecs = upload(m->frecs);
   m->d_gtab = upload(m->gtab);
   m->d_trans = upload(m->trans);
   m->d_cells = upload(m->col_cells);
@@ -567,7 +589,10 @@ struct cemc_batch {
   int64_t* d_nmoves = nullptr;
   int32_t* d_order = nullptr;
   struct Class {
-    bool periodic, gtab_lds;
+    bool periodic, canonical;
+    int kr;
+    bool gtab_lds;
+    int kbm;
     int begin, count;
     uint32_t lds;
   };
@@ -616,7 +641,10 @@ void batch_launch(cemc_batch* b, const int64_t* n_moves, int64_t moves, bool tim
     a.n_chains = cl.count;
     a.lds_bytes = cl.lds;
     a.periodic = cl.periodic;
+    a.canonical = cl.canonical;
+    a.kr = cl.kr;
     a.gtab_lds = cl.gtab_lds;
+    a.kbm = cl.kbm;
     a.accept_log = d_log;
     a.log_words = log_words;
     HIP_CHECK(launch_sweep(a, b->stream));
@@ -936,6 +964,12 @@ int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const c
             if (cnt > 0) p.present[p.n_present++] = s;
           }
           require(p.n_present >= 2, "canonical moves need at least two species present");
+          for (int k = 0; k < p.n_present; ++k) {
+            const int a = p.present[k];
+            p.present_pk |= uint32_t(a) << (4 * k);
+            p.count_pk[k >> 2] |= uint64_t(uint32_t(p.count[a]) & 0xFFFFu) << (16 * (k & 3));
+            p.segoff_pk[k >> 2] |= uint64_t(uint32_t(p.seg_off[a]) & 0xFFFFu) << (16 * (k & 3));
+          }
           trk_pool.resize(p.trk_off + N, 0);
         }
       }
@@ -947,36 +981,68 @@ int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const c
       b->d_trk = upload(trk_pool);
       HIP_CHECK(hipMalloc(&b->d_states, sizeof(ChainState) * n_chains));
       HIP_CHECK(hipMalloc(&b->d_nmoves, sizeof(int64_t) * n_chains));
-      // launch classes and longest-first order inside each class
+      // launch classes: chains sharing a kernel variant, longest first
       std::vector<int32_t> order;
       const uint32_t limit = lds_limit();
       const uint32_t gtab_budget = std::min(kGtabLdsBudget, limit);
-      for (int periodic = 1; periodic >= 0; --periodic) {
-        for (int glds = 1; glds >= 0; --glds) {
-          std::vector<int32_t> ids;
-          uint32_t lds = 0;
-          for (int c = 0; c < n_chains; ++c) {
-            const cemc_model* m = b->models[b->params[c].model];
-            if (int(m->periodic) != periodic) continue;
-            const int canon = b->params[c].kind == CEMC_MOVE_CANONICAL;
-            const uint32_t with_g = sweep_lds_bytes(m->n_sites, canon, int(m->recs.size()), m->K, m->periodic,
-                                                    int(m->gtab.size()));
-            const uint32_t no_g = sweep_lds_bytes(m->n_sites, canon, int(m->recs.size()), m->K, m->periodic, 0);
-            const bool fits = with_g <= gtab_budget;
-            if (int(fits) != glds) continue;
-            require(no_g <= limit, "chain state does not fit in the workgroup's LDS");
-            ids.push_back(c);
-            lds = std::max(lds, fits ? with_g : no_g);
-          }
-          if (ids.empty()) continue;
-          std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) {
-            const auto* mx = b->models[b->params[x].model];
-            const auto* my = b->models[b->params[y].model];
-            return int64_t(mx->n_sites) * mx->recs.size() > int64_t(my->n_sites) * my->recs.size();
-          });
-          b->classes.push_back(cemc_batch::Class{periodic != 0, glds != 0, int(order.size()), int(ids.size()), lds});
-          order.insert(order.end(), ids.begin(), ids.end());
+      struct Key {
+        bool periodic, canonical;
+        int kr;
+        bool glds;
+        int kbm;
+        uint32_t lds;
+      };
+      std::vector<Key> keys(n_chains);
+      for (int c = 0; c < n_chains; ++c) {
+        const cemc_model* m = b->models[b->params[c].model];
+        const bool canon = b->params[c].kind == CEMC_MOVE_CANONICAL;
+        Key k;
+        k.periodic = m->periodic;
+        k.canonical = canon;
+        k.kbm = m->n_species <= 4 ? 3 : 7;
+        const bool single = m->n_groups == 1 && int(m->movable.size()) == m->n_sites;
+        const int cnt = m->grp_rec_cnt[0];
+        const uint32_t with_g = sweep_lds_bytes(m->n_sites, canon, 0, int(m->gtab.size()));
+        k.kr = 0;
+        if (single && k.kbm == 3 && with_g <= gtab_budget && cnt <= 512) {
+          k.kr = cnt <= 128 ? 2 : (cnt <= 256 ? 4 : 8);
+          if (m->n_species == 2) k.kbm = 1;
         }
+        if (k.kr > 0) {
+          k.glds = true;
+          k.lds = with_g;
+        } else {
+          const uint32_t rec_g = sweep_lds_bytes(m->n_sites, canon, int(m->recs.size()), int(m->gtab.size()));
+          const uint32_t rec_only = sweep_lds_bytes(m->n_sites, canon, int(m->recs.size()), 0);
+          k.glds = rec_g <= gtab_budget;
+          k.lds = k.glds ? rec_g : rec_only;
+          require(k.lds <= limit, "chain state does not fit in the workgroup's LDS");
+        }
+        keys[c] = k;
+      }
+      std::vector<bool> done(n_chains, false);
+      for (int c0 = 0; c0 < n_chains; ++c0) {
+        if (done[c0]) continue;
+        const Key& k0 = keys[c0];
+        std::vector<int32_t> ids;
+        uint32_t lds = 0;
+        for (int c = c0; c < n_chains; ++c) {
+          const Key& k = keys[c];
+          if (done[c] || k.periodic != k0.periodic || k.canonical != k0.canonical || k.kr != k0.kr ||
+              k.glds != k0.glds || k.kbm != k0.kbm)
+            continue;
+          done[c] = true;
+          ids.push_back(c);
+          lds = std::max(lds, k.lds);
+        }
+        std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) {
+          const auto* mx = b->models[b->params[x].model];
+          const auto* my = b->models[b->params[y].model];
+          return int64_t(mx->n_sites) * mx->recs.size() > int64_t(my->n_sites) * my->recs.size();
+        });
+        b->classes.push_back(cemc_batch::Class{k0.periodic, k0.canonical, k0.kr, k0.glds, k0.kbm,
+                                               int(order.size()), int(ids.size()), lds});
+        order.insert(order.end(), ids.begin(), ids.end());
       }
       b->d_order = upload(order);
       uint32_t need = 0;

@@ -17,12 +17,16 @@ struct LaunchSweep {
   int n_chains;                // blocks in this launch
   uint32_t lds_bytes;
   bool periodic;
+  bool canonical;
+  int kr;          // records per lane held in VGPRs (2, 4, 8) or 0 = records in LDS
   bool gtab_lds;
+  int kbm;         // decorations per table read: 3 (up to 4 species) or 7
   uint32_t* accept_log;  // optional: bit t of chain c at accept_log[c * log_words + t / 32]
   int64_t log_words;
 };
 
-uint32_t sweep_lds_bytes(int n_sites, int canonical, int n_rec, int K, int periodic, int gtab_doubles);
+uint32_t sweep_lds_bytes(int n_sites, int canonical, int n_rec_lds, int gtab_doubles);
+bool sweep_variant_ok(int kr, bool gtab_lds, int kbm);
 // Raises the dynamic-LDS limit of every sweep kernel instantiation.
 hipError_t sweep_set_max_lds(uint32_t bytes);
 

@@ -27,50 +27,50 @@ __device__ inline int site_group(const ModelDev& m, int32_t site) {
 }
 
 struct LdsLayout {
-  uint32_t occ, trk, rec, cells, gtab, total;
+  uint32_t occ, trk, rec, gtab, total;
 };
 
 __host__ __device__ inline uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
 
-__host__ __device__ inline LdsLayout lds_layout(int n_sites, int canonical, int n_rec, int K, int periodic,
-                                               int gtab_doubles) {
+// occupation | species tracker (canonical) | records (large models) | energy tables
+__host__ __device__ inline LdsLayout lds_layout(int n_sites, int canonical, int n_rec_lds, int gtab_doubles) {
   LdsLayout L;
   L.occ = 0;
-  L.trk = align16(uint32_t(n_sites));
+  L.trk = align16(uint32_t(n_sites) + 1u);  // occ[n_sites] is a zero byte read by unused members
   L.rec = L.trk + (canonical ? align16(uint32_t(n_sites) * 2u) : 0u);
-  L.cells = L.rec + align16(uint32_t(n_rec) * 8u);
-  L.gtab = L.cells + (periodic ? align16(uint32_t(K) * 4u) : 0u);
+  L.gtab = L.rec + uint32_t(n_rec_lds) * 16u;
   L.total = L.gtab + uint32_t(gtab_doubles) * 8u;
   return L;
 }
 
 }  // namespace
 
-uint32_t sweep_lds_bytes(int n_sites, int canonical, int n_rec, int K, int periodic, int gtab_doubles) {
-  return lds_layout(n_sites, canonical, n_rec, K, periodic, gtab_doubles).total;
+uint32_t sweep_lds_bytes(int n_sites, int canonical, int n_rec_lds, int gtab_doubles) {
+  return lds_layout(n_sites, canonical, n_rec_lds, gtab_doubles).total;
 }
 
 // ---------------------------------------------------------------------------
 // Sweep kernel.
 //
-// Everything the move loop reads is staged before the loop: per-model scalars
-// in SGPRs (SweepCtx), small per-chain tables (species lists, basis
-// differences, singlet energy changes) and the model tables in LDS, and for
-// single-group models each lane's sub-cluster records in VGPRs.  One trial
-// move then costs: uniform decode of a pre-generated Philox block, one round
-// of LDS gathers of member species, one LDS read of the energy table per
-// sub-cluster, a DPP reduction and the Metropolis test.
-
-constexpr int kRegRecs = 4;  // sub-cluster records held in VGPRs per lane (<= 256 per group)
+// One 64-lane wavefront owns one chain.  Everything the move loop reads is
+// staged before the loop: model scalars and packed species tables in SGPRs,
+// the occupation, species tracker and energy tables in LDS, and each lane's
+// sub-cluster records in VGPRs (kR per lane) or, for large models, in LDS
+// (read in batches of 4 per lane).  Per trial move each lane computes the
+// member sites of its records (closed form: cell offsets carried by the
+// record), issues all member-species gathers at once, forms the table codes,
+// issues all energy-table reads at once, and the partial energy changes are
+// reduced across the wave with DPP.
 
 struct SweepCtx {
-  int S, B, K;
+  int S, B, N;
   int L1, L2, L3, L23;
-  uint32_t mag23, mag3;    // floor((2^32-1)/d) for the divisors L2*L3 and L3
+  uint32_t mag23, mag3;  // floor((2^32-1)/d) for the divisors L2*L3 and L3
+  int K;
   const int32_t* trans;
 };
 
-__device__ inline uint32_t fast_div(uint32_t n, uint32_t d, uint32_t magic) {
+__device__ __attribute__((always_inline)) inline uint32_t fast_div(uint32_t n, uint32_t d, uint32_t magic) {
   // exact for n, d < 2^16: the estimate is the quotient or one less
   uint32_t q = __umulhi(n, magic);
   if (n - q * d >= d) ++q;
@@ -98,49 +98,117 @@ __device__ __attribute__((always_inline)) inline Ref ref_of(const SweepCtx& c, i
   return r;
 }
 
+// Site playing the role of member `m` (cell offset or table column) when
+// `r` is the reference site (the translation matrix, ce_updater.cpp:264).
+// Unused members (bit 31 set, offset bits zero) give a valid site whose
+// species the caller masks to zero.
 template <bool kPeriodic>
-__device__ __attribute__((always_inline)) inline int nbr(const SweepCtx& c, const Ref& r, uint32_t slot, const uint32_t* cells) {
+__device__ __attribute__((always_inline)) inline int member_site(const SweepCtx& c, const Ref& r, uint32_t m) {
   if constexpr (kPeriodic) {
-    const uint32_t cc = cells[slot];
-    int x = r.x + int(cc & 0xFFu);
-    int y = r.y + int((cc >> 8) & 0xFFu);
-    int z = r.z + int((cc >> 16) & 0xFFu);
+    int x = r.x + int(m & 0xFFu);
+    int y = r.y + int((m >> 8) & 0xFFu);
+    int z = r.z + int((m >> 16) & 0xFFu);
     x -= (x >= c.L1) ? c.L1 : 0;
     y -= (y >= c.L2) ? c.L2 : 0;
     z -= (z >= c.L3) ? c.L3 : 0;
     return (x * c.L2 + y) * c.L3 + z;
   } else {
-    return c.trans[int64_t(r.site) * c.K + slot];
+    return c.trans[int64_t(r.site) * c.K + (m & 0xFFFFu)];
   }
 }
 
-// Energy-change contribution of one sub-cluster record when the reference
-// site's basis values change by dphi (decoration-indexed).
-template <bool kPeriodic>
-__device__ __attribute__((always_inline)) inline double rec_de(const SweepCtx& c, SubRec r, const Ref& ref, const int8_t* occ,
-                                const uint32_t* cells, const double* gtab, const double* dphi, int patch_site,
-                                int patch_species) {
-  const int nm = rec_size(r) - 1;
-  int code = 0;
-  for (int q = nm - 1; q >= 0; --q) {
-    const int nb = nbr<kPeriodic>(c, ref, rec_slot(r, q), cells);
-    int sp = occ[nb];
-    sp = (nb == patch_site) ? patch_species : sp;
-    code = code * c.S + sp;
+// Phase 1 of a flip: member sites of kR records (pure ALU, no branches),
+// issue of all member-species gathers, energy-table indices.  A member equal
+// to `patch_site` reads `patch_species`.
+template <bool kPeriodic, int kR>
+__device__ __attribute__((always_inline)) inline void table_index(const SweepCtx& c, const FatRec (&rec)[kR],
+                                                                  const Ref& ref, const int8_t* occ,
+                                                                  int patch_site, int patch_species,
+                                                                  int (&gi)[kR]) {
+  int nb[kR][3];
+#pragma unroll
+  for (int k = 0; k < kR; ++k) {
+    nb[k][0] = member_site<kPeriodic>(c, ref, rec[k].m0);
+    nb[k][1] = member_site<kPeriodic>(c, ref, rec[k].m1);
+    nb[k][2] = member_site<kPeriodic>(c, ref, rec[k].m2);
   }
-  // S^nm without indexing (a select over struct fields can end up in scratch)
-  int p = c.S;
-  p *= (nm >= 2) ? c.S : 1;
-  p *= (nm >= 3) ? c.S : 1;
-  const double* g = gtab + rec_gbase(r) + code;
+  int sp[kR][3];
+#pragma unroll
+  for (int k = 0; k < kR; ++k)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) sp[k][q] = occ[nb[k][q]];
+#pragma unroll
+  for (int k = 0; k < kR; ++k) {
+    int s0 = sp[k][0], s1 = sp[k][1], s2 = sp[k][2];
+    s0 = nb[k][0] == patch_site ? patch_species : s0;
+    s1 = nb[k][1] == patch_site ? patch_species : s1;
+    s2 = nb[k][2] == patch_site ? patch_species : s2;
+    // species of unused members masked to zero: code = sum_q s_q S^q over the used ones
+    s1 &= int(rec[k].m1 >> 31) - 1;
+    s2 &= int(rec[k].m2 >> 31) - 1;
+    const int code = s0 + c.S * (s1 + c.S * s2);
+    gi[k] = int(rec[k].meta & 0xFFFFFFu) + code * c.B;
+  }
+}
+
+// Phase 2: energy-table reads and the change for basis differences dphi;
+// records flagged invalid contribute zero.
+template <int kR, int kBM>
+__device__ __attribute__((always_inline)) inline double table_sum(const FatRec (&rec)[kR], const int (&gi)[kR],
+                                                                  const double* gtab, const double (&dphi)[kBM]) {
+  double g[kR][kBM];
+#pragma unroll
+  for (int k = 0; k < kR; ++k)
+#pragma unroll
+    for (int d = 0; d < kBM; ++d) g[k][d] = gtab[gi[k] + d];
   double v = 0.0;
 #pragma unroll
-  for (int d = 0; d < kMaxBasis; ++d)
-    if (d < c.B) v += dphi[d] * g[d * p];
+  for (int k = 0; k < kR; ++k) {
+    double e = 0.0;
+#pragma unroll
+    for (int d = 0; d < kBM; ++d) e += dphi[d] * g[k][d];
+    v += (rec[k].meta >> 31) ? e : 0.0;
+  }
   return v;
 }
 
-template <bool kPeriodic, bool kGtabLds>
+template <bool kPeriodic, int kR, int kBM>
+__device__ __attribute__((always_inline)) inline double batch_de(const SweepCtx& c, const FatRec (&rec)[kR],
+                                                                 const Ref& ref, const int8_t* occ,
+                                                                 const double* gtab, const double (&dphi)[kBM],
+                                                                 int patch_site, int patch_species) {
+  int gi[kR];
+  table_index<kPeriodic, kR>(c, rec, ref, occ, patch_site, patch_species, gi);
+  return table_sum<kR, kBM>(rec, gi, gtab, dphi);
+}
+
+// Records of one translation group read from LDS in batches of 4 per lane.
+template <bool kPeriodic, int kBM>
+__device__ __attribute__((always_inline)) inline double lds_flip_de(const SweepCtx& c, const FatRec* recs, int off,
+                                                                    int cnt, int lane, const Ref& ref,
+                                                                    const int8_t* occ, const double* gtab,
+                                                                    const double (&dphi)[kBM], int patch_site,
+                                                                    int patch_species) {
+  double v = 0.0;
+  for (int base = 0; base < cnt; base += 256) {
+    FatRec rb[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int it = base + lane + 64 * k;
+      rb[k] = it < cnt ? recs[off + it] : FatRec{kUnusedMember, kUnusedMember, kUnusedMember, 0u};
+    }
+    v += batch_de<kPeriodic, 4, kBM>(c, rb, ref, occ, gtab, dphi, patch_site, patch_species);
+  }
+  return v;
+}
+
+__device__ __attribute__((always_inline)) inline int unpack16(uint64_t lo, uint64_t hi, uint32_t i) {
+  return int(((i < 4u ? lo : hi) >> (16u * (i & 3u))) & 0xFFFFu);
+}
+
+// kR > 0: single-group model, records in VGPRs (kR per lane); kR == 0:
+// records in LDS.  kBM: decorations read per table lookup (>= n_species - 1).
+template <bool kPeriodic, bool kCanonical, int kR, bool kGtabLds, int kBM>
 __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict__ params,
                                                    ChainState* __restrict__ states,
                                                    const ModelDev* __restrict__ models,
@@ -150,33 +218,32 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
                                                    const int64_t* __restrict__ n_moves,
                                                    uint32_t* __restrict__ accept_log, int64_t log_words) {
   extern __shared__ __align__(16) unsigned char lds[];
-  __shared__ int4 sp_info[kMaxSpecies];                          // (species, count, seg_off, -)
   __shared__ double dphi_tab[kMaxSpecies * kMaxSpecies * kMaxBasis];  // [(old*S+new)*B + d]
-  __shared__ double dsingle_tab[kMaxSpecies * kMaxSpecies];      // singlet energy change per (old,new)
+  __shared__ double dsingle_tab[kMaxSpecies * kMaxSpecies];          // singlet energy change per (old,new)
   __shared__ int grp_tab[2 * kMaxGroups];
+  constexpr int kRecLds = kR == 0;
   const int chain = chain_order[blockIdx.x];
   const int lane = int(threadIdx.x);
   const ChainParams* __restrict__ pp = params + chain;
   const ModelDev& m = models[pp->model];
   const int N = pp->n_sites;
-  const bool canonical = pp->kind == 0;
-  const LdsLayout L = lds_layout(N, canonical, m.n_rec, m.K, m.periodic, kGtabLds ? m.gtab_size : 0);
+  const LdsLayout L = lds_layout(N, kCanonical, kRecLds ? m.n_rec : 0, kGtabLds ? m.gtab_size : 0);
   int8_t* occ = reinterpret_cast<int8_t*>(lds + L.occ);
   uint16_t* trk = reinterpret_cast<uint16_t*>(lds + L.trk);
-  SubRec* recs = reinterpret_cast<SubRec*>(lds + L.rec);
-  uint32_t* cells = reinterpret_cast<uint32_t*>(lds + L.cells);
+  FatRec* recs_l = reinterpret_cast<FatRec*>(lds + L.rec);
   double* gtab_l = reinterpret_cast<double*>(lds + L.gtab);
 
   SweepCtx c;
   c.S = m.n_species;
   c.B = m.n_bfs;
-  c.K = m.K;
+  c.N = N;
   c.L1 = m.dims[0];
   c.L2 = m.dims[1];
   c.L3 = m.dims[2];
   c.L23 = c.L2 * c.L3;
   c.mag23 = kPeriodic ? 0xFFFFFFFFu / uint32_t(c.L23) : 0u;
   c.mag3 = kPeriodic ? 0xFFFFFFFFu / uint32_t(c.L3) : 0u;
+  c.K = m.K;
   c.trans = m.trans;
   const bool multi_group = m.n_groups > 1 || m.has_background;
   const bool has_bkg = m.has_background != 0;
@@ -185,19 +252,15 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
   // ---- stage chain state, per-chain tables and model tables into LDS
   const int8_t* gocc = occ_pool + pp->occ_off;
   for (int i = lane; i < N; i += 64) occ[i] = gocc[i];
+  if (lane == 0) occ[N] = 0;
   uint16_t* gtrk = trk_pool + pp->trk_off;
-  if (canonical)
+  if constexpr (kCanonical)
     for (int i = lane; i < N; i += 64) trk[i] = gtrk[i];
-  for (int i = lane; i < m.n_rec; i += 64) recs[i] = m.recs[i];
-  if (kPeriodic)
-    for (int i = lane; i < m.K; i += 64) cells[i] = m.col_cells[i];
+  if constexpr (kRecLds)
+    for (int i = lane; i < m.n_rec; i += 64) recs_l[i] = m.frecs[i];
   if constexpr (kGtabLds)
     for (int i = lane; i < m.gtab_size; i += 64) gtab_l[i] = m.gtab[i];
   const double* gtab = kGtabLds ? gtab_l : m.gtab;
-  if (lane < kMaxSpecies) {
-    const int a = lane < pp->n_present ? pp->present[lane] : 0;
-    sp_info[lane] = make_int4(a, pp->count[a], pp->seg_off[a], 0);
-  }
   if (lane < 2 * kMaxGroups) grp_tab[lane] = lane < kMaxGroups ? m.grp_rec_off[lane] : m.grp_rec_cnt[lane - kMaxGroups];
   for (int i = lane; i < c.S * c.S; i += 64) {
     const int o = i / c.S, nw = i - o * c.S;
@@ -212,13 +275,14 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
   __syncthreads();
 
   // records of a single-group model stay in VGPRs for the whole launch
-  const int rcnt0 = grp_tab[kMaxGroups];
-  const bool reg_recs = !multi_group && rcnt0 <= 64 * kRegRecs;
-  SubRec rr[kRegRecs];
+  FatRec rr[kR > 0 ? kR : 1];
+  if constexpr (kR > 0) {
+    const int cnt = m.grp_rec_cnt[0];
 #pragma unroll
-  for (int k = 0; k < kRegRecs; ++k) {
-    const int it = lane + 64 * k;
-    rr[k] = (reg_recs && it < rcnt0) ? recs[it] : SubRec{0, 0};
+    for (int k = 0; k < kR; ++k) {
+      const int it = lane + 64 * k;
+      rr[k] = it < cnt ? m.frecs[it] : FatRec{kUnusedMember, kUnusedMember, kUnusedMember, 0u};
+    }
   }
 
   double energy = states[chain].energy;
@@ -229,7 +293,10 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
   const double kT = pp->kT;
   const double inv_kT = 1.0 / kT;
   const uint64_t seed = pp->seed, uid = pp->uid;
-  const int np = pp->n_present;
+  const uint32_t np = uint32_t(pp->n_present);
+  const uint32_t present_pk = pp->present_pk;
+  const uint64_t cnt_lo = pp->count_pk[0], cnt_hi = pp->count_pk[1];
+  const uint64_t seg_lo = pp->segoff_pk[0], seg_hi = pp->segoff_pk[1];
 
   uint32_t* acc_log = accept_log ? accept_log + int64_t(chain) * log_words : nullptr;
   uint32_t log_word = 0;
@@ -247,60 +314,65 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
     const uint64_t w3 = readlane_u64(blk.w3, l);
     const double lu = readlane_f64(log_u, l);
 
-    int si, sj = -1, old_i, new_i, sa = 0, sb = 0, off_a = 0, off_b = 0;
-    if (canonical) {
+    int si, sj = 0, old_i = 0, new_i = 0, sa = 0, sb = 0, off_a = 0, off_b = 0;
+    if constexpr (kCanonical) {
       const uint64_t w2 = readlane_u64(blk.w2, l);
-      const uint32_t ia = bounded32(uint32_t(w0 >> 32), uint32_t(np));
-      const uint32_t ib = (ia + 1u + bounded32(uint32_t(w0), uint32_t(np - 1))) % uint32_t(np);
-      const int4 A = sp_info[ia];
-      const int4 Bq = sp_info[ib];
-      sa = int(bounded64(w1, uint64_t(A.y)));
-      sb = int(bounded64(w2, uint64_t(Bq.y)));
-      off_a = A.z;
-      off_b = Bq.z;
+      const uint32_t ia = bounded32(uint32_t(w0 >> 32), np);
+      const uint32_t ib = (ia + 1u + bounded32(uint32_t(w0), np - 1u)) % np;
+      old_i = int((present_pk >> (4u * ia)) & 15u);
+      new_i = int((present_pk >> (4u * ib)) & 15u);
+      sa = int(bounded64(w1, uint64_t(unpack16(cnt_lo, cnt_hi, ia))));
+      sb = int(bounded64(w2, uint64_t(unpack16(cnt_lo, cnt_hi, ib))));
+      off_a = unpack16(seg_lo, seg_hi, ia);
+      off_b = unpack16(seg_lo, seg_hi, ib);
       si = trk[off_a + sa];
       sj = trk[off_b + sb];
-      old_i = A.x;
-      new_i = Bq.x;
     } else {
       const uint64_t pick = bounded64(w1, uint64_t(n_pick));
       si = has_bkg ? m.movable[pick] : int(pick);
-      old_i = occ[si];
-      new_i = int((uint32_t(old_i) + 1u + bounded32(uint32_t(w0), uint32_t(c.S - 1))) % uint32_t(c.S));
     }
-    const int pair_i = old_i * c.S + new_i;
-    double dphi[kMaxBasis];
-#pragma unroll
-    for (int d = 0; d < kMaxBasis; ++d) dphi[d] = d < c.B ? dphi_tab[pair_i * c.B + d] : 0.0;
-    double de_single = dsingle_tab[pair_i];
-
     const Ref ri = ref_of<kPeriodic>(c, si);
-    double part = 0.0;
-    if (reg_recs) {
+    int gi_i[kR > 0 ? kR : 1], gi_j[kR > 0 ? kR : 1];
+    if constexpr (kR > 0) {
+      // all member gathers of the move are issued together with occ[si]
+      if constexpr (!kCanonical) old_i = occ[si];
+      table_index<kPeriodic, kR>(c, rr, ri, occ, -1, 0, gi_i);
+      if constexpr (kCanonical) {
+        const Ref rj = ref_of<kPeriodic>(c, sj);
+        table_index<kPeriodic, kR>(c, rr, rj, occ, si, new_i, gi_j);
+      }
+    } else {
+      if constexpr (!kCanonical) old_i = occ[si];
+    }
+    if constexpr (!kCanonical)
+      new_i = int((uint32_t(old_i) + 1u + bounded32(uint32_t(w0), uint32_t(c.S - 1))) % uint32_t(c.S));
+    const int pair_i = old_i * c.S + new_i;
+    double dphi[kBM];
 #pragma unroll
-      for (int k = 0; k < kRegRecs; ++k)
-        if (lane + 64 * k < rcnt0) part += rec_de<kPeriodic>(c, rr[k], ri, occ, cells, gtab, dphi, -1, 0);
+    for (int d = 0; d < kBM; ++d) dphi[d] = d < c.B ? dphi_tab[pair_i * c.B + d] : 0.0;
+    double de_single = dsingle_tab[pair_i];
+    double part;
+    if constexpr (kR > 0) {
+      part = table_sum<kR, kBM>(rr, gi_i, gtab, dphi);
     } else {
       const int gi = multi_group ? int(m.site_group[si]) : 0;
-      const int off = grp_tab[gi], cnt = grp_tab[kMaxGroups + gi];
-      for (int it = lane; it < cnt; it += 64) part += rec_de<kPeriodic>(c, recs[off + it], ri, occ, cells, gtab, dphi, -1, 0);
+      part = lds_flip_de<kPeriodic, kBM>(c, recs_l, grp_tab[gi], grp_tab[kMaxGroups + gi], lane, ri, occ, gtab,
+                                         dphi, -1, 0);
     }
-    if (canonical) {
+    if constexpr (kCanonical) {
       // second flip of the swap: j goes b -> a and sees site i already at b
-      double dphj[kMaxBasis];
+      // (ce_updater.cpp:845-852 runs update_cf twice in sequence)
+      double dphj[kBM];
 #pragma unroll
-      for (int d = 0; d < kMaxBasis; ++d) dphj[d] = -dphi[d];
+      for (int d = 0; d < kBM; ++d) dphj[d] = -dphi[d];
       de_single += dsingle_tab[new_i * c.S + old_i];
-      const Ref rj = ref_of<kPeriodic>(c, sj);
-      if (reg_recs) {
-#pragma unroll
-        for (int k = 0; k < kRegRecs; ++k)
-          if (lane + 64 * k < rcnt0) part += rec_de<kPeriodic>(c, rr[k], rj, occ, cells, gtab, dphj, si, new_i);
+      if constexpr (kR > 0) {
+        part += table_sum<kR, kBM>(rr, gi_j, gtab, dphj);
       } else {
+        const Ref rj = ref_of<kPeriodic>(c, sj);
         const int gj = multi_group ? int(m.site_group[sj]) : 0;
-        const int off = grp_tab[gj], cnt = grp_tab[kMaxGroups + gj];
-        for (int it = lane; it < cnt; it += 64)
-          part += rec_de<kPeriodic>(c, recs[off + it], rj, occ, cells, gtab, dphj, si, new_i);
+        part += lds_flip_de<kPeriodic, kBM>(c, recs_l, grp_tab[gj], grp_tab[kMaxGroups + gj], lane, rj, occ,
+                                            gtab, dphj, si, new_i);
       }
     }
     const double dE = wave_sum_f64(part) + de_single;
@@ -324,7 +396,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
     if (accept) {
       if (lane == 0) {
         occ[si] = int8_t(new_i);
-        if (canonical) {
+        if constexpr (kCanonical) {
           occ[sj] = int8_t(old_i);
           trk[off_a + sa] = uint16_t(sj);
           trk[off_b + sb] = uint16_t(si);
@@ -346,7 +418,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
   // ---- write back
   int8_t* wocc = occ_pool + pp->occ_off;
   for (int i = lane; i < N; i += 64) wocc[i] = occ[i];
-  if (canonical)
+  if constexpr (kCanonical)
     for (int i = lane; i < N; i += 64) gtrk[i] = trk[i];
   if (lane == 0) {
     ChainState* st = states + chain;
@@ -579,40 +651,68 @@ __global__ void slot_energy_kernel(const ModelDev* __restrict__ model, const dou
 
 // ---------------------------------------------------------------------------
 // Launch wrappers.
+// Instantiated variants: (periodic, canonical) x {records in VGPRs: kR = 2, 4,
+// 8 with tables in LDS and up to 4 species} and {records in LDS: tables in LDS
+// or HBM, up to 4 or 8 species}.
+template <bool P, bool C>
+struct SweepVariants {
+  static const void* fn(int kr, bool glds, int kbm) {
+    if (kbm == 1) {
+      if (kr == 2) return reinterpret_cast<const void*>(&sweep_kernel<P, C, 2, true, 1>);
+      if (kr == 4) return reinterpret_cast<const void*>(&sweep_kernel<P, C, 4, true, 1>);
+      return reinterpret_cast<const void*>(&sweep_kernel<P, C, 8, true, 1>);
+    }
+    if (kr == 2) return reinterpret_cast<const void*>(&sweep_kernel<P, C, 2, true, 3>);
+    if (kr == 4) return reinterpret_cast<const void*>(&sweep_kernel<P, C, 4, true, 3>);
+    if (kr == 8) return reinterpret_cast<const void*>(&sweep_kernel<P, C, 8, true, 3>);
+    if (glds) {
+      return kbm == 3 ? reinterpret_cast<const void*>(&sweep_kernel<P, C, 0, true, 3>)
+                      : reinterpret_cast<const void*>(&sweep_kernel<P, C, 0, true, 7>);
+    }
+    return kbm == 3 ? reinterpret_cast<const void*>(&sweep_kernel<P, C, 0, false, 3>)
+                    : reinterpret_cast<const void*>(&sweep_kernel<P, C, 0, false, 7>);
+  }
+};
+
+static const void* sweep_fn(bool periodic, bool canonical, int kr, bool glds, int kbm) {
+  if (periodic) return canonical ? SweepVariants<true, true>::fn(kr, glds, kbm) : SweepVariants<true, false>::fn(kr, glds, kbm);
+  return canonical ? SweepVariants<false, true>::fn(kr, glds, kbm) : SweepVariants<false, false>::fn(kr, glds, kbm);
+}
+
+bool sweep_variant_ok(int kr, bool glds, int kbm) {
+  if (kr == 2 || kr == 4 || kr == 8) return glds && (kbm == 1 || kbm == 3);
+  return kr == 0 && (kbm == 3 || kbm == 7);
+}
+
 hipError_t sweep_set_max_lds(uint32_t bytes) {
   const int b = int(bytes);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&sweep_kernel<true, true>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&sweep_kernel<true, false>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&sweep_kernel<false, true>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&sweep_kernel<false, false>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  return e;
+  for (int p = 0; p < 2; ++p)
+    for (int cn = 0; cn < 2; ++cn)
+      for (int kr : {0, 2, 4, 8})
+        for (int g = 0; g < 2; ++g)
+          for (int kbm : {1, 3, 7}) {
+            if (!sweep_variant_ok(kr, g != 0, kbm)) continue;
+            const hipError_t e = hipFuncSetAttribute(sweep_fn(p != 0, cn != 0, kr, g != 0, kbm),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            if (e != hipSuccess) return e;
+          }
+  return hipSuccess;
 }
 
 hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream) {
-  const dim3 grid(a.n_chains), block(64);
-  if (a.periodic) {
-    if (a.gtab_lds)
-      hipLaunchKernelGGL((sweep_kernel<true, true>), grid, block, a.lds_bytes, stream, a.params, a.states,
-                         a.models, a.occ_pool, a.trk_pool, a.chain_order, a.n_moves, a.accept_log, a.log_words);
-    else
-      hipLaunchKernelGGL((sweep_kernel<true, false>), grid, block, a.lds_bytes, stream, a.params, a.states,
-                         a.models, a.occ_pool, a.trk_pool, a.chain_order, a.n_moves, a.accept_log, a.log_words);
-  } else {
-    if (a.gtab_lds)
-      hipLaunchKernelGGL((sweep_kernel<false, true>), grid, block, a.lds_bytes, stream, a.params, a.states,
-                         a.models, a.occ_pool, a.trk_pool, a.chain_order, a.n_moves, a.accept_log, a.log_words);
-    else
-      hipLaunchKernelGGL((sweep_kernel<false, false>), grid, block, a.lds_bytes, stream, a.params, a.states,
-                         a.models, a.occ_pool, a.trk_pool, a.chain_order, a.n_moves, a.accept_log, a.log_words);
-  }
-  return hipGetLastError();
+  if (!sweep_variant_ok(a.kr, a.gtab_lds, a.kbm)) return hipErrorInvalidValue;
+  const void* fn = sweep_fn(a.periodic, a.canonical, a.kr, a.gtab_lds, a.kbm);
+  const ChainParams* params = a.params;
+  ChainState* states = a.states;
+  const ModelDev* models = a.models;
+  int8_t* occ_pool = a.occ_pool;
+  uint16_t* trk_pool = a.trk_pool;
+  const int32_t* order = a.chain_order;
+  const int64_t* n_moves = a.n_moves;
+  uint32_t* log = a.accept_log;
+  int64_t log_words = a.log_words;
+  void* args[] = {&params, &states, &models, &occ_pool, &trk_pool, &order, &n_moves, &log, &log_words};
+  return hipLaunchKernel(fn, dim3(a.n_chains), dim3(64), args, a.lds_bytes, stream);
 }
 
 hipError_t launch_cf(const ModelDev* models, const ChainParams* params, const int8_t* occ_pool,
