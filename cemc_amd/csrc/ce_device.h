@@ -33,19 +33,27 @@ __host__ __device__ inline int rec_size(SubRec r) { return int(r.w0 >> 30) + 2; 
 __host__ __device__ inline uint32_t rec_gbase(SubRec r) { return r.w1 & 0xFFFFFFu; }
 __host__ __device__ inline int rec_refpos(SubRec r) { return int((r.w1 >> 24) & 3u); }
 
-// Sub-cluster record of the sweep kernel (16 bytes, held in VGPRs or LDS):
-//   m0..m2  members in canonical order with the reference site removed:
-//           closed-form periodic lattices: packed cell offset x | y<<8 | z<<16;
-//           otherwise: translation-table column.  Unused members are
-//           kUnusedMember: they address the reference site and their species
-//           is masked to zero.
-//   meta    energy-table base (24 bits) | number of members << 24 | valid << 31
-// Energy table of a cluster entry: G[r][code][d], index base_r + code*B + d,
-// with base_r stored in meta for the record's reference position r.
+// Sub-cluster record of the sweep kernel (32 bytes, held in VGPRs or LDS):
+//   m[q]   members in canonical order with the reference site removed:
+//          closed-form periodic lattices: signed offset of the member in the
+//          padded (halo) lattice, index(member) = index(reference) + m[q];
+//          otherwise: translation-table column.
+//   w[q]   periodic lattices: member cell offset wrapped into the supercell,
+//          packed x | y<<8 | z<<16 (identifies the member as a site).
+//   meta   energy-table base (24 bits) | members << 24 (2 bits) |
+//          member 1 unused << 28 | member 2 unused << 29 | valid << 31
+// Unused members address the reference site (offset 0) and their species is
+// masked to zero.  Energy table of a cluster entry: G[r][code][d], index
+// base_r + code*B + d, with base_r in meta for the record's reference position.
 struct FatRec {
-  uint32_t m0, m1, m2, meta;
+  int32_t m0, m1, m2;
+  uint32_t w0, w1, w2;
+  uint32_t meta;
+  uint32_t pad;
 };
-constexpr uint32_t kUnusedMember = 0x80000000u;  // member slot of a smaller cluster (offset 0, flag bit 31)
+constexpr uint32_t kRecUnused1 = 1u << 28;
+constexpr uint32_t kRecUnused2 = 1u << 29;
+constexpr uint32_t kRecValid = 1u << 31;
 
 // Cluster entry = one (translation group, orbit) pair with its record range.
 struct ClusterEntry {
@@ -60,6 +68,8 @@ struct ModelDev {
   int32_t n_sites, n_species, n_bfs, n_groups;
   int32_t n_cf, n_entries, K, periodic;
   int32_t dims[3];
+  int32_t pad_h;          // halo width of the padded lattice (periodic only)
+  int32_t pdims[3];       // padded dims L + 2*pad_h
   int32_t has_background;
   int32_t n_movable;
   int32_t gtab_size;      // doubles in the energy table
@@ -170,12 +180,11 @@ __device__ inline double readlane_f64(double v, int lane) {
   return __longlong_as_double(int64_t(readlane_u64(uint64_t(__double_as_longlong(v)), lane)));
 }
 
-template <int kCtrl, int kRowMask = 0xF, int kBankMask = 0xF>
+template <int kCtrl>
 __device__ inline double dpp_f64(double v) {
   const int64_t bits = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, int(uint32_t(bits)), kCtrl, kRowMask, kBankMask, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, int(uint32_t(uint64_t(bits) >> 32)), kCtrl, kRowMask,
-                                             kBankMask, false);
+  const int lo = __builtin_amdgcn_mov_dpp(int(uint32_t(bits)), kCtrl, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(int(uint32_t(uint64_t(bits) >> 32)), kCtrl, 0xF, 0xF, true);
   return __longlong_as_double(int64_t((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo)));
 }
 
@@ -189,14 +198,24 @@ __device__ inline double row_sum_f64(double v) {
   return v;
 }
 
-// Sum over all 64 lanes, returned as a wave-uniform value.
+// Sum over all 64 lanes; every lane ends with the same total (rows combined
+// with the gfx950 cross-row swaps, (r0 + r1) + (r2 + r3) in every lane).
 __device__ inline double wave_sum_f64(double v) {
   v = row_sum_f64(v);
-  const double r0 = readlane_f64(v, 0);
-  const double r1 = readlane_f64(v, 16);
-  const double r2 = readlane_f64(v, 32);
-  const double r3 = readlane_f64(v, 48);
-  return (r0 + r1) + (r2 + r3);
+  const int64_t b = __double_as_longlong(v);
+  const auto lo16 = __builtin_amdgcn_permlane16_swap(uint32_t(b), uint32_t(b), false, false);
+  const auto hi16 = __builtin_amdgcn_permlane16_swap(uint32_t(uint64_t(b) >> 32), uint32_t(uint64_t(b) >> 32),
+                                                     false, false);
+  const double even = __longlong_as_double(int64_t((uint64_t(hi16[0]) << 32) | lo16[0]));  // r0 / r2
+  const double odd = __longlong_as_double(int64_t((uint64_t(hi16[1]) << 32) | lo16[1]));   // r1 / r3
+  v = even + odd;
+  const int64_t c = __double_as_longlong(v);
+  const auto lo32 = __builtin_amdgcn_permlane32_swap(uint32_t(c), uint32_t(c), false, false);
+  const auto hi32 = __builtin_amdgcn_permlane32_swap(uint32_t(uint64_t(c) >> 32), uint32_t(uint64_t(c) >> 32),
+                                                     false, false);
+  const double lower = __longlong_as_double(int64_t((uint64_t(hi32[0]) << 32) | lo32[0]));  // r0 + r1
+  const double upper = __longlong_as_double(int64_t((uint64_t(hi32[1]) << 32) | lo32[1]));  // r2 + r3
+  return lower + upper;
 }
 
 // ---------------------------------------------------------------------------

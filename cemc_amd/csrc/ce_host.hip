@@ -91,6 +91,9 @@ struct cemc_model {
   int n_sites = 0, n_species = 0, n_bfs = 0, n_groups = 0, n_cf = 0, n_entries = 0, K = 0;
   bool periodic = false;
   int dims[3] = {0, 0, 0};
+  int pad_h = 0;
+  int pdims[3] = {0, 0, 0};
+  std::vector<int> signed_cells;
   std::vector<double> basis, eci, gtab;
   std::vector<int32_t> site_group, cf_kind, cf_dec, cf_entry, cf_eq_off, movable;
   std::vector<uint8_t> background;
@@ -219,6 +222,8 @@ void cemc_model::refresh_view() {
   v.K = K;
   v.periodic = periodic ? 1 : 0;
   for (int i = 0; i < 3; ++i) v.dims[i] = dims[i];
+  v.pad_h = pad_h;
+  for (int i = 0; i < 3; ++i) v.pdims[i] = pdims[i];
   v.has_background = int(movable.size()) != n_sites;
   v.n_movable = int(movable.size());
   v.gtab_size = int(gtab.size());
@@ -370,8 +375,9 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
         const uint32_t gbase = uint32_t(m->entry_goff[c] + r * B * m->pow_s(n - 1));
         const uint32_t w1 = gbase | (uint32_t(r) << 24) | (uint32_t(c) << 26);
         m->recs.push_back(SubRec{w0, w1});
-        m->rec_slots.push_back({rec_slot(SubRec{w0, w1}, 0), rec_slot(SubRec{w0, w1}, 1),
-                                rec_slot(SubRec{w0, w1}, 2), gbase | (uint32_t(n - 1) << 24) | (1u << 31)});
+        m->rec_slots.push_back(FatRec{int32_t(rec_slot(SubRec{w0, w1}, 0)), int32_t(rec_slot(SubRec{w0, w1}, 1)),
+                                      int32_t(rec_slot(SubRec{w0, w1}, 2)), 0u, 0u, 0u,
+                                      gbase | (uint32_t(n - 1) << 24) | kRecValid, 0u});
       }
     }
     m->grp_rec_cnt[g] = int(m->recs.size()) - m->grp_rec_off[g];
@@ -394,27 +400,60 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
         ok = m->trans[int64_t(i) * m->K + j] == (xx * L2 + yy) * L3 + zz;
       }
     }
-    if (ok) {
+    // halo width: largest member offset, offsets recovered from the wrapped
+    // ones; the padded box needs L > 2h in every dimension
+    int h = 0;
+    for (int j = 0; ok && j < m->K; ++j) {
+      const int L[3] = {L1, L2, L3};
+      for (int dim = 0; dim < 3; ++dim) {
+        int o = d->col_cells[3 * j + dim];
+        if (2 * o > L[dim]) o -= L[dim];
+        h = std::max(h, std::abs(o));
+      }
+    }
+    if (ok && 2 * h < L1 && 2 * h < L2 && 2 * h < L3 &&
+        int64_t(L1 + 2 * h) * (L2 + 2 * h) * (L3 + 2 * h) < (1 << 20)) {
       m->periodic = true;
       m->dims[0] = L1;
       m->dims[1] = L2;
       m->dims[2] = L3;
+      m->pad_h = h;
+      m->pdims[0] = L1 + 2 * h;
+      m->pdims[1] = L2 + 2 * h;
+      m->pdims[2] = L3 + 2 * h;
+      m->signed_cells.resize(3 * size_t(m->K));
+      for (int j = 0; j < m->K; ++j) {
+        const int L[3] = {L1, L2, L3};
+        for (int dim = 0; dim < 3; ++dim) {
+          int o = d->col_cells[3 * j + dim];
+          if (2 * o > L[dim]) o -= L[dim];
+          m->signed_cells[3 * j + dim] = o;
+        }
+      }
     }
   }
   m->build_gtab(m->eci);
-  // sweep records: member cell offsets on closed-form lattices, else columns
+  // sweep records: padded-box offsets and wrapped offsets of the members on
+  // closed-form lattices, else translation-table columns
   m->frecs.clear();
-  for (const auto& fr : m->rec_slots) {
-    FatRec f = fr;
-    const int nm = int((fr.meta >> 24) & 3u);
-    if (m->periodic) {
-      f.m0 = m->col_cells[fr.m0];
-      f.m1 = m->col_cells[fr.m1];
-      f.m2 = m->col_cells[fr.m2];
+  for (const auto& sr : m->rec_slots) {
+    const int nm = int((sr.meta >> 24) & 3u);
+    const uint32_t slot[3] = {uint32_t(sr.m0), uint32_t(sr.m1), uint32_t(sr.m2)};
+    int32_t mm[3] = {0, 0, 0};
+    uint32_t ww[3] = {0u, 0u, 0u};
+    for (int q = 0; q < nm; ++q) {
+      if (m->periodic) {
+        const int* o = &m->signed_cells[3 * slot[q]];
+        mm[q] = (o[0] * m->pdims[1] + o[1]) * m->pdims[2] + o[2];
+        ww[q] = m->col_cells[slot[q]];
+      } else {
+        mm[q] = int32_t(slot[q]);
+      }
     }
-    if (nm < 2) f.m1 = kUnusedMember;
-    if (nm < 3) f.m2 = kUnusedMember;
-    m->frecs.push_back(f);
+    uint32_t meta = sr.meta;
+    if (nm < 2) meta |= kRecUnused1;
+    if (nm < 3) meta |= kRecUnused2;
+    m->frecs.push_back(FatRec{mm[0], mm[1], mm[2], ww[0], ww[1], ww[2], meta, 0u});
   }
   // upload
   std::vector<int8_t> sg(N);
@@ -1002,7 +1041,9 @@ int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const c
         k.kbm = m->n_species <= 4 ? 3 : 7;
         const bool single = m->n_groups == 1 && int(m->movable.size()) == m->n_sites;
         const int cnt = m->grp_rec_cnt[0];
-        const uint32_t with_g = sweep_lds_bytes(m->n_sites, canon, 0, int(m->gtab.size()));
+        const int occ_bytes = m->periodic ? m->pdims[0] * m->pdims[1] * m->pdims[2] : m->n_sites;
+        const int trk_sites = canon ? m->n_sites : 0;
+        const uint32_t with_g = sweep_lds_bytes(occ_bytes, trk_sites, 0, int(m->gtab.size()));
         k.kr = 0;
         if (single && k.kbm == 3 && with_g <= gtab_budget && cnt <= 512) {
           k.kr = cnt <= 128 ? 2 : (cnt <= 256 ? 4 : 8);
@@ -1012,8 +1053,8 @@ int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const c
           k.glds = true;
           k.lds = with_g;
         } else {
-          const uint32_t rec_g = sweep_lds_bytes(m->n_sites, canon, int(m->recs.size()), int(m->gtab.size()));
-          const uint32_t rec_only = sweep_lds_bytes(m->n_sites, canon, int(m->recs.size()), 0);
+          const uint32_t rec_g = sweep_lds_bytes(occ_bytes, trk_sites, int(m->recs.size()), int(m->gtab.size()));
+          const uint32_t rec_only = sweep_lds_bytes(occ_bytes, trk_sites, int(m->recs.size()), 0);
           k.glds = rec_g <= gtab_budget;
           k.lds = k.glds ? rec_g : rec_only;
           require(k.lds <= limit, "chain state does not fit in the workgroup's LDS");

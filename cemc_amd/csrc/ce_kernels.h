@@ -25,7 +25,9 @@ struct LaunchSweep {
   int64_t log_words;
 };
 
-uint32_t sweep_lds_bytes(int n_sites, int canonical, int n_rec_lds, int gtab_doubles);
+// Dynamic LDS of one chain: occupation bytes (padded box on periodic
+// lattices), tracker entries (canonical chains), records held in LDS, table doubles.
+uint32_t sweep_lds_bytes(int occ_bytes, int trk_sites, int n_rec_lds, int gtab_doubles);
 bool sweep_variant_ok(int kr, bool gtab_lds, int kbm);
 // Raises the dynamic-LDS limit of every sweep kernel instantiation.
 hipError_t sweep_set_max_lds(uint32_t bytes);

@@ -32,21 +32,22 @@ struct LdsLayout {
 
 __host__ __device__ inline uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
 
-// occupation | species tracker (canonical) | records (large models) | energy tables
-__host__ __device__ inline LdsLayout lds_layout(int n_sites, int canonical, int n_rec_lds, int gtab_doubles) {
+// occupation (padded box on periodic lattices) | species tracker (canonical)
+// | records (large models) | energy tables
+__host__ __device__ inline LdsLayout lds_layout(int occ_bytes, int trk_sites, int n_rec_lds, int gtab_doubles) {
   LdsLayout L;
   L.occ = 0;
-  L.trk = align16(uint32_t(n_sites) + 1u);  // occ[n_sites] is a zero byte read by unused members
-  L.rec = L.trk + (canonical ? align16(uint32_t(n_sites) * 2u) : 0u);
-  L.gtab = L.rec + uint32_t(n_rec_lds) * 16u;
+  L.trk = align16(uint32_t(occ_bytes));
+  L.rec = L.trk + align16(uint32_t(trk_sites) * 2u);
+  L.gtab = L.rec + uint32_t(n_rec_lds) * uint32_t(sizeof(FatRec));
   L.total = L.gtab + uint32_t(gtab_doubles) * 8u;
   return L;
 }
 
 }  // namespace
 
-uint32_t sweep_lds_bytes(int n_sites, int canonical, int n_rec_lds, int gtab_doubles) {
-  return lds_layout(n_sites, canonical, n_rec_lds, gtab_doubles).total;
+uint32_t sweep_lds_bytes(int occ_bytes, int trk_sites, int n_rec_lds, int gtab_doubles) {
+  return lds_layout(occ_bytes, trk_sites, n_rec_lds, gtab_doubles).total;
 }
 
 // ---------------------------------------------------------------------------
@@ -56,15 +57,23 @@ uint32_t sweep_lds_bytes(int n_sites, int canonical, int n_rec_lds, int gtab_dou
 // staged before the loop: model scalars and packed species tables in SGPRs,
 // the occupation, species tracker and energy tables in LDS, and each lane's
 // sub-cluster records in VGPRs (kR per lane) or, for large models, in LDS
-// (read in batches of 4 per lane).  Per trial move each lane computes the
-// member sites of its records (closed form: cell offsets carried by the
-// record), issues all member-species gathers at once, forms the table codes,
-// issues all energy-table reads at once, and the partial energy changes are
-// reduced across the wave with DPP.
+// (read in batches of 4 per lane).
+//
+// On closed-form periodic lattices the occupation is held in a padded
+// (halo) box of (L1+2h) x (L2+2h) x (L3+2h) bytes, each cell holding the
+// species of the site it wraps to; a member's address is then the reference
+// site's padded index plus a per-record constant, and an accepted flip
+// writes the site's (at most 8) images.  Other lattices read members through
+// the translation table in HBM.
+//
+// Per trial move each lane issues all member-species gathers of its records
+// at once (one LDS round trip), then all energy-table reads (a second round
+// trip), and the partial energy changes are summed across the wave.
 
 struct SweepCtx {
   int S, B, N;
   int L1, L2, L3, L23;
+  int h, P2, P3;         // halo width and padded dims (periodic)
   uint32_t mag23, mag3;  // floor((2^32-1)/d) for the divisors L2*L3 and L3
   int K;
   const int32_t* trans;
@@ -78,7 +87,9 @@ __device__ __attribute__((always_inline)) inline uint32_t fast_div(uint32_t n, u
 }
 
 struct Ref {
-  int site, x, y, z;
+  int site;     // lattice site
+  int p;        // padded index (periodic) or site (table)
+  int x, y, z;  // cell coordinates (periodic)
 };
 
 template <bool kPeriodic>
@@ -92,45 +103,48 @@ __device__ __attribute__((always_inline)) inline Ref ref_of(const SweepCtx& c, i
     r.x = int(x);
     r.y = int(y);
     r.z = int(rem - y * uint32_t(c.L3));
+    r.p = __umul24(__umul24(uint32_t(r.x + c.h), uint32_t(c.P2)) + uint32_t(r.y + c.h), uint32_t(c.P3)) +
+          uint32_t(r.z + c.h);
   } else {
     r.x = r.y = r.z = 0;
+    r.p = site;
   }
   return r;
 }
 
-// Site playing the role of member `m` (cell offset or table column) when
-// `r` is the reference site (the translation matrix, ce_updater.cpp:264).
-// Unused members (bit 31 set, offset bits zero) give a valid site whose
-// species the caller masks to zero.
-template <bool kPeriodic>
-__device__ __attribute__((always_inline)) inline int member_site(const SweepCtx& c, const Ref& r, uint32_t m) {
-  if constexpr (kPeriodic) {
-    int x = r.x + int(m & 0xFFu);
-    int y = r.y + int((m >> 8) & 0xFFu);
-    int z = r.z + int((m >> 16) & 0xFFu);
-    x -= (x >= c.L1) ? c.L1 : 0;
-    y -= (y >= c.L2) ? c.L2 : 0;
-    z -= (z >= c.L3) ? c.L3 : 0;
-    return (x * c.L2 + y) * c.L3 + z;
-  } else {
-    return c.trans[int64_t(r.site) * c.K + (m & 0xFFFFu)];
-  }
+// Packed wrapped cell offset from site j to site i (x | y<<8 | z<<16): a
+// member of j with this wrapped offset is site i.
+__device__ __attribute__((always_inline)) inline uint32_t wrapped_delta(const SweepCtx& c, const Ref& i,
+                                                                        const Ref& j) {
+  int dx = i.x - j.x, dy = i.y - j.y, dz = i.z - j.z;
+  dx += dx < 0 ? c.L1 : 0;
+  dy += dy < 0 ? c.L2 : 0;
+  dz += dz < 0 ? c.L3 : 0;
+  return uint32_t(dx) | (uint32_t(dy) << 8) | (uint32_t(dz) << 16);
 }
 
-// Phase 1 of a flip: member sites of kR records (pure ALU, no branches),
-// issue of all member-species gathers, energy-table indices.  A member equal
-// to `patch_site` reads `patch_species`.
+// Phase 1 of a flip: addresses of every member of kR records, issue of all
+// member-species gathers, energy-table indices.  Members identified as the
+// patch site (periodic: wrapped offset == patch_w; table: site == patch_w)
+// read `patch_species`.
 template <bool kPeriodic, int kR>
 __device__ __attribute__((always_inline)) inline void table_index(const SweepCtx& c, const FatRec (&rec)[kR],
                                                                   const Ref& ref, const int8_t* occ,
-                                                                  int patch_site, int patch_species,
+                                                                  uint32_t patch_w, int patch_species,
                                                                   int (&gi)[kR]) {
   int nb[kR][3];
 #pragma unroll
   for (int k = 0; k < kR; ++k) {
-    nb[k][0] = member_site<kPeriodic>(c, ref, rec[k].m0);
-    nb[k][1] = member_site<kPeriodic>(c, ref, rec[k].m1);
-    nb[k][2] = member_site<kPeriodic>(c, ref, rec[k].m2);
+    if constexpr (kPeriodic) {
+      nb[k][0] = ref.p + rec[k].m0;
+      nb[k][1] = ref.p + rec[k].m1;
+      nb[k][2] = ref.p + rec[k].m2;
+    } else {
+      const int32_t* row = c.trans + int64_t(ref.site) * c.K;
+      nb[k][0] = row[rec[k].m0];
+      nb[k][1] = row[rec[k].m1];
+      nb[k][2] = row[rec[k].m2];
+    }
   }
   int sp[kR][3];
 #pragma unroll
@@ -140,14 +154,20 @@ __device__ __attribute__((always_inline)) inline void table_index(const SweepCtx
 #pragma unroll
   for (int k = 0; k < kR; ++k) {
     int s0 = sp[k][0], s1 = sp[k][1], s2 = sp[k][2];
-    s0 = nb[k][0] == patch_site ? patch_species : s0;
-    s1 = nb[k][1] == patch_site ? patch_species : s1;
-    s2 = nb[k][2] == patch_site ? patch_species : s2;
+    if constexpr (kPeriodic) {
+      s0 = rec[k].w0 == patch_w ? patch_species : s0;
+      s1 = rec[k].w1 == patch_w ? patch_species : s1;
+      s2 = rec[k].w2 == patch_w ? patch_species : s2;
+    } else {
+      s0 = uint32_t(nb[k][0]) == patch_w ? patch_species : s0;
+      s1 = uint32_t(nb[k][1]) == patch_w ? patch_species : s1;
+      s2 = uint32_t(nb[k][2]) == patch_w ? patch_species : s2;
+    }
     // species of unused members masked to zero: code = sum_q s_q S^q over the used ones
-    s1 &= int(rec[k].m1 >> 31) - 1;
-    s2 &= int(rec[k].m2 >> 31) - 1;
-    const int code = s0 + c.S * (s1 + c.S * s2);
-    gi[k] = int(rec[k].meta & 0xFFFFFFu) + code * c.B;
+    s1 &= (rec[k].meta & kRecUnused1) ? 0 : -1;
+    s2 &= (rec[k].meta & kRecUnused2) ? 0 : -1;
+    const uint32_t code = uint32_t(s0) + __umul24(uint32_t(c.S), uint32_t(s1) + __umul24(uint32_t(c.S), uint32_t(s2)));
+    gi[k] = int(rec[k].meta & 0xFFFFFFu) + int(__umul24(code, uint32_t(c.B)));
   }
 }
 
@@ -167,19 +187,9 @@ __device__ __attribute__((always_inline)) inline double table_sum(const FatRec (
     double e = 0.0;
 #pragma unroll
     for (int d = 0; d < kBM; ++d) e += dphi[d] * g[k][d];
-    v += (rec[k].meta >> 31) ? e : 0.0;
+    v += (rec[k].meta & kRecValid) ? e : 0.0;
   }
   return v;
-}
-
-template <bool kPeriodic, int kR, int kBM>
-__device__ __attribute__((always_inline)) inline double batch_de(const SweepCtx& c, const FatRec (&rec)[kR],
-                                                                 const Ref& ref, const int8_t* occ,
-                                                                 const double* gtab, const double (&dphi)[kBM],
-                                                                 int patch_site, int patch_species) {
-  int gi[kR];
-  table_index<kPeriodic, kR>(c, rec, ref, occ, patch_site, patch_species, gi);
-  return table_sum<kR, kBM>(rec, gi, gtab, dphi);
 }
 
 // Records of one translation group read from LDS in batches of 4 per lane.
@@ -187,7 +197,7 @@ template <bool kPeriodic, int kBM>
 __device__ __attribute__((always_inline)) inline double lds_flip_de(const SweepCtx& c, const FatRec* recs, int off,
                                                                     int cnt, int lane, const Ref& ref,
                                                                     const int8_t* occ, const double* gtab,
-                                                                    const double (&dphi)[kBM], int patch_site,
+                                                                    const double (&dphi)[kBM], uint32_t patch_w,
                                                                     int patch_species) {
   double v = 0.0;
   for (int base = 0; base < cnt; base += 256) {
@@ -195,15 +205,39 @@ __device__ __attribute__((always_inline)) inline double lds_flip_de(const SweepC
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int it = base + lane + 64 * k;
-      rb[k] = it < cnt ? recs[off + it] : FatRec{kUnusedMember, kUnusedMember, kUnusedMember, 0u};
+      rb[k] = it < cnt ? recs[off + it] : FatRec{0, 0, 0, 0u, 0u, 0u, kRecUnused1 | kRecUnused2, 0u};
     }
-    v += batch_de<kPeriodic, 4, kBM>(c, rb, ref, occ, gtab, dphi, patch_site, patch_species);
+    int gi[4];
+    table_index<kPeriodic, 4>(c, rb, ref, occ, patch_w, patch_species, gi);
+    v += table_sum<4, kBM>(rb, gi, gtab, dphi);
   }
   return v;
 }
 
 __device__ __attribute__((always_inline)) inline int unpack16(uint64_t lo, uint64_t hi, uint32_t i) {
   return int(((i < 4u ? lo : hi) >> (16u * (i & 3u))) & 0xFFFFu);
+}
+
+// Padded index of image `bits` (bit 0: x, 1: y, 2: z take the other image)
+// of the site at cell (x, y, z), or -1 when that image does not exist.
+__device__ __attribute__((always_inline)) inline int image_index(const SweepCtx& c, int x, int y, int z, int bits) {
+  int ix = x, iy = y, iz = z;
+  bool ok = true;
+  if (bits & 1) {
+    ix = x < c.h ? x + c.L1 : (x >= c.L1 - c.h ? x - c.L1 : x);
+    ok = ok && ix != x;
+  }
+  if (bits & 2) {
+    iy = y < c.h ? y + c.L2 : (y >= c.L2 - c.h ? y - c.L2 : y);
+    ok = ok && iy != y;
+  }
+  if (bits & 4) {
+    iz = z < c.h ? z + c.L3 : (z >= c.L3 - c.h ? z - c.L3 : z);
+    ok = ok && iz != z;
+  }
+  return ok ? int(__umul24(__umul24(uint32_t(ix + c.h), uint32_t(c.P2)) + uint32_t(iy + c.h), uint32_t(c.P3)) +
+                  uint32_t(iz + c.h))
+            : -1;
 }
 
 // kR > 0: single-group model, records in VGPRs (kR per lane); kR == 0:
@@ -227,7 +261,8 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
   const ChainParams* __restrict__ pp = params + chain;
   const ModelDev& m = models[pp->model];
   const int N = pp->n_sites;
-  const LdsLayout L = lds_layout(N, kCanonical, kRecLds ? m.n_rec : 0, kGtabLds ? m.gtab_size : 0);
+  const int occ_bytes = kPeriodic ? m.pdims[0] * m.pdims[1] * m.pdims[2] : N;
+  const LdsLayout L = lds_layout(occ_bytes, kCanonical ? N : 0, kRecLds ? m.n_rec : 0, kGtabLds ? m.gtab_size : 0);
   int8_t* occ = reinterpret_cast<int8_t*>(lds + L.occ);
   uint16_t* trk = reinterpret_cast<uint16_t*>(lds + L.trk);
   FatRec* recs_l = reinterpret_cast<FatRec*>(lds + L.rec);
@@ -241,6 +276,9 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
   c.L2 = m.dims[1];
   c.L3 = m.dims[2];
   c.L23 = c.L2 * c.L3;
+  c.h = m.pad_h;
+  c.P2 = m.pdims[1];
+  c.P3 = m.pdims[2];
   c.mag23 = kPeriodic ? 0xFFFFFFFFu / uint32_t(c.L23) : 0u;
   c.mag3 = kPeriodic ? 0xFFFFFFFFu / uint32_t(c.L3) : 0u;
   c.K = m.K;
@@ -251,8 +289,24 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
 
   // ---- stage chain state, per-chain tables and model tables into LDS
   const int8_t* gocc = occ_pool + pp->occ_off;
-  for (int i = lane; i < N; i += 64) occ[i] = gocc[i];
-  if (lane == 0) occ[N] = 0;
+  if constexpr (kPeriodic) {
+    const uint32_t p23 = uint32_t(c.P2 * c.P3), mag_p23 = 0xFFFFFFFFu / p23, mag_p3 = 0xFFFFFFFFu / uint32_t(c.P3);
+    for (int p = lane; p < occ_bytes; p += 64) {
+      const uint32_t px = fast_div(uint32_t(p), p23, mag_p23);
+      const uint32_t rem = uint32_t(p) - px * p23;
+      const uint32_t py = fast_div(rem, uint32_t(c.P3), mag_p3);
+      int x = int(px) - c.h, y = int(py) - c.h, z = int(rem - py * uint32_t(c.P3)) - c.h;
+      x += x < 0 ? c.L1 : 0;
+      x -= x >= c.L1 ? c.L1 : 0;
+      y += y < 0 ? c.L2 : 0;
+      y -= y >= c.L2 ? c.L2 : 0;
+      z += z < 0 ? c.L3 : 0;
+      z -= z >= c.L3 ? c.L3 : 0;
+      occ[p] = gocc[(x * c.L2 + y) * c.L3 + z];
+    }
+  } else {
+    for (int i = lane; i < N; i += 64) occ[i] = gocc[i];
+  }
   uint16_t* gtrk = trk_pool + pp->trk_off;
   if constexpr (kCanonical)
     for (int i = lane; i < N; i += 64) trk[i] = gtrk[i];
@@ -281,7 +335,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
 #pragma unroll
     for (int k = 0; k < kR; ++k) {
       const int it = lane + 64 * k;
-      rr[k] = it < cnt ? m.frecs[it] : FatRec{kUnusedMember, kUnusedMember, kUnusedMember, 0u};
+      rr[k] = it < cnt ? m.frecs[it] : FatRec{0, 0, 0, 0u, 0u, 0u, kRecUnused1 | kRecUnused2, 0u};
     }
   }
 
@@ -311,7 +365,6 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
     }
     const uint64_t w0 = readlane_u64(blk.w0, l);
     const uint64_t w1 = readlane_u64(blk.w1, l);
-    const uint64_t w3 = readlane_u64(blk.w3, l);
     const double lu = readlane_f64(log_u, l);
 
     int si, sj = 0, old_i = 0, new_i = 0, sa = 0, sb = 0, off_a = 0, off_b = 0;
@@ -332,17 +385,18 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
       si = has_bkg ? m.movable[pick] : int(pick);
     }
     const Ref ri = ref_of<kPeriodic>(c, si);
+    Ref rj = ri;
+    uint32_t patch_j = 0xFFFFFFFFu;
+    if constexpr (kCanonical) {
+      rj = ref_of<kPeriodic>(c, sj);
+      patch_j = kPeriodic ? wrapped_delta(c, ri, rj) : uint32_t(si);
+    }
     int gi_i[kR > 0 ? kR : 1], gi_j[kR > 0 ? kR : 1];
+    if constexpr (!kCanonical) old_i = occ[ri.p];
     if constexpr (kR > 0) {
-      // all member gathers of the move are issued together with occ[si]
-      if constexpr (!kCanonical) old_i = occ[si];
-      table_index<kPeriodic, kR>(c, rr, ri, occ, -1, 0, gi_i);
-      if constexpr (kCanonical) {
-        const Ref rj = ref_of<kPeriodic>(c, sj);
-        table_index<kPeriodic, kR>(c, rr, rj, occ, si, new_i, gi_j);
-      }
-    } else {
-      if constexpr (!kCanonical) old_i = occ[si];
+      // all member gathers of the move are issued together
+      table_index<kPeriodic, kR>(c, rr, ri, occ, 0xFFFFFFFFu, 0, gi_i);
+      if constexpr (kCanonical) table_index<kPeriodic, kR>(c, rr, rj, occ, patch_j, new_i, gi_j);
     }
     if constexpr (!kCanonical)
       new_i = int((uint32_t(old_i) + 1u + bounded32(uint32_t(w0), uint32_t(c.S - 1))) % uint32_t(c.S));
@@ -357,7 +411,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
     } else {
       const int gi = multi_group ? int(m.site_group[si]) : 0;
       part = lds_flip_de<kPeriodic, kBM>(c, recs_l, grp_tab[gi], grp_tab[kMaxGroups + gi], lane, ri, occ, gtab,
-                                         dphi, -1, 0);
+                                         dphi, 0xFFFFFFFFu, 0);
     }
     if constexpr (kCanonical) {
       // second flip of the swap: j goes b -> a and sees site i already at b
@@ -369,10 +423,9 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
       if constexpr (kR > 0) {
         part += table_sum<kR, kBM>(rr, gi_j, gtab, dphj);
       } else {
-        const Ref rj = ref_of<kPeriodic>(c, sj);
         const int gj = multi_group ? int(m.site_group[sj]) : 0;
         part += lds_flip_de<kPeriodic, kBM>(c, recs_l, grp_tab[gj], grp_tab[kMaxGroups + gj], lane, rj, occ,
-                                            gtab, dphj, si, new_i);
+                                            gtab, dphj, patch_j, new_i);
       }
     }
     const double dE = wave_sum_f64(part) + de_single;
@@ -391,13 +444,21 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
       else if (lu > x + margin)
         accept = false;
       else
-        accept = unit_double(w3) <= exp(-dE / kT);
+        accept = unit_double(readlane_u64(blk.w3, l)) <= exp(-dE / kT);
     }
     if (accept) {
-      if (lane == 0) {
-        occ[si] = int8_t(new_i);
-        if constexpr (kCanonical) {
-          occ[sj] = int8_t(old_i);
+      if constexpr (kPeriodic) {
+        // lanes 0-7 write the images of site i, lanes 8-15 those of site j
+        const bool second = kCanonical && lane >= 8 && lane < 16;
+        const Ref& rs = second ? rj : ri;
+        const int idx = lane < (kCanonical ? 16 : 8) ? image_index(c, rs.x, rs.y, rs.z, lane & 7) : -1;
+        if (idx >= 0) occ[idx] = int8_t(second ? old_i : new_i);
+      } else {
+        if (lane == 0) occ[si] = int8_t(new_i);
+        if (kCanonical && lane == 1) occ[sj] = int8_t(old_i);
+      }
+      if constexpr (kCanonical) {
+        if (lane == 0) {
           trk[off_a + sa] = uint16_t(sj);
           trk[off_b + sb] = uint16_t(si);
         }
@@ -417,7 +478,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
 
   // ---- write back
   int8_t* wocc = occ_pool + pp->occ_off;
-  for (int i = lane; i < N; i += 64) wocc[i] = occ[i];
+  for (int i = lane; i < N; i += 64) wocc[i] = occ[ref_of<kPeriodic>(c, i).p];
   if constexpr (kCanonical)
     for (int i = lane; i < N; i += 64) gtrk[i] = trk[i];
   if (lane == 0) {
