@@ -128,6 +128,8 @@ struct cemc_model {
   }
 
   void build_gtab(const std::vector<double>& J);
+  int code_bits() const;    // bits per species in an energy-table code
+  int deco_slices() const;  // decoration slices per table block (>= n_species - 1)
   void refresh_view();
   ~cemc_model();
 };
@@ -158,23 +160,31 @@ cemc_model::~cemc_model() {
 // the energy form of the per-CF increment (ce_updater.cpp:393-404) times
 // N * J_k (ce_updater.cpp:236-242).  With the reference site at canonical
 // position r and decoration digit d there, the table stores
-//   G_c[r][code][d] = sum_{e: e_r = d} W_c[e] * prod_{q != r} phi[e_q](s_q)
-// with code = sum_q s_q * S^q over the other members in canonical order, so
-// a flip a->b of the reference site changes the energy by
+//   G_c[r][d][code] = sum_{e: e_r = d} W_c[e] * prod_{q != r} phi[e_q](s_q)
+// so a flip a->b of the reference site changes the energy by
 //   sum_d (phi_d(b) - phi_d(a)) * G_c[r][d][code].
+// Layout for the sweep kernel: code = s_0 | s_1 << k | s_2 << 2k over the
+// (up to three) other members in canonical order, k = code bits per species;
+// a block holds kBM decoration slices of 2^(3k) codes.  Entries of clusters
+// with fewer than three other members repeat over the unused species fields,
+// so the kernel never masks them.  Block 0 is all zeros (padding records).
+int cemc_model::code_bits() const { return n_species <= 2 ? 1 : (n_species <= 4 ? 2 : 3); }
+int cemc_model::deco_slices() const { return n_species <= 2 ? 1 : (n_species <= 4 ? 3 : 7); }
+
 void cemc_model::build_gtab(const std::vector<double>& J) {
   gtab.assign(gtab.size(), 0.0);
   const int S = n_species, B = n_bfs;
+  const int k = code_bits(), Pc = 1 << (3 * k), Bp = deco_slices();
   for (int c = 0; c < n_entries; ++c) {
     const int n = entry_size[c];
     const int g = entry_group[c];
     int n_deco = 1;
     for (int i = 0; i < n; ++i) n_deco *= B;
     std::vector<double> W(n_deco, 0.0);
-    for (int k = 0; k < n_cf; ++k) {
-      if (cf_kind[k] != CEMC_CF_CLUSTER || cf_entry[g * n_cf + k] != c) continue;
-      const int e0 = cf_eq_off[k], e1 = cf_eq_off[k + 1];
-      const double w = double(n_sites) * J[k] * double(n) /
+    for (int kk = 0; kk < n_cf; ++kk) {
+      if (cf_kind[kk] != CEMC_CF_CLUSTER || cf_entry[g * n_cf + kk] != c) continue;
+      const int e0 = cf_eq_off[kk], e1 = cf_eq_off[kk + 1];
+      const double w = double(n_sites) * J[kk] * double(n) /
                        (double(e1 - e0) * double(entry_norm[c]) * double(grp_count[g]));
       for (int e = e0; e < e1; ++e) {
         int idx = 0;
@@ -182,8 +192,12 @@ void cemc_model::build_gtab(const std::vector<double>& J) {
         W[idx] += w;
       }
     }
-    const int P = pow_s(n - 1);
     for (int r = 0; r < n; ++r) {
+      int pos_of[3];  // canonical position of member q
+      int q = 0;
+      for (int pos = 0; pos < n; ++pos)
+        if (pos != r) pos_of[q++] = pos;
+      const int nm = n - 1;
       for (int idx = 0; idx < n_deco; ++idx) {
         if (W[idx] == 0.0) continue;
         int deco[kMaxClusterSize];
@@ -192,18 +206,19 @@ void cemc_model::build_gtab(const std::vector<double>& J) {
           deco[pos] = t % B;
           t /= B;
         }
-        const int d = deco[r];
-        double* G = gtab.data() + entry_goff[c] + r * P * B + d;
-        for (int code = 0; code < P; ++code) {
+        double* G = gtab.data() + entry_goff[c] + (r * Bp + deco[r]) * Pc;
+        for (int code = 0; code < Pc; ++code) {
           double prod = W[idx];
-          int cc = code;
-          for (int pos = 0; pos < n; ++pos) {
-            if (pos == r) continue;
-            const int s = cc % S;
-            cc /= S;
-            prod *= basis[deco[pos] * S + s];
+          bool valid = true;
+          for (int qq = 0; qq < nm; ++qq) {
+            const int sp = (code >> (k * qq)) & ((1 << k) - 1);
+            if (sp >= S) {
+              valid = false;
+              break;
+            }
+            prod *= basis[deco[pos_of[qq]] * S + sp];
           }
-          G[code * B] += prod;
+          if (valid) G[code] += prod;
         }
       }
     }
@@ -329,18 +344,18 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
   m->entry_norm.assign(d->entry_norm, d->entry_norm + m->n_entries);
   m->entries.assign(m->n_entries, ClusterEntry{0, 0, 0, 0});
   m->entry_goff.assign(m->n_entries, 0);
-  int goff = 0;
+  const int block = m->deco_slices() << (3 * m->code_bits());
+  int goff = block;  // block 0: zeros read by padding records
   for (int c = 0; c < m->n_entries; ++c) {
     const int n = m->entry_size[c];
     require(n >= 2 && n <= kMaxClusterSize, "only cluster sizes 2, 3 and 4 are supported");
     require(m->entry_M[c] >= 1, "cluster without sub-clusters");
     require(m->entry_norm[c] >= m->entry_M[c], "bad cluster normalisation");
     m->entry_goff[c] = goff;
-    goff += n * B * m->pow_s(n - 1);
+    goff += n * block;
   }
-  require(goff < (1 << 24) - 8, "energy tables too large");
-  // padded so fixed-width reads of decorations past the last one stay inside
-  m->gtab.assign(goff + 8, 0.0);
+  require(goff < (1 << 24), "energy tables too large");
+  m->gtab.assign(goff, 0.0);
   m->grp_rec_off.assign(G, 0);
   m->grp_rec_cnt.assign(G, 0);
   for (int g = 0; g < G; ++g) {
@@ -372,7 +387,7 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
           w0 |= uint32_t(slot) << (10 * q);
           ++q;
         }
-        const uint32_t gbase = uint32_t(m->entry_goff[c] + r * B * m->pow_s(n - 1));
+        const uint32_t gbase = uint32_t(m->entry_goff[c] + r * block);
         const uint32_t w1 = gbase | (uint32_t(r) << 24) | (uint32_t(c) << 26);
         m->recs.push_back(SubRec{w0, w1});
         m->rec_slots.push_back(FatRec{int32_t(rec_slot(SubRec{w0, w1}, 0)), int32_t(rec_slot(SubRec{w0, w1}, 1)),
@@ -1038,17 +1053,14 @@ int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const c
         Key k;
         k.periodic = m->periodic;
         k.canonical = canon;
-        k.kbm = m->n_species <= 4 ? 3 : 7;
+        k.kbm = m->deco_slices();
         const bool single = m->n_groups == 1 && int(m->movable.size()) == m->n_sites;
         const int cnt = m->grp_rec_cnt[0];
         const int occ_bytes = m->periodic ? m->pdims[0] * m->pdims[1] * m->pdims[2] : m->n_sites;
         const int trk_sites = canon ? m->n_sites : 0;
         const uint32_t with_g = sweep_lds_bytes(occ_bytes, trk_sites, 0, int(m->gtab.size()));
         k.kr = 0;
-        if (single && k.kbm == 3 && with_g <= gtab_budget && cnt <= 512) {
-          k.kr = cnt <= 128 ? 2 : (cnt <= 256 ? 4 : 8);
-          if (m->n_species == 2) k.kbm = 1;
-        }
+        if (single && k.kbm <= 3 && with_g <= gtab_budget && cnt <= 512) k.kr = cnt <= 128 ? 2 : (cnt <= 256 ? 4 : 8);
         if (k.kr > 0) {
           k.glds = true;
           k.lds = with_g;

@@ -123,15 +123,22 @@ __device__ __attribute__((always_inline)) inline uint32_t wrapped_delta(const Sw
   return uint32_t(dx) | (uint32_t(dy) << 8) | (uint32_t(dz) << 16);
 }
 
+template <int kBM>
+struct TableGeom {
+  static constexpr int kBits = kBM == 1 ? 1 : (kBM == 3 ? 2 : 3);  // code bits per species
+  static constexpr int kCodes = 1 << (3 * kBits);                   // codes per decoration slice
+};
+
 // Phase 1 of a flip: addresses of every member of kR records, issue of all
-// member-species gathers, energy-table indices.  Members identified as the
-// patch site (periodic: wrapped offset == patch_w; table: site == patch_w)
-// read `patch_species`.
-template <bool kPeriodic, int kR>
+// member-species gathers, energy-table indices.  With kPatch, members that
+// are the patch site (periodic: wrapped offset == patch_w; table: site ==
+// patch_w) read `patch_species`.
+template <bool kPeriodic, int kR, int kBM, bool kPatch>
 __device__ __attribute__((always_inline)) inline void table_index(const SweepCtx& c, const FatRec (&rec)[kR],
-                                                                  const Ref& ref, const int8_t* occ,
-                                                                  uint32_t patch_w, int patch_species,
+                                                                  const Ref& ref, const uint8_t* occ,
+                                                                  uint32_t patch_w, uint32_t patch_species,
                                                                   int (&gi)[kR]) {
+  constexpr int kBits = TableGeom<kBM>::kBits;
   int nb[kR][3];
 #pragma unroll
   for (int k = 0; k < kR; ++k) {
@@ -146,70 +153,74 @@ __device__ __attribute__((always_inline)) inline void table_index(const SweepCtx
       nb[k][2] = row[rec[k].m2];
     }
   }
-  int sp[kR][3];
+  uint32_t sp[kR][3];
 #pragma unroll
   for (int k = 0; k < kR; ++k)
 #pragma unroll
     for (int q = 0; q < 3; ++q) sp[k][q] = occ[nb[k][q]];
 #pragma unroll
   for (int k = 0; k < kR; ++k) {
-    int s0 = sp[k][0], s1 = sp[k][1], s2 = sp[k][2];
-    if constexpr (kPeriodic) {
-      s0 = rec[k].w0 == patch_w ? patch_species : s0;
-      s1 = rec[k].w1 == patch_w ? patch_species : s1;
-      s2 = rec[k].w2 == patch_w ? patch_species : s2;
-    } else {
-      s0 = uint32_t(nb[k][0]) == patch_w ? patch_species : s0;
-      s1 = uint32_t(nb[k][1]) == patch_w ? patch_species : s1;
-      s2 = uint32_t(nb[k][2]) == patch_w ? patch_species : s2;
+    uint32_t s0 = sp[k][0], s1 = sp[k][1], s2 = sp[k][2];
+    if constexpr (kPatch) {
+      if constexpr (kPeriodic) {
+        s0 = rec[k].w0 == patch_w ? patch_species : s0;
+        s1 = rec[k].w1 == patch_w ? patch_species : s1;
+        s2 = rec[k].w2 == patch_w ? patch_species : s2;
+      } else {
+        s0 = uint32_t(nb[k][0]) == patch_w ? patch_species : s0;
+        s1 = uint32_t(nb[k][1]) == patch_w ? patch_species : s1;
+        s2 = uint32_t(nb[k][2]) == patch_w ? patch_species : s2;
+      }
     }
-    // species of unused members masked to zero: code = sum_q s_q S^q over the used ones
-    s1 &= (rec[k].meta & kRecUnused1) ? 0 : -1;
-    s2 &= (rec[k].meta & kRecUnused2) ? 0 : -1;
-    const uint32_t code = uint32_t(s0) + __umul24(uint32_t(c.S), uint32_t(s1) + __umul24(uint32_t(c.S), uint32_t(s2)));
-    gi[k] = int(rec[k].meta & 0xFFFFFFu) + int(__umul24(code, uint32_t(c.B)));
+    // unused members need no masking: their table entries repeat over them
+    const uint32_t code = s0 | (s1 << kBits) | (s2 << (2 * kBits));
+    gi[k] = int(rec[k].meta & 0xFFFFFFu) + int(code);
   }
 }
 
-// Phase 2: energy-table reads and the change for basis differences dphi;
-// records flagged invalid contribute zero.
+// Phase 2: energy-table reads.  kBM == 1 (two species): the plain sum of the
+// entries, to be scaled by the single basis difference after the reduction;
+// otherwise the change for basis differences dphi.
 template <int kR, int kBM>
-__device__ __attribute__((always_inline)) inline double table_sum(const FatRec (&rec)[kR], const int (&gi)[kR],
-                                                                  const double* gtab, const double (&dphi)[kBM]) {
+__device__ __attribute__((always_inline)) inline double table_sum(const int (&gi)[kR], const double* gtab,
+                                                                  const double (&dphi)[kBM]) {
+  constexpr int kCodes = TableGeom<kBM>::kCodes;
   double g[kR][kBM];
 #pragma unroll
   for (int k = 0; k < kR; ++k)
 #pragma unroll
-    for (int d = 0; d < kBM; ++d) g[k][d] = gtab[gi[k] + d];
+    for (int d = 0; d < kBM; ++d) g[k][d] = gtab[gi[k] + d * kCodes];
   double v = 0.0;
 #pragma unroll
   for (int k = 0; k < kR; ++k) {
-    double e = 0.0;
+    if constexpr (kBM == 1) {
+      v += g[k][0];
+    } else {
 #pragma unroll
-    for (int d = 0; d < kBM; ++d) e += dphi[d] * g[k][d];
-    v += (rec[k].meta & kRecValid) ? e : 0.0;
+      for (int d = 0; d < kBM; ++d) v += dphi[d] * g[k][d];
+    }
   }
   return v;
 }
 
 // Records of one translation group read from LDS in batches of 4 per lane.
-template <bool kPeriodic, int kBM>
+template <bool kPeriodic, int kBM, bool kPatch>
 __device__ __attribute__((always_inline)) inline double lds_flip_de(const SweepCtx& c, const FatRec* recs, int off,
                                                                     int cnt, int lane, const Ref& ref,
-                                                                    const int8_t* occ, const double* gtab,
+                                                                    const uint8_t* occ, const double* gtab,
                                                                     const double (&dphi)[kBM], uint32_t patch_w,
-                                                                    int patch_species) {
+                                                                    uint32_t patch_species) {
   double v = 0.0;
   for (int base = 0; base < cnt; base += 256) {
     FatRec rb[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int it = base + lane + 64 * k;
-      rb[k] = it < cnt ? recs[off + it] : FatRec{0, 0, 0, 0u, 0u, 0u, kRecUnused1 | kRecUnused2, 0u};
+      rb[k] = it < cnt ? recs[off + it] : FatRec{0, 0, 0, 0u, 0u, 0u, 0u, 0u};
     }
     int gi[4];
-    table_index<kPeriodic, 4>(c, rb, ref, occ, patch_w, patch_species, gi);
-    v += table_sum<4, kBM>(rb, gi, gtab, dphi);
+    table_index<kPeriodic, 4, kBM, kPatch>(c, rb, ref, occ, patch_w, patch_species, gi);
+    v += table_sum<4, kBM>(gi, gtab, dphi);
   }
   return v;
 }
@@ -241,7 +252,8 @@ __device__ __attribute__((always_inline)) inline int image_index(const SweepCtx&
 }
 
 // kR > 0: single-group model, records in VGPRs (kR per lane); kR == 0:
-// records in LDS.  kBM: decorations read per table lookup (>= n_species - 1).
+// records in LDS.  kBM: decoration slices of the energy tables (1: two
+// species, 3: up to four, 7: up to eight).
 template <bool kPeriodic, bool kCanonical, int kR, bool kGtabLds, int kBM>
 __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict__ params,
                                                    ChainState* __restrict__ states,
@@ -263,7 +275,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
   const int N = pp->n_sites;
   const int occ_bytes = kPeriodic ? m.pdims[0] * m.pdims[1] * m.pdims[2] : N;
   const LdsLayout L = lds_layout(occ_bytes, kCanonical ? N : 0, kRecLds ? m.n_rec : 0, kGtabLds ? m.gtab_size : 0);
-  int8_t* occ = reinterpret_cast<int8_t*>(lds + L.occ);
+  uint8_t* occ = reinterpret_cast<uint8_t*>(lds + L.occ);
   uint16_t* trk = reinterpret_cast<uint16_t*>(lds + L.trk);
   FatRec* recs_l = reinterpret_cast<FatRec*>(lds + L.rec);
   double* gtab_l = reinterpret_cast<double*>(lds + L.gtab);
@@ -288,7 +300,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
   const int n_pick = has_bkg ? m.n_movable : N;
 
   // ---- stage chain state, per-chain tables and model tables into LDS
-  const int8_t* gocc = occ_pool + pp->occ_off;
+  const uint8_t* gocc = reinterpret_cast<const uint8_t*>(occ_pool + pp->occ_off);
   if constexpr (kPeriodic) {
     const uint32_t p23 = uint32_t(c.P2 * c.P3), mag_p23 = 0xFFFFFFFFu / p23, mag_p3 = 0xFFFFFFFFu / uint32_t(c.P3);
     for (int p = lane; p < occ_bytes; p += 64) {
@@ -335,9 +347,12 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
 #pragma unroll
     for (int k = 0; k < kR; ++k) {
       const int it = lane + 64 * k;
-      rr[k] = it < cnt ? m.frecs[it] : FatRec{0, 0, 0, 0u, 0u, 0u, kRecUnused1 | kRecUnused2, 0u};
+      rr[k] = it < cnt ? m.frecs[it] : FatRec{0, 0, 0, 0u, 0u, 0u, 0u, 0u};
     }
   }
+  // two species: the basis difference and singlet change of 0->1 and 1->0
+  const double dphi01 = dphi_tab[1], dphi10 = dphi_tab[c.S];
+  const double dsin01 = dsingle_tab[1], dsin10 = dsingle_tab[c.S];
 
   double energy = states[chain].energy;
   const uint64_t t0 = states[chain].moves_done;
@@ -351,133 +366,143 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
   const uint32_t present_pk = pp->present_pk;
   const uint64_t cnt_lo = pp->count_pk[0], cnt_hi = pp->count_pk[1];
   const uint64_t seg_lo = pp->segoff_pk[0], seg_hi = pp->segoff_pk[1];
-
   uint32_t* acc_log = accept_log ? accept_log + int64_t(chain) * log_words : nullptr;
-  uint32_t log_word = 0;
-  Philox4 blk{0, 0, 0, 0};
-  double log_u = 0.0;
-  for (int64_t t = 0; t < nmov; ++t) {
-    const int l = int(t & 63);
-    // lane l holds the Philox block (and log u) of move t - (t mod 64) + l
-    if (l == 0) {
-      blk = philox4x64_10(t0 + uint64_t(t) + uint64_t(lane), 0, 0, 0, seed, uid);
-      log_u = log(unit_double(blk.w3));
-    }
-    const uint64_t w0 = readlane_u64(blk.w0, l);
-    const uint64_t w1 = readlane_u64(blk.w1, l);
-    const double lu = readlane_f64(log_u, l);
 
-    int si, sj = 0, old_i = 0, new_i = 0, sa = 0, sb = 0, off_a = 0, off_b = 0;
-    if constexpr (kCanonical) {
-      const uint64_t w2 = readlane_u64(blk.w2, l);
-      const uint32_t ia = bounded32(uint32_t(w0 >> 32), np);
-      const uint32_t ib = (ia + 1u + bounded32(uint32_t(w0), np - 1u)) % np;
-      old_i = int((present_pk >> (4u * ia)) & 15u);
-      new_i = int((present_pk >> (4u * ib)) & 15u);
-      sa = int(bounded64(w1, uint64_t(unpack16(cnt_lo, cnt_hi, ia))));
-      sb = int(bounded64(w2, uint64_t(unpack16(cnt_lo, cnt_hi, ib))));
-      off_a = unpack16(seg_lo, seg_hi, ia);
-      off_b = unpack16(seg_lo, seg_hi, ib);
-      si = trk[off_a + sa];
-      sj = trk[off_b + sb];
-    } else {
-      const uint64_t pick = bounded64(w1, uint64_t(n_pick));
-      si = has_bkg ? m.movable[pick] : int(pick);
-    }
-    const Ref ri = ref_of<kPeriodic>(c, si);
-    Ref rj = ri;
-    uint32_t patch_j = 0xFFFFFFFFu;
-    if constexpr (kCanonical) {
-      rj = ref_of<kPeriodic>(c, sj);
-      patch_j = kPeriodic ? wrapped_delta(c, ri, rj) : uint32_t(si);
-    }
-    int gi_i[kR > 0 ? kR : 1], gi_j[kR > 0 ? kR : 1];
-    if constexpr (!kCanonical) old_i = occ[ri.p];
-    if constexpr (kR > 0) {
-      // all member gathers of the move are issued together
-      table_index<kPeriodic, kR>(c, rr, ri, occ, 0xFFFFFFFFu, 0, gi_i);
-      if constexpr (kCanonical) table_index<kPeriodic, kR>(c, rr, rj, occ, patch_j, new_i, gi_j);
-    }
-    if constexpr (!kCanonical)
-      new_i = int((uint32_t(old_i) + 1u + bounded32(uint32_t(w0), uint32_t(c.S - 1))) % uint32_t(c.S));
-    const int pair_i = old_i * c.S + new_i;
-    double dphi[kBM];
+  for (int64_t blk0 = 0; blk0 < nmov; blk0 += 64) {
+    // lane l holds the Philox block and log u of move blk0 + l
+    const Philox4 blk = philox4x64_10(t0 + uint64_t(blk0) + uint64_t(lane), 0, 0, 0, seed, uid);
+    const double log_u = log(unit_double(blk.w3));
+    const int n_in = int(nmov - blk0 < 64 ? nmov - blk0 : 64);
+    uint64_t acc_bits = 0;
+    for (int l = 0; l < n_in; ++l) {
+      const uint64_t w0 = readlane_u64(blk.w0, l);
+      const uint64_t w1 = readlane_u64(blk.w1, l);
+      const double lu = readlane_f64(log_u, l);
+
+      int si, sj = 0, sa = 0, sb = 0, off_a = 0, off_b = 0;
+      uint32_t old_i = 0, new_i = 0;
+      if constexpr (kCanonical) {
+        const uint64_t w2 = readlane_u64(blk.w2, l);
+        const uint32_t ia = bounded32(uint32_t(w0 >> 32), np);
+        uint32_t ib = ia + 1u + bounded32(uint32_t(w0), np - 1u);
+        ib -= ib >= np ? np : 0u;
+        old_i = (present_pk >> (4u * ia)) & 15u;
+        new_i = (present_pk >> (4u * ib)) & 15u;
+        sa = int(bounded64(w1, uint64_t(unpack16(cnt_lo, cnt_hi, ia))));
+        sb = int(bounded64(w2, uint64_t(unpack16(cnt_lo, cnt_hi, ib))));
+        off_a = unpack16(seg_lo, seg_hi, ia);
+        off_b = unpack16(seg_lo, seg_hi, ib);
+        si = __builtin_amdgcn_readfirstlane(trk[off_a + sa]);
+        sj = __builtin_amdgcn_readfirstlane(trk[off_b + sb]);
+      } else {
+        const int pick = int(bounded64(w1, uint64_t(n_pick)));
+        si = has_bkg ? __builtin_amdgcn_readfirstlane(m.movable[pick]) : pick;
+      }
+      const Ref ri = ref_of<kPeriodic>(c, si);
+      Ref rj = ri;
+      uint32_t patch_j = 0u;
+      if constexpr (kCanonical) {
+        rj = ref_of<kPeriodic>(c, sj);
+        patch_j = kPeriodic ? wrapped_delta(c, ri, rj) : uint32_t(si);
+      }
+      int gi_i[kR > 0 ? kR : 1], gi_j[kR > 0 ? kR : 1];
+      if constexpr (!kCanonical) old_i = occ[ri.p];
+      if constexpr (kR > 0) {
+        // all member gathers of the move are issued together
+        table_index<kPeriodic, kR, kBM, false>(c, rr, ri, occ, 0u, 0u, gi_i);
+        if constexpr (kCanonical) table_index<kPeriodic, kR, kBM, true>(c, rr, rj, occ, patch_j, new_i, gi_j);
+      }
+      if constexpr (!kCanonical) {
+        new_i = old_i + 1u + bounded32(uint32_t(w0), uint32_t(c.S - 1));
+        new_i -= new_i >= uint32_t(c.S) ? uint32_t(c.S) : 0u;
+      }
+      double dphi[kBM];
+      double de_single = 0.0;  // a swap's two singlet changes cancel exactly
+      if constexpr (kBM == 1) {
+        dphi[0] = old_i == 0 ? dphi01 : dphi10;
+        if constexpr (!kCanonical) de_single = old_i == 0 ? dsin01 : dsin10;
+      } else {
+        const int pair_i = int(old_i) * c.S + int(new_i);
 #pragma unroll
-    for (int d = 0; d < kBM; ++d) dphi[d] = d < c.B ? dphi_tab[pair_i * c.B + d] : 0.0;
-    double de_single = dsingle_tab[pair_i];
-    double part;
-    if constexpr (kR > 0) {
-      part = table_sum<kR, kBM>(rr, gi_i, gtab, dphi);
-    } else {
-      const int gi = multi_group ? int(m.site_group[si]) : 0;
-      part = lds_flip_de<kPeriodic, kBM>(c, recs_l, grp_tab[gi], grp_tab[kMaxGroups + gi], lane, ri, occ, gtab,
-                                         dphi, 0xFFFFFFFFu, 0);
-    }
-    if constexpr (kCanonical) {
-      // second flip of the swap: j goes b -> a and sees site i already at b
-      // (ce_updater.cpp:845-852 runs update_cf twice in sequence)
+        for (int d = 0; d < kBM; ++d) dphi[d] = d < c.B ? dphi_tab[pair_i * c.B + d] : 0.0;
+        if constexpr (!kCanonical) de_single = dsingle_tab[pair_i];
+      }
       double dphj[kBM];
 #pragma unroll
       for (int d = 0; d < kBM; ++d) dphj[d] = -dphi[d];
-      de_single += dsingle_tab[new_i * c.S + old_i];
+      double part;
       if constexpr (kR > 0) {
-        part += table_sum<kR, kBM>(rr, gi_j, gtab, dphj);
+        part = table_sum<kR, kBM>(gi_i, gtab, dphi);
+        if constexpr (kCanonical) {
+          // second flip of the swap: j goes b -> a and sees site i already at b
+          // (ce_updater.cpp:845-852 runs update_cf twice in sequence)
+          const double pj = table_sum<kR, kBM>(gi_j, gtab, dphj);
+          part = kBM == 1 ? part - pj : part + pj;
+        }
       } else {
-        const int gj = multi_group ? int(m.site_group[sj]) : 0;
-        part += lds_flip_de<kPeriodic, kBM>(c, recs_l, grp_tab[gj], grp_tab[kMaxGroups + gj], lane, rj, occ,
-                                            gtab, dphj, patch_j, new_i);
-      }
-    }
-    const double dE = wave_sum_f64(part) + de_single;
-    // Metropolis test of the reference: accept if E_new < E_cur, else if
-    // u <= exp(-(E_new - E_cur) / kT) (cemc/mcmc/montecarlo.py:950-956).  The
-    // comparison is first made against log(u) with a margin far above the
-    // rounding of log/exp; only moves inside the margin evaluate exp.
-    bool accept;
-    if (dE < 0.0) {
-      accept = true;
-    } else {
-      const double x = -dE * inv_kT;
-      const double margin = 1e-12 * (1.0 + fabs(x));
-      if (lu < x - margin)
-        accept = true;
-      else if (lu > x + margin)
-        accept = false;
-      else
-        accept = unit_double(readlane_u64(blk.w3, l)) <= exp(-dE / kT);
-    }
-    if (accept) {
-      if constexpr (kPeriodic) {
-        // lanes 0-7 write the images of site i, lanes 8-15 those of site j
-        const bool second = kCanonical && lane >= 8 && lane < 16;
-        const Ref& rs = second ? rj : ri;
-        const int idx = lane < (kCanonical ? 16 : 8) ? image_index(c, rs.x, rs.y, rs.z, lane & 7) : -1;
-        if (idx >= 0) occ[idx] = int8_t(second ? old_i : new_i);
-      } else {
-        if (lane == 0) occ[si] = int8_t(new_i);
-        if (kCanonical && lane == 1) occ[sj] = int8_t(old_i);
-      }
-      if constexpr (kCanonical) {
-        if (lane == 0) {
-          trk[off_a + sa] = uint16_t(sj);
-          trk[off_b + sb] = uint16_t(si);
+        const int gi = multi_group ? int(m.site_group[si]) : 0;
+        part = lds_flip_de<kPeriodic, kBM, false>(c, recs_l, grp_tab[gi], grp_tab[kMaxGroups + gi], lane, ri, occ,
+                                                  gtab, dphi, 0u, 0u);
+        if constexpr (kCanonical) {
+          const int gj = multi_group ? int(m.site_group[sj]) : 0;
+          const double pj = lds_flip_de<kPeriodic, kBM, true>(c, recs_l, grp_tab[gj], grp_tab[kMaxGroups + gj], lane,
+                                                              rj, occ, gtab, dphj, patch_j, new_i);
+          part = kBM == 1 ? part - pj : part + pj;
         }
       }
-      energy += dE;
-      ++n_acc;
-      log_word |= 1u << (t & 31);
+      const double sum = wave_sum_f64(part);
+      const double dE = (kBM == 1 ? dphi[0] * sum : sum) + de_single;
+      // Metropolis test of the reference: accept if E_new < E_cur, else if
+      // u <= exp(-(E_new - E_cur) / kT) (cemc/mcmc/montecarlo.py:950-956).  The
+      // comparison is first made against log(u) with a margin far above the
+      // rounding of log/exp; only moves inside the margin evaluate exp.
+      bool accept;
+      if (dE < 0.0) {
+        accept = true;
+      } else {
+        const double x = -dE * inv_kT;
+        const double margin = 1e-12 * (1.0 + fabs(x));
+        if (lu < x - margin)
+          accept = true;
+        else if (lu > x + margin)
+          accept = false;
+        else
+          accept = unit_double(readlane_u64(blk.w3, l)) <= exp(-dE / kT);
+      }
+      if (accept) {
+        if constexpr (kPeriodic) {
+          // lanes 0-7 write the images of site i, lanes 8-15 those of site j
+          const bool second = kCanonical && lane >= 8 && lane < 16;
+          const Ref& rs = second ? rj : ri;
+          const int idx = lane < (kCanonical ? 16 : 8) ? image_index(c, rs.x, rs.y, rs.z, lane & 7) : -1;
+          if (idx >= 0) occ[idx] = uint8_t(second ? old_i : new_i);
+        } else {
+          if (lane == 0) occ[si] = uint8_t(new_i);
+          if (kCanonical && lane == 1) occ[sj] = uint8_t(old_i);
+        }
+        if constexpr (kCanonical) {
+          if (lane == 0) {
+            trk[off_a + sa] = uint16_t(sj);
+            trk[off_b + sb] = uint16_t(si);
+          }
+        }
+        energy += dE;
+        acc_bits |= uint64_t(1) << l;
+      }
+      sum_e += energy;
+      sum_e2 += energy * energy;
+      __syncthreads();
     }
-    if (acc_log != nullptr && ((t & 31) == 31 || t + 1 == nmov)) {
-      if (lane == 0) acc_log[t >> 5] = log_word;
-      log_word = 0;
+    n_acc += uint64_t(__builtin_popcountll(acc_bits));
+    if (acc_log != nullptr && lane < 2) {
+      const int64_t w = blk0 >> 5;
+      if (lane == 0) acc_log[w] = uint32_t(acc_bits);
+      if (lane == 1 && n_in > 32) acc_log[w + 1] = uint32_t(acc_bits >> 32);
     }
-    sum_e += energy;
-    sum_e2 += energy * energy;
-    __syncthreads();
   }
 
   // ---- write back
-  int8_t* wocc = occ_pool + pp->occ_off;
+  uint8_t* wocc = reinterpret_cast<uint8_t*>(occ_pool + pp->occ_off);
   for (int i = lane; i < N; i += 64) wocc[i] = occ[ref_of<kPeriodic>(c, i).p];
   if constexpr (kCanonical)
     for (int i = lane; i < N; i += 64) gtrk[i] = trk[i];
@@ -713,25 +738,22 @@ __global__ void slot_energy_kernel(const ModelDev* __restrict__ model, const dou
 // ---------------------------------------------------------------------------
 // Launch wrappers.
 // Instantiated variants: (periodic, canonical) x {records in VGPRs: kR = 2, 4,
-// 8 with tables in LDS and up to 4 species} and {records in LDS: tables in LDS
-// or HBM, up to 4 or 8 species}.
+// 8 with tables in LDS, up to 4 species} and {records in LDS: tables in LDS
+// or HBM, up to 8 species}.
 template <bool P, bool C>
 struct SweepVariants {
+  template <int KBM>
+  static const void* pick(int kr, bool glds) {
+    if (kr == 2) return reinterpret_cast<const void*>(&sweep_kernel<P, C, 2, true, KBM>);
+    if (kr == 4) return reinterpret_cast<const void*>(&sweep_kernel<P, C, 4, true, KBM>);
+    if (kr == 8) return reinterpret_cast<const void*>(&sweep_kernel<P, C, 8, true, KBM>);
+    return glds ? reinterpret_cast<const void*>(&sweep_kernel<P, C, 0, true, KBM>)
+                : reinterpret_cast<const void*>(&sweep_kernel<P, C, 0, false, KBM>);
+  }
   static const void* fn(int kr, bool glds, int kbm) {
-    if (kbm == 1) {
-      if (kr == 2) return reinterpret_cast<const void*>(&sweep_kernel<P, C, 2, true, 1>);
-      if (kr == 4) return reinterpret_cast<const void*>(&sweep_kernel<P, C, 4, true, 1>);
-      return reinterpret_cast<const void*>(&sweep_kernel<P, C, 8, true, 1>);
-    }
-    if (kr == 2) return reinterpret_cast<const void*>(&sweep_kernel<P, C, 2, true, 3>);
-    if (kr == 4) return reinterpret_cast<const void*>(&sweep_kernel<P, C, 4, true, 3>);
-    if (kr == 8) return reinterpret_cast<const void*>(&sweep_kernel<P, C, 8, true, 3>);
-    if (glds) {
-      return kbm == 3 ? reinterpret_cast<const void*>(&sweep_kernel<P, C, 0, true, 3>)
-                      : reinterpret_cast<const void*>(&sweep_kernel<P, C, 0, true, 7>);
-    }
-    return kbm == 3 ? reinterpret_cast<const void*>(&sweep_kernel<P, C, 0, false, 3>)
-                    : reinterpret_cast<const void*>(&sweep_kernel<P, C, 0, false, 7>);
+    if (kbm == 1) return pick<1>(kr, glds);
+    if (kbm == 3) return pick<3>(kr, glds);
+    return reinterpret_cast<const void*>(glds ? &sweep_kernel<P, C, 0, true, 7> : &sweep_kernel<P, C, 0, false, 7>);
   }
 };
 
@@ -742,7 +764,7 @@ static const void* sweep_fn(bool periodic, bool canonical, int kr, bool glds, in
 
 bool sweep_variant_ok(int kr, bool glds, int kbm) {
   if (kr == 2 || kr == 4 || kr == 8) return glds && (kbm == 1 || kbm == 3);
-  return kr == 0 && (kbm == 3 || kbm == 7);
+  return kr == 0 && (kbm == 1 || kbm == 3 || kbm == 7);
 }
 
 hipError_t sweep_set_max_lds(uint32_t bytes) {
