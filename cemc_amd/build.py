@@ -16,7 +16,7 @@ ARCH = "gfx950"
 
 def hipcc_command(out=LIB_PATH):
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-    return [hipcc, "--offload-arch=%s" % ARCH, "-O3", "-fPIC", "-shared", "-std=c++17",
+    return [hipcc, "--offload-arch=%s" % ARCH, "-O3", "-fPIC", "-shared", "-std=c++17", "-fvisibility=hidden",
             # no packed FP32 instructions in any HIP code object
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]\

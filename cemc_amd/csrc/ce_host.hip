@@ -81,7 +81,8 @@ uint32_t lds_limit() {
   hipDeviceProp_t prop;
   HIP_CHECK(hipGetDeviceProperties(&prop, dev));
   size_t v = prop.sharedMemPerBlock;
-  return uint32_t(std::min<size_t>(v, 160u * 1024u));
+  // dynamic part: what remains after the sweep kernel's static tables
+  return uint32_t(std::min<size_t>(v, 160u * 1024u)) - sweep_static_lds() - 256u;
 }
 
 }  // namespace
@@ -1100,7 +1101,7 @@ int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const c
       b->d_order = upload(order);
       uint32_t need = 0;
       for (const auto& cl : b->classes) need = std::max(need, cl.lds);
-      if (need > 64u * 1024u) HIP_CHECK(sweep_set_max_lds(limit));
+      if (need + sweep_static_lds() > 64u * 1024u) HIP_CHECK(sweep_set_max_lds(limit + sweep_static_lds()));
       // initial energies from correlation functions computed on the GPU
       std::vector<int32_t> all(n_chains);
       std::iota(all.begin(), all.end(), 0);

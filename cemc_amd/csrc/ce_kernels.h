@@ -29,8 +29,11 @@ struct LaunchSweep {
 // lattices), tracker entries (canonical chains), records held in LDS, table doubles.
 uint32_t sweep_lds_bytes(int occ_bytes, int trk_sites, int n_rec_lds, int gtab_doubles);
 bool sweep_variant_ok(int kr, bool gtab_lds, int kbm);
-// Raises the dynamic-LDS limit of every sweep kernel instantiation.
+// Raises the dynamic-LDS limit of every sweep kernel instantiation to
+// `bytes` minus that kernel's static LDS.
 hipError_t sweep_set_max_lds(uint32_t bytes);
+// Static LDS of the sweep kernel (bound on what the dynamic part may use).
+uint32_t sweep_static_lds();
 
 hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream);
 hipError_t launch_cf(const ModelDev* models, const ChainParams* params, const int8_t* occ_pool,

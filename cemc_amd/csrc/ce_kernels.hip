@@ -767,16 +767,25 @@ bool sweep_variant_ok(int kr, bool glds, int kbm) {
   return kr == 0 && (kbm == 1 || kbm == 3 || kbm == 7);
 }
 
+uint32_t sweep_static_lds() {
+  return uint32_t(sizeof(double) * (kMaxSpecies * kMaxSpecies * kMaxBasis + kMaxSpecies * kMaxSpecies) +
+                  sizeof(int) * 2 * kMaxGroups);
+}
+
 hipError_t sweep_set_max_lds(uint32_t bytes) {
-  const int b = int(bytes);
   for (int p = 0; p < 2; ++p)
     for (int cn = 0; cn < 2; ++cn)
       for (int kr : {0, 2, 4, 8})
         for (int g = 0; g < 2; ++g)
           for (int kbm : {1, 3, 7}) {
             if (!sweep_variant_ok(kr, g != 0, kbm)) continue;
-            const hipError_t e = hipFuncSetAttribute(sweep_fn(p != 0, cn != 0, kr, g != 0, kbm),
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, b);
+            const void* fn = sweep_fn(p != 0, cn != 0, kr, g != 0, kbm);
+            hipFuncAttributes attr;
+            hipError_t e = hipFuncGetAttributes(&attr, fn);
+            if (e != hipSuccess) return e;
+            const int dyn = int(bytes) - int(attr.sharedSizeBytes);
+            if (dyn <= 0) return hipErrorInvalidValue;
+            e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             if (e != hipSuccess) return e;
           }
   return hipSuccess;

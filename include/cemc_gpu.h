@@ -15,6 +15,10 @@
 
 #include <stdint.h>
 
+/* Only the entry points below are exported; the library is built with hidden
+ * default visibility. */
+#define CEMC_API __attribute__((visibility("default")))
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -34,10 +38,10 @@ typedef struct cemc_updater cemc_updater;
 typedef struct cemc_batch cemc_batch;
 
 /* Message of the calling thread's last failed call. */
-const char* cemc_last_error(void);
+CEMC_API const char* cemc_last_error(void);
 
 /* Number of visible GPUs (0 when none). */
-int cemc_device_count(int* count);
+CEMC_API int cemc_device_count(int* count);
 
 /* ------------------------------------------------------------------ model
  * Lattice, clusters, basis and ECIs: the data CEUpdater::init reads from the
@@ -89,13 +93,13 @@ typedef struct cemc_model_desc {
 
 /* Builds device tables; replaces CEUpdater::init's parsing
  * (cpp/src/ce_updater.cpp:32-234, cpp/src/cluster.cpp:41-232). */
-int cemc_model_create(const cemc_model_desc* desc, cemc_model** out);
+CEMC_API int cemc_model_create(const cemc_model_desc* desc, cemc_model** out);
 /* ECIs of the batched path (energy tables rebuilt); CEUpdater::set_ecis
  * (cpp/src/ce_updater.cpp:615-629). */
-int cemc_model_set_ecis(cemc_model* model, const double* eci);
+CEMC_API int cemc_model_set_ecis(cemc_model* model, const double* eci);
 /* 1 when the closed-form periodic neighbour path is active, else 0. */
-int cemc_model_is_periodic(const cemc_model* model, int* periodic);
-int cemc_model_destroy(cemc_model* model);
+CEMC_API int cemc_model_is_periodic(const cemc_model* model, int* periodic);
+CEMC_API int cemc_model_destroy(cemc_model* model);
 
 /* --------------------------------------------------------------- updater
  * One lattice with the reference's history/undo semantics
@@ -105,29 +109,29 @@ int cemc_model_destroy(cemc_model* model);
 /* PyCEUpdater.__init__ / CEUpdater::init (ce_updater.cpp:32-234).
  * cf_init may be NULL: the correlation functions are then computed on the
  * GPU from the occupation. */
-int cemc_updater_create(cemc_model* model, const int8_t* occupation, const double* cf_init,
+CEMC_API int cemc_updater_create(cemc_model* model, const int8_t* occupation, const double* cf_init,
                         const double* eci, cemc_updater** out);
-int cemc_updater_destroy(cemc_updater* upd);
+CEMC_API int cemc_updater_destroy(cemc_updater* upd);
 /* PyCEUpdater.update_cf / CEUpdater::update_cf (ce_updater.cpp:287-406). */
-int cemc_updater_update_cf(cemc_updater* upd, int32_t site, int32_t old_id, int32_t new_id);
+CEMC_API int cemc_updater_update_cf(cemc_updater* upd, int32_t site, int32_t old_id, int32_t new_id);
 /* PyCEUpdater.calculate / CEUpdater::calculate (ce_updater.cpp:471-550, 831-852).
  * Returns the total energy after the changes. */
-int cemc_updater_calculate(cemc_updater* upd, int32_t n, const int32_t* site, const int32_t* old_id,
+CEMC_API int cemc_updater_calculate(cemc_updater* upd, int32_t n, const int32_t* site, const int32_t* old_id,
                            const int32_t* new_id, double* energy);
 /* PyCEUpdater.undo_changes / CEUpdater::undo_changes (ce_updater.cpp:408-448). */
-int cemc_updater_undo_changes(cemc_updater* upd);
+CEMC_API int cemc_updater_undo_changes(cemc_updater* upd);
 /* PyCEUpdater.clear_history / CEUpdater::clear_history (ce_updater.cpp:552-555). */
-int cemc_updater_clear_history(cemc_updater* upd);
+CEMC_API int cemc_updater_clear_history(cemc_updater* upd);
 /* PyCEUpdater.get_energy / CEUpdater::get_energy (ce_updater.cpp:236-242). */
-int cemc_updater_get_energy(cemc_updater* upd, double* energy);
+CEMC_API int cemc_updater_get_energy(cemc_updater* upd, double* energy);
 /* PyCEUpdater.get_cf / CEUpdater::get_cf (ce_updater.cpp:570-583); n_cf values. */
-int cemc_updater_get_cf(cemc_updater* upd, double* cf);
+CEMC_API int cemc_updater_get_cf(cemc_updater* upd, double* cf);
 /* PyCEUpdater.set_ecis / CEUpdater::set_ecis (ce_updater.cpp:615-629); n_cf values. */
-int cemc_updater_set_ecis(cemc_updater* upd, const double* eci);
+CEMC_API int cemc_updater_set_ecis(cemc_updater* upd, const double* eci);
 /* PyCEUpdater.get_symbols (ce_updater.hpp:111) as species ids; n_sites values. */
-int cemc_updater_get_occupation(cemc_updater* upd, int8_t* occupation);
+CEMC_API int cemc_updater_get_occupation(cemc_updater* upd, int8_t* occupation);
 /* Number of history entries (CFHistoryTracker::history_size, cf_history_tracker.hpp:48). */
-int cemc_updater_history_size(cemc_updater* upd, int32_t* size);
+CEMC_API int cemc_updater_history_size(cemc_updater* upd, int32_t* size);
 
 /* ----------------------------------------------------------------- batch
  * Many independent Markov chains resident on the GPU, advanced by one
@@ -147,32 +151,32 @@ typedef struct cemc_chain_desc {
   const int8_t* occupation; /* n_sites species ids                              */
 } cemc_chain_desc;
 
-int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const cemc_chain_desc* chains,
+CEMC_API int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const cemc_chain_desc* chains,
                       cemc_batch** out);
-int cemc_batch_destroy(cemc_batch* batch);
+CEMC_API int cemc_batch_destroy(cemc_batch* batch);
 /* Advance chain c by n_moves[c] trial moves (n_moves NULL: `moves` each).
  * Montecarlo.runMC main loop (montecarlo.py:732-844) without equilibration. */
-int cemc_batch_run(cemc_batch* batch, const int64_t* n_moves, int64_t moves);
+CEMC_API int cemc_batch_run(cemc_batch* batch, const int64_t* n_moves, int64_t moves);
 /* Same as cemc_batch_run, also returning the launch's device time in ms
  * measured with HIP events on the batch's stream. */
-int cemc_batch_run_timed(cemc_batch* batch, const int64_t* n_moves, int64_t moves, double* kernel_ms);
+CEMC_API int cemc_batch_run_timed(cemc_batch* batch, const int64_t* n_moves, int64_t moves, double* kernel_ms);
 /* Same as cemc_batch_run, also recording accept (1) / reject (0) of every
  * move: bit t of chain c at accept_bits[c * words_per_chain + t / 32], bit
  * t % 32.  Used to compare acceptance sequences with the reference. */
-int cemc_batch_run_logged(cemc_batch* batch, const int64_t* n_moves, int64_t moves, uint32_t* accept_bits,
+CEMC_API int cemc_batch_run_logged(cemc_batch* batch, const int64_t* n_moves, int64_t moves, uint32_t* accept_bits,
                           int64_t words_per_chain);
 /* Current energies (n_chains). */
-int cemc_batch_get_energy(cemc_batch* batch, double* energy);
+CEMC_API int cemc_batch_get_energy(cemc_batch* batch, double* energy);
 /* Occupation of one chain (n_sites of that chain). */
-int cemc_batch_get_occupation(cemc_batch* batch, int32_t chain, int8_t* occupation);
+CEMC_API int cemc_batch_get_occupation(cemc_batch* batch, int32_t chain, int8_t* occupation);
 /* Per chain: moves done, accepted moves, sum of E, sum of E^2 over moves. */
-int cemc_batch_get_stats(cemc_batch* batch, int64_t* moves_done, int64_t* n_accept, double* sum_e,
+CEMC_API int cemc_batch_get_stats(cemc_batch* batch, int64_t* moves_done, int64_t* n_accept, double* sum_e,
                          double* sum_e2);
 /* Correlation functions of chain c computed from scratch on the GPU (n_cf of
  * that chain's model) and the energy they give. */
-int cemc_batch_get_cf(cemc_batch* batch, int32_t chain, double* cf, double* energy);
+CEMC_API int cemc_batch_get_cf(cemc_batch* batch, int32_t chain, double* cf, double* energy);
 /* Temperature of one chain (parallel tempering uses it to exchange replicas). */
-int cemc_batch_set_temperature(cemc_batch* batch, int32_t chain, double temperature);
+CEMC_API int cemc_batch_set_temperature(cemc_batch* batch, int32_t chain, double temperature);
 
 #ifdef __cplusplus
 }
