@@ -26,7 +26,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "aggregate MC trial moves/sec at 1 MI355X (batched vs plain); p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
-LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
+# HBM bytes of one SGC launch of the default workload (1024 chains x 500
+# sweeps): FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
+# counter reads for a copy kernel of known size (x2.0 and x1.0).  Measured
+# with rocprofv3 --pmc; summary in profiles/sweep_kernel_sgc_binary_fcc10_1024chains.txt
+TRAFFIC_DEFAULT_LAUNCH = 2491371
 
 
 def log(*a):
@@ -86,6 +90,23 @@ def cpu_baseline(settings, eci, chains, target_s=12.0):
                       % (threads, per_thread, dt)}
 
 
+def copy_probe(mib):
+    """One copy kernel that reads and writes `mib` MiB each: the known traffic
+    against which the HBM byte counters are scaled."""
+    from cemc_amd import probe
+    ms, gbs = probe.hbm_copy(mib << 20, reps=1)
+    log("copy probe: %d MiB read + %d MiB written in %.3f ms (%.0f GB/s)" % (mib, mib, ms, gbs))
+
+
+def measured_peaks():
+    """HBM copy and LDS read rates measured on this GPU by the probe kernels."""
+    from cemc_amd import probe
+    _, hbm = probe.hbm_copy(1 << 30, reps=10)
+    _, lds = probe.lds_read()
+    log("measured peaks: HBM copy %.0f GB/s, LDS read %.0f GB/s" % (hbm, lds))
+    return hbm, lds
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +115,8 @@ def main():
     ap.add_argument("--sweeps", type=int, default=500)
     ap.add_argument("--chains", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one device copy of this size first (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: each chain lives on one device and this build runs "
@@ -105,6 +128,9 @@ def main():
     from cemc_amd.lattice import random_occupation
     _capi.require_gpu()
 
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+    hbm_meas, lds_meas = measured_peaks()
     settings, eci, model = binary_model()
     log("model: periodic=%s sub-clusters=%d gathers/flip=%d" % (model.periodic, model.n_subclusters,
                                                                 model.gathers_per_flip()))
@@ -112,7 +138,7 @@ def main():
     batch = ChainBatch([model] * len(chains), chains)
     moves_per_chain = args.sweeps * settings.n_sites
     for _ in range(args.warmup):
-        batch.run(max(moves_per_chain // 10, 1000))
+        batch.run(moves_per_chain)
     kernel_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -139,9 +165,11 @@ def main():
     bytes_per_flip = 5.0 * model.gathers_per_flip()
     launch_bytes = bytes_per_flip * moves_per_chain * len(chains)
     achieved = launch_bytes / (avg_kernel_ms * 1e-3) / 1e9
+    default_shape = len(chains) == 1024 and args.sweeps == 500
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "lds_peak": LDS_PEAK_GBS, "lds_frac": achieved / LDS_PEAK_GBS,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_DEFAULT_LAUNCH if default_shape else None,
+                "lds_peak_measured": lds_meas, "lds_frac": achieved / lds_meas,
+                "hbm_copy_measured": hbm_meas,
                 "bytes_per_move": bytes_per_flip, "kernel_ms": avg_kernel_ms}
     out = {
         "metric": METRIC, "value": value, "unit": "moves/s", "n_gpus": 1, "steps": args.steps,
