@@ -42,9 +42,11 @@ __host__ __device__ inline int rec_refpos(SubRec r) { return int((r.w1 >> 24) & 
 //          packed x | y<<8 | z<<16 (identifies the member as a site).
 //   meta   energy-table base (24 bits) | members << 24 (2 bits) |
 //          member 1 unused << 28 | member 2 unused << 29 | valid << 31
-// Unused members address the reference site (offset 0) and their species is
-// masked to zero.  Energy table of a cluster entry: G[r][code][d], index
-// base_r + code*B + d, with base_r in meta for the record's reference position.
+// Unused members address the reference site (offset 0); the energy table
+// repeats over their species field, so no masking is needed.  Energy table of
+// a cluster entry: index base_r + d * 2^(3k) + code, with base_r in meta for
+// the record's reference position, d the decoration slice and code the
+// species of the other members, k bits each.
 struct FatRec {
   int32_t m0, m1, m2;
   uint32_t w0, w1, w2;
