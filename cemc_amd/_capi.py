@@ -39,6 +39,16 @@ class ChainDesc(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int32), ("temperature", ctypes.c_double), ("chem_pot", _dp),
         ("seed", ctypes.c_uint64), ("uid", ctypes.c_uint64), ("occupation", _i8p),
+        ("first_move", ctypes.c_uint64),
+    ]
+
+
+class Thermo(ctypes.Structure):
+    _fields_ = [
+        ("n_samples", ctypes.c_int64), ("energy", ctypes.c_double), ("mean_energy", ctypes.c_double),
+        ("var_energy", ctypes.c_double), ("n_singlets", ctypes.c_int32),
+        ("singlet", ctypes.c_double * 7), ("var_singlet", ctypes.c_double * 7),
+        ("cov_singlet_energy", ctypes.c_double * 7),
     ]
 
 
@@ -61,6 +71,7 @@ SIGNATURES = [
     ("cemc_updater_set_ecis", ctypes.c_int, [_vp, _dp]),
     ("cemc_updater_get_occupation", ctypes.c_int, [_vp, _i8p]),
     ("cemc_updater_history_size", ctypes.c_int, [_vp, _i32p]),
+    ("cemc_updater_set_occupation", ctypes.c_int, [_vp, _i8p]),
     ("cemc_batch_create", ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(_vp), ctypes.POINTER(ChainDesc),
                                          ctypes.POINTER(_vp)]),
     ("cemc_batch_destroy", ctypes.c_int, [_vp]),
@@ -71,6 +82,8 @@ SIGNATURES = [
     ("cemc_batch_get_energy", ctypes.c_int, [_vp, _dp]),
     ("cemc_batch_get_occupation", ctypes.c_int, [_vp, ctypes.c_int32, _i8p]),
     ("cemc_batch_get_stats", ctypes.c_int, [_vp, _i64p, _i64p, _dp, _dp]),
+    ("cemc_batch_reset_stats", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("cemc_batch_get_thermo", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(Thermo)]),
     ("cemc_batch_get_cf", ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
     ("cemc_batch_set_temperature", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double]),
 ]

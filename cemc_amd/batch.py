@@ -40,7 +40,8 @@ def chem_pot_vector(model, chem_pot):
 class ChainBatch:
     def __init__(self, models, chains):
         """models: one CEModel per chain; chains: dicts with keys kind,
-        temperature, occupation, seed, uid and optionally chem_pot."""
+        temperature, occupation, seed, uid and optionally chem_pot and
+        first_move (Philox counter of the first move, default 0)."""
         lib = _capi.load()
         n = len(chains)
         if n == 0 or len(models) != n:
@@ -66,6 +67,7 @@ class ChainBatch:
             d.seed = int(ch.get("seed", 0)) & 0xFFFFFFFFFFFFFFFF
             d.uid = int(ch.get("uid", c)) & 0xFFFFFFFFFFFFFFFF
             d.occupation = op
+            d.first_move = int(ch.get("first_move", 0)) & 0xFFFFFFFFFFFFFFFF
             handles[c] = m.handle
             self.n_sites[c] = m.n_sites
             self.kinds[c] = d.kind
@@ -132,6 +134,23 @@ class ChainBatch:
                                                    acc.ctypes.data_as(_capi._i64p), se.ctypes.data_as(_capi._dp),
                                                    se2.ctypes.data_as(_capi._dp)))
         return {"moves": moves, "accepted": acc, "sum_e": se, "sum_e2": se2}
+
+    def reset_stats(self, chain=-1):
+        """Zero the sample sums of one chain (or all, chain=-1); the current
+        energy becomes their reference."""
+        _capi.check(self._lib.cemc_batch_reset_stats(self.handle, int(chain)))
+
+    def thermo(self, chain):
+        """Sample averages of one chain since its last reset: n_samples,
+        energy (current), mean_energy, var_energy, singlet, var_singlet,
+        cov_singlet_energy (arrays over decorations)."""
+        t = _capi.Thermo()
+        _capi.check(self._lib.cemc_batch_get_thermo(self.handle, int(chain), ctypes.byref(t)))
+        b = int(t.n_singlets)
+        return {"n_samples": int(t.n_samples), "energy": t.energy, "mean_energy": t.mean_energy,
+                "var_energy": t.var_energy, "singlet": np.array(t.singlet[:b]),
+                "var_singlet": np.array(t.var_singlet[:b]),
+                "cov_singlet_energy": np.array(t.cov_singlet_energy[:b])}
 
     def cf(self, chain):
         """Correlation functions of one chain, computed from scratch on the GPU,

@@ -1,0 +1,178 @@
+"""Cluster-expansion calculator on the GPU: the reference's ``CE`` class
+(cemc/ce_calculator.py:136-470) over ``cemc_amd.updater.PyCEUpdater``.
+
+Same constructor and methods; the correlation functions, the history and the
+occupation live on the GPU.  The reference derives from ASE's ``Calculator``;
+this class keeps the parts of that contract its callers use (``results``,
+``get_potential_energy``, ``atoms.set_calculator``) without importing ASE.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .updater import PyCEUpdater
+
+
+class SelfInteractionError(ValueError):
+    """A site appears twice in one cluster (cemc/ce_calculator.py:20, 152-157)."""
+
+
+def _has_self_interaction(cluster_info):
+    """True when a cluster lists its own reference site or one site twice
+    (the condition CE._has_self_interaction tests, ce_calculator.py:596-608)."""
+    for info in cluster_info:
+        for cl in info.values():
+            if int(cl.get("size", 0)) < 2:
+                continue
+            ref = int(cl["ref_indx"])
+            for sub in cl["indices"]:
+                sub = [int(v) for v in sub]
+                if ref in sub or len(set(sub)) != len(sub):
+                    return True
+    return False
+
+
+class CE:
+    """Energy of an atoms object from a cluster expansion.
+
+    :param atoms: object with ``len``, ``atoms[i].symbol`` (read and write)
+        and ``set_calculator``
+    :param BC: settings (the attribute contract of ``CEUpdater::init``,
+        cpp/src/ce_updater.cpp:32-234)
+    :param dict eci: effective cluster interactions by CF name
+    :param initial_cf: correlation functions by name; None computes them on
+        the GPU from the occupation (the reference uses CLEASE's
+        ``CorrFunction`` here, ce_calculator.py:164-172)
+    """
+
+    implemented_properties = ["energy"]
+
+    def __init__(self, atoms, BC, eci=None, initial_cf=None):
+        self.BC = BC
+        if _has_self_interaction(BC.cluster_info):
+            raise SelfInteractionError(
+                "The simulation cell is so small that the same site is present multiple times within one "
+                "cluster. Increase the size of the simulation cell.")
+        eci = dict(eci) if eci is not None else {}
+        # an ECI for the empty cluster is always present (ce_calculator.py:152-158)
+        if "c0" not in eci:
+            eci["c0"] = 0.0
+            if initial_cf is not None:
+                initial_cf = dict(initial_cf)
+                initial_cf["c0"] = 1.0
+        self.eci = eci
+        self.atoms = atoms
+        self._check_trans_mat_dimensions()
+        self.results = {}
+        self.updater = PyCEUpdater(atoms, BC, initial_cf, eci)
+        self.cf = self.updater.get_cf()
+        self.clear_history = self.updater.clear_history
+        self.undo_changes = self.updater.undo_changes
+        self.update_cf = self.updater.update_cf
+        if hasattr(atoms, "set_calculator"):
+            atoms.set_calculator(self)
+        self._linear_vib_correction = None
+
+    @property
+    def model(self):
+        """The device model (lattice, clusters, ECIs) behind this calculator."""
+        return self.updater.model
+
+    def _check_trans_mat_dimensions(self):
+        tm = self.BC.trans_matrix
+        n_sites = tm.shape[0] if isinstance(tm, np.ndarray) else len(tm)
+        if len(self.atoms) != n_sites:
+            raise ValueError("The number of atoms and the dimension of the translation matrix is inconsistent\n"
+                             "Num atoms: {}. Num row trans mat: {}".format(len(self.atoms), n_sites))
+
+    def copy(self):
+        """A new calculator on a copy of the atoms with the current CFs."""
+        atoms = self.atoms.copy()
+        return CE(atoms, self.BC, eci=dict(self.eci), initial_cf=self.get_cf())
+
+    def get_energy(self):
+        return self.updater.get_energy()
+
+    def get_potential_energy(self, atoms=None):
+        return self.get_energy()
+
+    def calculate(self, atoms, properties, system_changes):
+        """Energy after applying ``system_changes`` = [(index, old, new), ...]
+        (ce_calculator.py:345-364)."""
+        energy = self.updater.calculate(system_changes)
+        self.cf = self.updater.get_cf()
+        self.results["energy"] = energy
+        return energy
+
+    def get_cf(self):
+        return self.updater.get_cf()
+
+    def update_ecis(self, new_ecis):
+        """ECI update by name (ce_calculator.py:378-386); the device model's
+        energy tables follow, so samplers built on it see the new values."""
+        self.eci.update({k: float(v) for k, v in new_ecis.items()})
+        self.updater.set_ecis(self.eci)
+        self.model.set_ecis(self.eci)
+
+    def get_singlets(self):
+        return self.updater.get_singlets()
+
+    def set_composition(self, comp):
+        """Set concentrations, e.g. {"Mg": 0.2, "Al": 0.8}: the element with
+        the largest fraction everywhere, then the others on consecutive sites
+        (ce_calculator.py:387-423)."""
+        tot = 0.0
+        max_elem, max_conc = None, 0.0
+        for key, conc in comp.items():
+            tot += conc
+            if conc > max_conc:
+                max_elem, max_conc = key, conc
+        if abs(tot - 1.0) > 1e-6:
+            raise ValueError("The specified concentration does not sum to 1!")
+        symbols = [max_elem] * len(self.atoms)
+        start = 0
+        for elem, conc in comp.items():
+            if elem == max_elem:
+                continue
+            n_at = int(round(conc * len(self.atoms)))
+            for i in range(start, start + n_at):
+                symbols[i] = elem
+            start += n_at
+        self.set_symbols(symbols)
+
+    def set_symbols(self, symbs):
+        """Change every symbol at once (ce_calculator.py:436-448); CFs are
+        recomputed on the GPU and the history cleared."""
+        if len(symbs) != len(self.atoms):
+            raise ValueError("Length of the symbols array has to match the length of the atoms object.!")
+        self.updater.set_symbols(list(symbs))
+        self.cf = self.updater.get_cf()
+
+    def singlet2comp(self, singlets):
+        """Concentrations from singlet values (ce_calculator.py:450-470)."""
+        bfs = self.BC.basis_functions
+        if len(singlets) != len(bfs):
+            raise ValueError("The number singlet terms specified is different from the number of basis "
+                             "functions\nGiven singlet terms: {}\nBasis functions: {}\n".format(singlets, bfs))
+        species = sorted(self.BC.unique_elements)
+        n = len(species)
+        mat = np.zeros((n, n))
+        rhs = np.zeros(n)
+        names = sorted(singlets)
+        for row, name in enumerate(names):
+            d = int(name.rsplit("_", 1)[1])
+            for col, s in enumerate(species):
+                mat[row, col] = bfs[d][s]
+            rhs[row] = singlets[name]
+        mat[-1, :] = 1.0
+        rhs[-1] = 1.0
+        x = np.linalg.solve(mat, rhs)
+        # round-off below zero is clipped as the reference does (ce_calculator.py:486-494)
+        x[(x < 0.0) & (x > -1e-6)] = 0.0
+        if np.any(x < 0.0) or np.any(x > 1.0):
+            raise ValueError("Something strange happened when converting singlets to composition\n"
+                             "Concentrations: {}".format(x))
+        return {s: float(x[i]) for i, s in enumerate(species)}
+
+    def vib_energy(self, T):
+        return self.updater.vib_energy(T)

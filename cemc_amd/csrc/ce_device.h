@@ -124,14 +124,27 @@ struct ChainParams {
 };
 
 // Per-chain mutable state (kernel written).
+// Per-chain mutable state.  The sample sums run over the trial moves since
+// the last reset; each move samples the state after its accept/reject, as the
+// reference's observers do (cemc/mcmc/mc_observers.py:222-270).  Energies
+// enter relative to e_ref, the energy at that reset, which keeps the squared
+// sums well conditioned (the purpose of the reference's energy-bias probe,
+// cemc/mcmc/montecarlo.py:178-217).  Singlet sums are kept for SGC chains.
 struct ChainState {
   double energy;
-  uint64_t moves_done;
+  uint64_t moves_done;  // moves ever made: the Philox counter of the next move
   uint64_t n_accept;
-  double sum_e;
-  double sum_e2;
-  double pad[3];
+  uint64_t n_samples;   // moves since the last reset
+  double e_ref;
+  double sum_de;        // sum of (E - e_ref)
+  double sum_de2;       // sum of (E - e_ref)^2
+  double pad0;
+  double sum_s[kMaxBasis];   // sum of singlet d
+  double sum_s2[kMaxBasis];  // sum of singlet d squared
+  double sum_se[kMaxBasis];  // sum of singlet d * (E - e_ref)
+  double pad1[3];
 };
+static_assert(sizeof(ChainState) == 256, "ChainState layout");
 
 // ---------------------------------------------------------------------------
 // Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11).

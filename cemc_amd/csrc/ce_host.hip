@@ -832,6 +832,47 @@ int cemc_updater_destroy(cemc_updater* u) {
   return guarded([&] { delete u; });
 }
 
+int cemc_updater_set_occupation(cemc_updater* u, const int8_t* occupation) {
+  int rc = CEMC_OK;
+  const int g = guarded([&] {
+    require(u && occupation, "NULL argument");
+    cemc_model* model = u->model;
+    const int N = model->n_sites, ncf = model->n_cf;
+    for (int i = 0; i < N; ++i)
+      require(occupation[i] >= 0 && occupation[i] < model->n_species, "occupation species id out of range");
+    for (int i = 0; i < N; ++i)
+      if (model->background[i] && occupation[i] != u->occ[i]) {
+        g_last_error = "Attempting to move a background atom!";
+        rc = CEMC_ERUNTIME;
+        return;
+      }
+    HIP_CHECK(hipStreamSynchronize(u->stream));
+    u->occ.assign(occupation, occupation + N);
+    HIP_CHECK(hipMemcpy(u->d_occ, u->occ.data(), size_t(N), hipMemcpyHostToDevice));
+    ChainParams p;
+    std::memset(&p, 0, sizeof p);
+    p.model = 0;
+    p.n_sites = N;
+    for (int k = 0; k < ncf; ++k)
+      if (model->cf_kind[k] == CEMC_CF_SINGLET) p.j1[model->cf_dec[k]] = u->eci[k];
+    ChainParams* d_p = upload(std::vector<ChainParams>{p});
+    std::vector<double> cf;
+    try {
+      compute_cf(model->d_view, {p}, d_p, u->d_occ, {0}, ncf, &cf, nullptr, u->stream);
+    } catch (...) {
+      dev_free(d_p);
+      throw;
+    }
+    dev_free(d_p);
+    // history holds only the new state (cf_history_tracker.cpp:146-149)
+    HIP_CHECK(hipMemcpy(u->d_ring, cf.data(), sizeof(double) * ncf, hipMemcpyHostToDevice));
+    u->current = 1;
+    u->buffer_size = 1;
+    u->slot_epoch.assign(kMaxHistory, -1);
+  });
+  return g != CEMC_OK ? g : rc;
+}
+
 int cemc_updater_update_cf(cemc_updater* u, int32_t site, int32_t old_id, int32_t new_id) {
   int rc = CEMC_OK;
   const int g = guarded([&] {
@@ -1109,7 +1150,10 @@ int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const c
       compute_cf(b->d_models, b->params, b->d_params, b->d_occ, all, b->n_cf_max, nullptr, &energies, b->stream);
       std::vector<ChainState> st(n_chains);
       std::memset(st.data(), 0, sizeof(ChainState) * st.size());
-      for (int c = 0; c < n_chains; ++c) st[c].energy = energies[c];
+      for (int c = 0; c < n_chains; ++c) {
+        st[c].energy = st[c].e_ref = energies[c];
+        st[c].moves_done = chains[c].first_move;
+      }
       HIP_CHECK(hipMemcpy(b->d_states, st.data(), sizeof(ChainState) * st.size(), hipMemcpyHostToDevice));
     } catch (...) {
       delete b;
@@ -1170,10 +1214,66 @@ int cemc_batch_get_stats(cemc_batch* b, int64_t* moves_done, int64_t* n_accept, 
     std::vector<ChainState> st(b->n_chains);
     HIP_CHECK(hipMemcpy(st.data(), b->d_states, sizeof(ChainState) * st.size(), hipMemcpyDeviceToHost));
     for (int c = 0; c < b->n_chains; ++c) {
+      const double n = double(st[c].n_samples), r = st[c].e_ref;
       if (moves_done) moves_done[c] = int64_t(st[c].moves_done);
       if (n_accept) n_accept[c] = int64_t(st[c].n_accept);
-      if (sum_e) sum_e[c] = st[c].sum_e;
-      if (sum_e2) sum_e2[c] = st[c].sum_e2;
+      if (sum_e) sum_e[c] = n * r + st[c].sum_de;
+      if (sum_e2) sum_e2[c] = st[c].sum_de2 + 2.0 * r * st[c].sum_de + n * r * r;
+    }
+  });
+}
+
+int cemc_batch_reset_stats(cemc_batch* b, int32_t chain) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(chain >= -1 && chain < b->n_chains, "chain index out of range");
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    std::vector<ChainState> st(b->n_chains);
+    HIP_CHECK(hipMemcpy(st.data(), b->d_states, sizeof(ChainState) * st.size(), hipMemcpyDeviceToHost));
+    for (int c = 0; c < b->n_chains; ++c) {
+      if (chain >= 0 && c != chain) continue;
+      ChainState& x = st[c];
+      x.e_ref = x.energy;
+      x.n_samples = 0;
+      x.sum_de = x.sum_de2 = 0.0;
+      for (int d = 0; d < kMaxBasis; ++d) x.sum_s[d] = x.sum_s2[d] = x.sum_se[d] = 0.0;
+    }
+    HIP_CHECK(hipMemcpy(b->d_states, st.data(), sizeof(ChainState) * st.size(), hipMemcpyHostToDevice));
+  });
+}
+
+int cemc_batch_get_thermo(cemc_batch* b, int32_t chain, cemc_thermo* out) {
+  return guarded([&] {
+    require(b && out, "NULL argument");
+    require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    ChainState x;
+    HIP_CHECK(hipMemcpy(&x, b->d_states + chain, sizeof(ChainState), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(cemc_thermo));
+    const ChainParams& p = b->params[chain];
+    const cemc_model* m = b->models[p.model];
+    const int B = m->n_bfs, S = m->n_species;
+    out->n_samples = int64_t(x.n_samples);
+    out->n_singlets = B;
+    out->energy = x.energy;
+    if (x.n_samples == 0) return;
+    const double n = double(x.n_samples);
+    const double mde = x.sum_de / n;
+    out->mean_energy = x.e_ref + mde;
+    out->var_energy = x.sum_de2 / n - mde * mde;
+    if (p.kind == CEMC_MOVE_SGC) {
+      for (int d = 0; d < B; ++d) {
+        const double ms = x.sum_s[d] / n;
+        out->singlet[d] = ms;
+        out->var_singlet[d] = x.sum_s2[d] / n - ms * ms;
+        out->cov_singlet_energy[d] = x.sum_se[d] / n - ms * mde;
+      }
+    } else {
+      // swaps keep the composition: the singlets are constant
+      for (int d = 0; d < B; ++d) {
+        double v = 0.0;
+        for (int k = 0; k < p.n_present; ++k) v += double(p.count[p.present[k]]) * m->basis[d * S + p.present[k]];
+        out->singlet[d] = v / double(p.n_sites);
+      }
     }
   });
 }

@@ -266,6 +266,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ double dphi_tab[kMaxSpecies * kMaxSpecies * kMaxBasis];  // [(old*S+new)*B + d]
   __shared__ double dsingle_tab[kMaxSpecies * kMaxSpecies];          // singlet energy change per (old,new)
+  __shared__ double phi_tab[kMaxBasis * kMaxSpecies];                // phi[d*S + s]
   __shared__ int grp_tab[2 * kMaxGroups];
   constexpr int kRecLds = kR == 0;
   const int chain = chain_order[blockIdx.x];
@@ -328,6 +329,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
     for (int i = lane; i < m.gtab_size; i += 64) gtab_l[i] = m.gtab[i];
   const double* gtab = kGtabLds ? gtab_l : m.gtab;
   if (lane < 2 * kMaxGroups) grp_tab[lane] = lane < kMaxGroups ? m.grp_rec_off[lane] : m.grp_rec_cnt[lane - kMaxGroups];
+  for (int i = lane; i < c.B * c.S; i += 64) phi_tab[i] = m.basis[i];
   for (int i = lane; i < c.S * c.S; i += 64) {
     const int o = i / c.S, nw = i - o * c.S;
     double ds = 0.0;
@@ -356,9 +358,32 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
 
   double energy = states[chain].energy;
   const uint64_t t0 = states[chain].moves_done;
+  const double e_ref = states[chain].e_ref;
   const int64_t nmov = n_moves[chain];
   uint64_t n_acc = 0;
-  double sum_e = 0.0, sum_e2 = 0.0;
+  double sum_de = 0.0, sum_de2 = 0.0;
+  // SGC: lane d < B holds singlet d of the current state, sum over the
+  // non-background sites of phi_d / N (ce_updater.cpp:366-371 keeps it
+  // incrementally the same way)
+  const double inv_n = 1.0 / double(N);
+  double sing = 0.0, sum_s = 0.0, sum_s2 = 0.0, sum_se = 0.0;
+  if constexpr (!kCanonical) {
+    double part_s[kBM];
+#pragma unroll
+    for (int d = 0; d < kBM; ++d) part_s[d] = 0.0;
+    for (int i = lane; i < N; i += 64) {
+      if (has_bkg && m.background[i]) continue;
+      const int s = occ[ref_of<kPeriodic>(c, i).p];
+#pragma unroll
+      for (int d = 0; d < kBM; ++d)
+        if (d < c.B) part_s[d] += phi_tab[d * c.S + s];
+    }
+#pragma unroll
+    for (int d = 0; d < kBM; ++d) {
+      const double tot = wave_sum_f64(part_s[d]);
+      if (lane == d) sing = tot / double(N);
+    }
+  }
   const double kT = pp->kT;
   const double inv_kT = 1.0 / kT;
   const uint64_t seed = pp->seed, uid = pp->uid;
@@ -427,6 +452,14 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
         for (int d = 0; d < kBM; ++d) dphi[d] = d < c.B ? dphi_tab[pair_i * c.B + d] : 0.0;
         if constexpr (!kCanonical) de_single = dsingle_tab[pair_i];
       }
+      // SGC: change of lane d's singlet if the flip is accepted
+      double dsing = 0.0;
+      if constexpr (!kCanonical) {
+        if constexpr (kBM == 1)
+          dsing = dphi[0] * inv_n;
+        else
+          dsing = dphi_tab[(int(old_i) * c.S + int(new_i)) * c.B + (lane < c.B ? lane : 0)] * inv_n;
+      }
       double dphj[kBM];
 #pragma unroll
       for (int d = 0; d < kBM; ++d) dphj[d] = -dphi[d];
@@ -487,10 +520,17 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
           }
         }
         energy += dE;
+        if constexpr (!kCanonical) sing += dsing;
         acc_bits |= uint64_t(1) << l;
       }
-      sum_e += energy;
-      sum_e2 += energy * energy;
+      const double de_ref = energy - e_ref;
+      sum_de += de_ref;
+      sum_de2 = fma(de_ref, de_ref, sum_de2);
+      if constexpr (!kCanonical) {
+        sum_s += sing;
+        sum_s2 = fma(sing, sing, sum_s2);
+        sum_se = fma(sing, de_ref, sum_se);
+      }
       __syncthreads();
     }
     n_acc += uint64_t(__builtin_popcountll(acc_bits));
@@ -506,13 +546,19 @@ __global__ __launch_bounds__(64) void sweep_kernel(const ChainParams* __restrict
   for (int i = lane; i < N; i += 64) wocc[i] = occ[ref_of<kPeriodic>(c, i).p];
   if constexpr (kCanonical)
     for (int i = lane; i < N; i += 64) gtrk[i] = trk[i];
+  ChainState* st = states + chain;
   if (lane == 0) {
-    ChainState* st = states + chain;
     st->energy = energy;
     st->moves_done = t0 + uint64_t(nmov);
     st->n_accept += n_acc;
-    st->sum_e += sum_e;
-    st->sum_e2 += sum_e2;
+    st->n_samples += uint64_t(nmov);
+    st->sum_de += sum_de;
+    st->sum_de2 += sum_de2;
+  }
+  if (!kCanonical && lane < c.B) {
+    st->sum_s[lane] += sum_s;
+    st->sum_s2[lane] += sum_s2;
+    st->sum_se[lane] += sum_se;
   }
 }
 
@@ -768,7 +814,8 @@ bool sweep_variant_ok(int kr, bool glds, int kbm) {
 }
 
 uint32_t sweep_static_lds() {
-  return uint32_t(sizeof(double) * (kMaxSpecies * kMaxSpecies * kMaxBasis + kMaxSpecies * kMaxSpecies) +
+  return uint32_t(sizeof(double) * (kMaxSpecies * kMaxSpecies * kMaxBasis + kMaxSpecies * kMaxSpecies +
+                                    kMaxBasis * kMaxSpecies) +
                   sizeof(int) * 2 * kMaxGroups);
 }
 

@@ -42,13 +42,16 @@ class PyCEUpdater:
         self._num_threads = 1
         self._vib = None
         occ = self.model.species_ids(symbols)
-        # CFHistoryTracker::insert keeps only names that have an ECI and
-        # leaves the others at zero (cpp/src/cf_history_tracker.cpp:82-95)
-        cf0 = np.array([float(corr_func.get(n, 0.0)) for n in self.model.names])
         eci_arr = np.array([float(eci[n]) for n in self.model.names])
         o, op = _capi.i8(occ)
-        c, cp = _capi.f64(cf0)
         e, ep = _capi.f64(eci_arr)
+        cp = None
+        if corr_func is not None:
+            # CFHistoryTracker::insert keeps only names that have an ECI and
+            # leaves the others at zero (cpp/src/cf_history_tracker.cpp:82-95)
+            cf0 = np.array([float(corr_func.get(n, 0.0)) for n in self.model.names])
+            c, cp = _capi.f64(cf0)
+        # corr_func None: the correlation functions are computed on the GPU
         h = ctypes.c_void_p()
         _capi.check(lib.cemc_updater_create(self.model.handle, op, cp, ep, ctypes.byref(h)))
         self.handle = h
@@ -135,6 +138,26 @@ class PyCEUpdater:
         occ = np.zeros(self.model.n_sites, dtype=np.int8)
         _capi.check(self._lib.cemc_updater_get_occupation(self.handle, occ.ctypes.data_as(_capi._i8p)))
         return occ
+
+    def set_occupation(self, occupation):
+        """Replace the whole occupation (species ids); CFs are recomputed on
+        the GPU and the history is cleared."""
+        occ = np.ascontiguousarray(occupation, dtype=np.int8)
+        if occ.shape != (self.model.n_sites,):
+            raise ValueError("Length of the symbols array has to match the length of the atoms object.!")
+        rc = self._lib.cemc_updater_set_occupation(self.handle, occ.ctypes.data_as(_capi._i8p))
+        _capi.check(rc)
+        self._sync_atoms()
+
+    def set_symbols(self, symbols):
+        """CE.set_symbols (cemc/ce_calculator.py:436-448) in one device call."""
+        if len(symbols) != self.model.n_sites:
+            raise ValueError("Length of the symbols array has to match the length of the atoms object.!")
+        try:
+            occ = self.model.species_ids(symbols)
+        except KeyError as exc:
+            raise ValueError("unknown symbol %s" % exc)
+        self.set_occupation(occ)
 
     def history_size(self):
         v = ctypes.c_int32(0)

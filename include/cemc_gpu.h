@@ -132,6 +132,11 @@ CEMC_API int cemc_updater_set_ecis(cemc_updater* upd, const double* eci);
 CEMC_API int cemc_updater_get_occupation(cemc_updater* upd, int8_t* occupation);
 /* Number of history entries (CFHistoryTracker::history_size, cf_history_tracker.hpp:48). */
 CEMC_API int cemc_updater_history_size(cemc_updater* upd, int32_t* size);
+/* Replace the whole occupation, recompute the correlation functions on the
+ * GPU and clear the history: CE.set_symbols (cemc/ce_calculator.py:436-448),
+ * which the reference runs as one update_cf per site then clear_history.
+ * Changing a background site fails as update_cf does (ce_updater.cpp:330-332). */
+CEMC_API int cemc_updater_set_occupation(cemc_updater* upd, const int8_t* occupation);
 
 /* ----------------------------------------------------------------- batch
  * Many independent Markov chains resident on the GPU, advanced by one
@@ -149,6 +154,8 @@ typedef struct cemc_chain_desc {
   uint64_t seed;
   uint64_t uid;
   const int8_t* occupation; /* n_sites species ids                              */
+  uint64_t first_move;      /* Philox counter of the chain's first move (0 for a
+                               new chain; a resumed chain continues its stream) */
 } cemc_chain_desc;
 
 CEMC_API int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const cemc_chain_desc* chains,
@@ -169,9 +176,31 @@ CEMC_API int cemc_batch_run_logged(cemc_batch* batch, const int64_t* n_moves, in
 CEMC_API int cemc_batch_get_energy(cemc_batch* batch, double* energy);
 /* Occupation of one chain (n_sites of that chain). */
 CEMC_API int cemc_batch_get_occupation(cemc_batch* batch, int32_t chain, int8_t* occupation);
-/* Per chain: moves done, accepted moves, sum of E, sum of E^2 over moves. */
+/* Per chain: moves done and accepted since creation; sum of E and of E^2
+ * over the moves since the last reset (each move samples the state after
+ * its accept/reject). */
 CEMC_API int cemc_batch_get_stats(cemc_batch* batch, int64_t* moves_done, int64_t* n_accept, double* sum_e,
                          double* sum_e2);
+/* Zero the sample sums of one chain (chain >= 0) or of all chains (-1) and
+ * take the current energy as their reference: Montecarlo.reset
+ * (cemc/mcmc/montecarlo.py:349-364), SGCMonteCarlo.reset (sgc_montecarlo.py:212-218). */
+CEMC_API int cemc_batch_reset_stats(cemc_batch* batch, int32_t chain);
+
+/* Sample averages of one chain since its last reset: the quantities
+ * Montecarlo.get_thermodynamic (montecarlo.py:861-888) and, for SGC chains,
+ * the SGCObserver sums (cemc/mcmc/mc_observers.py:222-270) feed into
+ * SGCMonteCarlo.get_thermodynamic (sgc_montecarlo.py:398-448). */
+typedef struct cemc_thermo {
+  int64_t n_samples;
+  double energy;                 /* current energy                                  */
+  double mean_energy;            /* <E>                                             */
+  double var_energy;             /* <E^2> - <E>^2                                   */
+  int32_t n_singlets;            /* n_species - 1                                   */
+  double singlet[7];             /* <c1_d>; canonical chains: the constant value    */
+  double var_singlet[7];         /* <c1_d^2> - <c1_d>^2 (SGC)                       */
+  double cov_singlet_energy[7];  /* <c1_d E> - <c1_d><E> (SGC)                      */
+} cemc_thermo;
+CEMC_API int cemc_batch_get_thermo(cemc_batch* batch, int32_t chain, cemc_thermo* out);
 /* Correlation functions of chain c computed from scratch on the GPU (n_cf of
  * that chain's model) and the energy they give. */
 CEMC_API int cemc_batch_get_cf(cemc_batch* batch, int32_t chain, double* cf, double* energy);
