@@ -162,6 +162,18 @@ class ChainBatch:
                                                 ctypes.byref(e)))
         return out, e.value
 
+    def set_waves(self, waves):
+        """Force the waves per chain (1, 2, 4, 8) where the kernel variant
+        allows it; 0 restores the automatic choice."""
+        _capi.check(self._lib.cemc_batch_set_waves(self.handle, int(waves)))
+
+    def launch_shape(self, chain):
+        """(waves per chain, records per lane in registers, tables in LDS)."""
+        w, r, g = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _capi.check(self._lib.cemc_batch_get_launch_shape(self.handle, int(chain), ctypes.byref(w),
+                                                          ctypes.byref(r), ctypes.byref(g)))
+        return w.value, r.value, bool(g.value)
+
     def set_temperature(self, chain, temperature):
         _capi.check(self._lib.cemc_batch_set_temperature(self.handle, int(chain), float(temperature)))
         self.temperatures[chain] = float(temperature)

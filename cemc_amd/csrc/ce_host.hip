@@ -648,10 +648,12 @@ struct cemc_batch {
     int kr;
     bool gtab_lds;
     int kbm;
+    int waves;
     int begin, count;
     uint32_t lds;
   };
   std::vector<Class> classes;
+  int forced_waves = 0;
   int n_cf_max = 1;
   ~cemc_batch() {
     dev_free(d_models);
@@ -668,6 +670,107 @@ struct cemc_batch {
 };
 
 namespace {
+
+// Launch classes: chains that share a sweep-kernel variant, each class
+// ordered longest chain first.  forced_waves > 0 fixes the waves per chain
+// (where the variant allows it); 0 chooses them from the batch size.
+void derive_classes(cemc_batch* b, int forced_waves) {
+  const int n_chains = b->n_chains;
+  // launch classes: chains sharing a kernel variant, longest first
+  std::vector<int32_t> order;
+  b->classes.clear();
+  const uint32_t limit = lds_limit();
+  const uint32_t gtab_budget = std::min(kGtabLdsBudget, limit);
+  struct Key {
+    bool periodic, canonical;
+    int kr;
+    bool glds;
+    int kbm;
+    int waves;
+    uint32_t lds;
+  };
+  // waves per chain: enough chains x waves to give every SIMD of the
+  // device a wave (4 per CU), at most 8, and no more waves than keep
+  // about two sub-clusters per lane busy
+  int n_cu = 256;
+  {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  int batch_waves = 1;
+  while (batch_waves < 8 && int64_t(n_chains) * batch_waves * 2 <= int64_t(4) * n_cu) batch_waves *= 2;
+  std::vector<Key> keys(n_chains);
+  for (int c = 0; c < n_chains; ++c) {
+    const cemc_model* m = b->models[b->params[c].model];
+    const bool canon = b->params[c].kind == CEMC_MOVE_CANONICAL;
+    Key k;
+    k.periodic = m->periodic;
+    k.canonical = canon;
+    k.kbm = m->deco_slices();
+    const bool single = m->n_groups == 1 && int(m->movable.size()) == m->n_sites;
+    const int cnt = m->grp_rec_cnt[0];
+    const int occ_bytes = m->periodic ? m->pdims[0] * m->pdims[1] * m->pdims[2] : m->n_sites;
+    const int trk_sites = canon ? m->n_sites : 0;
+    const uint32_t with_g = sweep_lds_bytes(occ_bytes, trk_sites, 0, int(m->gtab.size()));
+    const uint32_t no_g = sweep_lds_bytes(occ_bytes, trk_sites, 0, 0);
+    k.waves = 1;
+    if (forced_waves > 0)
+      k.waves = forced_waves;
+    else
+      while (k.waves < batch_waves && cnt > 128 * k.waves) k.waves *= 2;
+    k.kr = 0;
+    if (single && k.kbm <= 3 && no_g <= limit) {
+      for (int kr : {2, 4, 8})
+        if (cnt <= kr * 64 * k.waves) {
+          k.kr = kr;
+          break;
+        }
+    }
+    if (k.kr > 0) {
+      k.glds = with_g <= gtab_budget;
+      k.lds = k.glds ? with_g : no_g;
+    } else {
+      k.waves = 1;
+      const uint32_t rec_g = sweep_lds_bytes(occ_bytes, trk_sites, int(m->recs.size()), int(m->gtab.size()));
+      const uint32_t rec_only = sweep_lds_bytes(occ_bytes, trk_sites, int(m->recs.size()), 0);
+      k.glds = rec_g <= gtab_budget;
+      k.lds = k.glds ? rec_g : rec_only;
+      require(k.lds <= limit, "chain state does not fit in the workgroup's LDS");
+    }
+    keys[c] = k;
+  }
+  std::vector<bool> done(n_chains, false);
+  for (int c0 = 0; c0 < n_chains; ++c0) {
+    if (done[c0]) continue;
+    const Key& k0 = keys[c0];
+    std::vector<int32_t> ids;
+    uint32_t lds = 0;
+    for (int c = c0; c < n_chains; ++c) {
+      const Key& k = keys[c];
+      if (done[c] || k.periodic != k0.periodic || k.canonical != k0.canonical || k.kr != k0.kr ||
+          k.glds != k0.glds || k.kbm != k0.kbm || k.waves != k0.waves)
+        continue;
+      done[c] = true;
+      ids.push_back(c);
+      lds = std::max(lds, k.lds);
+    }
+    std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) {
+      const auto* mx = b->models[b->params[x].model];
+      const auto* my = b->models[b->params[y].model];
+      return int64_t(mx->n_sites) * mx->recs.size() > int64_t(my->n_sites) * my->recs.size();
+    });
+    b->classes.push_back(cemc_batch::Class{k0.periodic, k0.canonical, k0.kr, k0.glds, k0.kbm, k0.waves,
+                                           int(order.size()), int(ids.size()), lds});
+    order.insert(order.end(), ids.begin(), ids.end());
+  }
+  dev_free(b->d_order);
+  b->d_order = nullptr;
+  b->d_order = upload(order);
+  uint32_t need = 0;
+  for (const auto& cl : b->classes) need = std::max(need, cl.lds);
+  if (need + sweep_static_lds() > 64u * 1024u) HIP_CHECK(sweep_set_max_lds(limit + sweep_static_lds()));
+}
 
 void batch_launch(cemc_batch* b, const int64_t* n_moves, int64_t moves, bool timed, double* ms,
                   uint32_t* host_log = nullptr, int64_t log_words = 0) {
@@ -700,6 +803,7 @@ void batch_launch(cemc_batch* b, const int64_t* n_moves, int64_t moves, bool tim
     a.kr = cl.kr;
     a.gtab_lds = cl.gtab_lds;
     a.kbm = cl.kbm;
+    a.waves = cl.waves;
     a.accept_log = d_log;
     a.log_words = log_words;
     HIP_CHECK(launch_sweep(a, b->stream));
@@ -1077,72 +1181,7 @@ int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const c
       b->d_trk = upload(trk_pool);
       HIP_CHECK(hipMalloc(&b->d_states, sizeof(ChainState) * n_chains));
       HIP_CHECK(hipMalloc(&b->d_nmoves, sizeof(int64_t) * n_chains));
-      // launch classes: chains sharing a kernel variant, longest first
-      std::vector<int32_t> order;
-      const uint32_t limit = lds_limit();
-      const uint32_t gtab_budget = std::min(kGtabLdsBudget, limit);
-      struct Key {
-        bool periodic, canonical;
-        int kr;
-        bool glds;
-        int kbm;
-        uint32_t lds;
-      };
-      std::vector<Key> keys(n_chains);
-      for (int c = 0; c < n_chains; ++c) {
-        const cemc_model* m = b->models[b->params[c].model];
-        const bool canon = b->params[c].kind == CEMC_MOVE_CANONICAL;
-        Key k;
-        k.periodic = m->periodic;
-        k.canonical = canon;
-        k.kbm = m->deco_slices();
-        const bool single = m->n_groups == 1 && int(m->movable.size()) == m->n_sites;
-        const int cnt = m->grp_rec_cnt[0];
-        const int occ_bytes = m->periodic ? m->pdims[0] * m->pdims[1] * m->pdims[2] : m->n_sites;
-        const int trk_sites = canon ? m->n_sites : 0;
-        const uint32_t with_g = sweep_lds_bytes(occ_bytes, trk_sites, 0, int(m->gtab.size()));
-        k.kr = 0;
-        if (single && k.kbm <= 3 && with_g <= gtab_budget && cnt <= 512) k.kr = cnt <= 128 ? 2 : (cnt <= 256 ? 4 : 8);
-        if (k.kr > 0) {
-          k.glds = true;
-          k.lds = with_g;
-        } else {
-          const uint32_t rec_g = sweep_lds_bytes(occ_bytes, trk_sites, int(m->recs.size()), int(m->gtab.size()));
-          const uint32_t rec_only = sweep_lds_bytes(occ_bytes, trk_sites, int(m->recs.size()), 0);
-          k.glds = rec_g <= gtab_budget;
-          k.lds = k.glds ? rec_g : rec_only;
-          require(k.lds <= limit, "chain state does not fit in the workgroup's LDS");
-        }
-        keys[c] = k;
-      }
-      std::vector<bool> done(n_chains, false);
-      for (int c0 = 0; c0 < n_chains; ++c0) {
-        if (done[c0]) continue;
-        const Key& k0 = keys[c0];
-        std::vector<int32_t> ids;
-        uint32_t lds = 0;
-        for (int c = c0; c < n_chains; ++c) {
-          const Key& k = keys[c];
-          if (done[c] || k.periodic != k0.periodic || k.canonical != k0.canonical || k.kr != k0.kr ||
-              k.glds != k0.glds || k.kbm != k0.kbm)
-            continue;
-          done[c] = true;
-          ids.push_back(c);
-          lds = std::max(lds, k.lds);
-        }
-        std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) {
-          const auto* mx = b->models[b->params[x].model];
-          const auto* my = b->models[b->params[y].model];
-          return int64_t(mx->n_sites) * mx->recs.size() > int64_t(my->n_sites) * my->recs.size();
-        });
-        b->classes.push_back(cemc_batch::Class{k0.periodic, k0.canonical, k0.kr, k0.glds, k0.kbm,
-                                               int(order.size()), int(ids.size()), lds});
-        order.insert(order.end(), ids.begin(), ids.end());
-      }
-      b->d_order = upload(order);
-      uint32_t need = 0;
-      for (const auto& cl : b->classes) need = std::max(need, cl.lds);
-      if (need + sweep_static_lds() > 64u * 1024u) HIP_CHECK(sweep_set_max_lds(limit + sweep_static_lds()));
+      derive_classes(b, 0);
       // initial energies from correlation functions computed on the GPU
       std::vector<int32_t> all(n_chains);
       std::iota(all.begin(), all.end(), 0);
@@ -1286,6 +1325,35 @@ int cemc_batch_get_cf(cemc_batch* b, int32_t chain, double* cf, double* energy) 
     compute_cf(b->d_models, b->params, b->d_params, b->d_occ, {chain}, b->n_cf_max, &cfv, &ev, b->stream);
     if (cf) std::copy(cfv.begin(), cfv.begin() + b->n_cf_of_chain[chain], cf);
     if (energy) *energy = ev[0];
+  });
+}
+
+int cemc_batch_set_waves(cemc_batch* b, int32_t waves) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(waves == 0 || waves == 1 || waves == 2 || waves == 4 || waves == 8, "waves must be 0, 1, 2, 4 or 8");
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    derive_classes(b, waves);
+    b->forced_waves = waves;
+  });
+}
+
+int cemc_batch_get_launch_shape(cemc_batch* b, int32_t chain, int32_t* waves, int32_t* records_per_lane,
+                                int32_t* tables_in_lds) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    std::vector<int32_t> order(b->n_chains);
+    HIP_CHECK(hipMemcpy(order.data(), b->d_order, sizeof(int32_t) * order.size(), hipMemcpyDeviceToHost));
+    for (const auto& cl : b->classes)
+      for (int i = cl.begin; i < cl.begin + cl.count; ++i)
+        if (order[i] == chain) {
+          if (waves) *waves = cl.waves;
+          if (records_per_lane) *records_per_lane = cl.kr;
+          if (tables_in_lds) *tables_in_lds = cl.gtab_lds ? 1 : 0;
+          return;
+        }
+    throw std::runtime_error("chain not found in any launch class");
   });
 }
 

@@ -204,6 +204,17 @@ CEMC_API int cemc_batch_get_thermo(cemc_batch* batch, int32_t chain, cemc_thermo
 /* Correlation functions of chain c computed from scratch on the GPU (n_cf of
  * that chain's model) and the energy they give. */
 CEMC_API int cemc_batch_get_cf(cemc_batch* batch, int32_t chain, double* cf, double* energy);
+/* Launch shape.  Each chain runs on `waves` 64-lane wavefronts of one
+ * workgroup (1, 2, 4 or 8), chosen by default from the batch size so that
+ * the device gets about one wave per SIMD.  waves = 0 restores that choice;
+ * another value forces it wherever the chain's kernel variant allows (the
+ * autotuner in cemc_amd/autotune.py times the candidates).  The reference
+ * has no counterpart: its updater runs one chain per process. */
+CEMC_API int cemc_batch_set_waves(cemc_batch* batch, int32_t waves);
+/* Launch shape of one chain: waves per chain, sub-cluster records per lane
+ * held in registers (0: records in LDS), energy tables in LDS (1) or HBM (0). */
+CEMC_API int cemc_batch_get_launch_shape(cemc_batch* batch, int32_t chain, int32_t* waves,
+                                         int32_t* records_per_lane, int32_t* tables_in_lds);
 /* Temperature of one chain (parallel tempering uses it to exchange replicas). */
 CEMC_API int cemc_batch_set_temperature(cemc_batch* batch, int32_t chain, double temperature);
 

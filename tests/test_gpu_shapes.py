@@ -1,0 +1,119 @@
+"""GPU parity of every launch shape: one to eight waves per chain, energy
+tables in LDS or in HBM, periodic closed-form and translation-table lattices,
+and batches that mix models, move kinds and sizes.  Each chain must reproduce
+its reference fixture bit for bit whatever shape and company it runs in."""
+import numpy as np
+import pytest
+
+from golden_io import ChainFixture
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-10
+
+
+def _bits(packed, n):
+    return np.unpackbits(np.asarray(packed, dtype=np.uint8), bitorder="little")[:n]
+
+
+def _chain(fx, **extra):
+    d = dict(kind=fx.kind, temperature=fx.temperature, occupation=fx.occ0, seed=fx.seed, uid=fx.chain_uid,
+             chem_pot=fx.meta["mu"] or None)
+    d.update(extra)
+    return d
+
+
+def _table_settings(settings):
+    class Table:
+        pass
+
+    t = Table()
+    for key, val in vars(settings).items():
+        if key not in ("dims", "cells"):
+            setattr(t, key, val)
+    return t
+
+
+def _check(fx, batch, c, log):
+    ref = _bits(fx.accept_bits, fx.n_moves)
+    got = _bits(log[c], fx.n_moves)
+    mism = np.nonzero(got != ref)[0]
+    assert mism.size == 0, "first mismatching move %d" % mism[0]
+    assert np.array_equal(batch.occupation(c), fx.occ_final)
+    e = batch.energies()[c]
+    assert abs(e - fx.e_trace[-1]) / fx.n_sites <= E_TOL
+
+
+@pytest.mark.parametrize("fname", ["ternary_canonical.npz", "ternary_sgc.npz", "quaternary_canonical.npz"])
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+def test_waves_per_chain(gpu, fname, waves):
+    from cemc_amd.batch import ChainBatch
+    from cemc_amd.model import CEModel
+    fx = ChainFixture(fname)
+    print("fixture", fname, "waves", waves, flush=True)
+    model = CEModel(fx.settings, fx.eci)
+    batch = ChainBatch([model], [_chain(fx)])
+    batch.set_waves(waves)
+    w, kr, glds = batch.launch_shape(0)
+    print("shape", w, kr, glds, flush=True)
+    log = batch.run_logged(fx.n_moves)
+    _check(fx, batch, 0, log)
+
+
+@pytest.mark.parametrize("waves", [1, 4])
+def test_translation_table_lattice_with_waves(gpu, waves):
+    from cemc_amd.batch import ChainBatch
+    from cemc_amd.model import CEModel
+    fx = ChainFixture("ternary_canonical.npz")
+    model = CEModel(_table_settings(fx.settings), fx.eci)
+    assert not model.periodic
+    batch = ChainBatch([model], [_chain(fx)])
+    batch.set_waves(waves)
+    log = batch.run_logged(fx.n_moves)
+    _check(fx, batch, 0, log)
+
+
+def test_mixed_batch_matches_each_fixture(gpu):
+    """Binary SGC, ternary canonical/SGC and quaternary canonical chains in one
+    batch (several launch classes), each run as a whole and in two halves."""
+    from cemc_amd.batch import ChainBatch
+    from cemc_amd.model import CEModel
+    names = ["binary_sgc_chain0.npz", "binary_sgc_chain2.npz", "ternary_canonical.npz", "ternary_sgc.npz",
+             "quaternary_canonical.npz"]
+    fxs = [ChainFixture(n) for n in names]
+    models = [CEModel(fx.settings, fx.eci) for fx in fxs]
+    chains = [_chain(fx) for fx in fxs]
+    batch = ChainBatch(models, chains)
+    n_moves = np.array([fx.n_moves for fx in fxs], dtype=np.int64)
+    log = batch.run_logged(n_moves)
+    for c, fx in enumerate(fxs):
+        _check(fx, batch, c, log)
+    # the same chains advanced in two calls of uneven length
+    batch2 = ChainBatch(models, chains)
+    first = n_moves // 3
+    log_a = batch2.run_logged(first)
+    log_b = batch2.run_logged(n_moves - first)
+    for c, fx in enumerate(fxs):
+        a = _bits(log_a[c], int(first[c]))
+        b = _bits(log_b[c], int(n_moves[c] - first[c]))
+        assert np.array_equal(np.concatenate([a, b]), _bits(fx.accept_bits, fx.n_moves))
+        assert np.array_equal(batch2.occupation(c), fx.occ_final)
+
+
+def test_resumed_chain_continues_its_stream(gpu):
+    """A chain re-created from its occupation with first_move = t continues
+    exactly where the first batch stopped (canonical tracker rebuilt in
+    ascending order, as the reference's tracker is after set_symbols)."""
+    from cemc_amd.batch import ChainBatch
+    from cemc_amd.model import CEModel
+    fx = ChainFixture("binary_sgc_chain1.npz")
+    model = CEModel(fx.settings, fx.eci)
+    half = fx.n_moves // 2
+    b1 = ChainBatch([model], [_chain(fx)])
+    log1 = b1.run_logged(half)
+    occ_mid = b1.occupation(0)
+    b2 = ChainBatch([model], [_chain(fx, occupation=occ_mid, first_move=half)])
+    log2 = b2.run_logged(fx.n_moves - half)
+    got = np.concatenate([_bits(log1[0], half), _bits(log2[0], fx.n_moves - half)])
+    assert np.array_equal(got, _bits(fx.accept_bits, fx.n_moves))
+    assert np.array_equal(b2.occupation(0), fx.occ_final)
