@@ -88,6 +88,10 @@ SIGNATURES = [
     ("cemc_batch_set_waves", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("cemc_batch_get_launch_shape", ctypes.c_int, [_vp, ctypes.c_int32, _i32p, _i32p, _i32p]),
     ("cemc_batch_set_temperature", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double]),
+    ("cemc_pt_create", ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    ("cemc_pt_destroy", ctypes.c_int, [_vp]),
+    ("cemc_pt_run", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, _dp]),
+    ("cemc_pt_get", ctypes.c_int, [_vp, _i32p, _dp, _i64p, _i64p, _i64p, _dp, _dp]),
 ]
 
 _lib = None

@@ -146,6 +146,26 @@ struct ChainState {
 };
 static_assert(sizeof(ChainState) == 256, "ChainState layout");
 
+// Replica-exchange state of a parallel-tempering run over the R chains of a
+// batch.  Slot t keeps temperature kT[t] and its own energy sums; the chain
+// holding slot t is chain_of_slot[t].  Exchanges move chains between slots
+// (their kT in ChainParams follows), lattices never move.
+struct PTDev {
+  uint64_t cycle;        // exchange cycles done: Philox counter word 0
+  uint64_t seed, uid;    // Philox key of the exchange stream
+  int32_t n_rep;
+  int32_t pad;
+  int32_t* chain_of_slot;  // R
+  int32_t* slot_of_chain;  // R
+  double* kT;              // R, by slot
+  double* e_ref;           // R, by slot: reference of the slot's energy sums
+  double* sum_de;          // R
+  double* sum_de2;         // R
+  uint64_t* n_samples;     // R
+  uint64_t* n_accept;      // R-1, pair (t, t+1)
+  uint64_t* n_try;         // R-1
+};
+
 // ---------------------------------------------------------------------------
 // Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11).
 constexpr uint64_t kPhiloxM0 = 0xD2E7470EE14C6C93ull;

@@ -653,6 +653,7 @@ struct cemc_batch {
     uint32_t lds;
   };
   std::vector<Class> classes;
+  int64_t classes_version = 0;  // bumped whenever the launch classes change
   int forced_waves = 0;
   int n_cf_max = 1;
   ~cemc_batch() {
@@ -666,6 +667,39 @@ struct cemc_batch {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+// Parallel-tempering run over the chains of one batch (replica slot t starts
+// with chain t at that chain's temperature).
+constexpr int kPTGraphCycles = 16;  // sweep + exchange cycles per captured graph
+
+struct cemc_pt {
+  cemc_batch* batch = nullptr;
+  int n_rep = 0;
+  PTDev* d_pt = nullptr;
+  int32_t *d_cos = nullptr, *d_soc = nullptr;
+  double *d_kT = nullptr, *d_eref = nullptr, *d_sde = nullptr, *d_sde2 = nullptr;
+  uint64_t *d_n = nullptr, *d_acc = nullptr, *d_try = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_cycles = 0;
+  int64_t graph_version = -1;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+  }
+  ~cemc_pt() {
+    drop_graph();
+    dev_free(d_pt);
+    dev_free(d_cos);
+    dev_free(d_soc);
+    dev_free(d_kT);
+    dev_free(d_eref);
+    dev_free(d_sde);
+    dev_free(d_sde2);
+    dev_free(d_n);
+    dev_free(d_acc);
+    dev_free(d_try);
   }
 };
 
@@ -767,26 +801,15 @@ void derive_classes(cemc_batch* b, int forced_waves) {
   dev_free(b->d_order);
   b->d_order = nullptr;
   b->d_order = upload(order);
+  b->classes_version += 1;
   uint32_t need = 0;
   for (const auto& cl : b->classes) need = std::max(need, cl.lds);
   if (need + sweep_static_lds() > 64u * 1024u) HIP_CHECK(sweep_set_max_lds(limit + sweep_static_lds()));
 }
 
-void batch_launch(cemc_batch* b, const int64_t* n_moves, int64_t moves, bool timed, double* ms,
-                  uint32_t* host_log = nullptr, int64_t log_words = 0) {
-  std::vector<int64_t> nm(b->n_chains);
-  for (int c = 0; c < b->n_chains; ++c) {
-    nm[c] = n_moves ? n_moves[c] : moves;
-    require(nm[c] >= 0, "number of moves must be non-negative");
-    if (host_log) require(nm[c] <= 32 * log_words, "accept log too small for the number of moves");
-  }
-  uint32_t* d_log = nullptr;
-  if (host_log) {
-    HIP_CHECK(hipMalloc(&d_log, sizeof(uint32_t) * size_t(b->n_chains) * log_words));
-    HIP_CHECK(hipMemsetAsync(d_log, 0, sizeof(uint32_t) * size_t(b->n_chains) * log_words, b->stream));
-  }
-  HIP_CHECK(hipMemcpyAsync(b->d_nmoves, nm.data(), sizeof(int64_t) * nm.size(), hipMemcpyHostToDevice, b->stream));
-  if (timed) HIP_CHECK(hipEventRecord(b->ev0, b->stream));
+// One sweep-kernel launch per class on the batch's stream; each chain
+// advances by its entry of d_nmoves.
+void enqueue_sweeps(cemc_batch* b, uint32_t* d_log, int64_t log_words) {
   for (const auto& cl : b->classes) {
     LaunchSweep a;
     a.params = b->d_params;
@@ -808,6 +831,24 @@ void batch_launch(cemc_batch* b, const int64_t* n_moves, int64_t moves, bool tim
     a.log_words = log_words;
     HIP_CHECK(launch_sweep(a, b->stream));
   }
+}
+
+void batch_launch(cemc_batch* b, const int64_t* n_moves, int64_t moves, bool timed, double* ms,
+                  uint32_t* host_log = nullptr, int64_t log_words = 0) {
+  std::vector<int64_t> nm(b->n_chains);
+  for (int c = 0; c < b->n_chains; ++c) {
+    nm[c] = n_moves ? n_moves[c] : moves;
+    require(nm[c] >= 0, "number of moves must be non-negative");
+    if (host_log) require(nm[c] <= 32 * log_words, "accept log too small for the number of moves");
+  }
+  uint32_t* d_log = nullptr;
+  if (host_log) {
+    HIP_CHECK(hipMalloc(&d_log, sizeof(uint32_t) * size_t(b->n_chains) * log_words));
+    HIP_CHECK(hipMemsetAsync(d_log, 0, sizeof(uint32_t) * size_t(b->n_chains) * log_words, b->stream));
+  }
+  HIP_CHECK(hipMemcpyAsync(b->d_nmoves, nm.data(), sizeof(int64_t) * nm.size(), hipMemcpyHostToDevice, b->stream));
+  if (timed) HIP_CHECK(hipEventRecord(b->ev0, b->stream));
+  enqueue_sweeps(b, d_log, log_words);
   if (timed) HIP_CHECK(hipEventRecord(b->ev1, b->stream));
   HIP_CHECK(hipStreamSynchronize(b->stream));
   if (d_log) {
@@ -1364,6 +1405,154 @@ int cemc_batch_set_temperature(cemc_batch* b, int32_t chain, double temperature)
     require(temperature > 0.0 && std::isfinite(temperature), "temperature must be positive");
     b->params[chain].kT = temperature * kBoltzmannEV;
     HIP_CHECK(hipMemcpy(&b->d_params[chain].kT, &b->params[chain].kT, sizeof(double), hipMemcpyHostToDevice));
+  });
+}
+
+// ------------------------------------------------------------ tempering
+int cemc_pt_create(cemc_batch* b, uint64_t seed, uint64_t uid, cemc_pt** out) {
+  return guarded([&] {
+    require(b && out, "NULL argument");
+    *out = nullptr;
+    const int R = b->n_chains;
+    require(R >= 2, "parallel tempering needs at least two replicas");
+    auto* pt = new cemc_pt();
+    try {
+      pt->batch = b;
+      pt->n_rep = R;
+      HIP_CHECK(hipStreamSynchronize(b->stream));
+      std::vector<ChainState> st(R);
+      HIP_CHECK(hipMemcpy(st.data(), b->d_states, sizeof(ChainState) * R, hipMemcpyDeviceToHost));
+      std::vector<int32_t> ident(R);
+      std::iota(ident.begin(), ident.end(), 0);
+      std::vector<double> kT(R), eref(R);
+      for (int t = 0; t < R; ++t) {
+        kT[t] = b->params[t].kT;
+        eref[t] = st[t].energy;
+        // chain sums restart with the run
+        st[t].e_ref = st[t].energy;
+        st[t].n_samples = 0;
+        st[t].sum_de = st[t].sum_de2 = 0.0;
+      }
+      HIP_CHECK(hipMemcpy(b->d_states, st.data(), sizeof(ChainState) * R, hipMemcpyHostToDevice));
+      PTDev h;
+      std::memset(&h, 0, sizeof h);
+      h.seed = seed;
+      h.uid = uid;
+      h.n_rep = R;
+      h.chain_of_slot = upload(ident);
+      pt->d_cos = h.chain_of_slot;
+      h.slot_of_chain = upload(ident);
+      pt->d_soc = h.slot_of_chain;
+      h.kT = upload(kT);
+      pt->d_kT = h.kT;
+      h.e_ref = upload(eref);
+      pt->d_eref = h.e_ref;
+      std::vector<double> zd(R, 0.0);
+      std::vector<uint64_t> zu(R, 0);
+      h.sum_de = upload(zd);
+      pt->d_sde = h.sum_de;
+      h.sum_de2 = upload(zd);
+      pt->d_sde2 = h.sum_de2;
+      h.n_samples = upload(zu);
+      pt->d_n = h.n_samples;
+      h.n_accept = upload(zu);
+      pt->d_acc = h.n_accept;
+      h.n_try = upload(zu);
+      pt->d_try = h.n_try;
+      pt->d_pt = upload(std::vector<PTDev>{h});
+    } catch (...) {
+      delete pt;
+      throw;
+    }
+    *out = pt;
+  });
+}
+
+int cemc_pt_destroy(cemc_pt* pt) {
+  return guarded([&] { delete pt; });
+}
+
+int cemc_pt_run(cemc_pt* pt, int64_t n_cycles, int64_t moves_per_cycle, int32_t use_graph, double* kernel_ms) {
+  return guarded([&] {
+    require(pt != nullptr, "NULL tempering run");
+    require(n_cycles >= 0 && moves_per_cycle >= 0, "cycles and moves must be non-negative");
+    cemc_batch* b = pt->batch;
+    std::vector<int64_t> nm(b->n_chains, moves_per_cycle);
+    HIP_CHECK(hipMemcpyAsync(b->d_nmoves, nm.data(), sizeof(int64_t) * nm.size(), hipMemcpyHostToDevice, b->stream));
+    int64_t done = 0;
+    if (kernel_ms) HIP_CHECK(hipEventRecord(b->ev0, b->stream));
+    if (use_graph && n_cycles >= 2) {
+      const int G = int(std::min<int64_t>(n_cycles, kPTGraphCycles));
+      if (pt->exec == nullptr || pt->graph_cycles != G || pt->graph_version != b->classes_version) {
+        pt->drop_graph();
+        hipGraph_t graph = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        try {
+          for (int g = 0; g < G; ++g) {
+            enqueue_sweeps(b, nullptr, 0);
+            HIP_CHECK(launch_pt_exchange(pt->d_pt, b->d_states, b->d_params, b->stream));
+          }
+        } catch (...) {
+          (void)hipStreamEndCapture(b->stream, &graph);
+          if (graph) (void)hipGraphDestroy(graph);
+          throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(b->stream, &graph));
+        const hipError_t e = hipGraphInstantiate(&pt->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_CHECK(e);
+        pt->graph_cycles = G;
+        pt->graph_version = b->classes_version;
+      }
+      for (; done + pt->graph_cycles <= n_cycles; done += pt->graph_cycles)
+        HIP_CHECK(hipGraphLaunch(pt->exec, b->stream));
+    }
+    for (; done < n_cycles; ++done) {
+      enqueue_sweeps(b, nullptr, 0);
+      HIP_CHECK(launch_pt_exchange(pt->d_pt, b->d_states, b->d_params, b->stream));
+    }
+    if (kernel_ms) HIP_CHECK(hipEventRecord(b->ev1, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    if (kernel_ms) {
+      float t = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&t, b->ev0, b->ev1));
+      *kernel_ms = double(t);
+    }
+    // the exchanges moved temperatures on the device: refresh the host copy
+    HIP_CHECK(hipMemcpy(b->params.data(), b->d_params, sizeof(ChainParams) * b->params.size(), hipMemcpyDeviceToHost));
+  });
+}
+
+int cemc_pt_get(cemc_pt* pt, int32_t* chain_of_slot, double* temperatures, int64_t* n_accept, int64_t* n_try,
+                int64_t* n_samples, double* mean_energy, double* var_energy) {
+  return guarded([&] {
+    require(pt != nullptr, "NULL tempering run");
+    const int R = pt->n_rep;
+    HIP_CHECK(hipStreamSynchronize(pt->batch->stream));
+    std::vector<int32_t> cos(R);
+    std::vector<double> kT(R), eref(R), sde(R), sde2(R);
+    std::vector<uint64_t> n(R), acc(R), tr(R);
+    HIP_CHECK(hipMemcpy(cos.data(), pt->d_cos, sizeof(int32_t) * R, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(kT.data(), pt->d_kT, sizeof(double) * R, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(eref.data(), pt->d_eref, sizeof(double) * R, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(sde.data(), pt->d_sde, sizeof(double) * R, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(sde2.data(), pt->d_sde2, sizeof(double) * R, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(n.data(), pt->d_n, sizeof(uint64_t) * R, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(acc.data(), pt->d_acc, sizeof(uint64_t) * R, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(tr.data(), pt->d_try, sizeof(uint64_t) * R, hipMemcpyDeviceToHost));
+    for (int t = 0; t < R; ++t) {
+      if (chain_of_slot) chain_of_slot[t] = cos[t];
+      if (temperatures) temperatures[t] = kT[t] / kBoltzmannEV;
+      if (n_samples) n_samples[t] = int64_t(n[t]);
+      const double nn = double(n[t]);
+      const double m = n[t] ? sde[t] / nn : 0.0;
+      if (mean_energy) mean_energy[t] = eref[t] + m;
+      if (var_energy) var_energy[t] = n[t] ? sde2[t] / nn - m * m : 0.0;
+      if (t < R - 1) {
+        if (n_accept) n_accept[t] = int64_t(acc[t]);
+        if (n_try) n_try[t] = int64_t(tr[t]);
+      }
+    }
   });
 }
 

@@ -37,6 +37,7 @@ hipError_t sweep_set_max_lds(uint32_t bytes);
 uint32_t sweep_static_lds();
 
 hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream);
+hipError_t launch_pt_exchange(PTDev* pt, ChainState* states, ChainParams* params, hipStream_t stream);
 hipError_t launch_cf(const ModelDev* models, const ChainParams* params, const int8_t* occ_pool,
                      const int32_t* chain_ids, int n_chains, int max_sites, double* partial, double* cf_out,
                      double* energy_out, hipStream_t stream);

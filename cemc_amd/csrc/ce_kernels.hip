@@ -246,6 +246,70 @@ __global__ void slot_energy_kernel(const ModelDev* __restrict__ model, const dou
 }
 
 // ---------------------------------------------------------------------------
+// Replica exchange after a sweep of every replica (parallel tempering,
+// cemc/mcmc/parallel_tempering.py:138-191).  One workgroup:
+//  1. each chain's energy sums since the last cycle move into its slot's sums;
+//  2. the direction of cycle c comes from Philox block (c, 0, 0, 0) of key
+//     (seed, uid): bit 63 of w0 set = "up", pairs (0,1), (2,3), ...;
+//     clear = "down", pairs (R-1,R-2), (R-3,R-4), ...  (the reference picks
+//     the direction with random.choice, parallel_tempering.py:188-191);
+//  3. pair k uses u = unit_double(w3) of block (c, 1 + k, 0, 0) and accepts
+//     if u < exp((1/kT_a - 1/kT_b) (E_a - E_b)) (parallel_tempering.py:138-144,
+//     166-168); an accepted pair swaps the chains of the two slots and each
+//     chain takes its new slot's temperature.
+__global__ __launch_bounds__(256) void pt_exchange_kernel(PTDev* __restrict__ pt_ptr, ChainState* __restrict__ states,
+                                                          ChainParams* __restrict__ params) {
+  __shared__ int up;
+  PTDev& pt = *pt_ptr;
+  const int R = pt.n_rep;
+  const int tid = int(threadIdx.x);
+  const uint64_t cyc = pt.cycle;
+  for (int c = tid; c < R; c += 256) {
+    ChainState& st = states[c];
+    const uint64_t n = st.n_samples;
+    if (n > 0) {
+      const int t = pt.slot_of_chain[c];
+      const double d = st.e_ref - pt.e_ref[t];
+      const double dn = double(n);
+      pt.sum_de[t] += st.sum_de + dn * d;
+      pt.sum_de2[t] += st.sum_de2 + 2.0 * d * st.sum_de + dn * d * d;
+      pt.n_samples[t] += n;
+      st.n_samples = 0;
+      st.sum_de = 0.0;
+      st.sum_de2 = 0.0;
+    }
+  }
+  if (tid == 0) {
+    const Philox4 b = philox4x64_10(cyc, 0, 0, 0, pt.seed, pt.uid);
+    up = int(b.w0 >> 63);
+  }
+  __syncthreads();
+  const int n_pairs = R / 2;  // pairs (i, i+1) for even i <= R-2, or the mirror
+  for (int k = tid; k < n_pairs; k += 256) {
+    const int a = up ? 2 * k : R - 1 - 2 * k;
+    const int b = up ? a + 1 : a - 1;
+    const int ca = pt.chain_of_slot[a], cb = pt.chain_of_slot[b];
+    const double ea = states[ca].energy, eb = states[cb].energy;
+    const double p = exp((1.0 / pt.kT[a] - 1.0 / pt.kT[b]) * (ea - eb));
+    const Philox4 blk = philox4x64_10(cyc, 1 + uint64_t(k), 0, 0, pt.seed, pt.uid);
+    const bool acc = unit_double(blk.w3) < p;
+    const int lo = a < b ? a : b;
+    pt.n_try[lo] += 1;
+    if (acc) {
+      pt.n_accept[lo] += 1;
+      pt.chain_of_slot[a] = cb;
+      pt.chain_of_slot[b] = ca;
+      pt.slot_of_chain[cb] = a;
+      pt.slot_of_chain[ca] = b;
+      params[cb].kT = pt.kT[a];
+      params[ca].kT = pt.kT[b];
+    }
+  }
+  __syncthreads();
+  if (tid == 0) pt.cycle = cyc + 1;
+}
+
+// ---------------------------------------------------------------------------
 // Launch wrappers.  The sweep kernel's instantiations are compiled in
 // separate objects (ce_sweep_inst.hip) and reached through these lookups.
 #define CEMC_DECLARE_SWEEP_FN(W, P) \
@@ -318,6 +382,11 @@ hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream) {
   int64_t log_words = a.log_words;
   void* args[] = {&params, &states, &models, &occ_pool, &trk_pool, &order, &n_moves, &log, &log_words};
   return hipLaunchKernel(fn, dim3(a.n_chains), dim3(64 * a.waves), args, a.lds_bytes, stream);
+}
+
+hipError_t launch_pt_exchange(PTDev* pt, ChainState* states, ChainParams* params, hipStream_t stream) {
+  hipLaunchKernelGGL(pt_exchange_kernel, dim3(1), dim3(256), 0, stream, pt, states, params);
+  return hipGetLastError();
 }
 
 hipError_t launch_cf(const ModelDev* models, const ChainParams* params, const int8_t* occ_pool,

@@ -36,6 +36,7 @@ enum { CEMC_MOVE_CANONICAL = 0, CEMC_MOVE_SGC = 1 };
 typedef struct cemc_model cemc_model;
 typedef struct cemc_updater cemc_updater;
 typedef struct cemc_batch cemc_batch;
+typedef struct cemc_pt cemc_pt;
 
 /* Message of the calling thread's last failed call. */
 CEMC_API const char* cemc_last_error(void);
@@ -217,6 +218,34 @@ CEMC_API int cemc_batch_get_launch_shape(cemc_batch* batch, int32_t chain, int32
                                          int32_t* records_per_lane, int32_t* tables_in_lds);
 /* Temperature of one chain (parallel tempering uses it to exchange replicas). */
 CEMC_API int cemc_batch_set_temperature(cemc_batch* batch, int32_t chain, double temperature);
+
+/* ------------------------------------------------------------ tempering
+ * Parallel tempering over the chains of a batch
+ * (ParallelTempering, cemc/mcmc/parallel_tempering.py:5-191), with the
+ * exchange step on the GPU.  Replica slot t starts with chain t at that
+ * chain's temperature.  One cycle = every chain makes `moves_per_cycle`
+ * trial moves (the reference's runMC per replica, :186-187), then one
+ * exchange step:
+ *   direction of cycle c: Philox4x64-10 block (c, 0, 0, 0) of key
+ *     (seed, uid), bit 63 of word 0 set = "up" (pairs (0,1), (2,3), ...),
+ *     clear = "down" (pairs (R-1,R-2), (R-3,R-4), ...)   (:153-160, 188-191)
+ *   pair k: u = (w3 >> 11) * 2^-53 of block (c, 1 + k, 0, 0); accepted if
+ *     u < exp((1/kT_a - 1/kT_b) (E_a - E_b))              (:138-144, 166-168)
+ * An accepted exchange moves the two chains between the slots (each takes
+ * the other's temperature); lattices stay where they are.  Each chain's
+ * energy samples of a cycle are added to the sums of the slot it held.
+ * The chains' own sample sums are consumed by the run. */
+CEMC_API int cemc_pt_create(cemc_batch* batch, uint64_t seed, uint64_t uid, cemc_pt** out);
+CEMC_API int cemc_pt_destroy(cemc_pt* pt);
+/* Run n_cycles cycles; use_graph != 0 replays them from a captured HIP graph
+ * of up to 16 cycles.  kernel_ms (may be NULL) receives the device time. */
+CEMC_API int cemc_pt_run(cemc_pt* pt, int64_t n_cycles, int64_t moves_per_cycle, int32_t use_graph,
+                         double* kernel_ms);
+/* Per slot (R values; NULL skips an output): chain held, temperature, energy
+ * samples, their mean and variance; per pair (t, t+1) (R-1 values):
+ * accepted and attempted exchanges. */
+CEMC_API int cemc_pt_get(cemc_pt* pt, int32_t* chain_of_slot, double* temperatures, int64_t* n_accept,
+                         int64_t* n_try, int64_t* n_samples, double* mean_energy, double* var_energy);
 
 #ifdef __cplusplus
 }

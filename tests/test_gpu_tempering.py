@@ -1,0 +1,80 @@
+"""Parallel tempering on the GPU against the oracle's restatement on the same
+fed stream: the slot permutation, the exchange acceptance counts, every
+lattice and the per-temperature energy averages must agree, with and without
+the captured HIP graph."""
+import numpy as np
+import pytest
+
+from cemc_amd.lattice import FCCSettings, random_occupation, seeded_ecis
+from oracle.brute_force import brute_force_cf
+from oracle.oracle import OracleModel, OracleUpdater
+from oracle.pt import run_pt
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(species, shells, size, R, seed=5):
+    s = FCCSettings(size, species, shells)
+    eci = seeded_ecis(s, seed, 0.02)
+    occs = [random_occupation(s.n_sites, len(species), 100 + r) for r in range(R)]
+    return s, eci, occs
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+@pytest.mark.parametrize("species,shells,size,R", [
+    (["A", "B"], {2: 4, 3: 2}, 6, 6),
+    (["Al", "Mg", "Si"], {2: 3, 3: 2, 4: 1}, 5, 5),
+])
+def test_tempering_matches_oracle(gpu, use_graph, species, shells, size, R):
+    from cemc_amd.batch import CANONICAL, ChainBatch
+    from cemc_amd.model import CEModel
+    from cemc_amd.tempering import ReplicaExchange
+    s, eci, occs = _setup(species, shells, size, R)
+    temps = np.geomspace(250.0, 2500.0, R).tolist()
+    model = CEModel(s, eci)
+    chains = [dict(kind=CANONICAL, temperature=T, occupation=occs[r], seed=11, uid=r) for r, T in enumerate(temps)]
+    batch = ChainBatch([model] * R, chains)
+    pt = ReplicaExchange(batch, seed=21, uid=99)
+    n_cycles, moves = 40, s.n_sites // 2
+    print("replicas", R, "cycles", n_cycles, "graph", use_graph, flush=True)
+    pt.run(n_cycles - 7, moves, use_graph=use_graph)
+    pt.run(7, moves, use_graph=use_graph)
+    st = pt.state()
+
+    names = s.cf_names()
+    om = OracleModel(s, names)
+    ups = [OracleUpdater(om, o, brute_force_cf(s, o, names), eci) for o in occs]
+    cos, n_acc, samples = run_pt(ups, 0, [11] * R, list(range(R)), temps, n_cycles, moves, 21, 99)
+    assert np.array_equal(st["chain_of_slot"], cos)
+    assert np.array_equal(st["n_accept"], n_acc)
+    assert st["n_try"].sum() == n_cycles * (R // 2)
+    for c in range(R):
+        assert np.array_equal(batch.occupation(c), ups[c].get_occupation())
+        assert abs(batch.energies()[c] - ups[c].get_energy()) <= 1e-10 * s.n_sites
+    for t in range(R):
+        assert st["n_samples"][t] == len(samples[t])
+        assert abs(st["mean_energy"][t] - samples[t].mean()) <= 1e-9
+        assert abs(st["var_energy"][t] - samples[t].var()) <= 1e-7 * max(1.0, samples[t].var())
+    # temperatures follow their slots
+    assert np.allclose(st["temperatures"], temps, rtol=1e-15)
+    assert n_acc.sum() > 0
+
+
+def test_parallel_tempering_facade(gpu):
+    from cemc_amd.calculator import CE
+    from cemc_amd.lattice import SimpleAtoms
+    from cemc_amd.mcmc import Montecarlo
+    from cemc_amd.tempering import ParallelTempering
+    s, eci, occs = _setup(["A", "B"], {2: 2}, 6, 1)
+    atoms = SimpleAtoms([s.species[v] for v in occs[0]])
+    CE(atoms, s, eci=dict(eci))
+    mc = Montecarlo(atoms, 800.0, seed=3)
+    pt = ParallelTempering(mc, Tmax=2000.0, Tmin=200.0, temp_scheme_file="", n_replicas=8)
+    pt.run(mc_args={"steps": s.n_sites}, num_exchange_cycles=20)
+    res = pt.get_thermodynamic()
+    assert len(res) == 8
+    assert all(r["n_samples"] == 20 * s.n_sites for r in res)
+    # lower temperature, lower mean energy (on average over the run)
+    assert res[-1]["energy"] <= res[0]["energy"] + 1e-9
+    occ = pt.replica_occupation(0)
+    assert occ.shape == (s.n_sites,)
