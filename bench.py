@@ -60,6 +60,79 @@ def batched_chains(settings, n_chains=1024):
     return chains
 
 
+def ternary_uneven(n_chains=256, sweeps=200, seed=3):
+    """256 canonical chains, ternary FCC with 2 polynomial basis functions,
+    pairs to the 4th shell, triplets to the 3rd, quadruplets to the 2nd,
+    ECIs N(0, 0.01 eV)/d^2; cells L^3 with L uniform in 8..16 (512-4096
+    sites), compositions Dirichlet(1,1,1), T uniform in 300-1500 K; every
+    chain makes `sweeps` sweeps in one launch."""
+    from cemc_amd.batch import CANONICAL, ChainBatch
+    from cemc_amd.lattice import FCCSettings, random_occupation, seeded_ecis
+    from cemc_amd.model import CEModel
+    rng = np.random.Generator(np.random.Philox(seed))
+    sizes = rng.integers(8, 17, n_chains)
+    temps = rng.uniform(300.0, 1500.0, n_chains)
+    fracs = rng.dirichlet([1.0, 1.0, 1.0], n_chains)
+    models = {}
+    chains, chain_models = [], []
+    for c in range(n_chains):
+        L = int(sizes[c])
+        if L not in models:
+            st = FCCSettings(L, ["A", "B", "C"], {2: 4, 3: 3, 4: 2})
+            models[L] = (st, CEModel(st, seeded_ecis(st, 2025, 0.01)))
+        st, model = models[L]
+        f = np.maximum(fracs[c], 0.02)
+        f = f / f.sum()
+        occ = random_occupation(st.n_sites, 3, 5000 + c, fractions=list(f))
+        chains.append(dict(kind=CANONICAL, temperature=float(temps[c]), occupation=occ, seed=500, uid=c))
+        chain_models.append(model)
+    batch = ChainBatch(chain_models, chains)
+    moves = np.array([sweeps * m.n_sites for m in chain_models], dtype=np.int64)
+    batch.run(moves // 20)  # warm-up: a tenth of a step
+    t0 = time.perf_counter()
+    kernel_ms = batch.run_timed(moves)
+    wall = time.perf_counter() - t0
+    shapes = sorted({batch.launch_shape(c) for c in range(n_chains)})
+    total = int(moves.sum())
+    m0 = models[int(sizes[0])][1]
+    return {"workload": "ternary FCC, 256 canonical chains, L^3 with L in 8..16 (512-4096 sites), pairs<=4th, "
+                        "triplets<=3rd, quadruplets<=2nd shell, %d sweeps each, one launch" % sweeps,
+            "value": total / wall, "unit": "moves/s", "moves": total, "kernel_ms": kernel_ms, "wall_s": wall,
+            "bytes_per_move": 10.0 * m0.gathers_per_flip(), "launch_shapes": [list(x) for x in shapes],
+            "acceptance": float(batch.stats()["accepted"].sum() / max(total + int((moves // 20).sum()), 1))}
+
+
+def tempering(n_rep=128, cycles=2000, L=32, seed=9):
+    """Quaternary FCC 32^3 (32768 sites), 3 basis functions, pairs,
+    triplets and quadruplets to the 3rd shell; 128 replicas on
+    geomspace(200, 2000) K; one canonical sweep per replica then one
+    on-device exchange step per cycle; cycles replayed from HIP graphs."""
+    from cemc_amd.batch import CANONICAL, ChainBatch
+    from cemc_amd.lattice import FCCSettings, random_occupation, seeded_ecis
+    from cemc_amd.model import CEModel
+    from cemc_amd.tempering import ReplicaExchange
+    st = FCCSettings(L, ["A", "B", "C", "D"], {2: 3, 3: 3, 4: 3})
+    model = CEModel(st, seeded_ecis(st, 2026, 0.02))
+    temps = np.geomspace(200.0, 2000.0, n_rep)
+    chains = [dict(kind=CANONICAL, temperature=float(T), occupation=random_occupation(st.n_sites, 4, 7000 + r),
+                   seed=seed, uid=r) for r, T in enumerate(temps)]
+    batch = ChainBatch([model] * n_rep, chains)
+    pt = ReplicaExchange(batch, seed=seed, uid=1 << 40)
+    pt.run(2, st.n_sites)  # warm-up and graph build
+    t0 = time.perf_counter()
+    kernel_ms = pt.run(cycles, st.n_sites)
+    wall = time.perf_counter() - t0
+    state = pt.state()
+    total = cycles * n_rep * st.n_sites
+    acc = state["n_accept"].sum() / max(state["n_try"].sum(), 1)
+    return {"workload": "quaternary FCC 32^3 (32768 sites), <=4-body to the 3rd shell (%d CFs, %d sub-clusters per "
+                        "site), %d replicas 200-2000 K, exchange every sweep, %d sweeps, HIP-graph cycles"
+                        % (model.n_cf, model.n_subclusters, n_rep, cycles),
+            "value": total / wall, "unit": "moves/s", "moves": total, "kernel_ms": kernel_ms, "wall_s": wall,
+            "bytes_per_move": 10.0 * model.gathers_per_flip(), "launch_shape": list(batch.launch_shape(0)),
+            "exchange_accept_rate": float(acc)}
+
+
 def cpu_baseline(settings, eci, chains, target_s=12.0):
     """Oracle (C restatement) chains on the host cores, one chain per thread."""
     from oracle.brute_force import brute_force_cf
@@ -115,6 +188,9 @@ def main():
     ap.add_argument("--sweeps", type=int, default=500)
     ap.add_argument("--chains", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workloads", default="batched,ternary,tempering",
+                    help="comma list of batched, ternary, tempering")
+    ap.add_argument("--pt-cycles", type=int, default=2000)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this size first (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -182,6 +258,15 @@ def main():
         "plain_moves_per_s": plain_value, "batched_over_plain": value / plain_value,
         "roofline": roofline,
     }
+    work = set(w.strip() for w in args.workloads.split(","))
+    if "ternary" in work:
+        out["ternary_uneven"] = ternary_uneven()
+        log("ternary uneven: %.3e moves/s (kernel %.1f ms)" % (out["ternary_uneven"]["value"],
+                                                               out["ternary_uneven"]["kernel_ms"]))
+    if "tempering" in work:
+        out["parallel_tempering"] = tempering(cycles=args.pt_cycles)
+        log("tempering: %.3e moves/s (kernel %.1f ms)" % (out["parallel_tempering"]["value"],
+                                                          out["parallel_tempering"]["kernel_ms"]))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(settings, eci, chains)
         log("cpu baseline: %.3e moves/s on %d threads" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["cores"]))

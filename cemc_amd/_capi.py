@@ -12,7 +12,11 @@ import numpy as np
 
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libcemcgpu.so")
 
-CEMC_OK, CEMC_EINVAL, CEMC_ERUNTIME, CEMC_EDEVICE = 0, 1, 2, 3
+CEMC_OK, CEMC_EINVAL, CEMC_ERUNTIME, CEMC_EDEVICE, CEMC_EFULL = 0, 1, 2, 3, 4
+
+
+class PoolFull(RuntimeError):
+    """A pool has no free slot or device memory for one more chain."""
 CF_C0, CF_SINGLET, CF_CLUSTER = 0, 1, 2
 MOVE_CANONICAL, MOVE_SGC = 0, 1
 
@@ -88,6 +92,11 @@ SIGNATURES = [
     ("cemc_batch_set_waves", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("cemc_batch_get_launch_shape", ctypes.c_int, [_vp, ctypes.c_int32, _i32p, _i32p, _i32p]),
     ("cemc_batch_set_temperature", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double]),
+    ("cemc_pool_create", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
+                                        ctypes.POINTER(_vp)]),
+    ("cemc_pool_footprint", ctypes.c_int, [_vp, ctypes.c_int32, _i64p, _i64p]),
+    ("cemc_pool_add", ctypes.c_int, [_vp, _vp, ctypes.POINTER(ChainDesc), _i32p]),
+    ("cemc_pool_remove", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("cemc_pt_create", ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     ("cemc_pt_destroy", ctypes.c_int, [_vp]),
     ("cemc_pt_run", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, _dp]),
@@ -124,6 +133,8 @@ def check(rc):
     msg = _lib.cemc_last_error().decode(errors="replace") if _lib is not None else "libcemcgpu error"
     if rc == CEMC_EINVAL:
         raise ValueError(msg)
+    if rc == CEMC_EFULL:
+        raise PoolFull(msg)
     raise RuntimeError(msg)
 
 

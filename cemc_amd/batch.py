@@ -37,6 +37,21 @@ def chem_pot_vector(model, chem_pot):
     return vec
 
 
+def _fill_desc(d, m, ch, keep):
+    occ, op = _capi.i8(ch["occupation"])
+    if occ.shape[0] != m.n_sites:
+        raise ValueError("occupation length does not match the model")
+    mu, mp = _capi.f64(chem_pot_vector(m, ch.get("chem_pot")))
+    keep += [occ, mu]
+    d.kind = int(ch.get("kind", CANONICAL))
+    d.temperature = float(ch["temperature"])
+    d.chem_pot = mp
+    d.seed = int(ch.get("seed", 0)) & 0xFFFFFFFFFFFFFFFF
+    d.uid = int(ch.get("uid", 0)) & 0xFFFFFFFFFFFFFFFF
+    d.occupation = op
+    d.first_move = int(ch.get("first_move", 0)) & 0xFFFFFFFFFFFFFFFF
+
+
 class ChainBatch:
     def __init__(self, models, chains):
         """models: one CEModel per chain; chains: dicts with keys kind,
@@ -55,19 +70,8 @@ class ChainBatch:
         self.kinds = np.zeros(n, dtype=np.int32)
         self.temperatures = np.zeros(n)
         for c, (m, ch) in enumerate(zip(models, chains)):
-            occ, op = _capi.i8(ch["occupation"])
-            if occ.shape[0] != m.n_sites:
-                raise ValueError("occupation length does not match the model")
-            mu, mp = _capi.f64(chem_pot_vector(m, ch.get("chem_pot")))
-            keep += [occ, mu]
             d = descs[c]
-            d.kind = int(ch.get("kind", CANONICAL))
-            d.temperature = float(ch["temperature"])
-            d.chem_pot = mp
-            d.seed = int(ch.get("seed", 0)) & 0xFFFFFFFFFFFFFFFF
-            d.uid = int(ch.get("uid", c)) & 0xFFFFFFFFFFFFFFFF
-            d.occupation = op
-            d.first_move = int(ch.get("first_move", 0)) & 0xFFFFFFFFFFFFFFFF
+            _fill_desc(d, m, dict(ch, uid=ch.get("uid", c)), keep)
             handles[c] = m.handle
             self.n_sites[c] = m.n_sites
             self.kinds[c] = d.kind
@@ -188,3 +192,53 @@ class ChainBatch:
             self.close()
         except Exception:
             pass
+
+
+class ChainPool(ChainBatch):
+    """A batch whose chains join and leave while it runs (the resident state
+    of a job server).  Capacity is fixed at creation: chain slots, distinct
+    models, occupation bytes and canonical tracker entries; ``add`` raises
+    ``cemc_amd._capi.PoolFull`` when a chain does not fit.  Runs take one
+    move count per slot (0 for slots that should not move)."""
+
+    def __init__(self, max_chains, max_models, occ_bytes, tracker_entries):
+        lib = _capi.load()
+        self._lib = lib
+        h = ctypes.c_void_p()
+        _capi.check(lib.cemc_pool_create(int(max_chains), int(max_models), int(occ_bytes), int(tracker_entries),
+                                         ctypes.byref(h)))
+        self.handle = h
+        self.n_chains = int(max_chains)
+        self.models = [None] * self.n_chains
+        self.n_sites = np.zeros(self.n_chains, dtype=np.int64)
+        self.kinds = np.zeros(self.n_chains, dtype=np.int32)
+        self.temperatures = np.zeros(self.n_chains)
+        self.active = np.zeros(self.n_chains, dtype=bool)
+
+    @staticmethod
+    def footprint(model, kind):
+        """(occupation bytes, tracker entries) one chain of `model` takes."""
+        o, t = ctypes.c_int64(), ctypes.c_int64()
+        _capi.check(_capi.load().cemc_pool_footprint(model.handle, int(kind), ctypes.byref(o), ctypes.byref(t)))
+        return o.value, t.value
+
+    def add(self, model, chain):
+        """Admit one chain (dict as for ChainBatch); returns its slot."""
+        d = _capi.ChainDesc()
+        keep = []
+        _fill_desc(d, model, chain, keep)
+        slot = ctypes.c_int32(-1)
+        _capi.check(self._lib.cemc_pool_add(self.handle, model.handle, ctypes.byref(d), ctypes.byref(slot)))
+        c = slot.value
+        self.models[c] = model
+        self.n_sites[c] = model.n_sites
+        self.kinds[c] = d.kind
+        self.temperatures[c] = d.temperature
+        self.active[c] = True
+        return c
+
+    def remove(self, slot):
+        _capi.check(self._lib.cemc_pool_remove(self.handle, int(slot)))
+        self.active[slot] = False
+        self.models[slot] = None
+        self.n_sites[slot] = 0

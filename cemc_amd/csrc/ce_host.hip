@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -36,6 +37,11 @@ struct DeviceError : std::runtime_error {
     if (err_ != hipSuccess) throw DeviceError(std::string(#expr) + ": " + hipGetErrorString(err_)); \
   } while (0)
 
+// A pool without room for a new chain (slot or device memory).
+struct PoolFull : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
 template <class F>
 int guarded(F&& f) {
   try {
@@ -47,6 +53,9 @@ int guarded(F&& f) {
   } catch (const DeviceError& e) {
     g_last_error = e.what();
     return CEMC_EDEVICE;
+  } catch (const PoolFull& e) {
+    g_last_error = e.what();
+    return CEMC_EFULL;
   } catch (const std::exception& e) {
     g_last_error = e.what();
     return CEMC_ERUNTIME;
@@ -531,6 +540,35 @@ static void compute_cf(const ModelDev* d_models, const std::vector<ChainParams>&
   dev_free(d_e);
 }
 
+// Energies of chains from scratch through the energy tables (the path the
+// batched sweep uses; correlation functions are not formed).
+static void compute_energy(const ModelDev* d_models, const std::vector<ChainParams>& params_host,
+                           const ChainParams* d_params, const int8_t* d_occ_pool, const std::vector<int32_t>& chains,
+                           std::vector<double>* energy_out, hipStream_t stream) {
+  int max_sites = 1;
+  for (int c : chains) max_sites = std::max(max_sites, params_host[c].n_sites);
+  const int n_blocks = (max_sites + 255) / 256;
+  const int nc = int(chains.size());
+  int32_t* d_ids = upload(chains);
+  double *d_partial = nullptr, *d_e = nullptr;
+  try {
+    HIP_CHECK(hipMalloc(&d_partial, sizeof(double) * size_t(nc) * n_blocks));
+    HIP_CHECK(hipMalloc(&d_e, sizeof(double) * size_t(nc)));
+    HIP_CHECK(launch_energy(d_models, d_params, d_occ_pool, d_ids, nc, max_sites, d_partial, d_e, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    energy_out->resize(nc);
+    HIP_CHECK(hipMemcpy(energy_out->data(), d_e, sizeof(double) * nc, hipMemcpyDeviceToHost));
+  } catch (...) {
+    dev_free(d_ids);
+    dev_free(d_partial);
+    dev_free(d_e);
+    throw;
+  }
+  dev_free(d_ids);
+  dev_free(d_partial);
+  dev_free(d_e);
+}
+
 // ============================================================================
 struct cemc_updater {
   cemc_model* model = nullptr;
@@ -655,6 +693,12 @@ struct cemc_batch {
   std::vector<Class> classes;
   int64_t classes_version = 0;  // bumped whenever the launch classes change
   int forced_waves = 0;
+  size_t order_cap = 0;         // entries allocated at d_order
+  // pool mode: chains join and leave; slots [0, n_chains) with active flags
+  bool dynamic = false;
+  std::vector<char> active;
+  int model_capacity = 0;
+  std::map<int64_t, int64_t> occ_free, trk_free;  // free ranges: offset -> length
   int n_cf_max = 1;
   ~cemc_batch() {
     dev_free(d_models);
@@ -705,11 +749,94 @@ struct cemc_pt {
 
 namespace {
 
+// Host-side parameters of chain c on model m (index mid in the batch's
+// model list) and, for canonical chains, its species index tracker: site
+// lists per species in ascending site order (swap_move_index_tracker.py:22-33).
+void setup_chain(cemc_batch* b, int c, cemc_model* m, int mid, const cemc_chain_desc& d, std::vector<uint16_t>& trk) {
+  require(d.occupation != nullptr, "NULL occupation");
+  require(d.kind == CEMC_MOVE_CANONICAL || d.kind == CEMC_MOVE_SGC, "unknown move kind");
+  require(d.temperature > 0.0 && std::isfinite(d.temperature), "temperature must be positive");
+  ChainParams& p = b->params[c];
+  std::memset(&p, 0, sizeof p);
+  p.model = mid;
+  p.kind = d.kind;
+  p.n_sites = m->n_sites;
+  p.kT = d.temperature * kBoltzmannEV;
+  p.seed = d.seed;
+  p.uid = d.uid;
+  b->n_cf_of_chain[c] = m->n_cf;
+  b->n_cf_max = std::max(b->n_cf_max, m->n_cf);
+  for (int k = 0; k < m->n_cf; ++k)
+    if (m->cf_kind[k] == CEMC_CF_SINGLET) p.j1[m->cf_dec[k]] = m->eci[k];
+  if (d.chem_pot)
+    for (int dd = 0; dd < m->n_bfs; ++dd) p.j1[dd] -= d.chem_pot[dd];
+  const int N = m->n_sites;
+  for (int i = 0; i < N; ++i)
+    require(d.occupation[i] >= 0 && d.occupation[i] < m->n_species, "occupation species id out of range");
+  trk.clear();
+  if (d.kind == CEMC_MOVE_CANONICAL) {
+    require(N <= 65535, "canonical chains support at most 65535 sites");
+    int off = 0;
+    for (int sp = 0; sp < m->n_species; ++sp) {
+      int cnt = 0;
+      for (int i = 0; i < N; ++i)
+        if (!m->background[i] && d.occupation[i] == sp) {
+          trk.push_back(uint16_t(i));
+          ++cnt;
+        }
+      p.seg_off[sp] = off;
+      p.count[sp] = cnt;
+      off += cnt;
+      if (cnt > 0) p.present[p.n_present++] = sp;
+    }
+    require(p.n_present >= 2, "canonical moves need at least two species present");
+    for (int k = 0; k < p.n_present; ++k) {
+      const int a = p.present[k];
+      p.present_pk |= uint32_t(a) << (4 * k);
+      p.count_pk[k >> 2] |= uint64_t(uint32_t(p.count[a]) & 0xFFFFu) << (16 * (k & 3));
+      p.segoff_pk[k >> 2] |= uint64_t(uint32_t(p.seg_off[a]) & 0xFFFFu) << (16 * (k & 3));
+    }
+    trk.resize(N, 0);
+  }
+}
+
+// First-fit allocation of `len` units from a free-range map; -1 when no range fits.
+int64_t range_alloc(std::map<int64_t, int64_t>& free_map, int64_t len) {
+  if (len <= 0) return 0;
+  for (auto it = free_map.begin(); it != free_map.end(); ++it) {
+    if (it->second < len) continue;
+    const int64_t off = it->first, rest = it->second - len;
+    free_map.erase(it);
+    if (rest > 0) free_map[off + len] = rest;
+    return off;
+  }
+  return -1;
+}
+
+void range_free(std::map<int64_t, int64_t>& free_map, int64_t off, int64_t len) {
+  if (len <= 0) return;
+  auto it = free_map.emplace(off, len).first;
+  auto next = std::next(it);
+  if (next != free_map.end() && it->first + it->second == next->first) {
+    it->second += next->second;
+    free_map.erase(next);
+  }
+  if (it != free_map.begin()) {
+    auto prev = std::prev(it);
+    if (prev->first + prev->second == it->first) {
+      prev->second += it->second;
+      free_map.erase(it);
+    }
+  }
+}
+
 // Launch classes: chains that share a sweep-kernel variant, each class
 // ordered longest chain first.  forced_waves > 0 fixes the waves per chain
 // (where the variant allows it); 0 chooses them from the batch size.
-void derive_classes(cemc_batch* b, int forced_waves) {
+void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>* nm = nullptr) {
   const int n_chains = b->n_chains;
+  // pool batches launch only the active chains that have moves to make
+  auto included = [&](int c) { return !b->dynamic || (b->active[c] && nm && (*nm)[c] > 0); };
   // launch classes: chains sharing a kernel variant, longest first
   std::vector<int32_t> order;
   b->classes.clear();
@@ -732,10 +859,17 @@ void derive_classes(cemc_batch* b, int forced_waves) {
     HIP_CHECK(hipGetDevice(&dev));
     HIP_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
   }
+  int n_launch = 0;
+  for (int c = 0; c < n_chains; ++c) n_launch += included(c) ? 1 : 0;
   int batch_waves = 1;
-  while (batch_waves < 8 && int64_t(n_chains) * batch_waves * 2 <= int64_t(4) * n_cu) batch_waves *= 2;
+  while (batch_waves < 8 && int64_t(n_launch) * batch_waves * 2 <= int64_t(4) * n_cu) batch_waves *= 2;
   std::vector<Key> keys(n_chains);
+  std::vector<bool> done(n_chains, false);
   for (int c = 0; c < n_chains; ++c) {
+    if (!included(c)) {
+      done[c] = true;
+      continue;
+    }
     const cemc_model* m = b->models[b->params[c].model];
     const bool canon = b->params[c].kind == CEMC_MOVE_CANONICAL;
     Key k;
@@ -774,7 +908,6 @@ void derive_classes(cemc_batch* b, int forced_waves) {
     }
     keys[c] = k;
   }
-  std::vector<bool> done(n_chains, false);
   for (int c0 = 0; c0 < n_chains; ++c0) {
     if (done[c0]) continue;
     const Key& k0 = keys[c0];
@@ -798,9 +931,15 @@ void derive_classes(cemc_batch* b, int forced_waves) {
                                            int(order.size()), int(ids.size()), lds});
     order.insert(order.end(), ids.begin(), ids.end());
   }
-  dev_free(b->d_order);
-  b->d_order = nullptr;
-  b->d_order = upload(order);
+  if (order.size() > b->order_cap || b->d_order == nullptr) {
+    dev_free(b->d_order);
+    b->d_order = nullptr;
+    b->order_cap = std::max<size_t>(order.size(), size_t(n_chains));
+    HIP_CHECK(hipMalloc(&b->d_order, sizeof(int32_t) * b->order_cap));
+  }
+  if (!order.empty())
+    HIP_CHECK(hipMemcpyAsync(b->d_order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice,
+                             b->stream));
   b->classes_version += 1;
   uint32_t need = 0;
   for (const auto& cl : b->classes) need = std::max(need, cl.lds);
@@ -846,6 +985,7 @@ void batch_launch(cemc_batch* b, const int64_t* n_moves, int64_t moves, bool tim
     HIP_CHECK(hipMalloc(&d_log, sizeof(uint32_t) * size_t(b->n_chains) * log_words));
     HIP_CHECK(hipMemsetAsync(d_log, 0, sizeof(uint32_t) * size_t(b->n_chains) * log_words, b->stream));
   }
+  if (b->dynamic) derive_classes(b, b->forced_waves, &nm);
   HIP_CHECK(hipMemcpyAsync(b->d_nmoves, nm.data(), sizeof(int64_t) * nm.size(), hipMemcpyHostToDevice, b->stream));
   if (timed) HIP_CHECK(hipEventRecord(b->ev0, b->stream));
   enqueue_sweeps(b, d_log, log_words);
@@ -1160,59 +1300,18 @@ int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const c
       for (int c = 0; c < n_chains; ++c) {
         cemc_model* m = chain_models[c];
         require(m != nullptr, "NULL model");
-        const cemc_chain_desc& d = chains[c];
-        require(d.occupation != nullptr, "NULL occupation");
-        require(d.kind == CEMC_MOVE_CANONICAL || d.kind == CEMC_MOVE_SGC, "unknown move kind");
-        require(d.temperature > 0.0 && std::isfinite(d.temperature), "temperature must be positive");
         auto it = std::find(b->models.begin(), b->models.end(), m);
         const int mid = int(it - b->models.begin());
         if (it == b->models.end()) b->models.push_back(m);
+        std::vector<uint16_t> trk;
+        setup_chain(b, c, m, mid, chains[c], trk);
         ChainParams& p = b->params[c];
-        std::memset(&p, 0, sizeof p);
-        p.model = mid;
-        p.kind = d.kind;
-        p.n_sites = m->n_sites;
-        p.kT = d.temperature * kBoltzmannEV;
-        p.seed = d.seed;
-        p.uid = d.uid;
-        b->n_cf_of_chain[c] = m->n_cf;
-        b->n_cf_max = std::max(b->n_cf_max, m->n_cf);
-        for (int k = 0; k < m->n_cf; ++k)
-          if (m->cf_kind[k] == CEMC_CF_SINGLET) p.j1[m->cf_dec[k]] = m->eci[k];
-        if (d.chem_pot)
-          for (int dd = 0; dd < m->n_bfs; ++dd) p.j1[dd] -= d.chem_pot[dd];
         const int N = m->n_sites;
-        for (int i = 0; i < N; ++i)
-          require(d.occupation[i] >= 0 && d.occupation[i] < m->n_species, "occupation species id out of range");
         p.occ_off = int64_t(occ_pool.size());
-        occ_pool.insert(occ_pool.end(), d.occupation, d.occupation + N);
+        occ_pool.insert(occ_pool.end(), chains[c].occupation, chains[c].occupation + N);
         occ_pool.resize((occ_pool.size() + 15) & ~size_t(15), 0);
         p.trk_off = int64_t(trk_pool.size());
-        if (d.kind == CEMC_MOVE_CANONICAL) {
-          require(N <= 65535, "canonical chains support at most 65535 sites");
-          // species index tracker in ascending site order (swap_move_index_tracker.py:22-33)
-          int off = 0;
-          for (int s = 0; s < m->n_species; ++s) {
-            int cnt = 0;
-            for (int i = 0; i < N; ++i)
-              if (!m->background[i] && d.occupation[i] == s) {
-                trk_pool.push_back(uint16_t(i));
-                ++cnt;
-              }
-            p.seg_off[s] = off;
-            p.count[s] = cnt;
-            off += cnt;
-            if (cnt > 0) p.present[p.n_present++] = s;
-          }
-          require(p.n_present >= 2, "canonical moves need at least two species present");
-          for (int k = 0; k < p.n_present; ++k) {
-            const int a = p.present[k];
-            p.present_pk |= uint32_t(a) << (4 * k);
-            p.count_pk[k >> 2] |= uint64_t(uint32_t(p.count[a]) & 0xFFFFu) << (16 * (k & 3));
-            p.segoff_pk[k >> 2] |= uint64_t(uint32_t(p.seg_off[a]) & 0xFFFFu) << (16 * (k & 3));
-          }
-          trk_pool.resize(p.trk_off + N, 0);
-        }
+        trk_pool.insert(trk_pool.end(), trk.begin(), trk.end());
       }
       std::vector<ModelDev> views;
       for (auto* m : b->models) views.push_back(m->view);
@@ -1223,11 +1322,11 @@ int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const c
       HIP_CHECK(hipMalloc(&b->d_states, sizeof(ChainState) * n_chains));
       HIP_CHECK(hipMalloc(&b->d_nmoves, sizeof(int64_t) * n_chains));
       derive_classes(b, 0);
-      // initial energies from correlation functions computed on the GPU
+      // initial energies from scratch through the energy tables
       std::vector<int32_t> all(n_chains);
       std::iota(all.begin(), all.end(), 0);
       std::vector<double> energies;
-      compute_cf(b->d_models, b->params, b->d_params, b->d_occ, all, b->n_cf_max, nullptr, &energies, b->stream);
+      compute_energy(b->d_models, b->params, b->d_params, b->d_occ, all, &energies, b->stream);
       std::vector<ChainState> st(n_chains);
       std::memset(st.data(), 0, sizeof(ChainState) * st.size());
       for (int c = 0; c < n_chains; ++c) {
@@ -1283,6 +1382,7 @@ int cemc_batch_get_occupation(cemc_batch* b, int32_t chain, int8_t* occupation) 
   return guarded([&] {
     require(b && occupation, "NULL argument");
     require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    require(!b->dynamic || b->active[chain], "slot is not in use");
     const ChainParams& p = b->params[chain];
     HIP_CHECK(hipMemcpy(occupation, b->d_occ + p.occ_off, size_t(p.n_sites), hipMemcpyDeviceToHost));
   });
@@ -1326,6 +1426,7 @@ int cemc_batch_get_thermo(cemc_batch* b, int32_t chain, cemc_thermo* out) {
   return guarded([&] {
     require(b && out, "NULL argument");
     require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    require(!b->dynamic || b->active[chain], "slot is not in use");
     ChainState x;
     HIP_CHECK(hipMemcpy(&x, b->d_states + chain, sizeof(ChainState), hipMemcpyDeviceToHost));
     std::memset(out, 0, sizeof(cemc_thermo));
@@ -1362,6 +1463,7 @@ int cemc_batch_get_cf(cemc_batch* b, int32_t chain, double* cf, double* energy) 
   return guarded([&] {
     require(b != nullptr, "NULL batch");
     require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    require(!b->dynamic || b->active[chain], "slot is not in use");
     std::vector<double> cfv, ev;
     compute_cf(b->d_models, b->params, b->d_params, b->d_occ, {chain}, b->n_cf_max, &cfv, &ev, b->stream);
     if (cf) std::copy(cfv.begin(), cfv.begin() + b->n_cf_of_chain[chain], cf);
@@ -1374,7 +1476,7 @@ int cemc_batch_set_waves(cemc_batch* b, int32_t waves) {
     require(b != nullptr, "NULL batch");
     require(waves == 0 || waves == 1 || waves == 2 || waves == 4 || waves == 8, "waves must be 0, 1, 2, 4 or 8");
     HIP_CHECK(hipStreamSynchronize(b->stream));
-    derive_classes(b, waves);
+    if (!b->dynamic) derive_classes(b, waves);
     b->forced_waves = waves;
   });
 }
@@ -1408,11 +1510,133 @@ int cemc_batch_set_temperature(cemc_batch* b, int32_t chain, double temperature)
   });
 }
 
+// ----------------------------------------------------------------- pool
+int cemc_pool_create(int32_t max_chains, int32_t max_models, int64_t occ_bytes, int64_t tracker_entries,
+                     cemc_batch** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    *out = nullptr;
+    require(max_chains > 0 && max_models > 0 && occ_bytes > 0 && tracker_entries >= 0, "bad pool sizes");
+    auto* b = new cemc_batch();
+    try {
+      b->dynamic = true;
+      b->n_chains = max_chains;
+      b->model_capacity = max_models;
+      b->active.assign(max_chains, 0);
+      b->params.assign(max_chains, ChainParams{});
+      for (auto& p : b->params) std::memset(&p, 0, sizeof p);
+      b->n_cf_of_chain.assign(max_chains, 0);
+      HIP_CHECK(hipStreamCreate(&b->stream));
+      HIP_CHECK(hipEventCreate(&b->ev0));
+      HIP_CHECK(hipEventCreate(&b->ev1));
+      HIP_CHECK(hipMalloc(&b->d_models, sizeof(ModelDev) * size_t(max_models)));
+      HIP_CHECK(hipMalloc(&b->d_params, sizeof(ChainParams) * size_t(max_chains)));
+      HIP_CHECK(hipMalloc(&b->d_states, sizeof(ChainState) * size_t(max_chains)));
+      HIP_CHECK(hipMalloc(&b->d_nmoves, sizeof(int64_t) * size_t(max_chains)));
+      const int64_t occ_cap = (occ_bytes + 15) & ~int64_t(15);
+      HIP_CHECK(hipMalloc(&b->d_occ, size_t(occ_cap)));
+      HIP_CHECK(hipMalloc(&b->d_trk, sizeof(uint16_t) * size_t(std::max<int64_t>(tracker_entries, 8))));
+      b->occ_free[0] = occ_cap;
+      if (tracker_entries > 0) b->trk_free[0] = tracker_entries;
+      std::vector<ChainState> st(max_chains);
+      std::memset(st.data(), 0, sizeof(ChainState) * st.size());
+      HIP_CHECK(hipMemcpy(b->d_states, st.data(), sizeof(ChainState) * st.size(), hipMemcpyHostToDevice));
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+int cemc_pool_footprint(const cemc_model* m, int32_t kind, int64_t* occ_bytes, int64_t* tracker_entries) {
+  return guarded([&] {
+    require(m != nullptr, "NULL model");
+    if (occ_bytes) *occ_bytes = (int64_t(m->n_sites) + 15) & ~int64_t(15);
+    if (tracker_entries) *tracker_entries = kind == CEMC_MOVE_CANONICAL ? int64_t(m->n_sites) : 0;
+  });
+}
+
+int cemc_pool_add(cemc_batch* b, cemc_model* m, const cemc_chain_desc* d, int32_t* slot_out) {
+  return guarded([&] {
+    require(b && m && d && slot_out, "NULL argument");
+    require(b->dynamic, "not a pool");
+    int slot = -1;
+    for (int c = 0; c < b->n_chains; ++c)
+      if (!b->active[c]) {
+        slot = c;
+        break;
+      }
+    if (slot < 0) throw PoolFull("no free chain slot in the pool");
+    // the chain's state must fit one workgroup's LDS without energy tables
+    const int occ_lds = m->periodic ? m->pdims[0] * m->pdims[1] * m->pdims[2] : m->n_sites;
+    const uint32_t need = sweep_lds_bytes(occ_lds, d->kind == CEMC_MOVE_CANONICAL ? m->n_sites : 0, 0, 0);
+    require(need <= lds_limit(), "chain state does not fit in the workgroup's LDS");
+    auto it = std::find(b->models.begin(), b->models.end(), m);
+    int mid = int(it - b->models.begin());
+    if (it == b->models.end()) {
+      if (int(b->models.size()) >= b->model_capacity) throw PoolFull("no free model slot in the pool");
+      b->models.push_back(m);
+      HIP_CHECK(hipMemcpy(b->d_models + mid, &m->view, sizeof(ModelDev), hipMemcpyHostToDevice));
+    }
+    std::vector<uint16_t> trk;
+    setup_chain(b, slot, m, mid, *d, trk);
+    ChainParams& p = b->params[slot];
+    const int64_t occ_len = (int64_t(m->n_sites) + 15) & ~int64_t(15);
+    const int64_t occ_off = range_alloc(b->occ_free, occ_len);
+    if (occ_off < 0) throw PoolFull("no room for the occupation in the pool");
+    int64_t trk_off = 0;
+    if (!trk.empty()) {
+      trk_off = range_alloc(b->trk_free, int64_t(trk.size()));
+      if (trk_off < 0) {
+        range_free(b->occ_free, occ_off, occ_len);
+        throw PoolFull("no room for the species tracker in the pool");
+      }
+    }
+    p.occ_off = occ_off;
+    p.trk_off = trk_off;
+    try {
+      HIP_CHECK(hipStreamSynchronize(b->stream));
+      HIP_CHECK(hipMemcpy(b->d_occ + occ_off, d->occupation, size_t(m->n_sites), hipMemcpyHostToDevice));
+      if (!trk.empty())
+        HIP_CHECK(hipMemcpy(b->d_trk + trk_off, trk.data(), sizeof(uint16_t) * trk.size(), hipMemcpyHostToDevice));
+      HIP_CHECK(hipMemcpy(b->d_params + slot, &p, sizeof(ChainParams), hipMemcpyHostToDevice));
+      std::vector<double> energies;
+      compute_energy(b->d_models, b->params, b->d_params, b->d_occ, {slot}, &energies, b->stream);
+      ChainState st;
+      std::memset(&st, 0, sizeof st);
+      st.energy = st.e_ref = energies[0];
+      st.moves_done = d->first_move;
+      HIP_CHECK(hipMemcpy(b->d_states + slot, &st, sizeof(ChainState), hipMemcpyHostToDevice));
+    } catch (...) {
+      range_free(b->occ_free, occ_off, occ_len);
+      if (!trk.empty()) range_free(b->trk_free, trk_off, int64_t(trk.size()));
+      throw;
+    }
+    b->active[slot] = 1;
+    *slot_out = slot;
+  });
+}
+
+int cemc_pool_remove(cemc_batch* b, int32_t slot) {
+  return guarded([&] {
+    require(b != nullptr, "NULL pool");
+    require(b->dynamic, "not a pool");
+    require(slot >= 0 && slot < b->n_chains && b->active[slot], "slot is not in use");
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    ChainParams& p = b->params[slot];
+    range_free(b->occ_free, p.occ_off, (int64_t(p.n_sites) + 15) & ~int64_t(15));
+    if (p.kind == CEMC_MOVE_CANONICAL) range_free(b->trk_free, p.trk_off, int64_t(p.n_sites));
+    b->active[slot] = 0;
+  });
+}
+
 // ------------------------------------------------------------ tempering
 int cemc_pt_create(cemc_batch* b, uint64_t seed, uint64_t uid, cemc_pt** out) {
   return guarded([&] {
     require(b && out, "NULL argument");
     *out = nullptr;
+    require(!b->dynamic, "parallel tempering runs on a fixed batch, not a pool");
     const int R = b->n_chains;
     require(R >= 2, "parallel tempering needs at least two replicas");
     auto* pt = new cemc_pt();

@@ -41,6 +41,9 @@ hipError_t launch_pt_exchange(PTDev* pt, ChainState* states, ChainParams* params
 hipError_t launch_cf(const ModelDev* models, const ChainParams* params, const int8_t* occ_pool,
                      const int32_t* chain_ids, int n_chains, int max_sites, double* partial, double* cf_out,
                      double* energy_out, hipStream_t stream);
+hipError_t launch_energy(const ModelDev* models, const ChainParams* params, const int8_t* occ_pool,
+                         const int32_t* chain_ids, int n_chains, int max_sites, double* partial, double* energy_out,
+                         hipStream_t stream);
 hipError_t launch_updater(const ModelDev* model, int8_t* occ, double* ring, double* slot_energy,
                           const int32_t* changes, int n, const double* eci, double* scratch, hipStream_t stream);
 hipError_t launch_restore(int8_t* occ, const int32_t* sites, const int32_t* species, int n, hipStream_t stream);

@@ -94,6 +94,79 @@ __global__ __launch_bounds__(256) void cf_partial_kernel(const ModelDev* __restr
   }
 }
 
+// Energy from scratch through the energy tables.  A sub-cluster of site i
+// with i at canonical position r contributes sum_d phi_d(s_i) G[r][d][code],
+// which is its cluster's full energy term times its size n (each cluster is
+// listed once for each of its n members, and the table weights carry the
+// factor n of the incremental update, ce_updater.cpp:400), so
+//   E = N J_c0 + sum_i [ sum_d j1_d phi_d(s_i) + sum_sub (1/n) sum_d phi_d(s_i) G ]
+// with the singlet term over non-background sites.  Block b of chain c writes
+// the sum over sites [256 b, 256 b + 256) to partial[c * n_blocks + b].
+__global__ __launch_bounds__(256) void energy_partial_kernel(const ModelDev* __restrict__ models,
+                                                             const ChainParams* __restrict__ params,
+                                                             const int8_t* __restrict__ occ_pool,
+                                                             const int32_t* __restrict__ chain_ids,
+                                                             double* __restrict__ partial, int n_blocks) {
+  __shared__ double red[4];
+  const int c = int(blockIdx.y);
+  const ChainParams* pp = params + chain_ids[c];
+  const ModelDev& m = models[pp->model];
+  const int8_t* occ = occ_pool + pp->occ_off;
+  const int N = pp->n_sites;
+  const int tid = int(threadIdx.x);
+  const int i = int(blockIdx.x) * 256 + tid;
+  const int S = m.n_species, B = m.n_bfs;
+  const int kbits = S <= 2 ? 1 : (S <= 4 ? 2 : 3);
+  const int pc = 1 << (3 * kbits);
+  double v = 0.0;
+  if (i < N) {
+    const int si = occ[i];
+    if (!(m.has_background && m.background[i]))
+      for (int d = 0; d < B; ++d) v += pp->j1[d] * m.basis[d * S + si];
+    const int g = site_group(m, i);
+    if (g >= 0) {
+      double phi[kMaxBasis];
+      for (int d = 0; d < B; ++d) phi[d] = m.basis[d * S + si];
+      const int32_t* row = m.trans + int64_t(i) * m.K;
+      const int r0 = m.grp_rec_off[g], r1 = r0 + m.grp_rec_cnt[g];
+      for (int r = r0; r < r1; ++r) {
+        const SubRec rec = m.recs[r];
+        const int nm = rec_size(rec) - 1;
+        uint32_t code = 0;
+        for (int q = 0; q < nm; ++q) code |= uint32_t(occ[row[rec_slot(rec, q)]]) << (kbits * q);
+        const double* gt = m.gtab + rec_gbase(rec) + code;
+        double e = 0.0;
+        for (int d = 0; d < B; ++d) e += phi[d] * gt[d * pc];
+        v += e / double(nm + 1);
+      }
+    }
+  }
+  v = wave_sum_f64(v);
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  if (tid == 0) partial[int64_t(c) * n_blocks + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(64) void energy_finalize_kernel(const ModelDev* __restrict__ models,
+                                                             const ChainParams* __restrict__ params,
+                                                             const int32_t* __restrict__ chain_ids,
+                                                             const double* __restrict__ partial, int n_blocks,
+                                                             double* __restrict__ energy_out) {
+  const int c = int(blockIdx.x);
+  const ChainParams* pp = params + chain_ids[c];
+  const ModelDev& m = models[pp->model];
+  const int nb = (pp->n_sites + 255) / 256;
+  double v = 0.0;
+  for (int b = int(threadIdx.x); b < nb; b += 64) v += partial[int64_t(c) * n_blocks + b];
+  v = wave_sum_f64(v);
+  if (threadIdx.x == 0) {
+    double j0 = 0.0;
+    for (int k = 0; k < m.n_cf; ++k)
+      if (m.cf_kind[k] == 0) j0 += m.eci[k];
+    energy_out[c] = v + double(pp->n_sites) * j0;
+  }
+}
+
 // Sums the block partials, sets c0 = 1 and computes E = N * sum_k J_k cf_k
 // (ce_updater.cpp:236-242) with the chain's singlet ECIs.
 __global__ __launch_bounds__(64) void cf_finalize_kernel(const ModelDev* __restrict__ models,
@@ -397,6 +470,17 @@ hipError_t launch_cf(const ModelDev* models, const ChainParams* params, const in
                      occ_pool, chain_ids, partial, n_blocks);
   hipLaunchKernelGGL(cf_finalize_kernel, dim3(n_chains), dim3(64), 0, stream, models, params, chain_ids,
                      partial, n_blocks, cf_out, energy_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy(const ModelDev* models, const ChainParams* params, const int8_t* occ_pool,
+                         const int32_t* chain_ids, int n_chains, int max_sites, double* partial, double* energy_out,
+                         hipStream_t stream) {
+  const int n_blocks = (max_sites + 255) / 256;
+  hipLaunchKernelGGL(energy_partial_kernel, dim3(n_blocks, n_chains), dim3(256), 0, stream, models, params,
+                     occ_pool, chain_ids, partial, n_blocks);
+  hipLaunchKernelGGL(energy_finalize_kernel, dim3(n_chains), dim3(64), 0, stream, models, params, chain_ids,
+                     partial, n_blocks, energy_out);
   return hipGetLastError();
 }
 

@@ -27,7 +27,8 @@ enum {
   CEMC_OK = 0,
   CEMC_EINVAL = 1,   /* maps to ValueError   (std::invalid_argument in the reference) */
   CEMC_ERUNTIME = 2, /* maps to RuntimeError (std::runtime_error in the reference)    */
-  CEMC_EDEVICE = 3   /* HIP runtime failure                                         */
+  CEMC_EDEVICE = 3,  /* HIP runtime failure                                         */
+  CEMC_EFULL = 4     /* a pool has no free slot or memory for one more chain        */
 };
 
 enum { CEMC_CF_C0 = 0, CEMC_CF_SINGLET = 1, CEMC_CF_CLUSTER = 2 };
@@ -218,6 +219,25 @@ CEMC_API int cemc_batch_get_launch_shape(cemc_batch* batch, int32_t chain, int32
                                          int32_t* records_per_lane, int32_t* tables_in_lds);
 /* Temperature of one chain (parallel tempering uses it to exchange replicas). */
 CEMC_API int cemc_batch_set_temperature(cemc_batch* batch, int32_t chain, double temperature);
+
+/* ------------------------------------------------------------------ pool
+ * A batch whose chains join and leave while it runs: the resident state of
+ * a job server.  Slots, model entries and device memory for occupations
+ * (1 byte per site, rounded up to 16) and canonical species trackers (one
+ * entry per site) are fixed at creation; a chain is admitted only when its
+ * footprint fits (CEMC_EFULL otherwise) and its lattice then stays on the
+ * GPU until it is removed.  cemc_batch_run* on a pool launches only the
+ * active slots with a non-zero move count; the other cemc_batch_get_* and
+ * cemc_batch_reset_stats calls take slot numbers as chain indices.  The
+ * reference has no counterpart (one sampler per Python process); a chain's
+ * moves are those of the same chain run alone. */
+CEMC_API int cemc_pool_create(int32_t max_chains, int32_t max_models, int64_t occ_bytes, int64_t tracker_entries,
+                              cemc_batch** out);
+/* Device memory one chain of `model` needs (kind: CEMC_MOVE_*). */
+CEMC_API int cemc_pool_footprint(const cemc_model* model, int32_t kind, int64_t* occ_bytes,
+                                 int64_t* tracker_entries);
+CEMC_API int cemc_pool_add(cemc_batch* pool, cemc_model* model, const cemc_chain_desc* chain, int32_t* slot);
+CEMC_API int cemc_pool_remove(cemc_batch* pool, int32_t slot);
 
 /* ------------------------------------------------------------ tempering
  * Parallel tempering over the chains of a batch

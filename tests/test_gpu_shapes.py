@@ -84,6 +84,10 @@ def test_mixed_batch_matches_each_fixture(gpu):
     models = [CEModel(fx.settings, fx.eci) for fx in fxs]
     chains = [_chain(fx) for fx in fxs]
     batch = ChainBatch(models, chains)
+    # initial energies (energy tables) equal N J.cf from the CF kernel
+    for c, fx in enumerate(fxs):
+        _, e_cf = batch.cf(c)
+        assert abs(batch.energies()[c] - e_cf) <= 1e-12 * fx.n_sites
     n_moves = np.array([fx.n_moves for fx in fxs], dtype=np.int64)
     log = batch.run_logged(n_moves)
     for c, fx in enumerate(fxs):
