@@ -86,7 +86,10 @@ def ternary_uneven(n_chains=256, sweeps=200, seed=3):
         occ = random_occupation(st.n_sites, 3, 5000 + c, fractions=list(f))
         chains.append(dict(kind=CANONICAL, temperature=float(temps[c]), occupation=occ, seed=500, uid=c))
         chain_models.append(model)
+    from cemc_amd import autotune
+    tune_w, tune_rates = autotune.tune(chain_models, chains, candidates=(1, 2, 4, 8), sweeps=1.0, log=log)
     batch = ChainBatch(chain_models, chains)
+    batch.set_waves(tune_w)
     moves = np.array([sweeps * m.n_sites for m in chain_models], dtype=np.int64)
     batch.run(moves // 20)  # warm-up: a tenth of a step
     t0 = time.perf_counter()
@@ -99,6 +102,7 @@ def ternary_uneven(n_chains=256, sweeps=200, seed=3):
                         "triplets<=3rd, quadruplets<=2nd shell, %d sweeps each, one launch" % sweeps,
             "value": total / wall, "unit": "moves/s", "moves": total, "kernel_ms": kernel_ms, "wall_s": wall,
             "bytes_per_move": 10.0 * m0.gathers_per_flip(), "launch_shapes": [list(x) for x in shapes],
+            "autotune_moves_per_s": {str(k): v for k, v in tune_rates.items()},
             "acceptance": float(batch.stats()["accepted"].sum() / max(total + int((moves // 20).sum()), 1))}
 
 
@@ -116,7 +120,11 @@ def tempering(n_rep=128, cycles=2000, L=32, seed=9):
     temps = np.geomspace(200.0, 2000.0, n_rep)
     chains = [dict(kind=CANONICAL, temperature=float(T), occupation=random_occupation(st.n_sites, 4, 7000 + r),
                    seed=seed, uid=r) for r, T in enumerate(temps)]
+    from cemc_amd import autotune
+    tune_w, tune_rates = autotune.tune([model] * n_rep, chains, candidates=(2, 4, 8), sweeps=0.25, repeats=1,
+                                       log=log)
     batch = ChainBatch([model] * n_rep, chains)
+    batch.set_waves(tune_w)
     pt = ReplicaExchange(batch, seed=seed, uid=1 << 40)
     pt.run(2, st.n_sites)  # warm-up and graph build
     t0 = time.perf_counter()
@@ -130,7 +138,25 @@ def tempering(n_rep=128, cycles=2000, L=32, seed=9):
                         % (model.n_cf, model.n_subclusters, n_rep, cycles),
             "value": total / wall, "unit": "moves/s", "moves": total, "kernel_ms": kernel_ms, "wall_s": wall,
             "bytes_per_move": 10.0 * model.gathers_per_flip(), "launch_shape": list(batch.launch_shape(0)),
+            "autotune_moves_per_s": {str(k): v for k, v in tune_rates.items()},
             "exchange_accept_rate": float(acc)}
+
+
+def serving(n_jobs=5000, rate=200.0):
+    """Poisson stream of MC jobs (binary or ternary FCC, L^3 with L in 6..24,
+    canonical or SGC, 100-5000 sweeps) through the continuous-batching job
+    server; latency = completion - arrival on the loop's wall clock."""
+    from cemc_amd.serving import JobServer, synthetic_stream
+    cache = {}
+    t0 = time.perf_counter()
+    jobs = synthetic_stream(n_jobs=n_jobs, rate=rate, seed=4, model_cache=cache)
+    setup = time.perf_counter() - t0
+    server = JobServer(max_chains=4096, hbm_bytes=1 << 30)
+    res = server.serve(jobs, realtime=True, progress=log)
+    res.update({"workload": "%d Poisson jobs at %.0f/s: binary or ternary FCC L^3, L in 6..24, canonical or SGC, "
+                            "T in 300-1500 K, 100-5000 sweeps; continuous batching on one GPU" % (n_jobs, rate),
+                "setup_s": setup, "models": len(cache)})
+    return res
 
 
 def cpu_baseline(settings, eci, chains, target_s=12.0):
@@ -188,9 +214,10 @@ def main():
     ap.add_argument("--sweeps", type=int, default=500)
     ap.add_argument("--chains", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workloads", default="batched,ternary,tempering",
-                    help="comma list of batched, ternary, tempering")
-    ap.add_argument("--pt-cycles", type=int, default=2000)
+    ap.add_argument("--workloads", default="batched,ternary,tempering,serving",
+                    help="comma list of batched, ternary, tempering, serving")
+    ap.add_argument("--pt-cycles", type=int, default=200)
+    ap.add_argument("--serving-jobs", type=int, default=500)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this size first (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -211,7 +238,10 @@ def main():
     log("model: periodic=%s sub-clusters=%d gathers/flip=%d" % (model.periodic, model.n_subclusters,
                                                                 model.gathers_per_flip()))
     chains = batched_chains(settings, args.chains)
+    from cemc_amd import autotune
+    tune_w, tune_rates = autotune.tune([model] * len(chains), chains, candidates=(1, 2, 4), sweeps=5.0, log=log)
     batch = ChainBatch([model] * len(chains), chains)
+    batch.set_waves(tune_w)
     moves_per_chain = args.sweeps * settings.n_sites
     for _ in range(args.warmup):
         batch.run(moves_per_chain)
@@ -257,6 +287,7 @@ def main():
                    "chains": len(chains), "sweeps_per_step": args.sweeps, "sites": settings.n_sites},
         "plain_moves_per_s": plain_value, "batched_over_plain": value / plain_value,
         "roofline": roofline,
+        "launch_shape": {"waves_per_chain": tune_w, "autotune_moves_per_s": {str(k): v for k, v in tune_rates.items()}},
     }
     work = set(w.strip() for w in args.workloads.split(","))
     if "ternary" in work:
@@ -267,6 +298,11 @@ def main():
         out["parallel_tempering"] = tempering(cycles=args.pt_cycles)
         log("tempering: %.3e moves/s (kernel %.1f ms)" % (out["parallel_tempering"]["value"],
                                                           out["parallel_tempering"]["kernel_ms"]))
+    if "serving" in work:
+        out["serving"] = serving(n_jobs=args.serving_jobs)
+        log("serving: %d jobs, p50 %.2f s, p99 %.2f s, %.3e moves/s" % (
+            out["serving"]["jobs"], out["serving"]["latency_p50_s"], out["serving"]["latency_p99_s"],
+            out["serving"]["throughput_moves_per_s"]))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(settings, eci, chains)
         log("cpu baseline: %.3e moves/s on %d threads" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["cores"]))

@@ -882,18 +882,21 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
     const int trk_sites = canon ? m->n_sites : 0;
     const uint32_t with_g = sweep_lds_bytes(occ_bytes, trk_sites, 0, int(m->gtab.size()));
     const uint32_t no_g = sweep_lds_bytes(occ_bytes, trk_sites, 0, 0);
-    k.waves = 1;
-    if (forced_waves > 0)
-      k.waves = forced_waves;
-    else
-      while (k.waves < batch_waves && cnt > 128 * k.waves) k.waves *= 2;
-    k.kr = 0;
-    if (single && k.kbm <= 3 && no_g <= limit) {
-      for (int kr : {2, 4, 8})
-        if (cnt <= kr * 64 * k.waves) {
-          k.kr = kr;
-          break;
-        }
+    int auto_waves = 1;
+    while (auto_waves < batch_waves && cnt > 128 * auto_waves) auto_waves *= 2;
+    auto pick_kr = [&](int waves) {
+      if (single && k.kbm <= 3 && no_g <= limit)
+        for (int kr : {2, 4, 8})
+          if (cnt <= kr * 64 * waves) return kr;
+      return 0;
+    };
+    // a forced wave count that leaves the records no room in registers falls
+    // back to the automatic choice for this chain
+    k.waves = forced_waves > 0 ? forced_waves : auto_waves;
+    k.kr = pick_kr(k.waves);
+    if (k.kr == 0 && forced_waves > 0) {
+      k.waves = auto_waves;
+      k.kr = pick_kr(k.waves);
     }
     if (k.kr > 0) {
       k.glds = with_g <= gtab_budget;

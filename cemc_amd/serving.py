@@ -128,13 +128,19 @@ class JobServer:
                 self.done.append(job)
         return len(finished)
 
-    def serve(self, jobs, clock=time.perf_counter, realtime=True):
+    def serve(self, jobs, clock=time.perf_counter, realtime=True, progress=None, progress_every=10.0):
         """Run a stream of jobs (sorted by arrival).  realtime: jobs become
-        visible at their arrival times on the wall clock; otherwise all at once."""
+        visible at their arrival times on the wall clock; otherwise all at
+        once.  progress(msg) is called about every progress_every seconds."""
         pending = deque(sorted(jobs, key=lambda j: j.arrival))
         t0 = clock()
+        next_report = progress_every
         while pending or self.queue or self.running:
             now = clock() - t0
+            if progress is not None and now >= next_report:
+                progress("serving t=%.0f s: %d done, %d running, %d queued, %d not arrived, %.3e moves"
+                         % (now, len(self.done), len(self.running), len(self.queue), len(pending), self.moves_done))
+                next_report = now + progress_every
             while pending and (not realtime or pending[0].arrival <= now):
                 self.submit(pending.popleft())
             self._admit(now)
