@@ -89,7 +89,7 @@ def ternary_uneven(n_chains=256, sweeps=200, seed=3):
     from cemc_amd import autotune
     tune_w, tune_rates = autotune.tune(chain_models, chains, candidates=(1, 2, 4, 8), sweeps=1.0, log=log)
     batch = ChainBatch(chain_models, chains)
-    batch.set_waves(tune_w)
+    autotune.apply(batch, tune_w)
     moves = np.array([sweeps * m.n_sites for m in chain_models], dtype=np.int64)
     batch.run(moves // 20)  # warm-up: a tenth of a step
     t0 = time.perf_counter()
@@ -102,7 +102,7 @@ def ternary_uneven(n_chains=256, sweeps=200, seed=3):
                         "triplets<=3rd, quadruplets<=2nd shell, %d sweeps each, one launch" % sweeps,
             "value": total / wall, "unit": "moves/s", "moves": total, "kernel_ms": kernel_ms, "wall_s": wall,
             "bytes_per_move": 10.0 * m0.gathers_per_flip(), "launch_shapes": [list(x) for x in shapes],
-            "autotune_moves_per_s": {str(k): v for k, v in tune_rates.items()},
+            "autotune_moves_per_s": {"%dx%d" % k: v for k, v in tune_rates.items()},
             "acceptance": float(batch.stats()["accepted"].sum() / max(total + int((moves // 20).sum()), 1))}
 
 
@@ -124,7 +124,7 @@ def tempering(n_rep=128, cycles=2000, L=32, seed=9):
     tune_w, tune_rates = autotune.tune([model] * n_rep, chains, candidates=(2, 4, 8), sweeps=0.25, repeats=1,
                                        log=log)
     batch = ChainBatch([model] * n_rep, chains)
-    batch.set_waves(tune_w)
+    autotune.apply(batch, tune_w)
     pt = ReplicaExchange(batch, seed=seed, uid=1 << 40)
     pt.run(2, st.n_sites)  # warm-up and graph build
     t0 = time.perf_counter()
@@ -138,7 +138,8 @@ def tempering(n_rep=128, cycles=2000, L=32, seed=9):
                         % (model.n_cf, model.n_subclusters, n_rep, cycles),
             "value": total / wall, "unit": "moves/s", "moves": total, "kernel_ms": kernel_ms, "wall_s": wall,
             "bytes_per_move": 10.0 * model.gathers_per_flip(), "launch_shape": list(batch.launch_shape(0)),
-            "autotune_moves_per_s": {str(k): v for k, v in tune_rates.items()},
+            "moves_per_round": tune_w[1],
+            "autotune_moves_per_s": {"%dx%d" % k: v for k, v in tune_rates.items()},
             "exchange_accept_rate": float(acc)}
 
 
@@ -241,7 +242,7 @@ def main():
     from cemc_amd import autotune
     tune_w, tune_rates = autotune.tune([model] * len(chains), chains, candidates=(1, 2, 4), sweeps=5.0, log=log)
     batch = ChainBatch([model] * len(chains), chains)
-    batch.set_waves(tune_w)
+    autotune.apply(batch, tune_w)
     moves_per_chain = args.sweeps * settings.n_sites
     for _ in range(args.warmup):
         batch.run(moves_per_chain)
@@ -287,7 +288,8 @@ def main():
                    "chains": len(chains), "sweeps_per_step": args.sweeps, "sites": settings.n_sites},
         "plain_moves_per_s": plain_value, "batched_over_plain": value / plain_value,
         "roofline": roofline,
-        "launch_shape": {"waves_per_chain": tune_w, "autotune_moves_per_s": {str(k): v for k, v in tune_rates.items()}},
+        "launch_shape": {"waves_per_chain": tune_w[0], "moves_per_round": tune_w[1],
+                         "autotune_moves_per_s": {"%dx%d" % k: v for k, v in tune_rates.items()}},
     }
     work = set(w.strip() for w in args.workloads.split(","))
     if "ternary" in work:

@@ -90,6 +90,7 @@ SIGNATURES = [
     ("cemc_batch_get_thermo", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(Thermo)]),
     ("cemc_batch_get_cf", ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
     ("cemc_batch_set_waves", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("cemc_batch_set_moves_per_round", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("cemc_batch_get_launch_shape", ctypes.c_int, [_vp, ctypes.c_int32, _i32p, _i32p, _i32p]),
     ("cemc_batch_set_temperature", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double]),
     ("cemc_pool_create", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,

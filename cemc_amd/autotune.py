@@ -1,6 +1,7 @@
 """Launch-shape autotuner: picks the number of wavefronts per chain (1, 2, 4
-or 8 cooperating waves) for a batch by timing each candidate on a scratch
-copy of the batch, so the real chains' trajectories are not touched.
+or 8 cooperating waves) and the moves evaluated per round (1 or 8) for a
+batch by timing each candidate on a scratch copy of the batch, so the real
+chains' trajectories are not touched.
 
 The kernel variant (records per lane, energy tables in LDS or HBM) follows
 from the waves and the model; the waves are the free parameter: more waves
@@ -31,48 +32,58 @@ def signature(models, chains):
     return ";".join("%s:%d" % kv for kv in sorted(sig.items()))
 
 
-def tune(models, chains, candidates=(1, 2, 4, 8), sweeps=2.0, repeats=2, cache_file=None, log=None):
-    """Returns (best waves, {waves: moves/s}) for the batch (models, chains).
+def tune(models, chains, candidates=(1, 2, 4, 8), per_round=(1, 8), sweeps=2.0, repeats=2, cache_file=None,
+         log=None):
+    """Returns ((waves, moves per round) of the best shape, {shape: moves/s})
+    for the batch (models, chains).
 
-    Each candidate runs ``sweeps`` sweeps of every chain on a scratch batch
-    built from the same inputs, ``repeats`` times; the best rate counts."""
+    Each candidate shape runs ``sweeps`` sweeps of every chain on a scratch
+    batch built from the same inputs, ``repeats`` times; the best rate counts."""
     sig = signature(models, chains)
     if cache_file and os.path.exists(cache_file):
         with open(cache_file) as f:
             _cache.update(json.load(f))
     if sig in _cache:
         entry = _cache[sig]
-        return int(entry["best"]), {int(k): v for k, v in entry["rates"].items()}
+        return tuple(entry["best"]), {tuple(int(v) for v in k.split("x")): r for k, r in entry["rates"].items()}
     scratch = ChainBatch(models, chains)
     moves = np.array([max(64, int(sweeps * m.n_sites)) for m in models], dtype=np.int64)
     rates = {}
     seen = set()
-    for w in candidates:
-        scratch.set_waves(w)
-        shape = tuple(sorted({scratch.launch_shape(c) for c in range(scratch.n_chains)}))
-        if shape in seen:  # the variant cannot take this many waves: same launch as before
-            continue
-        seen.add(shape)
-        scratch.run(moves // 4)  # first launch of a variant loads its code
-        best = 0.0
-        for _ in range(repeats):
-            ms = scratch.run_timed(moves)
-            best = max(best, float(moves.sum()) / (ms * 1e-3))
-        rates[w] = best
-        if log:
-            log("autotune %s: waves %d -> %.3e moves/s" % (sig, w, best))
+    for k in per_round:
+        scratch.set_moves_per_round(k)
+        for w in candidates:
+            scratch.set_waves(w)
+            shape = (k, tuple(sorted({scratch.launch_shape(c) for c in range(scratch.n_chains)})))
+            if shape in seen:  # the variant cannot take this shape: same launch as before
+                continue
+            seen.add(shape)
+            scratch.run(moves // 4)  # first launch of a variant loads its code
+            best = 0.0
+            for _ in range(repeats):
+                ms = scratch.run_timed(moves)
+                best = max(best, float(moves.sum()) / (ms * 1e-3))
+            rates[(w, k)] = best
+            if log:
+                log("autotune %s: waves %d, moves per round %d -> %.3e moves/s" % (sig, w, k, best))
     scratch.close()
-    best_w = max(rates, key=rates.get)
-    _cache[sig] = {"best": best_w, "rates": {str(k): v for k, v in rates.items()}}
+    best = max(rates, key=rates.get)
+    _cache[sig] = {"best": list(best), "rates": {"%dx%d" % kk: v for kk, v in rates.items()}}
     if cache_file:
         with open(cache_file, "w") as f:
             json.dump(_cache, f, indent=1, sort_keys=True)
-    return best_w, rates
+    return best, rates
+
+
+def apply(batch, shape):
+    """Set a tuned (waves, moves per round) on a batch."""
+    batch.set_moves_per_round(shape[1])
+    batch.set_waves(shape[0])
 
 
 def tuned_batch(models, chains, **kw):
-    """A ChainBatch with the autotuned waves per chain applied."""
+    """A ChainBatch with the autotuned launch shape applied."""
     best, _ = tune(models, chains, **kw)
     batch = ChainBatch(models, chains)
-    batch.set_waves(best)
+    apply(batch, best)
     return batch

@@ -171,6 +171,11 @@ class ChainBatch:
         allows it; 0 restores the automatic choice."""
         _capi.check(self._lib.cemc_batch_set_waves(self.handle, int(waves)))
 
+    def set_moves_per_round(self, moves):
+        """1 (default): every move evaluated on its own; 8: eight consecutive
+        moves evaluated together where the lattice allows it.  Same results."""
+        _capi.check(self._lib.cemc_batch_set_moves_per_round(self.handle, int(moves)))
+
     def launch_shape(self, chain):
         """(waves per chain, records per lane in registers, tables in LDS)."""
         w, r, g = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()

@@ -74,6 +74,13 @@ struct ModelDev {
   int32_t pdims[3];       // padded dims L + 2*pad_h
   int32_t has_background;
   int32_t n_movable;
+  // Speculative rounds (closed-form periodic, one translation group, no
+  // background, halo h <= 2): bit ((dx+h)(2h+1) + (dy+h))(2h+1) + (dz+h) of
+  // imask is set when a site at cell offset (dx, dy, dz) from a reference
+  // site is that site or a member of one of its sub-clusters.
+  int32_t spec_ok;
+  int32_t pad_spec;
+  uint64_t imask[2];
   int32_t gtab_size;      // doubles in the energy table
   int32_t n_rec;          // records over all groups
   int32_t pow_s[kMaxClusterSize];  // n_species^q

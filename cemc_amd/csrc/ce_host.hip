@@ -102,6 +102,8 @@ struct cemc_model {
   bool periodic = false;
   int dims[3] = {0, 0, 0};
   int pad_h = 0;
+  bool spec_ok = false;   // speculative rounds allowed (see ModelDev::imask)
+  uint64_t imask[2] = {0, 0};
   int pdims[3] = {0, 0, 0};
   std::vector<int> signed_cells;
   std::vector<double> basis, eci, gtab;
@@ -249,6 +251,9 @@ void cemc_model::refresh_view() {
   for (int i = 0; i < 3; ++i) v.dims[i] = dims[i];
   v.pad_h = pad_h;
   for (int i = 0; i < 3; ++i) v.pdims[i] = pdims[i];
+  v.spec_ok = spec_ok ? 1 : 0;
+  v.imask[0] = imask[0];
+  v.imask[1] = imask[1];
   v.has_background = int(movable.size()) != n_sites;
   v.n_movable = int(movable.size());
   v.gtab_size = int(gtab.size());
@@ -480,6 +485,25 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
     if (nm < 3) meta |= kRecUnused2;
     m->frecs.push_back(FatRec{mm[0], mm[1], mm[2], ww[0], ww[1], ww[2], meta, 0u});
   }
+  // interaction mask of the speculative rounds: the reference site and every
+  // member offset of its sub-clusters, in the (2h+1)^3 box
+  m->spec_ok = m->periodic && m->n_groups == 1 && int(m->movable.size()) == N && m->pad_h <= 2;
+  if (m->spec_ok) {
+    const int h = m->pad_h, w = 2 * h + 1;
+    auto set_bit = [&](int ox, int oy, int oz) {
+      const int idx = ((ox + h) * w + (oy + h)) * w + (oz + h);
+      m->imask[idx >> 6] |= uint64_t(1) << (idx & 63);
+    };
+    set_bit(0, 0, 0);
+    for (const auto& sr : m->rec_slots) {
+      const int nm = int((sr.meta >> 24) & 3u);
+      const uint32_t slot[3] = {uint32_t(sr.m0), uint32_t(sr.m1), uint32_t(sr.m2)};
+      for (int q = 0; q < nm; ++q) {
+        const int* o = &m->signed_cells[3 * slot[q]];
+        set_bit(o[0], o[1], o[2]);
+      }
+    }
+  }
   // upload
   std::vector<int8_t> sg(N);
   for (int i = 0; i < N; ++i) sg[i] = int8_t(m->site_group[i]);
@@ -687,9 +711,11 @@ struct cemc_batch {
     bool gtab_lds;
     int kbm;
     int waves;
+    int spec;  // moves per round
     int begin, count;
     uint32_t lds;
   };
+  int spec_moves = 1;  // moves per round where the model allows it (1 or kSpecMoves)
   std::vector<Class> classes;
   int64_t classes_version = 0;  // bumped whenever the launch classes change
   int forced_waves = 0;
@@ -848,6 +874,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
     bool glds;
     int kbm;
     int waves;
+    int spec;
     uint32_t lds;
   };
   // waves per chain: enough chains x waves to give every SIMD of the
@@ -898,6 +925,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
       k.waves = auto_waves;
       k.kr = pick_kr(k.waves);
     }
+    k.spec = (k.kr > 0 && m->spec_ok) ? b->spec_moves : 1;
     if (k.kr > 0) {
       k.glds = with_g <= gtab_budget;
       k.lds = k.glds ? with_g : no_g;
@@ -919,7 +947,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
     for (int c = c0; c < n_chains; ++c) {
       const Key& k = keys[c];
       if (done[c] || k.periodic != k0.periodic || k.canonical != k0.canonical || k.kr != k0.kr ||
-          k.glds != k0.glds || k.kbm != k0.kbm || k.waves != k0.waves)
+          k.glds != k0.glds || k.kbm != k0.kbm || k.waves != k0.waves || k.spec != k0.spec)
         continue;
       done[c] = true;
       ids.push_back(c);
@@ -930,7 +958,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
       const auto* my = b->models[b->params[y].model];
       return int64_t(mx->n_sites) * mx->recs.size() > int64_t(my->n_sites) * my->recs.size();
     });
-    b->classes.push_back(cemc_batch::Class{k0.periodic, k0.canonical, k0.kr, k0.glds, k0.kbm, k0.waves,
+    b->classes.push_back(cemc_batch::Class{k0.periodic, k0.canonical, k0.kr, k0.glds, k0.kbm, k0.waves, k0.spec,
                                            int(order.size()), int(ids.size()), lds});
     order.insert(order.end(), ids.begin(), ids.end());
   }
@@ -969,6 +997,7 @@ void enqueue_sweeps(cemc_batch* b, uint32_t* d_log, int64_t log_words) {
     a.gtab_lds = cl.gtab_lds;
     a.kbm = cl.kbm;
     a.waves = cl.waves;
+    a.spec = cl.spec;
     a.accept_log = d_log;
     a.log_words = log_words;
     HIP_CHECK(launch_sweep(a, b->stream));
@@ -1481,6 +1510,16 @@ int cemc_batch_set_waves(cemc_batch* b, int32_t waves) {
     HIP_CHECK(hipStreamSynchronize(b->stream));
     if (!b->dynamic) derive_classes(b, waves);
     b->forced_waves = waves;
+  });
+}
+
+int cemc_batch_set_moves_per_round(cemc_batch* b, int32_t moves) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(moves == 1 || moves == kSpecMoves, "moves per round must be 1 or 8");
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    b->spec_moves = moves;
+    if (!b->dynamic) derive_classes(b, b->forced_waves);
   });
 }
 

@@ -386,7 +386,7 @@ __global__ __launch_bounds__(256) void pt_exchange_kernel(PTDev* __restrict__ pt
 // Launch wrappers.  The sweep kernel's instantiations are compiled in
 // separate objects (ce_sweep_inst.hip) and reached through these lookups.
 #define CEMC_DECLARE_SWEEP_FN(W, P) \
-  const void* sweep_fn_w##W##_p##P(bool canonical, int kr, bool glds, int kbm);
+  const void* sweep_fn_w##W##_p##P(bool canonical, int kr, bool glds, int kbm, int spec);
 CEMC_DECLARE_SWEEP_FN(1, 0)
 CEMC_DECLARE_SWEEP_FN(1, 1)
 CEMC_DECLARE_SWEEP_FN(2, 0)
@@ -397,25 +397,27 @@ CEMC_DECLARE_SWEEP_FN(8, 0)
 CEMC_DECLARE_SWEEP_FN(8, 1)
 #undef CEMC_DECLARE_SWEEP_FN
 
-static const void* sweep_fn(bool periodic, bool canonical, int kr, bool glds, int kbm, int waves) {
+static const void* sweep_fn(bool periodic, bool canonical, int kr, bool glds, int kbm, int waves, int spec) {
   switch (waves) {
-    case 1: return periodic ? sweep_fn_w1_p1(canonical, kr, glds, kbm) : sweep_fn_w1_p0(canonical, kr, glds, kbm);
-    case 2: return periodic ? sweep_fn_w2_p1(canonical, kr, glds, kbm) : sweep_fn_w2_p0(canonical, kr, glds, kbm);
-    case 4: return periodic ? sweep_fn_w4_p1(canonical, kr, glds, kbm) : sweep_fn_w4_p0(canonical, kr, glds, kbm);
-    case 8: return periodic ? sweep_fn_w8_p1(canonical, kr, glds, kbm) : sweep_fn_w8_p0(canonical, kr, glds, kbm);
+    case 1: return periodic ? sweep_fn_w1_p1(canonical, kr, glds, kbm, spec) : sweep_fn_w1_p0(canonical, kr, glds, kbm, spec);
+    case 2: return periodic ? sweep_fn_w2_p1(canonical, kr, glds, kbm, spec) : sweep_fn_w2_p0(canonical, kr, glds, kbm, spec);
+    case 4: return periodic ? sweep_fn_w4_p1(canonical, kr, glds, kbm, spec) : sweep_fn_w4_p0(canonical, kr, glds, kbm, spec);
+    case 8: return periodic ? sweep_fn_w8_p1(canonical, kr, glds, kbm, spec) : sweep_fn_w8_p0(canonical, kr, glds, kbm, spec);
     default: return nullptr;
   }
 }
 
-bool sweep_variant_ok(int kr, bool glds, int kbm, int waves) {
+bool sweep_variant_ok(int kr, bool glds, int kbm, int waves, int spec, bool periodic) {
+  (void)glds;
   if (waves != 1 && waves != 2 && waves != 4 && waves != 8) return false;
+  if (spec != 1 && !(spec == kSpecMoves && periodic && kr > 0)) return false;
   if (kr == 2 || kr == 4 || kr == 8) return kbm == 1 || kbm == 3;
-  return waves == 1 && kr == 0 && (kbm == 1 || kbm == 3 || kbm == 7) && true;
+  return waves == 1 && kr == 0 && (kbm == 1 || kbm == 3 || kbm == 7);
 }
 
 uint32_t sweep_static_lds() {
   return uint32_t(sizeof(double) * (kMaxSpecies * kMaxSpecies * kMaxBasis + kMaxSpecies * kMaxSpecies +
-                                    kMaxBasis * kMaxSpecies + 2 * 8 + kMaxBasis * 8) +
+                                    kMaxBasis * kMaxSpecies + 2 * 8 + kMaxBasis * 8 + 2 * kSpecMoves * 8) +
                   sizeof(int) * 2 * kMaxGroups);
 }
 
@@ -425,9 +427,10 @@ hipError_t sweep_set_max_lds(uint32_t bytes) {
       for (int w : {1, 2, 4, 8})
         for (int kr : {0, 2, 4, 8})
           for (int g = 0; g < 2; ++g)
-            for (int kbm : {1, 3, 7}) {
-              if (!sweep_variant_ok(kr, g != 0, kbm, w)) continue;
-              const void* fn = sweep_fn(p != 0, cn != 0, kr, g != 0, kbm, w);
+            for (int kbm : {1, 3, 7})
+              for (int sp : {1, kSpecMoves}) {
+              if (!sweep_variant_ok(kr, g != 0, kbm, w, sp, p != 0)) continue;
+              const void* fn = sweep_fn(p != 0, cn != 0, kr, g != 0, kbm, w, sp);
               if (fn == nullptr) return hipErrorInvalidValue;
               hipFuncAttributes attr;
               hipError_t e = hipFuncGetAttributes(&attr, fn);
@@ -441,8 +444,8 @@ hipError_t sweep_set_max_lds(uint32_t bytes) {
 }
 
 hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream) {
-  if (!sweep_variant_ok(a.kr, a.gtab_lds, a.kbm, a.waves)) return hipErrorInvalidValue;
-  const void* fn = sweep_fn(a.periodic, a.canonical, a.kr, a.gtab_lds, a.kbm, a.waves);
+  if (!sweep_variant_ok(a.kr, a.gtab_lds, a.kbm, a.waves, a.spec, a.periodic)) return hipErrorInvalidValue;
+  const void* fn = sweep_fn(a.periodic, a.canonical, a.kr, a.gtab_lds, a.kbm, a.waves, a.spec);
   if (fn == nullptr) return hipErrorInvalidValue;
   const ChainParams* params = a.params;
   ChainState* states = a.states;

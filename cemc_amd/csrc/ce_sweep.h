@@ -216,20 +216,14 @@ __device__ __attribute__((always_inline)) inline int unpack16(uint64_t lo, uint6
 // Padded index of image `bits` (bit 0: x, 1: y, 2: z take the other image)
 // of the site at cell (x, y, z), or -1 when that image does not exist.
 __device__ __attribute__((always_inline)) inline int image_index(const SweepCtx& c, int x, int y, int z, int bits) {
-  int ix = x, iy = y, iz = z;
-  bool ok = true;
-  if (bits & 1) {
-    ix = x < c.h ? x + c.L1 : (x >= c.L1 - c.h ? x - c.L1 : x);
-    ok = ok && ix != x;
-  }
-  if (bits & 2) {
-    iy = y < c.h ? y + c.L2 : (y >= c.L2 - c.h ? y - c.L2 : y);
-    ok = ok && iy != y;
-  }
-  if (bits & 4) {
-    iz = z < c.h ? z + c.L3 : (z >= c.L3 - c.h ? z - c.L3 : z);
-    ok = ok && iz != z;
-  }
+  // other image coordinate in each dimension (equal to the coordinate when
+  // the site is not within h of that face); selects only, no branches
+  const int ax = x < c.h ? x + c.L1 : (x >= c.L1 - c.h ? x - c.L1 : x);
+  const int ay = y < c.h ? y + c.L2 : (y >= c.L2 - c.h ? y - c.L2 : y);
+  const int az = z < c.h ? z + c.L3 : (z >= c.L3 - c.h ? z - c.L3 : z);
+  const bool bx = (bits & 1) != 0, by = (bits & 2) != 0, bz = (bits & 4) != 0;
+  const bool ok = (!bx || ax != x) && (!by || ay != y) && (!bz || az != z);
+  const int ix = bx ? ax : x, iy = by ? ay : y, iz = bz ? az : z;
   return ok ? int(__umul24(__umul24(uint32_t(ix + c.h), uint32_t(c.P2)) + uint32_t(iy + c.h), uint32_t(c.P3)) +
                   uint32_t(iz + c.h))
             : -1;
@@ -260,7 +254,55 @@ __device__ __attribute__((always_inline)) inline double group_sum(double x, doub
 // from the same ordered sum, and every wave writes the same commit; one
 // barrier per move (inside group_sum) separates a move's commit from the
 // next move's gathers in other waves.
-template <bool kPeriodic, bool kCanonical, int kR, bool kGtabLds, int kBM, int kW>
+// Sums of kK values over the kW waves of the workgroup (see group_sum);
+// `slots` holds kK * 8 doubles.
+template <int kW, int kK>
+__device__ __attribute__((always_inline)) inline void group_sum_k(const double (&x)[kK], double (&out)[kK],
+                                                                  double* slots, int wave, int lane) {
+  double w[kK];
+#pragma unroll
+  for (int k = 0; k < kK; ++k) w[k] = wave_sum_f64(x[k]);
+  if constexpr (kW == 1) {
+#pragma unroll
+    for (int k = 0; k < kK; ++k) out[k] = w[k];
+  } else {
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < kK; ++k) slots[k * 8 + wave] = w[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kK; ++k) {
+      double t = slots[k * 8];
+#pragma unroll
+      for (int v = 1; v < kW; ++v) t += slots[k * 8 + v];
+      out[k] = t;
+    }
+  }
+}
+
+// True when a change at packed cell a (x | y<<8 | z<<16) can alter the
+// energy change of a flip at packed cell b: a - b (wrapped) is 0 or a
+// sub-cluster member offset, per the model's interaction mask.
+__device__ __attribute__((always_inline)) inline bool cells_interact(uint32_t a, uint32_t b, const SweepCtx& c,
+                                                                     uint64_t im0, uint64_t im1) {
+  int dx = int(a & 255u) - int(b & 255u);
+  int dy = int((a >> 8) & 255u) - int((b >> 8) & 255u);
+  int dz = int(a >> 16) - int(b >> 16);
+  dx -= 2 * dx > c.L1 ? c.L1 : 0;
+  dx += 2 * dx < -c.L1 ? c.L1 : 0;
+  dy -= 2 * dy > c.L2 ? c.L2 : 0;
+  dy += 2 * dy < -c.L2 ? c.L2 : 0;
+  dz -= 2 * dz > c.L3 ? c.L3 : 0;
+  dz += 2 * dz < -c.L3 ? c.L3 : 0;
+  const int h = c.h, w = 2 * c.h + 1;
+  if (dx < -h || dx > h || dy < -h || dy > h || dz < -h || dz > h) return false;
+  const int idx = ((dx + h) * w + (dy + h)) * w + (dz + h);
+  const uint64_t word = idx < 64 ? im0 : im1;
+  return ((word >> (idx & 63)) & 1u) != 0;
+}
+
+template <bool kPeriodic, bool kCanonical, int kR, bool kGtabLds, int kBM, int kW, int kK>
 __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __restrict__ params,
                                                    ChainState* __restrict__ states,
                                                    const ModelDev* __restrict__ models,
@@ -276,6 +318,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
   __shared__ int grp_tab[2 * kMaxGroups];
   __shared__ double red_move[2][8];             // per-move wave sums, double-buffered by move parity
   __shared__ double red_init[kMaxBasis][8];     // initial singlets
+  __shared__ double red_spec[2][kK > 1 ? kK * 8 : 1];  // speculative rounds: per-round wave sums
   constexpr int kRecLds = kR == 0;
   constexpr int kThreads = 64 * kW;
   const int chain = chain_order[blockIdx.x];
@@ -404,42 +447,73 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
   const uint64_t seg_lo = pp->segoff_pk[0], seg_hi = pp->segoff_pk[1];
   uint32_t* acc_log = accept_log ? accept_log + int64_t(chain) * log_words : nullptr;
 
+  if constexpr (kK == 1) {
+    // Samples between accepted moves see the same state: the sums take each
+    // run of equal samples at once (run = samples of the current state not
+    // yet added).
+    int64_t run = 0;
+    auto flush = [&]() {
+      const double dn = double(run);
+      const double de_ref = energy - e_ref;
+      sum_de = fma(dn, de_ref, sum_de);
+      sum_de2 = fma(dn * de_ref, de_ref, sum_de2);
+      if constexpr (!kCanonical) {
+        sum_s = fma(dn, sing, sum_s);
+        sum_s2 = fma(dn * sing, sing, sum_s2);
+        sum_se = fma(dn * sing, de_ref, sum_se);
+      }
+      run = 0;
+    };
   for (int64_t blk0 = 0; blk0 < nmov; blk0 += 64) {
-    // lane l holds the Philox block and log u of move blk0 + l
+    // lane l holds the Philox block and log u of move blk0 + l, and decodes
+    // the parts of that move that do not depend on the state
     const Philox4 blk = philox4x64_10(t0 + uint64_t(blk0) + uint64_t(lane), 0, 0, 0, seed, uid);
     const double log_u = log(unit_double(blk.w3));
     const int n_in = int(nmov - blk0 < 64 ? nmov - blk0 : 64);
+    uint32_t v_old = 0, v_new = 0;  // canonical: species a and b; SGC: offset of the new species
+    int v_ta = 0, v_tb = 0;         // canonical: tracker slots of the two sites
+    int v_site = 0, v_p = 0;        // SGC: the site and its LDS index
+    uint32_t v_cell = 0;            // SGC, periodic: packed cell of the site
+    if constexpr (kCanonical) {
+      const uint32_t ia = bounded32(uint32_t(blk.w0 >> 32), np);
+      uint32_t ib = ia + 1u + bounded32(uint32_t(blk.w0), np - 1u);
+      ib -= ib >= np ? np : 0u;
+      v_old = (present_pk >> (4u * ia)) & 15u;
+      v_new = (present_pk >> (4u * ib)) & 15u;
+      v_ta = unpack16(seg_lo, seg_hi, ia) + int(bounded64(blk.w1, uint64_t(unpack16(cnt_lo, cnt_hi, ia))));
+      v_tb = unpack16(seg_lo, seg_hi, ib) + int(bounded64(blk.w2, uint64_t(unpack16(cnt_lo, cnt_hi, ib))));
+    } else {
+      const int pick = int(bounded64(blk.w1, uint64_t(n_pick)));
+      v_site = has_bkg ? m.movable[pick] : pick;
+      const Ref r = ref_of<kPeriodic>(c, v_site);
+      v_p = r.p;
+      v_cell = uint32_t(r.x) | (uint32_t(r.y) << 8) | (uint32_t(r.z) << 16);
+      v_new = bounded32(uint32_t(blk.w0), uint32_t(c.S - 1));
+    }
     uint64_t acc_bits = 0;
     for (int l = 0; l < n_in; ++l) {
-      const uint64_t w0 = readlane_u64(blk.w0, l);
-      const uint64_t w1 = readlane_u64(blk.w1, l);
       const double lu = readlane_f64(log_u, l);
 
-      int si, sj = 0, sa = 0, sb = 0, off_a = 0, off_b = 0;
+      int si, sj = 0, ta = 0, tb = 0;
       uint32_t old_i = 0, new_i = 0;
-      if constexpr (kCanonical) {
-        const uint64_t w2 = readlane_u64(blk.w2, l);
-        const uint32_t ia = bounded32(uint32_t(w0 >> 32), np);
-        uint32_t ib = ia + 1u + bounded32(uint32_t(w0), np - 1u);
-        ib -= ib >= np ? np : 0u;
-        old_i = (present_pk >> (4u * ia)) & 15u;
-        new_i = (present_pk >> (4u * ib)) & 15u;
-        sa = int(bounded64(w1, uint64_t(unpack16(cnt_lo, cnt_hi, ia))));
-        sb = int(bounded64(w2, uint64_t(unpack16(cnt_lo, cnt_hi, ib))));
-        off_a = unpack16(seg_lo, seg_hi, ia);
-        off_b = unpack16(seg_lo, seg_hi, ib);
-        si = __builtin_amdgcn_readfirstlane(trk[off_a + sa]);
-        sj = __builtin_amdgcn_readfirstlane(trk[off_b + sb]);
-      } else {
-        const int pick = int(bounded64(w1, uint64_t(n_pick)));
-        si = has_bkg ? __builtin_amdgcn_readfirstlane(m.movable[pick]) : pick;
-      }
-      const Ref ri = ref_of<kPeriodic>(c, si);
-      Ref rj = ri;
+      Ref ri, rj;
       uint32_t patch_j = 0u;
       if constexpr (kCanonical) {
+        old_i = __builtin_amdgcn_readlane(v_old, l);
+        new_i = __builtin_amdgcn_readlane(v_new, l);
+        ta = __builtin_amdgcn_readlane(v_ta, l);
+        tb = __builtin_amdgcn_readlane(v_tb, l);
+        si = __builtin_amdgcn_readfirstlane(trk[ta]);
+        sj = __builtin_amdgcn_readfirstlane(trk[tb]);
+        ri = ref_of<kPeriodic>(c, si);
         rj = ref_of<kPeriodic>(c, sj);
         patch_j = kPeriodic ? wrapped_delta(c, ri, rj) : uint32_t(si);
+      } else {
+        si = __builtin_amdgcn_readlane(v_site, l);
+        ri.site = si;
+        ri.p = __builtin_amdgcn_readlane(v_p, l);
+        ri.x = ri.y = ri.z = 0;
+        rj = ri;
       }
       int gi_i[kR > 0 ? kR : 1], gi_j[kR > 0 ? kR : 1];
       if constexpr (!kCanonical) old_i = occ[ri.p];
@@ -449,7 +523,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         if constexpr (kCanonical) table_index<kPeriodic, kR, kBM, true>(c, rr, rj, occ, patch_j, new_i, gi_j);
       }
       if constexpr (!kCanonical) {
-        new_i = old_i + 1u + bounded32(uint32_t(w0), uint32_t(c.S - 1));
+        new_i = old_i + 1u + __builtin_amdgcn_readlane(v_new, l);
         new_i -= new_i >= uint32_t(c.S) ? uint32_t(c.S) : 0u;
       }
       double dphi[kBM];
@@ -499,49 +573,44 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
       // Metropolis test of the reference: accept if E_new < E_cur, else if
       // u <= exp(-(E_new - E_cur) / kT) (cemc/mcmc/montecarlo.py:950-956).  The
       // comparison is first made against log(u) with a margin far above the
-      // rounding of log/exp; only moves inside the margin evaluate exp.
-      bool accept;
-      if (dE < 0.0) {
-        accept = true;
-      } else {
-        const double x = -dE * inv_kT;
-        const double margin = 1e-12 * (1.0 + fabs(x));
-        if (lu < x - margin)
-          accept = true;
-        else if (lu > x + margin)
-          accept = false;
-        else
-          accept = unit_double(readlane_u64(blk.w3, l)) <= exp(-dE / kT);
-      }
-      if (accept) {
+      // rounding of log/exp; only moves inside the margin evaluate exp.  The
+      // outcome is made uniform (scalar) so the branches below need no masks.
+      const double x = -dE * inv_kT;
+      const double margin = 1e-12 * (1.0 + fabs(x));
+      int verdict = (dE < 0.0 || lu < x - margin) ? 1 : (lu > x + margin ? 0 : 2);
+      verdict = __builtin_amdgcn_readfirstlane(verdict);
+      if (verdict == 2)
+        verdict = __builtin_amdgcn_readfirstlane(int(unit_double(readlane_u64(blk.w3, l)) <= exp(-dE / kT)));
+      if (verdict) {
+        flush();
         if constexpr (kPeriodic) {
           // lanes 0-7 write the images of site i, lanes 8-15 those of site j
-          const bool second = kCanonical && lane >= 8 && lane < 16;
-          const Ref& rs = second ? rj : ri;
-          const int idx = lane < (kCanonical ? 16 : 8) ? image_index(c, rs.x, rs.y, rs.z, lane & 7) : -1;
-          if (idx >= 0) occ[idx] = uint8_t(second ? old_i : new_i);
+          uint32_t cell;
+          if constexpr (kCanonical) {
+            const bool second = lane >= 8;
+            cell = second ? (uint32_t(rj.x) | (uint32_t(rj.y) << 8) | (uint32_t(rj.z) << 16))
+                          : (uint32_t(ri.x) | (uint32_t(ri.y) << 8) | (uint32_t(ri.z) << 16));
+          } else {
+            cell = __builtin_amdgcn_readlane(v_cell, l);
+          }
+          const int idx = image_index(c, int(cell & 255u), int((cell >> 8) & 255u), int(cell >> 16), lane & 7);
+          const bool second = kCanonical && lane >= 8;
+          if (idx >= 0 && lane < (kCanonical ? 16 : 8)) occ[idx] = uint8_t(second ? old_i : new_i);
         } else {
           if (lane == 0) occ[si] = uint8_t(new_i);
           if (kCanonical && lane == 1) occ[sj] = uint8_t(old_i);
         }
         if constexpr (kCanonical) {
           if (lane == 0) {
-            trk[off_a + sa] = uint16_t(sj);
-            trk[off_b + sb] = uint16_t(si);
+            trk[ta] = uint16_t(sj);
+            trk[tb] = uint16_t(si);
           }
         }
         energy += dE;
         if constexpr (!kCanonical) sing += dsing;
         acc_bits |= uint64_t(1) << l;
       }
-      const double de_ref = energy - e_ref;
-      sum_de += de_ref;
-      sum_de2 = fma(de_ref, de_ref, sum_de2);
-      if constexpr (!kCanonical) {
-        sum_s += sing;
-        sum_s2 = fma(sing, sing, sum_s2);
-        sum_se = fma(sing, de_ref, sum_se);
-      }
+      ++run;
       if constexpr (kW == 1) {
         __syncthreads();
       } else {
@@ -556,6 +625,210 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
       const int64_t w = blk0 >> 5;
       if (lane == 0) acc_log[w] = uint32_t(acc_bits);
       if (lane == 1 && n_in > 32) acc_log[w + 1] = uint32_t(acc_bits >> 32);
+    }
+  }
+    flush();
+  } else {
+    // ---- speculative rounds (closed-form periodic lattice, records in
+    // VGPRs, one translation group, no background).  Up to kK consecutive
+    // moves are evaluated against the same state, then decided and
+    // committed in order.  An accepted move invalidates the later moves of
+    // the round that read what it changed (its sites or their sub-cluster
+    // members, or its tracker slots); the round ends before the first
+    // invalidated move, which the next round evaluates again.  Every
+    // committed move's energy change is therefore bitwise the one the
+    // sequential loop computes.
+    static_assert(kPeriodic && kR > 0, "speculative rounds need a periodic lattice and records in VGPRs");
+    const uint64_t im0 = m.imask[0], im1 = m.imask[1];
+    uint32_t rnd = 0;
+    for (int64_t blk0 = 0; blk0 < nmov; blk0 += 64) {
+      const Philox4 blk = philox4x64_10(t0 + uint64_t(blk0) + uint64_t(lane), 0, 0, 0, seed, uid);
+      const double log_u = log(unit_double(blk.w3));
+      const int n_in = int(nmov - blk0 < 64 ? nmov - blk0 : 64);
+      // lane l decodes move blk0 + l once per block
+      uint32_t v_old = 0, v_new = 0;  // canonical: species a and b; SGC: offset of the new species
+      int v_ta = 0, v_tb = 0;         // canonical: tracker slots of the two sites
+      int v_site = 0;                 // SGC: the site
+      if constexpr (kCanonical) {
+        const uint32_t ia = bounded32(uint32_t(blk.w0 >> 32), np);
+        uint32_t ib = ia + 1u + bounded32(uint32_t(blk.w0), np - 1u);
+        ib -= ib >= np ? np : 0u;
+        v_old = (present_pk >> (4u * ia)) & 15u;
+        v_new = (present_pk >> (4u * ib)) & 15u;
+        v_ta = unpack16(seg_lo, seg_hi, ia) + int(bounded64(blk.w1, uint64_t(unpack16(cnt_lo, cnt_hi, ia))));
+        v_tb = unpack16(seg_lo, seg_hi, ib) + int(bounded64(blk.w2, uint64_t(unpack16(cnt_lo, cnt_hi, ib))));
+      } else {
+        v_site = int(bounded64(blk.w1, uint64_t(N)));
+        v_new = bounded32(uint32_t(blk.w0), uint32_t(c.S - 1));
+      }
+      uint64_t acc_bits = 0;
+      int l0 = 0;
+      while (l0 < n_in) {
+        const int n = n_in - l0 < kK ? n_in - l0 : kK;
+        // every lane looks up its own move against the current state
+        int si_v = v_site, sj_v = 0;
+        if constexpr (kCanonical) {
+          si_v = trk[v_ta];
+          sj_v = trk[v_tb];
+        }
+        const Ref ri_v = ref_of<true>(c, si_v);
+        Ref rj_v = ri_v;
+        uint32_t patch_v = 0, old_v = v_old, new_v = v_new;
+        if constexpr (kCanonical) {
+          rj_v = ref_of<true>(c, sj_v);
+          patch_v = wrapped_delta(c, ri_v, rj_v);
+        } else {
+          old_v = occ[ri_v.p];
+          new_v = old_v + 1u + v_new;
+          new_v -= new_v >= uint32_t(c.S) ? uint32_t(c.S) : 0u;
+        }
+        const uint32_t cell_i = uint32_t(ri_v.x) | (uint32_t(ri_v.y) << 8) | (uint32_t(ri_v.z) << 16);
+        const uint32_t cell_j = uint32_t(rj_v.x) | (uint32_t(rj_v.y) << 8) | (uint32_t(rj_v.z) << 16);
+        // each lane's records for every move of the round
+        double part[kK], dphi0[kK], desng[kK], dsg[kK];
+#pragma unroll
+        for (int k = 0; k < kK; ++k) {
+          part[k] = 0.0;
+          dphi0[k] = 0.0;
+          desng[k] = 0.0;
+          dsg[k] = 0.0;
+          if (k < n) {
+            const int lk = l0 + k;
+            Ref ri;
+            ri.p = __builtin_amdgcn_readlane(ri_v.p, lk);
+            ri.site = 0;
+            ri.x = ri.y = ri.z = 0;
+            const uint32_t o_k = __builtin_amdgcn_readlane(old_v, lk);
+            const uint32_t n_k = __builtin_amdgcn_readlane(new_v, lk);
+            double dphi[kBM];
+            if constexpr (kBM == 1) {
+              dphi[0] = o_k == 0 ? dphi01 : dphi10;
+              dphi0[k] = dphi[0];
+              if constexpr (!kCanonical) {
+                desng[k] = o_k == 0 ? dsin01 : dsin10;
+                dsg[k] = dphi[0] * inv_n;
+              }
+            } else {
+              const int pair = int(o_k) * c.S + int(n_k);
+#pragma unroll
+              for (int d = 0; d < kBM; ++d) dphi[d] = d < c.B ? dphi_tab[pair * c.B + d] : 0.0;
+              if constexpr (!kCanonical) {
+                desng[k] = dsingle_tab[pair];
+                dsg[k] = dphi_tab[pair * c.B + (lane < c.B ? lane : 0)] * inv_n;
+              }
+            }
+            int gi_i[kR];
+            table_index<true, kR, kBM, false>(c, rr, ri, occ, 0u, 0u, gi_i);
+            double pk = table_sum<kR, kBM>(gi_i, gtab, dphi);
+            if constexpr (kCanonical) {
+              Ref rj;
+              rj.p = __builtin_amdgcn_readlane(rj_v.p, lk);
+              rj.site = 0;
+              rj.x = rj.y = rj.z = 0;
+              const uint32_t patch = __builtin_amdgcn_readlane(patch_v, lk);
+              double dphj[kBM];
+#pragma unroll
+              for (int d = 0; d < kBM; ++d) dphj[d] = -dphi[d];
+              int gi_j[kR];
+              table_index<true, kR, kBM, true>(c, rr, rj, occ, patch, n_k, gi_j);
+              const double pj = table_sum<kR, kBM>(gi_j, gtab, dphj);
+              pk = kBM == 1 ? pk - pj : pk + pj;
+            }
+            part[k] = pk;
+          }
+        }
+        double tot[kK];
+        group_sum_k<kW, kK>(part, tot, red_spec[rnd & 1u], wave, lane);
+        // in-order decisions and commits
+        uint32_t invalid = 0;
+        bool stop = false;
+        int done = 0;
+#pragma unroll
+        for (int k = 0; k < kK; ++k) {
+          if (!stop && k < n) {
+            if ((invalid >> k) & 1u) {
+              stop = true;
+            } else {
+              done = k + 1;
+              const int lk = l0 + k;
+              const double dE = (kBM == 1 ? dphi0[k] * tot[k] : tot[k]) + desng[k];
+              const double lu = readlane_f64(log_u, lk);
+              bool accept;
+              if (dE < 0.0) {
+                accept = true;
+              } else {
+                const double x = -dE * inv_kT;
+                const double margin = 1e-12 * (1.0 + fabs(x));
+                if (lu < x - margin)
+                  accept = true;
+                else if (lu > x + margin)
+                  accept = false;
+                else
+                  accept = unit_double(readlane_u64(blk.w3, lk)) <= exp(-dE / kT);
+              }
+              if (accept) {
+                const uint32_t ci = __builtin_amdgcn_readlane(cell_i, lk);
+                const uint32_t cj = __builtin_amdgcn_readlane(cell_j, lk);
+                const uint32_t o_k = __builtin_amdgcn_readlane(old_v, lk);
+                const uint32_t n_k = __builtin_amdgcn_readlane(new_v, lk);
+                // lanes 0-7 write the images of site i, lanes 8-15 those of site j
+                const bool second = kCanonical && lane >= 8 && lane < 16;
+                const uint32_t cs = second ? cj : ci;
+                const int idx = lane < (kCanonical ? 16 : 8)
+                                    ? image_index(c, int(cs & 255u), int((cs >> 8) & 255u), int(cs >> 16), lane & 7)
+                                    : -1;
+                if (idx >= 0) occ[idx] = uint8_t(second ? o_k : n_k);
+                int ta_k = 0, tb_k = 0;
+                if constexpr (kCanonical) {
+                  ta_k = __builtin_amdgcn_readlane(v_ta, lk);
+                  tb_k = __builtin_amdgcn_readlane(v_tb, lk);
+                  const int si = __builtin_amdgcn_readlane(si_v, lk);
+                  const int sj = __builtin_amdgcn_readlane(sj_v, lk);
+                  if (lane == 0) {
+                    trk[ta_k] = uint16_t(sj);
+                    trk[tb_k] = uint16_t(si);
+                  }
+                }
+                energy += dE;
+                if constexpr (!kCanonical) sing += dsg[k];
+                acc_bits |= uint64_t(1) << lk;
+                // later moves of the round that read what this move changed
+                bool hit = false;
+                if (lane > lk && lane < l0 + n) {
+                  hit = cells_interact(ci, cell_i, c, im0, im1);
+                  if constexpr (kCanonical) {
+                    hit = hit || cells_interact(ci, cell_j, c, im0, im1) || cells_interact(cj, cell_i, c, im0, im1) ||
+                          cells_interact(cj, cell_j, c, im0, im1) || v_ta == ta_k || v_ta == tb_k || v_tb == ta_k ||
+                          v_tb == tb_k;
+                  }
+                }
+                invalid |= uint32_t(__ballot(hit) >> l0);
+              }
+              const double de_ref = energy - e_ref;
+              sum_de += de_ref;
+              sum_de2 = fma(de_ref, de_ref, sum_de2);
+              if constexpr (!kCanonical) {
+                sum_s += sing;
+                sum_s2 = fma(sing, sing, sum_s2);
+                sum_se = fma(sing, de_ref, sum_se);
+              }
+            }
+          }
+        }
+        l0 += done;
+        ++rnd;
+        if constexpr (kW == 1) {
+          __syncthreads();
+        } else {
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        }
+      }
+      n_acc += uint64_t(__builtin_popcountll(acc_bits));
+      if (acc_log != nullptr && tid < 2) {
+        const int64_t w = blk0 >> 5;
+        if (lane == 0) acc_log[w] = uint32_t(acc_bits);
+        if (lane == 1 && n_in > 32) acc_log[w + 1] = uint32_t(acc_bits >> 32);
+      }
     }
   }
 

@@ -46,13 +46,15 @@ def _check(fx, batch, c, log):
 
 @pytest.mark.parametrize("fname", ["ternary_canonical.npz", "ternary_sgc.npz", "quaternary_canonical.npz"])
 @pytest.mark.parametrize("waves", [1, 2, 4, 8])
-def test_waves_per_chain(gpu, fname, waves):
+@pytest.mark.parametrize("per_round", [1, 8])
+def test_waves_per_chain(gpu, fname, waves, per_round):
     from cemc_amd.batch import ChainBatch
     from cemc_amd.model import CEModel
     fx = ChainFixture(fname)
-    print("fixture", fname, "waves", waves, flush=True)
+    print("fixture", fname, "waves", waves, "moves per round", per_round, flush=True)
     model = CEModel(fx.settings, fx.eci)
     batch = ChainBatch([model], [_chain(fx)])
+    batch.set_moves_per_round(per_round)
     batch.set_waves(waves)
     w, kr, glds = batch.launch_shape(0)
     print("shape", w, kr, glds, flush=True)
@@ -121,3 +123,22 @@ def test_resumed_chain_continues_its_stream(gpu):
     got = np.concatenate([_bits(log1[0], half), _bits(log2[0], fx.n_moves - half)])
     assert np.array_equal(got, _bits(fx.accept_bits, fx.n_moves))
     assert np.array_equal(b2.occupation(0), fx.occ_final)
+
+
+@pytest.mark.parametrize("fname", ["binary_canonical_plain.npz", "binary_sgc_chain0.npz", "binary_sgc_chain3.npz"])
+def test_sequential_and_speculative_rounds_agree(gpu, fname):
+    """Binary chains with one move per round and with eight: identical accept
+    sequences (both equal to the reference) and energies."""
+    from cemc_amd.batch import ChainBatch
+    from cemc_amd.model import CEModel
+    fx = ChainFixture(fname)
+    model = CEModel(fx.settings, fx.eci)
+    logs, es = [], []
+    for per_round in (1, 8):
+        batch = ChainBatch([model], [_chain(fx)])
+        batch.set_moves_per_round(per_round)
+        logs.append(batch.run_logged(fx.n_moves))
+        _check(fx, batch, 0, logs[-1])
+        es.append(batch.energies()[0])
+    assert np.array_equal(logs[0], logs[1])
+    assert es[0] == es[1]
