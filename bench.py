@@ -27,10 +27,11 @@ sys.path.insert(0, ROOT)
 METRIC = "aggregate MC trial moves/sec at 1 MI355X (batched vs plain); p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
 # HBM bytes of one SGC launch of the default workload (1024 chains x 500
-# sweeps): FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
-# counter reads for a copy kernel of known size (x2.0 and x1.0).  Measured
-# with rocprofv3 --pmc; summary in profiles/sweep_kernel_sgc_binary_fcc10_1024chains.txt
-TRAFFIC_DEFAULT_LAUNCH = 2491371
+# sweeps, 1 wave per chain, 1 move per round): FETCH_SIZE and WRITE_SIZE per
+# dispatch, each scaled by what the counter reads for a copy kernel of known
+# size (x2.0 and x1.0).  Measured with rocprofv3 --pmc; summary in
+# profiles/sweep_kernel_sgc_binary_fcc10_1024chains.txt
+TRAFFIC_DEFAULT_LAUNCH = 3656666
 
 
 def log(*a):
@@ -217,6 +218,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workloads", default="batched,ternary,tempering,serving",
                     help="comma list of batched, ternary, tempering, serving")
+    ap.add_argument("--shape", default=None,
+                    help="WxK: waves per chain and moves per round for the batched workload (skips the autotuner)")
     ap.add_argument("--pt-cycles", type=int, default=200)
     ap.add_argument("--serving-jobs", type=int, default=500)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -240,7 +243,10 @@ def main():
                                                                 model.gathers_per_flip()))
     chains = batched_chains(settings, args.chains)
     from cemc_amd import autotune
-    tune_w, tune_rates = autotune.tune([model] * len(chains), chains, candidates=(1, 2, 4), sweeps=5.0, log=log)
+    if args.shape:
+        tune_w, tune_rates = tuple(int(v) for v in args.shape.split("x")), {}
+    else:
+        tune_w, tune_rates = autotune.tune([model] * len(chains), chains, candidates=(1, 2, 4), sweeps=5.0, log=log)
     batch = ChainBatch([model] * len(chains), chains)
     autotune.apply(batch, tune_w)
     moves_per_chain = args.sweeps * settings.n_sites
@@ -272,7 +278,7 @@ def main():
     bytes_per_flip = 5.0 * model.gathers_per_flip()
     launch_bytes = bytes_per_flip * moves_per_chain * len(chains)
     achieved = launch_bytes / (avg_kernel_ms * 1e-3) / 1e9
-    default_shape = len(chains) == 1024 and args.sweeps == 500
+    default_shape = len(chains) == 1024 and args.sweeps == 500 and tuple(tune_w) == (1, 1)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_DEFAULT_LAUNCH if default_shape else None,
                 "lds_peak_measured": lds_meas, "lds_frac": achieved / lds_meas,
