@@ -736,8 +736,17 @@ struct cemc_batch {
     dev_free(d_order);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (auto e : side_done) (void)hipEventDestroy(e);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    for (auto st : side) (void)hipStreamDestroy(st);
     if (stream) (void)hipStreamDestroy(stream);
   }
+  // launches of different classes run concurrently on side streams (default
+  // priority), forked from and joined back to `stream` with events
+  static constexpr int kSideStreams = 4;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> side_done;
+  hipEvent_t fork_ev = nullptr;
 };
 
 // Parallel-tempering run over the chains of one batch (replica slot t starts
@@ -925,7 +934,11 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
       k.waves = auto_waves;
       k.kr = pick_kr(k.waves);
     }
-    k.spec = (k.kr > 0 && m->spec_ok) ? b->spec_moves : 1;
+    // pools (spec_moves = 0): eight-move rounds for chains that run on four
+    // or more waves (few resident chains), where they hide barrier and
+    // memory waits; one move per round otherwise
+    const int want_spec = b->spec_moves > 0 ? b->spec_moves : (k.waves >= 4 ? kSpecMoves : 1);
+    k.spec = (k.kr > 0 && m->spec_ok) ? want_spec : 1;
     if (k.kr > 0) {
       k.glds = with_g <= gtab_budget;
       k.lds = k.glds ? with_g : no_g;
@@ -980,6 +993,23 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
 // One sweep-kernel launch per class on the batch's stream; each chain
 // advances by its entry of d_nmoves.
 void enqueue_sweeps(cemc_batch* b, uint32_t* d_log, int64_t log_words) {
+  const bool fan_out = b->classes.size() > 1;
+  if (fan_out) {
+    if (b->side.empty()) {
+      for (int i = 0; i < cemc_batch::kSideStreams; ++i) {
+        hipStream_t st = nullptr;
+        HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        b->side.push_back(st);
+        hipEvent_t ev = nullptr;
+        HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        b->side_done.push_back(ev);
+      }
+      HIP_CHECK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
+    }
+    HIP_CHECK(hipEventRecord(b->fork_ev, b->stream));
+    for (auto st : b->side) HIP_CHECK(hipStreamWaitEvent(st, b->fork_ev, 0));
+  }
+  int ci = 0;
   for (const auto& cl : b->classes) {
     LaunchSweep a;
     a.params = b->d_params;
@@ -1000,7 +1030,16 @@ void enqueue_sweeps(cemc_batch* b, uint32_t* d_log, int64_t log_words) {
     a.spec = cl.spec;
     a.accept_log = d_log;
     a.log_words = log_words;
-    HIP_CHECK(launch_sweep(a, b->stream));
+    hipStream_t st = fan_out ? b->side[ci % cemc_batch::kSideStreams] : b->stream;
+    HIP_CHECK(launch_sweep(a, st));
+    ++ci;
+  }
+  if (fan_out) {
+    const int used = std::min<int>(ci, cemc_batch::kSideStreams);
+    for (int i = 0; i < used; ++i) {
+      HIP_CHECK(hipEventRecord(b->side_done[i], b->side[i]));
+      HIP_CHECK(hipStreamWaitEvent(b->stream, b->side_done[i], 0));
+    }
   }
 }
 
@@ -1516,7 +1555,7 @@ int cemc_batch_set_waves(cemc_batch* b, int32_t waves) {
 int cemc_batch_set_moves_per_round(cemc_batch* b, int32_t moves) {
   return guarded([&] {
     require(b != nullptr, "NULL batch");
-    require(moves == 1 || moves == kSpecMoves, "moves per round must be 1 or 8");
+    require(moves == 0 || moves == 1 || moves == kSpecMoves, "moves per round must be 0 (automatic), 1 or 8");
     HIP_CHECK(hipStreamSynchronize(b->stream));
     b->spec_moves = moves;
     if (!b->dynamic) derive_classes(b, b->forced_waves);
@@ -1562,6 +1601,7 @@ int cemc_pool_create(int32_t max_chains, int32_t max_models, int64_t occ_bytes, 
     auto* b = new cemc_batch();
     try {
       b->dynamic = true;
+      b->spec_moves = 0;  // moves per round chosen per class (see derive_classes)
       b->n_chains = max_chains;
       b->model_capacity = max_models;
       b->active.assign(max_chains, 0);
