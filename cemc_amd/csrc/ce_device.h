@@ -230,34 +230,65 @@ __device__ inline double dpp_f64(double v) {
   return __longlong_as_double(int64_t((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo)));
 }
 
-// Sum over the 16 lanes of each DPP row; every lane of the row ends with the
-// row total.  Order of additions is fixed, so the result is deterministic.
-__device__ inline double row_sum_f64(double v) {
-  v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
-  v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
-  v += dpp_f64<0x141>(v);  // row_half_mirror
-  v += dpp_f64<0x140>(v);  // row_mirror
+// Lane pairs of a wave sum, in the order they are added: lanes 32 apart,
+// 16 apart, 8 apart, then 1, 2 and 4 apart.  Every reduction of the sweep
+// kernel follows this one tree, so a sum taken alone (wave_sum_f64) and the
+// same sum taken among eight (wave_sum8_f64) are bitwise equal.
+
+__device__ inline double join_f64(uint32_t lo, uint32_t hi) {
+  return __longlong_as_double(int64_t((uint64_t(hi) << 32) | lo));
+}
+__device__ inline uint32_t lo_of(double v) { return uint32_t(uint64_t(__double_as_longlong(v))); }
+__device__ inline uint32_t hi_of(double v) { return uint32_t(uint64_t(__double_as_longlong(v)) >> 32); }
+
+// x + (x of the lane 32 apart) in lanes 0-31 and y + (y of the lane 32
+// apart) in lanes 32-63 (v_permlane32_swap exchanges the upper half of x
+// with the lower half of y).
+__device__ inline double pair32_f64(double x, double y) {
+  const auto lo = __builtin_amdgcn_permlane32_swap(lo_of(x), lo_of(y), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap(hi_of(x), hi_of(y), false, false);
+  return join_f64(lo[0], hi[0]) + join_f64(lo[1], hi[1]);
+}
+
+// x + (x of the lane 16 apart) in rows 0 and 2, y + (y 16 apart) in rows 1
+// and 3 (v_permlane16_swap exchanges odd rows of x with even rows of y).
+__device__ inline double pair16_f64(double x, double y) {
+  const auto lo = __builtin_amdgcn_permlane16_swap(lo_of(x), lo_of(y), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap(hi_of(x), hi_of(y), false, false);
+  return join_f64(lo[0], hi[0]) + join_f64(lo[1], hi[1]);
+}
+
+// Last three levels, within each group of 8 lanes: pairs 1 and 2 apart
+// (quad_perm [1,0,3,2], [2,3,0,1]), then 4 apart (row_half_mirror: the quads
+// are uniform by then).
+__device__ inline double octet_sum_f64(double v) {
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
   return v;
 }
 
-// Sum over all 64 lanes; every lane ends with the same total (rows combined
-// with the gfx950 cross-row swaps, (r0 + r1) + (r2 + r3) in every lane).
+// Sum over all 64 lanes; every lane ends with the same total.
 __device__ inline double wave_sum_f64(double v) {
-  v = row_sum_f64(v);
-  const int64_t b = __double_as_longlong(v);
-  const auto lo16 = __builtin_amdgcn_permlane16_swap(uint32_t(b), uint32_t(b), false, false);
-  const auto hi16 = __builtin_amdgcn_permlane16_swap(uint32_t(uint64_t(b) >> 32), uint32_t(uint64_t(b) >> 32),
-                                                     false, false);
-  const double even = __longlong_as_double(int64_t((uint64_t(hi16[0]) << 32) | lo16[0]));  // r0 / r2
-  const double odd = __longlong_as_double(int64_t((uint64_t(hi16[1]) << 32) | lo16[1]));   // r1 / r3
-  v = even + odd;
-  const int64_t c = __double_as_longlong(v);
-  const auto lo32 = __builtin_amdgcn_permlane32_swap(uint32_t(c), uint32_t(c), false, false);
-  const auto hi32 = __builtin_amdgcn_permlane32_swap(uint32_t(uint64_t(c) >> 32), uint32_t(uint64_t(c) >> 32),
-                                                     false, false);
-  const double lower = __longlong_as_double(int64_t((uint64_t(hi32[0]) << 32) | lo32[0]));  // r0 + r1
-  const double upper = __longlong_as_double(int64_t((uint64_t(hi32[1]) << 32) | lo32[1]));  // r2 + r3
-  return lower + upper;
+  v = pair32_f64(v, v);
+  v = pair16_f64(v, v);
+  v += dpp_f64<0x128>(v);  // row_ror:8, the lane 8 apart within the row
+  return octet_sum_f64(v);
+}
+
+// Eight wave sums at once: lane L ends with the wave sum of x[L >> 3], added
+// in the order of wave_sum_f64.  Each level halves the values a lane holds
+// (8 -> 4 -> 2 -> 1), so the eight sums cost ten additions per lane.
+__device__ inline double wave_sum8_f64(const double (&x)[8], int lane) {
+  double q[4], r[2];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) q[j] = pair32_f64(x[j], x[j + 4]);  // lane bit 5 picks j or j + 4
+#pragma unroll
+  for (int j = 0; j < 2; ++j) r[j] = pair16_f64(q[j], q[j + 2]);  // lane bit 4 picks j or j + 2
+  const bool b3 = (lane & 8) != 0;                                // lane bit 3 picks r[0] or r[1]
+  const double keep = b3 ? r[1] : r[0];
+  const double send = b3 ? r[0] : r[1];
+  return octet_sum_f64(keep + dpp_f64<0x128>(send));
 }
 
 // ---------------------------------------------------------------------------

@@ -161,11 +161,24 @@ __device__ __attribute__((always_inline)) inline void table_index(const SweepCtx
   }
 }
 
+// The energy tables: in LDS, or in HBM through a global-address-space
+// pointer (global loads count only against vmcnt; a flat load would also
+// wait for, and hold up, the LDS gathers).
+using GlobalDoublePtr = const __attribute__((address_space(1))) double*;
+
+template <bool kLds>
+__device__ __attribute__((always_inline)) inline auto table_ptr(const double* lds_tab, const double* hbm_tab) {
+  if constexpr (kLds)
+    return lds_tab;
+  else
+    return (GlobalDoublePtr)hbm_tab;
+}
+
 // Phase 2: energy-table reads.  kBM == 1 (two species): the plain sum of the
 // entries, to be scaled by the single basis difference after the reduction;
 // otherwise the change for basis differences dphi.
-template <int kR, int kBM>
-__device__ __attribute__((always_inline)) inline double table_sum(const int (&gi)[kR], const double* gtab,
+template <int kR, int kBM, typename TabPtr>
+__device__ __attribute__((always_inline)) inline double table_sum(const int (&gi)[kR], TabPtr gtab,
                                                                   const double (&dphi)[kBM]) {
   constexpr int kCodes = TableGeom<kBM>::kCodes;
   double g[kR][kBM];
@@ -173,14 +186,15 @@ __device__ __attribute__((always_inline)) inline double table_sum(const int (&gi
   for (int k = 0; k < kR; ++k)
 #pragma unroll
     for (int d = 0; d < kBM; ++d) g[k][d] = gtab[gi[k] + d * kCodes];
-  double v = 0.0;
+  double v = kBM == 1 ? g[0][0] : dphi[0] * g[0][0];
 #pragma unroll
   for (int k = 0; k < kR; ++k) {
     if constexpr (kBM == 1) {
-      v += g[k][0];
+      if (k > 0) v += g[k][0];
     } else {
 #pragma unroll
-      for (int d = 0; d < kBM; ++d) v += dphi[d] * g[k][d];
+      for (int d = 0; d < kBM; ++d)
+        if (k > 0 || d > 0) v = fma(dphi[d], g[k][d], v);
     }
   }
   return v;
@@ -188,10 +202,10 @@ __device__ __attribute__((always_inline)) inline double table_sum(const int (&gi
 
 // Records of one translation group read from LDS in batches of 4 per lane
 // (thread `tid` of `nthreads` working on the chain).
-template <bool kPeriodic, int kBM, bool kPatch>
+template <bool kPeriodic, int kBM, bool kPatch, typename TabPtr>
 __device__ __attribute__((always_inline)) inline double lds_flip_de(const SweepCtx& c, const FatRec* recs, int off,
                                                                     int cnt, int tid, int nthreads, const Ref& ref,
-                                                                    const uint8_t* occ, const double* gtab,
+                                                                    const uint8_t* occ, TabPtr gtab,
                                                                     const double (&dphi)[kBM], uint32_t patch_w,
                                                                     uint32_t patch_species) {
   double v = 0.0;
@@ -254,33 +268,6 @@ __device__ __attribute__((always_inline)) inline double group_sum(double x, doub
 // from the same ordered sum, and every wave writes the same commit; one
 // barrier per move (inside group_sum) separates a move's commit from the
 // next move's gathers in other waves.
-// Sums of kK values over the kW waves of the workgroup (see group_sum);
-// `slots` holds kK * 8 doubles.
-template <int kW, int kK>
-__device__ __attribute__((always_inline)) inline void group_sum_k(const double (&x)[kK], double (&out)[kK],
-                                                                  double* slots, int wave, int lane) {
-  double w[kK];
-#pragma unroll
-  for (int k = 0; k < kK; ++k) w[k] = wave_sum_f64(x[k]);
-  if constexpr (kW == 1) {
-#pragma unroll
-    for (int k = 0; k < kK; ++k) out[k] = w[k];
-  } else {
-    if (lane == 0) {
-#pragma unroll
-      for (int k = 0; k < kK; ++k) slots[k * 8 + wave] = w[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kK; ++k) {
-      double t = slots[k * 8];
-#pragma unroll
-      for (int v = 1; v < kW; ++v) t += slots[k * 8 + v];
-      out[k] = t;
-    }
-  }
-}
-
 // True when a change at packed cell a (x | y<<8 | z<<16) can alter the
 // energy change of a flip at packed cell b: a - b (wrapped) is 0 or a
 // sub-cluster member offset, per the model's interaction mask.
@@ -381,7 +368,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
     for (int i = tid; i < m.n_rec; i += kThreads) recs_l[i] = m.frecs[i];
   if constexpr (kGtabLds)
     for (int i = tid; i < m.gtab_size; i += kThreads) gtab_l[i] = m.gtab[i];
-  const double* gtab = kGtabLds ? gtab_l : m.gtab;
+  const auto gtab = table_ptr<kGtabLds>(gtab_l, m.gtab);
   if (tid < 2 * kMaxGroups) grp_tab[tid] = tid < kMaxGroups ? m.grp_rec_off[tid] : m.grp_rec_cnt[tid - kMaxGroups];
   for (int i = tid; i < c.B * c.S; i += kThreads) phi_tab[i] = m.basis[i];
   for (int i = tid; i < c.S * c.S; i += kThreads) {
@@ -447,23 +434,23 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
   const uint64_t seg_lo = pp->segoff_pk[0], seg_hi = pp->segoff_pk[1];
   uint32_t* acc_log = accept_log ? accept_log + int64_t(chain) * log_words : nullptr;
 
+  // Samples between accepted moves see the same state: the sums take each
+  // run of equal samples at once (run = samples of the current state not yet
+  // added).
+  int64_t run = 0;
+  auto flush = [&]() {
+    const double dn = double(run);
+    const double de_ref = energy - e_ref;
+    sum_de = fma(dn, de_ref, sum_de);
+    sum_de2 = fma(dn * de_ref, de_ref, sum_de2);
+    if constexpr (!kCanonical) {
+      sum_s = fma(dn, sing, sum_s);
+      sum_s2 = fma(dn * sing, sing, sum_s2);
+      sum_se = fma(dn * sing, de_ref, sum_se);
+    }
+    run = 0;
+  };
   if constexpr (kK == 1) {
-    // Samples between accepted moves see the same state: the sums take each
-    // run of equal samples at once (run = samples of the current state not
-    // yet added).
-    int64_t run = 0;
-    auto flush = [&]() {
-      const double dn = double(run);
-      const double de_ref = energy - e_ref;
-      sum_de = fma(dn, de_ref, sum_de);
-      sum_de2 = fma(dn * de_ref, de_ref, sum_de2);
-      if constexpr (!kCanonical) {
-        sum_s = fma(dn, sing, sum_s);
-        sum_s2 = fma(dn * sing, sing, sum_s2);
-        sum_se = fma(dn * sing, de_ref, sum_se);
-      }
-      run = 0;
-    };
   for (int64_t blk0 = 0; blk0 < nmov; blk0 += 64) {
     // lane l holds the Philox block and log u of move blk0 + l, and decodes
     // the parts of that move that do not depend on the state
@@ -541,9 +528,9 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
       double dsing = 0.0;
       if constexpr (!kCanonical) {
         if constexpr (kBM == 1)
-          dsing = dphi[0] * inv_n;
+          dsing = __dmul_rn(dphi[0], inv_n);
         else
-          dsing = dphi_tab[(int(old_i) * c.S + int(new_i)) * c.B + (lane < c.B ? lane : 0)] * inv_n;
+          dsing = __dmul_rn(dphi_tab[(int(old_i) * c.S + int(new_i)) * c.B + (lane < c.B ? lane : 0)], inv_n);
       }
       double dphj[kBM];
 #pragma unroll
@@ -569,7 +556,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         }
       }
       const double sum = group_sum<kW>(part, red_move[l & 1], wave, lane);
-      const double dE = (kBM == 1 ? dphi[0] * sum : sum) + de_single;
+      const double dE = kBM == 1 ? fma(dphi[0], sum, de_single) : sum + de_single;
       // Metropolis test of the reference: accept if E_new < E_cur, else if
       // u <= exp(-(E_new - E_cur) / kT) (cemc/mcmc/montecarlo.py:950-956).  The
       // comparison is first made against log(u) with a margin far above the
@@ -629,17 +616,22 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
   }
     flush();
   } else {
-    // ---- speculative rounds (closed-form periodic lattice, records in
-    // VGPRs, one translation group, no background).  Up to kK consecutive
-    // moves are evaluated against the same state, then decided and
-    // committed in order.  An accepted move invalidates the later moves of
-    // the round that read what it changed (its sites or their sub-cluster
-    // members, or its tracker slots); the round ends before the first
-    // invalidated move, which the next round evaluates again.  Every
-    // committed move's energy change is therefore bitwise the one the
-    // sequential loop computes.
-    static_assert(kPeriodic && kR > 0, "speculative rounds need a periodic lattice and records in VGPRs");
+    // ---- rounds of kK = 8 moves (closed-form periodic lattice, records in
+    // VGPRs, one translation group, no background).  The eight moves are
+    // evaluated against the same state and their eight wave sums taken
+    // together; lane L then decides move L >> 3 of the round.  An accepted
+    // move invalidates the later moves of the round that read what it
+    // changed (its sites or their sub-cluster members, or its tracker
+    // slots): lane L checks move L & 7 against move L >> 3, and the round
+    // ends before the first invalidated move, which the next round
+    // evaluates again.  The committed moves of a round have disjoint
+    // footprints, so their lanes write them at once; energy, singlets and
+    // sample sums then advance in move order.  Every value therefore equals
+    // the sequential loop's bitwise (same sums, same tree, same order).
+    static_assert(kPeriodic && kR > 0 && kK == 8, "rounds need a periodic lattice, records in VGPRs, 8 moves");
     const uint64_t im0 = m.imask[0], im1 = m.imask[1];
+    const int grp = lane >> 3;  // the move of the round this lane decides and commits
+    const int sub = lane & 7;   // the earlier move it is checked against; its image (commit)
     uint32_t rnd = 0;
     for (int64_t blk0 = 0; blk0 < nmov; blk0 += 64) {
       const Philox4 blk = philox4x64_10(t0 + uint64_t(blk0) + uint64_t(lane), 0, 0, 0, seed, uid);
@@ -648,7 +640,8 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
       // lane l decodes move blk0 + l once per block
       uint32_t v_old = 0, v_new = 0;  // canonical: species a and b; SGC: offset of the new species
       int v_ta = 0, v_tb = 0;         // canonical: tracker slots of the two sites
-      int v_site = 0;                 // SGC: the site
+      int v_p = 0;                    // SGC: LDS index of the site
+      uint32_t v_cell = 0;            // SGC: packed cell of the site
       if constexpr (kCanonical) {
         const uint32_t ia = bounded32(uint32_t(blk.w0 >> 32), np);
         uint32_t ib = ia + 1u + bounded32(uint32_t(blk.w0), np - 1u);
@@ -658,164 +651,194 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         v_ta = unpack16(seg_lo, seg_hi, ia) + int(bounded64(blk.w1, uint64_t(unpack16(cnt_lo, cnt_hi, ia))));
         v_tb = unpack16(seg_lo, seg_hi, ib) + int(bounded64(blk.w2, uint64_t(unpack16(cnt_lo, cnt_hi, ib))));
       } else {
-        v_site = int(bounded64(blk.w1, uint64_t(N)));
+        const Ref r = ref_of<true>(c, int(bounded64(blk.w1, uint64_t(N))));
+        v_p = r.p;
+        v_cell = uint32_t(r.x) | (uint32_t(r.y) << 8) | (uint32_t(r.z) << 16);
         v_new = bounded32(uint32_t(blk.w0), uint32_t(c.S - 1));
       }
       uint64_t acc_bits = 0;
       int l0 = 0;
       while (l0 < n_in) {
         const int n = n_in - l0 < kK ? n_in - l0 : kK;
-        // every lane looks up its own move against the current state
-        int si_v = v_site, sj_v = 0;
+        // every lane looks up its own move in the current state
+        int p_i = v_p, p_j = 0;
+        uint32_t cell_i = v_cell, cell_j = 0, patch = 0, o_v = v_old, n_v = v_new;
+        uint32_t sites = 0, slots = 0;  // canonical: si | sj << 16, ta | tb << 16
         if constexpr (kCanonical) {
-          si_v = trk[v_ta];
-          sj_v = trk[v_tb];
-        }
-        const Ref ri_v = ref_of<true>(c, si_v);
-        Ref rj_v = ri_v;
-        uint32_t patch_v = 0, old_v = v_old, new_v = v_new;
-        if constexpr (kCanonical) {
-          rj_v = ref_of<true>(c, sj_v);
-          patch_v = wrapped_delta(c, ri_v, rj_v);
+          const int si = trk[v_ta], sj = trk[v_tb];
+          const Ref ri = ref_of<true>(c, si), rj = ref_of<true>(c, sj);
+          p_i = ri.p;
+          p_j = rj.p;
+          cell_i = uint32_t(ri.x) | (uint32_t(ri.y) << 8) | (uint32_t(ri.z) << 16);
+          cell_j = uint32_t(rj.x) | (uint32_t(rj.y) << 8) | (uint32_t(rj.z) << 16);
+          patch = wrapped_delta(c, ri, rj);
+          sites = uint32_t(si) | (uint32_t(sj) << 16);
+          slots = uint32_t(v_ta) | (uint32_t(v_tb) << 16);
         } else {
-          old_v = occ[ri_v.p];
-          new_v = old_v + 1u + v_new;
-          new_v -= new_v >= uint32_t(c.S) ? uint32_t(c.S) : 0u;
+          o_v = occ[v_p];
+          n_v = o_v + 1u + v_new;
+          n_v -= n_v >= uint32_t(c.S) ? uint32_t(c.S) : 0u;
         }
-        const uint32_t cell_i = uint32_t(ri_v.x) | (uint32_t(ri_v.y) << 8) | (uint32_t(ri_v.z) << 16);
-        const uint32_t cell_j = uint32_t(rj_v.x) | (uint32_t(rj_v.y) << 8) | (uint32_t(rj_v.z) << 16);
+        // the lanes of move grp fetch it (lane l0 + grp) and the move they
+        // check it against (lane l0 + sub); slots past the block read lane 63
+        const int src_b = 4 * (l0 + grp < 63 ? l0 + grp : 63);
+        const int src_a = 4 * (l0 + sub < 63 ? l0 + sub : 63);
+        const double lu_b = join_f64(uint32_t(__builtin_amdgcn_ds_bpermute(src_b, int(lo_of(log_u)))),
+                                     uint32_t(__builtin_amdgcn_ds_bpermute(src_b, int(hi_of(log_u)))));
+        const uint32_t cib = uint32_t(__builtin_amdgcn_ds_bpermute(src_b, int(cell_i | (o_v << 24) | (n_v << 28))));
+        const uint32_t cia = uint32_t(__builtin_amdgcn_ds_bpermute(src_a, int(cell_i)));
+        uint32_t cjb = 0, cja = 0, slb = 0, sla = 0, stb = 0;
+        if constexpr (kCanonical) {
+          cjb = uint32_t(__builtin_amdgcn_ds_bpermute(src_b, int(cell_j)));
+          cja = uint32_t(__builtin_amdgcn_ds_bpermute(src_a, int(cell_j)));
+          slb = uint32_t(__builtin_amdgcn_ds_bpermute(src_b, int(slots)));
+          sla = uint32_t(__builtin_amdgcn_ds_bpermute(src_a, int(slots)));
+          stb = uint32_t(__builtin_amdgcn_ds_bpermute(src_b, int(sites)));
+        }
+        const uint32_t o_b = (cib >> 24) & 15u, n_b = cib >> 28;
         // each lane's records for every move of the round
-        double part[kK], dphi0[kK], desng[kK], dsg[kK];
+        double part[kK];
 #pragma unroll
         for (int k = 0; k < kK; ++k) {
-          part[k] = 0.0;
-          dphi0[k] = 0.0;
-          desng[k] = 0.0;
-          dsg[k] = 0.0;
-          if (k < n) {
-            const int lk = l0 + k;
-            Ref ri;
-            ri.p = __builtin_amdgcn_readlane(ri_v.p, lk);
-            ri.site = 0;
-            ri.x = ri.y = ri.z = 0;
-            const uint32_t o_k = __builtin_amdgcn_readlane(old_v, lk);
-            const uint32_t n_k = __builtin_amdgcn_readlane(new_v, lk);
-            double dphi[kBM];
-            if constexpr (kBM == 1) {
-              dphi[0] = o_k == 0 ? dphi01 : dphi10;
-              dphi0[k] = dphi[0];
-              if constexpr (!kCanonical) {
-                desng[k] = o_k == 0 ? dsin01 : dsin10;
-                dsg[k] = dphi[0] * inv_n;
-              }
-            } else {
-              const int pair = int(o_k) * c.S + int(n_k);
+          const int lk = l0 + k < 63 ? l0 + k : 63;
+          Ref ri;
+          ri.p = __builtin_amdgcn_readlane(p_i, lk);
+          ri.site = 0;
+          ri.x = ri.y = ri.z = 0;
+          int pj_k = kCanonical ? __builtin_amdgcn_readlane(p_j, lk) : 0;
+          // many-species models with many records per lane (or with eight
+          // waves, 256 registers each) evaluate the moves one after the
+          // other (the addresses of move k wait for the sum of move k - 1);
+          // hoisting every move's loads to the top of the round would
+          // overflow the registers
+          if constexpr (kBM > 1 && (kR >= 4 || kW >= 8))
+            if (k > 0) asm volatile("" : "+s"(ri.p), "+s"(pj_k) : "v"(part[k - 1]));
+          double dphi[kBM];
+          if constexpr (kBM == 1) {
+            dphi[0] = 0.0;  // two species: the entries are summed plain
+          } else {
+            const int pair =
+                int(__builtin_amdgcn_readlane(o_v, lk)) * c.S + int(__builtin_amdgcn_readlane(n_v, lk));
 #pragma unroll
-              for (int d = 0; d < kBM; ++d) dphi[d] = d < c.B ? dphi_tab[pair * c.B + d] : 0.0;
-              if constexpr (!kCanonical) {
-                desng[k] = dsingle_tab[pair];
-                dsg[k] = dphi_tab[pair * c.B + (lane < c.B ? lane : 0)] * inv_n;
-              }
-            }
-            int gi_i[kR];
-            table_index<true, kR, kBM, false>(c, rr, ri, occ, 0u, 0u, gi_i);
-            double pk = table_sum<kR, kBM>(gi_i, gtab, dphi);
+            for (int d = 0; d < kBM; ++d) dphi[d] = d < c.B ? dphi_tab[pair * c.B + d] : 0.0;
+          }
+          int gi_i[kR];
+          table_index<true, kR, kBM, false>(c, rr, ri, occ, 0u, 0u, gi_i);
+          double pk = table_sum<kR, kBM>(gi_i, gtab, dphi);
+          if constexpr (kCanonical) {
+            Ref rj;
+            rj.p = pj_k;
+            rj.site = 0;
+            rj.x = rj.y = rj.z = 0;
+            double dphj[kBM];
+#pragma unroll
+            for (int d = 0; d < kBM; ++d) dphj[d] = -dphi[d];
+            int gi_j[kR];
+            table_index<true, kR, kBM, true>(c, rr, rj, occ, __builtin_amdgcn_readlane(patch, lk),
+                                             __builtin_amdgcn_readlane(n_v, lk), gi_j);
+            const double pj = table_sum<kR, kBM>(gi_j, gtab, dphj);
+            pk = kBM == 1 ? pk - pj : pk + pj;
+          }
+          part[k] = pk;
+        }
+        // lane L: the energy sum of move L >> 3 (over the waves in order)
+        double tot = wave_sum8_f64(part, lane);
+        if constexpr (kW > 1) {
+          double* sl = red_spec[rnd & 1u];
+          if (sub == 0) sl[grp * 8 + wave] = tot;
+          __syncthreads();
+          tot = sl[grp * 8];
+#pragma unroll
+          for (int v = 1; v < kW; ++v) tot += sl[grp * 8 + v];
+        }
+        double dphi0 = 0.0, de_single = 0.0, dsl = 0.0;
+        if constexpr (kBM == 1) {
+          dphi0 = o_b == 0 ? dphi01 : dphi10;
+          if constexpr (!kCanonical) {
+            de_single = o_b == 0 ? dsin01 : dsin10;
+            dsl = __dmul_rn(dphi0, inv_n);
+          }
+        } else if constexpr (!kCanonical) {
+          const int pair = int(o_b) * c.S + int(n_b);
+          de_single = dsingle_tab[pair];
+          // lane (move, d): singlet d's change, d < B
+          dsl = __dmul_rn(dphi_tab[pair * c.B + (sub < c.B ? sub : 0)], inv_n);
+        }
+        const double dE = kBM == 1 ? fma(dphi0, tot, de_single) : tot + de_single;
+        // the Metropolis test of the sequential loop, per move
+        const double x = -dE * inv_kT;
+        const double margin = 1e-12 * (1.0 + fabs(x));
+        const bool valid = grp < n;
+        int verdict = (dE < 0.0 || lu_b < x - margin) ? 1 : (lu_b > x + margin ? 0 : 2);
+        uint64_t unsure = __builtin_amdgcn_ballot_w64(valid && verdict == 2);
+        while (unsure != 0) {
+          const int lb = int(__builtin_ctzll(unsure));
+          const int k = lb >> 3;
+          const double dEk = readlane_f64(dE, lb);
+          const int ok = int(unit_double(readlane_u64(blk.w3, l0 + k)) <= exp(-dEk / kT));
+          verdict = grp == k ? ok : verdict;
+          unsure &= ~(uint64_t(0xFF) << lb);
+        }
+        const uint64_t acc = __builtin_amdgcn_ballot_w64(valid && verdict == 1);  // byte k: move k accepted
+        int stop = n;
+        if (acc != 0) {
+          bool hit = false;
+          if (sub < grp && ((acc >> (8 * sub)) & 1u) != 0) {
+            const uint32_t ib = cib & 0xFFFFFFu;
+            hit = cells_interact(cia, ib, c, im0, im1);
             if constexpr (kCanonical) {
-              Ref rj;
-              rj.p = __builtin_amdgcn_readlane(rj_v.p, lk);
-              rj.site = 0;
-              rj.x = rj.y = rj.z = 0;
-              const uint32_t patch = __builtin_amdgcn_readlane(patch_v, lk);
-              double dphj[kBM];
-#pragma unroll
-              for (int d = 0; d < kBM; ++d) dphj[d] = -dphi[d];
-              int gi_j[kR];
-              table_index<true, kR, kBM, true>(c, rr, rj, occ, patch, n_k, gi_j);
-              const double pj = table_sum<kR, kBM>(gi_j, gtab, dphj);
-              pk = kBM == 1 ? pk - pj : pk + pj;
-            }
-            part[k] = pk;
-          }
-        }
-        double tot[kK];
-        group_sum_k<kW, kK>(part, tot, red_spec[rnd & 1u], wave, lane);
-        // in-order decisions and commits
-        uint32_t invalid = 0;
-        bool stop = false;
-        int done = 0;
-#pragma unroll
-        for (int k = 0; k < kK; ++k) {
-          if (!stop && k < n) {
-            if ((invalid >> k) & 1u) {
-              stop = true;
-            } else {
-              done = k + 1;
-              const int lk = l0 + k;
-              const double dE = (kBM == 1 ? dphi0[k] * tot[k] : tot[k]) + desng[k];
-              const double lu = readlane_f64(log_u, lk);
-              bool accept;
-              if (dE < 0.0) {
-                accept = true;
-              } else {
-                const double x = -dE * inv_kT;
-                const double margin = 1e-12 * (1.0 + fabs(x));
-                if (lu < x - margin)
-                  accept = true;
-                else if (lu > x + margin)
-                  accept = false;
-                else
-                  accept = unit_double(readlane_u64(blk.w3, lk)) <= exp(-dE / kT);
-              }
-              if (accept) {
-                const uint32_t ci = __builtin_amdgcn_readlane(cell_i, lk);
-                const uint32_t cj = __builtin_amdgcn_readlane(cell_j, lk);
-                const uint32_t o_k = __builtin_amdgcn_readlane(old_v, lk);
-                const uint32_t n_k = __builtin_amdgcn_readlane(new_v, lk);
-                // lanes 0-7 write the images of site i, lanes 8-15 those of site j
-                const bool second = kCanonical && lane >= 8 && lane < 16;
-                const uint32_t cs = second ? cj : ci;
-                const int idx = lane < (kCanonical ? 16 : 8)
-                                    ? image_index(c, int(cs & 255u), int((cs >> 8) & 255u), int(cs >> 16), lane & 7)
-                                    : -1;
-                if (idx >= 0) occ[idx] = uint8_t(second ? o_k : n_k);
-                int ta_k = 0, tb_k = 0;
-                if constexpr (kCanonical) {
-                  ta_k = __builtin_amdgcn_readlane(v_ta, lk);
-                  tb_k = __builtin_amdgcn_readlane(v_tb, lk);
-                  const int si = __builtin_amdgcn_readlane(si_v, lk);
-                  const int sj = __builtin_amdgcn_readlane(sj_v, lk);
-                  if (lane == 0) {
-                    trk[ta_k] = uint16_t(sj);
-                    trk[tb_k] = uint16_t(si);
-                  }
-                }
-                energy += dE;
-                if constexpr (!kCanonical) sing += dsg[k];
-                acc_bits |= uint64_t(1) << lk;
-                // later moves of the round that read what this move changed
-                bool hit = false;
-                if (lane > lk && lane < l0 + n) {
-                  hit = cells_interact(ci, cell_i, c, im0, im1);
-                  if constexpr (kCanonical) {
-                    hit = hit || cells_interact(ci, cell_j, c, im0, im1) || cells_interact(cj, cell_i, c, im0, im1) ||
-                          cells_interact(cj, cell_j, c, im0, im1) || v_ta == ta_k || v_ta == tb_k || v_tb == ta_k ||
-                          v_tb == tb_k;
-                  }
-                }
-                invalid |= uint32_t(__ballot(hit) >> l0);
-              }
-              const double de_ref = energy - e_ref;
-              sum_de += de_ref;
-              sum_de2 = fma(de_ref, de_ref, sum_de2);
-              if constexpr (!kCanonical) {
-                sum_s += sing;
-                sum_s2 = fma(sing, sing, sum_s2);
-                sum_se = fma(sing, de_ref, sum_se);
-              }
+              const uint32_t ta_a = sla & 0xFFFFu, tb_a = sla >> 16, ta_b = slb & 0xFFFFu, tb_b = slb >> 16;
+              hit = hit || cells_interact(cia, cjb, c, im0, im1) || cells_interact(cja, ib, c, im0, im1) ||
+                    cells_interact(cja, cjb, c, im0, im1) || ta_a == ta_b || ta_a == tb_b || tb_a == ta_b ||
+                    tb_a == tb_b;
             }
           }
+          const uint64_t hits = __builtin_amdgcn_ballot_w64(hit);
+          if (hits != 0) {
+            const int first = int(__builtin_ctzll(hits)) >> 3;
+            stop = first < stop ? first : stop;
+          }
         }
-        l0 += done;
+        const uint64_t com = acc & (stop >= 8 ? ~uint64_t(0) : (uint64_t(1) << (8 * stop)) - 1u);
+        int cursor = 0;
+        if (com != 0) {
+          if (((com >> lane) & 1u) != 0) {
+            // the lanes of a committed move write its images (site j's too)
+            const uint32_t ib = cib & 0xFFFFFFu;
+            const int idx = image_index(c, int(ib & 255u), int((ib >> 8) & 255u), int(ib >> 16), sub);
+            if (idx >= 0) occ[idx] = uint8_t(n_b);
+            if constexpr (kCanonical) {
+              const int jdx = image_index(c, int(cjb & 255u), int((cjb >> 8) & 255u), int(cjb >> 16), sub);
+              if (jdx >= 0) occ[jdx] = uint8_t(o_b);
+              if (sub == 0) trk[slb & 0xFFFFu] = uint16_t(stb >> 16);  // trk[ta] = sj
+              if (sub == 1) trk[slb >> 16] = uint16_t(stb & 0xFFFFu);  // trk[tb] = si
+            }
+          }
+          // energy, singlets and sample sums in move order
+          uint64_t rest = com;
+          while (rest != 0) {
+            const int lb = int(__builtin_ctzll(rest));
+            const int k = lb >> 3;
+            rest &= ~(uint64_t(0xFF) << lb);
+            run += k - cursor;
+            flush();
+            energy += readlane_f64(dE, lb);
+            if constexpr (!kCanonical) {
+              double ds = readlane_f64(dsl, lb);
+#pragma unroll
+              for (int d = 1; d < kBM; ++d) {
+                const double t = readlane_f64(dsl, lb + d);
+                ds = lane == d ? t : ds;
+              }
+              sing += ds;
+            }
+            run += 1;
+            cursor = k + 1;
+            acc_bits |= uint64_t(1) << (l0 + k);
+          }
+        }
+        run += stop - cursor;
+        l0 += stop;
         ++rnd;
         if constexpr (kW == 1) {
           __syncthreads();
@@ -830,6 +853,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         if (lane == 1 && n_in > 32) acc_log[w + 1] = uint32_t(acc_bits >> 32);
       }
     }
+    flush();
   }
 
   // ---- write back
