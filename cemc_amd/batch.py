@@ -172,8 +172,9 @@ class ChainBatch:
         _capi.check(self._lib.cemc_batch_set_waves(self.handle, int(waves)))
 
     def set_moves_per_round(self, moves):
-        """1 (default): every move evaluated on its own; 8: eight consecutive
-        moves evaluated together where the lattice allows it.  Same results."""
+        """0 (default): eight consecutive moves evaluated together wherever
+        the chain's kernel variant allows it; 1: every move on its own; 8:
+        rounds of eight where allowed.  Same results bit for bit."""
         _capi.check(self._lib.cemc_batch_set_moves_per_round(self.handle, int(moves)))
 
     def launch_shape(self, chain):

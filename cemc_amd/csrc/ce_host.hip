@@ -715,7 +715,7 @@ struct cemc_batch {
     int begin, count;
     uint32_t lds;
   };
-  int spec_moves = 1;  // moves per round where the model allows it (1 or kSpecMoves)
+  int spec_moves = 0;  // moves per round where the model allows it (0: kSpecMoves, 1 or kSpecMoves)
   std::vector<Class> classes;
   int64_t classes_version = 0;  // bumped whenever the launch classes change
   int forced_waves = 0;
@@ -934,10 +934,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
       k.waves = auto_waves;
       k.kr = pick_kr(k.waves);
     }
-    // pools (spec_moves = 0): eight-move rounds for chains that run on four
-    // or more waves (few resident chains), where they hide barrier and
-    // memory waits; one move per round otherwise
-    const int want_spec = b->spec_moves > 0 ? b->spec_moves : (k.waves >= 4 ? kSpecMoves : 1);
+    const int want_spec = b->spec_moves > 0 ? b->spec_moves : kSpecMoves;
     k.spec = (k.kr > 0 && m->spec_ok) ? want_spec : 1;
     if (k.kr > 0) {
       k.glds = with_g <= gtab_budget;
@@ -1601,7 +1598,6 @@ int cemc_pool_create(int32_t max_chains, int32_t max_models, int64_t occ_bytes, 
     auto* b = new cemc_batch();
     try {
       b->dynamic = true;
-      b->spec_moves = 0;  // moves per round chosen per class (see derive_classes)
       b->n_chains = max_chains;
       b->model_capacity = max_models;
       b->active.assign(max_chains, 0);

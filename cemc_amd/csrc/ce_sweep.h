@@ -58,6 +58,7 @@ struct SweepCtx {
   int L1, L2, L3, L23;
   int h, P2, P3;         // halo width and padded dims (periodic)
   uint32_t mag23, mag3;  // floor((2^32-1)/d) for the divisors L2*L3 and L3
+  int DX, DY, DZ;         // padded-index steps of one lattice period along x, y, z
   int K;
   const int32_t* trans;
 };
@@ -227,20 +228,29 @@ __device__ __attribute__((always_inline)) inline int unpack16(uint64_t lo, uint6
   return int(((i < 4u ? lo : hi) >> (16u * (i & 3u))) & 0xFFFFu);
 }
 
-// Padded index of image `bits` (bit 0: x, 1: y, 2: z take the other image)
-// of the site at cell (x, y, z), or -1 when that image does not exist.
-__device__ __attribute__((always_inline)) inline int image_index(const SweepCtx& c, int x, int y, int z, int bits) {
-  // other image coordinate in each dimension (equal to the coordinate when
-  // the site is not within h of that face); selects only, no branches
-  const int ax = x < c.h ? x + c.L1 : (x >= c.L1 - c.h ? x - c.L1 : x);
-  const int ay = y < c.h ? y + c.L2 : (y >= c.L2 - c.h ? y - c.L2 : y);
-  const int az = z < c.h ? z + c.L3 : (z >= c.L3 - c.h ? z - c.L3 : z);
-  const bool bx = (bits & 1) != 0, by = (bits & 2) != 0, bz = (bits & 4) != 0;
-  const bool ok = (!bx || ax != x) && (!by || ay != y) && (!bz || az != z);
-  const int ix = bx ? ax : x, iy = by ? ay : y, iz = bz ? az : z;
-  return ok ? int(__umul24(__umul24(uint32_t(ix + c.h), uint32_t(c.P2)) + uint32_t(iy + c.h), uint32_t(c.P3)) +
-                  uint32_t(iz + c.h))
-            : -1;
+// Image word of a site on the padded lattice: bits 0-19 its padded index,
+// bits 20+2d and 21+2d the other image along axis d as a signed 2-bit
+// value (0: none, 1: at +L_d, -1: at -L_d; a site within h of a face has
+// one other image along that axis, since L_d > 2h).
+template <bool kPeriodic>
+__device__ __attribute__((always_inline)) inline uint32_t image_word(const SweepCtx& c, const Ref& r) {
+  if constexpr (!kPeriodic) {
+    return uint32_t(r.p);
+  } else {
+    const uint32_t fx = r.x < c.h ? 1u : (r.x >= c.L1 - c.h ? 3u : 0u);
+    const uint32_t fy = r.y < c.h ? 1u : (r.y >= c.L2 - c.h ? 3u : 0u);
+    const uint32_t fz = r.z < c.h ? 1u : (r.z >= c.L3 - c.h ? 3u : 0u);
+    return uint32_t(r.p) | (fx << 20) | (fy << 22) | (fz << 24);
+  }
+}
+
+// Padded index of image `bits` (bit d: take the other image along axis d)
+// of the site with image word w, or -1 when that image does not exist.
+__device__ __attribute__((always_inline)) inline int image_of(const SweepCtx& c, uint32_t w, int bits) {
+  const int ex = int(w << 10) >> 30, ey = int(w << 8) >> 30, ez = int(w << 6) >> 30;
+  const int sx = (bits & 1) ? ex : 0, sy = (bits & 2) ? ey : 0, sz = (bits & 4) ? ez : 0;
+  const bool ok = ((bits & 1) == 0 || ex != 0) && ((bits & 2) == 0 || ey != 0) && ((bits & 4) == 0 || ez != 0);
+  return ok ? int(w & 0xFFFFFu) + sx * c.DX + sy * c.DY + sz * c.DZ : -1;
 }
 
 // Sum of x over the kW waves of the workgroup, added in wave order so every
@@ -261,6 +271,27 @@ __device__ __attribute__((always_inline)) inline double group_sum(double x, doub
   }
 }
 
+// True when a change at packed cell a (x | y<<8 | z<<16) can alter the
+// energy change of a flip at packed cell b: a - b (wrapped) is 0 or a
+// sub-cluster member offset, per the model's interaction mask over the
+// (2h+1)^3 box.  Per axis e = (a - b + h) mod L, the one residue in [0, L),
+// is the smallest of d, d + L, d - L as unsigned values (d = a - b + h); the
+// offset lies in the box when e <= 2h (L > 2h makes it unique).
+__device__ __attribute__((always_inline)) inline bool cells_interact(uint32_t a, uint32_t b, const SweepCtx& c,
+                                                                     uint64_t im0, uint64_t im1) {
+  const uint32_t h = uint32_t(c.h);
+  const uint32_t dx = (a & 255u) + h - (b & 255u);
+  const uint32_t dy = ((a >> 8) & 255u) + h - ((b >> 8) & 255u);
+  const uint32_t dz = (a >> 16) + h - (b >> 16);
+  const uint32_t ex = min(min(dx, dx + uint32_t(c.L1)), dx - uint32_t(c.L1));
+  const uint32_t ey = min(min(dy, dy + uint32_t(c.L2)), dy - uint32_t(c.L2));
+  const uint32_t ez = min(min(dz, dz + uint32_t(c.L3)), dz - uint32_t(c.L3));
+  const uint32_t w = 2u * h + 1u;
+  const uint32_t idx = (ex * w + ey) * w + ez;
+  const uint64_t word = idx < 64u ? im0 : im1;
+  return max(max(ex, ey), ez) <= 2u * h && ((word >> (idx & 63u)) & 1u) != 0;
+}
+
 // kR > 0: single-group model, records in VGPRs (kR per lane); kR == 0:
 // records in LDS.  kBM: decoration slices of the energy tables (1: two
 // species, 3: up to four, 7: up to eight).  kW: waves per chain.  With
@@ -268,27 +299,6 @@ __device__ __attribute__((always_inline)) inline double group_sum(double x, doub
 // from the same ordered sum, and every wave writes the same commit; one
 // barrier per move (inside group_sum) separates a move's commit from the
 // next move's gathers in other waves.
-// True when a change at packed cell a (x | y<<8 | z<<16) can alter the
-// energy change of a flip at packed cell b: a - b (wrapped) is 0 or a
-// sub-cluster member offset, per the model's interaction mask.
-__device__ __attribute__((always_inline)) inline bool cells_interact(uint32_t a, uint32_t b, const SweepCtx& c,
-                                                                     uint64_t im0, uint64_t im1) {
-  int dx = int(a & 255u) - int(b & 255u);
-  int dy = int((a >> 8) & 255u) - int((b >> 8) & 255u);
-  int dz = int(a >> 16) - int(b >> 16);
-  dx -= 2 * dx > c.L1 ? c.L1 : 0;
-  dx += 2 * dx < -c.L1 ? c.L1 : 0;
-  dy -= 2 * dy > c.L2 ? c.L2 : 0;
-  dy += 2 * dy < -c.L2 ? c.L2 : 0;
-  dz -= 2 * dz > c.L3 ? c.L3 : 0;
-  dz += 2 * dz < -c.L3 ? c.L3 : 0;
-  const int h = c.h, w = 2 * c.h + 1;
-  if (dx < -h || dx > h || dy < -h || dy > h || dz < -h || dz > h) return false;
-  const int idx = ((dx + h) * w + (dy + h)) * w + (dz + h);
-  const uint64_t word = idx < 64 ? im0 : im1;
-  return ((word >> (idx & 63)) & 1u) != 0;
-}
-
 template <bool kPeriodic, bool kCanonical, int kR, bool kGtabLds, int kBM, int kW, int kK>
 __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __restrict__ params,
                                                    ChainState* __restrict__ states,
@@ -335,6 +345,9 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
   c.P3 = m.pdims[2];
   c.mag23 = kPeriodic ? 0xFFFFFFFFu / uint32_t(c.L23) : 0u;
   c.mag3 = kPeriodic ? 0xFFFFFFFFu / uint32_t(c.L3) : 0u;
+  c.DX = c.L1 * c.P2 * c.P3;
+  c.DY = c.L2 * c.P3;
+  c.DZ = c.L3;
   c.K = m.K;
   c.trans = m.trans;
   const bool multi_group = m.n_groups > 1 || m.has_background;
@@ -460,7 +473,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
     uint32_t v_old = 0, v_new = 0;  // canonical: species a and b; SGC: offset of the new species
     int v_ta = 0, v_tb = 0;         // canonical: tracker slots of the two sites
     int v_site = 0, v_p = 0;        // SGC: the site and its LDS index
-    uint32_t v_cell = 0;            // SGC, periodic: packed cell of the site
+    uint32_t v_img = 0;             // SGC, periodic: image word of the site
     if constexpr (kCanonical) {
       const uint32_t ia = bounded32(uint32_t(blk.w0 >> 32), np);
       uint32_t ib = ia + 1u + bounded32(uint32_t(blk.w0), np - 1u);
@@ -474,7 +487,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
       v_site = has_bkg ? m.movable[pick] : pick;
       const Ref r = ref_of<kPeriodic>(c, v_site);
       v_p = r.p;
-      v_cell = uint32_t(r.x) | (uint32_t(r.y) << 8) | (uint32_t(r.z) << 16);
+      v_img = image_word<kPeriodic>(c, r);
       v_new = bounded32(uint32_t(blk.w0), uint32_t(c.S - 1));
     }
     uint64_t acc_bits = 0;
@@ -572,16 +585,13 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         flush();
         if constexpr (kPeriodic) {
           // lanes 0-7 write the images of site i, lanes 8-15 those of site j
-          uint32_t cell;
-          if constexpr (kCanonical) {
-            const bool second = lane >= 8;
-            cell = second ? (uint32_t(rj.x) | (uint32_t(rj.y) << 8) | (uint32_t(rj.z) << 16))
-                          : (uint32_t(ri.x) | (uint32_t(ri.y) << 8) | (uint32_t(ri.z) << 16));
-          } else {
-            cell = __builtin_amdgcn_readlane(v_cell, l);
-          }
-          const int idx = image_index(c, int(cell & 255u), int((cell >> 8) & 255u), int(cell >> 16), lane & 7);
           const bool second = kCanonical && lane >= 8;
+          uint32_t img;
+          if constexpr (kCanonical)
+            img = second ? image_word<true>(c, rj) : image_word<true>(c, ri);
+          else
+            img = __builtin_amdgcn_readlane(v_img, l);
+          const int idx = image_of(c, img, lane & 7);
           if (idx >= 0 && lane < (kCanonical ? 16 : 8)) occ[idx] = uint8_t(second ? old_i : new_i);
         } else {
           if (lane == 0) occ[si] = uint8_t(new_i);
@@ -642,6 +652,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
       int v_ta = 0, v_tb = 0;         // canonical: tracker slots of the two sites
       int v_p = 0;                    // SGC: LDS index of the site
       uint32_t v_cell = 0;            // SGC: packed cell of the site
+      uint32_t v_img = 0;             // SGC: image word of the site
       if constexpr (kCanonical) {
         const uint32_t ia = bounded32(uint32_t(blk.w0 >> 32), np);
         uint32_t ib = ia + 1u + bounded32(uint32_t(blk.w0), np - 1u);
@@ -654,6 +665,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         const Ref r = ref_of<true>(c, int(bounded64(blk.w1, uint64_t(N))));
         v_p = r.p;
         v_cell = uint32_t(r.x) | (uint32_t(r.y) << 8) | (uint32_t(r.z) << 16);
+        v_img = image_word<true>(c, r);
         v_new = bounded32(uint32_t(blk.w0), uint32_t(c.S - 1));
       }
       uint64_t acc_bits = 0;
@@ -663,6 +675,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         // every lane looks up its own move in the current state
         int p_i = v_p, p_j = 0;
         uint32_t cell_i = v_cell, cell_j = 0, patch = 0, o_v = v_old, n_v = v_new;
+        uint32_t img_i = v_img, img_j = 0;
         uint32_t sites = 0, slots = 0;  // canonical: si | sj << 16, ta | tb << 16
         if constexpr (kCanonical) {
           const int si = trk[v_ta], sj = trk[v_tb];
@@ -671,6 +684,8 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
           p_j = rj.p;
           cell_i = uint32_t(ri.x) | (uint32_t(ri.y) << 8) | (uint32_t(ri.z) << 16);
           cell_j = uint32_t(rj.x) | (uint32_t(rj.y) << 8) | (uint32_t(rj.z) << 16);
+          img_i = image_word<true>(c, ri);
+          img_j = image_word<true>(c, rj);
           patch = wrapped_delta(c, ri, rj);
           sites = uint32_t(si) | (uint32_t(sj) << 16);
           slots = uint32_t(v_ta) | (uint32_t(v_tb) << 16);
@@ -687,8 +702,10 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
                                      uint32_t(__builtin_amdgcn_ds_bpermute(src_b, int(hi_of(log_u)))));
         const uint32_t cib = uint32_t(__builtin_amdgcn_ds_bpermute(src_b, int(cell_i | (o_v << 24) | (n_v << 28))));
         const uint32_t cia = uint32_t(__builtin_amdgcn_ds_bpermute(src_a, int(cell_i)));
-        uint32_t cjb = 0, cja = 0, slb = 0, sla = 0, stb = 0;
+        const uint32_t wib = uint32_t(__builtin_amdgcn_ds_bpermute(src_b, int(img_i)));
+        uint32_t cjb = 0, cja = 0, slb = 0, sla = 0, stb = 0, wjb = 0;
         if constexpr (kCanonical) {
+          wjb = uint32_t(__builtin_amdgcn_ds_bpermute(src_b, int(img_j)));
           cjb = uint32_t(__builtin_amdgcn_ds_bpermute(src_b, int(cell_j)));
           cja = uint32_t(__builtin_amdgcn_ds_bpermute(src_a, int(cell_j)));
           slb = uint32_t(__builtin_amdgcn_ds_bpermute(src_b, int(slots)));
@@ -804,11 +821,10 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         if (com != 0) {
           if (((com >> lane) & 1u) != 0) {
             // the lanes of a committed move write its images (site j's too)
-            const uint32_t ib = cib & 0xFFFFFFu;
-            const int idx = image_index(c, int(ib & 255u), int((ib >> 8) & 255u), int(ib >> 16), sub);
+            const int idx = image_of(c, wib, sub);
             if (idx >= 0) occ[idx] = uint8_t(n_b);
             if constexpr (kCanonical) {
-              const int jdx = image_index(c, int(cjb & 255u), int((cjb >> 8) & 255u), int(cjb >> 16), sub);
+              const int jdx = image_of(c, wjb, sub);
               if (jdx >= 0) occ[jdx] = uint8_t(o_b);
               if (sub == 0) trk[slb & 0xFFFFu] = uint16_t(stb >> 16);  // trk[ta] = sj
               if (sub == 1) trk[slb >> 16] = uint16_t(stb & 0xFFFFu);  // trk[tb] = si

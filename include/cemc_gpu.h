@@ -217,12 +217,11 @@ CEMC_API int cemc_batch_set_waves(cemc_batch* batch, int32_t waves);
  * with one translation group, no background sites and sub-cluster records
  * in registers evaluate 8 consecutive moves together against the same state
  * and commit them in order up to the first move that an accepted earlier
- * move of the round affects (that move is evaluated again); 1 (default for
- * batches) evaluates every move on its own; 0 (default for pools) takes 8
- * for chains that run on four or more waves and 1 otherwise.  All give the
- * same moves, energies and accept sequence; 8 pays where a chain's waves wait
- * on barriers and memory (many waves per chain), 1 where one wave per SIMD
- * is issue-bound.  No counterpart in the reference (its moves are
+ * move of the round affects (that move is evaluated again); 1 evaluates
+ * every move on its own; 0 (default) takes 8 wherever the chain's variant
+ * allows it.  All give the same moves, energies and accept sequence, bit for
+ * bit; rounds share one wave reduction, one barrier and one decision step
+ * among their moves.  No counterpart in the reference (its moves are
  * sequential). */
 CEMC_API int cemc_batch_set_moves_per_round(cemc_batch* batch, int32_t moves);
 /* Launch shape of one chain: waves per chain, sub-cluster records per lane

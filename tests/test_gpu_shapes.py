@@ -142,3 +142,43 @@ def test_sequential_and_speculative_rounds_agree(gpu, fname):
         es.append(batch.energies()[0])
     assert np.array_equal(logs[0], logs[1])
     assert es[0] == es[1]
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("waves", [1, 2, 8])
+def test_many_records_per_lane_match_oracle(gpu, kind, waves):
+    """Ternary FCC 8^3 with triplets to the 3rd and quadruplets to the 2nd
+    shell (494 sub-clusters per site: 8 records per lane on one wave, 4 on
+    two, 2 on eight): one move per round and eight give the oracle's accept
+    sequence, lattice and energy, and the same energy bitwise."""
+    from cemc_amd.batch import ChainBatch
+    from cemc_amd.lattice import FCCSettings, random_occupation, seeded_ecis
+    from cemc_amd.model import CEModel
+    from oracle.brute_force import brute_force_cf
+    from oracle.oracle import OracleModel, OracleUpdater
+    s = FCCSettings(8, ["A", "B", "C"], {2: 4, 3: 3, 4: 2})
+    eci = seeded_ecis(s, 2027, 0.01)
+    occ = random_occupation(s.n_sites, 3, 9)
+    T, n_moves, seed, uid = 700.0, 4000, 5, 3
+    names = s.cf_names()
+    ora = OracleUpdater(OracleModel(s, names), occ, brute_force_cf(s, occ, names), eci)
+    ref_bits, _, _ = ora.run(kind, seed, uid, n_moves, T)
+    ref = _bits(ref_bits, n_moves)
+    model = CEModel(s, eci)
+    energies = []
+    for per_round in (1, 8):
+        print("kind", kind, "waves", waves, "per_round", per_round, flush=True)
+        batch = ChainBatch([model], [dict(kind=kind, temperature=T, occupation=occ, seed=seed, uid=uid)])
+        batch.set_moves_per_round(per_round)
+        batch.set_waves(waves)
+        assert batch.launch_shape(0)[:2] == (waves, {1: 8, 2: 4, 8: 2}[waves])
+        log = batch.run_logged(n_moves)
+        got = _bits(log[0], n_moves)
+        mism = np.nonzero(got != ref)[0]
+        assert mism.size == 0, "first mismatching move %d" % mism[0]
+        assert np.array_equal(batch.occupation(0), ora.get_occupation())
+        e = batch.energies()[0]
+        assert abs(e - ora.get_energy()) / s.n_sites <= E_TOL
+        energies.append(e)
+    assert energies[0] == energies[1]
+    assert ref.sum() > 0
