@@ -27,11 +27,12 @@ sys.path.insert(0, ROOT)
 METRIC = "aggregate MC trial moves/sec at 1 MI355X (batched vs plain); p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
 # HBM bytes of one SGC launch of the default workload (1024 chains x 500
-# sweeps, 1 wave per chain, 1 move per round): FETCH_SIZE and WRITE_SIZE per
+# sweeps, 1 wave per chain, rounds of 8 moves): FETCH_SIZE and WRITE_SIZE per
 # dispatch, each scaled by what the counter reads for a copy kernel of known
 # size (x2.0 and x1.0).  Measured with rocprofv3 --pmc; summary in
 # profiles/sweep_kernel_sgc_binary_fcc10_1024chains.txt
-TRAFFIC_DEFAULT_LAUNCH = 3656666
+TRAFFIC_DEFAULT_LAUNCH = 3686144
+TRAFFIC_SHAPE = (1, 8)
 
 
 def log(*a):
@@ -278,7 +279,7 @@ def main():
     bytes_per_flip = 5.0 * model.gathers_per_flip()
     launch_bytes = bytes_per_flip * moves_per_chain * len(chains)
     achieved = launch_bytes / (avg_kernel_ms * 1e-3) / 1e9
-    default_shape = len(chains) == 1024 and args.sweeps == 500 and tuple(tune_w) == (1, 1)
+    default_shape = len(chains) == 1024 and args.sweeps == 500 and tuple(tune_w) == TRAFFIC_SHAPE
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_DEFAULT_LAUNCH if default_shape else None,
                 "lds_peak_measured": lds_meas, "lds_frac": achieved / lds_meas,

@@ -173,8 +173,10 @@ class ChainBatch:
 
     def set_moves_per_round(self, moves):
         """0 (default): eight consecutive moves evaluated together wherever
-        the chain's kernel variant allows it; 1: every move on its own; 8:
-        rounds of eight where allowed.  Same results bit for bit."""
+        the chain's kernel variant allows it and the moves of a round seldom
+        conflict (interacting sites, times 4 for swaps, at most 12% of the
+        lattice), else one at a time; 1: every move on its own; 8: rounds of
+        eight where allowed.  Same results bit for bit."""
         _capi.check(self._lib.cemc_batch_set_moves_per_round(self.handle, int(moves)))
 
     def launch_shape(self, chain):

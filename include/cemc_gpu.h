@@ -219,9 +219,11 @@ CEMC_API int cemc_batch_set_waves(cemc_batch* batch, int32_t waves);
  * and commit them in order up to the first move that an accepted earlier
  * move of the round affects (that move is evaluated again); 1 evaluates
  * every move on its own; 0 (default) takes 8 wherever the chain's variant
- * allows it.  All give the same moves, energies and accept sequence, bit for
- * bit; rounds share one wave reduction, one barrier and one decision step
- * among their moves.  No counterpart in the reference (its moves are
+ * allows it and the moves of a round seldom conflict (the sites a move
+ * interacts with, times 4 for swaps, are at most 12% of the lattice), else 1.
+ * All give the same moves, energies and accept sequence, bit for bit; rounds
+ * share one wave reduction, one barrier and one decision step among their
+ * moves.  No counterpart in the reference (its moves are
  * sequential). */
 CEMC_API int cemc_batch_set_moves_per_round(cemc_batch* batch, int32_t moves);
 /* Launch shape of one chain: waves per chain, sub-cluster records per lane
