@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # size (x2.0 and x1.0).  Measured with rocprofv3 --pmc; summary in
 # profiles/sweep_kernel_sgc_binary_fcc10_1024chains.txt
 TRAFFIC_DEFAULT_LAUNCH = 3686144
-TRAFFIC_SHAPE = (1, 8)
+TRAFFIC_SHAPE = (1, 8, 1)
 
 
 def log(*a):
@@ -104,7 +104,8 @@ def ternary_uneven(n_chains=256, sweeps=200, seed=3):
                         "triplets<=3rd, quadruplets<=2nd shell, %d sweeps each, one launch" % sweeps,
             "value": total / wall, "unit": "moves/s", "moves": total, "kernel_ms": kernel_ms, "wall_s": wall,
             "bytes_per_move": 10.0 * m0.gathers_per_flip(), "launch_shapes": [list(x) for x in shapes],
-            "autotune_moves_per_s": {"%dx%d" % k: v for k, v in tune_rates.items()},
+            "shape": autotune.shape_name(tune_w),
+            "autotune_moves_per_s": {autotune.shape_name(k): v for k, v in tune_rates.items()},
             "acceptance": float(batch.stats()["accepted"].sum() / max(total + int((moves // 20).sum()), 1))}
 
 
@@ -140,8 +141,8 @@ def tempering(n_rep=128, cycles=2000, L=32, seed=9):
                         % (model.n_cf, model.n_subclusters, n_rep, cycles),
             "value": total / wall, "unit": "moves/s", "moves": total, "kernel_ms": kernel_ms, "wall_s": wall,
             "bytes_per_move": 10.0 * model.gathers_per_flip(), "launch_shape": list(batch.launch_shape(0)),
-            "moves_per_round": tune_w[1],
-            "autotune_moves_per_s": {"%dx%d" % k: v for k, v in tune_rates.items()},
+            "shape": autotune.shape_name(tune_w),
+            "autotune_moves_per_s": {autotune.shape_name(k): v for k, v in tune_rates.items()},
             "exchange_accept_rate": float(acc)}
 
 
@@ -220,7 +221,8 @@ def main():
     ap.add_argument("--workloads", default="batched,ternary,tempering,serving",
                     help="comma list of batched, ternary, tempering, serving")
     ap.add_argument("--shape", default=None,
-                    help="WxK: waves per chain and moves per round for the batched workload (skips the autotuner)")
+                    help="'fields' (per-site fields, one wave per chain) or WxK (waves per chain, moves per round, "
+                         "tables summed per move) for the batched workload; skips the autotuner")
     ap.add_argument("--pt-cycles", type=int, default=200)
     ap.add_argument("--serving-jobs", type=int, default=500)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -245,9 +247,10 @@ def main():
     chains = batched_chains(settings, args.chains)
     from cemc_amd import autotune
     if args.shape:
-        tune_w, tune_rates = tuple(int(v) for v in args.shape.split("x")), {}
+        tune_w, tune_rates = autotune.parse_shape(args.shape), {}
     else:
-        tune_w, tune_rates = autotune.tune([model] * len(chains), chains, candidates=(1, 2, 4), sweeps=5.0, log=log)
+        tune_w, tune_rates = autotune.tune([model] * len(chains), chains, candidates=(1, 2, 4), sweeps=20.0,
+                                           repeats=1, log=log)
     batch = ChainBatch([model] * len(chains), chains)
     autotune.apply(batch, tune_w)
     moves_per_chain = args.sweeps * settings.n_sites
@@ -295,8 +298,10 @@ def main():
                    "chains": len(chains), "sweeps_per_step": args.sweeps, "sites": settings.n_sites},
         "plain_moves_per_s": plain_value, "batched_over_plain": value / plain_value,
         "roofline": roofline,
-        "launch_shape": {"waves_per_chain": tune_w[0], "moves_per_round": tune_w[1],
-                         "autotune_moves_per_s": {"%dx%d" % k: v for k, v in tune_rates.items()}},
+        "launch_shape": {"shape": autotune.shape_name(tune_w), "waves_per_chain": tune_w[0],
+                         "moves_per_round": tune_w[1] if tune_w[2] != autotune.PATH_FIELDS else None,
+                         "energy_path": "fields" if tune_w[2] == autotune.PATH_FIELDS else "evaluate",
+                         "autotune_moves_per_s": {autotune.shape_name(k): v for k, v in tune_rates.items()}},
     }
     work = set(w.strip() for w in args.workloads.split(","))
     if "ternary" in work:

@@ -19,6 +19,7 @@ class PoolFull(RuntimeError):
     """A pool has no free slot or device memory for one more chain."""
 CF_C0, CF_SINGLET, CF_CLUSTER = 0, 1, 2
 MOVE_CANONICAL, MOVE_SGC = 0, 1
+PATH_AUTO, PATH_EVALUATE, PATH_FIELDS = 0, 1, 2
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -91,6 +92,8 @@ SIGNATURES = [
     ("cemc_batch_get_cf", ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
     ("cemc_batch_set_waves", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("cemc_batch_set_moves_per_round", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("cemc_batch_set_energy_path", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("cemc_batch_get_energy_path", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     ("cemc_batch_get_launch_shape", ctypes.c_int, [_vp, ctypes.c_int32, _i32p, _i32p, _i32p]),
     ("cemc_batch_set_temperature", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double]),
     ("cemc_pool_create", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,

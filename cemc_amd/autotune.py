@@ -1,14 +1,17 @@
-"""Launch-shape autotuner: picks the number of wavefronts per chain (1, 2, 4
-or 8 cooperating waves) and the moves evaluated per round (1 or 8) for a
-batch by timing each candidate on a scratch copy of the batch, so the real
-chains' trajectories are not touched.
+"""Launch-shape autotuner: picks, for a batch, the energy path (per-site
+fields updated on accepted moves, one wave per chain; or tables summed per
+trial move) and, for the latter, the number of wavefronts per chain (1, 2, 4
+or 8 cooperating waves) and the moves evaluated per round (1 or 8), by timing
+each candidate on a scratch copy of the batch, so the real chains'
+trajectories are not touched.
 
 The kernel variant (records per lane, energy tables in LDS or HBM) follows
-from the waves and the model; the waves are the free parameter: more waves
-split each move's sub-clusters further (shorter dependent chain per move,
-one barrier per move more), fewer waves leave more CUs to other chains.
-Results are cached per workload signature in memory and, optionally, in a
-JSON file.
+from the shape and the model.  More waves split each move's sub-clusters
+further (shorter dependent chain per move, one barrier per move or round
+more), fewer waves leave more CUs to other chains; fields make rejected
+moves cheap and accepted ones dearer.  A shape is (waves, moves per round,
+path); its name is "fields" or "WxK".  Results are cached per workload
+signature in memory and, optionally, in a JSON file.
 """
 from __future__ import annotations
 
@@ -17,7 +20,7 @@ import os
 
 import numpy as np
 
-from .batch import ChainBatch
+from .batch import PATH_EVALUATE, PATH_FIELDS, ChainBatch
 
 _cache = {}
 
@@ -32,43 +35,59 @@ def signature(models, chains):
     return ";".join("%s:%d" % kv for kv in sorted(sig.items()))
 
 
-def tune(models, chains, candidates=(1, 2, 4, 8), per_round=(1, 8), sweeps=2.0, repeats=2, cache_file=None,
-         log=None):
-    """Returns ((waves, moves per round) of the best shape, {shape: moves/s})
-    for the batch (models, chains).
+def shape_name(shape):
+    return "fields" if shape[2] == PATH_FIELDS else "%dx%d" % (shape[0], shape[1])
+
+
+def parse_shape(name):
+    """"fields" or "WxK" -> (waves, moves per round, path)."""
+    if name == "fields":
+        return (1, 1, PATH_FIELDS)
+    w, k = (int(v) for v in name.split("x"))
+    return (w, k, PATH_EVALUATE)
+
+
+def tune(models, chains, candidates=(1, 2, 4, 8), per_round=(1, 8), fields=True, sweeps=2.0, repeats=2,
+         cache_file=None, log=None):
+    """Returns (best shape, {shape: moves/s}) for the batch (models, chains).
 
     Each candidate shape runs ``sweeps`` sweeps of every chain on a scratch
-    batch built from the same inputs, ``repeats`` times; the best rate counts."""
+    batch built from the same inputs, ``repeats`` times; the best rate counts.
+    The fields path is a candidate when some chain of the batch can take it."""
     sig = signature(models, chains)
     if cache_file and os.path.exists(cache_file):
         with open(cache_file) as f:
             _cache.update(json.load(f))
     if sig in _cache:
         entry = _cache[sig]
-        return tuple(entry["best"]), {tuple(int(v) for v in k.split("x")): r for k, r in entry["rates"].items()}
+        return tuple(entry["best"]), {parse_shape(k): r for k, r in entry["rates"].items()}
     scratch = ChainBatch(models, chains)
     moves = np.array([max(64, int(sweeps * m.n_sites)) for m in models], dtype=np.int64)
+    shapes = []
+    if fields:
+        scratch.set_energy_path(PATH_FIELDS)
+        if any(scratch.energy_path(c) == PATH_FIELDS for c in range(scratch.n_chains)):
+            shapes.append((1, 1, PATH_FIELDS))
+    shapes += [(w, k, PATH_EVALUATE) for k in per_round for w in candidates]
     rates = {}
     seen = set()
-    for k in per_round:
-        scratch.set_moves_per_round(k)
-        for w in candidates:
-            scratch.set_waves(w)
-            shape = (k, tuple(sorted({scratch.launch_shape(c) for c in range(scratch.n_chains)})))
-            if shape in seen:  # the variant cannot take this shape: same launch as before
-                continue
-            seen.add(shape)
-            scratch.run(moves // 4)  # first launch of a variant loads its code
-            best = 0.0
-            for _ in range(repeats):
-                ms = scratch.run_timed(moves)
-                best = max(best, float(moves.sum()) / (ms * 1e-3))
-            rates[(w, k)] = best
-            if log:
-                log("autotune %s: waves %d, moves per round %d -> %.3e moves/s" % (sig, w, k, best))
+    for shape in shapes:
+        apply(scratch, shape)
+        launch = (shape[1], shape[2], tuple(sorted({scratch.launch_shape(c) for c in range(scratch.n_chains)})))
+        if launch in seen:  # the variant cannot take this shape: same launch as before
+            continue
+        seen.add(launch)
+        scratch.run(moves // 4)  # first launch of a variant loads its code
+        best = 0.0
+        for _ in range(repeats):
+            ms = scratch.run_timed(moves)
+            best = max(best, float(moves.sum()) / (ms * 1e-3))
+        rates[shape] = best
+        if log:
+            log("autotune %s: %s -> %.3e moves/s" % (sig, shape_name(shape), best))
     scratch.close()
     best = max(rates, key=rates.get)
-    _cache[sig] = {"best": list(best), "rates": {"%dx%d" % kk: v for kk, v in rates.items()}}
+    _cache[sig] = {"best": list(best), "rates": {shape_name(k): v for k, v in rates.items()}}
     if cache_file:
         with open(cache_file, "w") as f:
             json.dump(_cache, f, indent=1, sort_keys=True)
@@ -76,7 +95,8 @@ def tune(models, chains, candidates=(1, 2, 4, 8), per_round=(1, 8), sweeps=2.0, 
 
 
 def apply(batch, shape):
-    """Set a tuned (waves, moves per round) on a batch."""
+    """Set a tuned (waves, moves per round, path) on a batch."""
+    batch.set_energy_path(shape[2])
     batch.set_moves_per_round(shape[1])
     batch.set_waves(shape[0])
 

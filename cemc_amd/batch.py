@@ -19,6 +19,7 @@ from . import _capi
 
 CANONICAL = _capi.MOVE_CANONICAL
 SGC = _capi.MOVE_SGC
+PATH_AUTO, PATH_EVALUATE, PATH_FIELDS = _capi.PATH_AUTO, _capi.PATH_EVALUATE, _capi.PATH_FIELDS
 
 
 def chem_pot_vector(model, chem_pot):
@@ -178,6 +179,19 @@ class ChainBatch:
         lattice), else one at a time; 1: every move on its own; 8: rounds of
         eight where allowed.  Same results bit for bit."""
         _capi.check(self._lib.cemc_batch_set_moves_per_round(self.handle, int(moves)))
+
+    def set_energy_path(self, path):
+        """PATH_AUTO (default) and PATH_EVALUATE: every trial move sums its
+        sub-clusters' table entries; PATH_FIELDS: per-site fields kept in LDS
+        and updated on accepted moves, where the chain allows it (fast for
+        single chains at low acceptance).  Same accept sequences."""
+        _capi.check(self._lib.cemc_batch_set_energy_path(self.handle, int(path)))
+
+    def energy_path(self, chain):
+        """PATH_EVALUATE or PATH_FIELDS: the path chain `chain` runs on."""
+        v = ctypes.c_int32()
+        _capi.check(self._lib.cemc_batch_get_energy_path(self.handle, int(chain), ctypes.byref(v)))
+        return v.value
 
     def launch_shape(self, chain):
         """(waves per chain, records per lane in registers, tables in LDS)."""

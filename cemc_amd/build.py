@@ -37,7 +37,8 @@ def _flags():
 
 def _units():
     """(object name, source, extra defines) of every translation unit of the engine."""
-    units = [("ce_kernels", "ce_kernels.hip", []), ("ce_host", "ce_host.hip", [])]
+    units = [("ce_kernels", "ce_kernels.hip", []), ("ce_host", "ce_host.hip", []),
+             ("ce_field", "ce_field_inst.hip", [])]
     for w in SWEEP_WAVES:
         for p in (0, 1):
             units.append(("ce_sweep_w%d_p%d" % (w, p), "ce_sweep_inst.hip",

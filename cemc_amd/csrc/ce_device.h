@@ -104,6 +104,19 @@ struct ModelDev {
   const uint32_t* eq_decos;   // packed 4 x 8-bit decoration per equivalent decoration
   const ClusterEntry* entries;
   const double* eci;          // n_cf
+  // Local-field path (closed-form periodic, one translation group, no
+  // background; every sub-cluster seen from each of its members is a record
+  // of that member).  gtab_fx = round(gtab * 2^k) as integers, fx_scale =
+  // 2^-k with k chosen so that a site's field sum stays below 2^61.  duals[3r
+  // + q]: for member q of record r, the record of that member holding the
+  // same sub-cluster: bits 0-23 its table base, 24-25 the code position of
+  // the reference site of r, 26-27 and 28-29 those of the other members of r
+  // in order (q skipped).
+  int32_t field_ok;
+  int32_t pad_field;
+  double fx_scale;
+  const int64_t* gtab_fx;
+  const uint32_t* duals;
 };
 
 // Per-chain parameters (host written).

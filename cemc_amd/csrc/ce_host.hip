@@ -104,6 +104,10 @@ struct cemc_model {
   int pad_h = 0;
   bool spec_ok = false;   // speculative rounds allowed (see ModelDev::imask)
   uint64_t imask[2] = {0, 0};
+  bool field_ok = false;  // local-field path allowed (see ModelDev::duals)
+  double fx_scale = 0.0;
+  std::vector<int64_t> gtab_fx;
+  std::vector<uint32_t> duals;
   int pdims[3] = {0, 0, 0};
   std::vector<int> signed_cells;
   std::vector<double> basis, eci, gtab;
@@ -121,6 +125,8 @@ struct cemc_model {
   SubRec* d_recs = nullptr;
   FatRec* d_frecs = nullptr;
   double* d_gtab = nullptr;
+  int64_t* d_gtab_fx = nullptr;
+  uint32_t* d_duals = nullptr;
   int32_t* d_trans = nullptr;
   uint32_t* d_cells = nullptr;
   int8_t* d_site_group = nullptr;
@@ -140,6 +146,8 @@ struct cemc_model {
   }
 
   void build_gtab(const std::vector<double>& J);
+  void build_gtab_fx();
+  void build_duals();
   int code_bits() const;    // bits per species in an energy-table code
   int deco_slices() const;  // decoration slices per table block (>= n_species - 1)
   void refresh_view();
@@ -150,6 +158,8 @@ cemc_model::~cemc_model() {
   dev_free(d_recs);
   dev_free(d_frecs);
   dev_free(d_gtab);
+  dev_free(d_gtab_fx);
+  dev_free(d_duals);
   dev_free(d_trans);
   dev_free(d_cells);
   dev_free(d_site_group);
@@ -237,6 +247,116 @@ void cemc_model::build_gtab(const std::vector<double>& J) {
   }
 }
 
+// Integer copy of the energy tables for the local-field path: a site's field
+// F^d = sum over its records of G[d][code] is kept exactly as an integer sum
+// of round(G * 2^k), k chosen so that any field stays below 2^61 in
+// magnitude (the sum over records of each record's largest |G|).  Updates
+// then add and subtract integers only: no rounding drift and no dependence
+// on the order of concurrent updates.
+void cemc_model::build_gtab_fx() {
+  if (!field_ok) {
+    gtab_fx.assign(1, 0);
+    fx_scale = 0.0;
+    return;
+  }
+  const int block = deco_slices() << (3 * code_bits());
+  double bound = 0.0;
+  for (int r = 0; r < grp_rec_cnt[0]; ++r) {
+    const uint32_t gbase = frecs[r].meta & 0xFFFFFFu;
+    double mx = 0.0;
+    for (int i = 0; i < block; ++i) mx = std::max(mx, std::fabs(gtab[gbase + i]));
+    bound += mx;
+  }
+  int e = 0;
+  std::frexp(bound, &e);  // bound < 2^e
+  const int k = 61 - e;
+  fx_scale = std::ldexp(1.0, -k);
+  gtab_fx.resize(gtab.size());
+  for (size_t i = 0; i < gtab.size(); ++i) gtab_fx[i] = std::llround(std::ldexp(gtab[i], k));
+}
+
+// Duals of the records (see ModelDev::duals): the sub-cluster of record r
+// seen from its member q (cell offset o_q) has the other members -o_q and
+// o_p - o_q (p != q); the record of the same entry with exactly these
+// wrapped offsets is its dual.  The path is allowed only when every dual
+// exists and is unique.
+void cemc_model::build_duals() {
+  field_ok = false;
+  duals.clear();
+  if (!periodic || n_groups != 1 || int(movable.size()) != n_sites || code_bits() > 2 || n_sites > 65535) return;
+  const int cnt = grp_rec_cnt[0];
+  const int L[3] = {dims[0], dims[1], dims[2]};
+  auto cell = [&](uint32_t packed, int a) { return int((packed >> (8 * a)) & 255u); };
+  auto pack = [&](const int (&v)[3]) {
+    uint32_t w = 0;
+    for (int a = 0; a < 3; ++a) w |= uint32_t(((v[a] % L[a]) + L[a]) % L[a]) << (8 * a);
+    return w;
+  };
+  // record r: entry, member count, wrapped member offsets in code order
+  struct RecGeo {
+    int entry, nm;
+    uint32_t off[3];
+  };
+  std::vector<RecGeo> geo(cnt);
+  for (int r = 0; r < cnt; ++r) {
+    const SubRec& sr = recs[r];
+    geo[r].entry = int(sr.w1 >> 26);
+    geo[r].nm = int((rec_slots[r].meta >> 24) & 3u);
+    const int32_t sl[3] = {rec_slots[r].m0, rec_slots[r].m1, rec_slots[r].m2};
+    for (int q = 0; q < 3; ++q) geo[r].off[q] = q < geo[r].nm ? col_cells[sl[q]] : 0u;
+  }
+  std::map<std::vector<uint32_t>, std::vector<int>> by_key;
+  for (int r = 0; r < cnt; ++r) {
+    std::vector<uint32_t> key(geo[r].off, geo[r].off + geo[r].nm);
+    std::sort(key.begin(), key.end());
+    key.insert(key.begin(), uint32_t(geo[r].entry));
+    by_key[key].push_back(r);
+  }
+  std::vector<uint32_t> out(3 * size_t(cnt), 0u);
+  for (int r = 0; r < cnt; ++r) {
+    const int nm = geo[r].nm;
+    for (int q = 0; q < nm; ++q) {
+      // offsets seen from member q: the reference (-o_q), then the others
+      uint32_t want[3];
+      int n_want = 0;
+      int oq[3], v[3];
+      for (int a = 0; a < 3; ++a) {
+        oq[a] = cell(geo[r].off[q], a);
+        v[a] = -oq[a];
+      }
+      want[n_want++] = pack(v);
+      for (int p = 0; p < nm; ++p) {
+        if (p == q) continue;
+        for (int a = 0; a < 3; ++a) v[a] = cell(geo[r].off[p], a) - oq[a];
+        want[n_want++] = pack(v);
+      }
+      std::vector<uint32_t> key(want, want + n_want);
+      std::sort(key.begin(), key.end());
+      key.insert(key.begin(), uint32_t(geo[r].entry));
+      auto it = by_key.find(key);
+      if (it == by_key.end() || it->second.size() != 1) return;
+      const int rd = it->second[0];
+      auto pos_of = [&](uint32_t w) {
+        for (int t = 0; t < geo[rd].nm; ++t)
+          if (geo[rd].off[t] == w) return t;
+        return -1;
+      };
+      uint32_t word = rec_slots[rd].meta & 0xFFFFFFu;
+      const int pi = pos_of(want[0]);
+      if (pi < 0) return;
+      word |= uint32_t(pi) << 24;
+      for (int t = 1; t < n_want; ++t) {
+        const int pt = pos_of(want[t]);
+        if (pt < 0) return;
+        word |= uint32_t(pt) << (26 + 2 * (t - 1));
+      }
+      out[3 * size_t(r) + q] = word;
+    }
+  }
+  duals.swap(out);
+  field_ok = true;
+}
+
 void cemc_model::refresh_view() {
   ModelDev& v = view;
   std::memset(&v, 0, sizeof v);
@@ -280,6 +400,10 @@ void cemc_model::refresh_view() {
   v.eq_decos = d_eq_decos;
   v.entries = d_entries;
   v.eci = d_eci;
+  v.field_ok = field_ok ? 1 : 0;
+  v.fx_scale = fx_scale;
+  v.gtab_fx = d_gtab_fx;
+  v.duals = d_duals;
   if (!d_view) HIP_CHECK(hipMalloc(&d_view, sizeof(ModelDev)));
   HIP_CHECK(hipMemcpy(d_view, &v, sizeof(ModelDev), hipMemcpyHostToDevice));
 }
@@ -510,6 +634,10 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
   m->d_recs = upload(m->recs);
   m->d_frecs = upload(m->frecs);
   m->d_gtab = upload(m->gtab);
+  m->build_duals();
+  m->build_gtab_fx();
+  m->d_gtab_fx = upload(m->gtab_fx);
+  if (!m->duals.empty()) m->d_duals = upload(m->duals);
   m->d_trans = upload(m->trans);
   m->d_cells = upload(m->col_cells);
   m->d_site_group = upload(sg);
@@ -712,10 +840,12 @@ struct cemc_batch {
     int kbm;
     int waves;
     int spec;  // moves per round
+    bool field;  // local-field kernel
     int begin, count;
     uint32_t lds;
   };
   int spec_moves = 0;  // moves per round where the model allows it (0: kSpecMoves, 1 or kSpecMoves)
+  int energy_path = CEMC_PATH_AUTO;
   std::vector<Class> classes;
   int64_t classes_version = 0;  // bumped whenever the launch classes change
   int forced_waves = 0;
@@ -888,6 +1018,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
     int kbm;
     int waves;
     int spec;
+    bool field;
     uint32_t lds;
   };
   // waves per chain: enough chains x waves to give every SIMD of the
@@ -962,6 +1093,26 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
       k.lds = k.glds ? rec_g : rec_only;
       require(k.lds <= limit, "chain state does not fit in the workgroup's LDS");
     }
+    // local fields (one wave per chain, records in VGPRs) when asked for,
+    // where the model allows them, the chain's fields fit in LDS and no other
+    // wave count is forced.  Not automatic: a chain's cost on this path
+    // grows with its acceptance rate, so in a batch the hottest chain sets
+    // the launch time (config-style batches of 1024 chains over 300-1500 K
+    // ran 3x slower than evaluation rounds); it pays for single chains at
+    // low acceptance.
+    k.field = false;
+    if (b->energy_path == CEMC_PATH_FIELDS && m->field_ok && forced_waves <= 1) {
+      const int kr1 = pick_kr(1);
+      const uint32_t f_lds = field_lds_bytes(occ_bytes, trk_sites, m->n_sites * m->n_bfs, int(m->gtab_fx.size()));
+      if (kr1 > 0 && k.kbm <= 3 && f_lds <= limit) {
+        k.field = true;
+        k.waves = 1;
+        k.kr = kr1;
+        k.spec = 1;
+        k.glds = true;
+        k.lds = f_lds;
+      }
+    }
     keys[c] = k;
   }
   for (int c0 = 0; c0 < n_chains; ++c0) {
@@ -972,7 +1123,8 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
     for (int c = c0; c < n_chains; ++c) {
       const Key& k = keys[c];
       if (done[c] || k.periodic != k0.periodic || k.canonical != k0.canonical || k.kr != k0.kr ||
-          k.glds != k0.glds || k.kbm != k0.kbm || k.waves != k0.waves || k.spec != k0.spec)
+          k.glds != k0.glds || k.kbm != k0.kbm || k.waves != k0.waves || k.spec != k0.spec ||
+          k.field != k0.field)
         continue;
       done[c] = true;
       ids.push_back(c);
@@ -984,7 +1136,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
       return int64_t(mx->n_sites) * mx->recs.size() > int64_t(my->n_sites) * my->recs.size();
     });
     b->classes.push_back(cemc_batch::Class{k0.periodic, k0.canonical, k0.kr, k0.glds, k0.kbm, k0.waves, k0.spec,
-                                           int(order.size()), int(ids.size()), lds});
+                                           k0.field, int(order.size()), int(ids.size()), lds});
     order.insert(order.end(), ids.begin(), ids.end());
   }
   if (order.size() > b->order_cap || b->d_order == nullptr) {
@@ -1040,6 +1192,7 @@ void enqueue_sweeps(cemc_batch* b, uint32_t* d_log, int64_t log_words) {
     a.kbm = cl.kbm;
     a.waves = cl.waves;
     a.spec = cl.spec;
+    a.field = cl.field;
     a.accept_log = d_log;
     a.log_words = log_words;
     hipStream_t st = fan_out ? b->side[ci % cemc_batch::kSideStreams] : b->stream;
@@ -1125,6 +1278,10 @@ int cemc_model_set_ecis(cemc_model* m, const double* eci) {
     m->build_gtab(m->eci);
     HIP_CHECK(hipMemcpy(m->d_gtab, m->gtab.data(), sizeof(double) * std::max<size_t>(m->gtab.size(), 1),
                         hipMemcpyHostToDevice));
+    m->build_gtab_fx();
+    HIP_CHECK(hipMemcpy(m->d_gtab_fx, m->gtab_fx.data(), sizeof(int64_t) * m->gtab_fx.size(),
+                        hipMemcpyHostToDevice));
+    m->refresh_view();
     HIP_CHECK(hipMemcpy(m->d_eci, m->eci.data(), sizeof(double) * m->eci.size(), hipMemcpyHostToDevice));
   });
 }
@@ -1571,6 +1728,33 @@ int cemc_batch_set_moves_per_round(cemc_batch* b, int32_t moves) {
     HIP_CHECK(hipStreamSynchronize(b->stream));
     b->spec_moves = moves;
     if (!b->dynamic) derive_classes(b, b->forced_waves);
+  });
+}
+
+int cemc_batch_set_energy_path(cemc_batch* b, int32_t path) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(path == CEMC_PATH_AUTO || path == CEMC_PATH_EVALUATE || path == CEMC_PATH_FIELDS,
+            "energy path must be CEMC_PATH_AUTO, CEMC_PATH_EVALUATE or CEMC_PATH_FIELDS");
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    b->energy_path = path;
+    if (!b->dynamic) derive_classes(b, b->forced_waves);
+  });
+}
+
+int cemc_batch_get_energy_path(cemc_batch* b, int32_t chain, int32_t* path) {
+  return guarded([&] {
+    require(b != nullptr && path != nullptr, "NULL argument");
+    require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    std::vector<int32_t> order(b->n_chains);
+    HIP_CHECK(hipMemcpy(order.data(), b->d_order, sizeof(int32_t) * order.size(), hipMemcpyDeviceToHost));
+    for (const auto& cl : b->classes)
+      for (int i = cl.begin; i < cl.begin + cl.count; ++i)
+        if (order[i] == chain) {
+          *path = cl.field ? CEMC_PATH_FIELDS : CEMC_PATH_EVALUATE;
+          return;
+        }
+    throw std::runtime_error("chain not found in any launch class");
   });
 }
 

@@ -26,6 +26,7 @@ struct LaunchSweep {
   int kbm;         // decorations per table read: 3 (up to 4 species) or 7
   int waves;       // wavefronts per chain: 1, 2, 4 or 8 (more than 1: kr > 0)
   int spec;        // moves per round: 1, or kSpecMoves (periodic, kr > 0)
+  bool field;      // local-field kernel (periodic, one wave, kr > 0, kbm <= 3)
   uint32_t* accept_log;  // optional: bit t of chain c at accept_log[c * log_words + t / 32]
   int64_t log_words;
 };
@@ -34,6 +35,9 @@ struct LaunchSweep {
 // lattices), tracker entries (canonical chains), records held in LDS, table doubles.
 uint32_t sweep_lds_bytes(int occ_bytes, int trk_sites, int n_rec_lds, int gtab_doubles);
 bool sweep_variant_ok(int kr, bool gtab_lds, int kbm, int waves, int spec, bool periodic);
+// Dynamic LDS of one chain on the local-field kernel: padded occupation and
+// its site map, tracker entries, n_sites x B integer fields, integer tables.
+uint32_t field_lds_bytes(int occ_bytes, int trk_sites, int n_fields, int gtab_ints);
 // Raises the dynamic-LDS limit of every sweep kernel instantiation to
 // `bytes` minus that kernel's static LDS.
 hipError_t sweep_set_max_lds(uint32_t bytes);

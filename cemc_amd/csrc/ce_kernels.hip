@@ -397,6 +397,8 @@ CEMC_DECLARE_SWEEP_FN(8, 0)
 CEMC_DECLARE_SWEEP_FN(8, 1)
 #undef CEMC_DECLARE_SWEEP_FN
 
+const void* field_fn(bool canonical, int kr, int kbm);
+
 static const void* sweep_fn(bool periodic, bool canonical, int kr, bool glds, int kbm, int waves, int spec) {
   switch (waves) {
     case 1: return periodic ? sweep_fn_w1_p1(canonical, kr, glds, kbm, spec) : sweep_fn_w1_p0(canonical, kr, glds, kbm, spec);
@@ -440,12 +442,31 @@ hipError_t sweep_set_max_lds(uint32_t bytes) {
               e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
               if (e != hipSuccess) return e;
             }
+  for (int cn = 0; cn < 2; ++cn)
+    for (int kr : {2, 4, 8})
+      for (int kbm : {1, 3}) {
+        const void* fn = field_fn(cn != 0, kr, kbm);
+        if (fn == nullptr) return hipErrorInvalidValue;
+        hipFuncAttributes attr;
+        hipError_t e = hipFuncGetAttributes(&attr, fn);
+        if (e != hipSuccess) return e;
+        const int dyn = int(bytes) - int(attr.sharedSizeBytes);
+        if (dyn <= 0) return hipErrorInvalidValue;
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+        if (e != hipSuccess) return e;
+      }
   return hipSuccess;
 }
 
 hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream) {
-  if (!sweep_variant_ok(a.kr, a.gtab_lds, a.kbm, a.waves, a.spec, a.periodic)) return hipErrorInvalidValue;
-  const void* fn = sweep_fn(a.periodic, a.canonical, a.kr, a.gtab_lds, a.kbm, a.waves, a.spec);
+  const void* fn = nullptr;
+  if (a.field) {
+    if (!a.periodic || a.waves != 1) return hipErrorInvalidValue;
+    fn = field_fn(a.canonical, a.kr, a.kbm);
+  } else {
+    if (!sweep_variant_ok(a.kr, a.gtab_lds, a.kbm, a.waves, a.spec, a.periodic)) return hipErrorInvalidValue;
+    fn = sweep_fn(a.periodic, a.canonical, a.kr, a.gtab_lds, a.kbm, a.waves, a.spec);
+  }
   if (fn == nullptr) return hipErrorInvalidValue;
   const ChainParams* params = a.params;
   ChainState* states = a.states;

@@ -33,6 +33,8 @@ enum {
 
 enum { CEMC_CF_C0 = 0, CEMC_CF_SINGLET = 1, CEMC_CF_CLUSTER = 2 };
 enum { CEMC_MOVE_CANONICAL = 0, CEMC_MOVE_SGC = 1 };
+/* How a trial move's energy change is obtained (cemc_batch_set_energy_path). */
+enum { CEMC_PATH_AUTO = 0, CEMC_PATH_EVALUATE = 1, CEMC_PATH_FIELDS = 2 };
 
 typedef struct cemc_model cemc_model;
 typedef struct cemc_updater cemc_updater;
@@ -226,6 +228,23 @@ CEMC_API int cemc_batch_set_waves(cemc_batch* batch, int32_t waves);
  * moves.  No counterpart in the reference (its moves are
  * sequential). */
 CEMC_API int cemc_batch_set_moves_per_round(cemc_batch* batch, int32_t moves);
+/* Energy path: CEMC_PATH_EVALUATE sums the energy tables over the flipped
+ * site's sub-clusters for every trial move (the reference's update_cf per
+ * move, ce_updater.cpp:313-406, in energy form); CEMC_PATH_FIELDS keeps every
+ * site's field F_j^d = sum of its sub-clusters' table entries in LDS as an
+ * exact integer sum, so a trial move reads one field and only accepted moves
+ * update the fields of their sub-cluster partners, where the chain allows
+ * it (closed-form periodic lattice, one translation group, no background, up
+ * to four species, at most 512 sub-clusters per site, state within the
+ * workgroup's LDS, one wave per chain); CEMC_PATH_AUTO (default) evaluates.
+ * A chain's cost on the field path grows with its acceptance rate: it pays
+ * for single chains at low acceptance, while in large batches the chain with
+ * the highest acceptance sets the launch time.  Accept sequences and
+ * lattices are the same on both paths; energies agree to rounding.  No
+ * counterpart in the reference. */
+CEMC_API int cemc_batch_set_energy_path(cemc_batch* batch, int32_t path);
+/* Energy path a chain runs on (CEMC_PATH_EVALUATE or CEMC_PATH_FIELDS). */
+CEMC_API int cemc_batch_get_energy_path(cemc_batch* batch, int32_t chain, int32_t* path);
 /* Launch shape of one chain: waves per chain, sub-cluster records per lane
  * held in registers (0: records in LDS), energy tables in LDS (1) or HBM (0). */
 CEMC_API int cemc_batch_get_launch_shape(cemc_batch* batch, int32_t chain, int32_t* waves,

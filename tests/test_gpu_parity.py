@@ -19,20 +19,29 @@ def _bits(packed, n):
     return np.unpackbits(np.asarray(packed, dtype=np.uint8), bitorder="little")[:n]
 
 
-def _batch_for(fx, settings=None):
+def _batch_for(fx, settings=None, path=0):
     from cemc_amd.batch import ChainBatch
     from cemc_amd.model import CEModel
     model = CEModel(settings if settings is not None else fx.settings, fx.eci)
     chain = dict(kind=fx.kind, temperature=fx.temperature, occupation=fx.occ0, seed=fx.seed,
                  uid=fx.chain_uid, chem_pot=fx.meta["mu"] or None)
-    return model, ChainBatch([model], [chain])
+    batch = ChainBatch([model], [chain])
+    if path:
+        batch.set_energy_path(path)
+    return model, batch
 
 
+@pytest.mark.parametrize("path", ["evaluate", "fields"])
 @pytest.mark.parametrize("fname", CHAIN_FIXTURES)
-def test_batch_reproduces_reference_chain(gpu, fname):
+def test_batch_reproduces_reference_chain(gpu, fname, path):
+    """Both energy paths: tables summed per trial move, and per-site fields
+    updated on accepted moves (periodic fixtures; the others evaluate)."""
+    from cemc_amd.batch import PATH_EVALUATE, PATH_FIELDS
     fx = ChainFixture(fname)
-    print("fixture", fname, "moves", fx.n_moves, flush=True)
-    model, batch = _batch_for(fx)
+    print("fixture", fname, "moves", fx.n_moves, "path", path, flush=True)
+    model, batch = _batch_for(fx, path=PATH_FIELDS if path == "fields" else PATH_EVALUATE)
+    if path == "fields" and model.periodic and model.n_subclusters <= 512:
+        assert batch.energy_path(0) == PATH_FIELDS
     n = fx.n_sites
     bits = []
     for seg in range(fx.n_moves // fx.cf_every):

@@ -129,13 +129,14 @@ def test_resumed_chain_continues_its_stream(gpu):
 def test_sequential_and_speculative_rounds_agree(gpu, fname):
     """Binary chains with one move per round and with eight: identical accept
     sequences (both equal to the reference) and energies."""
-    from cemc_amd.batch import ChainBatch
+    from cemc_amd.batch import PATH_EVALUATE, ChainBatch
     from cemc_amd.model import CEModel
     fx = ChainFixture(fname)
     model = CEModel(fx.settings, fx.eci)
     logs, es = [], []
     for per_round in (1, 8):
         batch = ChainBatch([model], [_chain(fx)])
+        batch.set_energy_path(PATH_EVALUATE)
         batch.set_moves_per_round(per_round)
         logs.append(batch.run_logged(fx.n_moves))
         _check(fx, batch, 0, logs[-1])
@@ -151,7 +152,7 @@ def test_many_records_per_lane_match_oracle(gpu, kind, waves):
     shell (494 sub-clusters per site: 8 records per lane on one wave, 4 on
     two, 2 on eight): one move per round and eight give the oracle's accept
     sequence, lattice and energy, and the same energy bitwise."""
-    from cemc_amd.batch import ChainBatch
+    from cemc_amd.batch import PATH_EVALUATE, ChainBatch
     from cemc_amd.lattice import FCCSettings, random_occupation, seeded_ecis
     from cemc_amd.model import CEModel
     from oracle.brute_force import brute_force_cf
@@ -169,6 +170,7 @@ def test_many_records_per_lane_match_oracle(gpu, kind, waves):
     for per_round in (1, 8):
         print("kind", kind, "waves", waves, "per_round", per_round, flush=True)
         batch = ChainBatch([model], [dict(kind=kind, temperature=T, occupation=occ, seed=seed, uid=uid)])
+        batch.set_energy_path(PATH_EVALUATE)
         batch.set_moves_per_round(per_round)
         batch.set_waves(waves)
         assert batch.launch_shape(0)[:2] == (waves, {1: 8, 2: 4, 8: 2}[waves])
@@ -182,3 +184,69 @@ def test_many_records_per_lane_match_oracle(gpu, kind, waves):
         energies.append(e)
     assert energies[0] == energies[1]
     assert ref.sum() > 0
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("species,shells,size,T", [
+    (["A", "B"], {2: 4, 3: 2}, 10, 600.0),
+    (["A", "B", "C"], {2: 4, 3: 3, 4: 2}, 8, 700.0),
+    (["A", "B", "C", "D"], {2: 3, 3: 2}, 7, 900.0),
+])
+def test_fields_match_oracle_and_evaluation(gpu, kind, species, shells, size, T):
+    """Per-site fields updated on accepted moves: the oracle's accept
+    sequence and lattice, energies within rounding of the oracle's and of the
+    evaluating path; swaps of interacting sites included (small lattices)."""
+    from cemc_amd.batch import PATH_EVALUATE, PATH_FIELDS, ChainBatch
+    from cemc_amd.lattice import FCCSettings, random_occupation, seeded_ecis
+    from cemc_amd.model import CEModel
+    from oracle.brute_force import brute_force_cf
+    from oracle.oracle import OracleModel, OracleUpdater
+    s = FCCSettings(size, species, shells)
+    eci = seeded_ecis(s, 77, 0.02)
+    occ = random_occupation(s.n_sites, len(species), 12)
+    n_moves, seed, uid = 6000, 9, 4
+    names = s.cf_names()
+    ora = OracleUpdater(OracleModel(s, names), occ, brute_force_cf(s, occ, names), eci)
+    ref_bits, _, _ = ora.run(kind, seed, uid, n_moves, T)
+    ref = _bits(ref_bits, n_moves)
+    model = CEModel(s, eci)
+    energies = {}
+    for path in (PATH_FIELDS, PATH_EVALUATE):
+        print("species", len(species), "kind", kind, "path", path, flush=True)
+        batch = ChainBatch([model], [dict(kind=kind, temperature=T, occupation=occ, seed=seed, uid=uid)])
+        batch.set_energy_path(path)
+        assert batch.energy_path(0) == path
+        log = batch.run_logged(n_moves // 2)
+        log2 = batch.run_logged(n_moves - n_moves // 2)  # fields rebuilt at the next launch
+        got = np.concatenate([_bits(log[0], n_moves // 2), _bits(log2[0], n_moves - n_moves // 2)])
+        mism = np.nonzero(got != ref)[0]
+        assert mism.size == 0, "first mismatching move %d" % mism[0]
+        assert np.array_equal(batch.occupation(0), ora.get_occupation())
+        energies[path] = batch.energies()[0]
+        assert abs(energies[path] - ora.get_energy()) / s.n_sites <= E_TOL
+        st = batch.stats()
+        assert st["accepted"][0] == int(ref.sum())
+    assert abs(energies[PATH_FIELDS] - energies[PATH_EVALUATE]) / s.n_sites <= 1e-12
+    assert ref.sum() > 0
+
+
+def test_energy_path_choice(gpu):
+    """Evaluation by default; fields when asked for on closed-form periodic
+    lattices, not with more waves per chain forced nor on translation-table
+    lattices."""
+    from cemc_amd.batch import PATH_EVALUATE, PATH_FIELDS, ChainBatch
+    from cemc_amd.model import CEModel
+    fx = ChainFixture("binary_sgc_chain0.npz")
+    model = CEModel(fx.settings, fx.eci)
+    batch = ChainBatch([model], [_chain(fx)])
+    assert batch.energy_path(0) == PATH_EVALUATE
+    batch.set_energy_path(PATH_FIELDS)
+    assert batch.energy_path(0) == PATH_FIELDS
+    batch.set_waves(2)
+    assert batch.energy_path(0) == PATH_EVALUATE
+    batch.set_waves(0)
+    assert batch.energy_path(0) == PATH_FIELDS
+    table = CEModel(_table_settings(fx.settings), fx.eci)
+    tb = ChainBatch([table], [_chain(fx)])
+    tb.set_energy_path(PATH_FIELDS)
+    assert tb.energy_path(0) == PATH_EVALUATE
