@@ -418,25 +418,24 @@ __global__ __launch_bounds__(64) void field_kernel(const ChainParams* __restrict
         if (v == 2) v = int(unit_double(readlane_u64(blk.w3, f)) <= exp(-dEf / kT));
         if (v == 1) {
           flush();
-          // site i: fields of its partners, then its images
-          field_flip<kR, kBM, false>(rr, du, fi.p, of, nf, occ, map, fld, gfx, B, 0, unused);
-          uint32_t img_i;
-          if constexpr (kCanonical) {
-            Ref full = ref_of<true>(c, fi.site);
-            img_i = image_word<true>(c, full);
-          } else {
-            img_i = __builtin_amdgcn_readlane(v_img, f);
+          // each flipped site (i, then j of a swap with i already flipped):
+          // the fields of its partners, then its images; one copy of the
+          // update code serves both flips
+#pragma unroll 1
+          for (int fl = 0; fl < (kCanonical ? 2 : 1); ++fl) {
+            const int p_fl = fl == 0 ? fi.p : fj.p;
+            const int site_fl = fl == 0 ? fi.site : fj.site;
+            const uint32_t so = fl == 0 ? of : nf, sn = fl == 0 ? nf : of;
+            field_flip<kR, kBM, false>(rr, du, p_fl, so, sn, occ, map, fld, gfx, B, 0, unused);
+            uint32_t img;
+            if constexpr (kCanonical)
+              img = image_word<true>(c, ref_of<true>(c, site_fl));
+            else
+              img = __builtin_amdgcn_readlane(v_img, f);
+            const int idx = image_of(c, img, lane & 7);
+            if (lane < 8 && idx >= 0) occ[idx] = uint8_t(sn);
           }
-          {
-            const int idx = image_of(c, img_i, lane & 7);
-            if (lane < 8 && idx >= 0) occ[idx] = uint8_t(nf);
-          }
           if constexpr (kCanonical) {
-            // site j flips b -> a with site i already at b
-            field_flip<kR, kBM, false>(rr, du, fj.p, nf, of, occ, map, fld, gfx, B, 0, unused);
-            const Ref full = ref_of<true>(c, fj.site);
-            const int idx = image_of(c, image_word<true>(c, full), lane & 7);
-            if (lane < 8 && idx >= 0) occ[idx] = uint8_t(of);
             if (lane == 0) {
               trk[__builtin_amdgcn_readlane(v_ta, f)] = uint16_t(fj.site);
               trk[__builtin_amdgcn_readlane(v_tb, f)] = uint16_t(fi.site);

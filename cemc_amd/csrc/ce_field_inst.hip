@@ -20,6 +20,7 @@ const void* field_by_kr(int kr) {
 
 const void* field_fn(bool canonical, int kr, int kbm) {
   if (kbm == 1) return canonical ? field_by_kr<true, 1>(kr) : field_by_kr<false, 1>(kr);
+  if (kbm == 2) return canonical ? field_by_kr<true, 2>(kr) : field_by_kr<false, 2>(kr);
   if (kbm == 3) return canonical ? field_by_kr<true, 3>(kr) : field_by_kr<false, 3>(kr);
   return nullptr;
 }

@@ -150,6 +150,7 @@ struct cemc_model {
   void build_duals();
   int code_bits() const;    // bits per species in an energy-table code
   int deco_slices() const;  // decoration slices per table block (>= n_species - 1)
+  int read_slices() const;  // slices the sweep kernel reads per record (its kBM)
   void refresh_view();
   ~cemc_model();
 };
@@ -192,6 +193,9 @@ cemc_model::~cemc_model() {
 // so the kernel never masks them.  Block 0 is all zeros (padding records).
 int cemc_model::code_bits() const { return n_species <= 2 ? 1 : (n_species <= 4 ? 2 : 3); }
 int cemc_model::deco_slices() const { return n_species <= 2 ? 1 : (n_species <= 4 ? 3 : 7); }
+// up to four species every slice of a block holds a basis function in use
+// except the third for three species, which the kernel need not read
+int cemc_model::read_slices() const { return n_species <= 4 ? n_species - 1 : 7; }
 
 void cemc_model::build_gtab(const std::vector<double>& J) {
   gtab.assign(gtab.size(), 0.0);
@@ -1046,7 +1050,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
     Key k;
     k.periodic = m->periodic;
     k.canonical = canon;
-    k.kbm = m->deco_slices();
+    k.kbm = m->read_slices();
     const bool single = m->n_groups == 1 && int(m->movable.size()) == m->n_sites;
     const int cnt = m->grp_rec_cnt[0];
     const int occ_bytes = m->periodic ? m->pdims[0] * m->pdims[1] * m->pdims[2] : m->n_sites;

@@ -23,7 +23,7 @@ struct LaunchSweep {
   bool canonical;
   int kr;          // records per lane held in VGPRs (2, 4, 8) or 0 = records in LDS
   bool gtab_lds;
-  int kbm;         // decorations per table read: 3 (up to 4 species) or 7
+  int kbm;         // decoration slices read per record: n_bfs for up to 4 species (1, 2, 3), else 7
   int waves;       // wavefronts per chain: 1, 2, 4 or 8 (more than 1: kr > 0)
   int spec;        // moves per round: 1, or kSpecMoves (periodic, kr > 0)
   bool field;      // local-field kernel (periodic, one wave, kr > 0, kbm <= 3)

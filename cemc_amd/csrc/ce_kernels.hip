@@ -413,8 +413,8 @@ bool sweep_variant_ok(int kr, bool glds, int kbm, int waves, int spec, bool peri
   (void)glds;
   if (waves != 1 && waves != 2 && waves != 4 && waves != 8) return false;
   if (spec != 1 && !(spec == kSpecMoves && periodic && kr > 0)) return false;
-  if (kr == 2 || kr == 4 || kr == 8) return kbm == 1 || kbm == 3;
-  return waves == 1 && kr == 0 && (kbm == 1 || kbm == 3 || kbm == 7);
+  if (kr == 2 || kr == 4 || kr == 8) return kbm == 1 || kbm == 2 || kbm == 3;
+  return waves == 1 && kr == 0 && (kbm == 1 || kbm == 2 || kbm == 3 || kbm == 7);
 }
 
 uint32_t sweep_static_lds() {
@@ -429,7 +429,7 @@ hipError_t sweep_set_max_lds(uint32_t bytes) {
       for (int w : {1, 2, 4, 8})
         for (int kr : {0, 2, 4, 8})
           for (int g = 0; g < 2; ++g)
-            for (int kbm : {1, 3, 7})
+            for (int kbm : {1, 2, 3, 7})
               for (int sp : {1, kSpecMoves}) {
               if (!sweep_variant_ok(kr, g != 0, kbm, w, sp, p != 0)) continue;
               const void* fn = sweep_fn(p != 0, cn != 0, kr, g != 0, kbm, w, sp);
@@ -444,7 +444,7 @@ hipError_t sweep_set_max_lds(uint32_t bytes) {
             }
   for (int cn = 0; cn < 2; ++cn)
     for (int kr : {2, 4, 8})
-      for (int kbm : {1, 3}) {
+      for (int kbm : {1, 2, 3}) {
         const void* fn = field_fn(cn != 0, kr, kbm);
         if (fn == nullptr) return hipErrorInvalidValue;
         hipFuncAttributes attr;

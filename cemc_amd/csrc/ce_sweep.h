@@ -109,7 +109,7 @@ __device__ __attribute__((always_inline)) inline uint32_t wrapped_delta(const Sw
 
 template <int kBM>
 struct TableGeom {
-  static constexpr int kBits = kBM == 1 ? 1 : (kBM == 3 ? 2 : 3);  // code bits per species
+  static constexpr int kBits = kBM == 1 ? 1 : (kBM <= 3 ? 2 : 3);  // code bits per species
   static constexpr int kCodes = 1 << (3 * kBits);                   // codes per decoration slice
 };
 

@@ -39,6 +39,7 @@ const void* by_kr(int kr) {
 template <bool C, int K>
 const void* lookup(int kr, bool glds, int kbm) {
   if (kbm == 1) return glds ? by_kr<C, 1, true, K>(kr) : by_kr<C, 1, false, K>(kr);
+  if (kbm == 2) return glds ? by_kr<C, 2, true, K>(kr) : by_kr<C, 2, false, K>(kr);
   if (kbm == 3) return glds ? by_kr<C, 3, true, K>(kr) : by_kr<C, 3, false, K>(kr);
   if constexpr (W == 1 && K == 1)
     if (kr == 0)

@@ -250,3 +250,30 @@ def test_energy_path_choice(gpu):
     tb = ChainBatch([table], [_chain(fx)])
     tb.set_energy_path(PATH_FIELDS)
     assert tb.energy_path(0) == PATH_EVALUATE
+
+
+@pytest.mark.parametrize("species", [["A", "B"], ["A", "B", "C"]])
+@pytest.mark.parametrize("shells", [{2: 4}, {2: 4, 3: 2}, {2: 4, 3: 3}, {2: 2, 4: 2}, {2: 4, 3: 3, 4: 2}])
+def test_fields_equal_evaluation_across_cluster_sets(gpu, species, shells):
+    """The field path against the evaluating path over 30 launches of 64
+    moves (fields rebuilt per launch, updated within it): identical accept
+    sequences and energies within rounding, for flips and swaps, with 2 to 8
+    records per lane (ternary swaps with triplets to the 3rd shell included)."""
+    from cemc_amd.batch import PATH_EVALUATE, PATH_FIELDS, ChainBatch
+    from cemc_amd.lattice import FCCSettings, random_occupation, seeded_ecis
+    from cemc_amd.model import CEModel
+    st = FCCSettings(8, species, shells)
+    model = CEModel(st, seeded_ecis(st, 5, 0.02))
+    for kind in (0, 1):
+        chain = dict(kind=kind, temperature=1200.0, occupation=random_occupation(st.n_sites, len(species), 3),
+                     seed=2, uid=1)
+        out = {}
+        for path in (PATH_EVALUATE, PATH_FIELDS):
+            b = ChainBatch([model], [chain])
+            b.set_energy_path(path)
+            assert b.energy_path(0) == path
+            logs = [_bits(b.run_logged(64)[0], 64) for _ in range(30)]
+            out[path] = (np.concatenate(logs), b.energies()[0], b.occupation(0))
+        assert np.array_equal(out[PATH_EVALUATE][0], out[PATH_FIELDS][0]), (kind, shells)
+        assert np.array_equal(out[PATH_EVALUATE][2], out[PATH_FIELDS][2])
+        assert abs(out[PATH_EVALUATE][1] - out[PATH_FIELDS][1]) / st.n_sites <= 1e-12
