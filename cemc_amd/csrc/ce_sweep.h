@@ -608,14 +608,10 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         acc_bits |= uint64_t(1) << l;
       }
       ++run;
-      if constexpr (kW == 1) {
-        __syncthreads();
-      } else {
-        // this wave's own commit precedes its next gathers in program order
-        // (LDS operations of one wave complete in order); other waves are
-        // ordered by the barrier in the next move's group_sum
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      }
+      // this wave's own commit precedes its next gathers in program order
+      // (LDS operations of one wave complete in order, so no wait is needed);
+      // other waves are ordered by the barrier in the next move's group_sum
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
     }
     n_acc += uint64_t(__builtin_popcountll(acc_bits));
     if (acc_log != nullptr && tid < 2) {
@@ -856,11 +852,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         run += stop - cursor;
         l0 += stop;
         ++rnd;
-        if constexpr (kW == 1) {
-          __syncthreads();
-        } else {
-          __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);  // as in the one-move loop
       }
       n_acc += uint64_t(__builtin_popcountll(acc_bits));
       if (acc_log != nullptr && tid < 2) {
