@@ -12,10 +12,21 @@ across calls.  Energy sums (and, in the SGC ensemble, the SGCObserver's
 singlet sums, cemc/mcmc/mc_observers.py:222-270) are kept per move on the
 device; ``get_thermodynamic`` turns them into the reference's quantities.
 
-Not carried over (listed as later work): the adaptive equilibration and
-correlation-time estimates (montecarlo.py:461-697), the ``"prec"`` stopping
-mode, waste recycling, constraints, bias potentials and Python observers
-called per move.  ``equil=True`` runs a fixed burn-in instead (see runMC).
+``runMC`` follows the reference's protocol: with ``equil=True`` a
+correlation-time estimate from a 1000-move energy trace
+(montecarlo.py:461-526) and adaptive equilibration over windows of 10 N moves
+compared by a z-test on their mean energies (and, in the SGC ensemble, on
+their mean singlets; montecarlo.py:541-697, sgc_montecarlo.py:172-210);
+``mode="prec"`` stops once the variance of the mean energy (SGC: of the mean
+singlets) is below (prec / z_{1-confidence})^2, checked every ten
+correlation times (montecarlo.py:699-716, 790-844).  Each window is one GPU
+launch whose sums come from the device; the trace is read move by move.
+The energy-bias probe of the reference (montecarlo.py:178-217) keeps its
+squared sums well conditioned; the device sums here are taken relative to
+the energy at the last reset for the same purpose.
+
+Not carried over: waste recycling, constraints, bias potentials and Python
+observers called per move.
 """
 from __future__ import annotations
 
@@ -27,6 +38,73 @@ from .lattice import KB_EV
 
 def _fresh_seed():
     return int(np.random.SeedSequence().generate_state(1, dtype=np.uint64)[0])
+
+
+class DidNotReachEquillibriumError(Exception):
+    """montecarlo.py:27."""
+
+
+def _norm_ppf(p):
+    from scipy.stats import norm
+    return float(norm.ppf(p))
+
+
+def correlation_time(energies):
+    """Correlation time of an energy series (montecarlo.py:461-526, after
+    van de Walle and Asta 2002): the autocorrelation function normalised by
+    N * var drops below 1/2 first at lag i; then rho = 2^(-1/i) and
+    tau = -1 / ln(rho).  Returns the reference's info dictionary."""
+    e = np.asarray(energies, dtype=np.float64)
+    n = len(e)
+    info = {"correlation_time_found": False, "correlation_time": 0.0, "msg": ""}
+    dev = e - e.mean()
+    var = np.var(dev)
+    if var == 0.0:
+        info["msg"] = "Zero variance leads to infinite correlation time"
+        info["correlation_time_found"] = True
+        info["correlation_time"] = n
+        return info
+    ac = np.correlate(dev, dev, mode="full")
+    ac = ac[int(len(ac) / 2):] / (n * var)
+    if np.min(ac) > 0.5:
+        info["msg"] = "Window is too short. Add more samples"
+        info["correlation_time"] = n
+        return info
+    indx = 0
+    for i in range(len(ac)):
+        if ac[i] < 0.5:
+            indx = i
+            break
+    rho = 2.0 ** (-1.0 / indx)
+    info["correlation_time"] = -1.0 / np.log(rho)
+    info["correlation_time_found"] = True
+    return info
+
+
+def variance_of_mean(var, n, corr_info):
+    """Variance of an average of n correlated samples with variance var
+    (montecarlo.py:417-447): var / n without a correlation time, else
+    2 var tau / n with tau at least 1."""
+    var = np.abs(var)
+    if corr_info is None or not corr_info["correlation_time_found"]:
+        return var / n
+    tau = max(corr_info["correlation_time"], 1.0)
+    return 2.0 * var * tau / n
+
+
+def prec_check_steps(check_every, total):
+    """Step counts after which the reference's prec mode tests convergence
+    (the first step past each multiple of check_every, montecarlo.py:790-838),
+    up to total."""
+    out = []
+    nxt = check_every
+    while True:
+        s = int(np.floor(nxt)) + 1
+        if s >= total:
+            break
+        out.append(s)
+        nxt += check_every
+    return out
 
 
 class Montecarlo:
@@ -66,6 +144,8 @@ class Montecarlo:
         self._batch_occ = None   # occupation the batch was built from / last synced
         self._batch_eci = None
         self._chem_pot_vec = None
+        self.correlation_info = None
+        self.converged = False
         self.current_energy = calc.get_energy()
 
     # ------------------------------------------------------------ properties
@@ -128,38 +208,131 @@ class Montecarlo:
         self.calc.set_symbols(symbs)
         self.update_current_energy()
 
-    def _burn_in(self, equil_params):
-        steps = int(equil_params.get("steps", 10 * len(self.atoms)))
-        if steps > 0:
-            batch = self._ensure_batch()
-            batch.run(steps)
-            self._moves += steps
+    def _advance(self, n):
+        batch = self._ensure_batch()
+        batch.run(int(n))
+        self._moves += int(n)
+        return batch
+
+    def _energy_trace(self, n):
+        """Energy after each of the next n moves (one launch per move)."""
+        batch = self._ensure_batch()
+        out = np.empty(int(n))
+        for i in range(int(n)):
+            batch.run(1)
+            out[i] = batch.energies()[0]
+        self._moves += int(n)
+        return out
+
+    def _estimate_correlation_time(self, window_length=1000, restart=False):
+        """montecarlo.py:461-526."""
+        if restart or not hasattr(self, "corrtime_energies"):
+            self.corrtime_energies = []
+        self.corrtime_energies.extend(self._energy_trace(window_length).tolist())
+        self.correlation_info = correlation_time(self.corrtime_energies)
+        return self.correlation_info
+
+    def _window(self, n):
+        """Reset the sums, run n moves, return the device averages."""
+        batch = self._ensure_batch()
+        batch.reset_stats(0)
+        self._advance(n)
+        return batch.thermo(0)
+
+    def _get_var_average_energy(self, th=None):
+        """montecarlo.py:417-447."""
+        th = th if th is not None else self._batch.thermo(0)
+        return variance_of_mean(th["var_energy"], max(th["n_samples"], 1), self.correlation_info)
+
+    def _composition_reached_equillibrium(self, prev_composition, var_prev, th, confidence_level=0.05):
+        """Fixed composition: always (montecarlo.py:530-539)."""
+        return True, prev_composition, var_prev, 0.0
+
+    def _equillibriate(self, window_length="auto", confidence_level=0.05, maxiter=1000, mode="stat_equiv"):
+        """montecarlo.py:541-697: windows of ``window_length`` moves (10 N
+        for "auto") until the mean energies of two consecutive windows differ
+        by less than the confidence band of a z-test (and, in the SGC
+        ensemble, the mean singlets as well); mode "fixed" runs one window."""
+        if mode not in ("stat_equiv", "fixed"):
+            raise ValueError("Equilibration mode has to be one of ['stat_equiv', 'fixed']")
+        if window_length == "auto":
+            window_length = 10 * len(self.atoms)
+        window_length = int(window_length)
+        self.reset()
+        if mode == "fixed":
+            self._advance(window_length)
+            return 0
+        lo, hi = _norm_ppf(confidence_level), _norm_ppf(1.0 - confidence_level)
+        e_prev = var_prev = None
+        composition, var_comp = [], []
+        for it in range(maxiter):
+            th = self._window(window_length)
+            e_new = th["mean_energy"]
+            var_new = self._get_var_average_energy(th)
+            comp_conv, composition, var_comp, _ = self._composition_reached_equillibrium(
+                composition, var_comp, th, confidence_level=confidence_level)
+            if e_prev is None:
+                e_prev, var_prev = e_new, var_new
+                continue
+            var_diff = var_new + var_prev
+            if var_diff < 1e-6:
+                z = 0.0
+                comp_conv = True
+            else:
+                z = (e_new - e_prev) / np.sqrt(var_diff)
+            if lo < z < hi and comp_conv:
+                self.reset()
+                return (it + 2) * window_length
+            e_prev, var_prev = e_new, var_new
+        raise DidNotReachEquillibriumError("Did not manage to reach equillibrium!")
+
+    def _has_converged_prec_mode(self, prec=0.01, confidence_level=0.05, th=None):
+        """montecarlo.py:699-716."""
+        percentile = _norm_ppf(1.0 - confidence_level)
+        return self._get_var_average_energy(th) < (prec / percentile) ** 2
 
     def runMC(self, mode="fixed", steps=10, verbose=False, equil=True, equil_params=None, prec=0.01,
               prec_confidence=0.05):
-        """Run ``steps`` trial moves on the GPU (montecarlo.py:732-844).
+        """Run up to ``steps`` trial moves on the GPU (montecarlo.py:732-844).
 
-        ``equil=True`` first runs a burn-in of ``equil_params["steps"]``
-        moves (default 10 sweeps) whose samples are discarded; the reference
-        instead equilibrates adaptively.  Only ``mode="fixed"`` is available.
+        ``equil=True``: correlation time from a 1000-move trace, then
+        adaptive equilibration (``equil_params`` as ``_equillibriate``).
+        ``mode="prec"``: the correlation time is estimated (longer traces
+        until found) and the run stops at the first convergence check, every
+        ten correlation times, that meets ``prec``.  The averages of
+        ``get_thermodynamic`` cover the production run only.
         """
         if mode not in ("fixed", "prec"):
             raise ValueError("Mode has to be one of ['fixed', 'prec']")
-        if mode == "prec":
-            raise NotImplementedError("precision mode needs the correlation-time estimate, not on the GPU path")
         self._check_symbols()
         self.update_current_energy()
-        equil_params = dict(equil_params or {})
+        self.correlation_info = None
         if equil:
-            self._burn_in(equil_params)
+            res = self._estimate_correlation_time(restart=True)
+            if not res["correlation_time_found"]:
+                res["correlation_time"] = 1000
+                res["correlation_time_found"] = True
+            self._equillibriate(**dict(equil_params or {}))
+        checks = []
+        if mode == "prec":
+            res = self._estimate_correlation_time(restart=True)
+            while not res["correlation_time_found"]:
+                res = self._estimate_correlation_time()
+            checks = prec_check_steps(10 * self.correlation_info["correlation_time"], int(steps))
         batch = self._ensure_batch()
         batch.reset_stats(0)
         stats0 = batch.stats()
         steps = int(steps)
-        batch.run(steps)
-        self._moves += steps
+        done = 0
+        self.converged = False
+        for s in checks + [steps]:
+            self._advance(s - done)
+            done = s
+            if s < steps and self._has_converged_prec_mode(prec, prec_confidence, batch.thermo(0)):
+                self.converged = True
+                break
         stats = batch.stats()
-        self.current_step = steps
+        self.current_step = done
         self.num_accepted = int(stats["accepted"][0] - stats0["accepted"][0])
         self._sync_back()
         return [self.current_energy]
@@ -231,6 +404,33 @@ class SGCMonteCarlo(Montecarlo):
 
     def _check_symbols(self):
         pass
+
+    def _get_var_average_singlets(self, th):
+        """sgc_montecarlo.py:94-130."""
+        return variance_of_mean(np.asarray(th["var_singlet"]), max(th["n_samples"], 1), self.correlation_info)
+
+    def _composition_reached_equillibrium(self, prev_composition, var_prev, th, confidence_level=0.05):
+        """sgc_montecarlo.py:172-210: z-test on the largest change of the
+        mean singlets between windows."""
+        lo, hi = _norm_ppf(confidence_level), _norm_ppf(1.0 - confidence_level)
+        singlets = np.asarray(th["singlet"], dtype=np.float64)
+        var_n = np.maximum(self._get_var_average_singlets(th), 0.0)
+        if len(prev_composition) != len(singlets):
+            return False, singlets, var_n, 0.0
+        diff = singlets - np.asarray(prev_composition)
+        var_diff = var_n + np.asarray(var_prev)
+        pos = var_diff > 0.0
+        if not np.any(pos):
+            return True, singlets, var_n, 0.0
+        z = float(np.max(np.abs(diff[pos]) / np.sqrt(var_diff[pos])))
+        return lo < z < hi, singlets, var_n, z
+
+    def _has_converged_prec_mode(self, prec=0.01, confidence_level=0.05, th=None):
+        """sgc_montecarlo.py:132-155: the reference decides on the singlets
+        alone (its energy test is computed and not used)."""
+        th = th if th is not None else self._batch.thermo(0)
+        percentile = _norm_ppf(1.0 - confidence_level)
+        return float(np.max(self._get_var_average_singlets(th))) < (prec / percentile) ** 2
 
     def runMC(self, mode="fixed", steps=10, verbose=False, chem_potential=None, equil=True, equil_params=None,
               prec_confidence=0.05, prec=0.01):

@@ -140,3 +140,71 @@ def test_sampler_requires_chemical_potential(gpu):
         mc.runMC(steps=10)
     with pytest.raises(ValueError):
         mc.runMC(steps=10, chem_potential={"c2_d0000_0_00": 0.1})
+
+
+def _replay_protocol(trace, window, steps, prec, conf):
+    """runMC(equil=True, mode="prec") decided on a per-move energy trace
+    (energy after each move): returns (moves consumed, production steps,
+    converged)."""
+    from cemc_amd.mcmc import _norm_ppf, correlation_time, prec_check_steps, variance_of_mean
+    pos = 1000
+    info = correlation_time(trace[:1000])
+    if not info["correlation_time_found"]:
+        info = {"correlation_time_found": True, "correlation_time": 1000, "msg": ""}
+    lo, hi = _norm_ppf(conf), _norm_ppf(1.0 - conf)
+    e_prev = v_prev = None
+    while True:
+        w = trace[pos:pos + window]
+        pos += window
+        e_new, v_new = w.mean(), variance_of_mean(w.var(), window, info)
+        if e_prev is not None:
+            vd = v_new + v_prev
+            z = 0.0 if vd < 1e-6 else (e_new - e_prev) / np.sqrt(vd)
+            if lo < z < hi:
+                break
+        e_prev, v_prev = e_new, v_new
+    series = list(trace[pos:pos + 1000])
+    pos += 1000
+    info = correlation_time(series)
+    while not info["correlation_time_found"]:
+        series += list(trace[pos:pos + 1000])
+        pos += 1000
+        info = correlation_time(series)
+    pct = _norm_ppf(1.0 - conf)
+    for s in prec_check_steps(10 * info["correlation_time"], steps):
+        seg = trace[pos:pos + s]
+        if variance_of_mean(seg.var(), s, info) < (prec / pct) ** 2:
+            return pos + s, s, True
+    return pos + steps, steps, False
+
+
+def test_runmc_equilibration_and_precision_mode_replay_oracle(gpu):
+    """Adaptive equilibration and the precision stopping rule on the GPU take
+    the same decisions as the same protocol replayed on the oracle's
+    per-move energies of the same chain: moves consumed, stop step, lattice
+    and energy agree."""
+    s, eci, occ, atoms = _system(6, ["A", "B"], {2: 4, 3: 2}, 41)
+    calc = CE(atoms, s, eci=dict(eci))
+    T, steps, prec, conf, window = 700.0, 100000, 0.05, 0.05, 10 * s.n_sites
+    mc = Montecarlo(atoms, T, seed=77, uid=3)
+    mc.runMC(mode="prec", steps=steps, equil=True, equil_params={"window_length": window, "maxiter": 200},
+             prec=prec, prec_confidence=conf)
+    ora = _oracle(s, occ, eci)
+    _, trace, _ = ora.run(0, 77, 3, mc._moves, T, trace_every=1)
+    used, prod, conv = _replay_protocol(trace, window, steps, prec, conf)
+    print("moves", mc._moves, "production", mc.current_step, "converged", mc.converged, flush=True)
+    assert (mc._moves, mc.current_step, mc.converged) == (used, prod, conv)
+    assert conv and prod < steps
+    assert np.array_equal(calc.updater.get_occupation(), ora.get_occupation())
+    assert abs(calc.get_energy() - ora.get_energy()) < TOL * s.n_sites
+    assert abs(mc.get_thermodynamic()["energy"] - trace[used - prod:used].mean()) < 1e-9
+
+
+def test_sgc_precision_mode_converges(gpu):
+    s, eci, occ, atoms = _system(6, ["A", "B"], {2: 4, 3: 2}, 43)
+    CE(atoms, s, eci=dict(eci))
+    mc = SGCMonteCarlo(atoms, 900.0, symbols=["A", "B"], seed=5)
+    mc.runMC(mode="prec", steps=200000, chem_potential={"c1_0": 0.02}, equil=True, prec=0.01)
+    th = mc.get_thermodynamic()
+    assert mc.converged and th["n_mc_steps"] == mc.current_step < 200000
+    assert -1.0 <= th["singlet_c1_0"] <= 1.0
