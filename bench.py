@@ -188,9 +188,40 @@ def cpu_baseline(settings, eci, chains, target_s=12.0):
     t0 = time.perf_counter()
     run_many(ups, 1, 77, uids, per_thread, temps, threads, move_begin=2000)
     dt = time.perf_counter() - t0
+    # reference-shaped: one chain, the flip's CF updates split over threads
+    # (the reference's set_num_threads, ce_updater.cpp:352); config-1 swaps
+    from cemc_amd.lattice import random_occupation
+    occ1 = random_occupation(settings.n_sites, 2, 42)
+    shaped = {}
+    for t in (1, 2, 4, 8, 16):
+        if t > threads:
+            break
+        up = OracleUpdater(model, occ1, brute_force_cf(settings, occ1, names), dict(eci))
+        up.set_num_threads(t)
+        n1 = 2000
+        t1 = time.perf_counter()
+        up.run(0, 42, 0, n1, 800.0)
+        d1 = max(time.perf_counter() - t1, 1e-3)
+        n1 = int(max(2000, min(n1 * 1.5 / d1, 5e6)))
+        t1 = time.perf_counter()
+        up.run(0, 42, 0, n1, 800.0, move_begin=2000)
+        shaped[str(t)] = n1 / (time.perf_counter() - t1)
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": threads * per_thread / dt, "unit": "moves/s", "cores": threads, "kind": "port",
             "sample": "%d independent SGC chains of the same workload (one per thread), %d flips each, %.1f s"
-                      % (threads, per_thread, dt)}
+                      % (threads, per_thread, dt),
+            "reference_shaped": {"workload": "one canonical chain of the plain configuration (binary FCC 10^3, "
+                                             "800 K), CF updates of each flip split over T threads; swaps/s by T",
+                                 "moves_per_s": shaped},
+            "cpu_model": cpu_model, "nproc": os.cpu_count()}
 
 
 def copy_probe(mib):
