@@ -85,6 +85,51 @@ class CE:
             raise ValueError("The number of atoms and the dimension of the translation matrix is inconsistent\n"
                              "Num atoms: {}. Num row trans mat: {}".format(len(self.atoms), n_sites))
 
+    def backup_dict(self):
+        """Everything needed to rebuild the calculator in its current state
+        (ce_calculator.py:532-546): CFs, symbols, settings arguments with
+        their class, ECIs."""
+        kwargs = dict(getattr(self.BC, "kwargs", {}))
+        kwargs["classtype"] = type(self.BC).__name__
+        return {"cf": dict(self.get_cf()), "symbols": [a.symbol for a in self.atoms],
+                "setting_kwargs": kwargs, "eci": dict(self.eci)}
+
+    def save(self, fname):
+        """JSON file with backup_dict() (ce_calculator.py:548-557)."""
+        import json
+        with open(fname, "w") as outfile:
+            json.dump(self.backup_dict(), outfile, indent=2, separators=(",", ": "))
+
+    @staticmethod
+    def load(fname):
+        """Rebuild a calculator saved by save() (ce_calculator.py:562-571)."""
+        import json
+        with open(fname, "r") as infile:
+            return CE.load_from_dict(json.load(infile))
+
+    @staticmethod
+    def load_from_dict(backup_data):
+        """ce_calculator.py:573-592: settings from their arguments (see
+        cemc_amd.lattice.settings_from_kwargs), atoms with the saved symbols
+        site by site, the saved ECIs and CFs.  ECI names the settings do not
+        define raise ValueError."""
+        from .lattice import SimpleAtoms, settings_from_kwargs
+        bc = settings_from_kwargs(backup_data["setting_kwargs"])
+        names = set(bc.cf_names()) | {"c0"}
+        unknown = sorted(set(backup_data["eci"]) - names)
+        if unknown:
+            raise ValueError("ECIs not defined by the settings: {}".format(unknown))
+        symbols = list(backup_data["symbols"])
+        if len(symbols) != bc.n_sites:
+            raise ValueError("{} symbols for {} sites".format(len(symbols), bc.n_sites))
+        atoms = SimpleAtoms(symbols)
+        cf = backup_data.get("cf")
+        cf = {k: v for k, v in cf.items() if k in names} if cf else None
+        return CE(atoms, bc, eci=backup_data["eci"], initial_cf=cf)
+
+    def __reduce__(self):
+        return (CE.load_from_dict, (self.backup_dict(),))
+
     def copy(self):
         """A new calculator on a copy of the atoms with the current CFs."""
         atoms = self.atoms.copy()

@@ -333,6 +333,10 @@ class FCCSettings:
         self.index_by_trans_symm = [list(range(self.n_sites))]
         self.symmetrize_decorations = bool(symmetrize_decorations)
         self.self_interaction = False
+        # arguments that rebuild these settings (CE.save / CE.load)
+        self.kwargs = {"size": list(self.dims), "species": list(self.species),
+                       "max_shell_by_size": {str(k): int(v) for k, v in dict(max_shell_by_size).items()},
+                       "symmetrize_decorations": self.symmetrize_decorations}
         self.orbits = enumerate_orbits(dict(max_shell_by_size))
         phi = polynomial_basis(len(self.species))
         self.basis = phi
@@ -507,3 +511,60 @@ class SimpleAtoms:
 
     def copy(self):
         return SimpleAtoms(self.get_chemical_symbols())
+
+
+def fcc_shells_within(diameter, a):
+    """Number of FCC neighbour shells whose radius is at most ``diameter``
+    for lattice constant ``a`` (shell radii sqrt(d2) * a / 2)."""
+    n = 1
+    while True:
+        sh = fcc_shells(n + 1)
+        if len(sh) <= n or np.sqrt(sh[n]) * a / 2.0 > diameter + 1e-9:
+            break
+        n += 1
+    return n if np.sqrt(fcc_shells(1)[0]) * a / 2.0 <= diameter + 1e-9 else 0
+
+
+def settings_from_kwargs(kwargs):
+    """Settings from the ``setting_kwargs`` of a saved calculator.
+
+    ``classtype`` "FCCSettings": this module's own arguments.  ``classtype``
+    "CEBulk" with ``crystalstructure`` "fcc" (the reference's CLEASE bulk
+    settings, ce_calculator.py:577-592): ``size`` is the repetition of the
+    primitive cell, the species come from ``basis_elements`` (or
+    ``concentration["basis_elements"]``), and each cluster size up to
+    ``max_cluster_size`` keeps the neighbour shells within
+    ``max_cluster_dia`` (or ``max_cluster_dist``): one diameter for every
+    size, or a list over sizes 2, 3, ... (a list of max_cluster_size + 1
+    entries is indexed by size).  Other CLEASE settings are not available."""
+    kw = dict(kwargs)
+    classtype = kw.pop("classtype", "FCCSettings")
+    if classtype == "FCCSettings":
+        shells = {int(k): int(v) for k, v in kw["max_shell_by_size"].items()}
+        return FCCSettings(kw["size"], kw["species"], shells, kw.get("symmetrize_decorations", True))
+    if classtype != "CEBulk" or str(kw.get("crystalstructure", "")).lower() != "fcc":
+        raise ValueError("Unknown setting classtype: {} (only FCC bulk settings are available)".format(kwargs))
+    elements = kw.get("basis_elements")
+    if elements is None and isinstance(kw.get("concentration"), dict):
+        elements = kw["concentration"].get("basis_elements")
+    if not elements or len(elements) != 1:
+        raise ValueError("FCC bulk settings need one basis with its elements")
+    size = kw["size"]
+    if isinstance(size, int):
+        size = [size] * 3
+    max_size = int(kw.get("max_cluster_size", 4))
+    dia = kw.get("max_cluster_dia", kw.get("max_cluster_dist"))
+    if dia is None:
+        raise ValueError("max_cluster_dia is required")
+    a = float(kw["a"])
+    if np.isscalar(dia):
+        dias = {n: float(dia) for n in range(2, max_size + 1)}
+    elif len(dia) == max_size + 1:
+        dias = {n: float(dia[n]) for n in range(2, max_size + 1)}
+    elif len(dia) == max_size - 1:
+        dias = {n: float(dia[n - 2]) for n in range(2, max_size + 1)}
+    else:
+        raise ValueError("max_cluster_dia must be a number or one value per cluster size")
+    shells = {n: fcc_shells_within(d, a) for n, d in dias.items()}
+    shells = {n: k for n, k in shells.items() if k > 0}
+    return FCCSettings(list(size), list(elements[0]), shells)

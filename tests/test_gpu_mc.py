@@ -208,3 +208,29 @@ def test_sgc_precision_mode_converges(gpu):
     th = mc.get_thermodynamic()
     assert mc.converged and th["n_mc_steps"] == mc.current_step < 200000
     assert -1.0 <= th["singlet_c1_0"] <= 1.0
+
+
+def test_calculator_save_load_and_pickle(gpu, tmp_path):
+    """CE.save / CE.load and pickling rebuild the calculator: same symbols,
+    CFs and energy (ce_calculator.py:532-592); ECIs the settings do not
+    define are refused."""
+    import json
+    import pickle
+    s, eci, occ, atoms = _system(5, ["Al", "Mg", "Si"], {2: 3, 3: 2}, 51)
+    calc = CE(atoms, s, eci=dict(eci))
+    calc.calculate(atoms, ["energy"], [(3, atoms[3].symbol, "Si" if atoms[3].symbol != "Si" else "Al")])
+    calc.clear_history()
+    fname = str(tmp_path / "ce.json")
+    calc.save(fname)
+    loaded = CE.load(fname)
+    assert [a.symbol for a in loaded.atoms] == [a.symbol for a in atoms]
+    assert abs(loaded.get_energy() - calc.get_energy()) < TOL * s.n_sites
+    cf, cf2 = calc.get_cf(), loaded.get_cf()
+    assert max(abs(cf[n] - cf2[n]) for n in cf) < 1e-12
+    clone = pickle.loads(pickle.dumps(calc))
+    assert abs(clone.get_energy() - calc.get_energy()) < TOL * s.n_sites
+    with open(fname) as f:
+        data = json.load(f)
+    data["eci"]["c9_d0000_0_000"] = 1.0
+    with pytest.raises(ValueError):
+        CE.load_from_dict(data)

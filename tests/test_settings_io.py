@@ -1,0 +1,44 @@
+"""Settings from saved calculator arguments: this package's FCC settings and
+the reference's CLEASE FCC bulk arguments (ce_calculator.py:573-592)."""
+import numpy as np
+import pytest
+
+from cemc_amd.lattice import FCCSettings, fcc_shells_within, settings_from_kwargs
+
+
+def test_fcc_shell_count_from_diameter():
+    """FCC shells lie at a*sqrt(k/2) for k = 1, 2, 3, 4 (a/sqrt 2, a, a*sqrt 1.5,
+    a*sqrt 2): with a = 4.05 a diameter of 4.5 keeps two shells, 4.05 keeps two
+    (boundary included), 5.8 keeps four, 2.0 none."""
+    a = 4.05
+    assert fcc_shells_within(4.5, a) == 2
+    assert fcc_shells_within(4.05, a) == 2
+    assert fcc_shells_within(5.8, a) == 4
+    assert fcc_shells_within(a * np.sqrt(1.5), a) == 3
+    assert fcc_shells_within(2.0, a) == 0
+
+
+def test_cebulk_fcc_arguments():
+    kw = {"classtype": "CEBulk", "crystalstructure": "fcc", "a": 4.05, "size": [3, 4, 5],
+          "concentration": {"basis_elements": [["Mg", "Al"]]}, "max_cluster_size": 3, "max_cluster_dia": 4.5}
+    s = settings_from_kwargs(kw)
+    assert s.dims == (3, 4, 5) and s.species == ["Al", "Mg"]
+    assert s.kwargs["max_shell_by_size"] == {"2": 2, "3": 2}
+    s2 = settings_from_kwargs(dict(kw, max_cluster_dia=[5.8, 4.5]))
+    assert s2.kwargs["max_shell_by_size"] == {"2": 4, "3": 2}
+    s3 = settings_from_kwargs(dict(kw, max_cluster_dia=[0, 0, 5.8, 4.5]))
+    assert s3.kwargs["max_shell_by_size"] == {"2": 4, "3": 2}
+    assert s2.cf_names() == s3.cf_names()
+    with pytest.raises(ValueError):
+        settings_from_kwargs(dict(kw, crystalstructure="bcc"))
+    with pytest.raises(ValueError):
+        settings_from_kwargs({"classtype": "CECrystal"})
+
+
+def test_fcc_settings_round_trip():
+    s = FCCSettings(4, ["C", "A", "B"], {2: 3, 3: 2}, symmetrize_decorations=False)
+    kw = dict(s.kwargs, classtype="FCCSettings")
+    r = settings_from_kwargs(kw)
+    assert r.dims == s.dims and r.species == s.species
+    assert r.cf_names() == s.cf_names()
+    assert r.symmetrize_decorations is False
