@@ -57,6 +57,14 @@ class Thermo(ctypes.Structure):
     ]
 
 
+class WLDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_bins", ctypes.c_int32), ("current_bin", ctypes.c_int32), ("is_first", ctypes.c_int32),
+        ("converged", ctypes.c_int32), ("e_min", ctypes.c_double), ("e_max", ctypes.c_double),
+        ("f", ctypes.c_double), ("f_min", ctypes.c_double), ("flatness", ctypes.c_double),
+    ]
+
+
 # (name, restype, argtypes) of every entry point declared in include/cemc_gpu.h
 SIGNATURES = [
     ("cemc_last_error", ctypes.c_char_p, []),
@@ -96,6 +104,12 @@ SIGNATURES = [
     ("cemc_batch_get_energy_path", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     ("cemc_batch_get_launch_shape", ctypes.c_int, [_vp, ctypes.c_int32, _i32p, _i32p, _i32p]),
     ("cemc_batch_set_temperature", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double]),
+    ("cemc_batch_wl_set", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(WLDesc), _dp, _i32p,
+                                         ctypes.POINTER(ctypes.c_uint8)]),
+    ("cemc_batch_wl_get", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(WLDesc), _dp, _i32p,
+                                         ctypes.POINTER(ctypes.c_uint8), _i64p]),
+    ("cemc_batch_wl_run", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64,
+                                         ctypes.POINTER(ctypes.c_uint32), ctypes.c_int64, _dp]),
     ("cemc_pool_create", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
                                         ctypes.POINTER(_vp)]),
     ("cemc_pool_footprint", ctypes.c_int, [_vp, ctypes.c_int32, _i64p, _i64p]),

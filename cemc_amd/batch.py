@@ -204,6 +204,62 @@ class ChainBatch:
         _capi.check(self._lib.cemc_batch_set_temperature(self.handle, int(chain), float(temperature)))
         self.temperatures[chain] = float(temperature)
 
+    # ------------------------------------------------------- Wang-Landau
+    def wl_set(self, chain, n_bins, e_min, e_max, f, f_min=1e-6, flatness=0.8, current_bin=0, is_first=True,
+               converged=False, logdos=None, hist=None, known=None):
+        """Give canonical chain `chain` a Wang-Landau walker (cemc_batch_wl_set)."""
+        d = _capi.WLDesc(int(n_bins), int(current_bin), int(bool(is_first)), int(bool(converged)), float(e_min),
+                         float(e_max), float(f), float(f_min), float(flatness))
+        arrs = []
+        for a, dt, ct in ((logdos, np.float64, ctypes.c_double), (hist, np.int32, ctypes.c_int32),
+                          (known, np.uint8, ctypes.c_uint8)):
+            if a is None:
+                arrs.append(None)
+                continue
+            a = np.ascontiguousarray(a, dtype=dt)
+            if a.shape != (int(n_bins),):
+                raise ValueError("histogram arrays need n_bins entries")
+            arrs.append(a)
+        ptrs = [a.ctypes.data_as(ctypes.POINTER(ct)) if a is not None else None
+                for a, ct in zip(arrs, (ctypes.c_double, ctypes.c_int32, ctypes.c_uint8))]
+        _capi.check(self._lib.cemc_batch_wl_set(self.handle, int(chain), ctypes.byref(d), *ptrs))
+
+    def wl_get(self, chain):
+        """Walker of chain `chain`: dict with the descriptor fields, the
+        arrays logdos, hist, known and the counters."""
+        d = _capi.WLDesc()
+        _capi.check(self._lib.cemc_batch_wl_get(self.handle, int(chain), ctypes.byref(d), None, None, None, None))
+        n = d.n_bins
+        logdos = np.zeros(n)
+        hist = np.zeros(n, dtype=np.int32)
+        known = np.zeros(n, dtype=np.uint8)
+        cnt = np.zeros(5, dtype=np.int64)
+        _capi.check(self._lib.cemc_batch_wl_get(self.handle, int(chain), None, logdos.ctypes.data_as(_capi._dp),
+                                                hist.ctypes.data_as(_capi._i32p),
+                                                known.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                                cnt.ctypes.data_as(_capi._i64p)))
+        out = {k: getattr(d, k) for k, _ in _capi.WLDesc._fields_}
+        out.update(logdos=logdos, hist=hist, known=known, n_moves=int(cnt[0]), n_accept=int(cnt[1]),
+                   n_outside=int(cnt[2]), n_self=int(cnt[3]), n_flat=int(cnt[4]))
+        return out
+
+    def wl_run(self, n_checks, check_every, log=False, timed=False):
+        """n_checks rounds of check_every moves on every walker, each followed
+        by the flatness test.  With log: per-chain accept bits (uint8, LSB
+        first); with timed: the device time in ms.  Returns (log, ms)."""
+        bits = None
+        words = 0
+        if log:
+            words = max((int(n_checks) * int(check_every) + 31) // 32, 1)
+            bits = np.zeros((self.n_chains, words), dtype=np.uint32)
+        ms = ctypes.c_double(0.0)
+        _capi.check(self._lib.cemc_batch_wl_run(
+            self.handle, int(n_checks), int(check_every),
+            bits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if bits is not None else None, words,
+            ctypes.byref(ms) if timed else None))
+        out_log = bits.view(np.uint8).reshape(self.n_chains, -1) if bits is not None else None
+        return out_log, (ms.value if timed else None)
+
     def close(self):
         if getattr(self, "handle", None):
             self._lib.cemc_batch_destroy(self.handle)

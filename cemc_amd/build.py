@@ -4,7 +4,8 @@
 * cemc_amd/lib/libcemcprobe.so — roofline probes (HBM copy, LDS read)
 
 The sweep kernel's instantiations are compiled as separate objects, one per
-(waves per chain, lattice path), in parallel, then linked with the rest.
+(waves per chain, lattice path), and the Wang-Landau kernel's one per
+lattice path, in parallel, then linked with the rest.
 """
 from __future__ import annotations
 
@@ -43,6 +44,8 @@ def _units():
         for p in (0, 1):
             units.append(("ce_sweep_w%d_p%d" % (w, p), "ce_sweep_inst.hip",
                           ["-DCEMC_SWEEP_WAVES=%d" % w, "-DCEMC_SWEEP_PERIODIC=%d" % p]))
+    for p in (0, 1):
+        units.append(("ce_wl_p%d" % p, "ce_wl_inst.hip", ["-DCEMC_WL_PERIODIC=%d" % p]))
     return units
 
 

@@ -166,6 +166,24 @@ struct ChainState {
 };
 static_assert(sizeof(ChainState) == 256, "ChainState layout");
 
+// Wang-Landau walker on a canonical chain (wang_landau_sampler.cpp:232-404,
+// histogram.cpp:66-172): one histogram per walker, ln g, visit counts and
+// known flags of its bins at bin_off in the batch's bin pools.  Counters
+// accumulate over launches.
+struct WLDev {
+  double e_min, e_max;  // histogram range (total energy, eV)
+  double f;             // modification factor added to ln g per visit
+  double f_min;         // converged once f < f_min
+  double flatness;      // flat when min(count) > flatness * mean(count) over known bins
+  int32_t n_bins;
+  int32_t cur_bin;      // bin of the walker's state
+  int32_t is_first;     // the next in-range proposal is accepted whatever ln g says
+  int32_t converged;
+  int64_t bin_off;
+  uint64_t n_moves, n_accept, n_outside, n_self, n_flat;
+};
+static_assert(sizeof(WLDev) == 104, "WLDev layout");
+
 // Replica-exchange state of a parallel-tempering run over the R chains of a
 // batch.  Slot t keeps temperature kT[t] and its own energy sums; the chain
 // holding slot t is chain_of_slot[t].  Exchanges move chains between slots

@@ -15,6 +15,7 @@
 #include <map>
 #include <numeric>
 #include <stdexcept>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -860,7 +861,20 @@ struct cemc_batch {
   int model_capacity = 0;
   std::map<int64_t, int64_t> occ_free, trk_free;  // free ranges: offset -> length
   int n_cf_max = 1;
+  // Wang-Landau walkers: descriptors per chain (n_bins 0: none), their bins
+  // packed in chain order in the three pools
+  std::vector<WLDev> wl;
+  WLDev* d_wl = nullptr;
+  double* d_wl_logdos = nullptr;
+  uint32_t* d_wl_hist = nullptr;
+  uint8_t* d_wl_known = nullptr;
+  int32_t* d_wl_order = nullptr;
   ~cemc_batch() {
+    dev_free(d_wl);
+    dev_free(d_wl_logdos);
+    dev_free(d_wl_hist);
+    dev_free(d_wl_known);
+    dev_free(d_wl_order);
     dev_free(d_models);
     dev_free(d_params);
     dev_free(d_states);
@@ -1238,6 +1252,188 @@ void batch_launch(cemc_batch* b, const int64_t* n_moves, int64_t moves, bool tim
     HIP_CHECK(e);
   }
   if (timed) {
+    float t = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&t, b->ev0, b->ev1));
+    *ms = double(t);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wang-Landau walkers.
+
+// Bins of every walker, read back from the device (empty before the pools
+// exist).
+struct WLBins {
+  std::vector<double> logdos;
+  std::vector<uint32_t> hist;
+  std::vector<uint8_t> known;
+};
+
+WLBins wl_download(cemc_batch* b) {
+  WLBins h;
+  int64_t total = 0;
+  for (const auto& w : b->wl) total += w.n_bins;
+  if (b->d_wl == nullptr || total == 0) return h;
+  h.logdos.resize(size_t(total));
+  h.hist.resize(size_t(total));
+  h.known.resize(size_t(total));
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  HIP_CHECK(hipMemcpy(h.logdos.data(), b->d_wl_logdos, sizeof(double) * size_t(total), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(h.hist.data(), b->d_wl_hist, sizeof(uint32_t) * size_t(total), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(h.known.data(), b->d_wl_known, size_t(total), hipMemcpyDeviceToHost));
+  std::vector<WLDev> st(b->wl.size());
+  HIP_CHECK(hipMemcpy(st.data(), b->d_wl, sizeof(WLDev) * st.size(), hipMemcpyDeviceToHost));
+  b->wl = st;
+  return h;
+}
+
+// Re-packs the pools after walker `chain` got n_bins bins (its new values
+// follow), keeping every other walker's bins and state.
+void wl_store(cemc_batch* b, int chain, const WLDev& desc, const double* logdos, const uint32_t* hist,
+              const uint8_t* known) {
+  if (b->wl.empty()) b->wl.assign(size_t(b->n_chains), WLDev{});
+  const WLBins old = wl_download(b);
+  std::vector<WLDev> next = b->wl;
+  next[chain] = desc;
+  WLBins nb;
+  int64_t off = 0;
+  for (int c = 0; c < b->n_chains; ++c) {
+    WLDev& w = next[c];
+    const int64_t n = w.n_bins;
+    if (c == chain) {
+      for (int64_t i = 0; i < n; ++i) {
+        nb.logdos.push_back(logdos ? logdos[i] : 0.0);
+        nb.hist.push_back(hist ? hist[i] : 0u);
+        nb.known.push_back(known ? known[i] : uint8_t(0));
+      }
+    } else {
+      const int64_t o = b->wl[c].bin_off;
+      for (int64_t i = 0; i < n; ++i) {
+        nb.logdos.push_back(old.logdos[size_t(o + i)]);
+        nb.hist.push_back(old.hist[size_t(o + i)]);
+        nb.known.push_back(old.known[size_t(o + i)]);
+      }
+    }
+    w.bin_off = off;
+    off += n;
+  }
+  dev_free(b->d_wl_logdos);
+  dev_free(b->d_wl_hist);
+  dev_free(b->d_wl_known);
+  b->d_wl_logdos = nullptr;
+  b->d_wl_hist = nullptr;
+  b->d_wl_known = nullptr;
+  const size_t n = size_t(std::max<int64_t>(off, 1));
+  HIP_CHECK(hipMalloc(&b->d_wl_logdos, sizeof(double) * n));
+  HIP_CHECK(hipMalloc(&b->d_wl_hist, sizeof(uint32_t) * n));
+  HIP_CHECK(hipMalloc(&b->d_wl_known, n));
+  if (off > 0) {
+    HIP_CHECK(hipMemcpy(b->d_wl_logdos, nb.logdos.data(), sizeof(double) * size_t(off), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(b->d_wl_hist, nb.hist.data(), sizeof(uint32_t) * size_t(off), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(b->d_wl_known, nb.known.data(), size_t(off), hipMemcpyHostToDevice));
+  }
+  if (b->d_wl == nullptr) HIP_CHECK(hipMalloc(&b->d_wl, sizeof(WLDev) * size_t(b->n_chains)));
+  HIP_CHECK(hipMemcpy(b->d_wl, next.data(), sizeof(WLDev) * next.size(), hipMemcpyHostToDevice));
+  if (b->d_wl_order == nullptr) HIP_CHECK(hipMalloc(&b->d_wl_order, sizeof(int32_t) * size_t(b->n_chains)));
+  b->wl = next;
+}
+
+// One launch per kernel variant over the chains that carry a walker.
+void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* host_log, int64_t log_words,
+               double* ms) {
+  require(!b->dynamic, "Wang-Landau walkers run on fixed batches, not pools");
+  require(n_checks >= 0, "number of convergence checks must be non-negative");
+  require(check_every > 0, "check_every must be positive");
+  if (host_log) require(n_checks * check_every <= 32 * log_words, "accept log too small for the number of moves");
+  const uint32_t limit = lds_limit();
+  struct WKey {
+    bool periodic;
+    int kr;
+    bool glds;
+    int kbm;
+    bool operator<(const WKey& o) const {
+      return std::tie(periodic, kr, glds, kbm) < std::tie(o.periodic, o.kr, o.glds, o.kbm);
+    }
+  };
+  std::map<WKey, std::pair<std::vector<int32_t>, uint32_t>> groups;
+  for (int c = 0; c < b->n_chains; ++c) {
+    if (b->wl.empty() || b->wl[c].n_bins <= 0) continue;
+    const cemc_model* m = b->models[b->params[c].model];
+    WKey k;
+    k.periodic = m->periodic;
+    k.kbm = m->read_slices();
+    const bool single = m->n_groups == 1 && int(m->movable.size()) == m->n_sites;
+    const int cnt = m->grp_rec_cnt[0];
+    k.kr = 0;
+    if (single && k.kbm <= 3)
+      for (int kr : {2, 4, 8})
+        if (cnt <= kr * 64) {
+          k.kr = kr;
+          break;
+        }
+    const int occ_bytes = m->periodic ? m->pdims[0] * m->pdims[1] * m->pdims[2] : m->n_sites;
+    const int n_rec = k.kr == 0 ? int(m->recs.size()) : 0;
+    const int nbins = b->wl[c].n_bins;
+    const uint32_t with_g = wl_lds_bytes(sweep_lds_bytes(occ_bytes, m->n_sites, n_rec, int(m->gtab.size())), nbins);
+    const uint32_t no_g = wl_lds_bytes(sweep_lds_bytes(occ_bytes, m->n_sites, n_rec, 0), nbins);
+    k.glds = with_g <= std::min(kGtabLdsBudget, limit);
+    const uint32_t lds = k.glds ? with_g : no_g;
+    require(lds <= limit, "the walker's chain and histogram do not fit in the workgroup's LDS");
+    auto& g = groups[k];
+    g.first.push_back(c);
+    g.second = std::max(g.second, lds);
+  }
+  require(!groups.empty(), "no chain of the batch carries a Wang-Landau walker");
+  std::vector<int32_t> order;
+  uint32_t need = 0;
+  for (auto& kv : groups) {
+    order.insert(order.end(), kv.second.first.begin(), kv.second.first.end());
+    need = std::max(need, kv.second.second);
+  }
+  if (need + sweep_static_lds() > 64u * 1024u) HIP_CHECK(sweep_set_max_lds(limit + sweep_static_lds()));
+  HIP_CHECK(hipMemcpyAsync(b->d_wl_order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice,
+                           b->stream));
+  uint32_t* d_log = nullptr;
+  if (host_log) {
+    HIP_CHECK(hipMalloc(&d_log, sizeof(uint32_t) * size_t(b->n_chains) * log_words));
+    HIP_CHECK(hipMemsetAsync(d_log, 0, sizeof(uint32_t) * size_t(b->n_chains) * log_words, b->stream));
+  }
+  if (ms) HIP_CHECK(hipEventRecord(b->ev0, b->stream));
+  int begin = 0;
+  for (auto& kv : groups) {
+    LaunchWL a;
+    a.params = b->d_params;
+    a.states = b->d_states;
+    a.models = b->d_models;
+    a.occ_pool = b->d_occ;
+    a.trk_pool = b->d_trk;
+    a.chain_order = b->d_wl_order + begin;
+    a.walkers = b->d_wl;
+    a.logdos = b->d_wl_logdos;
+    a.hist = b->d_wl_hist;
+    a.known = b->d_wl_known;
+    a.n_chains = int(kv.second.first.size());
+    a.lds_bytes = kv.second.second;
+    a.periodic = kv.first.periodic;
+    a.kr = kv.first.kr;
+    a.gtab_lds = kv.first.glds;
+    a.kbm = kv.first.kbm;
+    a.n_checks = n_checks;
+    a.check_every = check_every;
+    a.accept_log = d_log;
+    a.log_words = log_words;
+    HIP_CHECK(launch_wl(a, b->stream));
+    begin += a.n_chains;
+  }
+  if (ms) HIP_CHECK(hipEventRecord(b->ev1, b->stream));
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  if (d_log) {
+    const hipError_t e = hipMemcpy(host_log, d_log, sizeof(uint32_t) * size_t(b->n_chains) * log_words,
+                                   hipMemcpyDeviceToHost);
+    (void)hipFree(d_log);
+    HIP_CHECK(e);
+  }
+  if (ms) {
     float t = 0.f;
     HIP_CHECK(hipEventElapsedTime(&t, b->ev0, b->ev1));
     *ms = double(t);
@@ -1788,6 +1984,77 @@ int cemc_batch_set_temperature(cemc_batch* b, int32_t chain, double temperature)
     require(temperature > 0.0 && std::isfinite(temperature), "temperature must be positive");
     b->params[chain].kT = temperature * kBoltzmannEV;
     HIP_CHECK(hipMemcpy(&b->d_params[chain].kT, &b->params[chain].kT, sizeof(double), hipMemcpyHostToDevice));
+  });
+}
+
+// ------------------------------------------------------------ Wang-Landau
+int cemc_batch_wl_set(cemc_batch* b, int32_t chain, const cemc_wl_desc* desc, const double* logdos,
+                      const int32_t* hist, const uint8_t* known) {
+  return guarded([&] {
+    require(b && desc, "NULL argument");
+    require(!b->dynamic, "Wang-Landau walkers run on fixed batches, not pools");
+    require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    require(b->params[chain].kind == CEMC_MOVE_CANONICAL, "Wang-Landau walkers make canonical swaps");
+    require(desc->n_bins > 0, "the histogram needs at least one bin");
+    require(std::isfinite(desc->e_min) && std::isfinite(desc->e_max) && desc->e_max > desc->e_min,
+            "the histogram range needs e_min < e_max");
+    require(desc->current_bin >= 0 && desc->current_bin < desc->n_bins, "current bin outside the histogram");
+    require(std::isfinite(desc->f) && desc->f >= 0.0, "modification factor must be finite and non-negative");
+    WLDev w{};
+    w.e_min = desc->e_min;
+    w.e_max = desc->e_max;
+    w.f = desc->f;
+    w.f_min = desc->f_min;
+    w.flatness = desc->flatness;
+    w.n_bins = desc->n_bins;
+    w.cur_bin = desc->current_bin;
+    w.is_first = desc->is_first ? 1 : 0;
+    w.converged = desc->converged ? 1 : 0;
+    wl_store(b, chain, w, logdos, reinterpret_cast<const uint32_t*>(hist), known);
+  });
+}
+
+int cemc_batch_wl_get(cemc_batch* b, int32_t chain, cemc_wl_desc* desc, double* logdos, int32_t* hist,
+                      uint8_t* known, int64_t* counters) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    require(!b->wl.empty() && b->wl[chain].n_bins > 0, "the chain carries no Wang-Landau walker");
+    const WLBins h = wl_download(b);
+    const WLDev& w = b->wl[chain];
+    if (desc) {
+      desc->n_bins = w.n_bins;
+      desc->current_bin = w.cur_bin;
+      desc->is_first = w.is_first;
+      desc->converged = w.converged;
+      desc->e_min = w.e_min;
+      desc->e_max = w.e_max;
+      desc->f = w.f;
+      desc->f_min = w.f_min;
+      desc->flatness = w.flatness;
+    }
+    for (int i = 0; i < w.n_bins; ++i) {
+      const size_t o = size_t(w.bin_off + i);
+      if (logdos) logdos[i] = h.logdos[o];
+      if (hist) hist[i] = int32_t(h.hist[o]);
+      if (known) known[i] = h.known[o];
+    }
+    if (counters) {
+      counters[0] = int64_t(w.n_moves);
+      counters[1] = int64_t(w.n_accept);
+      counters[2] = int64_t(w.n_outside);
+      counters[3] = int64_t(w.n_self);
+      counters[4] = int64_t(w.n_flat);
+    }
+  });
+}
+
+int cemc_batch_wl_run(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* accept_bits,
+                      int64_t words_per_chain, double* kernel_ms) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    if (accept_bits) require(words_per_chain > 0, "words_per_chain must be positive");
+    wl_launch(b, n_checks, check_every, accept_bits, words_per_chain, kernel_ms);
   });
 }
 

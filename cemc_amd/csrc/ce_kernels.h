@@ -45,6 +45,34 @@ hipError_t sweep_set_max_lds(uint32_t bytes);
 uint32_t sweep_static_lds();
 
 hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream);
+
+// Wang-Landau walkers (canonical chains, one wave each): n_checks rounds of
+// check_every moves, each followed by the flatness test.
+struct LaunchWL {
+  const ChainParams* params;
+  ChainState* states;
+  const ModelDev* models;
+  int8_t* occ_pool;
+  uint16_t* trk_pool;
+  const int32_t* chain_order;  // blockIdx -> chain
+  WLDev* walkers;              // per chain
+  double* logdos;              // bin pools
+  uint32_t* hist;
+  uint8_t* known;
+  int n_chains;
+  uint32_t lds_bytes;
+  bool periodic;
+  int kr;
+  bool gtab_lds;
+  int kbm;
+  int64_t n_checks, check_every;
+  uint32_t* accept_log;  // optional, as for the sweep kernel (bits OR-ed in)
+  int64_t log_words;
+};
+// Dynamic LDS of one walker: the sweep layout of its chain (base bytes)
+// followed by its ln g, counts and known flags.
+uint32_t wl_lds_bytes(uint32_t base, int n_bins);
+hipError_t launch_wl(const LaunchWL& a, hipStream_t stream);
 hipError_t launch_pt_exchange(PTDev* pt, ChainState* states, ChainParams* params, hipStream_t stream);
 hipError_t launch_cf(const ModelDev* models, const ChainParams* params, const int8_t* occ_pool,
                      const int32_t* chain_ids, int n_chains, int max_sites, double* partial, double* cf_out,

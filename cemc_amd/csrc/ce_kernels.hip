@@ -398,6 +398,12 @@ CEMC_DECLARE_SWEEP_FN(8, 1)
 #undef CEMC_DECLARE_SWEEP_FN
 
 const void* field_fn(bool canonical, int kr, int kbm);
+const void* wl_fn_p0(int kr, bool glds, int kbm);
+const void* wl_fn_p1(int kr, bool glds, int kbm);
+
+static const void* wl_fn(bool periodic, int kr, bool glds, int kbm) {
+  return periodic ? wl_fn_p1(kr, glds, kbm) : wl_fn_p0(kr, glds, kbm);
+}
 
 static const void* sweep_fn(bool periodic, bool canonical, int kr, bool glds, int kbm, int waves, int spec) {
   switch (waves) {
@@ -442,6 +448,21 @@ hipError_t sweep_set_max_lds(uint32_t bytes) {
               e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
               if (e != hipSuccess) return e;
             }
+  for (int p = 0; p < 2; ++p)
+    for (int kr : {0, 2, 4, 8})
+      for (int g = 0; g < 2; ++g)
+        for (int kbm : {1, 2, 3, 7}) {
+          if (kbm == 7 && kr != 0) continue;
+          const void* fn = wl_fn(p != 0, kr, g != 0, kbm);
+          if (fn == nullptr) return hipErrorInvalidValue;
+          hipFuncAttributes attr;
+          hipError_t e = hipFuncGetAttributes(&attr, fn);
+          if (e != hipSuccess) return e;
+          const int dyn = int(bytes) - int(attr.sharedSizeBytes);
+          if (dyn <= 0) return hipErrorInvalidValue;
+          e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+          if (e != hipSuccess) return e;
+        }
   for (int cn = 0; cn < 2; ++cn)
     for (int kr : {2, 4, 8})
       for (int kbm : {1, 2, 3}) {
@@ -479,6 +500,36 @@ hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream) {
   int64_t log_words = a.log_words;
   void* args[] = {&params, &states, &models, &occ_pool, &trk_pool, &order, &n_moves, &log, &log_words};
   return hipLaunchKernel(fn, dim3(a.n_chains), dim3(64 * a.waves), args, a.lds_bytes, stream);
+}
+
+uint32_t wl_lds_bytes(uint32_t base, int n_bins) {
+  const uint32_t a = (base + 15u) & ~15u;
+  const auto up = [](uint32_t v) { return (v + 15u) & ~15u; };
+  return a + up(uint32_t(n_bins) * 8u) + up(uint32_t(n_bins) * 4u) + up(uint32_t(n_bins));
+}
+
+hipError_t launch_wl(const LaunchWL& a, hipStream_t stream) {
+  if (!((a.kr == 0 && (a.kbm == 1 || a.kbm == 2 || a.kbm == 3 || a.kbm == 7)) ||
+        ((a.kr == 2 || a.kr == 4 || a.kr == 8) && (a.kbm == 1 || a.kbm == 2 || a.kbm == 3))))
+    return hipErrorInvalidValue;
+  const void* fn = wl_fn(a.periodic, a.kr, a.gtab_lds, a.kbm);
+  if (fn == nullptr) return hipErrorInvalidValue;
+  const ChainParams* params = a.params;
+  ChainState* states = a.states;
+  const ModelDev* models = a.models;
+  int8_t* occ_pool = a.occ_pool;
+  uint16_t* trk_pool = a.trk_pool;
+  const int32_t* order = a.chain_order;
+  WLDev* walkers = a.walkers;
+  double* logdos = a.logdos;
+  uint32_t* hist = a.hist;
+  uint8_t* known = a.known;
+  int64_t n_checks = a.n_checks, check_every = a.check_every;
+  uint32_t* log = a.accept_log;
+  int64_t log_words = a.log_words;
+  void* args[] = {&params, &states, &models, &occ_pool, &trk_pool, &order, &walkers, &logdos, &hist, &known,
+                  &n_checks, &check_every, &log, &log_words};
+  return hipLaunchKernel(fn, dim3(a.n_chains), dim3(64), args, a.lds_bytes, stream);
 }
 
 hipError_t launch_pt_exchange(PTDev* pt, ChainState* states, ChainParams* params, hipStream_t stream) {

@@ -252,6 +252,45 @@ CEMC_API int cemc_batch_get_launch_shape(cemc_batch* batch, int32_t chain, int32
 /* Temperature of one chain (parallel tempering uses it to exchange replicas). */
 CEMC_API int cemc_batch_set_temperature(cemc_batch* batch, int32_t chain, double temperature);
 
+/* ----------------------------------------------------------- Wang-Landau
+ * A Wang-Landau walker on a canonical chain of a (fixed) batch: the
+ * reference's native sampler (cpp/src/wang_landau_sampler.cpp, bound by
+ * cemc/cpp_ext/pywang_landau_sampler.pyx:4-30) with one histogram per walker
+ * instead of one shared under a lock, so each chain is an independent
+ * fixed-f Wang-Landau run and their ln g can be averaged.  Energies are the
+ * chain's total energy (eV). */
+typedef struct cemc_wl_desc {
+  int32_t n_bins;      /* Histogram::Nbins */
+  int32_t current_bin; /* bin of the chain's state (ignored while is_first) */
+  int32_t is_first;    /* 1: the next in-range proposal is accepted whatever ln g says */
+  int32_t converged;   /* 1 once f < f_min; a converged walker makes no moves */
+  double e_min, e_max; /* Histogram::Emin, Emax */
+  double f;            /* modification factor added to ln g per visit */
+  double f_min;        /* WangLandauSampler::min_f */
+  double flatness;     /* flatness_criteria: flat when min(count) > flatness * mean(count) over known bins */
+} cemc_wl_desc;
+
+/* Give chain `chain` a walker, or replace its walker.  The constructor's
+ * reads of the Python WangLandau object (wang_landau_sampler.cpp:17-150:
+ * fmin, f, flatness_criteria, histogram Nbins/Emin/Emax/logdos/histogram/
+ * known_state, current_bin).  logdos, hist and known hold n_bins entries
+ * each, or are NULL for zeros.  The counters restart at zero. */
+CEMC_API int cemc_batch_wl_set(cemc_batch* batch, int32_t chain, const cemc_wl_desc* desc, const double* logdos,
+                               const int32_t* hist, const uint8_t* known);
+/* Read a walker back (send_results_to_python, wang_landau_sampler.cpp:406-432).
+ * Any output may be NULL.  counters[5]: moves, accepted, proposals outside
+ * the range, proposals into the current bin, flat histograms seen. */
+CEMC_API int cemc_batch_wl_get(cemc_batch* batch, int32_t chain, cemc_wl_desc* desc, double* logdos,
+                               int32_t* hist, uint8_t* known, int64_t* counters);
+/* WangLandauSampler::run (wang_landau_sampler.cpp:319-404) on every walker
+ * of the batch at once: n_checks rounds of check_every moves (step,
+ * :232-317), each followed by the flatness test; a flat histogram halves f
+ * and clears the counts, and a walker stops at the first check with
+ * f < f_min.  accept_bits (optional) as for cemc_batch_run_logged;
+ * kernel_ms (optional) receives the device time. */
+CEMC_API int cemc_batch_wl_run(cemc_batch* batch, int64_t n_checks, int64_t check_every, uint32_t* accept_bits,
+                               int64_t words_per_chain, double* kernel_ms);
+
 /* ------------------------------------------------------------------ pool
  * A batch whose chains join and leave while it runs: the resident state of
  * a job server.  Slots, model entries and device memory for occupations

@@ -400,6 +400,35 @@ static inline uint32_t bounded32(uint32_t w, uint32_t n) { return (uint32_t)(((u
 static inline double unit_double(uint64_t w) { return (double)(w >> 11) * 0x1.0p-53; }
 
 /* --------------------------------------------------------- Metropolis */
+/* Species index tracker of a canonical chain (built once, then kept by the
+ * moves) and the species present, in id order. */
+static int canonical_setup(ceo_chain* c, int* present) {
+  const ceo_model* m = c->m;
+  const int N = m->n_sites;
+  int* count = c->trk_count;
+  int* trk_off = c->trk_off;
+  int* tracker = c->trk;
+  int n_present = 0;
+  if (!c->trk_valid) {
+    memset(count, 0, sizeof(int) * (size_t)m->n_species);
+    for (int i = 0; i < N; ++i)
+      if (!m->background[i]) count[c->occ[i]] += 1;
+    trk_off[0] = 0;
+    for (int s = 0; s < m->n_species; ++s) trk_off[s + 1] = trk_off[s] + count[s];
+    int* fill = (int*)calloc((size_t)(unsigned)m->n_species + 1, sizeof(int));
+    for (int i = 0; i < N; ++i) {
+      if (m->background[i]) continue;
+      int s = c->occ[i];
+      tracker[trk_off[s] + fill[s]++] = i;
+    }
+    free(fill);
+    c->trk_valid = 1;
+  }
+  for (int s = 0; s < m->n_species; ++s)
+    if (count[s] > 0 && n_present < 64) present[n_present++] = s;
+  return n_present;
+}
+
 int ceo_run(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,
             int64_t n_moves, double temperature, uint8_t* accept_bits, double* e_trace,
             int64_t trace_every, int64_t* n_accept) {
@@ -415,23 +444,7 @@ int ceo_run(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t 
   int* trk_off = c->trk_off;
   int* tracker = c->trk;
   if (kind == 0) {
-    if (!c->trk_valid) {
-      memset(count, 0, sizeof(int) * (size_t)m->n_species);
-      for (int i = 0; i < N; ++i)
-        if (!m->background[i]) count[c->occ[i]] += 1;
-      trk_off[0] = 0;
-      for (int s = 0; s < m->n_species; ++s) trk_off[s + 1] = trk_off[s] + count[s];
-      int* fill = (int*)calloc((size_t)(unsigned)m->n_species + 1, sizeof(int));
-      for (int i = 0; i < N; ++i) {
-        if (m->background[i]) continue;
-        int s = c->occ[i];
-        tracker[trk_off[s] + fill[s]++] = i;
-      }
-      free(fill);
-      c->trk_valid = 1;
-    }
-    for (int s = 0; s < m->n_species; ++s)
-      if (count[s] > 0 && n_present < 64) present[n_present++] = s;
+    n_present = canonical_setup(c, present);
     if (n_present < 2) return fail(CEO_EINVAL, "canonical moves need at least two species present");
   } else {
     c->trk_valid = 0;
@@ -517,6 +530,118 @@ int ceo_run_many(ceo_chain** chains, int n_chains, int kind, uint64_t seed,
     rc |= ceo_run(chains[i], kind, seed, chain_uids[i], move_begin, n_moves, temperatures[i], NULL, NULL,
                   0, &acc);
     if (n_accept) n_accept[i] = acc;
+  }
+  return rc;
+}
+
+/* ------------------------------------------------------- Wang-Landau */
+/* Bin of total energy e: histogram.cpp:66-69 converts
+ * (e - Emin) * Nbins / (Emax - Emin) to int (truncation toward zero) and
+ * :137-140 keeps 0 <= bin < Nbins.  Returns the bin, -1 below the range,
+ * n_bins when the value truncates to n_bins exactly, n_bins + 1 above that
+ * (wang_landau_sampler.cpp:257-258 tells the last two apart). */
+static int wl_bin(double e, double e_min, double e_max, int n_bins) {
+  const double x = (e - e_min) * (double)n_bins / (e_max - e_min);
+  const double t = trunc(x);
+  if (t < 0.0) return -1;
+  if (t > (double)n_bins) return n_bins + 1;
+  return (int)t;
+}
+
+/* Histogram::update (histogram.cpp:71-90): the bin is known, visited once
+ * more, and its ln g grows by f. */
+static void wl_visit(ceo_wl* w, int bin) {
+  w->known[bin] = 1;
+  w->hist[bin] += 1;
+  w->logdos[bin] += w->f;
+}
+
+/* Histogram::is_flat (histogram.cpp:102-128): over the known bins, the
+ * smallest count (starting from 100000000) exceeds criteria times the mean;
+ * the sum is an unsigned 32-bit integer. */
+static int wl_is_flat(const ceo_wl* w) {
+  uint32_t sum = 0, minimum = 100000000u, count = 0;
+  for (int i = 0; i < w->n_bins; ++i) {
+    if (!w->known[i]) continue;
+    sum += (uint32_t)w->hist[i];
+    count += 1;
+    if ((uint32_t)w->hist[i] < minimum) minimum = (uint32_t)w->hist[i];
+  }
+  const double mean = (double)sum / (double)count;
+  return (double)minimum > w->flatness * mean;
+}
+
+int ceo_wl_run(ceo_chain* c, ceo_wl* w, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,
+               int64_t n_checks, int64_t check_every, uint8_t* accept_bits) {
+  int present[64];
+  const int n_present = canonical_setup(c, present);
+  if (n_present < 2) return fail(CEO_EINVAL, "canonical moves need at least two species present");
+  if (w->n_bins < 1 || !(w->e_max > w->e_min)) return fail(CEO_EINVAL, "invalid Wang-Landau histogram");
+  if (w->cur_bin < 0 || w->cur_bin >= w->n_bins) return fail(CEO_EINVAL, "current bin outside the histogram");
+  int* count = c->trk_count;
+  int* trk_off = c->trk_off;
+  int* tracker = c->trk;
+  const uint64_t key[2] = {seed, chain_uid};
+  int64_t t = 0;
+  int rc = CEO_OK;
+  if (w->converged) return CEO_OK; /* a converged walker makes no more moves */
+  for (int64_t chk = 0; chk < n_checks && rc == CEO_OK; ++chk) {
+    /* WangLandauSampler::step, wang_landau_sampler.cpp:232-317 */
+    for (int64_t s = 0; s < check_every; ++s, ++t) {
+      const uint64_t ctr[4] = {move_begin + (uint64_t)t, 0, 0, 0};
+      uint64_t r[4];
+      ceo_philox4x64(ctr, key, r);
+      uint32_t ia = bounded32((uint32_t)(r[0] >> 32), (uint32_t)n_present);
+      uint32_t ib = (ia + 1 + bounded32((uint32_t)r[0], (uint32_t)(n_present - 1))) % (uint32_t)n_present;
+      const int a = present[ia], b = present[ib];
+      const int sa = (int)bounded64(r[1], (uint64_t)count[a]);
+      const int sb = (int)bounded64(r[2], (uint64_t)count[b]);
+      const int sites[2] = {tracker[trk_off[a] + sa], tracker[trk_off[b] + sb]};
+      const int olds[2] = {a, b}, news[2] = {b, a};
+      const double u = unit_double(r[3]);
+      double e_new = 0.0;
+      rc = calculate_impl(c, 2, sites, olds, news, &e_new);
+      if (rc != CEO_OK) break;
+      w->n_moves += 1;
+      const int bin = wl_bin(e_new, w->e_min, w->e_max, w->n_bins);
+      if (bin < 0 || bin >= w->n_bins) {
+        w->n_outside += 1;
+        undo_impl(c);
+        if (w->is_first) continue;
+        if (bin > w->n_bins) wl_visit(w, w->cur_bin);
+        continue;
+      }
+      if (bin == w->cur_bin) w->n_self += 1;
+      /* Histogram::get_dos_ratio_old_divided_by_new (histogram.cpp:130-135) */
+      const double diff = w->logdos[w->cur_bin] - w->logdos[bin];
+      const double ratio = diff > 0.0 ? 1.0 : exp(diff);
+      const int accept = (u < ratio) || w->is_first;
+      if (accept) {
+        w->cur_bin = bin;
+        w->is_first = 0;
+        w->n_accept += 1;
+        tracker[trk_off[a] + sa] = sites[1];
+        tracker[trk_off[b] + sb] = sites[0];
+        if (accept_bits) accept_bits[t >> 3] |= (uint8_t)(1u << (t & 7));
+      } else {
+        undo_impl(c);
+      }
+      ceo_clear_history(c);
+      wl_visit(w, w->cur_bin);
+    }
+    if (rc != CEO_OK) break;
+    /* WangLandauSampler::run, wang_landau_sampler.cpp:337-394: on a flat
+     * histogram f halves (fixed-f scheme) and the counts restart; below
+     * f_min the walker has converged */
+    if (wl_is_flat(w)) {
+      w->f /= 2.0;
+      for (int i = 0; i < w->n_bins; ++i) w->hist[i] = 0;
+      w->n_flat += 1;
+    }
+    if (w->f < w->f_min) {
+      w->converged = 1;
+      break;
+    }
   }
   return rc;
 }

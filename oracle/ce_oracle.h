@@ -85,6 +85,28 @@ int ceo_run_many(ceo_chain** chains, int n_chains, int kind, uint64_t seed,
                  const uint64_t* chain_uids, uint64_t move_begin, int64_t n_moves,
                  const double* temperatures, int n_threads, int64_t* n_accept);
 
+/* Wang-Landau walker (wang_landau_sampler.cpp:232-404, histogram.cpp:66-172)
+ * on a canonical chain: one histogram per walker, the fixed-f scheme.
+ * Counters accumulate over calls. */
+typedef struct ceo_wl {
+  int32_t n_bins;
+  int32_t cur_bin;   /* bin of the walker's state (ignored while is_first) */
+  int32_t is_first;  /* the next in-range move is accepted whatever ln g says */
+  int32_t converged; /* f fell below f_min */
+  double e_min, e_max;
+  double f, f_min, flatness;
+  double* logdos;
+  int32_t* hist;
+  uint8_t* known;
+  int64_t n_moves, n_accept, n_outside, n_self, n_flat;
+} ceo_wl;
+
+/* n_checks rounds of check_every moves, each round followed by the
+ * flatness test; stops early once converged.  Move t uses the Philox block
+ * of counter move_begin + t as ceo_run's canonical moves do. */
+int ceo_wl_run(ceo_chain* c, ceo_wl* w, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,
+               int64_t n_checks, int64_t check_every, uint8_t* accept_bits);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,8 +64,35 @@ def load_library():
     lib.ceo_run_many.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                  ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64, ctypes.c_int64,
                                  dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+    lib.ceo_wl_run.argtypes = [P, ctypes.POINTER(WLState), ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                               ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint8)]
     _lib = lib
     return lib
+
+
+class WLState(ctypes.Structure):
+    """ceo_wl of ce_oracle.h."""
+    _fields_ = [("n_bins", ctypes.c_int32), ("cur_bin", ctypes.c_int32), ("is_first", ctypes.c_int32),
+                ("converged", ctypes.c_int32), ("e_min", ctypes.c_double), ("e_max", ctypes.c_double),
+                ("f", ctypes.c_double), ("f_min", ctypes.c_double), ("flatness", ctypes.c_double),
+                ("logdos", ctypes.POINTER(ctypes.c_double)), ("hist", ctypes.POINTER(ctypes.c_int32)),
+                ("known", ctypes.POINTER(ctypes.c_uint8)), ("n_moves", ctypes.c_int64),
+                ("n_accept", ctypes.c_int64), ("n_outside", ctypes.c_int64), ("n_self", ctypes.c_int64),
+                ("n_flat", ctypes.c_int64)]
+
+
+class OracleWL:
+    """A Wang-Landau walker of the oracle: histogram arrays owned here."""
+
+    def __init__(self, n_bins, e_min, e_max, f, f_min=1e-6, flatness=0.8, cur_bin=0, is_first=True,
+                 logdos=None, hist=None, known=None):
+        self.logdos = np.zeros(n_bins) if logdos is None else np.array(logdos, dtype=np.float64)
+        self.hist = np.zeros(n_bins, dtype=np.int32) if hist is None else np.array(hist, dtype=np.int32)
+        self.known = np.zeros(n_bins, dtype=np.uint8) if known is None else np.array(known, dtype=np.uint8)
+        self.state = WLState(int(n_bins), int(cur_bin), int(bool(is_first)), 0, float(e_min), float(e_max),
+                             float(f), float(f_min), float(flatness), _dp(self.logdos),
+                             self.hist.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                             self.known.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), 0, 0, 0, 0, 0)
 
 
 def _ip(a):
@@ -307,6 +334,19 @@ class OracleUpdater:
         if rc:
             _raise(_lib, rc)
         return bits, trace[:n_tr], acc.value
+
+
+def wl_run(updater, wl, seed, chain_uid, n_checks, check_every, move_begin=0):
+    """Wang-Landau walker `wl` (OracleWL) on the canonical chain of `updater`;
+    returns the accept bits of the moves made (packed LSB-first)."""
+    lib = load_library()
+    n = int(n_checks) * int(check_every)
+    bits = np.zeros(max((n + 7) // 8, 1), dtype=np.uint8)
+    rc = lib.ceo_wl_run(updater.handle, ctypes.byref(wl.state), int(seed), int(chain_uid), int(move_begin),
+                        int(n_checks), int(check_every), bits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    if rc:
+        _raise(lib, rc)
+    return bits
 
 
 def run_many(updaters, kind, seed, chain_uids, n_moves, temperatures, n_threads, move_begin=0):
