@@ -366,3 +366,28 @@ def test_gpu_walker_resumes_across_launches(gpu):
     assert got["n_moves"] == wl.state.n_moves
     assert np.array_equal(got["hist"], wl.hist) and np.array_equal(got["logdos"], wl.logdos)
     assert np.array_equal(batch.occupation(0), ora.get_occupation())
+
+
+@pytest.mark.gpu
+def test_python_driver_converges_to_exact_dos(gpu, tiny):
+    """WangLandau on an atoms object with the CE calculator: eight walkers
+    to f < 1e-4; their averaged ln g matches the enumerated one."""
+    from cemc_amd.calculator import CE
+    from cemc_amd.lattice import SimpleAtoms
+    from cemc_amd.wang_landau import WangLandau
+    s = tiny["s"]
+    occ = tiny["states"][5]
+    atoms = SimpleAtoms([s.species[v] for v in occ])
+    CE(atoms, s, eci=dict(tiny["eci"]))
+    wl = WangLandau(atoms, Nbins=NB, initial_f=1.0, flatness_criteria=0.8, fmin=1e-4, Emin=EMIN, Emax=EMAX,
+                    check_convergence_every=2000, n_walkers=8, seed=4)
+    assert wl.run_fast_sampler(maxsteps=400000)
+    assert all(f == 2.0 ** -14 for f in wl.f_values)
+    g = tiny["g"]
+    known = g > 0
+    est = wl.logdos()
+    assert np.array_equal(~np.isnan(est), known)
+    d = est[known] - np.log(g[known])
+    d -= d.mean()
+    assert np.abs(d).max() < 0.2, d
+    wl.close()
