@@ -57,6 +57,15 @@ class Thermo(ctypes.Structure):
     ]
 
 
+class HooksDesc(ctypes.Structure):
+    _fields_ = [
+        ("fixed_species", ctypes.c_uint32), ("max_attempts", ctypes.c_int32),
+        ("allowed", ctypes.POINTER(ctypes.c_uint8)), ("pair_species", ctypes.c_int32 * 2),
+        ("pair_members", ctypes.c_int32), ("pair_neighbours", ctypes.POINTER(ctypes.c_int32)),
+        ("bias_species", ctypes.c_int32), ("reserved", ctypes.c_int32), ("bias_table", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
 class WLDesc(ctypes.Structure):
     _fields_ = [
         ("n_bins", ctypes.c_int32), ("current_bin", ctypes.c_int32), ("is_first", ctypes.c_int32),
@@ -104,6 +113,8 @@ SIGNATURES = [
     ("cemc_batch_get_energy_path", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     ("cemc_batch_get_launch_shape", ctypes.c_int, [_vp, ctypes.c_int32, _i32p, _i32p, _i32p]),
     ("cemc_batch_set_temperature", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double]),
+    ("cemc_batch_set_hooks", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(HooksDesc)]),
+    ("cemc_batch_get_hooks_state", ctypes.c_int, [_vp, ctypes.c_int32, _i64p, _i32p, _dp]),
     ("cemc_batch_wl_set", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(WLDesc), _dp, _i32p,
                                          ctypes.POINTER(ctypes.c_uint8)]),
     ("cemc_batch_wl_get", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(WLDesc), _dp, _i32p,

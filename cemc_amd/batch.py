@@ -204,6 +204,51 @@ class ChainBatch:
         _capi.check(self._lib.cemc_batch_set_temperature(self.handle, int(chain), float(temperature)))
         self.temperatures[chain] = float(temperature)
 
+    # ---------------------------------------------- constraints and bias
+    def set_hooks(self, chain, fixed_species=0, max_attempts=10000, allowed=None, pair=None, pair_neighbours=None,
+                  bias_species=-1, bias_table=None):
+        """Constraints and bias of chain `chain` (cemc_batch_set_hooks).
+        allowed: uint8 mask per site of the species it may hold; pair: (e1, e2)
+        species ids with pair_neighbours an (n_sites, members) int array;
+        bias_table: n_sites + 1 energies over the count of bias_species."""
+        d = _capi.HooksDesc()
+        d.fixed_species = int(fixed_species)
+        d.max_attempts = int(max_attempts)
+        keep = []
+        n = int(self.n_sites[chain])
+        if allowed is not None:
+            a = np.ascontiguousarray(allowed, dtype=np.uint8)
+            if a.shape != (n,):
+                raise ValueError("allowed needs one mask per site")
+            keep.append(a)
+            d.allowed = a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        d.pair_species[0], d.pair_species[1] = (-1, -1) if pair is None else (int(pair[0]), int(pair[1]))
+        if pair is not None:
+            nb = np.ascontiguousarray(pair_neighbours, dtype=np.int32)
+            if nb.ndim != 2 or nb.shape[0] != n:
+                raise ValueError("pair_neighbours needs one row per site")
+            keep.append(nb)
+            d.pair_members = nb.shape[1]
+            d.pair_neighbours = nb.ctypes.data_as(_capi._i32p)
+        d.bias_species = int(bias_species)
+        if bias_species >= 0:
+            t = np.ascontiguousarray(bias_table, dtype=np.float64)
+            if t.shape != (n + 1,):
+                raise ValueError("bias_table needs n_sites + 1 entries")
+            keep.append(t)
+            d.bias_table = t.ctypes.data_as(_capi._dp)
+        _capi.check(self._lib.cemc_batch_set_hooks(self.handle, int(chain), ctypes.byref(d)))
+
+    def clear_hooks(self, chain):
+        _capi.check(self._lib.cemc_batch_set_hooks(self.handle, int(chain), None))
+
+    def hooks_state(self, chain):
+        """(constrained pair count, biased species count, bias energy)."""
+        pc, bc, be = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_double()
+        _capi.check(self._lib.cemc_batch_get_hooks_state(self.handle, int(chain), ctypes.byref(pc), ctypes.byref(bc),
+                                                         ctypes.byref(be)))
+        return pc.value, bc.value, be.value
+
     # ------------------------------------------------------- Wang-Landau
     def wl_set(self, chain, n_bins, e_min, e_max, f, f_min=1e-6, flatness=0.8, current_bin=0, is_first=True,
                converged=False, logdos=None, hist=None, known=None):

@@ -184,6 +184,28 @@ struct WLDev {
 };
 static_assert(sizeof(WLDev) == 104, "WLDev layout");
 
+// Per-move predicates and bias of one chain (cemc/mcmc/mc_constraints.py,
+// cpp/src/pair_constraint.cpp, cemc/mcmc/bias_potential.py): a trial move
+// is redrawn while it violates a constraint, up to max_attempts proposals
+// (montecarlo.py:970-991), and the bias of the proposed state is added to
+// its energy (montecarlo.py:926-930).
+struct HookDev {
+  uint32_t fixed_species;  // bit s: a change from or to species s is invalid (FixedElement)
+  int32_t max_attempts;    // proposals per move
+  const uint8_t* allowed;  // per site, bit s: species s may occupy it (nullptr: any)
+  int32_t pair_e1, pair_e2;  // PairConstraint species (-1: none): no e1 site with an e2 member
+  int32_t pair_nm, pad0;     // members of the pair cluster per site (<= 32, symmetric lists)
+  const int32_t* pair_nb;    // n_sites x pair_nm member sites
+  int64_t pair_count;        // ordered (e1, e2) member pairs of the current state
+  int32_t bias_species;      // species whose count the bias depends on (-1: none)
+  int32_t bias_count;        // its count in the current state
+  const double* bias_tab;    // n_sites + 1 entries: bias energy at each count
+  double bias_energy;        // bias of the current state (part of ChainState::energy)
+  int32_t status;            // 1: a move found no valid proposal within max_attempts
+  int32_t pad1;
+  uint64_t failed_move;      // Philox counter of that move
+};
+
 // Replica-exchange state of a parallel-tempering run over the R chains of a
 // batch.  Slot t keeps temperature kT[t] and its own energy sums; the chain
 // holding slot t is chain_of_slot[t].  Exchanges move chains between slots

@@ -114,7 +114,8 @@ __global__ __launch_bounds__(64) void field_kernel(const ChainParams* __restrict
                                                    int8_t* __restrict__ occ_pool, uint16_t* __restrict__ trk_pool,
                                                    const int32_t* __restrict__ chain_order,
                                                    const int64_t* __restrict__ n_moves,
-                                                   uint32_t* __restrict__ accept_log, int64_t log_words) {
+                                                   uint32_t* __restrict__ accept_log, int64_t log_words,
+                                                   HookDev* __restrict__ /*hooks: not on this path*/) {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ double dphi_tab[kMaxSpecies * kMaxSpecies * kMaxBasis];  // [(old*S+new)*B + d]
   __shared__ double dsingle_tab[kMaxSpecies * kMaxSpecies];          // singlet energy change per (old,new)

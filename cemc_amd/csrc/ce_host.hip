@@ -846,6 +846,7 @@ struct cemc_batch {
     int waves;
     int spec;  // moves per round
     bool field;  // local-field kernel
+    bool hooks;  // constraints and bias
     int begin, count;
     uint32_t lds;
   };
@@ -863,6 +864,16 @@ struct cemc_batch {
   int n_cf_max = 1;
   // Wang-Landau walkers: descriptors per chain (n_bins 0: none), their bins
   // packed in chain order in the three pools
+  // constraints and bias per chain (host copies; device arrays they point to)
+  struct Hooks {
+    bool on = false;
+    HookDev dev{};
+    uint8_t* d_allowed = nullptr;
+    int32_t* d_nb = nullptr;
+    double* d_btab = nullptr;
+  };
+  std::vector<Hooks> hooks;
+  HookDev* d_hooks = nullptr;
   std::vector<WLDev> wl;
   WLDev* d_wl = nullptr;
   double* d_wl_logdos = nullptr;
@@ -870,6 +881,12 @@ struct cemc_batch {
   uint8_t* d_wl_known = nullptr;
   int32_t* d_wl_order = nullptr;
   ~cemc_batch() {
+    for (auto& h : hooks) {
+      dev_free(h.d_allowed);
+      dev_free(h.d_nb);
+      dev_free(h.d_btab);
+    }
+    dev_free(d_hooks);
     dev_free(d_wl);
     dev_free(d_wl_logdos);
     dev_free(d_wl_hist);
@@ -1037,6 +1054,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
     int waves;
     int spec;
     bool field;
+    bool hooks;
     uint32_t lds;
   };
   // waves per chain: enough chains x waves to give every SIMD of the
@@ -1131,6 +1149,24 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
         k.lds = f_lds;
       }
     }
+    // chains with constraints or a bias: one wave, one move at a time
+    k.hooks = !b->hooks.empty() && b->hooks[c].on;
+    if (k.hooks) {
+      k.field = false;
+      k.waves = 1;
+      k.spec = 1;
+      k.kr = pick_kr(1);
+      if (k.kr > 0) {
+        k.glds = with_g <= gtab_budget;
+        k.lds = k.glds ? with_g : no_g;
+      } else {
+        const uint32_t rec_g = sweep_lds_bytes(occ_bytes, trk_sites, int(m->recs.size()), int(m->gtab.size()));
+        const uint32_t rec_only = sweep_lds_bytes(occ_bytes, trk_sites, int(m->recs.size()), 0);
+        k.glds = rec_g <= gtab_budget;
+        k.lds = k.glds ? rec_g : rec_only;
+        require(k.lds <= limit, "chain state does not fit in the workgroup's LDS");
+      }
+    }
     keys[c] = k;
   }
   for (int c0 = 0; c0 < n_chains; ++c0) {
@@ -1142,7 +1178,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
       const Key& k = keys[c];
       if (done[c] || k.periodic != k0.periodic || k.canonical != k0.canonical || k.kr != k0.kr ||
           k.glds != k0.glds || k.kbm != k0.kbm || k.waves != k0.waves || k.spec != k0.spec ||
-          k.field != k0.field)
+          k.field != k0.field || k.hooks != k0.hooks)
         continue;
       done[c] = true;
       ids.push_back(c);
@@ -1154,7 +1190,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
       return int64_t(mx->n_sites) * mx->recs.size() > int64_t(my->n_sites) * my->recs.size();
     });
     b->classes.push_back(cemc_batch::Class{k0.periodic, k0.canonical, k0.kr, k0.glds, k0.kbm, k0.waves, k0.spec,
-                                           k0.field, int(order.size()), int(ids.size()), lds});
+                                           k0.field, k0.hooks, int(order.size()), int(ids.size()), lds});
     order.insert(order.end(), ids.begin(), ids.end());
   }
   if (order.size() > b->order_cap || b->d_order == nullptr) {
@@ -1170,6 +1206,31 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
   uint32_t need = 0;
   for (const auto& cl : b->classes) need = std::max(need, cl.lds);
   if (need + sweep_static_lds() > 64u * 1024u) HIP_CHECK(sweep_set_max_lds(limit + sweep_static_lds()));
+}
+
+// Reads the chains' constraint state back after a launch (the kernel keeps
+// pair counts, bias counts and energies there); returns the reference's
+// error message for a chain whose move found no valid proposal
+// (montecarlo.py:986-989), clearing the flag.
+std::string hooks_after_launch(cemc_batch* b) {
+  if (b->d_hooks == nullptr) return std::string();
+  std::vector<HookDev> st(b->hooks.size());
+  HIP_CHECK(hipMemcpy(st.data(), b->d_hooks, sizeof(HookDev) * st.size(), hipMemcpyDeviceToHost));
+  std::string err;
+  for (size_t c = 0; c < st.size(); ++c) {
+    if (!b->hooks[c].on) continue;
+    b->hooks[c].dev = st[c];
+    if (st[c].status != 0 && err.empty())
+      err = "Did not manage to produce a trial move that does not violate any of the constraints (chain " +
+            std::to_string(c) + ", move " + std::to_string(st[c].failed_move) + ")";
+    b->hooks[c].dev.status = 0;
+  }
+  if (!err.empty()) {
+    std::vector<HookDev> up;
+    for (const auto& h : b->hooks) up.push_back(h.dev);
+    HIP_CHECK(hipMemcpy(b->d_hooks, up.data(), sizeof(HookDev) * up.size(), hipMemcpyHostToDevice));
+  }
+  return err;
 }
 
 // One sweep-kernel launch per class on the batch's stream; each chain
@@ -1211,6 +1272,7 @@ void enqueue_sweeps(cemc_batch* b, uint32_t* d_log, int64_t log_words) {
     a.waves = cl.waves;
     a.spec = cl.spec;
     a.field = cl.field;
+    a.hooks = cl.hooks ? b->d_hooks : nullptr;
     a.accept_log = d_log;
     a.log_words = log_words;
     hipStream_t st = fan_out ? b->side[ci % cemc_batch::kSideStreams] : b->stream;
@@ -1245,6 +1307,9 @@ void batch_launch(cemc_batch* b, const int64_t* n_moves, int64_t moves, bool tim
   enqueue_sweeps(b, d_log, log_words);
   if (timed) HIP_CHECK(hipEventRecord(b->ev1, b->stream));
   HIP_CHECK(hipStreamSynchronize(b->stream));
+  std::string hook_error = hooks_after_launch(b);
+  if (!hook_error.empty() && d_log) (void)hipFree(d_log);
+  if (!hook_error.empty()) throw std::runtime_error(hook_error);
   if (d_log) {
     const hipError_t e = hipMemcpy(host_log, d_log, sizeof(uint32_t) * size_t(b->n_chains) * log_words,
                                    hipMemcpyDeviceToHost);
@@ -1984,6 +2049,100 @@ int cemc_batch_set_temperature(cemc_batch* b, int32_t chain, double temperature)
     require(temperature > 0.0 && std::isfinite(temperature), "temperature must be positive");
     b->params[chain].kT = temperature * kBoltzmannEV;
     HIP_CHECK(hipMemcpy(&b->d_params[chain].kT, &b->params[chain].kT, sizeof(double), hipMemcpyHostToDevice));
+  });
+}
+
+// ------------------------------------------------------ constraints, bias
+int cemc_batch_set_hooks(cemc_batch* b, int32_t chain, const cemc_hooks_desc* d) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(!b->dynamic, "constraints and bias apply to fixed batches, not pools");
+    require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    const cemc_model* m = b->models[b->params[chain].model];
+    const int N = m->n_sites, S = m->n_species;
+    if (b->hooks.empty()) b->hooks.resize(size_t(b->n_chains));
+    cemc_batch::Hooks& h = b->hooks[chain];
+    // the chain's energy carries the bias of its state: take the old one out
+    ChainState st{};
+    HIP_CHECK(hipMemcpy(&st, b->d_states + chain, sizeof(ChainState), hipMemcpyDeviceToHost));
+    if (h.on) st.energy -= h.dev.bias_energy;
+    dev_free(h.d_allowed);
+    dev_free(h.d_nb);
+    dev_free(h.d_btab);
+    h = cemc_batch::Hooks{};
+    if (d != nullptr) {
+      require(d->max_attempts > 0, "Max. constraint attempts has to be > 0!");
+      require(d->fixed_species < (1u << S), "fixed species outside the model's species");
+      std::vector<int8_t> occ(static_cast<size_t>(N));
+      HIP_CHECK(hipMemcpy(occ.data(), b->d_occ + b->params[chain].occ_off, size_t(N), hipMemcpyDeviceToHost));
+      HookDev& v = h.dev;
+      v.fixed_species = d->fixed_species;
+      v.max_attempts = d->max_attempts;
+      v.pair_e1 = v.pair_e2 = -1;
+      v.bias_species = -1;
+      if (d->allowed) {
+        for (int i = 0; i < N; ++i) require((d->allowed[i] >> S) == 0, "allowed-species mask names unknown species");
+        h.d_allowed = upload(std::vector<uint8_t>(d->allowed, d->allowed + N));
+        v.allowed = h.d_allowed;
+      }
+      if (d->pair_species[0] >= 0 || d->pair_species[1] >= 0) {
+        const int e1 = d->pair_species[0], e2 = d->pair_species[1], nm = d->pair_members;
+        require(e1 >= 0 && e1 < S && e2 >= 0 && e2 < S, "pair constraint species outside the model's species");
+        require(nm > 0 && nm <= 32 && d->pair_neighbours != nullptr, "pair constraint needs 1-32 members per site");
+        std::vector<int32_t> nb(d->pair_neighbours, d->pair_neighbours + int64_t(N) * nm);
+        int64_t count = 0;
+        for (int i = 0; i < N; ++i)
+          for (int q = 0; q < nm; ++q) {
+            const int j = nb[size_t(i) * nm + q];
+            require(j >= 0 && j < N && j != i, "pair constraint member outside the lattice");
+            bool back = false;
+            for (int r = 0; r < nm && !back; ++r) back = nb[size_t(j) * nm + r] == i;
+            require(back, "pair constraint member lists must be symmetric (j a member of i => i of j)");
+            count += (occ[i] == e1 && occ[j] == e2) ? 1 : 0;
+          }
+        h.d_nb = upload(nb);
+        v.pair_e1 = e1;
+        v.pair_e2 = e2;
+        v.pair_nm = nm;
+        v.pair_nb = h.d_nb;
+        v.pair_count = count;
+      }
+      if (d->bias_species >= 0) {
+        require(d->bias_species < S, "bias species outside the model's species");
+        require(d->bias_table != nullptr, "bias needs a table of n_sites + 1 entries");
+        std::vector<double> tab(d->bias_table, d->bias_table + N + 1);
+        for (double x : tab) require(std::isfinite(x), "bias table entries must be finite");
+        int cnt = 0;
+        for (int i = 0; i < N; ++i) cnt += occ[i] == d->bias_species ? 1 : 0;
+        h.d_btab = upload(tab);
+        v.bias_species = d->bias_species;
+        v.bias_count = cnt;
+        v.bias_tab = h.d_btab;
+        v.bias_energy = tab[size_t(cnt)];
+        st.energy += v.bias_energy;
+      }
+      h.on = true;
+    }
+    HIP_CHECK(hipMemcpy(&b->d_states[chain].energy, &st.energy, sizeof(double), hipMemcpyHostToDevice));
+    if (b->d_hooks == nullptr) HIP_CHECK(hipMalloc(&b->d_hooks, sizeof(HookDev) * size_t(b->n_chains)));
+    std::vector<HookDev> up;
+    for (const auto& x : b->hooks) up.push_back(x.dev);
+    HIP_CHECK(hipMemcpy(b->d_hooks, up.data(), sizeof(HookDev) * up.size(), hipMemcpyHostToDevice));
+    derive_classes(b, b->forced_waves);
+  });
+}
+
+int cemc_batch_get_hooks_state(cemc_batch* b, int32_t chain, int64_t* pair_count, int32_t* bias_count,
+                               double* bias_energy) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    require(!b->hooks.empty() && b->hooks[chain].on, "the chain has no constraints or bias");
+    HookDev v{};
+    HIP_CHECK(hipMemcpy(&v, b->d_hooks + chain, sizeof(HookDev), hipMemcpyDeviceToHost));
+    if (pair_count) *pair_count = v.pair_count;
+    if (bias_count) *bias_count = v.bias_count;
+    if (bias_energy) *bias_energy = v.bias_energy;
   });
 }
 

@@ -27,6 +27,7 @@ struct LaunchSweep {
   int waves;       // wavefronts per chain: 1, 2, 4 or 8 (more than 1: kr > 0)
   int spec;        // moves per round: 1, or kSpecMoves (periodic, kr > 0)
   bool field;      // local-field kernel (periodic, one wave, kr > 0, kbm <= 3)
+  HookDev* hooks;  // per chain constraints and bias (one wave, one move per round), or nullptr
   uint32_t* accept_log;  // optional: bit t of chain c at accept_log[c * log_words + t / 32]
   int64_t log_words;
 };

@@ -386,7 +386,7 @@ __global__ __launch_bounds__(256) void pt_exchange_kernel(PTDev* __restrict__ pt
 // Launch wrappers.  The sweep kernel's instantiations are compiled in
 // separate objects (ce_sweep_inst.hip) and reached through these lookups.
 #define CEMC_DECLARE_SWEEP_FN(W, P) \
-  const void* sweep_fn_w##W##_p##P(bool canonical, int kr, bool glds, int kbm, int spec);
+  const void* sweep_fn_w##W##_p##P(bool canonical, int kr, bool glds, int kbm, int spec, bool hooks);
 CEMC_DECLARE_SWEEP_FN(1, 0)
 CEMC_DECLARE_SWEEP_FN(1, 1)
 CEMC_DECLARE_SWEEP_FN(2, 0)
@@ -405,12 +405,14 @@ static const void* wl_fn(bool periodic, int kr, bool glds, int kbm) {
   return periodic ? wl_fn_p1(kr, glds, kbm) : wl_fn_p0(kr, glds, kbm);
 }
 
-static const void* sweep_fn(bool periodic, bool canonical, int kr, bool glds, int kbm, int waves, int spec) {
+static const void* sweep_fn(bool periodic, bool canonical, int kr, bool glds, int kbm, int waves, int spec,
+                            bool hooks = false) {
+  const bool h = hooks;
   switch (waves) {
-    case 1: return periodic ? sweep_fn_w1_p1(canonical, kr, glds, kbm, spec) : sweep_fn_w1_p0(canonical, kr, glds, kbm, spec);
-    case 2: return periodic ? sweep_fn_w2_p1(canonical, kr, glds, kbm, spec) : sweep_fn_w2_p0(canonical, kr, glds, kbm, spec);
-    case 4: return periodic ? sweep_fn_w4_p1(canonical, kr, glds, kbm, spec) : sweep_fn_w4_p0(canonical, kr, glds, kbm, spec);
-    case 8: return periodic ? sweep_fn_w8_p1(canonical, kr, glds, kbm, spec) : sweep_fn_w8_p0(canonical, kr, glds, kbm, spec);
+    case 1: return periodic ? sweep_fn_w1_p1(canonical, kr, glds, kbm, spec, h) : sweep_fn_w1_p0(canonical, kr, glds, kbm, spec, h);
+    case 2: return periodic ? sweep_fn_w2_p1(canonical, kr, glds, kbm, spec, h) : sweep_fn_w2_p0(canonical, kr, glds, kbm, spec, h);
+    case 4: return periodic ? sweep_fn_w4_p1(canonical, kr, glds, kbm, spec, h) : sweep_fn_w4_p0(canonical, kr, glds, kbm, spec, h);
+    case 8: return periodic ? sweep_fn_w8_p1(canonical, kr, glds, kbm, spec, h) : sweep_fn_w8_p0(canonical, kr, glds, kbm, spec, h);
     default: return nullptr;
   }
 }
@@ -436,9 +438,11 @@ hipError_t sweep_set_max_lds(uint32_t bytes) {
         for (int kr : {0, 2, 4, 8})
           for (int g = 0; g < 2; ++g)
             for (int kbm : {1, 2, 3, 7})
-              for (int sp : {1, kSpecMoves}) {
+              for (int sp : {1, kSpecMoves})
+                for (int hk = 0; hk < 2; ++hk) {
               if (!sweep_variant_ok(kr, g != 0, kbm, w, sp, p != 0)) continue;
-              const void* fn = sweep_fn(p != 0, cn != 0, kr, g != 0, kbm, w, sp);
+              if (hk && (w != 1 || sp != 1)) continue;
+              const void* fn = sweep_fn(p != 0, cn != 0, kr, g != 0, kbm, w, sp, hk != 0);
               if (fn == nullptr) return hipErrorInvalidValue;
               hipFuncAttributes attr;
               hipError_t e = hipFuncGetAttributes(&attr, fn);
@@ -482,11 +486,12 @@ hipError_t sweep_set_max_lds(uint32_t bytes) {
 hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream) {
   const void* fn = nullptr;
   if (a.field) {
-    if (!a.periodic || a.waves != 1) return hipErrorInvalidValue;
+    if (!a.periodic || a.waves != 1 || a.hooks != nullptr) return hipErrorInvalidValue;
     fn = field_fn(a.canonical, a.kr, a.kbm);
   } else {
     if (!sweep_variant_ok(a.kr, a.gtab_lds, a.kbm, a.waves, a.spec, a.periodic)) return hipErrorInvalidValue;
-    fn = sweep_fn(a.periodic, a.canonical, a.kr, a.gtab_lds, a.kbm, a.waves, a.spec);
+    if (a.hooks != nullptr && (a.waves != 1 || a.spec != 1)) return hipErrorInvalidValue;
+    fn = sweep_fn(a.periodic, a.canonical, a.kr, a.gtab_lds, a.kbm, a.waves, a.spec, a.hooks != nullptr);
   }
   if (fn == nullptr) return hipErrorInvalidValue;
   const ChainParams* params = a.params;
@@ -498,7 +503,8 @@ hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream) {
   const int64_t* n_moves = a.n_moves;
   uint32_t* log = a.accept_log;
   int64_t log_words = a.log_words;
-  void* args[] = {&params, &states, &models, &occ_pool, &trk_pool, &order, &n_moves, &log, &log_words};
+  HookDev* hooks = a.hooks;
+  void* args[] = {&params, &states, &models, &occ_pool, &trk_pool, &order, &n_moves, &log, &log_words, &hooks};
   return hipLaunchKernel(fn, dim3(a.n_chains), dim3(64 * a.waves), args, a.lds_bytes, stream);
 }
 

@@ -292,6 +292,48 @@ __device__ __attribute__((always_inline)) inline bool cells_interact(uint32_t a,
   return max(max(ex, ey), ez) <= 2u * h && ((word >> (idx & 63u)) & 1u) != 0;
 }
 
+// A change of `site` from species o to n passes the per-species and
+// per-site constraints (FixedElement, mc_constraints.py:84-105;
+// ConstrainElementByTag and fixed sites, :171-190).
+__device__ __attribute__((always_inline)) inline bool hook_allows(uint32_t fixed, const uint8_t* allowed, int site,
+                                                                  uint32_t o, uint32_t n) {
+  if (((fixed >> o) | (fixed >> n)) & 1u) return false;
+  return allowed == nullptr || ((allowed[site] >> n) & 1u) != 0;
+}
+
+// Change of the number of ordered pairs (i, j), j a member of i in the
+// constrained pair cluster, with species e1 at i and e2 at j, when site s1
+// goes o1 -> n1 and (two) site s2 goes o2 -> n2.  Lanes 0-31 take the
+// members of s1, lanes 32-63 those of s2.  A member outside the changed
+// sites contributes both orientations; a pair of two changed sites is
+// counted once from each end (member lists are symmetric).  The reference
+// rescans the whole lattice for such a pair after every proposal
+// (pair_constraint.cpp:14-46); with the current state's count kept, the
+// proposal is valid when count + change is zero.
+template <bool kPeriodic>
+__device__ __attribute__((always_inline)) inline int64_t hook_pair_delta(
+    const SweepCtx& c, const uint8_t* occ, const int32_t* nb_tab, int nm, int e1, int e2, int s1, uint32_t o1,
+    uint32_t n1, int s2, uint32_t o2, uint32_t n2, bool two, int lane) {
+  const int q = lane & 31;
+  const bool second = lane >= 32;
+  int v = 0;
+  if (q < nm && (!second || two)) {
+    const int s = second ? s2 : s1;
+    const int os = int(second ? o2 : o1), ns = int(second ? n2 : n1);
+    const int nb = nb_tab[int64_t(s) * nm + q];
+    const bool is1 = nb == s1, is2 = two && nb == s2;
+    const int on = occ[ref_of<kPeriodic>(c, nb).p];
+    const int nn = is1 ? int(n1) : (is2 ? int(n2) : on);
+    const bool inside = is1 || is2;
+    const int before = int(os == e1 && on == e2) + int(!inside && on == e1 && os == e2);
+    const int after = int(ns == e1 && nn == e2) + int(!inside && nn == e1 && ns == e2);
+    v = after - before;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return int64_t(v);
+}
+
 // kR > 0: single-group model, records in VGPRs (kR per lane); kR == 0:
 // records in LDS.  kBM: decoration slices of the energy tables (1: two
 // species, 3: up to four, 7: up to eight).  kW: waves per chain.  With
@@ -299,7 +341,11 @@ __device__ __attribute__((always_inline)) inline bool cells_interact(uint32_t a,
 // from the same ordered sum, and every wave writes the same commit; one
 // barrier per move (inside group_sum) separates a move's commit from the
 // next move's gathers in other waves.
-template <bool kPeriodic, bool kCanonical, int kR, bool kGtabLds, int kBM, int kW, int kK>
+//
+// kHooks (one wave, one move at a time): the chain's constraints and bias
+// (HookDev) apply; each move's proposal is redrawn while it violates a
+// constraint.
+template <bool kPeriodic, bool kCanonical, int kR, bool kGtabLds, int kBM, int kW, int kK, bool kHooks = false>
 __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __restrict__ params,
                                                    ChainState* __restrict__ states,
                                                    const ModelDev* __restrict__ models,
@@ -307,7 +353,9 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
                                                    uint16_t* __restrict__ trk_pool,
                                                    const int32_t* __restrict__ chain_order,
                                                    const int64_t* __restrict__ n_moves,
-                                                   uint32_t* __restrict__ accept_log, int64_t log_words) {
+                                                   uint32_t* __restrict__ accept_log, int64_t log_words,
+                                                   HookDev* __restrict__ hooks) {
+  static_assert(!kHooks || (kW == 1 && kK == 1), "constraints and bias run one wave, one move at a time");
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ double dphi_tab[kMaxSpecies * kMaxSpecies * kMaxBasis];  // [(old*S+new)*B + d]
   __shared__ double dsingle_tab[kMaxSpecies * kMaxSpecies];          // singlet energy change per (old,new)
@@ -463,6 +511,32 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
     }
     run = 0;
   };
+  int64_t moves_made = nmov;
+  // constraints and bias of the chain (kHooks)
+  [[maybe_unused]] uint32_t hk_fixed = 0u;
+  [[maybe_unused]] const uint8_t* hk_allow = nullptr;
+  [[maybe_unused]] int hk_att = 1, hk_e1 = -1, hk_e2 = -1, hk_nm = 0, hk_bsp = -1, hk_bcnt = 0;
+  [[maybe_unused]] const int32_t* hk_nb = nullptr;
+  [[maybe_unused]] const double* hk_btab = nullptr;
+  [[maybe_unused]] int64_t hk_pairs = 0;
+  [[maybe_unused]] double hk_benergy = 0.0;
+  [[maybe_unused]] bool hk_failed = false;
+  [[maybe_unused]] uint64_t hk_fail_move = 0;
+  if constexpr (kHooks) {
+    const HookDev* hk = hooks + chain;
+    hk_fixed = hk->fixed_species;
+    hk_allow = hk->allowed;
+    hk_att = hk->max_attempts;
+    hk_e1 = hk->pair_e1;
+    hk_e2 = hk->pair_e2;
+    hk_nm = hk->pair_nm;
+    hk_nb = hk->pair_nb;
+    hk_pairs = hk->pair_count;
+    hk_bsp = hk->bias_species;
+    hk_bcnt = hk->bias_count;
+    hk_btab = hk->bias_tab;
+    hk_benergy = hk->bias_energy;
+  }
   if constexpr (kK == 1) {
   for (int64_t blk0 = 0; blk0 < nmov; blk0 += 64) {
     // lane l holds the Philox block and log u of move blk0 + l, and decodes
@@ -491,14 +565,78 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
       v_new = bounded32(uint32_t(blk.w0), uint32_t(c.S - 1));
     }
     uint64_t acc_bits = 0;
+    int l_end = n_in;  // moves of the block made (a failed constraint search ends the launch early)
     for (int l = 0; l < n_in; ++l) {
-      const double lu = readlane_f64(log_u, l);
+      double lu = readlane_f64(log_u, l);
+      uint64_t u_bits = 0;  // hooks: the uniform of the proposal that passed
 
       int si, sj = 0, ta = 0, tb = 0;
       uint32_t old_i = 0, new_i = 0;
       Ref ri, rj;
       uint32_t patch_j = 0u;
-      if constexpr (kCanonical) {
+      uint32_t img_h = 0u;  // hooks, SGC: image word of the flipped site
+      int64_t dpair = 0;    // hooks: change of the constrained pair count
+      int dcount = 0;       // hooks: change of the biased species count
+      if constexpr (kHooks) {
+        // proposal k of move t uses Philox counter (t, k, 0, 0); the first
+        // that violates no constraint is made (montecarlo.py:970-991: with
+        // max_attempts proposals none valid, the reference raises)
+        const uint64_t tmove = t0 + uint64_t(blk0) + uint64_t(l);
+        Philox4 pb{readlane_u64(blk.w0, l), readlane_u64(blk.w1, l), readlane_u64(blk.w2, l),
+                   readlane_u64(blk.w3, l)};
+        bool ok = false;
+        for (int k = 0;;) {
+          if constexpr (kCanonical) {
+            const uint32_t ia = bounded32(uint32_t(pb.w0 >> 32), np);
+            uint32_t ib = ia + 1u + bounded32(uint32_t(pb.w0), np - 1u);
+            ib -= ib >= np ? np : 0u;
+            old_i = (present_pk >> (4u * ia)) & 15u;
+            new_i = (present_pk >> (4u * ib)) & 15u;
+            ta = unpack16(seg_lo, seg_hi, ia) + int(bounded64(pb.w1, uint64_t(unpack16(cnt_lo, cnt_hi, ia))));
+            tb = unpack16(seg_lo, seg_hi, ib) + int(bounded64(pb.w2, uint64_t(unpack16(cnt_lo, cnt_hi, ib))));
+            si = __builtin_amdgcn_readfirstlane(trk[ta]);
+            sj = __builtin_amdgcn_readfirstlane(trk[tb]);
+            ok = hook_allows(hk_fixed, hk_allow, si, old_i, new_i) && hook_allows(hk_fixed, hk_allow, sj, new_i, old_i);
+            if (ok && hk_e1 >= 0) {
+              dpair = hook_pair_delta<kPeriodic>(c, occ, hk_nb, hk_nm, hk_e1, hk_e2, si, old_i, new_i, sj, new_i,
+                                                 old_i, true, lane);
+              ok = hk_pairs + dpair == 0;
+            }
+          } else {
+            const int pick = int(bounded64(pb.w1, uint64_t(n_pick)));
+            si = has_bkg ? m.movable[pick] : pick;
+            old_i = occ[ref_of<kPeriodic>(c, si).p];
+            old_i = __builtin_amdgcn_readfirstlane(old_i);
+            new_i = old_i + 1u + bounded32(uint32_t(pb.w0), uint32_t(c.S - 1));
+            new_i -= new_i >= uint32_t(c.S) ? uint32_t(c.S) : 0u;
+            ok = hook_allows(hk_fixed, hk_allow, si, old_i, new_i);
+            if (ok && hk_e1 >= 0) {
+              dpair = hook_pair_delta<kPeriodic>(c, occ, hk_nb, hk_nm, hk_e1, hk_e2, si, old_i, new_i, si, new_i,
+                                                 new_i, false, lane);
+              ok = hk_pairs + dpair == 0;
+            }
+          }
+          if (ok || ++k >= hk_att) break;
+          pb = philox4x64_10(tmove, uint64_t(k), 0, 0, seed, uid);
+        }
+        if (!ok) {
+          hk_failed = true;
+          hk_fail_move = tmove;
+          l_end = l;
+          break;
+        }
+        lu = log(unit_double(pb.w3));
+        u_bits = pb.w3;
+        ri = ref_of<kPeriodic>(c, si);
+        if constexpr (kCanonical) {
+          rj = ref_of<kPeriodic>(c, sj);
+          patch_j = kPeriodic ? wrapped_delta(c, ri, rj) : uint32_t(si);
+        } else {
+          rj = ri;
+          img_h = image_word<kPeriodic>(c, ri);
+          if (hk_bsp >= 0) dcount = int(new_i == uint32_t(hk_bsp)) - int(old_i == uint32_t(hk_bsp));
+        }
+      } else if constexpr (kCanonical) {
         old_i = __builtin_amdgcn_readlane(v_old, l);
         new_i = __builtin_amdgcn_readlane(v_new, l);
         ta = __builtin_amdgcn_readlane(v_ta, l);
@@ -516,13 +654,13 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         rj = ri;
       }
       int gi_i[kR > 0 ? kR : 1], gi_j[kR > 0 ? kR : 1];
-      if constexpr (!kCanonical) old_i = occ[ri.p];
+      if constexpr (!kCanonical && !kHooks) old_i = occ[ri.p];
       if constexpr (kR > 0) {
         // all member gathers of the move are issued together
         table_index<kPeriodic, kR, kBM, false>(c, rr, ri, occ, 0u, 0u, gi_i);
         if constexpr (kCanonical) table_index<kPeriodic, kR, kBM, true>(c, rr, rj, occ, patch_j, new_i, gi_j);
       }
-      if constexpr (!kCanonical) {
+      if constexpr (!kCanonical && !kHooks) {
         new_i = old_i + 1u + __builtin_amdgcn_readlane(v_new, l);
         new_i -= new_i >= uint32_t(c.S) ? uint32_t(c.S) : 0u;
       }
@@ -569,7 +707,13 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         }
       }
       const double sum = group_sum<kW>(part, red_move[l & 1], wave, lane);
-      const double dE = kBM == 1 ? fma(dphi[0], sum, de_single) : sum + de_single;
+      double dE = kBM == 1 ? fma(dphi[0], sum, de_single) : sum + de_single;
+      double dbias = 0.0;
+      if constexpr (kHooks) {
+        // the bias of the proposed state joins its energy (montecarlo.py:926-930)
+        if (dcount != 0) dbias = hk_btab[hk_bcnt + dcount] - hk_btab[hk_bcnt];
+        dE += dbias;
+      }
       // Metropolis test of the reference: accept if E_new < E_cur, else if
       // u <= exp(-(E_new - E_cur) / kT) (cemc/mcmc/montecarlo.py:950-956).  The
       // comparison is first made against log(u) with a margin far above the
@@ -580,9 +724,15 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
       int verdict = (dE < 0.0 || lu < x - margin) ? 1 : (lu > x + margin ? 0 : 2);
       verdict = __builtin_amdgcn_readfirstlane(verdict);
       if (verdict == 2)
-        verdict = __builtin_amdgcn_readfirstlane(int(unit_double(readlane_u64(blk.w3, l)) <= exp(-dE / kT)));
+        verdict = __builtin_amdgcn_readfirstlane(
+            int(unit_double(kHooks ? u_bits : readlane_u64(blk.w3, l)) <= exp(-dE / kT)));
       if (verdict) {
         flush();
+        if constexpr (kHooks) {
+          hk_pairs += dpair;
+          hk_bcnt += dcount;
+          hk_benergy += dbias;
+        }
         if constexpr (kPeriodic) {
           // lanes 0-7 write the images of site i, lanes 8-15 those of site j
           const bool second = kCanonical && lane >= 8;
@@ -590,7 +740,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
           if constexpr (kCanonical)
             img = second ? image_word<true>(c, rj) : image_word<true>(c, ri);
           else
-            img = __builtin_amdgcn_readlane(v_img, l);
+            img = kHooks ? img_h : __builtin_amdgcn_readlane(v_img, l);
           const int idx = image_of(c, img, lane & 7);
           if (idx >= 0 && lane < (kCanonical ? 16 : 8)) occ[idx] = uint8_t(second ? old_i : new_i);
         } else {
@@ -618,6 +768,12 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
       const int64_t w = blk0 >> 5;
       if (lane == 0) acc_log[w] = uint32_t(acc_bits);
       if (lane == 1 && n_in > 32) acc_log[w + 1] = uint32_t(acc_bits >> 32);
+    }
+    if constexpr (kHooks) {
+      if (hk_failed) {
+        moves_made = blk0 + l_end;
+        break;
+      }
     }
   }
     flush();
@@ -873,9 +1029,9 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
   ChainState* st = states + chain;
   if (tid == 0) {
     st->energy = energy;
-    st->moves_done = t0 + uint64_t(nmov);
+    st->moves_done = t0 + uint64_t(moves_made);
     st->n_accept += n_acc;
-    st->n_samples += uint64_t(nmov);
+    st->n_samples += uint64_t(moves_made);
     st->sum_de += sum_de;
     st->sum_de2 += sum_de2;
   }
@@ -883,6 +1039,18 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
     st->sum_s[lane] += sum_s;
     st->sum_s2[lane] += sum_s2;
     st->sum_se[lane] += sum_se;
+  }
+  if constexpr (kHooks) {
+    if (tid == 0) {
+      HookDev* hk = hooks + chain;
+      hk->pair_count = hk_pairs;
+      hk->bias_count = hk_bcnt;
+      hk->bias_energy = hk_benergy;
+      if (hk_failed) {
+        hk->status = 1;
+        hk->failed_move = hk_fail_move;
+      }
+    }
   }
 }
 

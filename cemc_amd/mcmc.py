@@ -33,11 +33,16 @@ from __future__ import annotations
 import numpy as np
 
 from .batch import CANONICAL, SGC, ChainBatch
+from .constraints import hooks_for
 from .lattice import KB_EV
 
 
 def _fresh_seed():
     return int(np.random.SeedSequence().generate_state(1, dtype=np.uint64)[0])
+
+
+class CanNotFindLegalMoveError(Exception):
+    """No proposal of a move passed the constraints (montecarlo.py:986-989)."""
 
 
 class DidNotReachEquillibriumError(Exception):
@@ -146,6 +151,9 @@ class Montecarlo:
         self._chem_pot_vec = None
         self.correlation_info = None
         self.converged = False
+        self.constraints = []
+        self.bias_potentials = []
+        self.max_allowed_constraint_pass_attempts = int(max_constraint_attempts)
         self.current_energy = calc.get_energy()
 
     # ------------------------------------------------------------ properties
@@ -176,6 +184,9 @@ class Montecarlo:
         self._batch = ChainBatch([self.calc.model], [chain])
         self._batch_occ = occ.copy()
         self._batch_eci = eci
+        if self.constraints or self.bias_potentials:
+            self._batch.set_hooks(0, **hooks_for(self.calc.BC, self.calc.model.species, self.constraints,
+                                                 self.bias_potentials, self.max_allowed_constraint_pass_attempts))
         return self._batch
 
     def _sync_back(self):
@@ -184,7 +195,36 @@ class Montecarlo:
         self.calc.updater.set_occupation(occ)
         self.calc.cf = self.calc.updater.get_cf()
         self._batch_occ = occ.copy()
-        self.current_energy = self.calc.get_energy()
+        self.current_energy = self.calc.get_energy() + self._bias_energy()
+
+    def _bias_energy(self):
+        """Bias of the current state; part of current_energy as in the
+        reference (montecarlo.py:279-282)."""
+        if not self.bias_potentials or self._batch is None:
+            return 0.0
+        return self._batch.hooks_state(0)[2]
+
+    def add_constraint(self, constraint):
+        """Add a constraint checked per trial move on the device
+        (montecarlo.py:386-392); see cemc_amd.constraints."""
+        if not hasattr(constraint, "device_hooks"):
+            raise TypeError("constraints must be cemc_amd.constraints classes (checked on the device)")
+        self.constraints.append(constraint)
+        self._drop_batch()
+
+    def add_bias(self, potential):
+        """Add a bias potential (montecarlo.py:394-401); see
+        cemc_amd.constraints.ConcentrationBias."""
+        if not hasattr(potential, "device_table"):
+            raise TypeError("potential has to be a cemc_amd.constraints bias potential")
+        self.bias_potentials.append(potential)
+        self._drop_batch()
+
+    def _drop_batch(self):
+        if self._batch is not None:
+            self._sync_back()
+            self._batch.close()
+            self._batch = None
 
     def _check_symbols(self):
         """Canonical swaps need two species (montecarlo.py:297-319)."""
@@ -210,7 +250,14 @@ class Montecarlo:
 
     def _advance(self, n):
         batch = self._ensure_batch()
-        batch.run(int(n))
+        try:
+            batch.run(int(n))
+        except RuntimeError as exc:
+            if "Did not manage to produce a trial move" not in str(exc):
+                raise
+            self._moves = int(batch.stats()["moves"][0])
+            self._sync_back()
+            raise CanNotFindLegalMoveError(str(exc)) from None
         self._moves += int(n)
         return batch
 
@@ -372,9 +419,10 @@ class SGCMonteCarlo(Montecarlo):
     kind = SGC
 
     def __init__(self, atoms, temp, indeces=None, symbols=None, logfile="", plot_debug=False, min_acc_rate=0.0,
-                 recycle_waste=False, seed=None, uid=0):
+                 recycle_waste=False, seed=None, uid=0, max_constraint_attempts=10000):
         Montecarlo.__init__(self, atoms, temp, indeces=indeces, logfile=logfile, plot_debug=plot_debug,
-                            min_acc_rate=min_acc_rate, recycle_waste=recycle_waste, seed=seed, uid=uid)
+                            min_acc_rate=min_acc_rate, recycle_waste=recycle_waste,
+                            max_constraint_attempts=max_constraint_attempts, seed=seed, uid=uid)
         if symbols is not None:
             self.symbols = list(symbols)
         if len(self.symbols) <= 1:
@@ -449,7 +497,7 @@ class SGCMonteCarlo(Montecarlo):
         self.calc.updater.set_occupation(occ)
         self.calc.cf = self.calc.updater.get_cf()
         self._batch_occ = occ.copy()
-        self.current_energy = self.calc.get_energy()
+        self.current_energy = self.calc.get_energy() + self._bias_energy()
 
     def singlet2composition(self, avg_singlets):
         """sgc_montecarlo.py:376-396."""

@@ -252,6 +252,37 @@ CEMC_API int cemc_batch_get_launch_shape(cemc_batch* batch, int32_t chain, int32
 /* Temperature of one chain (parallel tempering uses it to exchange replicas). */
 CEMC_API int cemc_batch_set_temperature(cemc_batch* batch, int32_t chain, double temperature);
 
+/* ------------------------------------------------- constraints and bias
+ * Per-move predicates and a bias on one chain of a fixed batch, checked on
+ * the device.  A trial move is redrawn while it violates a constraint
+ * (Montecarlo._mc_step, cemc/mcmc/montecarlo.py:970-991): proposal k of
+ * move t uses Philox counter (t, k, 0, 0), and a move whose first
+ * max_attempts proposals all violate ends the launch with CEMC_ERUNTIME and
+ * the reference's message.  The bias of the proposed state joins its energy
+ * (montecarlo.py:926-930, 998) and the chain's energy includes it.  Chains
+ * with hooks run one wave and one move at a time. */
+typedef struct cemc_hooks_desc {
+  uint32_t fixed_species;  /* FixedElement (mc_constraints.py:84-105): bit s set = species s never changes */
+  int32_t max_attempts;    /* max_constraint_attempts (> 0) */
+  const uint8_t* allowed;  /* n_sites entries, bit s = species s may occupy the site (ConstrainElementByTag,
+                              :171-190; a fixed site allows nothing, as FixEdgeLayers, :108-168); NULL: any */
+  int32_t pair_species[2]; /* PairConstraint (pair_constraint.cpp:14-46): no site of species [0] with a member
+                              of species [1] in the pair cluster; -1, -1: none */
+  int32_t pair_members;    /* members of the pair cluster per site, 1-32 */
+  const int32_t* pair_neighbours; /* n_sites x pair_members member sites; symmetric lists */
+  int32_t bias_species;    /* bias on the count of this species (a SampledBiasPotential over a concentration,
+                              bias_potential.py:52-140); -1: none */
+  int32_t reserved;
+  const double* bias_table; /* n_sites + 1 entries: bias energy (eV) at each count of bias_species */
+} cemc_hooks_desc;
+
+/* Set (desc != NULL) or remove (NULL) the constraints and bias of a chain. */
+CEMC_API int cemc_batch_set_hooks(cemc_batch* batch, int32_t chain, const cemc_hooks_desc* desc);
+/* Constraint state of a chain: ordered constrained pairs in its lattice,
+ * count of the biased species and the bias energy (any output may be NULL). */
+CEMC_API int cemc_batch_get_hooks_state(cemc_batch* batch, int32_t chain, int64_t* pair_count, int32_t* bias_count,
+                                        double* bias_energy);
+
 /* ----------------------------------------------------------- Wang-Landau
  * A Wang-Landau walker on a canonical chain of a (fixed) batch: the
  * reference's native sampler (cpp/src/wang_landau_sampler.cpp, bound by

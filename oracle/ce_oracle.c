@@ -429,9 +429,33 @@ static int canonical_setup(ceo_chain* c, int* present) {
   return n_present;
 }
 
-int ceo_run(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,
-            int64_t n_moves, double temperature, uint8_t* accept_bits, double* e_trace,
-            int64_t trace_every, int64_t* n_accept) {
+/* Constraint checks of the reference on a proposed change, restated
+ * literally: FixedElement (mc_constraints.py:84-105), ConstrainElementByTag
+ * (:171-190), and PairConstraint::elems_in_pair (pair_constraint.cpp:14-46),
+ * which applies the change to a copy of the symbols and scans every site of
+ * species e1 for a member of species e2. */
+static int hooks_valid(const ceo_chain* c, const ceo_hooks* h, int n_ch, const int* sites, const int* olds,
+                       const int* news, int8_t* scratch) {
+  const int N = c->m->n_sites;
+  for (int k = 0; k < n_ch; ++k) {
+    if (((h->fixed_species >> olds[k]) & 1u) || ((h->fixed_species >> news[k]) & 1u)) return 0;
+    if (h->allowed && !((h->allowed[sites[k]] >> news[k]) & 1u)) return 0;
+  }
+  if (h->pair_e1 >= 0) {
+    memcpy(scratch, c->occ, (size_t)N);
+    for (int k = 0; k < n_ch; ++k) scratch[sites[k]] = (int8_t)news[k];
+    for (int i = 0; i < N; ++i) {
+      if (scratch[i] != h->pair_e1) continue;
+      for (int q = 0; q < h->pair_nm; ++q)
+        if (scratch[h->pair_nb[(size_t)i * h->pair_nm + q]] == h->pair_e2) return 0;
+    }
+  }
+  return 1;
+}
+
+static int run_impl(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,
+                    int64_t n_moves, double temperature, uint8_t* accept_bits, double* e_trace,
+                    int64_t trace_every, int64_t* n_accept, ceo_hooks* h) {
   const ceo_model* m = c->m;
   const int N = m->n_sites;
   int present[64];
@@ -456,42 +480,66 @@ int ceo_run(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t 
       return fail(CEO_EINVAL, "SGC moves need two species and a movable site");
     }
   }
+  const int attempts = h ? h->max_attempts : 1;
+  int8_t* scratch = h ? (int8_t*)malloc((size_t)(N ? N : 1)) : NULL;
+  int bias_cnt = 0;
+  if (h && h->bias_species >= 0)
+    for (int i = 0; i < N; ++i) bias_cnt += c->occ[i] == h->bias_species;
+  if (h) h->failed_move = -1;
   const double kT = temperature * CEO_KB;
+  /* the current energy carries the bias of the state (montecarlo.py:279-282) */
   double e_cur = ceo_get_energy(c);
+  if (h && h->bias_species >= 0) e_cur += h->bias_table[bias_cnt];
   const uint64_t key[2] = {seed, chain_uid};
   for (int64_t t = 0; t < n_moves; ++t) {
-    const uint64_t ctr[4] = {move_begin + (uint64_t)t, 0, 0, 0};
-    uint64_t w[4];
-    ceo_philox4x64(ctr, key, w);
-    int sites[2], olds[2], news[2], n_ch;
+    int sites[2], olds[2], news[2], n_ch = 0;
     int sa = 0, sb = 0, a = 0, b = 0;
-    if (kind == 0) {
-      uint32_t ia = bounded32((uint32_t)(w[0] >> 32), (uint32_t)n_present);
-      uint32_t ib = (ia + 1 + bounded32((uint32_t)w[0], (uint32_t)(n_present - 1))) % (uint32_t)n_present;
-      a = present[ia];
-      b = present[ib];
-      sa = (int)bounded64(w[1], (uint64_t)count[a]);
-      sb = (int)bounded64(w[2], (uint64_t)count[b]);
-      sites[0] = tracker[trk_off[a] + sa];
-      sites[1] = tracker[trk_off[b] + sb];
-      olds[0] = a;
-      news[0] = b;
-      olds[1] = b;
-      news[1] = a;
-      n_ch = 2;
-    } else {
-      int site = movable[bounded64(w[1], (uint64_t)n_movable)];
-      int old = c->occ[site];
-      int nw = (int)((old + 1 + (int)bounded32((uint32_t)w[0], (uint32_t)(m->n_species - 1))) % m->n_species);
-      sites[0] = site;
-      olds[0] = old;
-      news[0] = nw;
-      n_ch = 1;
+    uint64_t w[4];
+    int ok = 0;
+    /* proposal k of move t: counter (t, k, 0, 0); the first valid one is
+     * made (montecarlo.py:970-991) */
+    for (int k = 0; k < attempts && !ok; ++k) {
+      const uint64_t ctr[4] = {move_begin + (uint64_t)t, (uint64_t)k, 0, 0};
+      ceo_philox4x64(ctr, key, w);
+      if (kind == 0) {
+        uint32_t ia = bounded32((uint32_t)(w[0] >> 32), (uint32_t)n_present);
+        uint32_t ib = (ia + 1 + bounded32((uint32_t)w[0], (uint32_t)(n_present - 1))) % (uint32_t)n_present;
+        a = present[ia];
+        b = present[ib];
+        sa = (int)bounded64(w[1], (uint64_t)count[a]);
+        sb = (int)bounded64(w[2], (uint64_t)count[b]);
+        sites[0] = tracker[trk_off[a] + sa];
+        sites[1] = tracker[trk_off[b] + sb];
+        olds[0] = a;
+        news[0] = b;
+        olds[1] = b;
+        news[1] = a;
+        n_ch = 2;
+      } else {
+        int site = movable[bounded64(w[1], (uint64_t)n_movable)];
+        int old = c->occ[site];
+        int nw = (int)((old + 1 + (int)bounded32((uint32_t)w[0], (uint32_t)(m->n_species - 1))) % m->n_species);
+        sites[0] = site;
+        olds[0] = old;
+        news[0] = nw;
+        n_ch = 1;
+      }
+      ok = h ? hooks_valid(c, h, n_ch, sites, olds, news, scratch) : 1;
+    }
+    if (!ok) {
+      h->failed_move = (int64_t)(move_begin + (uint64_t)t);
+      rc = fail(CEO_ERUNTIME, "Did not manage to produce a trial move that does not violate any of the constraints");
+      break;
     }
     const double u = unit_double(w[3]);
     double e_new = 0.0;
     rc = calculate_impl(c, n_ch, sites, olds, news, &e_new);
     if (rc != CEO_OK) break;
+    int dcnt = 0;
+    if (h && h->bias_species >= 0) {
+      for (int k = 0; k < n_ch; ++k) dcnt += (news[k] == h->bias_species) - (olds[k] == h->bias_species);
+      e_new += h->bias_table[bias_cnt + dcnt];
+    }
     int accept;
     if (e_new < e_cur) {
       accept = 1;
@@ -503,6 +551,7 @@ int ceo_run(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t 
     if (accept) {
       ceo_clear_history(c);
       e_cur = e_new;
+      bias_cnt += dcnt;
       acc += 1;
       if (kind == 0) {
         tracker[trk_off[a] + sa] = sites[1];
@@ -515,8 +564,24 @@ int ceo_run(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t 
     if (e_trace && trace_every > 0 && ((t + 1) % trace_every) == 0) e_trace[(t + 1) / trace_every - 1] = e_cur;
   }
   free(movable);
+  free(scratch);
   if (n_accept) *n_accept = acc;
   return rc;
+}
+
+int ceo_run(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,
+            int64_t n_moves, double temperature, uint8_t* accept_bits, double* e_trace,
+            int64_t trace_every, int64_t* n_accept) {
+  return run_impl(c, kind, seed, chain_uid, move_begin, n_moves, temperature, accept_bits, e_trace, trace_every,
+                  n_accept, NULL);
+}
+
+int ceo_run_hooked(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,
+                   int64_t n_moves, double temperature, uint8_t* accept_bits, double* e_trace,
+                   int64_t trace_every, int64_t* n_accept, ceo_hooks* hooks) {
+  if (!hooks || hooks->max_attempts <= 0) return fail(CEO_EINVAL, "Max. constraint attempts has to be > 0!");
+  return run_impl(c, kind, seed, chain_uid, move_begin, n_moves, temperature, accept_bits, e_trace, trace_every,
+                  n_accept, hooks);
 }
 
 int ceo_run_many(ceo_chain** chains, int n_chains, int kind, uint64_t seed,

@@ -79,6 +79,28 @@ int ceo_run(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t 
             int64_t n_moves, double temperature, uint8_t* accept_bits, double* e_trace,
             int64_t trace_every, int64_t* n_accept);
 
+/* Constraints and bias of a run (cemc/mcmc/mc_constraints.py,
+ * cpp/src/pair_constraint.cpp, cemc/mcmc/bias_potential.py).  Proposal k of
+ * move t uses counter (t, k, 0, 0); the first of max_attempts proposals
+ * that passes every constraint is made, else the run stops with CEO_ERUNTIME
+ * and failed_move set.  The bias at the count of bias_species joins the
+ * energy. */
+typedef struct ceo_hooks {
+  uint32_t fixed_species;   /* bit s: species s never changes */
+  int32_t max_attempts;
+  const uint8_t* allowed;   /* n_sites masks of species allowed per site, or NULL */
+  int32_t pair_e1, pair_e2; /* -1: no pair constraint */
+  int32_t pair_nm;
+  const int32_t* pair_nb;   /* n_sites x pair_nm member sites */
+  int32_t bias_species;     /* -1: no bias */
+  const double* bias_table; /* n_sites + 1 entries */
+  int64_t failed_move;      /* out: -1 or the Philox counter of the failed move */
+} ceo_hooks;
+
+int ceo_run_hooked(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,
+                   int64_t n_moves, double temperature, uint8_t* accept_bits, double* e_trace,
+                   int64_t trace_every, int64_t* n_accept, ceo_hooks* hooks);
+
 /* Run many independent chains on OpenMP threads (one chain per thread at a
  * time), each over the same number of moves. */
 int ceo_run_many(ceo_chain** chains, int n_chains, int kind, uint64_t seed,

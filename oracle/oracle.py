@@ -64,10 +64,22 @@ def load_library():
     lib.ceo_run_many.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                  ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64, ctypes.c_int64,
                                  dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+    lib.ceo_run_hooked.argtypes = [P, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                   ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_uint8), dp,
+                                   ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(Hooks)]
     lib.ceo_wl_run.argtypes = [P, ctypes.POINTER(WLState), ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint8)]
     _lib = lib
     return lib
+
+
+class Hooks(ctypes.Structure):
+    """ceo_hooks of ce_oracle.h."""
+    _fields_ = [("fixed_species", ctypes.c_uint32), ("max_attempts", ctypes.c_int32),
+                ("allowed", ctypes.POINTER(ctypes.c_uint8)), ("pair_e1", ctypes.c_int32),
+                ("pair_e2", ctypes.c_int32), ("pair_nm", ctypes.c_int32), ("pair_nb", ctypes.POINTER(ctypes.c_int32)),
+                ("bias_species", ctypes.c_int32), ("bias_table", ctypes.POINTER(ctypes.c_double)),
+                ("failed_move", ctypes.c_int64)]
 
 
 class WLState(ctypes.Structure):
@@ -347,6 +359,45 @@ def wl_run(updater, wl, seed, chain_uid, n_checks, check_every, move_begin=0):
     if rc:
         _raise(lib, rc)
     return bits
+
+
+def run_hooked(updater, kind, seed, chain_uid, n_moves, temperature, fixed_species=0, max_attempts=10000,
+               allowed=None, pair=None, pair_neighbours=None, bias_species=-1, bias_table=None, move_begin=0,
+               trace_every=0):
+    """Fed-stream run with constraints and bias; returns (accept bits, energy
+    trace, n_accept, failed_move or None).  pair: (e1, e2) species ids with
+    pair_neighbours an (n_sites, n_members) array."""
+    lib = load_library()
+    keep = []
+    h = Hooks()
+    h.fixed_species = int(fixed_species)
+    h.max_attempts = int(max_attempts)
+    if allowed is not None:
+        a = np.ascontiguousarray(allowed, dtype=np.uint8)
+        keep.append(a)
+        h.allowed = a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    h.pair_e1, h.pair_e2 = (-1, -1) if pair is None else (int(pair[0]), int(pair[1]))
+    if pair is not None:
+        nb = np.ascontiguousarray(pair_neighbours, dtype=np.int32)
+        keep.append(nb)
+        h.pair_nm = nb.shape[1]
+        h.pair_nb = nb.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    h.bias_species = int(bias_species)
+    if bias_species >= 0:
+        bt = np.ascontiguousarray(bias_table, dtype=np.float64)
+        keep.append(bt)
+        h.bias_table = _dp(bt)
+    bits = np.zeros((n_moves + 7) // 8 + 1, dtype=np.uint8)
+    n_tr = n_moves // trace_every if trace_every else 0
+    trace = np.zeros(max(n_tr, 1))
+    acc = ctypes.c_int64(0)
+    rc = lib.ceo_run_hooked(updater.handle, int(kind), int(seed), int(chain_uid), int(move_begin), int(n_moves),
+                            float(temperature), bits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _dp(trace),
+                            int(trace_every), ctypes.byref(acc), ctypes.byref(h))
+    failed = None if h.failed_move < 0 else int(h.failed_move)
+    if rc and failed is None:
+        _raise(lib, rc)
+    return bits, trace[:n_tr], acc.value, failed
 
 
 def run_many(updaters, kind, seed, chain_uids, n_moves, temperatures, n_threads, move_begin=0):
