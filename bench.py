@@ -146,6 +146,26 @@ def tempering(n_rep=128, cycles=2000, L=32, seed=9):
             "exchange_accept_rate": float(acc)}
 
 
+def wang_landau(model, settings, n_walkers=1024, n_checks=4, check_every=5000):
+    """Wang-Landau walkers on the binary FCC 10^3 model: one canonical walker
+    per wave, 100 bins over +-2 eV around each start, f = 1."""
+    from cemc_amd.batch import CANONICAL, ChainBatch
+    from cemc_amd.lattice import random_occupation
+    chains = [dict(kind=CANONICAL, temperature=1000.0, occupation=random_occupation(settings.n_sites, 2, 5000 + c),
+                   seed=91, uid=c) for c in range(n_walkers)]
+    b = ChainBatch([model] * n_walkers, chains)
+    e = b.energies()
+    for c in range(n_walkers):
+        b.wl_set(c, 100, e[c] - 2.0, e[c] + 2.0, 1.0, 1e-8, 0.8, current_bin=50)
+    b.wl_run(1, 1000)
+    _, ms = b.wl_run(n_checks, check_every, timed=True)
+    moves = n_walkers * n_checks * check_every
+    w0 = b.wl_get(0)
+    b.close()
+    return {"value": moves / (ms * 1e-3), "unit": "swaps/s", "kernel_ms": ms, "walkers": n_walkers,
+            "moves": moves, "bins": 100, "acceptance_walker0": w0["n_accept"] / max(w0["n_moves"], 1)}
+
+
 def serving(n_jobs=5000, rate=200.0):
     """Poisson stream of MC jobs (binary or ternary FCC, L^3 with L in 6..24,
     canonical or SGC, 100-5000 sweeps) through the continuous-batching job
@@ -249,8 +269,8 @@ def main():
     ap.add_argument("--sweeps", type=int, default=500)
     ap.add_argument("--chains", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workloads", default="batched,ternary,tempering,serving",
-                    help="comma list of batched, ternary, tempering, serving")
+    ap.add_argument("--workloads", default="batched,ternary,tempering,serving,wang_landau",
+                    help="comma list of batched, ternary, tempering, serving, wang_landau")
     ap.add_argument("--shape", default=None,
                     help="'fields' (per-site fields, one wave per chain) or WxK (waves per chain, moves per round, "
                          "tables summed per move) for the batched workload; skips the autotuner")
@@ -343,6 +363,10 @@ def main():
         out["parallel_tempering"] = tempering(cycles=args.pt_cycles)
         log("tempering: %.3e moves/s (kernel %.1f ms)" % (out["parallel_tempering"]["value"],
                                                           out["parallel_tempering"]["kernel_ms"]))
+    if "wang_landau" in work:
+        out["wang_landau"] = wang_landau(model, settings)
+        log("wang-landau: %.3e swaps/s (kernel %.1f ms)" % (out["wang_landau"]["value"],
+                                                             out["wang_landau"]["kernel_ms"]))
     if "serving" in work:
         out["serving"] = serving(n_jobs=args.serving_jobs)
         log("serving: %d jobs, p50 %.2f s, p99 %.2f s, %.3e moves/s" % (

@@ -1,4 +1,5 @@
-"""Cluster-expansion settings for periodic FCC supercells.
+"""Cluster-expansion settings for periodic supercells of the cubic Bravais
+lattices (FCC, BCC, SC).
 
 This module builds, from lattice geometry alone, the settings object that the
 reference updater consumes at initialisation (the attribute contract read by
@@ -17,8 +18,9 @@ reference updater consumes at initialisation (the attribute contract read by
 In the reference these come from the external ``ase.clease`` package, which is
 not part of the reference tree.  Here they are derived directly:
 
-* sites of an ``L1 x L2 x L3`` supercell of the FCC primitive cell, with
-  integer coordinates in units of a/2 (a = cubic lattice constant);
+* sites of an ``L1 x L2 x L3`` supercell of the primitive cell of the FCC,
+  BCC or SC lattice, with integer coordinates in units of a/2 (a = cubic
+  lattice constant);
 * clusters containing the reference site, grouped into orbits under the 48
   operations of the cubic point group plus lattice translations;
 * a canonical site order per cluster that does not depend on which member is
@@ -38,8 +40,11 @@ import math
 
 import numpy as np
 
-# FCC primitive vectors in units of a/2 (rows).
+# Primitive vectors (rows) of the cubic Bravais lattices in units of a/2.
 FCC_PRIM = np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]], dtype=np.int64)
+BCC_PRIM = np.array([[-1, 1, 1], [1, -1, 1], [1, 1, -1]], dtype=np.int64)
+SC_PRIM = np.array([[2, 0, 0], [0, 2, 0], [0, 0, 2]], dtype=np.int64)
+PRIMITIVE = {"fcc": FCC_PRIM, "bcc": BCC_PRIM, "sc": SC_PRIM}
 
 # Boltzmann constant in eV/K (CODATA 2014, the value ase.units.kB carried when
 # the reference was written).
@@ -72,12 +77,24 @@ def _encode(points):
 _CODE_ZERO = _encode(np.zeros(3, dtype=np.int64))
 
 
-def fcc_points_within(d2_max):
-    """FCC lattice points (a/2 units) with 0 < |p|^2 <= d2_max."""
+def _on_lattice(pts, lattice):
+    """Mask of integer points (a/2 units) that are sites of the lattice: FCC
+    even coordinate sum, BCC coordinates of one parity, SC even coordinates."""
+    if lattice == "fcc":
+        return (pts.sum(axis=1) % 2) == 0
+    if lattice == "bcc":
+        return ((pts[:, 0] - pts[:, 1]) % 2 == 0) & ((pts[:, 1] - pts[:, 2]) % 2 == 0)
+    if lattice == "sc":
+        return np.all(pts % 2 == 0, axis=1)
+    raise ValueError("unknown lattice %r (fcc, bcc or sc)" % (lattice,))
+
+
+def lattice_points_within(d2_max, lattice="fcc"):
+    """Lattice points (a/2 units) with 0 < |p|^2 <= d2_max, by (|p|^2, x, y, z)."""
     r = int(math.isqrt(int(d2_max))) + 1
     g = np.arange(-r, r + 1)
     pts = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T
-    pts = pts[(pts.sum(axis=1) % 2) == 0]
+    pts = pts[_on_lattice(pts, lattice)]
     d2 = (pts * pts).sum(axis=1)
     keep = (d2 > 0) & (d2 <= d2_max)
     pts = pts[keep]
@@ -85,18 +102,28 @@ def fcc_points_within(d2_max):
     return pts[order]
 
 
-def fcc_shells(n_shells):
-    """Squared radii (a/2 units) of the first ``n_shells`` FCC neighbour shells."""
-    pts = fcc_points_within(8 * n_shells + 8)
+def fcc_points_within(d2_max):
+    """FCC lattice points (a/2 units) with 0 < |p|^2 <= d2_max."""
+    return lattice_points_within(d2_max, "fcc")
+
+
+def lattice_shells(n_shells, lattice="fcc"):
+    """Squared radii (a/2 units) of the first ``n_shells`` neighbour shells."""
+    pts = lattice_points_within(12 * n_shells + 12, lattice)
     d2 = np.unique((pts * pts).sum(axis=1))
     return [int(v) for v in d2[:n_shells]]
 
 
-def _clusters_with_origin(size, d2_cut):
+def fcc_shells(n_shells):
+    """Squared radii (a/2 units) of the first ``n_shells`` FCC neighbour shells."""
+    return lattice_shells(n_shells, "fcc")
+
+
+def _clusters_with_origin(size, d2_cut, lattice="fcc"):
     """All clusters of ``size`` sites containing the origin whose pairwise squared
     distances are all <= d2_cut.  Returns an int array (n_clusters, size, 3) with
     the origin first."""
-    cand = fcc_points_within(d2_cut)
+    cand = lattice_points_within(d2_cut, lattice)
     n_c = len(cand)
     diff = cand[:, None, :] - cand[None, :, :]
     ok = (diff * diff).sum(axis=2) <= d2_cut
@@ -195,8 +222,8 @@ class Orbit:
         return len(self.offsets)
 
 
-def enumerate_orbits(max_shell_by_size):
-    """Cluster orbits of the FCC lattice.
+def enumerate_orbits(max_shell_by_size, lattice="fcc"):
+    """Cluster orbits of a cubic Bravais lattice (default FCC).
 
     ``max_shell_by_size`` maps cluster size (2, 3 or 4) to the outermost
     neighbour shell allowed for every pair of sites in the cluster.
@@ -206,13 +233,13 @@ def enumerate_orbits(max_shell_by_size):
     if not max_shell_by_size:
         return []
     n_sh = max(max_shell_by_size.values())
-    shells = fcc_shells(n_sh)
+    shells = lattice_shells(n_sh, lattice)
     orbits = []
     for size in sorted(max_shell_by_size):
         if size not in (2, 3, 4):
             raise ValueError("Only cluster sizes 2, 3 and 4 are supported")
         d2_cut = shells[max_shell_by_size[size] - 1]
-        cl = _clusters_with_origin(size, d2_cut)
+        cl = _clusters_with_origin(size, d2_cut, lattice)
         if len(cl) == 0:
             continue
         best, canon_rank, stab, ranks = _canonical(cl)
@@ -302,13 +329,14 @@ class _TransMatrixRows:
         return [self[i] for i in range(len(self))]
 
 
-class FCCSettings:
-    """Settings of a periodic FCC supercell cluster expansion.
+class CubicSettings:
+    """Settings of a periodic supercell cluster expansion on a cubic Bravais
+    lattice.
 
     Parameters
     ----------
     size : int or (int, int, int)
-        Supercell repetitions of the FCC primitive cell.
+        Supercell repetitions of the primitive cell.
     species : list[str]
         Chemical species; ids follow sorted-name order as in the reference
         (cpp/src/symbols_with_numbers.cpp:11-21).
@@ -318,11 +346,23 @@ class FCCSettings:
         Emit ``equiv_sites`` so decorations related by cluster symmetry share
         one correlation function.  With ``False`` every decoration is its own
         correlation function and ``equiv_sites`` is empty.
+    lattice : str
+        "fcc", "bcc" or "sc".
     """
 
-    def __init__(self, size, species, max_shell_by_size, symmetrize_decorations=True):
+    classtype = "CubicSettings"
+
+    def __init__(self, size, species, max_shell_by_size, symmetrize_decorations=True, lattice="fcc"):
         if isinstance(size, int):
             size = (size, size, size)
+        lattice = str(lattice).lower()
+        if lattice not in PRIMITIVE:
+            raise ValueError("unknown lattice %r (fcc, bcc or sc)" % (lattice,))
+        self.lattice = lattice
+        self.prim = PRIMITIVE[lattice]
+        # integer inverse of the primitive matrix: cell = point @ adj / det
+        self._det = int(round(np.linalg.det(self.prim)))
+        self._adj = np.rint(np.linalg.inv(self.prim) * self._det).astype(np.int64)
         self.dims = tuple(int(v) for v in size)
         self.species = sorted(species)
         self.unique_elements = list(self.species)
@@ -336,8 +376,8 @@ class FCCSettings:
         # arguments that rebuild these settings (CE.save / CE.load)
         self.kwargs = {"size": list(self.dims), "species": list(self.species),
                        "max_shell_by_size": {str(k): int(v) for k, v in dict(max_shell_by_size).items()},
-                       "symmetrize_decorations": self.symmetrize_decorations}
-        self.orbits = enumerate_orbits(dict(max_shell_by_size))
+                       "symmetrize_decorations": self.symmetrize_decorations, "lattice": self.lattice}
+        self.orbits = enumerate_orbits(dict(max_shell_by_size), self.lattice)
         phi = polynomial_basis(len(self.species))
         self.basis = phi
         self.basis_functions = [
@@ -358,12 +398,14 @@ class FCCSettings:
         L1, L2, L3 = self.dims
         n = np.arange(self.n_sites)
         self.cells = np.stack([n // (L2 * L3), (n // L3) % L2, n % L3], axis=1)
-        self.positions = self.cells @ FCC_PRIM  # a/2 units
+        self.positions = self.cells @ self.prim  # a/2 units
 
-    @staticmethod
-    def _to_cell(points):
-        x, y, z = points[..., 0], points[..., 1], points[..., 2]
-        return np.stack([(y + z - x) // 2, (x + z - y) // 2, (x + y - z) // 2], axis=-1)
+    def _to_cell(self, points):
+        """Integer cell coordinates of lattice points (a/2 units)."""
+        num = np.asarray(points, dtype=np.int64) @ self._adj
+        if np.any(num % self._det):
+            raise ValueError("point is not a lattice site")
+        return num // self._det
 
     # -------------------------------------------------------- cluster info
     def _build_cluster_info(self):
@@ -450,6 +492,34 @@ class FCCSettings:
         return {"dims": self.dims, "cols": self.trans_cols, "col_cells": self.trans_col_cells}
 
 
+class FCCSettings(CubicSettings):
+    """Settings of a periodic FCC supercell (CLEASE ``CEBulk`` with
+    ``crystalstructure="fcc"``)."""
+
+    classtype = "FCCSettings"
+
+    def __init__(self, size, species, max_shell_by_size, symmetrize_decorations=True):
+        super().__init__(size, species, max_shell_by_size, symmetrize_decorations, "fcc")
+
+
+class BCCSettings(CubicSettings):
+    """Settings of a periodic BCC supercell (``crystalstructure="bcc"``)."""
+
+    classtype = "BCCSettings"
+
+    def __init__(self, size, species, max_shell_by_size, symmetrize_decorations=True):
+        super().__init__(size, species, max_shell_by_size, symmetrize_decorations, "bcc")
+
+
+class SCSettings(CubicSettings):
+    """Settings of a periodic simple-cubic supercell (``crystalstructure="sc"``)."""
+
+    classtype = "SCSettings"
+
+    def __init__(self, size, species, max_shell_by_size, symmetrize_decorations=True):
+        super().__init__(size, species, max_shell_by_size, symmetrize_decorations, "sc")
+
+
 def seeded_ecis(settings, seed, sigma, scale_by_diameter=True):
     """ECIs drawn from N(0, sigma) in sorted-name order from a Philox stream and
     scaled by 1/diameter^2 for clusters of two or more sites."""
@@ -513,23 +583,32 @@ class SimpleAtoms:
         return SimpleAtoms(self.get_chemical_symbols())
 
 
-def fcc_shells_within(diameter, a):
-    """Number of FCC neighbour shells whose radius is at most ``diameter``
-    for lattice constant ``a`` (shell radii sqrt(d2) * a / 2)."""
+def shells_within(diameter, a, lattice="fcc"):
+    """Number of neighbour shells of the lattice whose radius is at most
+    ``diameter`` for lattice constant ``a`` (shell radii sqrt(d2) * a / 2)."""
     n = 1
     while True:
-        sh = fcc_shells(n + 1)
+        sh = lattice_shells(n + 1, lattice)
         if len(sh) <= n or np.sqrt(sh[n]) * a / 2.0 > diameter + 1e-9:
             break
         n += 1
-    return n if np.sqrt(fcc_shells(1)[0]) * a / 2.0 <= diameter + 1e-9 else 0
+    return n if np.sqrt(lattice_shells(1, lattice)[0]) * a / 2.0 <= diameter + 1e-9 else 0
+
+
+def fcc_shells_within(diameter, a):
+    """Number of FCC neighbour shells whose radius is at most ``diameter``."""
+    return shells_within(diameter, a, "fcc")
+
+
+_BY_NAME = {"fcc": FCCSettings, "bcc": BCCSettings, "sc": SCSettings}
 
 
 def settings_from_kwargs(kwargs):
     """Settings from the ``setting_kwargs`` of a saved calculator.
 
-    ``classtype`` "FCCSettings": this module's own arguments.  ``classtype``
-    "CEBulk" with ``crystalstructure`` "fcc" (the reference's CLEASE bulk
+    ``classtype`` "FCCSettings", "BCCSettings", "SCSettings" or
+    "CubicSettings": this module's own arguments.  ``classtype`` "CEBulk" with
+    ``crystalstructure`` "fcc", "bcc" or "sc" (the reference's CLEASE bulk
     settings, ce_calculator.py:577-592): ``size`` is the repetition of the
     primitive cell, the species come from ``basis_elements`` (or
     ``concentration["basis_elements"]``), and each cluster size up to
@@ -539,16 +618,19 @@ def settings_from_kwargs(kwargs):
     entries is indexed by size).  Other CLEASE settings are not available."""
     kw = dict(kwargs)
     classtype = kw.pop("classtype", "FCCSettings")
-    if classtype == "FCCSettings":
+    if classtype in ("FCCSettings", "BCCSettings", "SCSettings", "CubicSettings"):
         shells = {int(k): int(v) for k, v in kw["max_shell_by_size"].items()}
-        return FCCSettings(kw["size"], kw["species"], shells, kw.get("symmetrize_decorations", True))
-    if classtype != "CEBulk" or str(kw.get("crystalstructure", "")).lower() != "fcc":
-        raise ValueError("Unknown setting classtype: {} (only FCC bulk settings are available)".format(kwargs))
+        lattice = kw.get("lattice", {"FCCSettings": "fcc", "BCCSettings": "bcc", "SCSettings": "sc"}.get(classtype))
+        return _BY_NAME[str(lattice).lower()](kw["size"], kw["species"], shells, kw.get("symmetrize_decorations", True))
+    lattice = str(kw.get("crystalstructure", "")).lower()
+    if classtype != "CEBulk" or lattice not in _BY_NAME:
+        raise ValueError("Unknown setting classtype: {} (only fcc, bcc and sc bulk settings are available)"
+                         .format(kwargs))
     elements = kw.get("basis_elements")
     if elements is None and isinstance(kw.get("concentration"), dict):
         elements = kw["concentration"].get("basis_elements")
     if not elements or len(elements) != 1:
-        raise ValueError("FCC bulk settings need one basis with its elements")
+        raise ValueError("Cubic bulk settings need one basis with its elements")
     size = kw["size"]
     if isinstance(size, int):
         size = [size] * 3
@@ -565,6 +647,6 @@ def settings_from_kwargs(kwargs):
         dias = {n: float(dia[n - 2]) for n in range(2, max_size + 1)}
     else:
         raise ValueError("max_cluster_dia must be a number or one value per cluster size")
-    shells = {n: fcc_shells_within(d, a) for n, d in dias.items()}
+    shells = {n: shells_within(d, a, lattice) for n, d in dias.items()}
     shells = {n: k for n, k in shells.items() if k > 0}
-    return FCCSettings(list(size), list(elements[0]), shells)
+    return _BY_NAME[lattice](list(size), list(elements[0]), shells)

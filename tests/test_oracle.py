@@ -19,7 +19,7 @@ import math
 import numpy as np
 import pytest
 
-from cemc_amd.lattice import FCCSettings, FCC_PRIM, KB_EV, random_occupation, seeded_ecis
+from cemc_amd.lattice import CubicSettings, FCCSettings, FCC_PRIM, KB_EV, random_occupation, seeded_ecis
 from golden_io import CHAIN_FIXTURES, ChainFixture, load_api_fixture
 from oracle.brute_force import brute_force_cf
 from oracle.oracle import OracleModel, OracleUpdater, philox4x64
@@ -94,9 +94,9 @@ def test_oracle_replays_reference_api_log():
     assert meta["errors"]["background"][0] == "RuntimeError"
 
 
-@pytest.mark.parametrize("symmetrize", [False, True])
-def test_incremental_cf_equals_bruteforce(symmetrize):
-    s = FCCSettings(5, ["Al", "Mg", "Si"], {2: 3, 3: 2, 4: 1}, symmetrize_decorations=symmetrize)
+@pytest.mark.parametrize("lattice,symmetrize", [("fcc", False), ("fcc", True), ("bcc", True), ("sc", True)])
+def test_incremental_cf_equals_bruteforce(lattice, symmetrize):
+    s = CubicSettings(5, ["Al", "Mg", "Si"], {2: 3, 3: 2, 4: 1}, symmetrize_decorations=symmetrize, lattice=lattice)
     names = s.cf_names()
     eci = seeded_ecis(s, 4, 0.01)
     occ = random_occupation(s.n_sites, 3, 8)
@@ -145,8 +145,9 @@ def _ising_delta(s, eci, occ, i):
     return delta
 
 
-def test_binary_energy_change_equals_ising_local_field():
-    s = FCCSettings(6, ["A", "B"], {2: 4, 3: 2})
+@pytest.mark.parametrize("lattice", ["fcc", "bcc", "sc"])
+def test_binary_energy_change_equals_ising_local_field(lattice):
+    s = CubicSettings(6, ["A", "B"], {2: 4, 3: 2}, lattice=lattice)
     names = s.cf_names()
     eci = seeded_ecis(s, 19, 0.02)
     occ = random_occupation(s.n_sites, 2, 23)
@@ -160,6 +161,27 @@ def test_binary_energy_change_equals_ising_local_field():
         assert abs((e1 - e0) - closed) < 1e-12
         upd.undo_changes()
         assert upd.get_energy() == e0
+
+
+@pytest.mark.parametrize("lattice,counts", [("fcc", (12, 6, 24, 12)), ("bcc", (8, 6, 12, 24)),
+                                            ("sc", (6, 12, 8, 6))])
+def test_pair_orbits_are_the_neighbour_shells(lattice, counts):
+    """Pair orbits of each cubic lattice are its neighbour shells: the
+    members of site 0 are exactly the sites at that minimum-image distance
+    (from Cartesian positions), with the textbook coordination numbers."""
+    s = CubicSettings(7, ["A", "B"], {2: 4}, lattice=lattice)
+    pos = s.positions.astype(float)  # a/2 units
+    box = np.array(s.dims)[:, None] * s.prim.astype(float)  # supercell vectors (rows)
+    frac = (pos - pos[0]) @ np.linalg.inv(box)
+    frac -= np.round(frac)
+    d2 = np.round(((frac @ box) ** 2).sum(axis=1)).astype(int)
+    shells = sorted(set(d2.tolist()) - {0})[:4]
+    pairs = sorted((o for o in s.orbits if o.size == 2), key=lambda o: o.d2)
+    assert [o.multiplicity for o in pairs] == list(counts)
+    for o, r2 in zip(pairs, shells):
+        assert o.d2 == r2
+        members = sorted(m[0] for m in s.cluster_info[0][o.name]["indices"])
+        assert members == sorted(np.nonzero(d2 == r2)[0].tolist())
 
 
 def test_exhaustive_tiny_cell_every_state_every_flip():

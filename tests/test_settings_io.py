@@ -30,9 +30,35 @@ def test_cebulk_fcc_arguments():
     assert s3.kwargs["max_shell_by_size"] == {"2": 4, "3": 2}
     assert s2.cf_names() == s3.cf_names()
     with pytest.raises(ValueError):
-        settings_from_kwargs(dict(kw, crystalstructure="bcc"))
+        settings_from_kwargs(dict(kw, crystalstructure="hcp"))
     with pytest.raises(ValueError):
         settings_from_kwargs({"classtype": "CECrystal"})
+
+
+def test_cebulk_bcc_and_sc_arguments():
+    """BCC shells at a*sqrt(3)/2, a, a*sqrt(2), a*sqrt(11)/2 and SC shells at
+    a, a*sqrt(2), a*sqrt(3): the diameters map to those shell counts, and
+    the settings round-trip through their own arguments."""
+    from cemc_amd.lattice import BCCSettings, SCSettings, shells_within
+    a = 3.0
+    assert shells_within(a * np.sqrt(3) / 2, a, "bcc") == 1
+    assert shells_within(a * 1.01, a, "bcc") == 2
+    assert shells_within(a * np.sqrt(2), a, "bcc") == 3
+    assert shells_within(a * np.sqrt(11) / 2, a, "bcc") == 4
+    assert shells_within(a * 0.99, a, "sc") == 0
+    assert shells_within(a * np.sqrt(2), a, "sc") == 2
+    assert shells_within(a * np.sqrt(3), a, "sc") == 3
+    kw = {"classtype": "CEBulk", "crystalstructure": "bcc", "a": a, "size": [4, 4, 4],
+          "basis_elements": [["Fe", "Cr"]], "max_cluster_size": 3, "max_cluster_dia": [a * 1.01, a * 0.9]}
+    b = settings_from_kwargs(kw)
+    assert isinstance(b, BCCSettings) and b.n_sites == 64
+    assert sorted(n.rsplit("_", 1)[0] for n in b.cf_names() if n.startswith("c2")) == ["c2_d0000_0", "c2_d0001_0"]
+    s = settings_from_kwargs(dict(kw, crystalstructure="sc", size=5, basis_elements=[["Cu", "Au", "Ag"]]))
+    assert isinstance(s, SCSettings) and s.n_sites == 125
+    for st in (b, s):
+        again = settings_from_kwargs(dict(st.kwargs, classtype=type(st).__name__))
+        assert type(again) is type(st) and again.cf_names() == st.cf_names()
+        assert np.array_equal(again.trans_table, st.trans_table)
 
 
 def test_fcc_settings_round_trip():
