@@ -492,6 +492,242 @@ class CubicSettings:
         return {"dims": self.dims, "cols": self.trans_cols, "col_cells": self.trans_col_cells}
 
 
+# Two-sublattice cubic crystals: primitive vectors and basis (a/2 units).
+# Rocksalt (B1): FCC lattice, basis (0,0,0) and (1,0,0); the sublattice of a
+# point is the parity of its coordinate sum.  CsCl (B2): simple-cubic
+# lattice, basis (0,0,0) and (1,1,1); the sublattice is the parity of x.
+CRYSTALS = {
+    "rocksalt": (FCC_PRIM, np.array([[0, 0, 0], [1, 0, 0]], dtype=np.int64), "sc_half"),
+    "cscl": (SC_PRIM, np.array([[0, 0, 0], [1, 1, 1]], dtype=np.int64), "bcc"),
+}
+
+
+def _sublattice(points, structure):
+    p = np.asarray(points, dtype=np.int64)
+    if structure == "rocksalt":
+        return p.sum(axis=-1) % 2
+    return p[..., 0] % 2
+
+
+def _crystal_points_within(d2_max, structure):
+    """Points of the combined (all-sublattice) point set with 0 < |p|^2 <=
+    d2_max: every integer point for rocksalt, BCC points for CsCl."""
+    r = int(math.isqrt(int(d2_max))) + 1
+    g = np.arange(-r, r + 1)
+    pts = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T
+    if structure == "cscl":
+        pts = pts[_on_lattice(pts, "bcc")]
+    d2 = (pts * pts).sum(axis=1)
+    pts = pts[(d2 > 0) & (d2 <= d2_max)]
+    order = np.lexsort((pts[:, 2], pts[:, 1], pts[:, 0], (pts * pts).sum(axis=1)))
+    return pts[order]
+
+
+class CrystalSettings:
+    """Settings of a periodic two-sublattice cubic crystal (CLEASE
+    ``CECrystal`` of rocksalt or CsCl type) with one translation group per
+    active sublattice.
+
+    Parameters
+    ----------
+    size : int or (int, int, int)
+        Supercell repetitions of the primitive cell (two sites per cell).
+    basis_elements : [list[str], list[str]]
+        Species allowed on each sublattice.  A sublattice with one species is
+        background: its sites are in ``background_indices``, in no
+        translation group and in no cluster (ce_updater.cpp:81-97, 733-776,
+        1015-1027).
+    max_shell_by_size : dict[int, int]
+        Outermost neighbour shell (of the combined point set) per cluster size.
+    structure : "rocksalt" or "cscl"
+    """
+
+    classtype = "CrystalSettings"
+
+    def __init__(self, size, basis_elements, max_shell_by_size, structure="rocksalt",
+                 symmetrize_decorations=True):
+        if isinstance(size, int):
+            size = (size, size, size)
+        structure = str(structure).lower()
+        if structure not in CRYSTALS:
+            raise ValueError("unknown structure %r (rocksalt or cscl)" % (structure,))
+        if len(basis_elements) != 2:
+            raise ValueError("two sublattices are required")
+        self.structure = structure
+        self.prim, self.basis_pos, _ = CRYSTALS[structure]
+        self._det = int(round(np.linalg.det(self.prim)))
+        self._adj = np.rint(np.linalg.inv(self.prim) * self._det).astype(np.int64)
+        self.supercell = tuple(int(v) for v in size)
+        self.basis_elements = [sorted(set(b)) for b in basis_elements]
+        self.species = sorted(set(e for b in self.basis_elements for e in b))
+        self.unique_elements = list(self.species)
+        self.num_unique_elements = len(self.species)
+        self.symmetrize_decorations = bool(symmetrize_decorations)
+        self.self_interaction = False
+        self.kwargs = {"size": list(self.supercell), "basis_elements": [list(b) for b in self.basis_elements],
+                       "max_shell_by_size": {str(k): int(v) for k, v in dict(max_shell_by_size).items()},
+                       "structure": structure, "symmetrize_decorations": self.symmetrize_decorations}
+        L1, L2, L3 = self.supercell
+        n_cells = L1 * L2 * L3
+        self.n_sites = 2 * n_cells
+        c = np.arange(n_cells)
+        self.cell_of_site = np.repeat(np.stack([c // (L2 * L3), (c // L3) % L2, c % L3], axis=1), 2, axis=0)
+        self.sublattice_of_site = np.tile(np.array([0, 1]), n_cells)
+        self.positions = self.cell_of_site @ self.prim + self.basis_pos[self.sublattice_of_site]
+        self.active = [len(b) > 1 for b in self.basis_elements]
+        if not any(self.active):
+            raise ValueError("at least one sublattice needs two species")
+        self.background_indices = [int(i) for i in range(self.n_sites) if not self.active[i % 2]]
+        self.groups = [b for b in range(2) if self.active[b]]
+        self.index_by_trans_symm = [[int(i) for i in range(self.n_sites) if i % 2 == b] for b in self.groups]
+        phi = polynomial_basis(len(self.species))
+        self.basis = phi
+        self.basis_functions = [{sp: float(phi[d, k]) for k, sp in enumerate(self.species)}
+                                for d in range(phi.shape[0])]
+        self._build(dict(max_shell_by_size))
+
+    def site_of(self, points):
+        """Site index of absolute lattice points (a/2 units), wrapped."""
+        p = np.asarray(points, dtype=np.int64)
+        b = _sublattice(p, self.structure)
+        num = (p - self.basis_pos[b]) @ self._adj
+        if np.any(num % self._det):
+            raise ValueError("point is not a lattice site")
+        cell = num // self._det
+        L1, L2, L3 = self.supercell
+        ci = ((cell[..., 0] % L1) * L2 + (cell[..., 1] % L2)) * L3 + (cell[..., 2] % L3)
+        return 2 * ci + b
+
+    def _build(self, max_shell_by_size):
+        from itertools import combinations
+        info = [dict() for _ in self.groups]
+        for gi in range(len(self.groups)):
+            info[gi]["c0"] = {"name": "c0", "descriptor": "empty", "size": 0, "ref_indx": self.groups[gi],
+                              "symm_group": gi, "max_cluster_dia": 0.0, "indices": [], "order": [],
+                              "equiv_sites": []}
+            info[gi]["c1"] = {"name": "c1", "descriptor": "point_cluster", "size": 1, "ref_indx": self.groups[gi],
+                              "symm_group": gi, "max_cluster_dia": 0.0, "indices": [], "order": [[0]],
+                              "equiv_sites": []}
+        found = []  # (size, d2, key, group index, absolute clusters, ranks, equiv)
+        if max_shell_by_size:
+            pts = _crystal_points_within(400, self.structure)
+            shells = sorted(set((pts * pts).sum(axis=1).tolist()))
+            for size in sorted(max_shell_by_size):
+                if size not in (2, 3, 4):
+                    raise ValueError("Only cluster sizes 2, 3 and 4 are supported")
+                d2_cut = shells[max_shell_by_size[size] - 1]
+                for gi, g in enumerate(self.groups):
+                    origin = self.basis_pos[g]
+                    cand = _crystal_points_within(d2_cut, self.structure) + origin
+                    cand = cand[np.array([self.active[int(v)] for v in _sublattice(cand, self.structure)],
+                                         dtype=bool)]
+                    diff = cand[:, None, :] - cand[None, :, :]
+                    ok = (diff * diff).sum(axis=2) <= d2_cut
+                    combos = [cmb for cmb in combinations(range(len(cand)), size - 1)
+                              if all(ok[a, b] for a, b in combinations(cmb, 2))]
+                    if not combos:
+                        continue
+                    cl = np.concatenate([np.broadcast_to(origin, (len(combos), 1, 3)),
+                                         cand[np.array(combos, dtype=np.int64)]], axis=1)
+                    rot = np.einsum("rij,cnj->crni", _OPS, cl)
+                    codes = _encode(rot)
+                    ranks = np.argsort(np.argsort(codes, axis=2, kind="stable"), axis=2, kind="stable")
+                    srt_idx = np.argsort(codes, axis=2, kind="stable")
+                    srt = np.take_along_axis(codes, srt_idx, axis=2)
+                    first_pt = np.take_along_axis(rot, srt_idx[..., :1, None], axis=2)[..., 0, :]
+                    sub0 = _sublattice(first_pt, self.structure)  # (n_cl, 48)
+                    rel = srt - srt[..., :1] + _CODE_ZERO
+                    packed = np.zeros(codes.shape[:2], dtype=np.int64)
+                    shift = 0
+                    for j in range(1, size):
+                        packed += rel[..., j].astype(np.int64) << shift
+                        shift += 21
+                    # orbit key: smallest (sublattice of the first point, packed)
+                    smin = sub0.min(axis=1)
+                    big = np.iinfo(np.int64).max
+                    masked = np.where(sub0 == smin[:, None], packed, big)
+                    pmin = masked.min(axis=1)
+                    stab = (sub0 == smin[:, None]) & (packed == pmin[:, None])
+                    first_op = np.argmax(stab, axis=1)
+                    canon = ranks[np.arange(len(cl)), first_op]
+                    d = cl[:, :, None, :] - cl[:, None, :, :]
+                    d2max = (d * d).sum(axis=3).reshape(len(cl), -1).max(axis=1)
+                    for key in sorted(set(zip(smin.tolist(), pmin.tolist()))):
+                        sel = np.nonzero((smin == key[0]) & (pmin == key[1]))[0]
+                        rep = sel[0]
+                        eq = _equiv_groups(ranks[rep], stab[rep], first_op[rep])
+                        found.append((size, int(d2max[rep]), key, gi, cl[sel], canon[sel], eq))
+        # global names: orbits of one size by diameter, then key
+        keys = sorted({(f[0], f[1], f[2]) for f in found})
+        names = {}
+        for size in sorted({k[0] for k in keys}):
+            same = [k for k in keys if k[0] == size]
+            dias = sorted({k[1] for k in same})
+            for di, d2 in enumerate(dias):
+                for k_i, k in enumerate([k for k in same if k[1] == d2]):
+                    names[k] = "c%d_d%04d_%d" % (size, di, k_i)
+        self.orbit_names = names
+        self.orbit_diameter = {names[k]: math.sqrt(k[1]) / 2.0 for k in keys}
+        self.orbit_sizes = {names[k]: k[0] for k in keys}
+        self.orbit_equiv = {}
+        used = set()
+        for size, d2, key, gi, cls, canon, eq in found:
+            name = names[(size, d2, key)]
+            members, order = [], []
+            for m in range(len(cls)):
+                by_pos = np.argsort(canon[m])
+                r = int(canon[m][0])
+                mem = [int(self.site_of(cls[m][k])) for k in by_pos if k != 0]
+                members.append(mem)
+                order.append([k + 1 if k < r else (0 if k == r else k) for k in range(size)])
+                used.update(mem)
+            if any(self.groups[gi] in mem or len(set(mem)) != len(mem) for mem in members):
+                self.self_interaction = True
+            eqs = [list(g) for g in eq] if self.symmetrize_decorations else []
+            self.orbit_equiv[name] = eqs
+            info[gi][name] = {"name": name, "descriptor": "%d-body d2=%d sub=%d" % (size, d2, key[0]),
+                              "size": size, "ref_indx": self.groups[gi], "symm_group": gi,
+                              "max_cluster_dia": math.sqrt(d2) / 2.0, "indices": members, "order": order,
+                              "equiv_sites": eqs}
+        self.cluster_info = info
+        # trans(i, col): the site that plays col's role when site i is the
+        # reference site of its group: col translated by cell(i)
+        cols = np.array(sorted(used), dtype=np.int64)
+        self.trans_cols = cols.astype(np.int32)
+        shift = self.cell_of_site[:, None, :] @ self.prim  # (n, 1, 3) translation of each site's cell
+        tab = self.site_of(self.positions[cols][None, :, :] + shift)
+        self.trans_table = np.ascontiguousarray(tab.astype(np.int32))
+        self.trans_matrix = _TransMatrixRows(self.trans_cols, self.trans_table)
+
+    def decorations(self, name):
+        nb = len(self.basis_functions)
+        size = self.orbit_sizes[name]
+        decos = ["".join(str(v) for v in d) for d in itertools.product(range(nb), repeat=size)]
+        eqs = self.orbit_equiv.get(name, [])
+        if not self.symmetrize_decorations or not eqs:
+            return decos
+        keep, seen = [], set()
+        for d in decos:
+            cls = tuple(sorted("".join(str(v) for v in e)
+                               for e in equivalent_decorations([int(c) for c in d], eqs)))
+            if cls in seen:
+                continue
+            seen.add(cls)
+            keep.append(d)
+        return keep
+
+    def cf_names(self):
+        names = ["c0"] + ["c1_%d" % d for d in range(len(self.basis_functions))]
+        for name in sorted(self.orbit_sizes):
+            names += ["%s_%s" % (name, d) for d in self.decorations(name)]
+        return sorted(names)
+
+    def cf_diameter(self, name):
+        if name.startswith("c0") or name.startswith("c1"):
+            return 0.0
+        return self.orbit_diameter[name.rsplit("_", 1)[0]]
+
+
 class FCCSettings(CubicSettings):
     """Settings of a periodic FCC supercell (CLEASE ``CEBulk`` with
     ``crystalstructure="fcc"``)."""
@@ -518,6 +754,18 @@ class SCSettings(CubicSettings):
 
     def __init__(self, size, species, max_shell_by_size, symmetrize_decorations=True):
         super().__init__(size, species, max_shell_by_size, symmetrize_decorations, "sc")
+
+
+def random_crystal_occupation(settings, seed):
+    """Species ids of a CrystalSettings lattice, each site drawn uniformly
+    from its sublattice's elements (Philox stream of ``seed``)."""
+    rng = np.random.Generator(np.random.Philox(int(seed)))
+    sp = list(settings.species)
+    occ = np.empty(settings.n_sites, dtype=np.int8)
+    for i in range(settings.n_sites):
+        elems = settings.basis_elements[int(settings.sublattice_of_site[i])]
+        occ[i] = sp.index(elems[int(rng.integers(len(elems)))])
+    return occ
 
 
 def seeded_ecis(settings, seed, sigma, scale_by_diameter=True):
@@ -606,8 +854,8 @@ _BY_NAME = {"fcc": FCCSettings, "bcc": BCCSettings, "sc": SCSettings}
 def settings_from_kwargs(kwargs):
     """Settings from the ``setting_kwargs`` of a saved calculator.
 
-    ``classtype`` "FCCSettings", "BCCSettings", "SCSettings" or
-    "CubicSettings": this module's own arguments.  ``classtype`` "CEBulk" with
+    ``classtype`` "FCCSettings", "BCCSettings", "SCSettings",
+    "CubicSettings" or "CrystalSettings": this module's own arguments.  ``classtype`` "CEBulk" with
     ``crystalstructure`` "fcc", "bcc" or "sc" (the reference's CLEASE bulk
     settings, ce_calculator.py:577-592): ``size`` is the repetition of the
     primitive cell, the species come from ``basis_elements`` (or
@@ -618,6 +866,10 @@ def settings_from_kwargs(kwargs):
     entries is indexed by size).  Other CLEASE settings are not available."""
     kw = dict(kwargs)
     classtype = kw.pop("classtype", "FCCSettings")
+    if classtype == "CrystalSettings":
+        shells = {int(k): int(v) for k, v in kw["max_shell_by_size"].items()}
+        return CrystalSettings(kw["size"], kw["basis_elements"], shells, kw["structure"],
+                               kw.get("symmetrize_decorations", True))
     if classtype in ("FCCSettings", "BCCSettings", "SCSettings", "CubicSettings"):
         shells = {int(k): int(v) for k, v in kw["max_shell_by_size"].items()}
         lattice = kw.get("lattice", {"FCCSettings": "fcc", "BCCSettings": "bcc", "SCSettings": "sc"}.get(classtype))

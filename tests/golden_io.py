@@ -6,7 +6,7 @@ import os
 
 import numpy as np
 
-from cemc_amd.lattice import FCCSettings
+from cemc_amd.lattice import CrystalSettings, FCCSettings
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -21,6 +21,14 @@ CHAIN_FIXTURES = [
     "quaternary_canonical.npz",
 ]
 
+# two-sublattice crystals: two translation groups, or one group and a
+# background sublattice
+CRYSTAL_FIXTURES = [
+    "rocksalt_canonical.npz",
+    "rocksalt_background_sgc.npz",
+    "cscl_sgc.npz",
+]
+
 
 class ChainFixture:
     def __init__(self, fname):
@@ -29,7 +37,12 @@ class ChainFixture:
         self.meta = json.loads(str(z["meta"]))
         m = self.meta
         shells = {int(k): v for k, v in m["shells"].items()}
-        self.settings = FCCSettings(m["lattice"], m["species"], shells, symmetrize_decorations=m["symmetrize"])
+        if m.get("crystal"):
+            cr = m["crystal"]
+            self.settings = CrystalSettings(m["lattice"], cr["basis_elements"], shells, cr["structure"],
+                                            symmetrize_decorations=m["symmetrize"])
+        else:
+            self.settings = FCCSettings(m["lattice"], m["species"], shells, symmetrize_decorations=m["symmetrize"])
         self.names = m["names"]
         assert self.names == self.settings.cf_names()
         self.eci = dict(zip(self.names, z["eci"].tolist()))

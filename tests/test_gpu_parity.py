@@ -7,7 +7,7 @@ target: |E_gpu - E_ref| / N <= 1e-10 eV/site).
 import numpy as np
 import pytest
 
-from golden_io import CHAIN_FIXTURES, ChainFixture, load_api_fixture
+from golden_io import CHAIN_FIXTURES, CRYSTAL_FIXTURES, ChainFixture, load_api_fixture
 
 E_TOL = 1e-10  # eV per site
 CF_TOL = 1e-10
@@ -32,7 +32,7 @@ def _batch_for(fx, settings=None, path=0):
 
 
 @pytest.mark.parametrize("path", ["evaluate", "fields"])
-@pytest.mark.parametrize("fname", CHAIN_FIXTURES)
+@pytest.mark.parametrize("fname", CHAIN_FIXTURES + CRYSTAL_FIXTURES)
 def test_batch_reproduces_reference_chain(gpu, fname, path):
     """Both energy paths: tables summed per trial move, and per-site fields
     updated on accepted moves (periodic fixtures; the others evaluate)."""

@@ -20,7 +20,7 @@ import numpy as np
 import pytest
 
 from cemc_amd.lattice import CubicSettings, FCCSettings, FCC_PRIM, KB_EV, random_occupation, seeded_ecis
-from golden_io import CHAIN_FIXTURES, ChainFixture, load_api_fixture
+from golden_io import CHAIN_FIXTURES, CRYSTAL_FIXTURES, ChainFixture, load_api_fixture
 from oracle.brute_force import brute_force_cf
 from oracle.oracle import OracleModel, OracleUpdater, philox4x64
 
@@ -45,7 +45,7 @@ def test_philox_matches_numpy_and_random123():
         assert philox4x64(ctr, key) == _numpy_philox_block(ctr, key)
 
 
-@pytest.mark.parametrize("fname", CHAIN_FIXTURES)
+@pytest.mark.parametrize("fname", CHAIN_FIXTURES + CRYSTAL_FIXTURES)
 def test_oracle_reproduces_reference_fixture(fname):
     fx = ChainFixture(fname)
     model = OracleModel(fx.settings, fx.names)

@@ -1,5 +1,8 @@
-"""Settings from saved calculator arguments: this package's FCC settings and
-the reference's CLEASE FCC bulk arguments (ce_calculator.py:573-592)."""
+"""Settings from saved calculator arguments: this package's cubic and
+two-sublattice settings and the reference's CLEASE bulk arguments
+(ce_calculator.py:573-592)."""
+import itertools
+
 import numpy as np
 import pytest
 
@@ -68,3 +71,42 @@ def test_fcc_settings_round_trip():
     assert r.dims == s.dims and r.species == s.species
     assert r.cf_names() == s.cf_names()
     assert r.symmetrize_decorations is False
+
+
+def test_two_sublattice_crystals():
+    """Rocksalt and CsCl: one translation group per active sublattice, the
+    nearest-neighbour (unlike-sublattice) pair orbit shared by both groups
+    with 6 (rocksalt) or 8 (CsCl) members, like-sublattice orbits named
+    per sublattice; a single-element sublattice is background; the members
+    of every group's pair orbits lie at the orbit's distance; the settings
+    round-trip through their own arguments."""
+    from cemc_amd.lattice import CrystalSettings
+    rs = CrystalSettings(3, [["Na", "K"], ["Cl", "Br"]], {2: 3}, "rocksalt")
+    assert rs.n_sites == 54 and len(rs.index_by_trans_symm) == 2 and rs.background_indices == []
+    nn = [{k: len(v["indices"]) for k, v in g.items() if k.startswith("c2")} for g in rs.cluster_info]
+    assert nn[0]["c2_d0000_0"] == 6 and nn[1]["c2_d0000_0"] == 6
+    assert "c2_d0001_0" in nn[0] and "c2_d0001_0" not in nn[1] and "c2_d0001_1" in nn[1]
+    cs = CrystalSettings(4, [["Cu", "Zn"], ["Cu", "Zn"]], {2: 1}, "cscl")
+    assert all(len(g["c2_d0000_0"]["indices"]) == 8 for g in cs.cluster_info)
+    bg = CrystalSettings(3, [["Na", "K"], ["Cl"]], {2: 2}, "rocksalt")
+    assert len(bg.index_by_trans_symm) == 1 and bg.background_indices == list(range(1, 54, 2))
+    assert bg.species == ["Cl", "K", "Na"]
+    for st in (rs, cs, bg):
+        pos = st.positions.astype(float)
+        box = np.array(st.supercell)[:, None] * st.prim.astype(float)
+        inv = np.linalg.inv(box)
+        for g, info in enumerate(st.cluster_info):
+            ref = info["c0"]["ref_indx"]
+            for name, cl in info.items():
+                if cl["size"] != 2:
+                    continue
+                for (j,) in cl["indices"]:
+                    f = (pos[j] - pos[ref]) @ inv
+                    f -= np.round(f)
+                    # minimum image over the neighbouring supercells (skewed box)
+                    d2 = min(float((((f + np.array(k)) @ box) ** 2).sum())
+                             for k in itertools.product((-1, 0, 1), repeat=3))
+                    assert abs(np.sqrt(d2) / 2.0 - cl["max_cluster_dia"]) < 1e-12
+                    assert j not in st.background_indices
+        again = settings_from_kwargs(dict(st.kwargs, classtype="CrystalSettings"))
+        assert again.cf_names() == st.cf_names() and np.array_equal(again.trans_table, st.trans_table)
