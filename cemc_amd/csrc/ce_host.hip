@@ -2058,6 +2058,8 @@ int cemc_batch_set_hooks(cemc_batch* b, int32_t chain, const cemc_hooks_desc* d)
     require(b != nullptr, "NULL batch");
     require(!b->dynamic, "constraints and bias apply to fixed batches, not pools");
     require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    require(d == nullptr || b->wl.empty() || b->wl[chain].n_bins <= 0,
+            "constraints and bias do not apply to Wang-Landau walkers");
     const cemc_model* m = b->models[b->params[chain].model];
     const int N = m->n_sites, S = m->n_species;
     if (b->hooks.empty()) b->hooks.resize(size_t(b->n_chains));
@@ -2154,6 +2156,7 @@ int cemc_batch_wl_set(cemc_batch* b, int32_t chain, const cemc_wl_desc* desc, co
     require(!b->dynamic, "Wang-Landau walkers run on fixed batches, not pools");
     require(chain >= 0 && chain < b->n_chains, "chain index out of range");
     require(b->params[chain].kind == CEMC_MOVE_CANONICAL, "Wang-Landau walkers make canonical swaps");
+    require(b->hooks.empty() || !b->hooks[chain].on, "constraints and bias do not apply to Wang-Landau walkers");
     require(desc->n_bins > 0, "the histogram needs at least one bin");
     require(std::isfinite(desc->e_min) && std::isfinite(desc->e_max) && desc->e_max > desc->e_min,
             "the histogram range needs e_min < e_max");
@@ -2451,6 +2454,8 @@ int cemc_pt_run(cemc_pt* pt, int64_t n_cycles, int64_t moves_per_cycle, int32_t 
     }
     // the exchanges moved temperatures on the device: refresh the host copy
     HIP_CHECK(hipMemcpy(b->params.data(), b->d_params, sizeof(ChainParams) * b->params.size(), hipMemcpyDeviceToHost));
+    const std::string hook_error = hooks_after_launch(b);
+    if (!hook_error.empty()) throw std::runtime_error(hook_error);
   });
 }
 
