@@ -105,6 +105,7 @@ struct cemc_model {
   int pad_h = 0;
   bool spec_ok = false;   // speculative rounds allowed (see ModelDev::imask)
   uint64_t imask[2] = {0, 0};
+  int max_members = 3;  // most members other than the reference site over the records (1-3)
   bool field_ok = false;  // local-field path allowed (see ModelDev::duals)
   double fx_scale = 0.0;
   std::vector<int64_t> gtab_fx;
@@ -614,6 +615,8 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
     if (nm < 3) meta |= kRecUnused2;
     m->frecs.push_back(FatRec{mm[0], mm[1], mm[2], ww[0], ww[1], ww[2], meta, 0u});
   }
+  m->max_members = 1;
+  for (const auto& fr : m->frecs) m->max_members = std::max(m->max_members, int((fr.meta >> 24) & 3u));
   // interaction mask of the speculative rounds: the reference site and every
   // member offset of its sub-clusters, in the (2h+1)^3 box
   m->spec_ok = m->periodic && m->n_groups == 1 && int(m->movable.size()) == N && m->pad_h <= 2;
@@ -847,6 +850,7 @@ struct cemc_batch {
     int spec;  // moves per round
     bool field;  // local-field kernel
     bool hooks;  // constraints and bias
+    int nm;      // members gathered per record
     int begin, count;
     uint32_t lds;
   };
@@ -1055,6 +1059,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
     int spec;
     bool field;
     bool hooks;
+    int nm;
     uint32_t lds;
   };
   // waves per chain: enough chains x waves to give every SIMD of the
@@ -1149,6 +1154,8 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
         k.lds = f_lds;
       }
     }
+    // records in VGPRs gather two members each when no record has three
+    k.nm = (k.kr > 0 && !k.field && m->max_members <= 2) ? 2 : 3;
     // chains with constraints or a bias: one wave, one move at a time
     k.hooks = !b->hooks.empty() && b->hooks[c].on;
     if (k.hooks) {
@@ -1166,6 +1173,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
         k.lds = k.glds ? rec_g : rec_only;
         require(k.lds <= limit, "chain state does not fit in the workgroup's LDS");
       }
+      k.nm = (k.kr > 0 && m->max_members <= 2) ? 2 : 3;
     }
     keys[c] = k;
   }
@@ -1178,7 +1186,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
       const Key& k = keys[c];
       if (done[c] || k.periodic != k0.periodic || k.canonical != k0.canonical || k.kr != k0.kr ||
           k.glds != k0.glds || k.kbm != k0.kbm || k.waves != k0.waves || k.spec != k0.spec ||
-          k.field != k0.field || k.hooks != k0.hooks)
+          k.field != k0.field || k.hooks != k0.hooks || k.nm != k0.nm)
         continue;
       done[c] = true;
       ids.push_back(c);
@@ -1190,7 +1198,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
       return int64_t(mx->n_sites) * mx->recs.size() > int64_t(my->n_sites) * my->recs.size();
     });
     b->classes.push_back(cemc_batch::Class{k0.periodic, k0.canonical, k0.kr, k0.glds, k0.kbm, k0.waves, k0.spec,
-                                           k0.field, k0.hooks, int(order.size()), int(ids.size()), lds});
+                                           k0.field, k0.hooks, k0.nm, int(order.size()), int(ids.size()), lds});
     order.insert(order.end(), ids.begin(), ids.end());
   }
   if (order.size() > b->order_cap || b->d_order == nullptr) {
@@ -1273,6 +1281,7 @@ void enqueue_sweeps(cemc_batch* b, uint32_t* d_log, int64_t log_words) {
     a.spec = cl.spec;
     a.field = cl.field;
     a.hooks = cl.hooks ? b->d_hooks : nullptr;
+    a.nm = cl.nm;
     a.accept_log = d_log;
     a.log_words = log_words;
     hipStream_t st = fan_out ? b->side[ci % cemc_batch::kSideStreams] : b->stream;

@@ -28,6 +28,7 @@ struct LaunchSweep {
   int spec;        // moves per round: 1, or kSpecMoves (periodic, kr > 0)
   bool field;      // local-field kernel (periodic, one wave, kr > 0, kbm <= 3)
   HookDev* hooks;  // per chain constraints and bias (one wave, one move per round), or nullptr
+  int nm;          // members gathered per record: 3, or 2 for records in VGPRs of models without quadruplets
   uint32_t* accept_log;  // optional: bit t of chain c at accept_log[c * log_words + t / 32]
   int64_t log_words;
 };

@@ -386,7 +386,7 @@ __global__ __launch_bounds__(256) void pt_exchange_kernel(PTDev* __restrict__ pt
 // Launch wrappers.  The sweep kernel's instantiations are compiled in
 // separate objects (ce_sweep_inst.hip) and reached through these lookups.
 #define CEMC_DECLARE_SWEEP_FN(W, P) \
-  const void* sweep_fn_w##W##_p##P(bool canonical, int kr, bool glds, int kbm, int spec, bool hooks);
+  const void* sweep_fn_w##W##_p##P(bool canonical, int kr, bool glds, int kbm, int spec, bool hooks, int nm);
 CEMC_DECLARE_SWEEP_FN(1, 0)
 CEMC_DECLARE_SWEEP_FN(1, 1)
 CEMC_DECLARE_SWEEP_FN(2, 0)
@@ -406,13 +406,13 @@ static const void* wl_fn(bool periodic, int kr, bool glds, int kbm) {
 }
 
 static const void* sweep_fn(bool periodic, bool canonical, int kr, bool glds, int kbm, int waves, int spec,
-                            bool hooks = false) {
+                            bool hooks = false, int nm = 3) {
   const bool h = hooks;
   switch (waves) {
-    case 1: return periodic ? sweep_fn_w1_p1(canonical, kr, glds, kbm, spec, h) : sweep_fn_w1_p0(canonical, kr, glds, kbm, spec, h);
-    case 2: return periodic ? sweep_fn_w2_p1(canonical, kr, glds, kbm, spec, h) : sweep_fn_w2_p0(canonical, kr, glds, kbm, spec, h);
-    case 4: return periodic ? sweep_fn_w4_p1(canonical, kr, glds, kbm, spec, h) : sweep_fn_w4_p0(canonical, kr, glds, kbm, spec, h);
-    case 8: return periodic ? sweep_fn_w8_p1(canonical, kr, glds, kbm, spec, h) : sweep_fn_w8_p0(canonical, kr, glds, kbm, spec, h);
+    case 1: return periodic ? sweep_fn_w1_p1(canonical, kr, glds, kbm, spec, h, nm) : sweep_fn_w1_p0(canonical, kr, glds, kbm, spec, h, nm);
+    case 2: return periodic ? sweep_fn_w2_p1(canonical, kr, glds, kbm, spec, h, nm) : sweep_fn_w2_p0(canonical, kr, glds, kbm, spec, h, nm);
+    case 4: return periodic ? sweep_fn_w4_p1(canonical, kr, glds, kbm, spec, h, nm) : sweep_fn_w4_p0(canonical, kr, glds, kbm, spec, h, nm);
+    case 8: return periodic ? sweep_fn_w8_p1(canonical, kr, glds, kbm, spec, h, nm) : sweep_fn_w8_p0(canonical, kr, glds, kbm, spec, h, nm);
     default: return nullptr;
   }
 }
@@ -439,10 +439,12 @@ hipError_t sweep_set_max_lds(uint32_t bytes) {
           for (int g = 0; g < 2; ++g)
             for (int kbm : {1, 2, 3, 7})
               for (int sp : {1, kSpecMoves})
-                for (int hk = 0; hk < 2; ++hk) {
+                for (int hk = 0; hk < 2; ++hk)
+                  for (int nm : {2, 3}) {
               if (!sweep_variant_ok(kr, g != 0, kbm, w, sp, p != 0)) continue;
               if (hk && (w != 1 || sp != 1)) continue;
-              const void* fn = sweep_fn(p != 0, cn != 0, kr, g != 0, kbm, w, sp, hk != 0);
+              if (nm == 2 && kr == 0) continue;
+              const void* fn = sweep_fn(p != 0, cn != 0, kr, g != 0, kbm, w, sp, hk != 0, nm);
               if (fn == nullptr) return hipErrorInvalidValue;
               hipFuncAttributes attr;
               hipError_t e = hipFuncGetAttributes(&attr, fn);
@@ -491,7 +493,8 @@ hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream) {
   } else {
     if (!sweep_variant_ok(a.kr, a.gtab_lds, a.kbm, a.waves, a.spec, a.periodic)) return hipErrorInvalidValue;
     if (a.hooks != nullptr && (a.waves != 1 || a.spec != 1)) return hipErrorInvalidValue;
-    fn = sweep_fn(a.periodic, a.canonical, a.kr, a.gtab_lds, a.kbm, a.waves, a.spec, a.hooks != nullptr);
+    if (a.nm != 3 && !(a.nm == 2 && a.kr > 0)) return hipErrorInvalidValue;
+    fn = sweep_fn(a.periodic, a.canonical, a.kr, a.gtab_lds, a.kbm, a.waves, a.spec, a.hooks != nullptr, a.nm);
   }
   if (fn == nullptr) return hipErrorInvalidValue;
   const ChainParams* params = a.params;

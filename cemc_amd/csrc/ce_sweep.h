@@ -117,11 +117,16 @@ struct TableGeom {
 // member-species gathers, energy-table indices.  With kPatch, members that
 // are the patch site (periodic: wrapped offset == patch_w; table: site ==
 // patch_w) read `patch_species`.
-template <bool kPeriodic, int kR, int kBM, bool kPatch>
+// kNM: members gathered per record (2 when no record has more than two
+// other members, i.e. the model has no quadruplets; the third field of the
+// code is then 0, which reads the same entry since entries repeat over the
+// fields of unused members).
+template <bool kPeriodic, int kR, int kBM, bool kPatch, int kNM = 3>
 __device__ __attribute__((always_inline)) inline void table_index(const SweepCtx& c, const FatRec (&rec)[kR],
                                                                   const Ref& ref, const uint8_t* occ,
                                                                   uint32_t patch_w, uint32_t patch_species,
                                                                   int (&gi)[kR]) {
+  static_assert(kNM == 2 || kNM == 3, "two or three gathered members");
   constexpr int kBits = TableGeom<kBM>::kBits;
   int nb[kR][3];
 #pragma unroll
@@ -134,14 +139,14 @@ __device__ __attribute__((always_inline)) inline void table_index(const SweepCtx
       const int32_t* row = c.trans + int64_t(ref.site) * c.K;
       nb[k][0] = row[rec[k].m0];
       nb[k][1] = row[rec[k].m1];
-      nb[k][2] = row[rec[k].m2];
+      nb[k][2] = kNM == 3 ? row[rec[k].m2] : 0;
     }
   }
   uint32_t sp[kR][3];
 #pragma unroll
   for (int k = 0; k < kR; ++k)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) sp[k][q] = occ[nb[k][q]];
+    for (int q = 0; q < 3; ++q) sp[k][q] = q < kNM ? uint32_t(occ[nb[k][q]]) : 0u;
 #pragma unroll
   for (int k = 0; k < kR; ++k) {
     uint32_t s0 = sp[k][0], s1 = sp[k][1], s2 = sp[k][2];
@@ -149,11 +154,11 @@ __device__ __attribute__((always_inline)) inline void table_index(const SweepCtx
       if constexpr (kPeriodic) {
         s0 = rec[k].w0 == patch_w ? patch_species : s0;
         s1 = rec[k].w1 == patch_w ? patch_species : s1;
-        s2 = rec[k].w2 == patch_w ? patch_species : s2;
+        if constexpr (kNM == 3) s2 = rec[k].w2 == patch_w ? patch_species : s2;
       } else {
         s0 = uint32_t(nb[k][0]) == patch_w ? patch_species : s0;
         s1 = uint32_t(nb[k][1]) == patch_w ? patch_species : s1;
-        s2 = uint32_t(nb[k][2]) == patch_w ? patch_species : s2;
+        if constexpr (kNM == 3) s2 = uint32_t(nb[k][2]) == patch_w ? patch_species : s2;
       }
     }
     // unused members need no masking: their table entries repeat over them
@@ -345,7 +350,8 @@ __device__ __attribute__((always_inline)) inline int64_t hook_pair_delta(
 // kHooks (one wave, one move at a time): the chain's constraints and bias
 // (HookDev) apply; each move's proposal is redrawn while it violates a
 // constraint.
-template <bool kPeriodic, bool kCanonical, int kR, bool kGtabLds, int kBM, int kW, int kK, bool kHooks = false>
+template <bool kPeriodic, bool kCanonical, int kR, bool kGtabLds, int kBM, int kW, int kK, bool kHooks = false,
+          int kNM = 3>
 __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __restrict__ params,
                                                    ChainState* __restrict__ states,
                                                    const ModelDev* __restrict__ models,
@@ -356,6 +362,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
                                                    uint32_t* __restrict__ accept_log, int64_t log_words,
                                                    HookDev* __restrict__ hooks) {
   static_assert(!kHooks || (kW == 1 && kK == 1), "constraints and bias run one wave, one move at a time");
+  static_assert(kNM == 3 || kR > 0, "records read from LDS gather three members");
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ double dphi_tab[kMaxSpecies * kMaxSpecies * kMaxBasis];  // [(old*S+new)*B + d]
   __shared__ double dsingle_tab[kMaxSpecies * kMaxSpecies];          // singlet energy change per (old,new)
@@ -657,8 +664,9 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
       if constexpr (!kCanonical && !kHooks) old_i = occ[ri.p];
       if constexpr (kR > 0) {
         // all member gathers of the move are issued together
-        table_index<kPeriodic, kR, kBM, false>(c, rr, ri, occ, 0u, 0u, gi_i);
-        if constexpr (kCanonical) table_index<kPeriodic, kR, kBM, true>(c, rr, rj, occ, patch_j, new_i, gi_j);
+        table_index<kPeriodic, kR, kBM, false, kNM>(c, rr, ri, occ, 0u, 0u, gi_i);
+        if constexpr (kCanonical)
+          table_index<kPeriodic, kR, kBM, true, kNM>(c, rr, rj, occ, patch_j, new_i, gi_j);
       }
       if constexpr (!kCanonical && !kHooks) {
         new_i = old_i + 1u + __builtin_amdgcn_readlane(v_new, l);
@@ -892,7 +900,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
             for (int d = 0; d < kBM; ++d) dphi[d] = d < c.B ? dphi_tab[pair * c.B + d] : 0.0;
           }
           int gi_i[kR];
-          table_index<true, kR, kBM, false>(c, rr, ri, occ, 0u, 0u, gi_i);
+          table_index<true, kR, kBM, false, kNM>(c, rr, ri, occ, 0u, 0u, gi_i);
           double pk = table_sum<kR, kBM>(gi_i, gtab, dphi);
           if constexpr (kCanonical) {
             Ref rj;
@@ -903,8 +911,8 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
 #pragma unroll
             for (int d = 0; d < kBM; ++d) dphj[d] = -dphi[d];
             int gi_j[kR];
-            table_index<true, kR, kBM, true>(c, rr, rj, occ, __builtin_amdgcn_readlane(patch, lk),
-                                             __builtin_amdgcn_readlane(n_v, lk), gi_j);
+            table_index<true, kR, kBM, true, kNM>(c, rr, rj, occ, __builtin_amdgcn_readlane(patch, lk),
+                                                  __builtin_amdgcn_readlane(n_v, lk), gi_j);
             const double pj = table_sum<kR, kBM>(gi_j, gtab, dphj);
             pk = kBM == 1 ? pk - pj : pk + pj;
           }
