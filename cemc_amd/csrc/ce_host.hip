@@ -1425,8 +1425,9 @@ void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* h
     int kr;
     bool glds;
     int kbm;
+    int nm;
     bool operator<(const WKey& o) const {
-      return std::tie(periodic, kr, glds, kbm) < std::tie(o.periodic, o.kr, o.glds, o.kbm);
+      return std::tie(periodic, kr, glds, kbm, nm) < std::tie(o.periodic, o.kr, o.glds, o.kbm, o.nm);
     }
   };
   std::map<WKey, std::pair<std::vector<int32_t>, uint32_t>> groups;
@@ -1445,6 +1446,7 @@ void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* h
           k.kr = kr;
           break;
         }
+    k.nm = (k.kr > 0 && m->max_members <= 2) ? 2 : 3;
     const int occ_bytes = m->periodic ? m->pdims[0] * m->pdims[1] * m->pdims[2] : m->n_sites;
     const int n_rec = k.kr == 0 ? int(m->recs.size()) : 0;
     const int nbins = b->wl[c].n_bins;
@@ -1492,6 +1494,7 @@ void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* h
     a.kr = kv.first.kr;
     a.gtab_lds = kv.first.glds;
     a.kbm = kv.first.kbm;
+    a.nm = kv.first.nm;
     a.n_checks = n_checks;
     a.check_every = check_every;
     a.accept_log = d_log;

@@ -67,6 +67,7 @@ struct LaunchWL {
   int kr;
   bool gtab_lds;
   int kbm;
+  int nm;  // members gathered per record (2 or 3, as for the sweep kernel)
   int64_t n_checks, check_every;
   uint32_t* accept_log;  // optional, as for the sweep kernel (bits OR-ed in)
   int64_t log_words;

@@ -398,11 +398,11 @@ CEMC_DECLARE_SWEEP_FN(8, 1)
 #undef CEMC_DECLARE_SWEEP_FN
 
 const void* field_fn(bool canonical, int kr, int kbm);
-const void* wl_fn_p0(int kr, bool glds, int kbm);
-const void* wl_fn_p1(int kr, bool glds, int kbm);
+const void* wl_fn_p0(int kr, bool glds, int kbm, int nm);
+const void* wl_fn_p1(int kr, bool glds, int kbm, int nm);
 
-static const void* wl_fn(bool periodic, int kr, bool glds, int kbm) {
-  return periodic ? wl_fn_p1(kr, glds, kbm) : wl_fn_p0(kr, glds, kbm);
+static const void* wl_fn(bool periodic, int kr, bool glds, int kbm, int nm = 3) {
+  return periodic ? wl_fn_p1(kr, glds, kbm, nm) : wl_fn_p0(kr, glds, kbm, nm);
 }
 
 static const void* sweep_fn(bool periodic, bool canonical, int kr, bool glds, int kbm, int waves, int spec,
@@ -457,9 +457,11 @@ hipError_t sweep_set_max_lds(uint32_t bytes) {
   for (int p = 0; p < 2; ++p)
     for (int kr : {0, 2, 4, 8})
       for (int g = 0; g < 2; ++g)
-        for (int kbm : {1, 2, 3, 7}) {
+        for (int kbm : {1, 2, 3, 7})
+          for (int nm : {2, 3}) {
           if (kbm == 7 && kr != 0) continue;
-          const void* fn = wl_fn(p != 0, kr, g != 0, kbm);
+          if (nm == 2 && kr == 0) continue;
+          const void* fn = wl_fn(p != 0, kr, g != 0, kbm, nm);
           if (fn == nullptr) return hipErrorInvalidValue;
           hipFuncAttributes attr;
           hipError_t e = hipFuncGetAttributes(&attr, fn);
@@ -521,7 +523,8 @@ hipError_t launch_wl(const LaunchWL& a, hipStream_t stream) {
   if (!((a.kr == 0 && (a.kbm == 1 || a.kbm == 2 || a.kbm == 3 || a.kbm == 7)) ||
         ((a.kr == 2 || a.kr == 4 || a.kr == 8) && (a.kbm == 1 || a.kbm == 2 || a.kbm == 3))))
     return hipErrorInvalidValue;
-  const void* fn = wl_fn(a.periodic, a.kr, a.gtab_lds, a.kbm);
+  if (a.nm != 3 && !(a.nm == 2 && a.kr > 0)) return hipErrorInvalidValue;
+  const void* fn = wl_fn(a.periodic, a.kr, a.gtab_lds, a.kbm, a.nm);
   if (fn == nullptr) return hipErrorInvalidValue;
   const ChainParams* params = a.params;
   ChainState* states = a.states;

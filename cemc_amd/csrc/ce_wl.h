@@ -58,7 +58,7 @@ __device__ __attribute__((always_inline)) inline uint32_t wave_min_u32(uint32_t 
 
 }  // namespace
 
-template <bool kPeriodic, int kR, bool kGtabLds, int kBM>
+template <bool kPeriodic, int kR, bool kGtabLds, int kBM, int kNM = 3>
 __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ params,
                                                 ChainState* __restrict__ states,
                                                 const ModelDev* __restrict__ models,
@@ -225,8 +225,8 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
         double part;
         if constexpr (kR > 0) {
           int gi_i[kR], gi_j[kR];
-          table_index<kPeriodic, kR, kBM, false>(c, rr, ri, occ, 0u, 0u, gi_i);
-          table_index<kPeriodic, kR, kBM, true>(c, rr, rj, occ, patch_j, new_i, gi_j);
+          table_index<kPeriodic, kR, kBM, false, kNM>(c, rr, ri, occ, 0u, 0u, gi_i);
+          table_index<kPeriodic, kR, kBM, true, kNM>(c, rr, rj, occ, patch_j, new_i, gi_j);
           part = table_sum<kR, kBM>(gi_i, gtab, dphi);
           // the swap's second flip sees site i already changed
           // (ce_updater.cpp:845-852)

@@ -1,8 +1,9 @@
 // Instantiations of the Wang-Landau kernel for one lattice path, chosen by
 // CEMC_WL_PERIODIC (0 or 1) at compile time.  Exposes
-// wl_fn_p<P>(kr, glds, kbm), the kernel's address for hipLaunchKernel, or
-// nullptr for a variant that is not built: records in VGPRs (kR = 2, 4, 8;
-// up to 4 species) or in LDS (kR = 0; up to 8), energy tables in LDS or HBM.
+// wl_fn_p<P>(kr, glds, kbm, nm), the kernel's address for hipLaunchKernel,
+// or nullptr for a variant that is not built: records in VGPRs (kR = 2, 4,
+// 8; up to 4 species; nm = 2 or 3 gathered members) or in LDS (kR = 0; up
+// to 8 species), energy tables in LDS or HBM.
 #include "ce_wl.h"
 
 #ifndef CEMC_WL_PERIODIC
@@ -16,7 +17,14 @@ namespace {
 constexpr bool P = CEMC_WL_PERIODIC != 0;
 
 template <int KBM, bool G>
-const void* wl_by_kr(int kr) {
+const void* wl_by_kr(int kr, int nm) {
+  if (nm == 2) {
+    if (kr == 2) return reinterpret_cast<const void*>(&wl_kernel<P, 2, G, KBM, 2>);
+    if (kr == 4) return reinterpret_cast<const void*>(&wl_kernel<P, 4, G, KBM, 2>);
+    if (kr == 8) return reinterpret_cast<const void*>(&wl_kernel<P, 8, G, KBM, 2>);
+    return nullptr;
+  }
+  if (nm != 3) return nullptr;
   if (kr == 2) return reinterpret_cast<const void*>(&wl_kernel<P, 2, G, KBM>);
   if (kr == 4) return reinterpret_cast<const void*>(&wl_kernel<P, 4, G, KBM>);
   if (kr == 8) return reinterpret_cast<const void*>(&wl_kernel<P, 8, G, KBM>);
@@ -29,11 +37,11 @@ const void* wl_by_kr(int kr) {
 #define CEMC_WL_FN_NAME2(P_) wl_fn_p##P_
 #define CEMC_WL_FN_NAME(P_) CEMC_WL_FN_NAME2(P_)
 
-const void* CEMC_WL_FN_NAME(CEMC_WL_PERIODIC)(int kr, bool glds, int kbm) {
-  if (kbm == 1) return glds ? wl_by_kr<1, true>(kr) : wl_by_kr<1, false>(kr);
-  if (kbm == 2) return glds ? wl_by_kr<2, true>(kr) : wl_by_kr<2, false>(kr);
-  if (kbm == 3) return glds ? wl_by_kr<3, true>(kr) : wl_by_kr<3, false>(kr);
-  if (kbm == 7 && kr == 0)
+const void* CEMC_WL_FN_NAME(CEMC_WL_PERIODIC)(int kr, bool glds, int kbm, int nm) {
+  if (kbm == 1) return glds ? wl_by_kr<1, true>(kr, nm) : wl_by_kr<1, false>(kr, nm);
+  if (kbm == 2) return glds ? wl_by_kr<2, true>(kr, nm) : wl_by_kr<2, false>(kr, nm);
+  if (kbm == 3) return glds ? wl_by_kr<3, true>(kr, nm) : wl_by_kr<3, false>(kr, nm);
+  if (kbm == 7 && kr == 0 && nm == 3)
     return reinterpret_cast<const void*>(glds ? &wl_kernel<P, 0, true, 7> : &wl_kernel<P, 0, false, 7>);
   return nullptr;
 }
