@@ -67,26 +67,108 @@ class ReplicaExchange:
             pass
 
 
+def _accept_probability(E1, E2, T1, T2):
+    """parallel_tempering.py:138-144 (not capped at 1)."""
+    from .lattice import KB_EV
+    return float(np.exp((1.0 / (KB_EV * T1) - 1.0 / (KB_EV * T2)) * (E1 - E2)))
+
+
+def find_temperature_scheme(mc_obj, Tmax, Tmin, target_accept=0.2, log=None):
+    """Temperature scheme of the reference's search
+    (parallel_tempering.py:47-136), each trial a GPU launch of 10 N moves on
+    a chain of its own.  The first replica runs 10 N moves at Tmax; each next
+    temperature starts from a copy of the previous replica's lattice: the
+    trial temperature is halved until the estimated exchange acceptance
+    exp((1/kT_prev - 1/kT) (E_prev - E)) drops to target_accept (E: the mean
+    energy since the copy, accumulated over all trial temperatures as the
+    reference's averages are), then bisected until the acceptance is within
+    0.01 of the target or the interval is below 1e-4 K.  Stops at Tmin, or
+    when halving reaches Tmin without a candidate.  Returns (temperatures,
+    acceptances), the first acceptance 0."""
+    log = log or (lambda msg: None)
+    natoms = len(mc_obj.atoms)
+    nsteps = 10 * natoms
+    model = mc_obj.calc.model
+    extra = mc_obj._chain_extra()
+    uid0 = int(mc_obj.uid)
+
+    def chain(T, occ, r):
+        d = dict(kind=mc_obj.kind, temperature=float(T), occupation=occ, seed=mc_obj.seed, uid=uid0 + r)
+        d.update(extra)
+        return ChainBatch([model], [d])
+
+    cur = chain(Tmax, mc_obj.calc.updater.get_occupation(), 0)
+    cur.reset_stats(0)
+    cur.run(nsteps)
+    T_cur = float(Tmax)
+    temps, accepts = [T_cur], [0.0]
+    lowest = T_cur
+    r = 1
+    while lowest > Tmin:
+        cur_E = cur.thermo(0)["mean_energy"]
+        new = chain(T_cur, cur.occupation(0), r)
+        new.reset_stats(0)
+        trial = T_cur / 2.0
+        accept = 1.0
+        found = False
+        while accept > target_accept and trial > Tmin:
+            new.set_temperature(0, trial)
+            new.run(nsteps)
+            accept = _accept_probability(cur_E, new.thermo(0)["mean_energy"], T_cur, trial)
+            found = accept <= target_accept
+            trial /= 2.0
+        if not found:
+            new.close()
+            break
+        upper, lower = T_cur, trial
+        while True:
+            new_T = 0.5 * (upper + lower)
+            new.set_temperature(0, new_T)
+            new.run(nsteps)
+            new_accept = _accept_probability(cur_E, new.thermo(0)["mean_energy"], T_cur, new_T)
+            if new_accept > target_accept:
+                upper = new_T
+            else:
+                lower = new_T
+            if abs(new_accept - target_accept) < 0.01 or upper - lower < 1e-4:
+                break
+        log("temperature %d: %.4f K (exchange acceptance %.3f)" % (r, new_T, new_accept))
+        temps.append(new_T)
+        accepts.append(new_accept)
+        cur.close()
+        cur, T_cur, lowest = new, new_T, new_T
+        r += 1
+    cur.close()
+    return temps, accepts
+
+
 class ParallelTempering:
     """Replicas of a ``cemc_amd.mcmc.Montecarlo`` object over a temperature
     scheme (parallel_tempering.py:5-20).
 
     The scheme is ``temperatures`` when given, else the first column of
     ``temp_scheme_file`` when it exists (parallel_tempering.py:32-45), else
-    ``n_replicas`` temperatures log-spaced from Tmax down to Tmin.  (The
-    reference's adaptive scheme search, :47-136, is not carried over.)
+    ``n_replicas`` temperatures log-spaced from Tmax down to Tmin when
+    ``n_replicas`` is given, else the reference's search
+    (``find_temperature_scheme``, parallel_tempering.py:47-136), whose
+    result is written to ``temp_scheme_file`` as the reference does.
     """
 
     def __init__(self, mc_obj=None, Tmax=1500.0, Tmin=100.0, temp_scheme_file="temp_scheme.csv",
-                 temperatures=None, n_replicas=16, seed=None, uid=None):
+                 temperatures=None, n_replicas=None, seed=None, uid=None, target_accept=0.2):
         from .mcmc import Montecarlo
         if not isinstance(mc_obj, Montecarlo):
             raise TypeError("mc_obj has to be of type Montecarlo!")
         if temperatures is None and temp_scheme_file and os.path.exists(temp_scheme_file):
             data = np.loadtxt(temp_scheme_file, delimiter=",", ndmin=2)
             temperatures = data[:, 0].tolist()
-        if temperatures is None:
+        if temperatures is None and n_replicas is not None:
             temperatures = np.geomspace(Tmax, Tmin, int(n_replicas)).tolist()
+        if temperatures is None:
+            temperatures, accepts = find_temperature_scheme(mc_obj, Tmax, Tmin, target_accept=target_accept)
+            if temp_scheme_file:
+                np.savetxt(temp_scheme_file, np.vstack((temperatures, accepts)).T, delimiter=",",
+                           header="Temperature (K), Acceptance probabability")
         temperatures = [float(t) for t in temperatures]
         if len(temperatures) < 2:
             raise ValueError("parallel tempering needs at least two temperatures")

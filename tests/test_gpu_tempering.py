@@ -78,3 +78,76 @@ def test_parallel_tempering_facade(gpu):
     assert res[-1]["energy"] <= res[0]["energy"] + 1e-9
     occ = pt.replica_occupation(0)
     assert occ.shape == (s.n_sites,)
+
+
+def _oracle_scheme(s, eci, occ, kind, seed, uid0, Tmax, Tmin, target=0.2):
+    """The reference's scheme search (parallel_tempering.py:47-136) driven by
+    oracle chains on the same streams: replica r is a new chain with uid
+    uid0 + r started from the previous replica's lattice; means over the
+    per-move energies since the copy."""
+    from cemc_amd.tempering import _accept_probability
+    names = s.cf_names()
+    model = OracleModel(s, names)
+    n = 10 * s.n_sites
+
+    def new(o):
+        return {"upd": OracleUpdater(model, o, brute_force_cf(s, o, names), eci), "t": 0, "E": []}
+
+    def run(ch, T, r):
+        _, tr, _ = ch["upd"].run(kind, seed, uid0 + r, n, T, move_begin=ch["t"], trace_every=1)
+        ch["t"] += n
+        ch["E"].extend(tr.tolist())
+        return float(np.mean(ch["E"]))
+
+    cur = new(occ)
+    run(cur, Tmax, 0)
+    T_cur, temps, accs, r = Tmax, [Tmax], [0.0], 1
+    while T_cur > Tmin:
+        cur_E = float(np.mean(cur["E"]))
+        nw = new(cur["upd"].get_occupation())
+        trial, acc, found = T_cur / 2.0, 1.0, False
+        while acc > target and trial > Tmin:
+            acc = _accept_probability(cur_E, run(nw, trial, r), T_cur, trial)
+            found = acc <= target
+            trial /= 2.0
+        if not found:
+            break
+        hi, lo = T_cur, trial
+        while True:
+            t = 0.5 * (hi + lo)
+            acc = _accept_probability(cur_E, run(nw, t, r), T_cur, t)
+            if acc > target:
+                hi = t
+            else:
+                lo = t
+            if abs(acc - target) < 0.01 or hi - lo < 1e-4:
+                break
+        temps.append(t)
+        accs.append(acc)
+        cur, T_cur, r = nw, t, r + 1
+    return temps, accs
+
+
+def test_temperature_scheme_search_matches_oracle(gpu):
+    """ParallelTempering without a scheme runs the reference's search on the
+    GPU; the same search on oracle chains with the same streams picks the
+    same temperatures; consecutive replicas meet the acceptance target."""
+    from cemc_amd.calculator import CE
+    from cemc_amd.lattice import SimpleAtoms
+    from cemc_amd.mcmc import Montecarlo
+    from cemc_amd.tempering import ParallelTempering, find_temperature_scheme
+    s = FCCSettings(4, ["Al", "Mg"], {2: 2, 3: 1})
+    eci = seeded_ecis(s, 8, 0.02)
+    occ = random_occupation(s.n_sites, 2, 4)
+    atoms = SimpleAtoms([s.species[v] for v in occ])
+    CE(atoms, s, eci=dict(eci))
+    mc = Montecarlo(atoms, 1000.0, seed=17, uid=40)
+    temps, accs = find_temperature_scheme(mc, 2000.0, 150.0)
+    ref_t, ref_a = _oracle_scheme(s, eci, occ, 0, 17, 40, 2000.0, 150.0)
+    print("scheme", [round(t, 3) for t in temps], flush=True)
+    assert len(temps) >= 3 and len(temps) == len(ref_t)
+    assert np.allclose(temps, ref_t, rtol=0, atol=1e-9) and np.allclose(accs, ref_a, rtol=0, atol=1e-9)
+    assert all(a > b for a, b in zip(temps, temps[1:]))
+    pt = ParallelTempering(mc, Tmax=2000.0, Tmin=150.0, temp_scheme_file="")
+    assert np.allclose(pt.temperature_scheme, temps, rtol=0, atol=1e-9)
+    pt.run(mc_args={"steps": s.n_sites}, num_exchange_cycles=4)
