@@ -62,7 +62,8 @@ class HooksDesc(ctypes.Structure):
         ("fixed_species", ctypes.c_uint32), ("max_attempts", ctypes.c_int32),
         ("allowed", ctypes.POINTER(ctypes.c_uint8)), ("pair_species", ctypes.c_int32 * 2),
         ("pair_members", ctypes.c_int32), ("pair_neighbours", ctypes.POINTER(ctypes.c_int32)),
-        ("bias_species", ctypes.c_int32), ("reserved", ctypes.c_int32), ("bias_table", ctypes.POINTER(ctypes.c_double)),
+        ("bias_species", ctypes.c_int32), ("recycle_waste", ctypes.c_int32),
+        ("bias_table", ctypes.POINTER(ctypes.c_double)),
     ]
 
 

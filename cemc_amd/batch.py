@@ -206,14 +206,16 @@ class ChainBatch:
 
     # ---------------------------------------------- constraints and bias
     def set_hooks(self, chain, fixed_species=0, max_attempts=10000, allowed=None, pair=None, pair_neighbours=None,
-                  bias_species=-1, bias_table=None):
+                  bias_species=-1, bias_table=None, recycle_waste=False):
         """Constraints and bias of chain `chain` (cemc_batch_set_hooks).
         allowed: uint8 mask per site of the species it may hold; pair: (e1, e2)
         species ids with pair_neighbours an (n_sites, members) int array;
-        bias_table: n_sites + 1 energies over the count of bias_species."""
+        bias_table: n_sites + 1 energies over the count of bias_species;
+        recycle_waste: heat-bath acceptance and recycled samples."""
         d = _capi.HooksDesc()
         d.fixed_species = int(fixed_species)
         d.max_attempts = int(max_attempts)
+        d.recycle_waste = int(bool(recycle_waste))
         keep = []
         n = int(self.n_sites[chain])
         if allowed is not None:

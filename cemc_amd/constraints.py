@@ -217,9 +217,10 @@ class ConcentrationBias(BiasPotential):
         return species.index(self.symbol), self.get(np.arange(n + 1) / n)
 
 
-def hooks_for(settings, species, constraints, biases, max_attempts):
-    """Merge constraints and biases into the arguments of ChainBatch.set_hooks."""
-    out = {"fixed_species": 0, "max_attempts": int(max_attempts)}
+def hooks_for(settings, species, constraints, biases, max_attempts, recycle_waste=False):
+    """Merge constraints, biases and the waste-recycling flag into the
+    arguments of ChainBatch.set_hooks."""
+    out = {"fixed_species": 0, "max_attempts": int(max_attempts), "recycle_waste": bool(recycle_waste)}
     n = settings.n_sites
     allowed = None
     for c in constraints:

@@ -202,7 +202,9 @@ struct HookDev {
   const double* bias_tab;    // n_sites + 1 entries: bias energy at each count
   double bias_energy;        // bias of the current state (part of ChainState::energy)
   int32_t status;            // 1: a move found no valid proposal within max_attempts
-  int32_t pad1;
+  int32_t recycle;           // 1: waste recycling (Frenkel, PNAS 101, 17571 (2004)): heat-bath
+                             // acceptance and samples weighting current and proposed state
+                             // (montecarlo.py:937-948, 805-811; util.py:41-63)
   uint64_t failed_move;      // Philox counter of that move
 };
 

@@ -2094,6 +2094,7 @@ int cemc_batch_set_hooks(cemc_batch* b, int32_t chain, const cemc_hooks_desc* d)
       v.max_attempts = d->max_attempts;
       v.pair_e1 = v.pair_e2 = -1;
       v.bias_species = -1;
+      v.recycle = d->recycle_waste ? 1 : 0;
       if (d->allowed) {
         for (int i = 0; i < N; ++i) require((d->allowed[i] >> S) == 0, "allowed-species mask names unknown species");
         h.d_allowed = upload(std::vector<uint8_t>(d->allowed, d->allowed + N));

@@ -529,6 +529,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
   [[maybe_unused]] double hk_benergy = 0.0;
   [[maybe_unused]] bool hk_failed = false;
   [[maybe_unused]] uint64_t hk_fail_move = 0;
+  [[maybe_unused]] bool hk_recycle = false;
   if constexpr (kHooks) {
     const HookDev* hk = hooks + chain;
     hk_fixed = hk->fixed_species;
@@ -543,6 +544,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
     hk_bcnt = hk->bias_count;
     hk_btab = hk->bias_tab;
     hk_benergy = hk->bias_energy;
+    hk_recycle = hk->recycle != 0;
   }
   if constexpr (kK == 1) {
   for (int64_t blk0 = 0; blk0 < nmov; blk0 += 64) {
@@ -730,6 +732,21 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
       const double x = -dE * inv_kT;
       const double margin = 1e-12 * (1.0 + fabs(x));
       int verdict = (dE < 0.0 || lu < x - margin) ? 1 : (lu > x + margin ? 0 : 2);
+      [[maybe_unused]] double rw0 = 0.0, rw1 = 0.0, re0 = 0.0, rs0 = 0.0;
+      if constexpr (kHooks) {
+        if (hk_recycle) {
+          // heat-bath choice between the current and the proposed state with
+          // weights w = exp(-(E - min E) / kT): the proposed one when
+          // u > w0 / (w0 + w1) (util.py:50-63, montecarlo.py:937-948)
+          const double e0 = energy, e1 = energy + dE;
+          const double emin = e0 < e1 ? e0 : e1;
+          rw0 = exp(-(inv_kT * (e0 - emin)));
+          rw1 = exp(-(inv_kT * (e1 - emin)));
+          verdict = int(unit_double(u_bits) > rw0 / (rw0 + rw1));
+          re0 = e0;
+          rs0 = sing;
+        }
+      }
       verdict = __builtin_amdgcn_readfirstlane(verdict);
       if (verdict == 2)
         verdict = __builtin_amdgcn_readfirstlane(
@@ -765,7 +782,25 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         if constexpr (!kCanonical) sing += dsing;
         acc_bits |= uint64_t(1) << l;
       }
-      ++run;
+      bool counted = false;
+      if constexpr (kHooks) {
+        if (hk_recycle) {
+          // this move's sample weights the current and the proposed state
+          // (montecarlo.py:805-811; singlets as mc_observers.py:231-252: the
+          // state before the move and the state after it)
+          const double inv_w = 1.0 / (rw0 + rw1);
+          const double a = re0 - e_ref, b = re0 + dE - e_ref;
+          sum_de += (a * rw0 + b * rw1) * inv_w;
+          sum_de2 += (a * a * rw0 + b * b * rw1) * inv_w;
+          if constexpr (!kCanonical) {
+            sum_s += (rs0 * rw0 + sing * rw1) * inv_w;
+            sum_s2 += (rs0 * rs0 * rw0 + sing * sing * rw1) * inv_w;
+            sum_se += (rs0 * a * rw0 + sing * b * rw1) * inv_w;
+          }
+          counted = true;
+        }
+      }
+      if (!counted) ++run;
       // this wave's own commit precedes its next gathers in program order
       // (LDS operations of one wave complete in order, so no wait is needed);
       // other waves are ordered by the barrier in the next move's group_sum

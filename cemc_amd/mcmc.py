@@ -25,8 +25,11 @@ The energy-bias probe of the reference (montecarlo.py:178-217) keeps its
 squared sums well conditioned; the device sums here are taken relative to
 the energy at the last reset for the same purpose.
 
-Not carried over: waste recycling, constraints, bias potentials and Python
-observers called per move.
+Constraints and bias potentials (``add_constraint``, ``add_bias``; see
+cemc_amd.constraints) and waste recycling (``recycle_waste=True``:
+heat-bath acceptance, each move's sample weighting the current and the
+proposed state, montecarlo.py:805-811, 937-948) run on the device, one wave
+and one move at a time.  Not carried over: Python observers called per move.
 """
 from __future__ import annotations
 
@@ -129,8 +132,6 @@ class Montecarlo:
         calc = atoms.get_calculator() if hasattr(atoms, "get_calculator") else getattr(atoms, "calc", None)
         if calc is None or not hasattr(calc, "updater"):
             raise ValueError("the atoms object needs a cemc_amd CE calculator attached")
-        if recycle_waste:
-            raise NotImplementedError("waste recycling is not available on the GPU path")
         if max_constraint_attempts <= 0:
             raise ValueError("Max. constraint attempts has to be > 0!")
         self.name = "MonteCarlo"
@@ -151,6 +152,7 @@ class Montecarlo:
         self._chem_pot_vec = None
         self.correlation_info = None
         self.converged = False
+        self.recycle_waste = bool(recycle_waste)
         self.constraints = []
         self.bias_potentials = []
         self.max_allowed_constraint_pass_attempts = int(max_constraint_attempts)
@@ -184,9 +186,10 @@ class Montecarlo:
         self._batch = ChainBatch([self.calc.model], [chain])
         self._batch_occ = occ.copy()
         self._batch_eci = eci
-        if self.constraints or self.bias_potentials:
+        if self.constraints or self.bias_potentials or self.recycle_waste:
             self._batch.set_hooks(0, **hooks_for(self.calc.BC, self.calc.model.species, self.constraints,
-                                                 self.bias_potentials, self.max_allowed_constraint_pass_attempts))
+                                                 self.bias_potentials, self.max_allowed_constraint_pass_attempts,
+                                                 recycle_waste=self.recycle_waste))
         return self._batch
 
     def _sync_back(self):

@@ -272,7 +272,9 @@ typedef struct cemc_hooks_desc {
   const int32_t* pair_neighbours; /* n_sites x pair_members member sites; symmetric lists */
   int32_t bias_species;    /* bias on the count of this species (a SampledBiasPotential over a concentration,
                               bias_potential.py:52-140); -1: none */
-  int32_t reserved;
+  int32_t recycle_waste;   /* 1: waste recycling (montecarlo.py:937-948, 805-811; util.py:41-63):
+                              heat-bath choice between the current and the proposed state, and each
+                              move's sample weights both by exp(-E/kT) */
   const double* bias_table; /* n_sites + 1 entries: bias energy (eV) at each count of bias_species */
 } cemc_hooks_desc;
 

@@ -487,6 +487,12 @@ static int run_impl(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, u
     for (int i = 0; i < N; ++i) bias_cnt += c->occ[i] == h->bias_species;
   if (h) h->failed_move = -1;
   const double kT = temperature * CEO_KB;
+  const int recycle = h && h->recycle;
+  const int B = m->n_bfs;
+  int sing_idx[8] = {0};
+  double s0[8] = {0.0};
+  for (int k = 0; k < m->n_cf; ++k)
+    if (m->cf_kind[k] == 1 && m->cf_dec[k] < 8) sing_idx[m->cf_dec[k]] = k;
   /* the current energy carries the bias of the state (montecarlo.py:279-282) */
   double e_cur = ceo_get_energy(c);
   if (h && h->bias_species >= 0) e_cur += h->bias_table[bias_cnt];
@@ -533,6 +539,11 @@ static int run_impl(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, u
     }
     const double u = unit_double(w[3]);
     double e_new = 0.0;
+    if (recycle) {
+      /* singlets before the move (sgc_montecarlo.py:69) */
+      const double* cf0 = ring_slot(c, current_slot(c));
+      for (int d = 0; d < B && d < 8; ++d) s0[d] = cf0[sing_idx[d]];
+    }
     rc = calculate_impl(c, n_ch, sites, olds, news, &e_new);
     if (rc != CEO_OK) break;
     int dcnt = 0;
@@ -541,7 +552,18 @@ static int run_impl(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, u
       e_new += h->bias_table[bias_cnt + dcnt];
     }
     int accept;
-    if (e_new < e_cur) {
+    double w0 = 0.0, w1 = 0.0;
+    const double e_before = e_cur;
+    if (recycle) {
+      /* util.py:41-63: w = exp(-beta (E - min E)) over the current and the
+       * proposed state; the new state is drawn from w / sum(w) by
+       * searchsorted(cumsum, u), i.e. the proposed one when u > w0/(w0+w1) */
+      const double beta = 1.0 / (CEO_KB * temperature);
+      const double emin = e_cur < e_new ? e_cur : e_new;
+      w0 = exp(-beta * (e_cur - emin));
+      w1 = exp(-beta * (e_new - emin));
+      accept = u > w0 / (w0 + w1);
+    } else if (e_new < e_cur) {
       accept = 1;
     } else {
       const double energy_diff = e_new - e_cur;
@@ -560,6 +582,22 @@ static int run_impl(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, u
       if (accept_bits) accept_bits[t >> 3] |= (uint8_t)(1u << (t & 7));
     } else {
       undo_impl(c);
+    }
+    if (recycle && h->rec_sums) {
+      /* montecarlo.py:805-811 and mc_observers.py:231-252: energies of the
+       * current and the proposed state; singlets before and after the move */
+      double* r = h->rec_sums;
+      const double W = w0 + w1;
+      const double* cf1 = ring_slot(c, current_slot(c));
+      r[0] += 1.0;
+      r[1] += (e_before * w0 + e_new * w1) / W;
+      r[2] += (e_before * e_before * w0 + e_new * e_new * w1) / W;
+      for (int d = 0; d < B && d < 8; ++d) {
+        const double s1 = cf1[sing_idx[d]];
+        r[3 + d] += (s0[d] * w0 + s1 * w1) / W;
+        r[3 + B + d] += (s0[d] * s0[d] * w0 + s1 * s1 * w1) / W;
+        r[3 + 2 * B + d] += (s0[d] * e_before * w0 + s1 * e_new * w1) / W;
+      }
     }
     if (e_trace && trace_every > 0 && ((t + 1) % trace_every) == 0) e_trace[(t + 1) / trace_every - 1] = e_cur;
   }

@@ -95,6 +95,10 @@ typedef struct ceo_hooks {
   int32_t bias_species;     /* -1: no bias */
   const double* bias_table; /* n_sites + 1 entries */
   int64_t failed_move;      /* out: -1 or the Philox counter of the failed move */
+  int32_t recycle;          /* waste recycling (montecarlo.py:937-948, 805-811; util.py:41-63) */
+  int32_t pad;
+  double* rec_sums;         /* out (recycle): n, sum E, sum E^2, then per singlet d: sum s_d,
+                               sum s_d^2, sum s_d E (n_bfs entries each), accumulated */
 } ceo_hooks;
 
 int ceo_run_hooked(ceo_chain* c, int kind, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,

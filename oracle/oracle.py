@@ -79,7 +79,8 @@ class Hooks(ctypes.Structure):
                 ("allowed", ctypes.POINTER(ctypes.c_uint8)), ("pair_e1", ctypes.c_int32),
                 ("pair_e2", ctypes.c_int32), ("pair_nm", ctypes.c_int32), ("pair_nb", ctypes.POINTER(ctypes.c_int32)),
                 ("bias_species", ctypes.c_int32), ("bias_table", ctypes.POINTER(ctypes.c_double)),
-                ("failed_move", ctypes.c_int64)]
+                ("failed_move", ctypes.c_int64), ("recycle", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("rec_sums", ctypes.POINTER(ctypes.c_double))]
 
 
 class WLState(ctypes.Structure):
@@ -363,10 +364,11 @@ def wl_run(updater, wl, seed, chain_uid, n_checks, check_every, move_begin=0):
 
 def run_hooked(updater, kind, seed, chain_uid, n_moves, temperature, fixed_species=0, max_attempts=10000,
                allowed=None, pair=None, pair_neighbours=None, bias_species=-1, bias_table=None, move_begin=0,
-               trace_every=0):
+               trace_every=0, recycle_sums=None):
     """Fed-stream run with constraints and bias; returns (accept bits, energy
     trace, n_accept, failed_move or None).  pair: (e1, e2) species ids with
-    pair_neighbours an (n_sites, n_members) array."""
+    pair_neighbours an (n_sites, n_members) array.  recycle_sums: a float64
+    array of 3 + 3 n_bfs accumulators switches waste recycling on."""
     lib = load_library()
     keep = []
     h = Hooks()
@@ -382,6 +384,11 @@ def run_hooked(updater, kind, seed, chain_uid, n_moves, temperature, fixed_speci
         keep.append(nb)
         h.pair_nm = nb.shape[1]
         h.pair_nb = nb.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    if recycle_sums is not None:
+        if recycle_sums.dtype != np.float64 or not recycle_sums.flags["C_CONTIGUOUS"]:
+            raise ValueError("recycle_sums must be a contiguous float64 array")
+        h.recycle = 1
+        h.rec_sums = _dp(recycle_sums)
     h.bias_species = int(bias_species)
     if bias_species >= 0:
         bt = np.ascontiguousarray(bias_table, dtype=np.float64)
