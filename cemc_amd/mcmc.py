@@ -401,16 +401,28 @@ class Montecarlo:
             q["energy"] = th["mean_energy"]
             q["heat_capacity"] = th["var_energy"] / (KB_EV * self._T ** 2)
             q["energy_var"] = th["var_energy"]
+            q["energy_std"] = float(np.sqrt(max(self._get_var_average_energy(th), 0.0)))
         else:
             q["energy"] = self.current_energy
             q["heat_capacity"] = 0.0
             q["energy_var"] = 0.0
+            q["energy_std"] = 0.0
         q["temperature"] = self._T
         q["accept_rate"] = self.num_accepted / self.current_step if self.current_step else 0.0
         n = len(self.atoms)
         for key, value in self.count_atoms().items():
             q["{}_conc".format(key)] = float(value) / n
+        q.update(self.meta_info)
         return q
+
+    @property
+    def meta_info(self):
+        """Timestamp and Python version (montecarlo.py:846-859)."""
+        import datetime
+        import sys
+        v = sys.version_info
+        return {"timestamp": datetime.datetime.now().strftime("%Y-%m-%d %H:%M:%S"),
+                "python_version": "{}.{}.{}".format(v.major, v.minor, v.micro)}
 
 
 class SGCMonteCarlo(Montecarlo):
@@ -536,8 +548,16 @@ class SGCMonteCarlo(Montecarlo):
             q["singlet_{}".format(name)] = float(th["singlet"][k])
             q["var_singlet_{}".format(name)] = float(th["var_singlet"][k])
             q["mu_{}".format(name)] = float(mu[k])
+        q.update(self.meta_info)
         try:
             q.update(self.singlet2composition(th["singlet"]))
         except np.linalg.LinAlgError:
             pass
         return q
+
+    def reset_ecis(self):
+        """sgc_montecarlo.py:276-281.  The calculator here always holds the
+        plain ECIs (the chemical potentials live in the chain's parameters
+        only), so there is nothing to undo; ``get_thermodynamic``'s
+        ``reset_ecis`` argument is accepted for the same reason."""
+        return None

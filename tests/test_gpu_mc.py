@@ -83,6 +83,9 @@ def test_montecarlo_follows_oracle_and_averages(gpu):
     th = mc.get_thermodynamic()
     assert abs(th["energy"] - trace.mean()) < 1e-9
     assert abs(th["heat_capacity"] - trace.var() / (KB_EV * T ** 2)) < 1e-6 * max(1.0, th["heat_capacity"])
+    # no correlation time was estimated (equil=False): var / n, montecarlo.py:417-447
+    assert abs(th["energy_std"] - np.sqrt(trace.var() / n)) < 1e-6 * np.sqrt(trace.var() / n) + 1e-12
+    assert {"timestamp", "python_version", "temperature"} <= set(th)
     # a second run continues the move stream
     mc.runMC(steps=n, equil=False)
     _, trace2, n_acc2 = ora.run(0, 1234, 5, n, T, move_begin=n, trace_every=1)
