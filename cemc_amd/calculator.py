@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import numpy as np
 
+from .linear_vib import LinearVibCorrection
 from .updater import PyCEUpdater
 
 
@@ -219,5 +220,52 @@ class CE:
                              "Concentrations: {}".format(x))
         return {s: float(x[i]) for i, s in enumerate(species)}
 
+    @property
+    def linear_vib_correction(self):
+        return self._linear_vib_correction
+
+    @linear_vib_correction.setter
+    def linear_vib_correction(self, linvib):
+        """ce_calculator.py:269-287: a previous correction's terms leave the
+        ECIs first; the updater gets the new coefficients (the ECIs themselves
+        change only in ``include_linvib_in_ecis``)."""
+        if not isinstance(linvib, LinearVibCorrection):
+            raise TypeError("Linear vib correction has to be of type LinearVibCorrection!")
+        if self._linear_vib_correction is not None:
+            orig = self._linear_vib_correction.reset(self.eci)
+            if orig is not None:
+                self.eci = orig
+            self.update_ecis(self.eci)
+        self._linear_vib_correction = linvib
+        self.updater.add_linear_vib_correction(linvib.eci_per_kbT)
+
+    def include_linvib_in_ecis(self, T):
+        """Add v_k kB T to the ECIs at temperature T (ce_calculator.py:289-301)."""
+        if self._linear_vib_correction is None:
+            return
+        self._linear_vib_correction.include(self.eci, T)
+        self.update_ecis(self.eci)
+
     def vib_energy(self, T):
+        """Vibrational energy per site at the current CFs (ce_calculator.py:303-317)."""
         return self.updater.vib_energy(T)
+
+    def get_full_cluster_names(self, cnames):
+        """Names with their decoration suffix for cluster prefixes; one-body
+        names are kept and listed first (ce_calculator.py:248-267)."""
+        full_names = list(self.cf.keys())
+        only_prefix = [name.rpartition("_")[0] for name in full_names]
+        out = [name for name in cnames if name.startswith("c1")]
+        for name in cnames:
+            if not name.startswith("c1"):
+                out.append(full_names[only_prefix.index(name)])
+        return out
+
+    def set_singlets(self, singlets):
+        """Composition that gives these singlets, as close as the cell
+        allows (ce_calculator.py:521-532)."""
+        self.set_composition(self.singlet2comp(singlets))
+
+    def set_num_threads(self, num_threads):
+        """ce_calculator.py:610-617; the updater's work runs on GPU lanes."""
+        self.updater.set_num_threads(num_threads)

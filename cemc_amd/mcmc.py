@@ -169,6 +169,60 @@ class Montecarlo:
         if self._batch is not None:
             self._batch.set_temperature(0, self._T)
 
+    @property
+    def linear_vib_correction(self):
+        return self.calc.linear_vib_correction
+
+    @linear_vib_correction.setter
+    def linear_vib_correction(self, linvib):
+        """montecarlo.py:147-154.  The averages leave the vibrational energy
+        out exactly when it depends on the chain's sums only: terms on the
+        empty cluster and the singlets (in the semi-grand-canonical ensemble,
+        on at most one singlet, whose variance and covariance with the energy
+        the device tracks)."""
+        names = self.calc.model.names
+        singl = [k for k in linvib.eci_per_kbT if k.startswith("c1_")]
+        for key in linvib.eci_per_kbT:
+            if key not in names:
+                raise KeyError("The cluster {} is not in the original ECIs!".format(key))
+            if key != "c0" and not key.startswith("c1_"):
+                raise NotImplementedError("vibrational terms on cluster %s: the samplers average the energy "
+                                          "without vibrations for the empty cluster and the singlets only" % key)
+        if self.kind == SGC and len(singl) > 1:
+            raise NotImplementedError("vibrational terms on more than one singlet in the semi-grand-canonical "
+                                      "ensemble")
+        self.calc.linear_vib_correction = linvib
+
+    def _include_vib(self):
+        """montecarlo.py:366-370."""
+        self.calc.include_linvib_in_ecis(self._T)
+
+    def current_energy_without_vib(self):
+        """montecarlo.py:452-459."""
+        return self.current_energy - self.calc.vib_energy(self._T) * len(self.atoms)
+
+    def _vib_shift(self, th):
+        """Mean and variance corrections that take the vibrational energy
+        N kB T (v_c0 + sum_d v_d c1_d) out of the chain's energy averages:
+        (mean of that energy, variance change)."""
+        lv = self.calc.linear_vib_correction
+        if lv is None or not lv.eci_per_kbT:
+            return 0.0, 0.0
+        from .linear_vib import KB_VIB
+        n = len(self.atoms)
+        names = [k for k in self.calc.model.names if k.startswith("c1_")]
+        scale = n * KB_VIB * self._T
+        mean = scale * lv.eci_per_kbT.get("c0", 0.0)
+        dvar = 0.0
+        for d, name in enumerate(names):
+            a = scale * lv.eci_per_kbT.get(name, 0.0)
+            if a == 0.0:
+                continue
+            mean += a * th["singlet"][d]
+            if self.kind == SGC:
+                dvar += a * a * th["var_singlet"][d] - 2.0 * a * th["cov_singlet_energy"][d]
+        return mean, dvar
+
     # -------------------------------------------------------------- helpers
     def _chain_extra(self):
         return {}
@@ -355,6 +409,7 @@ class Montecarlo:
         if mode not in ("fixed", "prec"):
             raise ValueError("Mode has to be one of ['fixed', 'prec']")
         self._check_symbols()
+        self._include_vib()
         self.update_current_energy()
         self.correlation_info = None
         if equil:
@@ -398,6 +453,8 @@ class Montecarlo:
         th = self._batch.thermo(0) if self._batch is not None else None
         q = {}
         if th is not None and th["n_samples"] > 0:
+            vib_mean, vib_dvar = self._vib_shift(th)
+            th = dict(th, mean_energy=th["mean_energy"] - vib_mean, var_energy=th["var_energy"] + vib_dvar)
             q["energy"] = th["mean_energy"]
             q["heat_capacity"] = th["var_energy"] / (KB_EV * self._T ** 2)
             q["energy_var"] = th["var_energy"]
@@ -533,6 +590,8 @@ class SGCMonteCarlo(Montecarlo):
     def get_thermodynamic(self, reset_ecis=True):
         """sgc_montecarlo.py:398-448."""
         th = self._batch.thermo(0)
+        vib_mean, vib_dvar = self._vib_shift(th)
+        th = dict(th, mean_energy=th["mean_energy"] - vib_mean, var_energy=th["var_energy"] + vib_dvar)
         N = len(self.atoms)
         names = [n for n in self.calc.model.names if n.startswith("c1_")]
         mu = np.zeros(len(names))
