@@ -175,7 +175,18 @@ class PairConstraint(MCConstraint):
         return np.array(rows, dtype=np.int32)
 
     def __call__(self, system_changes):
-        raise NotImplementedError("PairConstraint is checked on the device")
+        """Host form of the check, for changes made outside a sampler's
+        launch (Montecarlo.insert_symbol): True when, after the changes, no
+        site of elements[0] has a member of elements[1] in the cluster
+        (the negation of elems_in_pair, pair_constraint.cpp:14-46)."""
+        species = list(self.calc.model.species)
+        occ = np.array(self.calc.updater.get_occupation(), dtype=np.int64)
+        for change in system_changes:
+            occ[int(change[0])] = species.index(change[2])
+        if getattr(self, "_nb", None) is None:
+            self._nb = self.neighbours(self.calc.BC)
+        e1, e2 = (species.index(e) for e in self.elements)
+        return not bool(np.any((occ[:, None] == e1) & (occ[self._nb] == e2)))
 
     def device_hooks(self, settings, species):
         e1, e2 = (species.index(e) for e in self.elements)

@@ -29,9 +29,13 @@ Constraints and bias potentials (``add_constraint``, ``add_bias``; see
 cemc_amd.constraints) and waste recycling (``recycle_waste=True``:
 heat-bath acceptance, each move's sample weighting the current and the
 proposed state, montecarlo.py:805-811, 937-948) run on the device, one wave
-and one move at a time.  Not carried over: Python observers called per move.
+and one move at a time.  Observers (``attach``) are called between launches:
+the run pauses at their intervals and each sees the sites changed since its
+previous call.
 """
 from __future__ import annotations
+
+import logging
 
 import numpy as np
 
@@ -155,6 +159,13 @@ class Montecarlo:
         self.recycle_waste = bool(recycle_waste)
         self.constraints = []
         self.bias_potentials = []
+        self.observers = []
+        self.logger = logging.getLogger("cemc_amd.mcmc.%x" % id(self))
+        if logfile:
+            handler = logging.FileHandler(logfile)
+            handler.setFormatter(logging.Formatter("%(asctime)s %(message)s"))
+            self.logger.addHandler(handler)
+            self.logger.setLevel(logging.INFO)
         self.max_allowed_constraint_pass_attempts = int(max_constraint_attempts)
         self.current_energy = calc.get_energy()
 
@@ -268,6 +279,97 @@ class Montecarlo:
             raise TypeError("constraints must be cemc_amd.constraints classes (checked on the device)")
         self.constraints.append(constraint)
         self._drop_batch()
+
+    def log(self, msg, mode="info"):
+        """montecarlo.py:321-332: to the sampler's logger (a file when
+        ``logfile`` was given, else the process's logging setup)."""
+        if mode not in ("info", "warning"):
+            raise ValueError("Mode has to be one of ['info', 'warning']")
+        getattr(self.logger, mode)(msg)
+
+    def _no_constraint_violations(self, system_changes):
+        """Host check of changes against every constraint
+        (montecarlo.py:334-347); trial moves are checked on the device."""
+        return all(c(system_changes) for c in self.constraints)
+
+    def insert_symbol(self, symb, indices):
+        """Put ``symb`` on the given sites (montecarlo.py:218-233)."""
+        for indx in indices:
+            change = (int(indx), self.atoms[int(indx)].symbol, symb)
+            if not self._no_constraint_violations([change]):
+                raise ValueError("The indices given results in an update that violate one or more constraints!")
+            self.calc.update_cf(change)
+        self.calc.clear_history()
+        self.calc.cf = self.calc.updater.get_cf()
+        self.update_current_energy()
+
+    def insert_symbol_random_places(self, symbol, num=1, swap_symbs=()):
+        """Put ``symbol`` on ``num`` random sites holding one of
+        ``swap_symbs`` (all symbols present when empty), skipping changes a
+        constraint forbids, with at most 10 N attempts
+        (montecarlo.py:235-270).  Sites are drawn from the sampler's seed."""
+        rng = np.random.default_rng(None if self.seed is None else [int(self.seed), int(self.uid), 0x1a5e])
+        symbols = [a.symbol for a in self.atoms]
+        swap_symbs = list(swap_symbs) or sorted(set(symbols))
+        inserted = attempts = 0
+        max_attempts = 10 * len(self.atoms)
+        while inserted < num and attempts < max_attempts:
+            attempts += 1
+            old = swap_symbs[int(rng.integers(len(swap_symbs)))]
+            if old == symbol:
+                continue
+            sites = [i for i, s in enumerate(symbols) if s == old]
+            if not sites:
+                continue
+            indx = sites[int(rng.integers(len(sites)))]
+            change = (indx, old, symbol)
+            if not self._no_constraint_violations([change]):
+                continue
+            self.calc.update_cf(change)
+            symbols[indx] = symbol
+            inserted += 1
+        self.calc.clear_history()
+        self.calc.cf = self.calc.updater.get_cf()
+        self.update_current_energy()
+        if attempts == max_attempts:
+            raise RuntimeError("Could insert {} {} atoms!".format(num, symbol))
+
+    def attach(self, obs, interval=1):
+        """Call ``obs(system_changes)`` every ``interval`` moves of a
+        production run (montecarlo.py:403-415, 1032-1037).  The run then
+        pauses at those moves: the lattice is written back to the atoms and
+        the calculator first, and ``system_changes`` lists the sites changed
+        since this observer's previous call as (index, old symbol, new
+        symbol); for interval 1 that is the move's changes when it was
+        accepted and an empty list when it was rejected."""
+        if not callable(obs):
+            raise ValueError("The observer has to be a callable class!")
+        self.observers.append((int(interval), obs))
+
+    def _advance_observed(self, begin, end):
+        """Moves begin..end of the production run, pausing for observers."""
+        if not self.observers:
+            self._advance(end - begin)
+            return
+        species = self.calc.model.species
+        self._ensure_batch()
+        last = [self._batch_occ.copy() for _ in self.observers]
+        cur = begin
+        while cur < end:
+            nxt = min(min((cur // i + 1) * i for i, _ in self.observers), end)
+            self._advance(nxt - cur)
+            cur = nxt
+            due = [k for k, (i, _) in enumerate(self.observers) if cur % i == 0]
+            if not due:
+                continue
+            self._sync_back()
+            occ = self._batch_occ
+            self.current_step = cur
+            for k in due:
+                prev = last[k]
+                changes = [(int(j), species[prev[j]], species[occ[j]]) for j in np.nonzero(prev != occ)[0]]
+                last[k] = occ.copy()
+                self.observers[k][1](changes)
 
     def add_bias(self, potential):
         """Add a bias potential (montecarlo.py:394-401); see
@@ -431,7 +533,7 @@ class Montecarlo:
         done = 0
         self.converged = False
         for s in checks + [steps]:
-            self._advance(s - done)
+            self._advance_observed(done, s)
             done = s
             if s < steps and self._has_converged_prec_mode(prec, prec_confidence, batch.thermo(0)):
                 self.converged = True

@@ -237,3 +237,63 @@ def test_calculator_save_load_and_pickle(gpu, tmp_path):
     data["eci"]["c9_d0000_0_000"] = 1.0
     with pytest.raises(ValueError):
         CE.load_from_dict(data)
+
+
+def test_observers_insert_symbol_and_constraint_checks(gpu):
+    """attach: observers run every `interval` moves and see the lattice
+    changes since their previous call, applied to the atoms; the oracle's
+    state at those moves agrees.  insert_symbol and
+    insert_symbol_random_places change the lattice through update_cf and
+    respect the constraints (montecarlo.py:218-270, 403-415)."""
+    from cemc_amd.constraints import FixedElement, PairConstraint
+    s, eci, occ, atoms = _system(5, ["Al", "Mg"], {2: 3, 3: 1}, 61)
+    calc = CE(atoms, s, eci=dict(eci))
+    T, n = 1000.0, 600
+    mc = Montecarlo(atoms, T, seed=21, uid=4)
+    seen = []
+    shadow = [a.symbol for a in atoms]
+
+    def obs(changes):
+        for i, old, new in changes:
+            assert shadow[i] == old
+            shadow[i] = new
+        assert shadow == [a.symbol for a in atoms]
+        seen.append((mc.current_step, calc.get_energy()))
+
+    calls3 = []
+    mc.attach(obs, interval=50)
+    mc.attach(lambda ch: calls3.append(len(ch)), interval=1)
+    with pytest.raises(ValueError):
+        mc.attach(3)
+    mc.runMC(steps=n, equil=False)
+    assert [st for st, _ in seen] == list(range(50, n + 1, 50))
+    assert len(calls3) == n and set(calls3) <= {0, 2}
+    ora = _oracle(s, occ, eci)
+    for k, (st, e) in enumerate(seen):
+        ora.run(0, 21, 4, 50, T, move_begin=k * 50)
+        assert abs(ora.get_energy() - e) < TOL * s.n_sites
+    assert sum(calls3) // 2 == mc.num_accepted
+    # insertions
+    mc2 = Montecarlo(atoms, T, seed=3, uid=0)
+    al = [i for i, a in enumerate(atoms) if a.symbol == "Al"][:3]
+    mc2.insert_symbol("Mg", al)
+    assert all(atoms[i].symbol == "Mg" for i in al)
+    assert abs(mc2.current_energy - calc.get_energy()) < 1e-12
+    names = s.cf_names()
+    cf = brute_force_cf(s, calc.updater.get_occupation(), names)
+    assert max(abs(calc.get_cf()[k] - cf[k]) for k in names) < TOL
+    n_mg = sum(a.symbol == "Mg" for a in atoms)
+    mc2.insert_symbol_random_places("Mg", num=5, swap_symbs=["Al"])
+    assert sum(a.symbol == "Mg" for a in atoms) == n_mg + 5
+    mc2.add_constraint(FixedElement("Al"))
+    with pytest.raises(ValueError):
+        mc2.insert_symbol("Mg", [[i for i, a in enumerate(atoms) if a.symbol == "Al"][0]])
+    # the host form of the pair constraint agrees with a direct scan
+    pc = PairConstraint(calc=calc, cluster_name="c2_d0000_0", elements=["Mg", "Mg"])
+    nb = pc.neighbours(s)
+    o = calc.updater.get_occupation()
+    i = int(np.nonzero(o == 0)[0][0])
+    o2 = o.copy()
+    o2[i] = 1
+    want = not np.any((o2[:, None] == 1) & (o2[nb] == 1))
+    assert pc([(i, "Al", "Mg")]) == want
