@@ -572,7 +572,17 @@ class Montecarlo:
         for key, value in self.count_atoms().items():
             q["{}_conc".format(key)] = float(value) / n
         q.update(self.meta_info)
+        q.update(self._observer_averages())
         return q
+
+    def _observer_averages(self):
+        """Averages of the attached observers that report them
+        (montecarlo.py:885-887)."""
+        out = {}
+        for _, obs in self.observers:
+            if hasattr(obs, "get_averages"):
+                out.update(obs.get_averages())
+        return out
 
     @property
     def meta_info(self):

@@ -297,3 +297,41 @@ def test_observers_insert_symbol_and_constraint_checks(gpu):
     o2[i] = 1
     want = not np.any((o2[:, None] == 1) & (o2[nb] == 1))
     assert pc([(i, "Al", "Mg")]) == want
+
+
+def test_observer_classes(gpu):
+    """CorrelationFunctionTracker, PairCorrelationObserver,
+    LowestEnergyStructure and EnergyEvolution record the states the oracle
+    passes through at the same moves; the pair averages reach
+    get_thermodynamic (mc_observers.py:40-182, 688-706)."""
+    from cemc_amd.observers import (CorrelationFunctionTracker, EnergyEvolution, LowestEnergyStructure,
+                                    PairCorrelationObserver)
+    s, eci, occ, atoms = _system(5, ["Al", "Mg"], {2: 2, 3: 1}, 71)
+    calc = CE(atoms, s, eci=dict(eci))
+    T, n, every = 600.0, 400, 20
+    mc = Montecarlo(atoms, T, seed=13, uid=2)
+    cft, pco = CorrelationFunctionTracker(calc), PairCorrelationObserver(calc)
+    low, evo = LowestEnergyStructure(calc, mc), EnergyEvolution(mc)
+    for o in (cft, pco, low, evo):
+        mc.attach(o, interval=every)
+    mc.runMC(steps=n, equil=False)
+    names = s.cf_names()
+    ora = _oracle(s, occ, eci)
+    energies, cfs = [], []
+    for k in range(n // every):
+        ora.run(0, 13, 2, every, T, move_begin=k * every)
+        energies.append(ora.get_energy())
+        cfs.append(brute_force_cf(s, ora.get_occupation(), names))
+    assert len(evo.energies) == len(cft.cf) == pco.n_entries == n // every
+    assert np.max(np.abs(np.array(evo.energies) - energies)) < TOL * s.n_sites
+    for got, want in zip(cft.cf, cfs):
+        assert max(abs(got[k] - want[k]) for k in names) < TOL
+    pairs = [k for k in names if k.startswith("c2_")]
+    avg = pco.get_averages()
+    for k in pairs:
+        assert abs(avg[k] - np.mean([c[k] for c in cfs])) < TOL
+    assert abs(low.lowest_energy - min(energies)) < TOL * s.n_sites
+    best = int(np.argmin(energies))
+    assert max(abs(low.lowest_energy_cf[k] - cfs[best][k]) for k in names) < TOL
+    th = mc.get_thermodynamic()
+    assert all(abs(th[k] - avg[k]) < 1e-15 for k in pairs)
