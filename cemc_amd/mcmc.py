@@ -575,6 +575,51 @@ class Montecarlo:
         q.update(self._observer_averages())
         return q
 
+    def save(self, fname):
+        """Write the sampler's state as JSON (montecarlo.py:1040-1057 pickles
+        it): the calculator's backup, temperature, seed, uid, the number of
+        moves made (a loaded sampler continues the same move stream), the
+        chemical potentials and the vibrational correction.  Constraints,
+        bias potentials and observers are not stored and are added again
+        after ``load``, as the reference says of those it cannot pickle."""
+        import json
+        lv = self.calc.linear_vib_correction
+        state = {"class": type(self).__name__, "calc": self.calc.backup_dict(), "temperature": self._T,
+                 "seed": self.seed, "uid": self.uid, "moves": int(self._moves),
+                 "recycle_waste": self.recycle_waste,
+                 "max_constraint_attempts": self.max_allowed_constraint_pass_attempts,
+                 "chem_pot": getattr(self, "_chemical_potential", None),
+                 "linear_vib": None if lv is None else {"eci_per_kbT": lv.eci_per_kbT, "included": lv.vibs_included,
+                                                        "temperature": lv.temperature}}
+        with open(fname, "w") as f:
+            json.dump(state, f, indent=2)
+
+    @staticmethod
+    def load(fname):
+        """Rebuild a sampler written by ``save`` (montecarlo.py:1059-1075)."""
+        import json
+        from .calculator import CE
+        from .linear_vib import LinearVibCorrection
+        with open(fname) as f:
+            st = json.load(f)
+        calc = CE.load_from_dict(st["calc"])
+        kw = dict(seed=st["seed"], uid=st["uid"], recycle_waste=st["recycle_waste"],
+                  max_constraint_attempts=st["max_constraint_attempts"])
+        if st["class"] == "SGCMonteCarlo":
+            mc = SGCMonteCarlo(calc.atoms, st["temperature"], symbols=list(calc.model.species), **kw)
+            if st["chem_pot"] is not None:
+                mc.chemical_potential = st["chem_pot"]
+        else:
+            mc = Montecarlo(calc.atoms, st["temperature"], **kw)
+        if st["linear_vib"] is not None:
+            lv = LinearVibCorrection(st["linear_vib"]["eci_per_kbT"])
+            lv.vibs_included = bool(st["linear_vib"]["included"])
+            lv.temperature = float(st["linear_vib"]["temperature"])
+            calc._linear_vib_correction = lv
+            calc.updater.add_linear_vib_correction(lv.eci_per_kbT)
+        mc._moves = int(st["moves"])
+        return mc
+
     def _observer_averages(self):
         """Averages of the attached observers that report them
         (montecarlo.py:885-887)."""

@@ -335,3 +335,29 @@ def test_observer_classes(gpu):
     assert max(abs(low.lowest_energy_cf[k] - cfs[best][k]) for k in names) < TOL
     th = mc.get_thermodynamic()
     assert all(abs(th[k] - avg[k]) < 1e-15 for k in pairs)
+
+
+def test_sampler_save_load_continues_the_move_stream(gpu, tmp_path):
+    """A sampler written by save() and rebuilt by load() continues the same
+    chain: after more moves its lattice and energy equal those of the
+    sampler that never stopped (SGC, chemical potential, vibrational
+    correction; montecarlo.py:1040-1075)."""
+    from cemc_amd.linear_vib import LinearVibCorrection
+    s, eci, occ, atoms = _system(5, ["Al", "Mg"], {2: 2, 3: 1}, 81)
+    calc = CE(atoms, s, eci=dict(eci))
+    mu = {"c1_0": 0.02}
+    mc = SGCMonteCarlo(atoms, 700.0, symbols=["Al", "Mg"], seed=17, uid=6)
+    mc.linear_vib_correction = LinearVibCorrection({"c0": 0.1, "c1_0": -0.3})
+    mc.runMC(steps=400, chem_potential=mu, equil=False)
+    fname = str(tmp_path / "mc.json")
+    mc.save(fname)
+    loaded = SGCMonteCarlo.load(fname)
+    assert type(loaded) is SGCMonteCarlo and loaded.chemical_potential == mu
+    assert loaded.calc.eci == calc.eci
+    mc.runMC(steps=400, equil=False)
+    loaded.runMC(steps=400, equil=False)
+    assert np.array_equal(loaded.calc.updater.get_occupation(), calc.updater.get_occupation())
+    assert abs(loaded.current_energy - mc.current_energy) < TOL * s.n_sites
+    assert loaded.num_accepted == mc.num_accepted
+    q1, q2 = mc.get_thermodynamic(), loaded.get_thermodynamic()
+    assert abs(q1["sgc_energy"] - q2["sgc_energy"]) < 1e-9
