@@ -337,6 +337,41 @@ def test_observer_classes(gpu):
     assert all(abs(th[k] - avg[k]) < 1e-15 for k in pairs)
 
 
+def test_site_order_histogram_and_backup_observers(gpu, tmp_path):
+    """SiteOrderParameter counts the sites that differ from the start,
+    EnergyHistogram bins the energies of the calls, MCBackup writes a
+    sampler that load() continues (mc_observers.py:614-805)."""
+    from cemc_amd.observers import EnergyEvolution, EnergyHistogram, MCBackup, SiteOrderParameter
+    s, eci, occ, atoms = _system(5, ["Al", "Mg"], {2: 2, 3: 1}, 91)
+    calc = CE(atoms, s, eci=dict(eci))
+    mc = Montecarlo(atoms, 900.0, seed=23, uid=1)
+    sop, evo = SiteOrderParameter(atoms), EnergyEvolution(mc)
+    hist = EnergyHistogram(mc, buffer_size=10, n_bins=5)
+    fname = str(tmp_path / "backup.json")
+    for o in (sop, evo, hist):
+        mc.attach(o, interval=10)
+    mc.attach(MCBackup(mc, backup_file=fname), interval=100)
+    mc.runMC(steps=300, equil=False)
+    ora = _oracle(s, occ, eci)
+    counts = []
+    for k in range(30):
+        ora.run(0, 23, 1, 10, 900.0, move_begin=k * 10)
+        counts.append(int(np.count_nonzero(ora.get_occupation() != occ)))
+    th = mc.get_thermodynamic()
+    assert abs(th["site_order_average"] - np.mean(counts)) < 1e-12
+    assert abs(th["site_order_std"] - np.std(counts)) < 1e-9
+    e = np.array(evo.energies)
+    assert hist.Emin == e[:10].min() and hist.Emax == e[:10].max()
+    inside = e[10:][(e[10:] >= hist.Emin) & (e[10:] <= hist.Emax)]
+    assert hist.histogram.sum() == 10 + inside.size and hist.n_outside == 20 - inside.size
+    want = np.zeros(5)
+    for x in np.concatenate([e[:10], inside]):
+        want[int((x - hist.Emin) * 4 / (hist.Emax - hist.Emin))] += 1
+    assert np.array_equal(hist.histogram, want)
+    loaded = Montecarlo.load(fname)
+    assert np.array_equal(loaded.calc.updater.get_occupation(), calc.updater.get_occupation())
+
+
 def test_sampler_save_load_continues_the_move_stream(gpu, tmp_path):
     """A sampler written by save() and rebuilt by load() continues the same
     chain: after more moves its lattice and energy equal those of the
