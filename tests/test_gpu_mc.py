@@ -396,3 +396,43 @@ def test_sampler_save_load_continues_the_move_stream(gpu, tmp_path):
     assert loaded.num_accepted == mc.num_accepted
     q1, q2 = mc.get_thermodynamic(), loaded.get_thermodynamic()
     assert abs(q1["sgc_energy"] - q2["sgc_energy"]) < 1e-9
+
+
+def test_annealing_and_parameter_sweep_drivers(gpu, tmp_path):
+    """SimulatedAnnealingCanonical keeps the lowest energy the oracle's
+    chains meet at the same temperatures and moves; MCParameterSweep runs one
+    SGC run per parameter set and appends the results (sa_canonical.py,
+    mc_parameter_sweep.py)."""
+    from cemc_amd.drivers import MCParameterSweep, SimulatedAnnealingCanonical
+    s, eci, occ, atoms = _system(4, ["Al", "Mg"], {2: 2, 3: 1}, 101)
+    CE(atoms, s, eci=dict(eci))
+    temps = [1200.0, 600.0, 200.0]
+    sa = SimulatedAnnealingCanonical(atoms, temps, seed=40, uid=2)
+    e_min = sa.run(steps_per_temp=150)
+    # each temperature starts a new sampler, whose swap tracker is built
+    # from the lattice (montecarlo.py _build_atoms_list): a new oracle too
+    cur = occ.copy()
+    want = np.inf
+    for k, T in enumerate(temps):
+        ora = _oracle(s, cur, eci)
+        _, trace, _ = ora.run(0, 40 + k, 2, 150, T, trace_every=1)
+        want = min(want, trace.min())
+        cur = ora.get_occupation()
+    assert abs(e_min - want) < TOL * s.n_sites
+    with pytest.raises(NotImplementedError):
+        SimulatedAnnealingCanonical(atoms, temps, mode="maximize")
+    mc = SGCMonteCarlo(atoms, 800.0, symbols=["Al", "Mg"], seed=3, uid=0)
+    params = [{"temperature": 900.0, "chemical_potential": {"c1_0": 0.01}},
+              {"temperature": 500.0, "chemical_potential": {"c1_0": -0.02}}]
+    out = str(tmp_path / "sweep.npz")
+    sweep = MCParameterSweep(params, mc, nsteps=500, outfile=out,
+                             equil_params={"window_length": 200, "maxiter": 3})
+    data = sweep.run()
+    assert [d["temperature"] for d in data] == [900.0, 500.0]
+    assert [d["mu_c1_0"] for d in data] == [0.01, -0.02]
+    sweep.run()
+    with np.load(out) as f:
+        assert f["temperature"].tolist() == [900.0, 500.0, 900.0, 500.0]
+        assert f["sgc_energy"].shape == (4,)
+    with pytest.raises(ValueError):
+        MCParameterSweep([{"temperature": 300.0}], mc)
