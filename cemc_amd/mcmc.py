@@ -119,6 +119,10 @@ def prec_check_steps(check_every, total):
     return out
 
 
+# temperature at which every trial move is accepted (see Montecarlo._forced_first)
+_ACCEPT_ALL_T = 1e300
+
+
 class Montecarlo:
     """Canonical (fixed composition) Metropolis sampling with swap moves.
 
@@ -167,6 +171,10 @@ class Montecarlo:
             self.logger.addHandler(handler)
             self.logger.setLevel(logging.INFO)
         self.max_allowed_constraint_pass_attempts = int(max_constraint_attempts)
+        self.accept_first_trial_move_after_reset = bool(accept_first_trial_move_after_reset)
+        if self.accept_first_trial_move_after_reset and self.recycle_waste:
+            raise NotImplementedError("accept_first_trial_move_after_reset with recycle_waste")
+        self.is_first = self.accept_first_trial_move_after_reset
         self.current_energy = calc.get_energy()
 
     # ------------------------------------------------------------ properties
@@ -398,6 +406,8 @@ class Montecarlo:
         self.num_accepted = 0
         if self._batch is not None:
             self._batch.reset_stats(0)
+        if self.accept_first_trial_move_after_reset:
+            self.is_first = True
 
     def update_current_energy(self):
         self.current_energy = self.calc.get_energy()
@@ -406,9 +416,29 @@ class Montecarlo:
         """Replace every symbol (montecarlo.py:284-295)."""
         self.calc.set_symbols(symbs)
         self.update_current_energy()
+        if self.accept_first_trial_move_after_reset:
+            self.is_first = True
+
+    def _forced_first(self, batch, n):
+        """With accept_first_trial_move_after_reset, the first move after a
+        reset is accepted whatever its energy (montecarlo.py:932-935): it
+        runs at 1e300 K, where |dE / kT| < 2^-53 for any energy change, so
+        exp(-dE/kT) rounds to 1 and every uniform u < 1 accepts.  Returns
+        the moves left."""
+        if n <= 0 or not self.is_first:
+            return n
+        self.is_first = False
+        batch.set_temperature(0, _ACCEPT_ALL_T)
+        try:
+            batch.run(1)
+        finally:
+            batch.set_temperature(0, self._T)
+        self._moves += 1
+        return n - 1
 
     def _advance(self, n):
         batch = self._ensure_batch()
+        n = self._forced_first(batch, int(n))
         try:
             batch.run(int(n))
         except RuntimeError as exc:
@@ -425,9 +455,12 @@ class Montecarlo:
         batch = self._ensure_batch()
         out = np.empty(int(n))
         for i in range(int(n)):
-            batch.run(1)
+            if self.is_first:
+                self._forced_first(batch, 1)
+            else:
+                batch.run(1)
+                self._moves += 1
             out[i] = batch.energies()[0]
-        self._moves += int(n)
         return out
 
     def _estimate_correlation_time(self, window_length=1000, restart=False):

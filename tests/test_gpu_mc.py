@@ -436,3 +436,30 @@ def test_annealing_and_parameter_sweep_drivers(gpu, tmp_path):
         assert f["sgc_energy"].shape == (4,)
     with pytest.raises(ValueError):
         MCParameterSweep([{"temperature": 300.0}], mc)
+
+
+def test_accept_first_trial_move_after_reset(gpu):
+    """The first move after a reset is accepted whatever its energy
+    (montecarlo.py:128-131, 364, 932-935); the chain then follows the oracle
+    stream that accepts move 0 and runs the rest at the temperature."""
+    s, eci, occ, atoms = _system(4, ["Al", "Mg"], {2: 2, 3: 1}, 111)
+    calc = CE(atoms, s, eci=dict(eci))
+    T = 5.0
+    mc = Montecarlo(atoms, T, seed=31, uid=0, accept_first_trial_move_after_reset=True)
+    mc.runMC(steps=40, equil=False)
+    ora = _oracle(s, occ, eci)
+    _, _, a1 = ora.run(0, 31, 0, 1, 1e300)
+    assert a1 == 1
+    _, _, a2 = ora.run(0, 31, 0, 39, T, move_begin=1)
+    assert mc.num_accepted == a1 + a2
+    assert np.array_equal(calc.updater.get_occupation(), ora.get_occupation())
+    # after reset() the next move is forced again
+    mc.reset()
+    assert mc.is_first
+    mc._advance(3)
+    _, _, b1 = ora.run(0, 31, 0, 1, 1e300, move_begin=40)
+    _, _, b2 = ora.run(0, 31, 0, 2, T, move_begin=41)
+    mc._sync_back()
+    assert b1 == 1 and np.array_equal(calc.updater.get_occupation(), ora.get_occupation())
+    with pytest.raises(NotImplementedError):
+        Montecarlo(atoms, T, recycle_waste=True, accept_first_trial_move_after_reset=True)
