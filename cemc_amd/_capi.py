@@ -108,6 +108,7 @@ SIGNATURES = [
     ("cemc_batch_reset_stats", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("cemc_batch_get_thermo", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(Thermo)]),
     ("cemc_batch_get_cf", ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
+    ("cemc_batch_get_cf_noalloc", ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
     ("cemc_batch_set_waves", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("cemc_batch_set_moves_per_round", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("cemc_batch_set_energy_path", ctypes.c_int, [_vp, ctypes.c_int32]),

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <tuple>
@@ -728,6 +729,56 @@ static void compute_energy(const ModelDev* d_models, const std::vector<ChainPara
   dev_free(d_partial);
   dev_free(d_e);
 }
+
+// ============================================================================
+// Scratch of the allocation-free entry point (cemc_batch_get_cf_noalloc):
+// one file-scope device array per GPU, held until the process ends.  Host
+// code never passes the array's name to a kernel or a copy (in host code it
+// is a stand-in, not a device address): it uses the address that
+// hipGetSymbolAddress returns for the current device, looked up once per
+// device on the first call.  Every call on a GPU shares that GPU's array, so
+// calls that overlap on one GPU race.
+constexpr size_t kNoAllocDoubles = size_t(1) << 17;  // 1 MiB
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;  // device address of `scratch`, per device
+
+double* scratch_on_current_device() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr.size() <= size_t(dev)) g_scratch_ptr.resize(size_t(dev) + 1, nullptr);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (p == nullptr) throw DeviceError("hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+// Offsets (in doubles, 256-byte aligned) inside the scratch array: the chain
+// id (int32) at 0, the correlation functions, the energy, then the per-block
+// partial sums of cf_partial_kernel.
+struct NoAllocLayout {
+  size_t cf, energy, partial, total;
+};
+
+NoAllocLayout noalloc_layout(int n_cf, int n_sites) {
+  auto up = [](size_t v) { return (v + 31) / 32 * 32; };
+  const size_t n_blocks = (size_t(std::max(n_sites, 1)) + 255) / 256;
+  NoAllocLayout l;
+  l.cf = 32;
+  l.energy = l.cf + up(size_t(n_cf));
+  l.partial = l.energy + 32;
+  l.total = l.partial + size_t(n_cf) * n_blocks;
+  return l;
+}
+
+}  // namespace
 
 // ============================================================================
 struct cemc_updater {
@@ -1985,6 +2036,37 @@ int cemc_batch_get_cf(cemc_batch* b, int32_t chain, double* cf, double* energy) 
     compute_cf(b->d_models, b->params, b->d_params, b->d_occ, {chain}, b->n_cf_max, &cfv, &ev, b->stream);
     if (cf) std::copy(cfv.begin(), cfv.begin() + b->n_cf_of_chain[chain], cf);
     if (energy) *energy = ev[0];
+  });
+}
+
+int cemc_batch_get_cf_noalloc(cemc_batch* b, int32_t chain, double* cf, double* energy) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    require(!b->dynamic || b->active[chain], "slot is not in use");
+    const int n_cf = b->n_cf_of_chain[chain];
+    const int n_sites = b->params[chain].n_sites;
+    const NoAllocLayout lay = noalloc_layout(n_cf, n_sites);
+    if (lay.total > kNoAllocDoubles)
+      throw std::invalid_argument("cemc_batch_get_cf_noalloc: " + std::to_string(n_cf) + " correlation functions on " +
+                                  std::to_string(n_sites) + " sites need " + std::to_string(lay.total * 8) +
+                                  " bytes of scratch, more than its " + std::to_string(kNoAllocDoubles * 8) +
+                                  " bytes; use cemc_batch_get_cf");
+    double* base = scratch_on_current_device();
+    int32_t* d_id = reinterpret_cast<int32_t*>(base);
+    double* d_cf = base + lay.cf;
+    double* d_e = base + lay.energy;
+    double* d_partial = base + lay.partial;
+    // every partial, CF and energy entry is written by the kernels: nothing
+    // needs clearing between calls
+    const int32_t id = chain;
+    HIP_CHECK(hipMemcpyAsync(d_id, &id, sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+    HIP_CHECK(launch_cf(b->d_models, b->d_params, b->d_occ, d_id, 1, n_sites, d_partial, d_cf, d_e, b->stream));
+    double e = 0.0;
+    if (cf) HIP_CHECK(hipMemcpyAsync(cf, d_cf, sizeof(double) * size_t(n_cf), hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipMemcpyAsync(&e, d_e, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    if (energy) *energy = e;
   });
 }
 

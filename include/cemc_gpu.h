@@ -208,6 +208,16 @@ CEMC_API int cemc_batch_get_thermo(cemc_batch* batch, int32_t chain, cemc_thermo
 /* Correlation functions of chain c computed from scratch on the GPU (n_cf of
  * that chain's model) and the energy they give. */
 CEMC_API int cemc_batch_get_cf(cemc_batch* batch, int32_t chain, double* cf, double* energy);
+/* Same results as cemc_batch_get_cf (the same kernels), without allocating:
+ * no hipMalloc, hipFree or allocating library call.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * around HIP graph capture, tight serving loops, harnesses that forbid it).
+ * Its partial sums live in a 1 MiB device array of the library that each
+ * GPU holds until the process ends: a chain needs
+ * 8 * (64 + n_cf rounded up to 32 + n_cf * ceil(n_sites / 256)) bytes, and a
+ * larger request fails with CEMC_EINVAL.  All calls on one GPU share that
+ * array, so calls that overlap on one GPU race: serialise them. */
+CEMC_API int cemc_batch_get_cf_noalloc(cemc_batch* batch, int32_t chain, double* cf, double* energy);
 /* Launch shape.  Each chain runs on `waves` 64-lane wavefronts of one
  * workgroup (1, 2, 4 or 8), chosen by default from the batch size so that
  * the device gets about one wave per SIMD.  waves = 0 restores that choice;
