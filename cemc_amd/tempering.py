@@ -79,12 +79,16 @@ def find_temperature_scheme(mc_obj, Tmax, Tmin, target_accept=0.2, log=None):
     a chain of its own.  The first replica runs 10 N moves at Tmax; each next
     temperature starts from a copy of the previous replica's lattice: the
     trial temperature is halved until the estimated exchange acceptance
-    exp((1/kT_prev - 1/kT) (E_prev - E)) drops to target_accept (E: the mean
-    energy since the copy, accumulated over all trial temperatures as the
-    reference's averages are), then bisected until the acceptance is within
-    0.01 of the target or the interval is below 1e-4 K.  Stops at Tmin, or
-    when halving reaches Tmin without a candidate.  Returns (temperatures,
-    acceptances), the first acceptance 0."""
+    exp((1/kT_prev - 1/kT) (E_prev - E)) drops to target_accept, then
+    bisected until the acceptance is within 0.01 of the target or the
+    interval is below 1e-4 K.  Every trial is one runMC(steps=10 N,
+    equil=False), which resets the averages before it samples
+    (montecarlo.py:796-800): E is the mean energy of that trial's moves only,
+    E_prev the mean of the previous replica's last trial.  (The reference's
+    runMC also makes one move before and 1000 energy-bias probe moves,
+    montecarlo.py:762, 797-799, before the sampled moves; they are not made
+    here.)  Stops at Tmin, or when halving reaches Tmin without a candidate.
+    Returns (temperatures, acceptances), the first acceptance 0."""
     log = log or (lambda msg: None)
     natoms = len(mc_obj.atoms)
     nsteps = 10 * natoms
@@ -97,24 +101,26 @@ def find_temperature_scheme(mc_obj, Tmax, Tmin, target_accept=0.2, log=None):
         d.update(extra)
         return ChainBatch([model], [d])
 
+    def run_fixed(ch, T):
+        # runMC(steps=nsteps, equil=False): averages over this run's moves only
+        ch.set_temperature(0, T)
+        ch.reset_stats(0)
+        ch.run(nsteps)
+        return ch.thermo(0)["mean_energy"]
+
     cur = chain(Tmax, mc_obj.calc.updater.get_occupation(), 0)
-    cur.reset_stats(0)
-    cur.run(nsteps)
+    cur_E = run_fixed(cur, Tmax)
     T_cur = float(Tmax)
     temps, accepts = [T_cur], [0.0]
     lowest = T_cur
     r = 1
     while lowest > Tmin:
-        cur_E = cur.thermo(0)["mean_energy"]
         new = chain(T_cur, cur.occupation(0), r)
-        new.reset_stats(0)
         trial = T_cur / 2.0
         accept = 1.0
         found = False
         while accept > target_accept and trial > Tmin:
-            new.set_temperature(0, trial)
-            new.run(nsteps)
-            accept = _accept_probability(cur_E, new.thermo(0)["mean_energy"], T_cur, trial)
+            accept = _accept_probability(cur_E, run_fixed(new, trial), T_cur, trial)
             found = accept <= target_accept
             trial /= 2.0
         if not found:
@@ -123,9 +129,8 @@ def find_temperature_scheme(mc_obj, Tmax, Tmin, target_accept=0.2, log=None):
         upper, lower = T_cur, trial
         while True:
             new_T = 0.5 * (upper + lower)
-            new.set_temperature(0, new_T)
-            new.run(nsteps)
-            new_accept = _accept_probability(cur_E, new.thermo(0)["mean_energy"], T_cur, new_T)
+            new_E = run_fixed(new, new_T)
+            new_accept = _accept_probability(cur_E, new_E, T_cur, new_T)
             if new_accept > target_accept:
                 upper = new_T
             else:
@@ -136,7 +141,7 @@ def find_temperature_scheme(mc_obj, Tmax, Tmin, target_accept=0.2, log=None):
         temps.append(new_T)
         accepts.append(new_accept)
         cur.close()
-        cur, T_cur, lowest = new, new_T, new_T
+        cur, T_cur, lowest, cur_E = new, new_T, new_T, new_E
         r += 1
     cur.close()
     return temps, accepts
@@ -195,8 +200,25 @@ class ParallelTempering:
     def run(self, mc_args=None, num_exchange_cycles=10, use_graph=True):
         """num_exchange_cycles cycles of ``mc_args["steps"]`` trial moves per
         replica (default: one sweep) then one exchange step
-        (parallel_tempering.py:177-191).  Returns the device time in ms."""
+        (parallel_tempering.py:177-191).  Returns the device time in ms.
+
+        The reference passes ``mc_args`` to a full ``runMC`` per replica and
+        cycle; here a cycle is exactly ``steps`` moves per replica inside a
+        captured graph: no per-cycle equilibration (the reference's default
+        ``equil=True``), energy-bias probe or precision mode.  ``equil`` may
+        be given as False and ``mode`` as "fixed"; any other key, or
+        ``equil=True`` given explicitly, raises NotImplementedError rather
+        than being ignored."""
         mc_args = dict(mc_args or {})
+        unsupported = sorted(k for k in mc_args if k not in ("steps", "equil", "mode", "verbose"))
+        if unsupported:
+            raise NotImplementedError("ParallelTempering.run supports mc_args steps, equil=False and "
+                                      "mode='fixed' only; got %s" % ", ".join(unsupported))
+        if mc_args.get("equil", False):
+            raise NotImplementedError("per-cycle equilibration (equil=True) is not run on the device; "
+                                      "pass equil=False")
+        if mc_args.get("mode", "fixed") != "fixed":
+            raise NotImplementedError("only mode='fixed' cycles run on the device")
         steps = int(mc_args.get("steps", self.natoms))
         return self.exchange.run(num_exchange_cycles, steps, use_graph=use_graph)
 

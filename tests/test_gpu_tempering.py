@@ -83,31 +83,32 @@ def test_parallel_tempering_facade(gpu):
 def _oracle_scheme(s, eci, occ, kind, seed, uid0, Tmax, Tmin, target=0.2):
     """The reference's scheme search (parallel_tempering.py:47-136) driven by
     oracle chains on the same streams: replica r is a new chain with uid
-    uid0 + r started from the previous replica's lattice; means over the
-    per-move energies since the copy."""
-    from cemc_amd.tempering import _accept_probability
+    uid0 + r started from the previous replica's lattice; every trial is a
+    runMC that resets the averages first (montecarlo.py:796-800), so each
+    mean covers that trial's moves only."""
+    from oracle.pt import accept_probability
     names = s.cf_names()
     model = OracleModel(s, names)
     n = 10 * s.n_sites
 
     def new(o):
-        return {"upd": OracleUpdater(model, o, brute_force_cf(s, o, names), eci), "t": 0, "E": []}
+        return {"upd": OracleUpdater(model, o, brute_force_cf(s, o, names), eci), "t": 0}
 
     def run(ch, T, r):
         _, tr, _ = ch["upd"].run(kind, seed, uid0 + r, n, T, move_begin=ch["t"], trace_every=1)
         ch["t"] += n
-        ch["E"].extend(tr.tolist())
-        return float(np.mean(ch["E"]))
+        ch["E"] = float(np.mean(tr))
+        return ch["E"]
 
     cur = new(occ)
     run(cur, Tmax, 0)
     T_cur, temps, accs, r = Tmax, [Tmax], [0.0], 1
     while T_cur > Tmin:
-        cur_E = float(np.mean(cur["E"]))
+        cur_E = cur["E"]
         nw = new(cur["upd"].get_occupation())
         trial, acc, found = T_cur / 2.0, 1.0, False
         while acc > target and trial > Tmin:
-            acc = _accept_probability(cur_E, run(nw, trial, r), T_cur, trial)
+            acc = accept_probability(cur_E, run(nw, trial, r), T_cur, trial)
             found = acc <= target
             trial /= 2.0
         if not found:
@@ -115,7 +116,7 @@ def _oracle_scheme(s, eci, occ, kind, seed, uid0, Tmax, Tmin, target=0.2):
         hi, lo = T_cur, trial
         while True:
             t = 0.5 * (hi + lo)
-            acc = _accept_probability(cur_E, run(nw, t, r), T_cur, t)
+            acc = accept_probability(cur_E, run(nw, t, r), T_cur, t)
             if acc > target:
                 hi = t
             else:
