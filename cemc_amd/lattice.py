@@ -855,7 +855,8 @@ def settings_from_kwargs(kwargs):
     """Settings from the ``setting_kwargs`` of a saved calculator.
 
     ``classtype`` "FCCSettings", "BCCSettings", "SCSettings",
-    "CubicSettings" or "CrystalSettings": this module's own arguments.  ``classtype`` "CEBulk" with
+    "CubicSettings" or "CrystalSettings": this module's own arguments;
+    "HCPSettings" or "Spacegroup217Settings": those of cemc_amd.crystal.  ``classtype`` "CEBulk" with
     ``crystalstructure`` "fcc", "bcc" or "sc" (the reference's CLEASE bulk
     settings, ce_calculator.py:577-592): ``size`` is the repetition of the
     primitive cell, the species come from ``basis_elements`` (or
@@ -866,6 +867,14 @@ def settings_from_kwargs(kwargs):
     entries is indexed by size).  Other CLEASE settings are not available."""
     kw = dict(kwargs)
     classtype = kw.pop("classtype", "FCCSettings")
+    if classtype in ("HCPSettings", "Spacegroup217Settings"):
+        from .crystal import HCPSettings, Spacegroup217Settings
+        dia = {int(k): float(v) for k, v in kw["max_dia_by_size"].items()}
+        if classtype == "HCPSettings":
+            return HCPSettings(kw["size"], kw["species"], dia, kw.get("c_over_a", math.sqrt(8.0 / 3.0)),
+                               kw.get("symmetrize_decorations", True))
+        return Spacegroup217Settings(kw["size"], kw["a"], [tuple(w) for w in kw["wyckoff"]], kw["wyckoff_elements"],
+                                     dia, kw.get("symmetrize_decorations", True))
     if classtype == "CrystalSettings":
         shells = {int(k): int(v) for k, v in kw["max_shell_by_size"].items()}
         return CrystalSettings(kw["size"], kw["basis_elements"], shells, kw["structure"],

@@ -81,6 +81,9 @@ def brute_force_cf(bc, occupation, names, chunk=4096):
             norm[name] = norm.get(name, 0) + len(cl["indices"])
 
     out = {}
+    # cluster names grouped by prefix: the member species of a prefix's
+    # sub-clusters are gathered once and shared by all its decorations
+    by_prefix = {}
     for name in names:
         if name.startswith("c0"):
             out[name] = 1.0
@@ -91,7 +94,9 @@ def brute_force_cf(bc, occupation, names, chunk=4096):
             vals = phi[digits[0], occ]
             out[name] = float(vals[~bkg].sum() / n_sites)
             continue
-        total = 0.0
+        by_prefix.setdefault(prefix, []).append((name, digits))
+    for prefix, items in by_prefix.items():
+        total = {name: 0.0 for name, _ in items}
         for g, info in enumerate(bc.cluster_info):
             if prefix not in info:
                 continue
@@ -99,10 +104,10 @@ def brute_force_cf(bc, occupation, names, chunk=4096):
             n = int(cl["size"])
             members = np.array(cl["indices"], dtype=np.int64).reshape(-1, n - 1)
             order = np.array(cl["order"], dtype=np.int64).reshape(-1, n)
-            eq = equivalent_decorations(digits, cl["equiv_sites"]) if cl["equiv_sites"] else [digits]
+            eqs = [equivalent_decorations(d, cl["equiv_sites"]) if cl["equiv_sites"] else [d] for _, d in items]
             slots = np.vectorize(col_slot.get)(members) if members.size else members
             sites_g = groups[g]
-            acc = 0.0
+            acc = [0.0] * len(items)
             for lo in range(0, len(sites_g), chunk):
                 ref = sites_g[lo:lo + chunk]
                 idx = np.empty((len(ref), len(members), n), dtype=np.int64)
@@ -111,11 +116,14 @@ def brute_force_cf(bc, occupation, names, chunk=4096):
                     idx[:, :, j + 1] = trans[ref][:, slots[:, j]]
                 ordered = np.take_along_axis(idx, np.broadcast_to(order[None], idx.shape), axis=2)
                 spec = occ[ordered]
-                for e in eq:
-                    prod = np.ones(spec.shape[:2])
-                    for k in range(n):
-                        prod = prod * phi[e[k], spec[:, :, k]]
-                    acc += prod.sum()
-            total += acc / (len(eq) * norm[prefix] * len(sites_g))
-        out[name] = float(total)
-    return out
+                for t, eq in enumerate(eqs):
+                    for e in eq:
+                        prod = np.ones(spec.shape[:2])
+                        for k in range(n):
+                            prod = prod * phi[e[k], spec[:, :, k]]
+                        acc[t] += prod.sum()
+            for t, (name, _) in enumerate(items):
+                total[name] += acc[t] / (len(eqs[t]) * norm[prefix] * len(sites_g))
+        for name, _ in items:
+            out[name] = float(total[name])
+    return {name: out[name] for name in names}

@@ -36,31 +36,9 @@ sys.path.insert(0, os.path.join(ROOT, "oracle", "_ref"))
 import cemc_ref  # noqa: E402
 
 from cemc_amd.lattice import (CrystalSettings, FCCSettings, KB_EV, SimpleAtoms, random_crystal_occupation,  # noqa: E402
-                              random_occupation, seeded_ecis)
+                              random_occupation, seeded_ecis, settings_from_kwargs)
+from cemc_amd.dropin import run_updater_chain as run_reference  # noqa: E402
 from oracle.brute_force import brute_force_cf  # noqa: E402
-
-MASK64 = (1 << 64) - 1
-
-
-def philox_blocks(seed, chain_uid, move_begin, n_moves):
-    """Consecutive Philox4x64-10 blocks for counters (t, 0, 0, 0)."""
-    start = (move_begin - 1) & ((1 << 256) - 1)  # numpy increments before use
-    ctr = np.array([(start >> (64 * i)) & MASK64 for i in range(4)], dtype=np.uint64)
-    bg = np.random.Philox(key=np.array([seed, chain_uid], dtype=np.uint64), counter=ctr)
-    return bg.random_raw(4 * n_moves).reshape(n_moves, 4)
-
-
-def bounded64(w, n):
-    return (int(w) * int(n)) >> 64
-
-
-def bounded32(w, n):
-    return (int(w) * int(n)) >> 32
-
-
-def unit_double(w):
-    return float(int(w) >> 11) * 2.0 ** -53
-
 
 def make_updater(bc, occ, names, eci):
     species = sorted(bc.unique_elements)
@@ -72,69 +50,15 @@ def make_updater(bc, occ, names, eci):
     return upd, atoms, cf0
 
 
-def run_reference(upd, atoms, species, kind, seed, chain_uid, n_moves, temperature,
-                  trace_every, cf_every, names, background=()):
-    n_s = len(species)
-    sym = [a.symbol for a in atoms]
-    sid = {s: i for i, s in enumerate(species)}
-    occ = [sid[s] for s in sym]
-    n_sites = len(occ)
-    bkg = set(int(i) for i in background)
-    movable = [i for i in range(n_sites) if i not in bkg]
-    if kind == 0:
-        present = [s for s in range(n_s) if any(occ[i] == s for i in movable)]
-        tracker = {s: [i for i in movable if occ[i] == s] for s in present}
-    blocks = philox_blocks(seed, chain_uid, 0, n_moves)
-    bits = np.zeros((n_moves + 7) // 8, dtype=np.uint8)
-    e_trace, cf_ck = [], []
-    e_cur = upd.get_energy()
-    kT = temperature * KB_EV
-    n_acc = 0
-    for t in range(n_moves):
-        w0, w1, w2, w3 = (int(v) for v in blocks[t])
-        if kind == 0:
-            ia = bounded32(w0 >> 32, len(present))
-            ib = (ia + 1 + bounded32(w0 & 0xFFFFFFFF, len(present) - 1)) % len(present)
-            a, b = present[ia], present[ib]
-            sa = bounded64(w1, len(tracker[a]))
-            sb = bounded64(w2, len(tracker[b]))
-            pa, pb = tracker[a][sa], tracker[b][sb]
-            changes = [(pa, species[a], species[b]), (pb, species[b], species[a])]
-        else:
-            site = movable[bounded64(w1, len(movable))]
-            old = sid[atoms[site].symbol]
-            new = (old + 1 + bounded32(w0 & 0xFFFFFFFF, n_s - 1)) % n_s
-            changes = [(site, species[old], species[new])]
-        u = unit_double(w3)
-        e_new = upd.calculate(changes)
-        if e_new < e_cur:
-            accept = True
-        else:
-            energy_diff = e_new - e_cur
-            accept = u <= np.exp(-energy_diff / kT)
-        if accept:
-            upd.clear_history()
-            e_cur = e_new
-            n_acc += 1
-            bits[t >> 3] |= np.uint8(1 << (t & 7))
-            if kind == 0:
-                tracker[a][sa] = pb
-                tracker[b][sb] = pa
-        else:
-            upd.undo_changes()
-        if trace_every and (t + 1) % trace_every == 0:
-            e_trace.append(e_cur)
-        if cf_every and (t + 1) % cf_every == 0:
-            cf = upd.get_cf()
-            cf_ck.append([cf[n] for n in names])
-    final = np.array([sid[a.symbol] for a in atoms], dtype=np.int8)
-    return bits, np.array(e_trace), np.array(cf_ck), final, n_acc
-
-
 def chain_fixture(fname, lattice, species, shells, symmetrize, eci_seed, eci_sigma, occ_seed,
                   fractions, kind, seed, chain_uid, n_moves, temperature, mu, trace_every, cf_every, crystal=None):
     t0 = time.time()
-    if crystal:
+    if crystal and "classtype" in crystal:
+        # a crystal of cemc_amd.crystal rebuilt from its saved arguments
+        bc = settings_from_kwargs(dict(crystal, symmetrize_decorations=symmetrize))
+        species = list(bc.species)
+        occ = random_crystal_occupation(bc, occ_seed)
+    elif crystal:
         # two-sublattice crystal: species are the union of the sublattices'
         bc = CrystalSettings(lattice, crystal["basis_elements"], shells, crystal["structure"],
                              symmetrize_decorations=symmetrize)
@@ -283,6 +207,32 @@ def main():
                       occ_seed=9, fractions=None, kind=1, seed=15, chain_uid=4, n_moves=3000, temperature=700.0,
                       mu={"c1_0": -0.01}, trace_every=100, cf_every=1000,
                       crystal={"structure": "cscl", "basis_elements": [["Cu", "Zn"], ["Cu", "Zn"]]})
+    if want("hcp"):
+        chain_fixture("hcp_ternary_canonical.npz", None, None, {}, False, eci_seed=61, eci_sigma=0.01, occ_seed=10,
+                      fractions=None, kind=0, seed=16, chain_uid=5, n_moves=3000, temperature=900.0, mu=None,
+                      trace_every=100, cf_every=1000,
+                      crystal={"classtype": "HCPSettings", "size": [4, 4, 3], "species": ["Al", "Mg", "Si"],
+                               "max_dia_by_size": {"2": 1.45, "3": 1.05, "4": 1.05}})
+        chain_fixture("hcp_binary_sgc.npz", None, None, {}, False, eci_seed=62, eci_sigma=0.02, occ_seed=11,
+                      fractions=None, kind=1, seed=17, chain_uid=6, n_moves=3000, temperature=700.0,
+                      mu={"c1_0": 0.01}, trace_every=100, cf_every=1000,
+                      crystal={"classtype": "HCPSettings", "size": [5, 5, 4], "species": ["Al", "Mg"],
+                               "max_dia_by_size": {"2": 1.75, "3": 1.45}})
+    if want("sg217"):
+        chain_fixture("sg217_binary_sgc.npz", None, None, {}, False, eci_seed=63, eci_sigma=0.02, occ_seed=12,
+                      fractions=None, kind=1, seed=18, chain_uid=7, n_moves=3000, temperature=800.0,
+                      mu={"c1_0": -0.02}, trace_every=100, cf_every=1000,
+                      crystal={"classtype": "Spacegroup217Settings", "size": [1, 1, 1], "a": 10.553,
+                               "wyckoff": [[0, 0, 0], [0.324, 0.324, 0.324], [0.3582, 0.3582, 0.0393],
+                                           [0.0954, 0.0954, 0.2725]],
+                               "wyckoff_elements": [["Al", "Mg"]] * 4, "max_dia_by_size": {"2": 4.1, "3": 4.1}})
+        chain_fixture("sg217_binary_canonical.npz", None, None, {}, False, eci_seed=64, eci_sigma=0.02,
+                      occ_seed=13, fractions=None, kind=0, seed=19, chain_uid=8, n_moves=3000, temperature=900.0,
+                      mu=None, trace_every=100, cf_every=1000,
+                      crystal={"classtype": "Spacegroup217Settings", "size": [2, 1, 1], "a": 10.553,
+                               "wyckoff": [[0, 0, 0], [0.324, 0.324, 0.324], [0.3582, 0.3582, 0.0393],
+                                           [0.0954, 0.0954, 0.2725]],
+                               "wyckoff_elements": [["Al", "Mg"]] * 4, "max_dia_by_size": {"2": 4.1, "3": 4.1}})
     if want("api"):
         api_fixture("updater_api_ternary.json")
 

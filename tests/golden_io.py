@@ -6,7 +6,7 @@ import os
 
 import numpy as np
 
-from cemc_amd.lattice import CrystalSettings, FCCSettings
+from cemc_amd.lattice import CrystalSettings, FCCSettings, settings_from_kwargs
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -27,6 +27,12 @@ CRYSTAL_FIXTURES = [
     "rocksalt_canonical.npz",
     "rocksalt_background_sgc.npz",
     "cscl_sgc.npz",
+    # hexagonal close packing (two translation groups related by the 6_3
+    # screw) and a space-group-217 crystal (29 translation groups)
+    "hcp_ternary_canonical.npz",
+    "hcp_binary_sgc.npz",
+    "sg217_binary_sgc.npz",
+    "sg217_binary_canonical.npz",
 ]
 
 
@@ -37,7 +43,9 @@ class ChainFixture:
         self.meta = json.loads(str(z["meta"]))
         m = self.meta
         shells = {int(k): v for k, v in m["shells"].items()}
-        if m.get("crystal"):
+        if m.get("crystal") and "classtype" in m["crystal"]:
+            self.settings = settings_from_kwargs(dict(m["crystal"], symmetrize_decorations=m["symmetrize"]))
+        elif m.get("crystal"):
             cr = m["crystal"]
             self.settings = CrystalSettings(m["lattice"], cr["basis_elements"], shells, cr["structure"],
                                             symmetrize_decorations=m["symmetrize"])
