@@ -92,6 +92,10 @@ SIGNATURES = [
     ("cemc_updater_get_energy", ctypes.c_int, [_vp, _dp]),
     ("cemc_updater_get_cf", ctypes.c_int, [_vp, _dp]),
     ("cemc_updater_set_ecis", ctypes.c_int, [_vp, _dp]),
+    ("cemc_updater_get_singlets", ctypes.c_int, [_vp, _dp, ctypes.c_int32, _i32p]),
+    ("cemc_updater_add_linear_vib_correction", ctypes.c_int, [_vp, ctypes.c_int32, _i32p, _dp]),
+    ("cemc_updater_vib_energy", ctypes.c_int, [_vp, ctypes.c_double, _dp]),
+    ("cemc_updater_set_num_threads", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("cemc_updater_get_occupation", ctypes.c_int, [_vp, _i8p]),
     ("cemc_updater_history_size", ctypes.c_int, [_vp, _i32p]),
     ("cemc_updater_set_occupation", ctypes.c_int, [_vp, _i8p]),
@@ -100,6 +104,7 @@ SIGNATURES = [
     ("cemc_batch_destroy", ctypes.c_int, [_vp]),
     ("cemc_batch_run", ctypes.c_int, [_vp, _i64p, ctypes.c_int64]),
     ("cemc_batch_run_timed", ctypes.c_int, [_vp, _i64p, ctypes.c_int64, _dp]),
+    ("cemc_batch_run_sliced", ctypes.c_int, [_vp, _i64p, ctypes.c_int64, ctypes.c_double, _dp]),
     ("cemc_batch_run_logged", ctypes.c_int, [_vp, _i64p, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint32),
                                              ctypes.c_int64]),
     ("cemc_batch_get_energy", ctypes.c_int, [_vp, _dp]),

@@ -1,7 +1,8 @@
 """Launch-shape autotuner: picks, for a batch, the energy path (per-site
 fields updated on accepted moves, one wave per chain; or tables summed per
 trial move) and, for the latter, the number of wavefronts per chain (1, 2, 4
-or 8 cooperating waves) and the moves evaluated per round (1 or 8), by timing
+or 8 cooperating waves) and the moves evaluated per round (1, 8, or 8 per
+wave: wide rounds, where each wave evaluates its own eight), by timing
 each candidate on a scratch copy of the batch, so the real chains'
 trajectories are not touched.
 
@@ -48,7 +49,7 @@ def parse_shape(name):
 
 
 def tune(models, chains, candidates=(1, 2, 4, 8), per_round=(1, 8), fields=True, sweeps=2.0, repeats=2,
-         cache_file=None, log=None):
+         cache_file=None, log=None, wide=True):
     """Returns (best shape, {shape: moves/s}) for the batch (models, chains).
 
     Each candidate shape runs ``sweeps`` sweeps of every chain on a scratch
@@ -69,6 +70,9 @@ def tune(models, chains, candidates=(1, 2, 4, 8), per_round=(1, 8), fields=True,
         if any(scratch.energy_path(c) == PATH_FIELDS for c in range(scratch.n_chains)):
             shapes.append((1, 1, PATH_FIELDS))
     shapes += [(w, k, PATH_EVALUATE) for k in per_round for w in candidates]
+    if wide:
+        # wide rounds: each of w waves evaluates its own eight moves
+        shapes += [(w, 8 * w, PATH_EVALUATE) for w in candidates if w > 1]
     rates = {}
     seen = set()
     for shape in shapes:

@@ -104,6 +104,17 @@ class ChainBatch:
         _capi.check(self._lib.cemc_batch_run_timed(self.handle, ptr, scalar, ctypes.byref(ms)))
         return ms.value
 
+    def run_sliced(self, n_moves, max_ms):
+        """Like run_timed, but every chain stops at the first 64-move block
+        that starts more than max_ms after its own start on the device;
+        returns the device time in ms.  stats()["moves"] tells how far each
+        chain got; its stream continues there on the next run."""
+        a, scalar = self._moves(n_moves)
+        ptr = a.ctypes.data_as(_capi._i64p) if a is not None else None
+        ms = ctypes.c_double(0.0)
+        _capi.check(self._lib.cemc_batch_run_sliced(self.handle, ptr, scalar, float(max_ms), ctypes.byref(ms)))
+        return ms.value
+
     def run_logged(self, n_moves):
         """Like run; returns per-chain accept bits (uint8, LSB first, one bit per move)."""
         a, scalar = self._moves(n_moves)
@@ -176,8 +187,13 @@ class ChainBatch:
         """0 (default): eight consecutive moves evaluated together wherever
         the chain's kernel variant allows it and the moves of a round seldom
         conflict (interacting sites, times 4 for swaps, at most 12% of the
-        lattice), else one at a time; 1: every move on its own; 8: rounds of
-        eight where allowed.  Same results bit for bit."""
+        lattice), else one at a time; 1: every move on
+        its own; 8: rounds of eight where allowed; 16, 32, 64: wide rounds
+        on 2, 4, 8 waves per chain, each wave evaluating its own eight moves
+        of the round (periodic lattices whose records fit one wave; other
+        chains take rounds of eight); wide rounds pay on large lattices at
+        low acceptance, so the autotuner times them rather than choosing
+        them by default.  Same results bit for bit."""
         _capi.check(self._lib.cemc_batch_set_moves_per_round(self.handle, int(moves)))
 
     def set_energy_path(self, path):

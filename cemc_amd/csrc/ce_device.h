@@ -10,7 +10,7 @@ namespace cemc {
 
 constexpr int kMaxSpecies = 8;           // species per model
 constexpr int kMaxBasis = kMaxSpecies - 1;
-constexpr int kMaxGroups = 8;            // translation-symmetry groups per model
+constexpr int kMaxGroups = 32;           // translation-symmetry groups per model (2 x this <= 64 lanes)
 constexpr int kMaxClusterSize = 4;       // reference limit (cpp/src/cluster.cpp:131-170)
 constexpr double kBoltzmannEV = 8.617330337217213e-05;
 
@@ -20,8 +20,8 @@ constexpr double kBoltzmannEV = 8.617330337217213e-05;
 //   w0: slot of member 0..2 (10 bits each, canonical order with the reference
 //       site removed) | (size-2) << 30
 //   w1: energy-table base (24 bits) | canonical position of the reference
-//       site << 24 | cluster entry id << 26 (6 bits, for CF kernels use the
-//       separate per-entry range table)
+//       site << 24 (the record's cluster entry is implied by the per-entry
+//       record ranges)
 // Slot = column index into the translation table (used columns only).
 struct SubRec {
   uint32_t w0;

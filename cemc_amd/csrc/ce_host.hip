@@ -120,6 +120,7 @@ struct cemc_model {
   std::vector<int32_t> entry_group, entry_size, entry_M, entry_norm, entry_goff;
   std::vector<int32_t> trans;
   std::vector<SubRec> recs;
+  std::vector<int> rec_entry;  // cluster entry of each record
   std::vector<FatRec> rec_slots, frecs;
   std::vector<ClusterEntry> entries;
   std::vector<int32_t> grp_count;
@@ -306,8 +307,7 @@ void cemc_model::build_duals() {
   };
   std::vector<RecGeo> geo(cnt);
   for (int r = 0; r < cnt; ++r) {
-    const SubRec& sr = recs[r];
-    geo[r].entry = int(sr.w1 >> 26);
+    geo[r].entry = rec_entry[r];
     geo[r].nm = int((rec_slots[r].meta >> 24) & 3u);
     const int32_t sl[3] = {rec_slots[r].m0, rec_slots[r].m1, rec_slots[r].m2};
     for (int q = 0; q < 3; ++q) geo[r].off[q] = q < geo[r].nm ? col_cells[sl[q]] : 0u;
@@ -419,10 +419,10 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
   require(d != nullptr, "model description is NULL");
   require(d->n_sites > 0, "n_sites must be positive");
   require(d->n_species >= 2 && d->n_species <= kMaxSpecies, "n_species must be in [2, 8]");
-  require(d->n_groups >= 1 && d->n_groups <= kMaxGroups, "n_groups must be in [1, 8]");
+  require(d->n_groups >= 1 && d->n_groups <= kMaxGroups, "n_groups must be in [1, 32]");
   require(d->n_cf >= 1, "at least one correlation function is required");
   require(d->n_trans_cols >= 0 && d->n_trans_cols <= 1024, "at most 1024 translation columns are supported");
-  require(d->n_entries >= 0 && d->n_entries < 64, "at most 63 cluster entries are supported");
+  require(d->n_entries >= 0 && d->n_entries <= (1 << 20), "at most 2^20 cluster entries are supported");
   m->n_sites = d->n_sites;
   m->n_species = d->n_species;
   m->n_bfs = d->n_species - 1;
@@ -534,8 +534,9 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
           ++q;
         }
         const uint32_t gbase = uint32_t(m->entry_goff[c] + r * block);
-        const uint32_t w1 = gbase | (uint32_t(r) << 24) | (uint32_t(c) << 26);
+        const uint32_t w1 = gbase | (uint32_t(r) << 24);
         m->recs.push_back(SubRec{w0, w1});
+        m->rec_entry.push_back(c);
         m->rec_slots.push_back(FatRec{int32_t(rec_slot(SubRec{w0, w1}, 0)), int32_t(rec_slot(SubRec{w0, w1}, 1)),
                                       int32_t(rec_slot(SubRec{w0, w1}, 2)), 0u, 0u, 0u,
                                       gbase | (uint32_t(n - 1) << 24) | kRecValid, 0u});
@@ -792,6 +793,11 @@ struct cemc_updater {
   int32_t* d_changes = nullptr;
   int32_t* d_restore = nullptr;
   int change_cap = 0;
+  // pinned host memory: the energy read back after a call, and staging for
+  // change lists too long for the kernel arguments
+  double* h_energy = nullptr;
+  int32_t* h_stage = nullptr;
+  size_t stage_cap = 0;  // int32 entries at h_stage
   std::vector<int8_t> occ;  // host mirror of the device occupation
   std::vector<double> eci;
   std::vector<int64_t> slot_epoch;
@@ -801,7 +807,15 @@ struct cemc_updater {
     int site, old_id, new_id;
   };
   std::vector<Change> changes;
+  // LinearVibCorrection (cpp/src/linear_vib_correction.cpp): (CF index, v_k)
+  // in name order, and whether one was added
+  std::vector<std::pair<int, double>> vib;
+  bool has_vib = false;
+  int num_threads = 1;
   ~cemc_updater() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (h_energy) (void)hipHostFree(h_energy);
+    if (h_stage) (void)hipHostFree(h_stage);
     dev_free(d_occ);
     dev_free(d_ring);
     dev_free(d_slot_energy);
@@ -816,19 +830,44 @@ struct cemc_updater {
 
 namespace {
 
+// Copies `n` int32 values into the pinned staging buffer and from there to
+// `dst` on the updater's stream (after the stream's earlier work, which may
+// still read the staging buffer).
+void updater_stage_copy(cemc_updater* u, int32_t* dst, const int32_t* src, size_t n) {
+  HIP_CHECK(hipStreamSynchronize(u->stream));
+  if (n > u->stage_cap) {
+    if (u->h_stage) (void)hipHostFree(u->h_stage);
+    u->h_stage = nullptr;
+    u->stage_cap = 0;
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&u->h_stage), sizeof(int32_t) * n));
+    u->stage_cap = n;
+  }
+  std::memcpy(u->h_stage, src, sizeof(int32_t) * n);
+  HIP_CHECK(hipMemcpyAsync(dst, u->h_stage, sizeof(int32_t) * n, hipMemcpyHostToDevice, u->stream));
+}
+
+// One launch for a sequence of changes, queued on the updater's stream
+// without waiting: short sequences travel in the kernel arguments.  A read
+// of the result (updater_energy, get_cf) waits for it.
 void updater_flush(cemc_updater* u, const std::vector<int32_t>& packed) {
   const int n = int(packed.size() / 4);
   if (n == 0) return;
-  if (n > u->change_cap) {
-    dev_free(u->d_changes);
-    u->change_cap = std::max(n, 2 * u->change_cap);
-    HIP_CHECK(hipMalloc(&u->d_changes, sizeof(int32_t) * 4 * u->change_cap));
+  InlineChanges inl{};
+  const int32_t* d_list = nullptr;
+  if (n <= kInlineChanges) {
+    std::memcpy(inl.v, packed.data(), sizeof(int32_t) * packed.size());
+  } else {
+    if (n > u->change_cap) {
+      HIP_CHECK(hipStreamSynchronize(u->stream));
+      dev_free(u->d_changes);
+      u->change_cap = std::max(n, 2 * u->change_cap);
+      HIP_CHECK(hipMalloc(&u->d_changes, sizeof(int32_t) * 4 * u->change_cap));
+    }
+    updater_stage_copy(u, u->d_changes, packed.data(), packed.size());
+    d_list = u->d_changes;
   }
-  HIP_CHECK(hipMemcpyAsync(u->d_changes, packed.data(), sizeof(int32_t) * packed.size(), hipMemcpyHostToDevice,
-                           u->stream));
-  HIP_CHECK(launch_updater(u->model->d_view, u->d_occ, u->d_ring, u->d_slot_energy, u->d_changes, n, u->d_eci,
+  HIP_CHECK(launch_updater(u->model->d_view, u->d_occ, u->d_ring, u->d_slot_energy, inl, d_list, n, u->d_eci,
                            u->d_scratch, u->stream));
-  HIP_CHECK(hipStreamSynchronize(u->stream));
   u->slot_epoch[packed[packed.size() - 1]] = u->eci_epoch;
 }
 
@@ -863,16 +902,24 @@ int updater_stage(cemc_updater* u, int site, int old_id, int new_id, std::vector
   return CEMC_OK;
 }
 
+// Energy of the current slot: one copy into pinned memory on the stream
+// (after the queued updates) and one wait.
 double updater_energy(cemc_updater* u) {
   const unsigned slot = u->current_slot();
   if (u->slot_epoch[slot] != u->eci_epoch) {
     HIP_CHECK(launch_slot_energy(u->model->d_view, u->d_ring, u->d_slot_energy, u->d_eci, int(slot), u->stream));
-    HIP_CHECK(hipStreamSynchronize(u->stream));
     u->slot_epoch[slot] = u->eci_epoch;
   }
-  double e = 0.0;
-  HIP_CHECK(hipMemcpy(&e, u->d_slot_energy + slot, sizeof(double), hipMemcpyDeviceToHost));
-  return e;
+  HIP_CHECK(hipMemcpyAsync(u->h_energy, u->d_slot_energy + slot, sizeof(double), hipMemcpyDeviceToHost, u->stream));
+  HIP_CHECK(hipStreamSynchronize(u->stream));
+  return *u->h_energy;
+}
+
+// Correlation functions of the current slot (n_cf values).
+void updater_current_cf(cemc_updater* u, double* cf) {
+  HIP_CHECK(hipMemcpyAsync(cf, u->d_ring + size_t(u->current_slot()) * u->model->n_cf,
+                           sizeof(double) * u->model->n_cf, hipMemcpyDeviceToHost, u->stream));
+  HIP_CHECK(hipStreamSynchronize(u->stream));
 }
 
 }  // namespace
@@ -905,7 +952,7 @@ struct cemc_batch {
     int begin, count;
     uint32_t lds;
   };
-  int spec_moves = 0;  // moves per round where the model allows it (0: kSpecMoves, 1 or kSpecMoves)
+  int spec_moves = 0;  // moves per round where the model allows it (0: automatic; 1, 8, or 8 x waves = wide)
   int energy_path = CEMC_PATH_AUTO;
   std::vector<Class> classes;
   int64_t classes_version = 0;  // bumped whenever the launch classes change
@@ -935,7 +982,18 @@ struct cemc_batch {
   uint32_t* d_wl_hist = nullptr;
   uint8_t* d_wl_known = nullptr;
   int32_t* d_wl_order = nullptr;
+  // pools: device scratch of the initial-energy launch at admission, kept
+  // across admissions (a free per admission would wait for the whole device)
+  int32_t* d_adm_id = nullptr;
+  double* d_adm_e = nullptr;
+  double* d_adm_partial = nullptr;
+  double* h_adm = nullptr;  // pinned: chain id in and energy out
+  int adm_blocks = 0;
   ~cemc_batch() {
+    if (h_adm) (void)hipHostFree(h_adm);
+    dev_free(d_adm_id);
+    dev_free(d_adm_e);
+    dev_free(d_adm_partial);
     for (auto& h : hooks) {
       dev_free(h.d_allowed);
       dev_free(h.d_nb);
@@ -1168,10 +1226,25 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
     // too often and one move at a time is faster (measured on binary FCC 10^3:
     // flips, share 0.055, gain from rounds; swaps, share 0.22, loss).
     int want_spec = b->spec_moves;
+    const int kr1 = pick_kr(1);
     if (want_spec == 0) {
       const int box = __builtin_popcountll(m->imask[0]) + __builtin_popcountll(m->imask[1]);
       const double share = double(box) * (canon ? 4.0 : 1.0) / double(m->n_sites);
       want_spec = share <= kRoundShare ? kSpecMoves : 1;
+      // wide rounds are not chosen automatically: the k-th move of a round
+      // is cut with probability about (acceptance x share x k), and the
+      // acceptance is unknown here (measured on binary FCC 24^3 swaps: 2x
+      // faster than rounds of eight at acceptance 0.07, 10% slower at 0.4);
+      // the autotuner times them
+    }
+    if (want_spec > kSpecMoves) {
+      // wide rounds: every wave holds all the records
+      if (kr1 > 0 && m->spec_ok) {
+        k.waves = want_spec / kSpecMoves;
+        k.kr = kr1;
+      } else {
+        want_spec = kSpecMoves;
+      }
     }
     k.spec = (k.kr > 0 && m->spec_ok) ? want_spec : 1;
     if (k.kr > 0) {
@@ -1294,7 +1367,7 @@ std::string hooks_after_launch(cemc_batch* b) {
 
 // One sweep-kernel launch per class on the batch's stream; each chain
 // advances by its entry of d_nmoves.
-void enqueue_sweeps(cemc_batch* b, uint32_t* d_log, int64_t log_words) {
+void enqueue_sweeps(cemc_batch* b, uint32_t* d_log, int64_t log_words, int64_t time_limit = 0) {
   const bool fan_out = b->classes.size() > 1;
   if (fan_out) {
     if (b->side.empty()) {
@@ -1335,6 +1408,7 @@ void enqueue_sweeps(cemc_batch* b, uint32_t* d_log, int64_t log_words) {
     a.nm = cl.nm;
     a.accept_log = d_log;
     a.log_words = log_words;
+    a.time_limit = time_limit;
     hipStream_t st = fan_out ? b->side[ci % cemc_batch::kSideStreams] : b->stream;
     HIP_CHECK(launch_sweep(a, st));
     ++ci;
@@ -1348,8 +1422,17 @@ void enqueue_sweeps(cemc_batch* b, uint32_t* d_log, int64_t log_words) {
   }
 }
 
+// Ticks of the device's constant-rate wall clock (wall_clock64) per millisecond.
+double wall_ticks_per_ms() {
+  int dev = 0, khz = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  if (khz <= 0) throw DeviceError("the device reports no wall-clock rate");
+  return double(khz);
+}
+
 void batch_launch(cemc_batch* b, const int64_t* n_moves, int64_t moves, bool timed, double* ms,
-                  uint32_t* host_log = nullptr, int64_t log_words = 0) {
+                  uint32_t* host_log = nullptr, int64_t log_words = 0, int64_t time_limit = 0) {
   std::vector<int64_t> nm(b->n_chains);
   for (int c = 0; c < b->n_chains; ++c) {
     nm[c] = n_moves ? n_moves[c] : moves;
@@ -1364,7 +1447,7 @@ void batch_launch(cemc_batch* b, const int64_t* n_moves, int64_t moves, bool tim
   if (b->dynamic) derive_classes(b, b->forced_waves, &nm);
   HIP_CHECK(hipMemcpyAsync(b->d_nmoves, nm.data(), sizeof(int64_t) * nm.size(), hipMemcpyHostToDevice, b->stream));
   if (timed) HIP_CHECK(hipEventRecord(b->ev0, b->stream));
-  enqueue_sweeps(b, d_log, log_words);
+  enqueue_sweeps(b, d_log, log_words, time_limit);
   if (timed) HIP_CHECK(hipEventRecord(b->ev1, b->stream));
   HIP_CHECK(hipStreamSynchronize(b->stream));
   std::string hook_error = hooks_after_launch(b);
@@ -1648,6 +1731,7 @@ int cemc_updater_create(cemc_model* model, const int8_t* occupation, const doubl
       HIP_CHECK(hipMalloc(&u->d_slot_energy, sizeof(double) * kMaxHistory));
       HIP_CHECK(hipMalloc(&u->d_scratch, sizeof(double) * 4 * size_t(ncf)));
       HIP_CHECK(hipMalloc(&u->d_restore, sizeof(int32_t) * 2 * kMaxHistory));
+      HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&u->h_energy), sizeof(double)));
       std::vector<double> cf0(ncf);
       if (cf_init) {
         cf0.assign(cf_init, cf_init + ncf);
@@ -1793,12 +1877,19 @@ int cemc_updater_undo_changes(cemc_updater* u) {
       species.push_back(ch.old_id);
     }
     if (!sites.empty()) {
+      // restores are queued on the stream behind the updates they undo
       std::vector<int32_t> buf(sites);
       buf.insert(buf.end(), species.begin(), species.end());
-      HIP_CHECK(hipMemcpyAsync(u->d_restore, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice,
-                               u->stream));
-      HIP_CHECK(launch_restore(u->d_occ, u->d_restore, u->d_restore + sites.size(), int(sites.size()), u->stream));
-      HIP_CHECK(hipStreamSynchronize(u->stream));
+      const int n = int(sites.size());
+      InlineChanges inl{};
+      const int32_t* d_buf = nullptr;
+      if (n <= 2 * kInlineChanges) {
+        std::memcpy(inl.v, buf.data(), sizeof(int32_t) * buf.size());
+      } else {
+        updater_stage_copy(u, u->d_restore, buf.data(), buf.size());
+        d_buf = u->d_restore;
+      }
+      HIP_CHECK(launch_restore(u->d_occ, inl, d_buf, n, u->stream));
     }
   });
 }
@@ -1820,8 +1911,63 @@ int cemc_updater_get_energy(cemc_updater* u, double* energy) {
 int cemc_updater_get_cf(cemc_updater* u, double* cf) {
   return guarded([&] {
     require(u && cf, "NULL argument");
-    HIP_CHECK(hipMemcpy(cf, u->d_ring + size_t(u->current_slot()) * u->model->n_cf,
-                        sizeof(double) * u->model->n_cf, hipMemcpyDeviceToHost));
+    updater_current_cf(u, cf);
+  });
+}
+
+int cemc_updater_get_singlets(cemc_updater* u, double* singlets, int32_t capacity, int32_t* n_singlets) {
+  return guarded([&] {
+    require(u != nullptr, "NULL updater");
+    const cemc_model& m = *u->model;
+    std::vector<int> idx;
+    for (int k = 0; k < m.n_cf; ++k)
+      if (m.cf_kind[k] == CEMC_CF_SINGLET) idx.push_back(k);
+    if (n_singlets) *n_singlets = int32_t(idx.size());
+    if (singlets == nullptr) return;
+    if (capacity < int32_t(idx.size()))
+      throw std::runtime_error("The passed Numpy array is too small to hold all the singlets terms!\nMinimum size: " +
+                               std::to_string(idx.size()) + ". Given size: " + std::to_string(capacity));
+    std::vector<double> cf(m.n_cf);
+    updater_current_cf(u, cf.data());
+    for (size_t i = 0; i < idx.size(); ++i) singlets[i] = cf[idx[i]];
+  });
+}
+
+int cemc_updater_add_linear_vib_correction(cemc_updater* u, int32_t n, const int32_t* cf_index,
+                                           const double* eci_per_kbT) {
+  return guarded([&] {
+    require(u != nullptr && n >= 0, "bad arguments");
+    require(n == 0 || (cf_index && eci_per_kbT), "NULL correction arrays");
+    std::map<int, double> by_index;  // name order = index order (CFs are in sorted-name order)
+    for (int i = 0; i < n; ++i) {
+      require(cf_index[i] >= 0 && cf_index[i] < u->model->n_cf, "correlation-function index out of range");
+      by_index[cf_index[i]] = eci_per_kbT[i];
+    }
+    u->vib.assign(by_index.begin(), by_index.end());
+    u->has_vib = true;
+  });
+}
+
+int cemc_updater_vib_energy(cemc_updater* u, double temperature, double* energy) {
+  return guarded([&] {
+    require(u && energy, "NULL argument");
+    double e = 0.0;
+    if (u->has_vib) {
+      std::vector<double> cf(u->model->n_cf);
+      updater_current_cf(u, cf.data());
+      // LinearVibCorrection::energy (linear_vib_correction.cpp:8-16), kB as there
+      constexpr double kB = 8.6173303E-5;
+      for (const auto& kv : u->vib) e += cf[kv.first] * kv.second * kB * temperature;
+    }
+    *energy = e;
+  });
+}
+
+int cemc_updater_set_num_threads(cemc_updater* u, int32_t num_threads) {
+  return guarded([&] {
+    require(u != nullptr, "NULL updater");
+    require(num_threads > 0, "num_threads must be positive");
+    u->num_threads = num_threads;
   });
 }
 
@@ -1829,6 +1975,7 @@ int cemc_updater_set_ecis(cemc_updater* u, const double* eci) {
   return guarded([&] {
     require(u && eci, "NULL argument");
     u->eci.assign(eci, eci + u->model->n_cf);
+    HIP_CHECK(hipStreamSynchronize(u->stream));  // queued updates still read the old ECIs
     HIP_CHECK(hipMemcpy(u->d_eci, u->eci.data(), sizeof(double) * u->eci.size(), hipMemcpyHostToDevice));
     u->eci_epoch += 1;
   });
@@ -1837,7 +1984,9 @@ int cemc_updater_set_ecis(cemc_updater* u, const double* eci) {
 int cemc_updater_get_occupation(cemc_updater* u, int8_t* occupation) {
   return guarded([&] {
     require(u && occupation, "NULL argument");
-    HIP_CHECK(hipMemcpy(occupation, u->d_occ, u->occ.size(), hipMemcpyDeviceToHost));
+    // the host mirror follows every change, restore and replacement of the
+    // device occupation in call order
+    std::memcpy(occupation, u->occ.data(), u->occ.size());
   });
 }
 
@@ -1858,7 +2007,9 @@ int cemc_batch_create(int32_t n_chains, cemc_model* const* chain_models, const c
     auto* b = new cemc_batch();
     try {
       b->n_chains = n_chains;
-      HIP_CHECK(hipStreamCreate(&b->stream));
+      // non-blocking: a pool served by one host thread must not wait for
+      // null-stream copies of another pool's thread, nor they for its kernels
+      HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
       HIP_CHECK(hipEventCreate(&b->ev0));
       HIP_CHECK(hipEventCreate(&b->ev1));
       std::vector<int8_t> occ_pool;
@@ -1925,6 +2076,18 @@ int cemc_batch_run_timed(cemc_batch* b, const int64_t* n_moves, int64_t moves, d
   return guarded([&] {
     require(b && kernel_ms, "NULL argument");
     batch_launch(b, n_moves, moves, true, kernel_ms);
+  });
+}
+
+int cemc_batch_run_sliced(cemc_batch* b, const int64_t* n_moves, int64_t moves, double max_ms, double* kernel_ms) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(max_ms > 0.0, "max_ms must be positive");
+    const double ticks = max_ms * wall_ticks_per_ms();
+    require(ticks < 9.0e18, "max_ms too large");
+    double ms = 0.0;
+    batch_launch(b, n_moves, moves, true, &ms, nullptr, 0, std::max<int64_t>(1, int64_t(ticks)));
+    if (kernel_ms) *kernel_ms = ms;
   });
 }
 
@@ -2083,7 +2246,9 @@ int cemc_batch_set_waves(cemc_batch* b, int32_t waves) {
 int cemc_batch_set_moves_per_round(cemc_batch* b, int32_t moves) {
   return guarded([&] {
     require(b != nullptr, "NULL batch");
-    require(moves == 0 || moves == 1 || moves == kSpecMoves, "moves per round must be 0 (automatic), 1 or 8");
+    require(moves == 0 || moves == 1 || moves == kSpecMoves || moves == 2 * kSpecMoves || moves == 4 * kSpecMoves ||
+                moves == 8 * kSpecMoves,
+            "moves per round must be 0 (automatic), 1, 8, 16, 32 or 64");
     HIP_CHECK(hipStreamSynchronize(b->stream));
     b->spec_moves = moves;
     if (!b->dynamic) derive_classes(b, b->forced_waves);
@@ -2331,7 +2496,9 @@ int cemc_pool_create(int32_t max_chains, int32_t max_models, int64_t occ_bytes, 
       b->params.assign(max_chains, ChainParams{});
       for (auto& p : b->params) std::memset(&p, 0, sizeof p);
       b->n_cf_of_chain.assign(max_chains, 0);
-      HIP_CHECK(hipStreamCreate(&b->stream));
+      // non-blocking: a pool served by one host thread must not wait for
+      // null-stream copies of another pool's thread, nor they for its kernels
+      HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
       HIP_CHECK(hipEventCreate(&b->ev0));
       HIP_CHECK(hipEventCreate(&b->ev1));
       HIP_CHECK(hipMalloc(&b->d_models, sizeof(ModelDev) * size_t(max_models)));
@@ -2382,7 +2549,8 @@ int cemc_pool_add(cemc_batch* b, cemc_model* m, const cemc_chain_desc* d, int32_
     if (it == b->models.end()) {
       if (int(b->models.size()) >= b->model_capacity) throw PoolFull("no free model slot in the pool");
       b->models.push_back(m);
-      HIP_CHECK(hipMemcpy(b->d_models + mid, &m->view, sizeof(ModelDev), hipMemcpyHostToDevice));
+      HIP_CHECK(hipMemcpyAsync(b->d_models + mid, &m->view, sizeof(ModelDev), hipMemcpyHostToDevice, b->stream));
+      HIP_CHECK(hipStreamSynchronize(b->stream));
     }
     std::vector<uint16_t> trk;
     setup_chain(b, slot, m, mid, *d, trk);
@@ -2401,18 +2569,42 @@ int cemc_pool_add(cemc_batch* b, cemc_model* m, const cemc_chain_desc* d, int32_
     p.occ_off = occ_off;
     p.trk_off = trk_off;
     try {
+      // every copy and launch on the pool's own stream, and no allocation or
+      // free on the way (either could wait for other streams' work)
       HIP_CHECK(hipStreamSynchronize(b->stream));
-      HIP_CHECK(hipMemcpy(b->d_occ + occ_off, d->occupation, size_t(m->n_sites), hipMemcpyHostToDevice));
+      const int n_blocks = (m->n_sites + 255) / 256;
+      if (b->h_adm == nullptr) {
+        HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b->h_adm), 2 * sizeof(double)));
+        HIP_CHECK(hipMalloc(&b->d_adm_id, sizeof(int32_t)));
+        HIP_CHECK(hipMalloc(&b->d_adm_e, sizeof(double)));
+      }
+      if (n_blocks > b->adm_blocks) {
+        dev_free(b->d_adm_partial);
+        b->d_adm_partial = nullptr;
+        b->adm_blocks = 0;
+        // grow ahead so later, larger lattices rarely reallocate
+        const int cap = std::max(n_blocks, 64);
+        HIP_CHECK(hipMalloc(&b->d_adm_partial, sizeof(double) * size_t(cap)));
+        b->adm_blocks = cap;
+      }
+      HIP_CHECK(hipMemcpyAsync(b->d_occ + occ_off, d->occupation, size_t(m->n_sites), hipMemcpyHostToDevice,
+                               b->stream));
       if (!trk.empty())
-        HIP_CHECK(hipMemcpy(b->d_trk + trk_off, trk.data(), sizeof(uint16_t) * trk.size(), hipMemcpyHostToDevice));
-      HIP_CHECK(hipMemcpy(b->d_params + slot, &p, sizeof(ChainParams), hipMemcpyHostToDevice));
-      std::vector<double> energies;
-      compute_energy(b->d_models, b->params, b->d_params, b->d_occ, {slot}, &energies, b->stream);
+        HIP_CHECK(hipMemcpyAsync(b->d_trk + trk_off, trk.data(), sizeof(uint16_t) * trk.size(),
+                                 hipMemcpyHostToDevice, b->stream));
+      HIP_CHECK(hipMemcpyAsync(b->d_params + slot, &p, sizeof(ChainParams), hipMemcpyHostToDevice, b->stream));
+      *reinterpret_cast<int32_t*>(b->h_adm) = slot;
+      HIP_CHECK(hipMemcpyAsync(b->d_adm_id, b->h_adm, sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+      HIP_CHECK(launch_energy(b->d_models, b->d_params, b->d_occ, b->d_adm_id, 1, m->n_sites, b->d_adm_partial,
+                              b->d_adm_e, b->stream));
+      HIP_CHECK(hipMemcpyAsync(b->h_adm + 1, b->d_adm_e, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+      HIP_CHECK(hipStreamSynchronize(b->stream));
       ChainState st;
       std::memset(&st, 0, sizeof st);
-      st.energy = st.e_ref = energies[0];
+      st.energy = st.e_ref = b->h_adm[1];
       st.moves_done = d->first_move;
-      HIP_CHECK(hipMemcpy(b->d_states + slot, &st, sizeof(ChainState), hipMemcpyHostToDevice));
+      HIP_CHECK(hipMemcpyAsync(b->d_states + slot, &st, sizeof(ChainState), hipMemcpyHostToDevice, b->stream));
+      HIP_CHECK(hipStreamSynchronize(b->stream));
     } catch (...) {
       range_free(b->occ_free, occ_off, occ_len);
       if (!trk.empty()) range_free(b->trk_free, trk_off, int64_t(trk.size()));

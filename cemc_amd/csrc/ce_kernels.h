@@ -31,6 +31,10 @@ struct LaunchSweep {
   int nm;          // members gathered per record: 3, or 2 for records in VGPRs of models without quadruplets
   uint32_t* accept_log;  // optional: bit t of chain c at accept_log[c * log_words + t / 32]
   int64_t log_words;
+  // time-sliced launch: a chain stops at the first 64-move block that starts
+  // more than time_limit ticks of the constant-rate wall clock after the
+  // chain's own start (0: no limit; ignored by the local-field kernel)
+  int64_t time_limit = 0;
 };
 
 // Dynamic LDS of one chain: occupation bytes (padded box on periodic
@@ -83,9 +87,21 @@ hipError_t launch_cf(const ModelDev* models, const ChainParams* params, const in
 hipError_t launch_energy(const ModelDev* models, const ChainParams* params, const int8_t* occ_pool,
                          const int32_t* chain_ids, int n_chains, int max_sites, double* partial, double* energy_out,
                          hipStream_t stream);
+// Up to kInlineChanges changes of the updater interface travel in the kernel
+// arguments (4 ints each: site, new species, source slot, destination slot;
+// or, for a restore, site and species pairs), so a call needs no copy to the
+// device; longer sequences are read from `changes` in device memory.
+constexpr int kInlineChanges = 8;
+struct InlineChanges {
+  int32_t v[4 * kInlineChanges];
+};
 hipError_t launch_updater(const ModelDev* model, int8_t* occ, double* ring, double* slot_energy,
-                          const int32_t* changes, int n, const double* eci, double* scratch, hipStream_t stream);
-hipError_t launch_restore(int8_t* occ, const int32_t* sites, const int32_t* species, int n, hipStream_t stream);
+                          const InlineChanges& inl, const int32_t* changes, int n, const double* eci,
+                          double* scratch, hipStream_t stream);
+// Restores sites[i] = species[i] in order; n <= 2 * kInlineChanges pairs
+// come in `inl` (sites in v[0..n), species in v[n..2n)), more from `buf`
+// (sites then species).
+hipError_t launch_restore(int8_t* occ, const InlineChanges& inl, const int32_t* buf, int n, hipStream_t stream);
 hipError_t launch_slot_energy(const ModelDev* model, const double* ring, double* slot_energy, const double* eci,
                               int slot, hipStream_t stream);
 

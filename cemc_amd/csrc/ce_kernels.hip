@@ -207,7 +207,7 @@ __global__ __launch_bounds__(64) void cf_finalize_kernel(const ModelDev* __restr
 // the last change the energy of the final slot is stored.
 __global__ __launch_bounds__(256) void updater_kernel(const ModelDev* __restrict__ model,
                                                       int8_t* __restrict__ occ, double* __restrict__ ring,
-                                                      double* __restrict__ slot_energy,
+                                                      double* __restrict__ slot_energy, InlineChanges inl,
                                                       const int32_t* __restrict__ changes, int n,
                                                       const double* __restrict__ eci,
                                                       double* __restrict__ scratch) {
@@ -216,12 +216,13 @@ __global__ __launch_bounds__(256) void updater_kernel(const ModelDev* __restrict
   const int wave = tid >> 6;
   const int S = m.n_species;
   const int ncf = m.n_cf;
+  const bool in_args = n <= kInlineChanges;
   int last_dst = -1;
   for (int c = 0; c < n; ++c) {
-    const int site = changes[4 * c + 0];
-    const int nw = changes[4 * c + 1];
-    const int src = changes[4 * c + 2];
-    const int dst = changes[4 * c + 3];
+    const int site = in_args ? inl.v[4 * c + 0] : changes[4 * c + 0];
+    const int nw = in_args ? inl.v[4 * c + 1] : changes[4 * c + 1];
+    const int src = in_args ? inl.v[4 * c + 2] : changes[4 * c + 2];
+    const int dst = in_args ? inl.v[4 * c + 3] : changes[4 * c + 3];
     last_dst = dst;
     const int old = occ[site];
     __syncthreads();
@@ -300,11 +301,17 @@ __global__ __launch_bounds__(256) void updater_kernel(const ModelDev* __restrict
   }
 }
 
-__global__ void restore_kernel(int8_t* __restrict__ occ, const int32_t* __restrict__ sites,
-                               const int32_t* __restrict__ species, int n) {
+__global__ void restore_kernel(int8_t* __restrict__ occ, InlineChanges inl, const int32_t* __restrict__ buf,
+                               int n) {
   // Applied in order by one thread: later pops restore earlier states.
-  if (threadIdx.x == 0 && blockIdx.x == 0)
-    for (int i = 0; i < n; ++i) occ[sites[i]] = int8_t(species[i]);
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const bool in_args = n <= 2 * kInlineChanges;
+    for (int i = 0; i < n; ++i) {
+      const int site = in_args ? inl.v[i] : buf[i];
+      const int sp = in_args ? inl.v[n + i] : buf[n + i];
+      occ[site] = int8_t(sp);
+    }
+  }
 }
 
 __global__ void slot_energy_kernel(const ModelDev* __restrict__ model, const double* __restrict__ ring,
@@ -420,15 +427,18 @@ static const void* sweep_fn(bool periodic, bool canonical, int kr, bool glds, in
 bool sweep_variant_ok(int kr, bool glds, int kbm, int waves, int spec, bool periodic) {
   (void)glds;
   if (waves != 1 && waves != 2 && waves != 4 && waves != 8) return false;
-  if (spec != 1 && !(spec == kSpecMoves && periodic && kr > 0)) return false;
+  // spec: 1, kSpecMoves, or kSpecMoves * waves (wide rounds, 2-8 waves)
+  const bool wide = waves > 1 && spec == kSpecMoves * waves;
+  if (spec != 1 && !((spec == kSpecMoves || wide) && periodic && kr > 0)) return false;
   if (kr == 2 || kr == 4 || kr == 8) return kbm == 1 || kbm == 2 || kbm == 3;
   return waves == 1 && kr == 0 && (kbm == 1 || kbm == 2 || kbm == 3 || kbm == 7);
 }
 
 uint32_t sweep_static_lds() {
   return uint32_t(sizeof(double) * (kMaxSpecies * kMaxSpecies * kMaxBasis + kMaxSpecies * kMaxSpecies +
-                                    kMaxBasis * kMaxSpecies + 2 * 8 + kMaxBasis * 8 + 2 * kSpecMoves * 8) +
-                  sizeof(int) * 2 * kMaxGroups);
+                                    kMaxBasis * kMaxSpecies + 2 * 8 + kMaxBasis * 8 + 2 * kSpecMoves * 8 +
+                                    kSpecMoves * 8) +
+                  sizeof(int) * 2 * kMaxGroups + sizeof(int) * 2);
 }
 
 hipError_t sweep_set_max_lds(uint32_t bytes) {
@@ -438,9 +448,10 @@ hipError_t sweep_set_max_lds(uint32_t bytes) {
         for (int kr : {0, 2, 4, 8})
           for (int g = 0; g < 2; ++g)
             for (int kbm : {1, 2, 3, 7})
-              for (int sp : {1, kSpecMoves})
+              for (int sp : {1, kSpecMoves, kSpecMoves * w})
                 for (int hk = 0; hk < 2; ++hk)
                   for (int nm : {2, 3}) {
+              if (sp == kSpecMoves * w && w == 1) continue;
               if (!sweep_variant_ok(kr, g != 0, kbm, w, sp, p != 0)) continue;
               if (hk && (w != 1 || sp != 1)) continue;
               if (nm == 2 && kr == 0) continue;
@@ -509,7 +520,13 @@ hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream) {
   uint32_t* log = a.accept_log;
   int64_t log_words = a.log_words;
   HookDev* hooks = a.hooks;
-  void* args[] = {&params, &states, &models, &occ_pool, &trk_pool, &order, &n_moves, &log, &log_words, &hooks};
+  int64_t time_limit = a.time_limit;
+  if (a.field) {
+    void* fargs[] = {&params, &states, &models, &occ_pool, &trk_pool, &order, &n_moves, &log, &log_words, &hooks};
+    return hipLaunchKernel(fn, dim3(a.n_chains), dim3(64 * a.waves), fargs, a.lds_bytes, stream);
+  }
+  void* args[] = {&params, &states, &models, &occ_pool, &trk_pool, &order, &n_moves, &log, &log_words, &hooks,
+                  &time_limit};
   return hipLaunchKernel(fn, dim3(a.n_chains), dim3(64 * a.waves), args, a.lds_bytes, stream);
 }
 
@@ -572,14 +589,15 @@ hipError_t launch_energy(const ModelDev* models, const ChainParams* params, cons
 }
 
 hipError_t launch_updater(const ModelDev* model, int8_t* occ, double* ring, double* slot_energy,
-                          const int32_t* changes, int n, const double* eci, double* scratch, hipStream_t stream) {
-  hipLaunchKernelGGL(updater_kernel, dim3(1), dim3(256), 0, stream, model, occ, ring, slot_energy, changes, n,
+                          const InlineChanges& inl, const int32_t* changes, int n, const double* eci,
+                          double* scratch, hipStream_t stream) {
+  hipLaunchKernelGGL(updater_kernel, dim3(1), dim3(256), 0, stream, model, occ, ring, slot_energy, inl, changes, n,
                      eci, scratch);
   return hipGetLastError();
 }
 
-hipError_t launch_restore(int8_t* occ, const int32_t* sites, const int32_t* species, int n, hipStream_t stream) {
-  hipLaunchKernelGGL(restore_kernel, dim3(1), dim3(64), 0, stream, occ, sites, species, n);
+hipError_t launch_restore(int8_t* occ, const InlineChanges& inl, const int32_t* buf, int n, hipStream_t stream) {
+  hipLaunchKernelGGL(restore_kernel, dim3(1), dim3(64), 0, stream, occ, inl, buf, n);
   return hipGetLastError();
 }
 

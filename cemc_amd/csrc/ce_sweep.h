@@ -360,7 +360,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
                                                    const int32_t* __restrict__ chain_order,
                                                    const int64_t* __restrict__ n_moves,
                                                    uint32_t* __restrict__ accept_log, int64_t log_words,
-                                                   HookDev* __restrict__ hooks) {
+                                                   HookDev* __restrict__ hooks, int64_t time_limit) {
   static_assert(!kHooks || (kW == 1 && kK == 1), "constraints and bias run one wave, one move at a time");
   static_assert(kNM == 3 || kR > 0, "records read from LDS gather three members");
   extern __shared__ __align__(16) unsigned char lds[];
@@ -370,7 +370,8 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
   __shared__ int grp_tab[2 * kMaxGroups];
   __shared__ double red_move[2][8];             // per-move wave sums, double-buffered by move parity
   __shared__ double red_init[kMaxBasis][8];     // initial singlets
-  __shared__ double red_spec[2][kK > 1 ? kK * 8 : 1];  // speculative rounds: per-round wave sums
+  __shared__ double red_spec[2][kK == kSpecMoves ? kK * 8 : 1];  // speculative rounds: per-round wave sums
+  __shared__ double red_wide[kK > kSpecMoves ? kK : 1];            // wide rounds: the round's move sums
   constexpr int kRecLds = kR == 0;
   constexpr int kThreads = 64 * kW;
   const int chain = chain_order[blockIdx.x];
@@ -452,12 +453,14 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
   __syncthreads();
 
   // records of a single-group model stay in VGPRs for the whole launch
+  // (wide rounds: every wave holds all of them, laid out as for one wave)
+  constexpr bool kWide = kK > kSpecMoves;
   FatRec rr[kR > 0 ? kR : 1];
   if constexpr (kR > 0) {
     const int cnt = m.grp_rec_cnt[0];
 #pragma unroll
     for (int k = 0; k < kR; ++k) {
-      const int it = tid + kThreads * k;
+      const int it = kWide ? lane + 64 * k : tid + kThreads * k;
       rr[k] = it < cnt ? m.frecs[it] : FatRec{0, 0, 0, 0u, 0u, 0u, 0u, 0u};
     }
   }
@@ -480,7 +483,10 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
     double part_s[kBM];
 #pragma unroll
     for (int d = 0; d < kBM; ++d) part_s[d] = 0.0;
-    for (int i = tid; i < N; i += kThreads) {
+    // (wide rounds: every wave sums as one wave does, so the singlets and
+    // their sums equal the one-wave path's bitwise)
+    constexpr int kStride = kWide ? 64 : kThreads;
+    for (int i = kWide ? lane : tid; i < N; i += kStride) {
       if (has_bkg && m.background[i]) continue;
       const int s = occ[ref_of<kPeriodic>(c, i).p];
 #pragma unroll
@@ -489,7 +495,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
     }
 #pragma unroll
     for (int d = 0; d < kBM; ++d) {
-      const double tot = group_sum<kW>(part_s[d], red_init[d], wave, lane);
+      const double tot = kWide ? wave_sum_f64(part_s[d]) : group_sum<kW>(part_s[d], red_init[d], wave, lane);
       if (lane == d) sing = tot / double(N);
     }
   }
@@ -519,6 +525,25 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
     run = 0;
   };
   int64_t moves_made = nmov;
+  // Time-sliced launches (time_limit > 0): the chain stops at the first
+  // 64-move block that starts more than time_limit ticks of the constant-rate
+  // wall clock after its own start; the moves it made are recorded, and the
+  // next launch continues its stream where it stopped.  With several waves
+  // wave 0's clock reading decides for all of them (through LDS, double
+  // buffered by block parity, so every wave takes the same branch).
+  __shared__ int stop_flag[2];
+  const uint64_t t_begin = time_limit > 0 ? uint64_t(wall_clock64()) : 0u;
+  auto out_of_time = [&](int64_t blk0) -> bool {
+    if (time_limit <= 0 || blk0 == 0) return false;
+    int stop = int(uint64_t(wall_clock64()) - t_begin > uint64_t(time_limit));
+    if constexpr (kW > 1) {
+      int* f = &stop_flag[(blk0 >> 6) & 1];
+      if (tid == 0) *f = stop;
+      __syncthreads();
+      stop = *f;
+    }
+    return stop != 0;
+  };
   // constraints and bias of the chain (kHooks)
   [[maybe_unused]] uint32_t hk_fixed = 0u;
   [[maybe_unused]] const uint8_t* hk_allow = nullptr;
@@ -548,6 +573,10 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
   }
   if constexpr (kK == 1) {
   for (int64_t blk0 = 0; blk0 < nmov; blk0 += 64) {
+    if (out_of_time(blk0)) {
+      moves_made = blk0;
+      break;
+    }
     // lane l holds the Philox block and log u of move blk0 + l, and decodes
     // the parts of that move that do not depend on the state
     const Philox4 blk = philox4x64_10(t0 + uint64_t(blk0) + uint64_t(lane), 0, 0, 0, seed, uid);
@@ -820,7 +849,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
     }
   }
     flush();
-  } else {
+  } else if constexpr (kK == kSpecMoves) {
     // ---- rounds of kK = 8 moves (closed-form periodic lattice, records in
     // VGPRs, one translation group, no background).  The eight moves are
     // evaluated against the same state and their eight wave sums taken
@@ -839,6 +868,10 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
     const int sub = lane & 7;   // the earlier move it is checked against; its image (commit)
     uint32_t rnd = 0;
     for (int64_t blk0 = 0; blk0 < nmov; blk0 += 64) {
+      if (out_of_time(blk0)) {
+        moves_made = blk0;
+        break;
+      }
       const Philox4 blk = philox4x64_10(t0 + uint64_t(blk0) + uint64_t(lane), 0, 0, 0, seed, uid);
       const double log_u = log(unit_double(blk.w3));
       const int n_in = int(nmov - blk0 < 64 ? nmov - blk0 : 64);
@@ -1052,6 +1085,230 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
         l0 += stop;
         ++rnd;
         __atomic_signal_fence(__ATOMIC_SEQ_CST);  // as in the one-move loop
+      }
+      n_acc += uint64_t(__builtin_popcountll(acc_bits));
+      if (acc_log != nullptr && tid < 2) {
+        const int64_t w = blk0 >> 5;
+        if (lane == 0) acc_log[w] = uint32_t(acc_bits);
+        if (lane == 1 && n_in > 32) acc_log[w + 1] = uint32_t(acc_bits >> 32);
+      }
+    }
+    flush();
+  } else {
+    // ---- wide rounds of kK = 8 kW moves (closed-form periodic lattice,
+    // records in VGPRs, one translation group, no background).  Every wave
+    // holds all the records, laid out as for one wave, and wave w evaluates
+    // moves 8w .. 8w+7 of the round against the round's starting state.  The
+    // waves' move sums meet in LDS (one barrier); then every lane decides
+    // the move it decoded (lane l: move blk0 + l), and every wave finds the
+    // same cut: the first move that an accepted earlier move of the round
+    // affects (its sites or their sub-cluster members, or its tracker
+    // slots), where the round ends as in rounds of eight.  Wave w writes the
+    // commits of its own eight moves; a second barrier orders them before
+    // the next round's reads.  All waves carry the same energy, singlets and
+    // sums (wave 0's are written back).  Each move's energy change is the
+    // one-wave sum (same records per lane, same tree), so every value equals
+    // the sequential loop's bitwise.
+    static_assert(kPeriodic && kR > 0 && kW > 1 && kK == kSpecMoves * kW,
+                  "wide rounds need a periodic lattice, records in VGPRs and 8 moves per wave");
+    const uint64_t im0 = m.imask[0], im1 = m.imask[1];
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int grp = lane >> 3;  // which of this wave's eight moves the lane commits
+    const int sub = lane & 7;   // which image of it the lane writes
+    for (int64_t blk0 = 0; blk0 < nmov; blk0 += 64) {
+      if (out_of_time(blk0)) {
+        moves_made = blk0;
+        break;
+      }
+      const Philox4 blk = philox4x64_10(t0 + uint64_t(blk0) + uint64_t(lane), 0, 0, 0, seed, uid);
+      const double log_u = log(unit_double(blk.w3));
+      const int n_in = int(nmov - blk0 < 64 ? nmov - blk0 : 64);
+      uint32_t v_old = 0, v_new = 0;
+      int v_ta = 0, v_tb = 0;
+      int v_p = 0;
+      uint32_t v_cell = 0, v_img = 0;
+      if constexpr (kCanonical) {
+        const uint32_t ia = bounded32(uint32_t(blk.w0 >> 32), np);
+        uint32_t ib = ia + 1u + bounded32(uint32_t(blk.w0), np - 1u);
+        ib -= ib >= np ? np : 0u;
+        v_old = (present_pk >> (4u * ia)) & 15u;
+        v_new = (present_pk >> (4u * ib)) & 15u;
+        v_ta = unpack16(seg_lo, seg_hi, ia) + int(bounded64(blk.w1, uint64_t(unpack16(cnt_lo, cnt_hi, ia))));
+        v_tb = unpack16(seg_lo, seg_hi, ib) + int(bounded64(blk.w2, uint64_t(unpack16(cnt_lo, cnt_hi, ib))));
+      } else {
+        const Ref r = ref_of<true>(c, int(bounded64(blk.w1, uint64_t(N))));
+        v_p = r.p;
+        v_cell = uint32_t(r.x) | (uint32_t(r.y) << 8) | (uint32_t(r.z) << 16);
+        v_img = image_word<true>(c, r);
+        v_new = bounded32(uint32_t(blk.w0), uint32_t(c.S - 1));
+      }
+      uint64_t acc_bits = 0;
+      int l0 = 0;
+      while (l0 < n_in) {
+        const int n = n_in - l0 < kK ? n_in - l0 : kK;
+        // every lane looks up its own move in the current state
+        int p_i = v_p, p_j = 0;
+        uint32_t cell_i = v_cell, cell_j = 0, patch = 0, o_v = v_old, n_v = v_new;
+        uint32_t img_i = v_img, img_j = 0;
+        uint32_t sites = 0, slots = 0;  // canonical: si | sj << 16, ta | tb << 16
+        if constexpr (kCanonical) {
+          const int si = trk[v_ta], sj = trk[v_tb];
+          const Ref ri = ref_of<true>(c, si), rj = ref_of<true>(c, sj);
+          p_i = ri.p;
+          p_j = rj.p;
+          cell_i = uint32_t(ri.x) | (uint32_t(ri.y) << 8) | (uint32_t(ri.z) << 16);
+          cell_j = uint32_t(rj.x) | (uint32_t(rj.y) << 8) | (uint32_t(rj.z) << 16);
+          img_i = image_word<true>(c, ri);
+          img_j = image_word<true>(c, rj);
+          patch = wrapped_delta(c, ri, rj);
+          sites = uint32_t(si) | (uint32_t(sj) << 16);
+          slots = uint32_t(v_ta) | (uint32_t(v_tb) << 16);
+        } else {
+          o_v = occ[v_p];
+          n_v = o_v + 1u + v_new;
+          n_v -= n_v >= uint32_t(c.S) ? uint32_t(c.S) : 0u;
+        }
+        // this wave's eight moves: base + k
+        const int base = l0 + kSpecMoves * wv;
+        double part[kSpecMoves];
+#pragma unroll
+        for (int k = 0; k < kSpecMoves; ++k) {
+          const int lk = base + k < 63 ? base + k : 63;
+          Ref ri;
+          ri.p = __builtin_amdgcn_readlane(p_i, lk);
+          ri.site = 0;
+          ri.x = ri.y = ri.z = 0;
+          int pj_k = kCanonical ? __builtin_amdgcn_readlane(p_j, lk) : 0;
+          // as in rounds of eight: one move after the other where hoisting
+          // every move's loads would overflow the registers
+          if constexpr (kBM > 1 && (kR >= 4 || kW >= 8))
+            if (k > 0) asm volatile("" : "+s"(ri.p), "+s"(pj_k) : "v"(part[k - 1]));
+          double dphi[kBM];
+          if constexpr (kBM == 1) {
+            dphi[0] = 0.0;
+          } else {
+            const int pair =
+                int(__builtin_amdgcn_readlane(o_v, lk)) * c.S + int(__builtin_amdgcn_readlane(n_v, lk));
+#pragma unroll
+            for (int d = 0; d < kBM; ++d) dphi[d] = d < c.B ? dphi_tab[pair * c.B + d] : 0.0;
+          }
+          int gi_i[kR];
+          table_index<true, kR, kBM, false, kNM>(c, rr, ri, occ, 0u, 0u, gi_i);
+          double pk = table_sum<kR, kBM>(gi_i, gtab, dphi);
+          if constexpr (kCanonical) {
+            Ref rj;
+            rj.p = pj_k;
+            rj.site = 0;
+            rj.x = rj.y = rj.z = 0;
+            double dphj[kBM];
+#pragma unroll
+            for (int d = 0; d < kBM; ++d) dphj[d] = -dphi[d];
+            int gi_j[kR];
+            table_index<true, kR, kBM, true, kNM>(c, rr, rj, occ, __builtin_amdgcn_readlane(patch, lk),
+                                                  __builtin_amdgcn_readlane(n_v, lk), gi_j);
+            const double pj = table_sum<kR, kBM>(gi_j, gtab, dphj);
+            pk = kBM == 1 ? pk - pj : pk + pj;
+          }
+          part[k] = pk;
+        }
+        // lane (grp, sub) holds the sum of move base + grp
+        const double tot8 = wave_sum8_f64(part, lane);
+        if (sub == 0) red_wide[kSpecMoves * wv + grp] = tot8;
+        __syncthreads();
+        // every lane: its own move, round position r
+        const int r = lane - l0;
+        const bool valid = r >= 0 && r < n;
+        const double tot = red_wide[valid ? r : 0];
+        double dphi0 = 0.0, de_single = 0.0;
+        if constexpr (kBM == 1) {
+          dphi0 = o_v == 0 ? dphi01 : dphi10;
+          if constexpr (!kCanonical) de_single = o_v == 0 ? dsin01 : dsin10;
+        } else if constexpr (!kCanonical) {
+          de_single = dsingle_tab[int(o_v) * c.S + int(n_v)];
+        }
+        const double dE = kBM == 1 ? fma(dphi0, tot, de_single) : tot + de_single;
+        const double x = -dE * inv_kT;
+        const double margin = 1e-12 * (1.0 + fabs(x));
+        int verdict = (dE < 0.0 || log_u < x - margin) ? 1 : (log_u > x + margin ? 0 : 2);
+        if (valid && verdict == 2) verdict = int(unit_double(blk.w3) <= exp(-dE / kT));
+        const uint64_t acc = __builtin_amdgcn_ballot_w64(valid && verdict == 1);  // bit l: move blk0 + l
+        // the cut: the first move that an accepted earlier move affects
+        int stop = n;
+        uint64_t rest = acc;
+        while (rest != 0) {
+          const int i = int(__builtin_ctzll(rest));
+          if (i - l0 >= stop - 1) break;  // later moves past the cut need no check
+          rest &= rest - 1u;
+          const uint32_t ca = uint32_t(__builtin_amdgcn_readlane(cell_i, i));
+          bool hit = cells_interact(ca, cell_i, c, im0, im1);
+          if constexpr (kCanonical) {
+            const uint32_t cja = uint32_t(__builtin_amdgcn_readlane(cell_j, i));
+            const uint32_t sla = uint32_t(__builtin_amdgcn_readlane(slots, i));
+            const uint32_t ta_a = sla & 0xFFFFu, tb_a = sla >> 16, ta_b = slots & 0xFFFFu, tb_b = slots >> 16;
+            hit = hit || cells_interact(ca, cell_j, c, im0, im1) || cells_interact(cja, cell_i, c, im0, im1) ||
+                  cells_interact(cja, cell_j, c, im0, im1) || ta_a == ta_b || ta_a == tb_b || tb_a == ta_b ||
+                  tb_a == tb_b;
+          }
+          const uint64_t hits = __builtin_amdgcn_ballot_w64(hit && valid && lane > i);
+          if (hits != 0) {
+            const int first = int(__builtin_ctzll(hits)) - l0;
+            stop = first < stop ? first : stop;
+          }
+        }
+        const uint64_t below = l0 + stop >= 64 ? ~uint64_t(0) : (uint64_t(1) << (l0 + stop)) - 1u;
+        const uint64_t com = acc & below;
+        int cursor = l0;
+        if (com != 0) {
+          // wave w writes the commits of its moves base + grp: lane (grp, sub)
+          // image sub of the move's site i (and of site j), lanes 0 and 1 of
+          // the group the tracker entries
+          const int mb = base + grp;
+          const int src = 4 * (mb < 63 ? mb : 63);
+          const uint32_t wib = uint32_t(__builtin_amdgcn_ds_bpermute(src, int(img_i)));
+          const uint32_t onb = uint32_t(__builtin_amdgcn_ds_bpermute(src, int(o_v | (n_v << 4))));
+          uint32_t wjb = 0, slb = 0, stb = 0;
+          if constexpr (kCanonical) {
+            wjb = uint32_t(__builtin_amdgcn_ds_bpermute(src, int(img_j)));
+            slb = uint32_t(__builtin_amdgcn_ds_bpermute(src, int(slots)));
+            stb = uint32_t(__builtin_amdgcn_ds_bpermute(src, int(sites)));
+          }
+          if (mb < 64 && ((com >> (mb & 63)) & 1u) != 0) {
+            const int idx = image_of(c, wib, sub);
+            if (idx >= 0) occ[idx] = uint8_t(onb >> 4);
+            if constexpr (kCanonical) {
+              const int jdx = image_of(c, wjb, sub);
+              if (jdx >= 0) occ[jdx] = uint8_t(onb & 15u);
+              if (sub == 0) trk[slb & 0xFFFFu] = uint16_t(stb >> 16);  // trk[ta] = sj
+              if (sub == 1) trk[slb >> 16] = uint16_t(stb & 0xFFFFu);  // trk[tb] = si
+            }
+          }
+          // energy, singlets and sample sums in move order
+          uint64_t left = com;
+          while (left != 0) {
+            const int i = int(__builtin_ctzll(left));
+            left &= left - 1u;
+            run += i - cursor;
+            flush();
+            energy += readlane_f64(dE, i);
+            if constexpr (!kCanonical) {
+              const uint32_t oi = uint32_t(__builtin_amdgcn_readlane(o_v, i));
+              double ds;
+              if constexpr (kBM == 1) {
+                ds = __dmul_rn(oi == 0 ? dphi01 : dphi10, inv_n);
+              } else {
+                const int pair = int(oi) * c.S + int(__builtin_amdgcn_readlane(n_v, i));
+                ds = __dmul_rn(dphi_tab[pair * c.B + (lane < c.B ? lane : 0)], inv_n);
+              }
+              sing += ds;
+            }
+            run += 1;
+            cursor = i + 1;
+            acc_bits |= uint64_t(1) << i;
+          }
+        }
+        run += l0 + stop - cursor;
+        l0 += stop;
+        __syncthreads();  // the round's commits before the next round's reads
       }
       n_acc += uint64_t(__builtin_popcountll(acc_bits));
       if (acc_log != nullptr && tid < 2) {

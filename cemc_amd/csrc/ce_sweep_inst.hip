@@ -9,6 +9,9 @@
 //   2, 4 or 8 waves:     records in VGPRs only
 //   speculative rounds of kSpecMoves moves: periodic lattices, records in
 //                        VGPRs
+//   wide rounds of kSpecMoves x W moves (W = 2, 4, 8: each wave evaluates
+//                        its own eight): periodic lattices, every record in
+//                        each wave's VGPRs
 #include "ce_sweep.h"
 
 #ifndef CEMC_SWEEP_WAVES
@@ -76,6 +79,13 @@ const void* CEMC_SWEEP_FN_NAME(CEMC_SWEEP_WAVES, CEMC_SWEEP_PERIODIC)(bool canon
                        : lookup<false, kSpecMoves, false>(kr, glds, kbm, nm);
     }
     return nullptr;
+  }
+  if constexpr (P && W > 1) {
+    if (spec == kSpecMoves * W) {
+      if (kr == 0) return nullptr;
+      return canonical ? lookup<true, kSpecMoves * W, false>(kr, glds, kbm, nm)
+                       : lookup<false, kSpecMoves * W, false>(kr, glds, kbm, nm);
+    }
   }
   if (spec != 1) return nullptr;
   return canonical ? lookup<true, 1, false>(kr, glds, kbm, nm) : lookup<false, 1, false>(kr, glds, kbm, nm);

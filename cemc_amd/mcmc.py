@@ -560,7 +560,11 @@ class Montecarlo:
                 res = self._estimate_correlation_time()
             checks = prec_check_steps(10 * self.correlation_info["correlation_time"], int(steps))
         batch = self._ensure_batch()
+        # the reference calls reset() right before the production moves
+        # (montecarlo.py:796-800), which also re-arms the forced first move
         batch.reset_stats(0)
+        if self.accept_first_trial_move_after_reset:
+            self.is_first = True
         stats0 = batch.stats()
         steps = int(steps)
         done = 0
@@ -620,6 +624,8 @@ class Montecarlo:
         state = {"class": type(self).__name__, "calc": self.calc.backup_dict(), "temperature": self._T,
                  "seed": self.seed, "uid": self.uid, "moves": int(self._moves),
                  "recycle_waste": self.recycle_waste,
+                 "accept_first_trial_move_after_reset": self.accept_first_trial_move_after_reset,
+                 "is_first": bool(self.is_first),
                  "max_constraint_attempts": self.max_allowed_constraint_pass_attempts,
                  "chem_pot": getattr(self, "_chemical_potential", None),
                  "linear_vib": None if lv is None else {"eci_per_kbT": lv.eci_per_kbT, "included": lv.vibs_included,
@@ -639,11 +645,14 @@ class Montecarlo:
         kw = dict(seed=st["seed"], uid=st["uid"], recycle_waste=st["recycle_waste"],
                   max_constraint_attempts=st["max_constraint_attempts"])
         if st["class"] == "SGCMonteCarlo":
+            # the SGC sampler has no forced first move (sgc_montecarlo.py:31-36)
             mc = SGCMonteCarlo(calc.atoms, st["temperature"], symbols=list(calc.model.species), **kw)
             if st["chem_pot"] is not None:
                 mc.chemical_potential = st["chem_pot"]
         else:
-            mc = Montecarlo(calc.atoms, st["temperature"], **kw)
+            mc = Montecarlo(calc.atoms, st["temperature"],
+                            accept_first_trial_move_after_reset=st.get("accept_first_trial_move_after_reset", False),
+                            **kw)
         if st["linear_vib"] is not None:
             lv = LinearVibCorrection(st["linear_vib"]["eci_per_kbT"])
             lv.vibs_included = bool(st["linear_vib"]["included"])
@@ -651,6 +660,7 @@ class Montecarlo:
             calc._linear_vib_correction = lv
             calc.updater.add_linear_vib_correction(lv.eci_per_kbT)
         mc._moves = int(st["moves"])
+        mc.is_first = bool(st.get("is_first", mc.is_first))
         return mc
 
     def _observer_averages(self):

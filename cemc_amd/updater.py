@@ -18,10 +18,6 @@ import numpy as np
 from . import _capi
 from .model import CEModel
 
-# Boltzmann constant of the reference's vibrational correction
-# (cpp/src/linear_vib_correction.cpp:3).
-_KB_VIB = 8.6173303e-5
-
 
 class PyCEUpdater:
     def __init__(self, atoms, bc, corr_func, eci, model=None):
@@ -93,13 +89,29 @@ class PyCEUpdater:
         _capi.check(rc)
 
     def calculate(self, system_changes):
-        site, old, new = self._ids(list(system_changes))
+        changes = list(system_changes)
+        n = len(changes)
+        sid = self.model.sid
+        arr = ctypes.c_int32 * n
+        try:
+            site = arr(*[int(c[0]) for c in changes])
+            old = arr(*[sid[c[1]] for c in changes])
+            new = arr(*[sid[c[2]] for c in changes])
+        except KeyError as exc:
+            raise ValueError("unknown symbol %s" % exc)
         e = ctypes.c_double(0.0)
-        s, sp = _capi.i32(site)
-        o, opp = _capi.i32(old)
-        n, np_ = _capi.i32(new)
-        rc = self._lib.cemc_updater_calculate(self.handle, len(site), sp, opp, np_, ctypes.byref(e))
-        self._sync_atoms()
+        rc = self._lib.cemc_updater_calculate(self.handle, n, site, old, new, ctypes.byref(e))
+        if rc == _capi.CEMC_OK:
+            # the changes were made as given: set the same symbols in atoms
+            # (ce_updater.cpp:338-347) without reading the lattice back
+            atoms, occ = self.atoms, self._occ
+            for c in changes:
+                i = int(c[0])
+                if occ[i] != sid[c[2]]:
+                    atoms[i].symbol = c[2]
+                    occ[i] = sid[c[2]]
+        else:
+            self._sync_atoms()
         _capi.check(rc)
         return e.value
 
@@ -126,8 +138,11 @@ class PyCEUpdater:
         _capi.check(self._lib.cemc_updater_set_ecis(self.handle, p))
 
     def get_singlets(self):
-        cf = self.get_cf_array()
-        return np.array([cf[k] for k, n in enumerate(self.model.names) if n.startswith("c1")])
+        n = ctypes.c_int32(0)
+        _capi.check(self._lib.cemc_updater_get_singlets(self.handle, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value)
+        _capi.check(self._lib.cemc_updater_get_singlets(self.handle, out.ctypes.data_as(_capi._dp), n.value, None))
+        return out
 
     def get_symbols(self):
         occ = np.zeros(self.model.n_sites, dtype=np.int8)
@@ -165,23 +180,27 @@ class PyCEUpdater:
         return v.value
 
     def set_num_threads(self, num_threads):
-        """Kept for interface compatibility; the work is spread over GPU lanes."""
+        """Stored for interface compatibility; the work is spread over GPU lanes."""
         if int(num_threads) <= 0:
             raise ValueError("num_threads must be positive")
+        _capi.check(self._lib.cemc_updater_set_num_threads(self.handle, int(num_threads)))
         self._num_threads = int(num_threads)
 
     def add_linear_vib_correction(self, eci_per_kbT):
         """LinearVibCorrection (cpp/src/linear_vib_correction.cpp:6-16)."""
+        pos = {n: k for k, n in enumerate(self.model.names)}
+        unknown = sorted(set(eci_per_kbT) - set(pos))
+        if unknown:
+            raise ValueError("no correlation function named %s" % unknown[0])
+        idx, ip = _capi.i32([pos[n] for n in eci_per_kbT])
+        val, vp = _capi.f64([float(v) for v in eci_per_kbT.values()])
+        _capi.check(self._lib.cemc_updater_add_linear_vib_correction(self.handle, len(idx), ip, vp))
         self._vib = dict(eci_per_kbT)
 
     def vib_energy(self, T):
-        if self._vib is None:
-            return 0.0
-        cf = self.get_cf()
-        e = 0.0
-        for name in sorted(self._vib):
-            e += cf[name] * self._vib[name] * _KB_VIB * T
-        return e
+        e = ctypes.c_double(0.0)
+        _capi.check(self._lib.cemc_updater_vib_energy(self.handle, float(T), ctypes.byref(e)))
+        return e.value
 
     def close(self):
         if getattr(self, "handle", None):

@@ -132,6 +132,25 @@ CEMC_API int cemc_updater_get_energy(cemc_updater* upd, double* energy);
 CEMC_API int cemc_updater_get_cf(cemc_updater* upd, double* cf);
 /* PyCEUpdater.set_ecis / CEUpdater::set_ecis (ce_updater.cpp:615-629); n_cf values. */
 CEMC_API int cemc_updater_set_ecis(cemc_updater* upd, const double* eci);
+/* PyCEUpdater.get_singlets / CEUpdater::get_singlets (ce_updater.cpp:655-683):
+ * the singlet correlation functions of the current state in name order.
+ * *n_singlets (if not NULL) receives their number; singlets may be NULL to
+ * ask for it.  A capacity below that number fails with CEMC_ERUNTIME and the
+ * reference's message. */
+CEMC_API int cemc_updater_get_singlets(cemc_updater* upd, double* singlets, int32_t capacity, int32_t* n_singlets);
+/* PyCEUpdater.add_linear_vib_correction / CEUpdater::add_linear_vib_correction
+ * (ce_updater.cpp:685-689, 1002-1013): replaces the correction with
+ * v[i] eV per kB T on correlation function cf_index[i] (name-order index). */
+CEMC_API int cemc_updater_add_linear_vib_correction(cemc_updater* upd, int32_t n, const int32_t* cf_index,
+                                                    const double* eci_per_kbT);
+/* PyCEUpdater.vib_energy / CEUpdater::vib_energy (ce_updater.cpp:691-698,
+ * linear_vib_correction.cpp:8-16): sum over the correction's functions in
+ * name order of cf_k v_k kB T with kB = 8.6173303e-5 eV/K; 0 without one. */
+CEMC_API int cemc_updater_vib_energy(cemc_updater* upd, double temperature, double* energy);
+/* PyCEUpdater.set_num_threads / CEUpdater::set_num_threads (ce_updater.hpp:136):
+ * accepted and stored (positive values only); the work is spread over GPU
+ * lanes, so the count does not change the arithmetic or the results. */
+CEMC_API int cemc_updater_set_num_threads(cemc_updater* upd, int32_t num_threads);
 /* PyCEUpdater.get_symbols (ce_updater.hpp:111) as species ids; n_sites values. */
 CEMC_API int cemc_updater_get_occupation(cemc_updater* upd, int8_t* occupation);
 /* Number of history entries (CFHistoryTracker::history_size, cf_history_tracker.hpp:48). */
@@ -171,6 +190,15 @@ CEMC_API int cemc_batch_run(cemc_batch* batch, const int64_t* n_moves, int64_t m
 /* Same as cemc_batch_run, also returning the launch's device time in ms
  * measured with HIP events on the batch's stream. */
 CEMC_API int cemc_batch_run_timed(cemc_batch* batch, const int64_t* n_moves, int64_t moves, double* kernel_ms);
+/* Time-sliced run for servers: like cemc_batch_run_timed, but each chain
+ * stops at the first 64-move block that starts more than max_ms after its
+ * own start (device wall clock), having made fewer than n_moves[c] moves if
+ * its time ran out; cemc_batch_get_stats reports the moves each chain has
+ * made.  A chain's moves are the same however its stream is cut into
+ * launches.  Chains on the local-field energy path make all their moves.
+ * The reference has no counterpart (its samplers run one chain). */
+CEMC_API int cemc_batch_run_sliced(cemc_batch* batch, const int64_t* n_moves, int64_t moves, double max_ms,
+                                   double* kernel_ms);
 /* Same as cemc_batch_run, also recording accept (1) / reject (0) of every
  * move: bit t of chain c at accept_bits[c * words_per_chain + t / 32], bit
  * t % 32.  Used to compare acceptance sequences with the reference. */

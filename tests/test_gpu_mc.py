@@ -463,3 +463,38 @@ def test_accept_first_trial_move_after_reset(gpu):
     assert b1 == 1 and np.array_equal(calc.updater.get_occupation(), ora.get_occupation())
     with pytest.raises(NotImplementedError):
         Montecarlo(atoms, T, recycle_waste=True, accept_first_trial_move_after_reset=True)
+
+
+def test_accept_first_move_rearmed_by_each_runmc_and_kept_by_save(gpu, tmp_path):
+    """runMC resets right before its production moves (montecarlo.py:796-800),
+    so each of two consecutive runMC calls starts with a forced move; a saved
+    sampler keeps the flag and its pending forced move."""
+    s, eci, occ, atoms = _system(4, ["Al", "Mg"], {2: 2, 3: 1}, 112)
+    calc = CE(atoms, s, eci=dict(eci))
+    T = 5.0
+    mc = Montecarlo(atoms, T, seed=32, uid=0, accept_first_trial_move_after_reset=True)
+    mc.runMC(steps=10, equil=False)
+    mc.runMC(steps=10, equil=False)
+    ora = _oracle(s, occ, eci)
+    acc = 0
+    for begin in (0, 10):
+        _, _, a1 = ora.run(0, 32, 0, 1, 1e300, move_begin=begin)
+        _, _, a2 = ora.run(0, 32, 0, 9, T, move_begin=begin + 1)
+        assert a1 == 1
+        acc = a2
+    assert mc.num_accepted == 1 + acc
+    assert np.array_equal(calc.updater.get_occupation(), ora.get_occupation())
+    mc.reset()
+    fname = str(tmp_path / "mc.json")
+    mc.save(fname)
+    loaded = Montecarlo.load(fname)
+    assert loaded.accept_first_trial_move_after_reset and loaded.is_first
+    loaded._advance(3)
+    loaded._sync_back()
+    # a loaded canonical chain rebuilds its species lists from the lattice in
+    # ascending site order: the oracle restarts from the saved lattice too
+    ora2 = _oracle(s, ora.get_occupation(), eci)
+    _, _, b1 = ora2.run(0, 32, 0, 1, 1e300, move_begin=20)
+    ora2.run(0, 32, 0, 2, T, move_begin=21)
+    assert b1 == 1
+    assert np.array_equal(loaded.calc.updater.get_occupation(), ora2.get_occupation())

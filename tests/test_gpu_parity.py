@@ -186,3 +186,77 @@ def test_updater_matches_oracle_with_equivalent_sites(gpu):
         cur = [s.species.index(v) for v in upd.get_symbols()]
         assert cur == list(ora.get_occupation())
         assert np.max(np.abs(upd.get_cf_array() - ora.get_cf_array())) <= 1e-12
+
+
+@pytest.mark.parametrize("fname,n", [("binary_canonical_plain.npz", 3000), ("ternary_sgc.npz", 3000),
+                                     ("quaternary_canonical.npz", 400)])
+def test_dropin_updater_loop_reproduces_reference_chain(gpu, fname, n):
+    """The drop-in path: the per-move loop a caller of PyCEUpdater runs (one
+    calculate, then clear_history or undo_changes, per move, as the
+    reference's Montecarlo._mc_step does through CE) reproduces the
+    reference's chain move for move on the fed stream."""
+    from cemc_amd.dropin import run_updater_chain
+    from cemc_amd.lattice import SimpleAtoms
+    from cemc_amd.updater import PyCEUpdater
+    fx = ChainFixture(fname)
+    species = sorted(fx.settings.unique_elements)
+    atoms = SimpleAtoms([species[s] for s in fx.occ0])
+    upd = PyCEUpdater(atoms, fx.settings, dict(fx.cf0), dict(fx.eci_run))
+    print("drop-in loop", fname, n, flush=True)
+    bits, e_trace, _, final, n_acc = run_updater_chain(upd, atoms, species, fx.kind, fx.seed, fx.chain_uid, n,
+                                                       fx.temperature, trace_every=fx.trace_every)
+    assert np.array_equal(_bits(bits, n), _bits(fx.accept_bits, n))
+    ref_e = fx.e_trace[:len(e_trace)]
+    assert np.max(np.abs(e_trace - ref_e)) / fx.n_sites <= E_TOL
+    if n == fx.n_moves:
+        assert np.array_equal(final, fx.occ_final)
+        assert n_acc == fx.n_accept
+    # the atoms object follows the lattice, as the reference's updater keeps it
+    assert [a.symbol for a in atoms] == upd.get_symbols()
+
+
+def test_updater_singlets_vibrations_and_threads(gpu):
+    """get_singlets, add_linear_vib_correction / vib_energy and
+    set_num_threads through the C ABI (ce_updater.cpp:655-698,
+    linear_vib_correction.cpp:8-16)."""
+    import ctypes
+
+    from cemc_amd import _capi
+    from cemc_amd.lattice import FCCSettings, SimpleAtoms, random_occupation, seeded_ecis
+    from cemc_amd.updater import PyCEUpdater
+    from oracle.brute_force import brute_force_cf
+    s = FCCSettings(5, ["Al", "Mg", "Si"], {2: 2, 3: 1})
+    names = s.cf_names()
+    eci = seeded_ecis(s, 3, 0.02)
+    occ = random_occupation(s.n_sites, 3, 8)
+    atoms = SimpleAtoms([s.species[v] for v in occ])
+    upd = PyCEUpdater(atoms, s, None, dict(eci))
+    upd.calculate([(0, atoms[0].symbol, "Si" if atoms[0].symbol != "Si" else "Al")])
+    upd.clear_history()
+    cf = upd.get_cf()
+    sing = upd.get_singlets()
+    assert np.array_equal(sing, [cf["c1_0"], cf["c1_1"]])
+    assert np.allclose(sing, [brute_force_cf(s, upd.get_occupation(), names)[k] for k in ("c1_0", "c1_1")],
+                       atol=1e-12)
+    lib = _capi.load()
+    small = np.zeros(1)
+    rc = lib.cemc_updater_get_singlets(upd.handle, small.ctypes.data_as(_capi._dp), 1, None)
+    assert rc == _capi.CEMC_ERUNTIME
+    assert "too small to hold all the singlets" in lib.cemc_last_error().decode()
+    assert upd.vib_energy(800.0) == 0.0
+    pair = [n for n in names if n.startswith("c2")][0]
+    corr = {pair: 0.3, "c1_1": -0.2, "c0": 0.05}
+    upd.add_linear_vib_correction(corr)
+    kb = 8.6173303e-5
+    want = 0.0
+    for name in sorted(corr):
+        want += cf[name] * corr[name] * kb * 800.0
+    assert upd.vib_energy(800.0) == want
+    with pytest.raises(ValueError):
+        upd.add_linear_vib_correction({"c9_nonexistent": 1.0})
+    upd.set_num_threads(4)
+    with pytest.raises(ValueError):
+        upd.set_num_threads(0)
+    n = ctypes.c_int32(0)
+    _capi.check(lib.cemc_updater_get_singlets(upd.handle, None, 0, ctypes.byref(n)))
+    assert n.value == 2
