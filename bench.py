@@ -39,6 +39,103 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# What bounds the sweep kernel: neither bandwidth (the algorithmic bytes are
+# a few % of the measured LDS read rate and the lattices never leave LDS
+# during a launch) but each chain's dependent move sequence (evaluate ->
+# reduce -> decide -> commit), which the profiles' SQ counters show as a low
+# VALU issue fraction and waves that wait on their LDS and HBM table reads.
+LIMITER = ("latency of each chain's dependent move sequence: per-chain issue and LDS/L2 latency, not HBM or "
+           "LDS bandwidth (SQ counters in profiles/)")
+
+
+def roofline_of(algorithmic_bytes, kernel_ms, lds_peak):
+    """Roofline object of one launch: algorithmic bytes / launch time against
+    HBM's published peak, and against the LDS read rate measured here."""
+    achieved = algorithmic_bytes / (kernel_ms * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": None, "limiter": LIMITER}
+    if lds_peak:
+        out["lds_peak_measured"] = lds_peak
+        out["lds_frac"] = achieved / lds_peak
+    return out
+
+
+def oracle_sample(models, chains, occs, cfs, kind, seconds, what):
+    """CPU baseline: the oracle (C restatement) on up to 16 host threads, one
+    chain each, from the given lattices (their correlation functions seed
+    the oracle), for about `seconds`."""
+    from oracle.oracle import OracleModel, OracleUpdater, run_many
+    threads = max(1, min(len(models), int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1), 16))
+    models, chains, occs, cfs = models[:threads], chains[:threads], occs[:threads], cfs[:threads]
+    omodels = {}
+    ups, uids, temps = [], [], []
+    for m, ch, occ, cf in zip(models, chains, occs, cfs):
+        if id(m) not in omodels:
+            omodels[id(m)] = OracleModel(m.settings, m.names)
+        eci = dict(zip(m.names, np.asarray(m.eci).tolist()))
+        for name, mu in (ch.get("chem_pot") or {}).items():
+            eci[name] -= mu
+        ups.append(OracleUpdater(omodels[id(m)], occ, dict(zip(m.names, np.asarray(cf).tolist())), eci))
+        uids.append(ch["uid"])
+        temps.append(ch["temperature"])
+    seed = chains[0]["seed"]
+    n = 64
+    t0 = time.perf_counter()
+    run_many(ups, kind, seed, uids, n, temps, threads, move_begin=1 << 40)
+    dt = max(time.perf_counter() - t0, 1e-4)
+    n = int(max(64, min(n * seconds / dt, 5e7)))
+    t0 = time.perf_counter()
+    run_many(ups, kind, seed, uids, n, temps, threads, move_begin=(1 << 40) + 64)
+    dt = time.perf_counter() - t0
+    return {"value": threads * n / dt, "unit": "moves/s", "cores": threads, "kind": "port",
+            "sample": "%s, %d moves each, %.1f s" % (what, n, dt)}
+
+
+def reference_sample(kind, seconds, procs):
+    """The reference's own CEUpdater (oracle/_ref, built from the reference's
+    sources) driven one move at a time by the fed-stream sampler loop, in
+    `procs` parallel processes; None when its interpreter or build is absent."""
+    import subprocess
+    py39 = "/opt/conda/bin/python3.9"
+    so_dir = os.path.join(ROOT, "oracle", "_ref")
+    if not (os.path.exists(py39) and os.path.isdir(so_dir) and any(f.startswith("cemc_ref") for f in os.listdir(so_dir))):
+        return None
+    try:
+        res = subprocess.run([py39, os.path.join(ROOT, "oracle", "ref_rate.py"), kind, str(seconds), str(procs)],
+                             capture_output=True, text=True, timeout=seconds * 6 + 120)
+    except subprocess.TimeoutExpired:
+        return None
+    if res.returncode != 0:
+        log("reference sample failed: " + res.stderr[-400:])
+        return None
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+def dropin_rate(settings, eci, seconds=3.0):
+    """The drop-in path on the GPU: cemc_amd.updater.PyCEUpdater driven one
+    move at a time by the same fed-stream loop the reference's samplers use
+    (one calculate, then clear_history or undo_changes, per move), on the
+    plain configuration (canonical swaps at 800 K)."""
+    from cemc_amd.dropin import run_updater_chain
+    from cemc_amd.lattice import SimpleAtoms, random_occupation
+    from cemc_amd.updater import PyCEUpdater
+    occ = random_occupation(settings.n_sites, 2, 1)
+    atoms = SimpleAtoms([settings.species[v] for v in occ])
+    upd = PyCEUpdater(atoms, settings, None, dict(eci))
+    n = 200
+    t0 = time.perf_counter()
+    run_updater_chain(upd, atoms, settings.species, 0, 42, 0, n, 800.0)
+    dt = time.perf_counter() - t0
+    n = int(max(200, n * seconds / max(dt, 1e-3)))
+    t0 = time.perf_counter()
+    _, _, _, _, acc = run_updater_chain(upd, atoms, settings.species, 0, 42, 1, n, 800.0)
+    dt = time.perf_counter() - t0
+    upd.close()
+    return {"moves_per_s": n / dt, "moves": n, "accepted": acc, "seconds": dt,
+            "workload": "PyCEUpdater on binary FCC 10^3, canonical swaps at 800 K, one calculate + "
+                        "clear_history/undo_changes per move from a Python loop (the reference's per-move protocol)"}
+
+
 def binary_model():
     from cemc_amd.lattice import FCCSettings, seeded_ecis
     from cemc_amd.model import CEModel
@@ -62,7 +159,7 @@ def batched_chains(settings, n_chains=1024):
     return chains
 
 
-def ternary_uneven(n_chains=256, sweeps=200, seed=3):
+def ternary_uneven(n_chains=256, sweeps=200, seed=3, lds_peak=None, cpu=True):
     """256 canonical chains, ternary FCC with 2 polynomial basis functions,
     pairs to the 4th shell, triplets to the 3rd, quadruplets to the 2nd,
     ECIs N(0, 0.01 eV)/d^2; cells L^3 with L uniform in 8..16 (512-4096
@@ -100,16 +197,27 @@ def ternary_uneven(n_chains=256, sweeps=200, seed=3):
     shapes = sorted({batch.launch_shape(c) for c in range(n_chains)})
     total = int(moves.sum())
     m0 = models[int(sizes[0])][1]
-    return {"workload": "ternary FCC, 256 canonical chains, L^3 with L in 8..16 (512-4096 sites), pairs<=4th, "
-                        "triplets<=3rd, quadruplets<=2nd shell, %d sweeps each, one launch" % sweeps,
-            "value": total / wall, "unit": "moves/s", "moves": total, "kernel_ms": kernel_ms, "wall_s": wall,
-            "bytes_per_move": 10.0 * m0.gathers_per_flip(), "launch_shapes": [list(x) for x in shapes],
-            "shape": autotune.shape_name(tune_w),
-            "autotune_moves_per_s": {autotune.shape_name(k): v for k, v in tune_rates.items()},
-            "acceptance": float(batch.stats()["accepted"].sum() / max(total + int((moves // 20).sum()), 1))}
+    bytes_per_move = 10.0 * m0.gathers_per_flip()
+    out = {"workload": "ternary FCC, 256 canonical chains, L^3 with L in 8..16 (512-4096 sites), pairs<=4th, "
+                       "triplets<=3rd, quadruplets<=2nd shell, %d sweeps each, one launch" % sweeps,
+           "value": total / wall, "unit": "moves/s", "moves": total, "kernel_ms": kernel_ms, "wall_s": wall,
+           "bytes_per_move": bytes_per_move, "launch_shapes": [list(x) for x in shapes],
+           "shape": autotune.shape_name(tune_w),
+           "autotune_moves_per_s": {autotune.shape_name(k): v for k, v in tune_rates.items()},
+           "acceptance": float(batch.stats()["accepted"].sum() / max(total + int((moves // 20).sum()), 1)),
+           "roofline": roofline_of(bytes_per_move * total, kernel_ms, lds_peak)}
+    if cpu:
+        # 16 of the chains (one per host thread) from their current lattices
+        pick = list(range(0, n_chains, max(1, n_chains // 16)))[:16]
+        out["cpu_baseline"] = oracle_sample([chain_models[c] for c in pick], [chains[c] for c in pick],
+                                            [batch.occupation(c) for c in pick], [batch.cf(c)[0] for c in pick],
+                                            kind=CANONICAL, seconds=6.0,
+                                            what="16 of the 256 chains (one per thread, sizes as drawn)")
+    batch.close()
+    return out
 
 
-def tempering(n_rep=128, cycles=2000, L=32, seed=9):
+def tempering(n_rep=128, cycles=2000, L=32, seed=9, lds_peak=None, cpu=True):
     """Quaternary FCC 32^3 (32768 sites), 3 basis functions, pairs,
     triplets and quadruplets to the 3rd shell; 128 replicas on
     geomspace(200, 2000) K; one canonical sweep per replica then one
@@ -136,14 +244,26 @@ def tempering(n_rep=128, cycles=2000, L=32, seed=9):
     state = pt.state()
     total = cycles * n_rep * st.n_sites
     acc = state["n_accept"].sum() / max(state["n_try"].sum(), 1)
-    return {"workload": "quaternary FCC 32^3 (32768 sites), <=4-body to the 3rd shell (%d CFs, %d sub-clusters per "
-                        "site), %d replicas 200-2000 K, exchange every sweep, %d sweeps, HIP-graph cycles"
-                        % (model.n_cf, model.n_subclusters, n_rep, cycles),
-            "value": total / wall, "unit": "moves/s", "moves": total, "kernel_ms": kernel_ms, "wall_s": wall,
-            "bytes_per_move": 10.0 * model.gathers_per_flip(), "launch_shape": list(batch.launch_shape(0)),
-            "shape": autotune.shape_name(tune_w),
-            "autotune_moves_per_s": {autotune.shape_name(k): v for k, v in tune_rates.items()},
-            "exchange_accept_rate": float(acc)}
+    bytes_per_move = 10.0 * model.gathers_per_flip()
+    out = {"workload": "quaternary FCC 32^3 (32768 sites), <=4-body to the 3rd shell (%d CFs, %d sub-clusters per "
+                       "site), %d replicas 200-2000 K, exchange every sweep, %d of the workload's 2000 sweeps "
+                       "(a rate per cycle: every cycle is the same sweep + exchange)"
+                       % (model.n_cf, model.n_subclusters, n_rep, cycles),
+           "cycles": cycles, "value": total / wall, "unit": "moves/s", "moves": total, "kernel_ms": kernel_ms,
+           "wall_s": wall, "bytes_per_move": bytes_per_move, "launch_shape": list(batch.launch_shape(0)),
+           "shape": autotune.shape_name(tune_w),
+           "autotune_moves_per_s": {autotune.shape_name(k): v for k, v in tune_rates.items()},
+           "exchange_accept_rate": float(acc),
+           "roofline": roofline_of(bytes_per_move * total, kernel_ms, lds_peak)}
+    if cpu:
+        pick = list(range(0, n_rep, n_rep // 16))[:16]
+        out["cpu_baseline"] = oracle_sample([model] * len(pick), [chains[c] for c in pick],
+                                            [batch.occupation(c) for c in pick], [batch.cf(c)[0] for c in pick],
+                                            kind=CANONICAL, seconds=6.0,
+                                            what="16 of the 128 replicas (one per thread), sweeps without exchange")
+    pt.close()
+    batch.close()
+    return out
 
 
 def wang_landau(model, settings, n_walkers=1024, n_checks=4, check_every=5000):
@@ -177,9 +297,14 @@ def serving(n_jobs=5000, rate=200.0):
     setup = time.perf_counter() - t0
     server = JobServer(max_chains=4096, hbm_bytes=1 << 30)
     res = server.serve(jobs, realtime=True, progress=log)
+    moves = np.array([j.n_moves for j in jobs], dtype=np.float64)
+    span = max(j.arrival for j in jobs)
     res.update({"workload": "%d Poisson jobs at %.0f/s: binary or ternary FCC L^3, L in 6..24, canonical or SGC, "
                             "T in 300-1500 K, 100-5000 sweeps; continuous batching on one GPU" % (n_jobs, rate),
-                "setup_s": setup, "models": len(cache)})
+                "setup_s": setup, "models": len(cache), "rejected": len(server.rejected),
+                # the load the stream offers and the arrival rate the measured throughput would sustain
+                "offered_moves_per_s": float(moves.sum() / span),
+                "sustainable_jobs_per_s": res["throughput_moves_per_s"] / float(moves.mean())})
     return res
 
 
@@ -235,13 +360,21 @@ def cpu_baseline(settings, eci, chains, target_s=12.0):
                     break
     except OSError:
         pass
-    return {"value": threads * per_thread / dt, "unit": "moves/s", "cores": threads, "kind": "port",
+    port = {"value": threads * per_thread / dt, "unit": "moves/s", "cores": threads, "kind": "port",
             "sample": "%d independent SGC chains of the same workload (one per thread), %d flips each, %.1f s"
                       % (threads, per_thread, dt),
             "reference_shaped": {"workload": "one canonical chain of the plain configuration (binary FCC 10^3, "
                                              "800 K), CF updates of each flip split over T threads; swaps/s by T",
-                                 "moves_per_s": shaped},
-            "cpu_model": cpu_model, "nproc": os.cpu_count()}
+                                 "moves_per_s": shaped}}
+    ref = reference_sample("sgc", 8.0, threads)
+    if ref is None:
+        port.update({"cpu_model": cpu_model, "nproc": os.cpu_count()})
+        return port
+    return {"value": ref["moves_per_s"], "unit": "moves/s", "cores": ref["processes"], "kind": "reference",
+            "sample": "the reference's CEUpdater (compiled from its sources) in %d processes, one SGC chain of the "
+                      "workload each (its T and mu), driven one flip at a time from Python as its samplers do; "
+                      "%d flips in %.1f s" % (ref["processes"], ref["moves"], ref["wall_s"]),
+            "port": port, "cpu_model": cpu_model, "nproc": os.cpu_count()}
 
 
 def copy_probe(mib):
@@ -269,8 +402,8 @@ def main():
     ap.add_argument("--sweeps", type=int, default=500)
     ap.add_argument("--chains", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workloads", default="batched,ternary,tempering,serving,wang_landau",
-                    help="comma list of batched, ternary, tempering, serving, wang_landau")
+    ap.add_argument("--workloads", default="batched,dropin,ternary,tempering,serving,wang_landau",
+                    help="comma list of batched, dropin, ternary, tempering, serving, wang_landau")
     ap.add_argument("--shape", default=None,
                     help="'fields' (per-site fields, one wave per chain) or WxK (waves per chain, moves per round, "
                          "tables summed per move) for the batched workload; skips the autotuner")
@@ -332,13 +465,10 @@ def main():
 
     bytes_per_flip = 5.0 * model.gathers_per_flip()
     launch_bytes = bytes_per_flip * moves_per_chain * len(chains)
-    achieved = launch_bytes / (avg_kernel_ms * 1e-3) / 1e9
     default_shape = len(chains) == 1024 and args.sweeps == 500 and tuple(tune_w) == TRAFFIC_SHAPE
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_DEFAULT_LAUNCH if default_shape else None,
-                "lds_peak_measured": lds_meas, "lds_frac": achieved / lds_meas,
-                "hbm_copy_measured": hbm_meas,
-                "bytes_per_move": bytes_per_flip, "kernel_ms": avg_kernel_ms}
+    roofline = roofline_of(launch_bytes, avg_kernel_ms, lds_meas)
+    roofline.update({"traffic": TRAFFIC_DEFAULT_LAUNCH if default_shape else None, "hbm_copy_measured": hbm_meas,
+                     "bytes_per_move": bytes_per_flip, "kernel_ms": avg_kernel_ms})
     out = {
         "metric": METRIC, "value": value, "unit": "moves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -348,6 +478,8 @@ def main():
                                "per step, one persistent launch" % (len(chains), args.sweeps),
                    "chains": len(chains), "sweeps_per_step": args.sweeps, "sites": settings.n_sites},
         "plain_moves_per_s": plain_value, "batched_over_plain": value / plain_value,
+        # the batched workload counts SGC flips, the plain one canonical swaps (two flips each)
+        "batched_over_plain_flips": value / (2.0 * plain_value),
         "roofline": roofline,
         "launch_shape": {"shape": autotune.shape_name(tune_w), "waves_per_chain": tune_w[0],
                          "moves_per_round": tune_w[1] if tune_w[2] != autotune.PATH_FIELDS else None,
@@ -355,12 +487,19 @@ def main():
                          "autotune_moves_per_s": {autotune.shape_name(k): v for k, v in tune_rates.items()}},
     }
     work = set(w.strip() for w in args.workloads.split(","))
+    if "dropin" in work:
+        out["dropin_updater"] = dropin_rate(settings, eci)
+        if not args.no_cpu:
+            ref1 = reference_sample("canonical", 3.0, 1)
+            if ref1 is not None:
+                out["dropin_updater"]["reference_moves_per_s"] = ref1["moves_per_s"]
+        log("drop-in updater: %.3e moves/s" % out["dropin_updater"]["moves_per_s"])
     if "ternary" in work:
-        out["ternary_uneven"] = ternary_uneven()
+        out["ternary_uneven"] = ternary_uneven(lds_peak=lds_meas, cpu=not args.no_cpu)
         log("ternary uneven: %.3e moves/s (kernel %.1f ms)" % (out["ternary_uneven"]["value"],
                                                                out["ternary_uneven"]["kernel_ms"]))
     if "tempering" in work:
-        out["parallel_tempering"] = tempering(cycles=args.pt_cycles)
+        out["parallel_tempering"] = tempering(cycles=args.pt_cycles, lds_peak=lds_meas, cpu=not args.no_cpu)
         log("tempering: %.3e moves/s (kernel %.1f ms)" % (out["parallel_tempering"]["value"],
                                                           out["parallel_tempering"]["kernel_ms"]))
     if "wang_landau" in work:
