@@ -269,3 +269,76 @@ class CE:
     def set_num_threads(self, num_threads):
         """ce_calculator.py:610-617; the updater's work runs on GPU lanes."""
         self.updater.set_num_threads(num_threads)
+
+
+def get_max_size_eci(eci):
+    """Largest cluster size named in the ECIs: the digit after "c" in each
+    name (ce_calculator.py:122-133)."""
+    return max((int(key[1]) for key in eci), default=0)
+
+
+def _max_tracked_size(bc_kwargs):
+    """Largest cluster size the settings arguments track, or None when they
+    name none (the reference reads ``max_cluster_size``; this package's
+    settings key their cutoffs by cluster size)."""
+    if "max_cluster_size" in bc_kwargs:
+        return int(bc_kwargs["max_cluster_size"])
+    for key in ("max_shell_by_size", "max_dia_by_size"):
+        if bc_kwargs.get(key):
+            return max(int(k) for k in bc_kwargs[key])
+    return None
+
+
+def template_symbols(bc):
+    """Symbols of the settings' template cell: every site holds the first
+    element of its sublattice (background sites their only element)."""
+    sub = getattr(bc, "sublattice_of_site", None)
+    if sub is None:
+        return [bc.basis_elements[0][0]] * bc.n_sites
+    return [bc.basis_elements[int(s)][0] for s in sub]
+
+
+def get_atoms_with_ce_calc(small_bc, bc_kwargs, eci=None, size=(1, 1, 1), db_name="temp_db.db"):
+    """Atoms of a large cell with a calculator whose initial correlation
+    functions come from a small cell (ce_calculator.py:48-102).
+
+    The template cell of ``small_bc`` (every site the first element of its
+    sublattice) is evaluated on the GPU; the settings are rebuilt from
+    ``bc_kwargs`` with ``size`` as the supercell, and the large template cell
+    gets a calculator started from the small cell's correlation functions,
+    which equal the large cell's because both are the same periodic crystal.
+    ``bc_kwargs`` is not modified and ``db_name`` is accepted for the
+    reference's signature (no database is written).
+
+    :raises ValueError: the ECIs name a cluster size the settings do not
+        track
+    """
+    from .lattice import SimpleAtoms, settings_from_kwargs
+    del db_name
+    eci = dict(eci) if eci is not None else {}
+    tracked = _max_tracked_size(bc_kwargs)
+    if tracked is not None and get_max_size_eci(eci) > tracked:
+        raise ValueError("ECI specifies a cluster size larger than ClusterExpansionSetting tracks!")
+    small_atoms = SimpleAtoms(template_symbols(small_bc))
+    small_calc = CE(small_atoms, small_bc, eci)
+    init_cf = small_calc.get_cf()
+    small_calc.updater.close()
+    kwargs = dict(bc_kwargs)
+    if "classtype" not in kwargs:
+        kwargs["classtype"] = "CEBulk" if "crystalstructure" in kwargs else type(small_bc).__name__
+    kwargs["size"] = [int(v) for v in size] if not np.isscalar(size) else int(size)
+    large_bc = settings_from_kwargs(kwargs)
+    atoms = SimpleAtoms(template_symbols(large_bc))
+    CE(atoms, large_bc, eci, initial_cf=init_cf)
+    return atoms
+
+
+def get_atoms_with_ce_calc_JSON(jsonfile, eci=None, size=(1, 1, 1), db_name="temp_db.db"):
+    """get_atoms_with_ce_calc with the settings read from a JSON file of
+    their arguments and ``classtype`` (ce_calculator.py:104-119)."""
+    import json
+    from .lattice import settings_from_kwargs
+    with open(jsonfile, "r") as infile:
+        init_arguments = json.load(infile)
+    settings = settings_from_kwargs(init_arguments)
+    return get_atoms_with_ce_calc(settings, init_arguments, eci=eci, size=size, db_name=db_name)
