@@ -638,25 +638,61 @@ int ceo_run_many(ceo_chain** chains, int n_chains, int kind, uint64_t seed,
 }
 
 /* ------------------------------------------------------- Wang-Landau */
-/* Bin of total energy e: histogram.cpp:66-69 converts
- * (e - Emin) * Nbins / (Emax - Emin) to int (truncation toward zero) and
- * :137-140 keeps 0 <= bin < Nbins.  Returns the bin, -1 below the range,
- * n_bins when the value truncates to n_bins exactly, n_bins + 1 above that
- * (wang_landau_sampler.cpp:257-258 tells the last two apart). */
-static int wl_bin(double e, double e_min, double e_max, int n_bins) {
-  const double x = (e - e_min) * (double)n_bins / (e_max - e_min);
-  const double t = trunc(x);
-  if (t < 0.0) return -1;
-  if (t > (double)n_bins) return n_bins + 1;
-  return (int)t;
+/* Lower edge of a bin, Histogram::get_energy (histogram.cpp:61-64). */
+static double wl_bin_energy(const ceo_wl* w, int bin) {
+  return w->e_min + ((w->e_max - w->e_min) * bin) / (double)(unsigned)w->n_bins;
+}
+
+/* Truncated (e - Emin) * Nbins / (Emax - Emin) as a double, limited to the
+ * int range: the int Histogram::get_bin returns before any range check. */
+static double wl_raw_bin(const ceo_wl* w, double e) {
+  double t = trunc((e - w->e_min) * (double)w->n_bins / (w->e_max - w->e_min));
+  if (t < -2147483648.0) t = -2147483648.0;
+  if (t > 2147483647.0) t = 2147483647.0;
+  return t;
+}
+
+/* The walker's state: occupation, correlation functions, species tracker. */
+static void wl_snapshot(const ceo_chain* c, ceo_wl* w, int slot) {
+  const ceo_model* m = c->m;
+  const int N = m->n_sites;
+  memcpy(w->snap_occ + (size_t)slot * N, c->occ, (size_t)N);
+  memcpy(w->snap_cf + (size_t)slot * m->n_cf, ring_slot(c, current_slot(c)), sizeof(double) * (size_t)m->n_cf);
+  memcpy(w->snap_trk + (size_t)slot * N, c->trk, sizeof(int) * (size_t)c->trk_off[m->n_species]);
+  w->snap_bin[slot] = w->cur_bin;
+}
+
+/* WangLandauSampler::set_updaters (wang_landau_sampler.cpp:585-600) for one
+ * walker: the recorded state and its bin replace the walker's; the sub-bin
+ * energy is left as it was, as in the reference. */
+static void wl_restore(ceo_chain* c, ceo_wl* w, int slot) {
+  const ceo_model* m = c->m;
+  const int N = m->n_sites;
+  memcpy(c->occ, w->snap_occ + (size_t)slot * N, (size_t)N);
+  memcpy(ring_slot(c, current_slot(c)), w->snap_cf + (size_t)slot * m->n_cf, sizeof(double) * (size_t)m->n_cf);
+  c->buffer_size = 1;
+  memcpy(c->trk, w->snap_trk + (size_t)slot * N, sizeof(int) * (size_t)c->trk_off[m->n_species]);
+  w->cur_bin = w->snap_bin[slot];
 }
 
 /* Histogram::update (histogram.cpp:71-90): the bin is known, visited once
- * more, and its ln g grows by f. */
-static void wl_visit(ceo_wl* w, int bin) {
+ * more, and its ln g grows by f.  Adaptive windows first record the
+ * walker's state at the bin's first visit (adaptive_windows.cpp:240-260);
+ * `sub` adds the sub-bin count of the step's closing update
+ * (Histogram::update_sub_bin, histogram.cpp:92-100). */
+static void wl_visit(const ceo_chain* c, ceo_wl* w, int bin, int sub) {
+  if (w->adaptive && !w->saved[bin]) {
+    wl_snapshot(c, w, bin);
+    w->saved[bin] = 1;
+  }
   w->known[bin] = 1;
   w->hist[bin] += 1;
   w->logdos[bin] += w->f;
+  if (sub && w->adaptive) {
+    const double lo = wl_bin_energy(w, bin), hi = wl_bin_energy(w, bin + 1);
+    const double t = trunc((w->cur_energy - lo) * 10.0 / (hi - lo));
+    if (t >= 0.0 && t < 10.0) w->sub_hist[10 * bin + (int)t] += 1;
+  }
 }
 
 /* Histogram::is_flat (histogram.cpp:102-128): over the known bins, the
@@ -674,71 +710,240 @@ static int wl_is_flat(const ceo_wl* w) {
   return (double)minimum > w->flatness * mean;
 }
 
+/* One canonical swap proposal of counter t: sites, species and the energy
+ * after it (the change stays applied; the caller keeps or undoes it). */
+typedef struct wl_prop {
+  int a, b, sa, sb, sites[2];
+  double u, e_new;
+} wl_prop;
+
+static int wl_propose(ceo_chain* c, const int* present, int n_present, const uint64_t key[2], uint64_t ctr0,
+                      wl_prop* p) {
+  const uint64_t ctr[4] = {ctr0, 0, 0, 0};
+  uint64_t r[4];
+  ceo_philox4x64(ctr, key, r);
+  uint32_t ia = bounded32((uint32_t)(r[0] >> 32), (uint32_t)n_present);
+  uint32_t ib = (ia + 1 + bounded32((uint32_t)r[0], (uint32_t)(n_present - 1))) % (uint32_t)n_present;
+  p->a = present[ia];
+  p->b = present[ib];
+  p->sa = (int)bounded64(r[1], (uint64_t)c->trk_count[p->a]);
+  p->sb = (int)bounded64(r[2], (uint64_t)c->trk_count[p->b]);
+  p->sites[0] = c->trk[c->trk_off[p->a] + p->sa];
+  p->sites[1] = c->trk[c->trk_off[p->b] + p->sb];
+  p->u = unit_double(r[3]);
+  const int olds[2] = {p->a, p->b}, news[2] = {p->b, p->a};
+  return calculate_impl(c, 2, p->sites, olds, news, &p->e_new);
+}
+
+static void wl_keep(ceo_chain* c, const wl_prop* p) {
+  c->trk[c->trk_off[p->a] + p->sa] = p->sites[1];
+  c->trk[c->trk_off[p->b] + p->sb] = p->sites[0];
+}
+
+/* AdaptiveWindowHistogram::distribute_random_walkers_evenly
+ * (adaptive_windows.cpp:262-318) with one walker: when more than one bin
+ * of the window has a recorded state, the walker moves to the recorded
+ * state nearest the window's middle (the upper side on a tie). */
+static void wl_distribute(ceo_chain* c, ceo_wl* w) {
+  const unsigned upper = (unsigned)w->upper;
+  unsigned n_known = 0;
+  for (unsigned i = 0; i < upper; ++i) n_known += w->saved[i] ? 1u : 0u;
+  if (n_known <= 1u) return;
+  const double delta = upper / 2.0;
+  const unsigned indx = (unsigned)delta;
+  unsigned shift_pos = 2 * upper, shift_neg = 2 * upper;
+  for (unsigned j = indx; j < upper; ++j)
+    if (w->saved[j]) {
+      shift_pos = j - indx;
+      break;
+    }
+  for (int j = (int)indx; j >= 0; --j)
+    if (w->saved[j]) {
+      shift_neg = indx - (unsigned)j;
+      break;
+    }
+  wl_restore(c, w, (int)(shift_pos <= shift_neg ? indx + shift_pos : indx - shift_neg));
+}
+
+/* WangLandauSampler::run_until_valid_energy (wang_landau_sampler.cpp:434-560)
+ * with one walker outside the window: swaps toward the window (below emax:
+ * any swap to a higher energy under emax; above: any swap to a lower energy
+ * over emin, measured from the current bin's lower edge) until a proposal
+ * lands in the window, at most 10^7 swaps. */
+static int wl_walk_into_window(ceo_chain* c, ceo_wl* w, const int* present, int n_present, const uint64_t key[2],
+                               uint64_t move_begin, int64_t* t) {
+  if (w->cur_bin >= 0 && w->cur_bin < w->upper) return CEO_OK;
+  const double emin = w->e_min, emax = wl_bin_energy(w, w->upper);
+  for (int64_t i = 0; i < 10000000; ++i) {
+    wl_prop p;
+    const int rc = wl_propose(c, present, n_present, key, move_begin + (uint64_t)*t, &p);
+    *t += 1;
+    w->n_walk += 1;
+    if (rc != CEO_OK) return rc;
+    const double raw = wl_raw_bin(w, p.e_new);
+    const double cur_e = wl_bin_energy(w, w->cur_bin);
+    const int ok = cur_e >= emax ? (p.e_new < cur_e && p.e_new > emin) : (p.e_new > cur_e && p.e_new < emax);
+    if (ok) {
+      wl_keep(c, &p);
+      w->cur_bin = (int)raw;
+    } else {
+      undo_impl(c);
+    }
+    ceo_clear_history(c);
+    if (raw >= 0.0 && raw < (double)w->upper) return CEO_OK;
+  }
+  return fail(CEO_ERUNTIME, "Could not find any energy state inside the histogram range!");
+}
+
+/* AdaptiveWindowHistogram::is_flat (adaptive_windows.cpp:38-140): scanning
+ * down from upper - n_overlap, the window of known bins above the scan
+ * point is flat while its minimum exceeds criteria times its mean; the scan
+ * stops at the first bin that breaks this.  A scan that reaches bin 0 flat
+ * means the window has converged.  Otherwise, if some window of at least
+ * min_width bins (or of every known bin) was flat, the upper part is cut
+ * off: ln g of the n_overlap bins above the cut is saved, the walker's
+ * state is recorded at the first cut, and the walker moves into the new
+ * window [0, cut + n_overlap).  With n_overlap >= upper the scan is empty
+ * (the reference's unsigned start would wrap). */
+static int wl_window_flat(ceo_chain* c, ceo_wl* w, const int* present, int n_present, const uint64_t key[2],
+                          uint64_t move_begin, int64_t* t, int* rc) {
+  const int top = w->upper > w->n_overlap ? w->upper - w->n_overlap : 0;
+  unsigned max_possible = 0;
+  for (int i = 0; i < top; ++i) max_possible += w->known[i] ? 1u : 0u;
+  uint32_t sum = 0, count = 0, minimum = 100000000u;
+  int part = 0, min_ok = 0;
+  int fnc = 0;
+  for (int i = top; i > 0; --i) {
+    if (!w->known[i - 1]) continue;
+    const uint32_t h = (uint32_t)w->hist[i - 1];
+    sum += h;
+    count += 1;
+    if (h < minimum) minimum = h;
+    const double mean = (double)sum / count;
+    min_ok = (double)minimum > w->flatness * mean;
+    if ((count > (uint32_t)w->min_width || count >= max_possible) && min_ok) part = 1;
+    if (!min_ok) {
+      fnc = i - 1;
+      break;
+    }
+  }
+  if (fnc == 0 && min_ok) return 1;
+  if (!part) return 0;
+  /* is_valid_window (adaptive_windows.cpp:222-236) */
+  if (fnc <= w->min_width) return 0;
+  int n_known = 0;
+  for (int i = 0; i < fnc; ++i) n_known += w->known[i] ? 1 : 0;
+  if (n_known <= w->min_width / 2) return 0;
+  const int e = w->n_edges++;
+  w->edges[e] = fnc + 1;
+  w->edge_n[e] = w->n_overlap;
+  for (int k = 0; k < w->n_overlap; ++k) w->edge_vals[(size_t)e * w->n_bins + k] = w->logdos[fnc + 1 + k];
+  if (w->upper == w->n_bins) {
+    wl_snapshot(c, w, w->n_bins);
+    w->has_first = 1;
+  }
+  w->upper = fnc + w->n_overlap + 1;
+  wl_distribute(c, w);
+  *rc = wl_walk_into_window(c, w, present, n_present, key, move_begin, t);
+  return 0;
+}
+
+/* AdaptiveWindowHistogram::reset and make_dos_continous
+ * (adaptive_windows.cpp:142-185): the full range is back in the window; at
+ * every cut, in the order they were made, the saved values return to the
+ * bins above it and everything at or below the cut shifts so that ln g is
+ * continuous there (later cuts' saved values shift with it); the walker
+ * returns to the state recorded at the first cut. */
+static void wl_window_reset(ceo_chain* c, ceo_wl* w) {
+  w->upper = w->n_bins;
+  for (int e = 0; e < w->n_edges; ++e) {
+    const int E = w->edges[e], n = w->edge_n[e];
+    double* v = w->edge_vals + (size_t)e * w->n_bins;
+    for (int j = 1; j < n; ++j) w->logdos[E + j] = v[j];
+    const double diff = w->logdos[E] - v[0];
+    for (int j = 0; j < E + 1; ++j) w->logdos[j] -= diff;
+    for (int e2 = e + 1; e2 < w->n_edges; ++e2)
+      for (int k = 0; k < w->edge_n[e2]; ++k) w->edge_vals[(size_t)e2 * w->n_bins + k] -= diff;
+  }
+  w->n_edges = 0;
+  if (w->has_first) wl_restore(c, w, w->n_bins);
+}
+
 int ceo_wl_run(ceo_chain* c, ceo_wl* w, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,
-               int64_t n_checks, int64_t check_every, uint8_t* accept_bits) {
+               int64_t n_checks, int64_t check_every, uint8_t* accept_bits, int64_t bits_cap, int64_t* n_used) {
   int present[64];
   const int n_present = canonical_setup(c, present);
+  if (n_used) *n_used = 0;
   if (n_present < 2) return fail(CEO_EINVAL, "canonical moves need at least two species present");
   if (w->n_bins < 1 || !(w->e_max > w->e_min)) return fail(CEO_EINVAL, "invalid Wang-Landau histogram");
   if (w->cur_bin < 0 || w->cur_bin >= w->n_bins) return fail(CEO_EINVAL, "current bin outside the histogram");
-  int* count = c->trk_count;
-  int* trk_off = c->trk_off;
-  int* tracker = c->trk;
+  if (w->adaptive && (w->upper < 1 || w->upper > w->n_bins || w->n_overlap < 0 || w->n_overlap > w->n_bins))
+    return fail(CEO_EINVAL, "invalid adaptive window");
+  if (!w->adaptive) w->upper = w->n_bins;
   const uint64_t key[2] = {seed, chain_uid};
   int64_t t = 0;
   int rc = CEO_OK;
   if (w->converged) return CEO_OK; /* a converged walker makes no more moves */
   for (int64_t chk = 0; chk < n_checks && rc == CEO_OK; ++chk) {
     /* WangLandauSampler::step, wang_landau_sampler.cpp:232-317 */
-    for (int64_t s = 0; s < check_every; ++s, ++t) {
-      const uint64_t ctr[4] = {move_begin + (uint64_t)t, 0, 0, 0};
-      uint64_t r[4];
-      ceo_philox4x64(ctr, key, r);
-      uint32_t ia = bounded32((uint32_t)(r[0] >> 32), (uint32_t)n_present);
-      uint32_t ib = (ia + 1 + bounded32((uint32_t)r[0], (uint32_t)(n_present - 1))) % (uint32_t)n_present;
-      const int a = present[ia], b = present[ib];
-      const int sa = (int)bounded64(r[1], (uint64_t)count[a]);
-      const int sb = (int)bounded64(r[2], (uint64_t)count[b]);
-      const int sites[2] = {tracker[trk_off[a] + sa], tracker[trk_off[b] + sb]};
-      const int olds[2] = {a, b}, news[2] = {b, a};
-      const double u = unit_double(r[3]);
-      double e_new = 0.0;
-      rc = calculate_impl(c, 2, sites, olds, news, &e_new);
+    for (int64_t s = 0; s < check_every; ++s) {
+      wl_prop p;
+      const int64_t tm = t++;
+      rc = wl_propose(c, present, n_present, key, move_begin + (uint64_t)tm, &p);
       if (rc != CEO_OK) break;
       w->n_moves += 1;
-      const int bin = wl_bin(e_new, w->e_min, w->e_max, w->n_bins);
-      if (bin < 0 || bin >= w->n_bins) {
+      w->steps += 1;
+      w->since_flat += 1;
+      const double raw = wl_raw_bin(w, p.e_new);
+      w->bin_change += fabs(raw - (double)w->cur_bin);
+      const int bin = raw < 0.0 ? -1 : (raw > (double)w->n_bins ? w->n_bins + 1 : (int)raw);
+      if (bin < 0 || bin >= w->upper) {
         w->n_outside += 1;
         undo_impl(c);
         if (w->is_first) continue;
-        if (bin > w->n_bins) wl_visit(w, w->cur_bin);
+        if (bin > w->n_bins) wl_visit(c, w, w->cur_bin, 0);
         continue;
       }
       if (bin == w->cur_bin) w->n_self += 1;
       /* Histogram::get_dos_ratio_old_divided_by_new (histogram.cpp:130-135) */
       const double diff = w->logdos[w->cur_bin] - w->logdos[bin];
       const double ratio = diff > 0.0 ? 1.0 : exp(diff);
-      const int accept = (u < ratio) || w->is_first;
+      const int accept = (p.u < ratio) || w->is_first;
       if (accept) {
         w->cur_bin = bin;
         w->is_first = 0;
+        w->cur_energy = p.e_new;
         w->n_accept += 1;
-        tracker[trk_off[a] + sa] = sites[1];
-        tracker[trk_off[b] + sb] = sites[0];
-        if (accept_bits) accept_bits[t >> 3] |= (uint8_t)(1u << (t & 7));
+        wl_keep(c, &p);
+        if (accept_bits && tm < bits_cap) accept_bits[tm >> 3] |= (uint8_t)(1u << (tm & 7));
       } else {
         undo_impl(c);
       }
       ceo_clear_history(c);
-      wl_visit(w, w->cur_bin);
+      wl_visit(c, w, w->cur_bin, 1);
     }
     if (rc != CEO_OK) break;
-    /* WangLandauSampler::run, wang_landau_sampler.cpp:337-394: on a flat
-     * histogram f halves (fixed-f scheme) and the counts restart; below
-     * f_min the walker has converged */
-    if (wl_is_flat(w)) {
-      w->f /= 2.0;
+    /* WangLandauSampler::run, wang_landau_sampler.cpp:337-394 */
+    const int flat = w->adaptive ? wl_window_flat(c, w, present, n_present, key, move_begin, &t, &rc) : wl_is_flat(w);
+    if (rc != CEO_OK) break;
+    if (flat) {
+      const int64_t took = w->since_flat;
+      const double mc_time = (double)w->steps / (double)(unsigned)w->n_bins;
+      if (w->inverse_time && w->prev_conv >= 0 && took > w->prev_conv) {
+        if (!w->inv_active) w->inv_factor = mc_time * w->f;
+        w->inv_active = 1;
+      }
+      w->prev_conv = took;
+      w->f = w->inv_active ? w->inv_factor / mc_time : w->f / 2.0;
       for (int i = 0; i < w->n_bins; ++i) w->hist[i] = 0;
+      if (w->adaptive) {
+        wl_window_reset(c, w);
+        /* set_overlap(2 * average bin change per step) */
+        const double ov = 2.0 * (w->bin_change / (double)took);
+        w->n_overlap = ov >= (double)w->n_bins ? w->n_bins : (int)(unsigned)ov;
+      }
+      w->bin_change = 0.0;
+      w->since_flat = 0;
       w->n_flat += 1;
     }
     if (w->f < w->f_min) {
@@ -746,5 +951,6 @@ int ceo_wl_run(ceo_chain* c, ceo_wl* w, uint64_t seed, uint64_t chain_uid, uint6
       break;
     }
   }
+  if (n_used) *n_used = t;
   return rc;
 }

@@ -111,9 +111,26 @@ int ceo_run_many(ceo_chain** chains, int n_chains, int kind, uint64_t seed,
                  const uint64_t* chain_uids, uint64_t move_begin, int64_t n_moves,
                  const double* temperatures, int n_threads, int64_t* n_accept);
 
-/* Wang-Landau walker (wang_landau_sampler.cpp:232-404, histogram.cpp:66-172)
- * on a canonical chain: one histogram per walker, the fixed-f scheme.
- * Counters accumulate over calls. */
+/* Wang-Landau walker (wang_landau_sampler.cpp:232-404, histogram.cpp:66-172,
+ * adaptive_windows.cpp) on a canonical chain: one histogram per walker.
+ * Counters accumulate over calls.
+ *
+ * Schemes of WangLandauSampler::run:
+ *  inverse_time  once the steps a flat histogram took exceed those of the
+ *                previous one, f = f_act * t_act / t at every later flat
+ *                histogram, t = steps / n_bins (:344-367, get_mc_time :605);
+ *                the reference times convergence in CPU seconds, here in
+ *                steps so that runs repeat
+ *  adaptive      AdaptiveWindowHistogram: the window [0, upper) shrinks
+ *                from the top as its upper part converges, the walker's
+ *                state is recorded at the first visit of every bin, a cut
+ *                moves the walker to the recorded state nearest the window
+ *                middle, and a flat window restores the full range with
+ *                ln g made continuous at the cuts; sub-bin visit counts
+ *                (10 per bin) are kept in this mode only, as in the
+ *                reference (Histogram::init_sub_bins is called only there).
+ * The scheme fields are reset by the caller when a run starts (the
+ * reference builds a new sampler per run_fast_sampler call). */
 typedef struct ceo_wl {
   int32_t n_bins;
   int32_t cur_bin;   /* bin of the walker's state (ignored while is_first) */
@@ -125,13 +142,37 @@ typedef struct ceo_wl {
   int32_t* hist;
   uint8_t* known;
   int64_t n_moves, n_accept, n_outside, n_self, n_flat;
+  /* schemes */
+  int32_t inverse_time, adaptive, min_width, inv_active;
+  int32_t upper;      /* adaptive: bins [0, upper) are in range (n_bins otherwise) */
+  int32_t n_overlap;  /* adaptive: bins kept above a cut (starts at 7) */
+  int32_t n_edges, has_first;
+  double inv_factor;  /* f_act * t_act of the inverse-time scheme */
+  double bin_change;  /* sum of |proposed bin - current bin| since the last flat histogram */
+  double cur_energy;  /* energy of the last accepted state (the sub-bin energy) */
+  int64_t steps;      /* steps since the scheme was set (the time t counts these) */
+  int64_t since_flat; /* steps since the last flat histogram */
+  int64_t prev_conv;  /* steps the previous flat histogram took, -1 before the first */
+  int64_t n_walk;     /* moves made to bring the walker back into the window */
+  int32_t* edges;     /* adaptive: n_bins cut positions, in order */
+  int32_t* edge_n;    /* values saved at each cut */
+  double* edge_vals;  /* n_bins x n_bins: ln g above each cut when it was made */
+  uint8_t* saved;     /* n_bins: a state was recorded at the bin's first visit */
+  int8_t* snap_occ;   /* (n_bins + 1) x n_sites: recorded occupations (row n_bins: first cut) */
+  double* snap_cf;    /* (n_bins + 1) x n_cf */
+  int32_t* snap_trk;  /* (n_bins + 1) x n_sites: species index trackers */
+  int32_t* snap_bin;  /* n_bins + 1 */
+  uint32_t* sub_hist; /* n_bins x 10 */
 } ceo_wl;
 
-/* n_checks rounds of check_every moves, each round followed by the
- * flatness test; stops early once converged.  Move t uses the Philox block
- * of counter move_begin + t as ceo_run's canonical moves do. */
+/* n_checks rounds of check_every steps, each round followed by the
+ * flatness test; stops early once converged.  Proposal t (steps and the
+ * moves that bring the walker back into an adaptive window alike) uses the
+ * Philox block of counter move_begin + t as ceo_run's canonical moves do;
+ * accept_bits (bits_cap bits, may be NULL) gets bit t for accepted steps.
+ * *n_used (may be NULL) receives the number of counters used. */
 int ceo_wl_run(ceo_chain* c, ceo_wl* w, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,
-               int64_t n_checks, int64_t check_every, uint8_t* accept_bits);
+               int64_t n_checks, int64_t check_every, uint8_t* accept_bits, int64_t bits_cap, int64_t* n_used);
 
 #ifdef __cplusplus
 }

@@ -68,7 +68,8 @@ def load_library():
                                    ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_uint8), dp,
                                    ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(Hooks)]
     lib.ceo_wl_run.argtypes = [P, ctypes.POINTER(WLState), ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
-                               ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint8)]
+                               ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int64,
+                               ctypes.POINTER(ctypes.c_int64)]
     _lib = lib
     return lib
 
@@ -91,21 +92,60 @@ class WLState(ctypes.Structure):
                 ("logdos", ctypes.POINTER(ctypes.c_double)), ("hist", ctypes.POINTER(ctypes.c_int32)),
                 ("known", ctypes.POINTER(ctypes.c_uint8)), ("n_moves", ctypes.c_int64),
                 ("n_accept", ctypes.c_int64), ("n_outside", ctypes.c_int64), ("n_self", ctypes.c_int64),
-                ("n_flat", ctypes.c_int64)]
+                ("n_flat", ctypes.c_int64),
+                ("inverse_time", ctypes.c_int32), ("adaptive", ctypes.c_int32), ("min_width", ctypes.c_int32),
+                ("inv_active", ctypes.c_int32), ("upper", ctypes.c_int32), ("n_overlap", ctypes.c_int32),
+                ("n_edges", ctypes.c_int32), ("has_first", ctypes.c_int32), ("inv_factor", ctypes.c_double),
+                ("bin_change", ctypes.c_double), ("cur_energy", ctypes.c_double), ("steps", ctypes.c_int64),
+                ("since_flat", ctypes.c_int64), ("prev_conv", ctypes.c_int64), ("n_walk", ctypes.c_int64),
+                ("edges", ctypes.POINTER(ctypes.c_int32)), ("edge_n", ctypes.POINTER(ctypes.c_int32)),
+                ("edge_vals", ctypes.POINTER(ctypes.c_double)), ("saved", ctypes.POINTER(ctypes.c_uint8)),
+                ("snap_occ", ctypes.POINTER(ctypes.c_int8)), ("snap_cf", ctypes.POINTER(ctypes.c_double)),
+                ("snap_trk", ctypes.POINTER(ctypes.c_int32)), ("snap_bin", ctypes.POINTER(ctypes.c_int32)),
+                ("sub_hist", ctypes.POINTER(ctypes.c_uint32))]
 
 
 class OracleWL:
-    """A Wang-Landau walker of the oracle: histogram arrays owned here."""
+    """A Wang-Landau walker of the oracle: histogram arrays owned here.
+
+    inverse_time, adaptive and min_width choose the schemes of ce_oracle.h;
+    n_sites and n_cf size the state records of adaptive windows."""
 
     def __init__(self, n_bins, e_min, e_max, f, f_min=1e-6, flatness=0.8, cur_bin=0, is_first=True,
-                 logdos=None, hist=None, known=None):
-        self.logdos = np.zeros(n_bins) if logdos is None else np.array(logdos, dtype=np.float64)
-        self.hist = np.zeros(n_bins, dtype=np.int32) if hist is None else np.array(hist, dtype=np.int32)
-        self.known = np.zeros(n_bins, dtype=np.uint8) if known is None else np.array(known, dtype=np.uint8)
-        self.state = WLState(int(n_bins), int(cur_bin), int(bool(is_first)), 0, float(e_min), float(e_max),
+                 logdos=None, hist=None, known=None, inverse_time=False, adaptive=False, min_width=10,
+                 n_sites=0, n_cf=0):
+        nb = int(n_bins)
+        self.logdos = np.zeros(nb) if logdos is None else np.array(logdos, dtype=np.float64)
+        self.hist = np.zeros(nb, dtype=np.int32) if hist is None else np.array(hist, dtype=np.int32)
+        self.known = np.zeros(nb, dtype=np.uint8) if known is None else np.array(known, dtype=np.uint8)
+        rows = nb + 1 if adaptive else 1
+        self.edges = np.zeros(nb, dtype=np.int32)
+        self.edge_n = np.zeros(nb, dtype=np.int32)
+        self.edge_vals = np.zeros(nb * nb if adaptive else 1)
+        self.saved = np.zeros(nb, dtype=np.uint8)
+        self.snap_occ = np.zeros(rows * max(int(n_sites), 1), dtype=np.int8)
+        self.snap_cf = np.zeros(rows * max(int(n_cf), 1))
+        self.snap_trk = np.zeros(rows * max(int(n_sites), 1), dtype=np.int32)
+        self.snap_bin = np.zeros(nb + 1, dtype=np.int32)
+        self.sub_hist = np.zeros(nb * 10, dtype=np.uint32)
+        if adaptive and (n_sites <= 0 or n_cf <= 0):
+            raise ValueError("adaptive windows need n_sites and n_cf for the state records")
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        self.state = WLState(nb, int(cur_bin), int(bool(is_first)), 0, float(e_min), float(e_max),
                              float(f), float(f_min), float(flatness), _dp(self.logdos),
-                             self.hist.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                             self.known.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), 0, 0, 0, 0, 0)
+                             self.hist.ctypes.data_as(i32),
+                             self.known.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), 0, 0, 0, 0, 0,
+                             int(bool(inverse_time)), int(bool(adaptive)), int(min_width), 0, nb, 7, 0, 0,
+                             0.0, 0.0, 0.0, 0, 0, -1, 0,
+                             self.edges.ctypes.data_as(i32), self.edge_n.ctypes.data_as(i32),
+                             _dp(self.edge_vals), self.saved.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                             self.snap_occ.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)), _dp(self.snap_cf),
+                             self.snap_trk.ctypes.data_as(i32), self.snap_bin.ctypes.data_as(i32),
+                             self.sub_hist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+
+    def sub_bins(self):
+        """Sub-bin visit counts, (n_bins, 10)."""
+        return self.sub_hist.reshape(-1, 10)
 
 
 def _ip(a):
@@ -349,14 +389,20 @@ class OracleUpdater:
         return bits, trace[:n_tr], acc.value
 
 
-def wl_run(updater, wl, seed, chain_uid, n_checks, check_every, move_begin=0):
+def wl_run(updater, wl, seed, chain_uid, n_checks, check_every, move_begin=0, counters=None):
     """Wang-Landau walker `wl` (OracleWL) on the canonical chain of `updater`;
-    returns the accept bits of the moves made (packed LSB-first)."""
+    returns the accept bits of the steps made (packed LSB-first, bit t for
+    Philox counter move_begin + t).  counters: a list that receives the
+    number of Philox counters used (steps plus moves back into a window)."""
     lib = load_library()
     n = int(n_checks) * int(check_every)
     bits = np.zeros(max((n + 7) // 8, 1), dtype=np.uint8)
+    used = ctypes.c_int64(0)
     rc = lib.ceo_wl_run(updater.handle, ctypes.byref(wl.state), int(seed), int(chain_uid), int(move_begin),
-                        int(n_checks), int(check_every), bits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+                        int(n_checks), int(check_every), bits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                        int(8 * bits.size), ctypes.byref(used))
+    if counters is not None:
+        counters.append(int(used.value))
     if rc:
         _raise(lib, rc)
     return bits
