@@ -75,6 +75,16 @@ class WLDesc(ctypes.Structure):
     ]
 
 
+class WLScheme(ctypes.Structure):
+    _fields_ = [
+        ("inverse_time", ctypes.c_int32), ("adaptive", ctypes.c_int32), ("min_width", ctypes.c_int32),
+        ("inv_active", ctypes.c_int32), ("upper", ctypes.c_int32), ("n_overlap", ctypes.c_int32),
+        ("n_edges", ctypes.c_int32), ("has_first", ctypes.c_int32), ("inv_factor", ctypes.c_double),
+        ("bin_change", ctypes.c_double), ("steps", ctypes.c_int64), ("since_flat", ctypes.c_int64),
+        ("prev_conv", ctypes.c_int64), ("n_walk", ctypes.c_int64),
+    ]
+
+
 # (name, restype, argtypes) of every entry point declared in include/cemc_gpu.h
 SIGNATURES = [
     ("cemc_last_error", ctypes.c_char_p, []),
@@ -126,6 +136,11 @@ SIGNATURES = [
                                          ctypes.POINTER(ctypes.c_uint8)]),
     ("cemc_batch_wl_get", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(WLDesc), _dp, _i32p,
                                          ctypes.POINTER(ctypes.c_uint8), _i64p]),
+    ("cemc_batch_wl_set_scheme", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(WLScheme)]),
+    ("cemc_batch_wl_get_scheme", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(WLScheme), _i32p, _i32p, _dp,
+                                                ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint8)]),
+    ("cemc_batch_wl_run_shared", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64,
+                                                ctypes.POINTER(ctypes.c_uint32), ctypes.c_int64, _dp]),
     ("cemc_batch_wl_run", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64,
                                          ctypes.POINTER(ctypes.c_uint32), ctypes.c_int64, _dp]),
     ("cemc_pool_create", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,

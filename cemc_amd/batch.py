@@ -306,17 +306,51 @@ class ChainBatch:
                    n_outside=int(cnt[2]), n_self=int(cnt[3]), n_flat=int(cnt[4]))
         return out
 
-    def wl_run(self, n_checks, check_every, log=False, timed=False):
+    def wl_set_scheme(self, chain, inverse_time=False, adaptive=False, min_width=10):
+        """Schemes of walker `chain` (cemc_batch_wl_set_scheme): the
+        inverse-time law and adaptive energy windows; starts them afresh."""
+        sc = _capi.WLScheme()
+        sc.inverse_time = int(bool(inverse_time))
+        sc.adaptive = int(bool(adaptive))
+        sc.min_width = int(min_width)
+        _capi.check(self._lib.cemc_batch_wl_set_scheme(self.handle, int(chain), ctypes.byref(sc)))
+
+    def wl_scheme(self, chain):
+        """Scheme state of walker `chain`: dict of the cemc_wl_scheme fields
+        plus arrays edges, edge_n, edge_vals (n_bins x n_bins), sub_bins
+        (n_bins x 10) and saved."""
+        sc = _capi.WLScheme()
+        _capi.check(self._lib.cemc_batch_wl_get_scheme(self.handle, int(chain), ctypes.byref(sc), None, None, None,
+                                                       None, None))
+        n = int(self.wl_get(chain)["n_bins"])
+        edges = np.zeros(n, dtype=np.int32)
+        edge_n = np.zeros(n, dtype=np.int32)
+        vals = np.zeros((n, n))
+        sub = np.zeros((n, 10), dtype=np.uint32)
+        saved = np.zeros(n, dtype=np.uint8)
+        _capi.check(self._lib.cemc_batch_wl_get_scheme(
+            self.handle, int(chain), None, edges.ctypes.data_as(_capi._i32p), edge_n.ctypes.data_as(_capi._i32p),
+            vals.ctypes.data_as(_capi._dp), sub.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+            saved.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        out = {k: getattr(sc, k) for k, _ in _capi.WLScheme._fields_}
+        out.update(edges=edges, edge_n=edge_n, edge_vals=vals, sub_bins=sub, saved=saved)
+        return out
+
+    def wl_run(self, n_checks, check_every, log=False, timed=False, shared=False):
         """n_checks rounds of check_every moves on every walker, each followed
-        by the flatness test.  With log: per-chain accept bits (uint8, LSB
-        first); with timed: the device time in ms.  Returns (log, ms)."""
+        by the flatness test.  shared: every walker visits walker 0's
+        histogram and a round's check_every steps are split over the walkers
+        (cemc_batch_wl_run_shared).  With log: per-chain accept bits (uint8,
+        LSB first); with timed: the device time in ms.  Returns (log, ms)."""
         bits = None
         words = 0
+        per = -(-int(check_every) // self.n_chains) if shared else int(check_every)
         if log:
-            words = max((int(n_checks) * int(check_every) + 31) // 32, 1)
+            words = max((int(n_checks) * per + 31) // 32, 1)
             bits = np.zeros((self.n_chains, words), dtype=np.uint32)
         ms = ctypes.c_double(0.0)
-        _capi.check(self._lib.cemc_batch_wl_run(
+        fn = self._lib.cemc_batch_wl_run_shared if shared else self._lib.cemc_batch_wl_run
+        _capi.check(fn(
             self.handle, int(n_checks), int(check_every),
             bits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if bits is not None else None, words,
             ctypes.byref(ms) if timed else None))

@@ -181,8 +181,36 @@ struct WLDev {
   int32_t converged;
   int64_t bin_off;
   uint64_t n_moves, n_accept, n_outside, n_self, n_flat;
+  // schemes of WangLandauSampler::run (wang_landau_sampler.cpp:319-404)
+  int32_t inverse_time;  // use_inverse_time_algorithm
+  int32_t adaptive;      // AdaptiveWindowHistogram (adaptive_windows.cpp)
+  int32_t min_width;     // minimum_window_width
+  int32_t inv_active;    // the inverse-time law is in force
+  int32_t upper;         // bins [0, upper) are in range
+  int32_t n_overlap;     // bins kept above a cut
+  int32_t n_edges;       // cuts made since the last flat window
+  int32_t has_first;     // the state at the first cut is recorded
+  double inv_factor;     // f t at the activation of the inverse-time law
+  double bin_change;     // adaptive: sum of |proposed bin - current bin| since the last flat histogram
+  double cur_energy;     // energy of the last accepted state (the sub-bin energy)
+  int64_t steps;         // steps since the scheme was set (the time t = steps / n_bins)
+  int64_t since_flat;    // steps since the last flat histogram
+  int64_t prev_conv;     // steps the previous flat histogram took (-1: none yet)
+  uint64_t n_walk;       // moves made to bring the walker back into its window
+  int32_t status;        // 1: no state inside the window was found
+  int32_t pad;
+  // adaptive windows: the walker's own device arrays (null otherwise)
+  uint8_t* snap_occ;     // (n_bins + 1) x n_sites recorded lattices (row n_bins: the first cut)
+  uint16_t* snap_trk;    // (n_bins + 1) x n_sites species trackers
+  double* snap_e;        // n_bins + 1 energies
+  int32_t* snap_bin;     // n_bins + 1 bins
+  int32_t* edges;        // n_bins cut positions
+  int32_t* edge_n;       // n_bins saved value counts
+  double* edge_vals;     // n_bins x n_bins saved ln g above each cut
+  uint8_t* saved;        // n_bins: a state was recorded at the bin's first visit
+  uint32_t* sub;         // n_bins x 10 sub-bin visit counts
 };
-static_assert(sizeof(WLDev) == 104, "WLDev layout");
+static_assert(sizeof(WLDev) == 272, "WLDev layout");
 
 // Per-move predicates and bias of one chain (cemc/mcmc/mc_constraints.py,
 // cpp/src/pair_constraint.cpp, cemc/mcmc/bias_potential.py): a trial move

@@ -982,6 +982,7 @@ struct cemc_batch {
   uint32_t* d_wl_hist = nullptr;
   uint8_t* d_wl_known = nullptr;
   int32_t* d_wl_order = nullptr;
+  std::vector<void*> wl_arrays;  // per chain: the walker's adaptive-window arrays (one allocation)
   // pools: device scratch of the initial-energy launch at admission, kept
   // across admissions (a free per admission would wait for the whole device)
   int32_t* d_adm_id = nullptr;
@@ -1005,6 +1006,7 @@ struct cemc_batch {
     dev_free(d_wl_hist);
     dev_free(d_wl_known);
     dev_free(d_wl_order);
+    for (void* p : wl_arrays) dev_free(p);
     dev_free(d_models);
     dev_free(d_params);
     dev_free(d_states);
@@ -1547,12 +1549,29 @@ void wl_store(cemc_batch* b, int chain, const WLDev& desc, const double* logdos,
 }
 
 // One launch per kernel variant over the chains that carry a walker.
+// shared: every walker of the batch visits walker 0's histogram; each check
+// of check_every steps (rounded up to a multiple of the walker count, as
+// wang_landau_sampler.cpp:321-324 does) is one launch in which every walker
+// makes its share, followed by the group's flatness test.
 void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* host_log, int64_t log_words,
-               double* ms) {
+               double* ms, bool shared = false) {
   require(!b->dynamic, "Wang-Landau walkers run on fixed batches, not pools");
   require(n_checks >= 0, "number of convergence checks must be non-negative");
   require(check_every > 0, "check_every must be positive");
-  if (host_log) require(n_checks * check_every <= 32 * log_words, "accept log too small for the number of moves");
+  int64_t per_walker = check_every;
+  if (shared) {
+    require(!b->wl.empty(), "no chain of the batch carries a Wang-Landau walker");
+    const WLDev& w0 = b->wl[0];
+    for (const WLDev& w : b->wl) {
+      require(w.n_bins > 0, "a shared histogram needs a walker on every chain");
+      require(w.n_bins == w0.n_bins && w.e_min == w0.e_min && w.e_max == w0.e_max,
+              "walkers sharing a histogram need the same bins and range");
+      require(!w.adaptive, "adaptive windows run with one walker per histogram");
+    }
+    const int64_t W = b->n_chains;
+    per_walker = (check_every + W - 1) / W;
+  }
+  if (host_log) require(n_checks * per_walker <= 32 * log_words, "accept log too small for the number of moves");
   const uint32_t limit = lds_limit();
   struct WKey {
     bool periodic;
@@ -1583,9 +1602,11 @@ void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* h
     k.nm = (k.kr > 0 && m->max_members <= 2) ? 2 : 3;
     const int occ_bytes = m->periodic ? m->pdims[0] * m->pdims[1] * m->pdims[2] : m->n_sites;
     const int n_rec = k.kr == 0 ? int(m->recs.size()) : 0;
-    const int nbins = b->wl[c].n_bins;
-    const uint32_t with_g = wl_lds_bytes(sweep_lds_bytes(occ_bytes, m->n_sites, n_rec, int(m->gtab.size())), nbins);
-    const uint32_t no_g = wl_lds_bytes(sweep_lds_bytes(occ_bytes, m->n_sites, n_rec, 0), nbins);
+    const int nbins = shared ? 0 : b->wl[c].n_bins;
+    const bool adaptive = !shared && b->wl[c].adaptive != 0;
+    const uint32_t with_g =
+        wl_lds_bytes(sweep_lds_bytes(occ_bytes, m->n_sites, n_rec, int(m->gtab.size())), nbins, adaptive);
+    const uint32_t no_g = wl_lds_bytes(sweep_lds_bytes(occ_bytes, m->n_sites, n_rec, 0), nbins, adaptive);
     k.glds = with_g <= std::min(kGtabLdsBudget, limit);
     const uint32_t lds = k.glds ? with_g : no_g;
     require(lds <= limit, "the walker's chain and histogram do not fit in the workgroup's LDS");
@@ -1609,6 +1630,7 @@ void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* h
     HIP_CHECK(hipMemsetAsync(d_log, 0, sizeof(uint32_t) * size_t(b->n_chains) * log_words, b->stream));
   }
   if (ms) HIP_CHECK(hipEventRecord(b->ev0, b->stream));
+  for (int64_t chk = 0; chk < (shared ? n_checks : 1); ++chk) {
   int begin = 0;
   for (auto& kv : groups) {
     LaunchWL a;
@@ -1629,12 +1651,17 @@ void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* h
     a.gtab_lds = kv.first.glds;
     a.kbm = kv.first.kbm;
     a.nm = kv.first.nm;
-    a.n_checks = n_checks;
-    a.check_every = check_every;
+    a.n_checks = shared ? 1 : n_checks;
+    a.check_every = per_walker;
     a.accept_log = d_log;
     a.log_words = log_words;
+    a.shared = shared ? b->d_wl : nullptr;
+    a.log_base = shared ? chk * per_walker : 0;
     HIP_CHECK(launch_wl(a, b->stream));
     begin += a.n_chains;
+  }
+  if (shared)
+    HIP_CHECK(launch_wl_shared_check(b->d_wl, b->d_wl_hist, b->d_wl_known, per_walker * b->n_chains, b->stream));
   }
   if (ms) HIP_CHECK(hipEventRecord(b->ev1, b->stream));
   HIP_CHECK(hipStreamSynchronize(b->stream));
@@ -1649,6 +1676,36 @@ void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* h
     HIP_CHECK(hipEventElapsedTime(&t, b->ev0, b->ev1));
     *ms = double(t);
   }
+  HIP_CHECK(hipMemcpy(b->wl.data(), b->d_wl, sizeof(WLDev) * b->wl.size(), hipMemcpyDeviceToHost));
+  for (const WLDev& w : b->wl)
+    if (w.n_bins > 0 && w.status != 0)
+      throw std::runtime_error("Could not find any energy state inside the histogram range!");
+}
+
+// A walker's adaptive-window arrays in one device allocation (carved in
+// 256-byte aligned pieces); saved flags and sub-bin counts start at zero.
+void wl_alloc_arrays(cemc_batch* b, int chain, WLDev& w) {
+  if (b->wl_arrays.empty()) b->wl_arrays.assign(size_t(b->n_chains), nullptr);
+  dev_free(b->wl_arrays[chain]);
+  b->wl_arrays[chain] = nullptr;
+  const int64_t nb = w.n_bins, N = b->params[chain].n_sites, rows = nb + 1;
+  const auto up = [](int64_t v) { return (v + 255) & ~int64_t(255); };
+  const int64_t o_occ = 0, o_trk = o_occ + up(rows * N), o_e = o_trk + up(2 * rows * N), o_bin = o_e + up(8 * rows),
+                o_edges = o_bin + up(4 * rows), o_en = o_edges + up(4 * nb), o_vals = o_en + up(4 * nb),
+                o_saved = o_vals + up(8 * nb * nb), o_sub = o_saved + up(nb), total = o_sub + up(40 * nb);
+  uint8_t* p = nullptr;
+  HIP_CHECK(hipMalloc(&p, size_t(total)));
+  b->wl_arrays[chain] = p;
+  HIP_CHECK(hipMemset(p + o_saved, 0, size_t(up(nb) + up(40 * nb))));
+  w.snap_occ = p + o_occ;
+  w.snap_trk = reinterpret_cast<uint16_t*>(p + o_trk);
+  w.snap_e = reinterpret_cast<double*>(p + o_e);
+  w.snap_bin = reinterpret_cast<int32_t*>(p + o_bin);
+  w.edges = reinterpret_cast<int32_t*>(p + o_edges);
+  w.edge_n = reinterpret_cast<int32_t*>(p + o_en);
+  w.edge_vals = reinterpret_cast<double*>(p + o_vals);
+  w.saved = p + o_saved;
+  w.sub = reinterpret_cast<uint32_t*>(p + o_sub);
 }
 
 }  // namespace
@@ -2432,6 +2489,14 @@ int cemc_batch_wl_set(cemc_batch* b, int32_t chain, const cemc_wl_desc* desc, co
     w.cur_bin = desc->current_bin;
     w.is_first = desc->is_first ? 1 : 0;
     w.converged = desc->converged ? 1 : 0;
+    w.upper = w.n_bins;
+    w.n_overlap = 7;
+    w.prev_conv = -1;
+    w.cur_energy = 0.0;  // WangLandauSampler's current_energy starts at 0 (wang_landau_sampler.cpp:37)
+    if (!b->wl_arrays.empty()) {
+      dev_free(b->wl_arrays[chain]);
+      b->wl_arrays[chain] = nullptr;
+    }
     wl_store(b, chain, w, logdos, reinterpret_cast<const uint32_t*>(hist), known);
   });
 }
@@ -2471,12 +2536,104 @@ int cemc_batch_wl_get(cemc_batch* b, int32_t chain, cemc_wl_desc* desc, double* 
   });
 }
 
+int cemc_batch_wl_set_scheme(cemc_batch* b, int32_t chain, const cemc_wl_scheme* scheme) {
+  return guarded([&] {
+    require(b && scheme, "NULL argument");
+    require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    require(!b->wl.empty() && b->wl[chain].n_bins > 0, "the chain carries no Wang-Landau walker");
+    require(scheme->min_width >= 0, "minimum window width must be non-negative");
+    (void)wl_download(b);
+    WLDev& w = b->wl[chain];
+    w.inverse_time = scheme->inverse_time ? 1 : 0;
+    w.adaptive = scheme->adaptive ? 1 : 0;
+    w.min_width = scheme->min_width;
+    w.inv_active = 0;
+    w.inv_factor = 0.0;
+    w.upper = w.n_bins;
+    w.n_overlap = 7;
+    w.n_edges = 0;
+    w.has_first = 0;
+    w.bin_change = 0.0;
+    w.steps = 0;
+    w.since_flat = 0;
+    w.prev_conv = -1;
+    w.status = 0;
+    w.cur_energy = 0.0;
+    if (w.adaptive) {
+      wl_alloc_arrays(b, chain, w);
+    } else if (!b->wl_arrays.empty() && b->wl_arrays[chain]) {
+      dev_free(b->wl_arrays[chain]);
+      b->wl_arrays[chain] = nullptr;
+      w.snap_occ = nullptr;
+      w.snap_trk = nullptr;
+      w.snap_e = nullptr;
+      w.snap_bin = nullptr;
+      w.edges = nullptr;
+      w.edge_n = nullptr;
+      w.edge_vals = nullptr;
+      w.saved = nullptr;
+      w.sub = nullptr;
+    }
+    HIP_CHECK(hipMemcpy(b->d_wl, b->wl.data(), sizeof(WLDev) * b->wl.size(), hipMemcpyHostToDevice));
+  });
+}
+
+int cemc_batch_wl_get_scheme(cemc_batch* b, int32_t chain, cemc_wl_scheme* scheme, int32_t* edges,
+                             int32_t* edge_n, double* edge_vals, uint32_t* sub_bins, uint8_t* saved) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    require(chain >= 0 && chain < b->n_chains, "chain index out of range");
+    require(!b->wl.empty() && b->wl[chain].n_bins > 0, "the chain carries no Wang-Landau walker");
+    (void)wl_download(b);
+    const WLDev& w = b->wl[chain];
+    if (scheme) {
+      scheme->inverse_time = w.inverse_time;
+      scheme->adaptive = w.adaptive;
+      scheme->min_width = w.min_width;
+      scheme->inv_active = w.inv_active;
+      scheme->upper = w.adaptive ? w.upper : w.n_bins;
+      scheme->n_overlap = w.n_overlap;
+      scheme->n_edges = w.n_edges;
+      scheme->has_first = w.has_first;
+      scheme->inv_factor = w.inv_factor;
+      scheme->bin_change = w.bin_change;
+      scheme->steps = w.steps;
+      scheme->since_flat = w.since_flat;
+      scheme->prev_conv = w.prev_conv;
+      scheme->n_walk = int64_t(w.n_walk);
+    }
+    const size_t nb = size_t(w.n_bins);
+    const bool arr = w.adaptive && w.saved != nullptr;
+    auto get = [&](void* dst, const void* src, size_t bytes) {
+      if (!dst) return;
+      if (arr)
+        HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+      else
+        std::memset(dst, 0, bytes);
+    };
+    get(edges, w.edges, 4 * nb);
+    get(edge_n, w.edge_n, 4 * nb);
+    get(edge_vals, w.edge_vals, 8 * nb * nb);
+    get(sub_bins, w.sub, 40 * nb);
+    get(saved, w.saved, nb);
+  });
+}
+
 int cemc_batch_wl_run(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* accept_bits,
                       int64_t words_per_chain, double* kernel_ms) {
   return guarded([&] {
     require(b != nullptr, "NULL batch");
     if (accept_bits) require(words_per_chain > 0, "words_per_chain must be positive");
     wl_launch(b, n_checks, check_every, accept_bits, words_per_chain, kernel_ms);
+  });
+}
+
+int cemc_batch_wl_run_shared(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* accept_bits,
+                             int64_t words_per_chain, double* kernel_ms) {
+  return guarded([&] {
+    require(b != nullptr, "NULL batch");
+    if (accept_bits) require(words_per_chain > 0, "words_per_chain must be positive");
+    wl_launch(b, n_checks, check_every, accept_bits, words_per_chain, kernel_ms, true);
   });
 }
 

@@ -75,11 +75,18 @@ struct LaunchWL {
   int64_t n_checks, check_every;
   uint32_t* accept_log;  // optional, as for the sweep kernel (bits OR-ed in)
   int64_t log_words;
+  const WLDev* shared = nullptr;  // walkers on this leader's shared histogram (one check per launch)
+  int64_t log_base = 0;           // accept-log bit of the launch's first counter
 };
 // Dynamic LDS of one walker: the sweep layout of its chain (base bytes)
-// followed by its ln g, counts and known flags.
-uint32_t wl_lds_bytes(uint32_t base, int n_bins);
+// followed by its ln g, counts and known flags (and, with adaptive windows,
+// its sub-bin counts and recorded-state flags).
+uint32_t wl_lds_bytes(uint32_t base, int n_bins, bool adaptive);
 hipError_t launch_wl(const LaunchWL& a, hipStream_t stream);
+// Flatness test and f update of the shared histogram of `leader` after a
+// check of `total` steps over its walkers.
+hipError_t launch_wl_shared_check(WLDev* leader, uint32_t* hist, const uint8_t* known, int64_t total,
+                                  hipStream_t stream);
 hipError_t launch_pt_exchange(PTDev* pt, ChainState* states, ChainParams* params, hipStream_t stream);
 hipError_t launch_cf(const ModelDev* models, const ChainParams* params, const int8_t* occ_pool,
                      const int32_t* chain_ids, int n_chains, int max_sites, double* partial, double* cf_out,

@@ -17,6 +17,7 @@
 //  restore_kernel   undo of popped changes (ce_updater.cpp:414-448).
 //  slot_energy_kernel  E = N * sum_k J_k cf_k of one history slot.
 #include "ce_sweep.h"
+#include "ce_wl.h"
 
 
 namespace cemc {
@@ -405,11 +406,11 @@ CEMC_DECLARE_SWEEP_FN(8, 1)
 #undef CEMC_DECLARE_SWEEP_FN
 
 const void* field_fn(bool canonical, int kr, int kbm);
-const void* wl_fn_p0(int kr, bool glds, int kbm, int nm);
-const void* wl_fn_p1(int kr, bool glds, int kbm, int nm);
+const void* wl_fn_p0(int kr, bool glds, int kbm, int nm, bool shared);
+const void* wl_fn_p1(int kr, bool glds, int kbm, int nm, bool shared);
 
-static const void* wl_fn(bool periodic, int kr, bool glds, int kbm, int nm = 3) {
-  return periodic ? wl_fn_p1(kr, glds, kbm, nm) : wl_fn_p0(kr, glds, kbm, nm);
+static const void* wl_fn(bool periodic, int kr, bool glds, int kbm, int nm = 3, bool shared = false) {
+  return periodic ? wl_fn_p1(kr, glds, kbm, nm, shared) : wl_fn_p0(kr, glds, kbm, nm, shared);
 }
 
 static const void* sweep_fn(bool periodic, bool canonical, int kr, bool glds, int kbm, int waves, int spec,
@@ -469,19 +470,20 @@ hipError_t sweep_set_max_lds(uint32_t bytes) {
     for (int kr : {0, 2, 4, 8})
       for (int g = 0; g < 2; ++g)
         for (int kbm : {1, 2, 3, 7})
-          for (int nm : {2, 3}) {
-          if (kbm == 7 && kr != 0) continue;
-          if (nm == 2 && kr == 0) continue;
-          const void* fn = wl_fn(p != 0, kr, g != 0, kbm, nm);
-          if (fn == nullptr) return hipErrorInvalidValue;
-          hipFuncAttributes attr;
-          hipError_t e = hipFuncGetAttributes(&attr, fn);
-          if (e != hipSuccess) return e;
-          const int dyn = int(bytes) - int(attr.sharedSizeBytes);
-          if (dyn <= 0) return hipErrorInvalidValue;
-          e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-          if (e != hipSuccess) return e;
-        }
+          for (int nm : {2, 3})
+            for (int sh = 0; sh < 2; ++sh) {
+              if (kbm == 7 && kr != 0) continue;
+              if (nm == 2 && kr == 0) continue;
+              const void* fn = wl_fn(p != 0, kr, g != 0, kbm, nm, sh != 0);
+              if (fn == nullptr) return hipErrorInvalidValue;
+              hipFuncAttributes attr;
+              hipError_t e = hipFuncGetAttributes(&attr, fn);
+              if (e != hipSuccess) return e;
+              const int dyn = int(bytes) - int(attr.sharedSizeBytes);
+              if (dyn <= 0) return hipErrorInvalidValue;
+              e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+              if (e != hipSuccess) return e;
+            }
   for (int cn = 0; cn < 2; ++cn)
     for (int kr : {2, 4, 8})
       for (int kbm : {1, 2, 3}) {
@@ -530,18 +532,14 @@ hipError_t launch_sweep(const LaunchSweep& a, hipStream_t stream) {
   return hipLaunchKernel(fn, dim3(a.n_chains), dim3(64 * a.waves), args, a.lds_bytes, stream);
 }
 
-uint32_t wl_lds_bytes(uint32_t base, int n_bins) {
-  const uint32_t a = (base + 15u) & ~15u;
-  const auto up = [](uint32_t v) { return (v + 15u) & ~15u; };
-  return a + up(uint32_t(n_bins) * 8u) + up(uint32_t(n_bins) * 4u) + up(uint32_t(n_bins));
-}
+uint32_t wl_lds_bytes(uint32_t base, int n_bins, bool adaptive) { return wl_layout(base, n_bins, adaptive).total; }
 
 hipError_t launch_wl(const LaunchWL& a, hipStream_t stream) {
   if (!((a.kr == 0 && (a.kbm == 1 || a.kbm == 2 || a.kbm == 3 || a.kbm == 7)) ||
         ((a.kr == 2 || a.kr == 4 || a.kr == 8) && (a.kbm == 1 || a.kbm == 2 || a.kbm == 3))))
     return hipErrorInvalidValue;
   if (a.nm != 3 && !(a.nm == 2 && a.kr > 0)) return hipErrorInvalidValue;
-  const void* fn = wl_fn(a.periodic, a.kr, a.gtab_lds, a.kbm, a.nm);
+  const void* fn = wl_fn(a.periodic, a.kr, a.gtab_lds, a.kbm, a.nm, a.shared != nullptr);
   if (fn == nullptr) return hipErrorInvalidValue;
   const ChainParams* params = a.params;
   ChainState* states = a.states;
@@ -556,9 +554,69 @@ hipError_t launch_wl(const LaunchWL& a, hipStream_t stream) {
   int64_t n_checks = a.n_checks, check_every = a.check_every;
   uint32_t* log = a.accept_log;
   int64_t log_words = a.log_words;
+  const WLDev* shared = a.shared;
+  int64_t log_base = a.log_base;
   void* args[] = {&params, &states, &models, &occ_pool, &trk_pool, &order, &walkers, &logdos, &hist, &known,
-                  &n_checks, &check_every, &log, &log_words};
+                  &n_checks, &check_every, &log, &log_words, &shared, &log_base};
   return hipLaunchKernel(fn, dim3(a.n_chains), dim3(64), args, a.lds_bytes, stream);
+}
+
+// The flatness test and modification-factor update of a shared histogram
+// after every walker of the group made its share of a check's steps
+// (wang_landau_sampler.cpp:326-394 with the inverse-time law of
+// :344-367): `total` steps join the group's time.  One wave.
+__global__ __launch_bounds__(64) void wl_shared_check_kernel(WLDev* __restrict__ leader,
+                                                             uint32_t* __restrict__ hist_pool,
+                                                             const uint8_t* __restrict__ known_pool,
+                                                             int64_t total) {
+  const int lane = int(threadIdx.x);
+  if (leader->converged) return;
+  const int nb = leader->n_bins;
+  uint32_t* hist = hist_pool + leader->bin_off;
+  const uint8_t* known = known_pool + leader->bin_off;
+  uint32_t sum = 0u, cnt = 0u, mn = 100000000u;
+  for (int i = lane; i < nb; i += 64) {
+    if (known[i]) {
+      const uint32_t h = hist[i];
+      sum += h;
+      cnt += 1u;
+      mn = min(mn, h);
+    }
+  }
+  sum = wave_add_u32(sum);
+  cnt = wave_add_u32(cnt);
+  mn = wave_min_u32(mn);
+  const bool flat = __builtin_amdgcn_readfirstlane(int(double(mn) > leader->flatness * (double(sum) / double(cnt))));
+  const int64_t steps = leader->steps + total, took = leader->since_flat + total;
+  double f = leader->f, inv_factor = leader->inv_factor;
+  int inv_active = leader->inv_active;
+  if (flat) {
+    for (int i = lane; i < nb; i += 64) hist[i] = 0u;
+    const double mc_time = double(steps) / double(nb);
+    if (leader->inverse_time && leader->prev_conv >= 0 && took > leader->prev_conv) {
+      if (!inv_active) inv_factor = mc_time * f;
+      inv_active = 1;
+    }
+    f = inv_active ? inv_factor / mc_time : f / 2.0;
+  }
+  if (lane == 0) {
+    leader->steps = steps;
+    leader->since_flat = flat ? 0 : took;
+    if (flat) {
+      leader->prev_conv = took;
+      leader->n_flat += 1;
+    }
+    leader->inv_active = inv_active;
+    leader->inv_factor = inv_factor;
+    leader->f = f;
+    if (f < leader->f_min) leader->converged = 1;
+  }
+}
+
+hipError_t launch_wl_shared_check(WLDev* leader, uint32_t* hist, const uint8_t* known, int64_t total,
+                                  hipStream_t stream) {
+  hipLaunchKernelGGL(wl_shared_check_kernel, dim3(1), dim3(64), 0, stream, leader, hist, known, total);
+  return hipGetLastError();
 }
 
 hipError_t launch_pt_exchange(PTDev* pt, ChainState* states, ChainParams* params, hipStream_t stream) {

@@ -20,9 +20,29 @@ fixed-f runs of the same system (the reference's fixed_f scheme asks for
 several runs whose ln g are averaged: wang_landau_sampler.py:55-57); they
 run in one launch, one wavefront each, with ln g and the counts in LDS.
 
-Not carried over: the inverse-time scheme, adaptive energy windows
-(adaptive_windows.cpp), sub-bin distributions, the Python ``run`` loop with
-its SGC moves, and the database records.
+Schemes of the native sampler (``cemc_batch_wl_set_scheme``):
+
+* the inverse-time law (wang_landau_sampler.cpp:344-367): once a flat
+  histogram takes more steps than the previous one, f = f_act t_act / t at
+  every later flat histogram, with t = steps / Nbins.  The reference times
+  convergence in CPU seconds; here it is counted in steps so that runs
+  repeat.  ``scheme="inverse_time"`` switches it on for run_fast_sampler
+  (the reference's run_fast_sampler leaves it off whatever the scheme; its
+  Python-loop InverseTimeScheme is not carried over);
+* adaptive energy windows (``mode="adaptive_windows"``,
+  adaptive_windows.cpp, wang_landau_sampler.cpp:575-580): the window shrinks
+  from the top as its upper part converges, with the walker's lattice
+  recorded at the first visit of every bin and sub-bin visit counts kept.
+
+With ``shared_histogram=True`` the walkers visit one histogram, as the
+reference's OpenMP walkers do (wang_landau_sampler.cpp:288-298): ln g and
+the counts grow by atomic additions in device memory, each check's steps are
+split over the walkers, and walker 0 carries the result.  Such runs do not
+repeat exactly when f > 0, since a walker reads ln g as the others' visits
+land.
+
+Not carried over: the Python ``run`` loop with its SGC moves (it reads and
+writes the reference's database) and the database records.
 """
 from __future__ import annotations
 
@@ -68,16 +88,20 @@ class WangLandau:
     :param int n_walkers: independent walkers (chains), all started from atoms
     :param seed: Philox key word 0 (default: drawn from the OS)
     :param int uid: Philox key word 1 of walker 0 (walker w uses uid + w)
+    :param str scheme: "fixed_f", "square_root_reduction" (both halve f at
+        a flat histogram in the native sampler) or "inverse_time"
+    :param bool shared_histogram: the walkers visit one histogram (their
+        steps per check add up to check_convergence_every)
     """
 
     def __init__(self, atoms, Nbins=100, initial_f=2.71, flatness_criteria=0.8, fmin=1e-6, Emin=0.0, Emax=1.0,
                  check_convergence_every=1000, n_walkers=1, seed=None, uid=0, scheme="fixed_f",
-                 ensemble="canonical"):
+                 ensemble="canonical", shared_histogram=False):
         calc = atoms.get_calculator() if hasattr(atoms, "get_calculator") else getattr(atoms, "calc", None)
         if calc is None or not hasattr(calc, "updater"):
             raise ValueError("the atoms object needs a cemc_amd CE calculator attached")
-        if scheme != "fixed_f":
-            raise NotImplementedError("only the fixed_f scheme runs on the GPU")
+        if scheme not in ("fixed_f", "inverse_time", "square_root_reduction"):
+            raise ValueError("scheme has to be one of ['fixed_f', 'inverse_time', 'square_root_reduction']")
         if ensemble != "canonical":
             raise NotImplementedError("the GPU walkers make canonical swaps")
         if not Emax > Emin:
@@ -94,6 +118,8 @@ class WangLandau:
         self.flatness_criteria = float(flatness_criteria)
         self.check_convergence_every = int(check_convergence_every)
         self.n_walkers = int(n_walkers)
+        self.scheme = scheme
+        self.shared_histogram = bool(shared_histogram)
         self.seed = int.from_bytes(os.urandom(8), "little") if seed is None else int(seed)
         self.uid = int(uid)
         occ = calc.updater.get_occupation()
@@ -126,22 +152,60 @@ class WangLandau:
             h.known_state = st["known"]
             self.current_bin[w] = st["current_bin"]
             self.walkers.append(st)
-        self.f_values = [st["f"] for st in self.walkers]
+        if self.shared_histogram:  # walker 0 holds the group's histogram and f
+            self.f_values = [self.walkers[0]["f"]]
+            self.converged = bool(self.walkers[0]["converged"])
+            self.iter = sum(st["n_moves"] for st in self.walkers)
+        else:
+            self.f_values = [st["f"] for st in self.walkers]
+            self.converged = all(st["converged"] for st in self.walkers)
+            self.iter = max(st["n_moves"] for st in self.walkers)
         self.f = max(self.f_values)
-        self.converged = all(st["converged"] for st in self.walkers)
-        self.iter = max(st["n_moves"] for st in self.walkers)
 
-    def run_fast_sampler(self, maxsteps=10000000, mode="regular"):
+    def run_fast_sampler(self, maxsteps=10000000, mode="regular", minimum_window_width=10, sub_bin_file=None,
+                         use_inverse_time_algorithm=None):
         """``WangLandauSampler::run(maxsteps)`` on every walker: maxsteps //
         check_convergence_every rounds of check_convergence_every moves, each
-        followed by the flatness test (wang_landau_sampler.py:450-469)."""
-        if mode != "regular":
-            raise NotImplementedError("adaptive energy windows are not available on the GPU path")
+        followed by the flatness test (wang_landau_sampler.py:450-469).
+
+        :param mode: "regular" or "adaptive_windows"
+        :param minimum_window_width: smallest window of adaptive windows
+        :param sub_bin_file: CSV to write walker 0's sub-bin counts to (one
+            row of 10 counts per bin; adaptive windows only)
+        :param use_inverse_time_algorithm: default: scheme == "inverse_time"
+        The schemes start afresh at every call, as the reference builds a
+        new sampler per call."""
+        if mode not in ("regular", "adaptive_windows"):
+            raise ValueError("Unknown mode. Has to one of ['regular', 'adaptive_windows']")
+        inv = self.scheme == "inverse_time" if use_inverse_time_algorithm is None else bool(use_inverse_time_algorithm)
+        for w in range(self.n_walkers):
+            self._batch.wl_set_scheme(w, inverse_time=inv, adaptive=mode == "adaptive_windows",
+                                      min_width=int(minimum_window_width))
+        if self.shared_histogram and mode == "adaptive_windows":
+            raise ValueError("adaptive windows run with one walker per histogram")
         n_outer = int(maxsteps) // self.check_convergence_every
         if n_outer > 0:
-            self._batch.wl_run(n_outer, self.check_convergence_every)
+            self._batch.wl_run(n_outer, self.check_convergence_every, shared=self.shared_histogram)
         self._pull()
+        if sub_bin_file is not None and mode == "adaptive_windows":
+            self.save_sub_bin_distribution(sub_bin_file)
         return self.converged
+
+    def scheme_state(self, walker=0):
+        """The walker's scheme state (cemc_wl_scheme fields, window cuts,
+        sub-bin counts and recorded-state flags)."""
+        return self._batch.wl_scheme(walker)
+
+    def sub_bin_distribution(self, walker=0):
+        """Visit counts of 10 sub-bins per bin, (Nbins, 10)."""
+        return self._batch.wl_scheme(walker)["sub_bins"]
+
+    def save_sub_bin_distribution(self, fname, walker=0):
+        """Histogram::save_sub_bin_distribution (histogram.cpp:197-218): one
+        line per bin, each count followed by a comma."""
+        with open(fname, "w") as out:
+            for row in self.sub_bin_distribution(walker):
+                out.write("".join("%d," % v for v in row) + "\n")
 
     def occupation(self, walker=0):
         """Current lattice of one walker (species ids)."""
@@ -153,7 +217,10 @@ class WangLandau:
     def logdos(self, walker=None):
         """ln g per bin: of one walker, or the mean over the walkers of each
         walker's ln g shifted to zero at the lowest bin every walker knows
-        (NaN on bins some walker has not visited)."""
+        (NaN on bins some walker has not visited); with a shared histogram,
+        the shared ln g."""
+        if self.shared_histogram and walker is None:
+            walker = 0
         if walker is not None:
             h = self.histograms[walker]
             out = np.where(h.known_state != 0, h.logdos, np.nan)

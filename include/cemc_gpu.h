@@ -328,7 +328,7 @@ CEMC_API int cemc_batch_get_hooks_state(cemc_batch* batch, int32_t chain, int64_
  * reference's native sampler (cpp/src/wang_landau_sampler.cpp, bound by
  * cemc/cpp_ext/pywang_landau_sampler.pyx:4-30) with one histogram per walker
  * instead of one shared under a lock, so each chain is an independent
- * fixed-f Wang-Landau run and their ln g can be averaged.  Energies are the
+ * Wang-Landau run and their ln g can be averaged.  Energies are the
  * chain's total energy (eV). */
 typedef struct cemc_wl_desc {
   int32_t n_bins;      /* Histogram::Nbins */
@@ -353,14 +353,71 @@ CEMC_API int cemc_batch_wl_set(cemc_batch* batch, int32_t chain, const cemc_wl_d
  * the range, proposals into the current bin, flat histograms seen. */
 CEMC_API int cemc_batch_wl_get(cemc_batch* batch, int32_t chain, cemc_wl_desc* desc, double* logdos,
                                int32_t* hist, uint8_t* known, int64_t* counters);
+/* Schemes of a walker's WangLandauSampler::run (wang_landau_sampler.cpp:
+ * 319-404).  inverse_time: use_inverse_time_algorithm (:344-367): once a flat
+ * histogram took more steps than the previous one, f = f_act t_act / t at
+ * every later flat histogram, t = steps / n_bins (get_mc_time, :605; the
+ * reference times convergence in CPU seconds, here in steps so that runs
+ * repeat).  adaptive: use_adaptive_windows(min_width) (:575-580,
+ * cpp/src/adaptive_windows.cpp), with sub-bin visit counts (10 per bin,
+ * Histogram::update_sub_bin, histogram.cpp:92-100).  The other fields are
+ * outputs of cemc_batch_wl_get_scheme. */
+typedef struct cemc_wl_scheme {
+  int32_t inverse_time; /* in: use the inverse-time law */
+  int32_t adaptive;     /* in: adaptive energy windows */
+  int32_t min_width;    /* in: minimum_window_width */
+  int32_t inv_active;   /* the inverse-time law is in force */
+  int32_t upper;        /* bins [0, upper) are in range */
+  int32_t n_overlap;    /* bins kept above a cut */
+  int32_t n_edges;      /* cuts made since the last flat window */
+  int32_t has_first;    /* the state at the first cut is recorded */
+  double inv_factor;    /* f t at activation of the inverse-time law */
+  double bin_change;    /* adaptive: sum of |proposed bin - current bin| since the last flat histogram */
+  int64_t steps;        /* steps since the scheme was set */
+  int64_t since_flat;   /* steps since the last flat histogram */
+  int64_t prev_conv;    /* steps the previous flat histogram took (-1: none yet) */
+  int64_t n_walk;       /* moves made to bring the walker back into its window */
+} cemc_wl_scheme;
+
+/* Choose the schemes of walker `chain` and start them afresh (the
+ * reference builds a new sampler per run_fast_sampler call,
+ * wang_landau_sampler.py:450-469): counters, windows, recorded states and
+ * sub-bin counts restart.  Adaptive windows keep (n_bins + 1) lattices and
+ * trackers of the chain (3 bytes per site each) and n_bins^2 doubles of
+ * saved ln g in device memory. */
+CEMC_API int cemc_batch_wl_set_scheme(cemc_batch* batch, int32_t chain, const cemc_wl_scheme* scheme);
+/* Read a walker's scheme state; edges and edge_n (n_bins entries), edge_vals
+ * (n_bins x n_bins, row e: ln g saved above cut e), sub_bins (n_bins x 10)
+ * and saved (n_bins recorded-state flags) may be NULL, and are zero
+ * without adaptive windows. */
+CEMC_API int cemc_batch_wl_get_scheme(cemc_batch* batch, int32_t chain, cemc_wl_scheme* scheme, int32_t* edges,
+                                      int32_t* edge_n, double* edge_vals, uint32_t* sub_bins, uint8_t* saved);
+
 /* WangLandauSampler::run (wang_landau_sampler.cpp:319-404) on every walker
  * of the batch at once: n_checks rounds of check_every moves (step,
- * :232-317), each followed by the flatness test; a flat histogram halves f
- * and clears the counts, and a walker stops at the first check with
- * f < f_min.  accept_bits (optional) as for cemc_batch_run_logged;
- * kernel_ms (optional) receives the device time. */
+ * :232-317), each followed by the flatness test; a flat histogram lowers f
+ * (halves it, or follows the inverse-time law of cemc_wl_scheme) and clears
+ * the counts, and a walker stops at the first check with f < f_min.
+ * accept_bits (optional) as for cemc_batch_run_logged, bit t for
+ * Philox counter t of the launch (the moves that bring a walker back into
+ * an adaptive window use counters too and set no bit); kernel_ms (optional)
+ * receives the device time.  Fails when an adaptive window's walker finds
+ * no state inside its window within 10^7 moves. */
 CEMC_API int cemc_batch_wl_run(cemc_batch* batch, int64_t n_checks, int64_t check_every, uint32_t* accept_bits,
                                int64_t words_per_chain, double* kernel_ms);
+/* WangLandauSampler::run with its walkers on one shared histogram
+ * (wang_landau_sampler.cpp:288-298, :321-335): every chain's walker visits
+ * walker 0's bins, f and scheme; check_every steps per check (rounded up to
+ * a multiple of the chain count) are split evenly over the walkers, and the
+ * flatness test and f update follow each check.  Walkers read ln g as the
+ * others' visits land, so runs with several walkers and f > 0 do not
+ * repeat exactly (as the reference's threads do not); every walker needs
+ * the same bins and range, and adaptive windows are not allowed.  Walker
+ * 0's descriptor and bins hold the group's results; the others keep their
+ * own lattice, current bin and counters.  accept_bits: bit t of chain c for
+ * its t-th step. */
+CEMC_API int cemc_batch_wl_run_shared(cemc_batch* batch, int64_t n_checks, int64_t check_every,
+                                      uint32_t* accept_bits, int64_t words_per_chain, double* kernel_ms);
 
 /* ------------------------------------------------------------------ pool
  * A batch whose chains join and leave while it runs: the resident state of

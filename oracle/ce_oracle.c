@@ -895,7 +895,7 @@ int ceo_wl_run(ceo_chain* c, ceo_wl* w, uint64_t seed, uint64_t chain_uid, uint6
       w->steps += 1;
       w->since_flat += 1;
       const double raw = wl_raw_bin(w, p.e_new);
-      w->bin_change += fabs(raw - (double)w->cur_bin);
+      if (w->adaptive) w->bin_change += fabs(raw - (double)w->cur_bin); /* only set_overlap reads it */
       const int bin = raw < 0.0 ? -1 : (raw > (double)w->n_bins ? w->n_bins + 1 : (int)raw);
       if (bin < 0 || bin >= w->upper) {
         w->n_outside += 1;

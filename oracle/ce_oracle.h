@@ -148,7 +148,7 @@ typedef struct ceo_wl {
   int32_t n_overlap;  /* adaptive: bins kept above a cut (starts at 7) */
   int32_t n_edges, has_first;
   double inv_factor;  /* f_act * t_act of the inverse-time scheme */
-  double bin_change;  /* sum of |proposed bin - current bin| since the last flat histogram */
+  double bin_change;  /* adaptive: sum of |proposed bin - current bin| since the last flat histogram */
   double cur_energy;  /* energy of the last accepted state (the sub-bin energy) */
   int64_t steps;      /* steps since the scheme was set (the time t counts these) */
   int64_t since_flat; /* steps since the last flat histogram */

@@ -223,3 +223,291 @@ def test_inverse_time_rule_and_convergence(cell):
     for fl in flats[act:]:
         assert abs(fl["f"] * fl["steps"] / NB - ft) <= 1e-12 * ft
     assert _ln_g_error(cell, wl) < 0.6
+
+
+def _walk_setup(c, lib_wl):
+    """A walker on a high-energy state, every occupied bin known with flat
+    counts except a low one, one step per check: the first check cuts the
+    window below the walker with at most one recorded state, so the walker
+    must walk down into the window."""
+    known = (c["g"] > 0).astype(np.uint8)
+    kb = np.flatnonzero(known)
+    low = int(kb[len(kb) // 2])
+    hist = np.where(known, 100000, 0).astype(np.int32)
+    hist[low] = 10
+    k = int(np.argmax(c["energies"]))
+    b = int((c["energies"][k] - EMIN) * NB / (EMAX - EMIN))
+    assert b > low + 8
+    return k, dict(cur_bin=b, is_first=False, hist=hist, known=known), low
+
+
+def test_walk_into_a_window_below_the_walker(cell):
+    k, preset, low = _walk_setup(cell, None)
+    upd = _updater(cell, k)
+    wl = _walker(cell, f=1e-3, f_min=-1.0, adaptive=True, min_width=3, **preset)
+    used = []
+    wl_run(upd, wl, 8, 3, 1, 1, counters=used)
+    st = wl.state
+    assert st.n_edges == 1 and st.upper == low + 8
+    assert wl.saved.sum() <= 1 and st.n_walk > 0
+    assert used[0] == 1 + st.n_walk
+    assert 0 <= st.cur_bin < st.upper
+    e = _energy_of(cell, upd.get_occupation())
+    assert int((e - EMIN) * NB / (EMAX - EMIN)) == st.cur_bin
+    # the walk went downhill only: every move it kept lowered the energy
+    assert e < cell["energies"][k]
+
+
+# ------------------------------------------------------------------ GPU
+def _gpu_batch(c, occ, seed, uid):
+    from cemc_amd.batch import CANONICAL, ChainBatch
+    from cemc_amd.model import CEModel
+    return ChainBatch([CEModel(c["s"], c["eci"])], [dict(kind=CANONICAL, temperature=1000.0, occupation=occ,
+                                                          seed=seed, uid=uid)])
+
+
+def _compare(batch, chain, wl, ora, log, n_counters, s):
+    got = batch.wl_get(chain)
+    sch = batch.wl_scheme(chain)
+    st = wl.state
+    bits = np.unpackbits(log[chain], bitorder="little")
+    ref = np.unpackbits(np.asarray(wl.ref_bits, dtype=np.uint8), bitorder="little")
+    # both logs keep the bits of the counters that fit (moves back into a
+    # window extend the counters past the steps)
+    n = min(int(n_counters), bits.size, ref.size)
+    bits, ref = bits[:n], ref[:n]
+    mism = np.nonzero(bits != ref)[0]
+    assert mism.size == 0, "first mismatching counter %d" % mism[0]
+    assert got["n_moves"] == st.n_moves and sch["n_walk"] == st.n_walk
+    assert np.array_equal(got["hist"], wl.hist)
+    assert np.array_equal(got["known"], wl.known)
+    assert np.array_equal(got["logdos"], wl.logdos)
+    assert got["f"] == st.f and got["current_bin"] == st.cur_bin and got["converged"] == st.converged
+    assert (got["n_accept"], got["n_outside"], got["n_self"], got["n_flat"]) == \
+        (st.n_accept, st.n_outside, st.n_self, st.n_flat)
+    for key in ("inv_active", "n_overlap", "n_edges", "has_first", "steps", "since_flat", "prev_conv"):
+        assert sch[key] == getattr(st, key), key
+    assert sch["inv_factor"] == st.inv_factor
+    assert abs(sch["bin_change"] - st.bin_change) <= 1e-9 * max(1.0, st.bin_change)
+    if st.adaptive:
+        assert sch["upper"] == st.upper
+        assert np.array_equal(sch["sub_bins"].ravel(), wl.sub_hist)
+        assert np.array_equal(sch["saved"], wl.saved)
+        ne = st.n_edges
+        assert np.array_equal(sch["edges"][:ne], wl.edges[:ne])
+        assert np.array_equal(sch["edge_n"][:ne], wl.edge_n[:ne])
+        vals = wl.edge_vals.reshape(NB, NB)
+        for e in range(ne):  # row e holds edge_n[e] values
+            assert np.array_equal(sch["edge_vals"][e, :wl.edge_n[e]], vals[e, :wl.edge_n[e]])
+    assert np.array_equal(batch.occupation(chain), ora.get_occupation())
+    assert abs(batch.energies()[chain] - ora.get_energy()) / s.n_sites <= 1e-10
+
+
+def _run_pair(c, k, seed, uid, wl_kw, n_checks, check_every, batch=None, chain=0):
+    """The oracle walker and the GPU walker `chain` of `batch` (a new batch
+    on state k when None) from the same start; returns (batch, oracle
+    walker, oracle updater, counters used)."""
+    ora = _updater(c, k)
+    wl = _walker(c, **wl_kw)
+    used = []
+    wl.ref_bits = wl_run(ora, wl, seed, uid, n_checks, check_every, counters=used)
+    if batch is None:
+        batch = _gpu_batch(c, c["states"][k], seed, uid)
+    kw = dict(wl_kw)
+    scheme = {key: kw.pop(key) for key in ("inverse_time", "adaptive", "min_width") if key in kw}
+    for key in ("f", "f_min", "flatness", "cur_bin"):
+        kw.pop(key, None)
+    init = wl_kw.get("cur_bin", 0)
+    batch.wl_set(chain, NB, EMIN, EMAX, wl_kw.get("f", 1.0), f_min=wl.state.f_min, flatness=wl.state.flatness,
+                 current_bin=init, **kw)
+    batch.wl_set_scheme(chain, **scheme)
+    return batch, wl, ora, used[0]
+
+
+def _sub_edges_clear(c):
+    """No energy level within 1e-6 of a sub-bin edge (the running energy on
+    the GPU and the oracle's recomputed one differ in the last bits)."""
+    e = c["energies"]
+    x = (e - EMIN) * NB / (EMAX - EMIN)
+    b = np.trunc(x).astype(int)
+    lo = EMIN + ((EMAX - EMIN) * b) / NB
+    hi = EMIN + ((EMAX - EMIN) * (b + 1)) / NB
+    y = (e - lo) * 10.0 / (hi - lo)
+    return np.min(np.abs(y - np.round(y)))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n_checks", [40, 3000])
+def test_gpu_adaptive_walker_matches_oracle(gpu, cell, n_checks):
+    """Mid-run (windows cut, not yet joined) and to convergence."""
+    assert _sub_edges_clear(cell) > 1e-6
+    print("adaptive walker", n_checks, flush=True)
+    batch, wl, ora, n = _run_pair(cell, 101, 1, 3, dict(adaptive=True, min_width=3), n_checks, 1000)
+    log, _ = batch.wl_run(n_checks, 1000, log=True)
+    _compare(batch, 0, wl, ora, log, n, cell["s"])
+    if n_checks == 3000:
+        assert wl.state.converged == 1 and wl.state.has_first == 1
+    else:
+        assert wl.state.n_edges > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_gpu_inverse_time_walker_matches_oracle(gpu, cell, adaptive):
+    print("inverse-time walker, adaptive", adaptive, flush=True)
+    batch, wl, ora, n = _run_pair(cell, 13, 7, 1, dict(inverse_time=True, adaptive=adaptive, min_width=3,
+                                                       f_min=2e-3), 600, 1000)
+    log, _ = batch.wl_run(600, 1000, log=True)
+    _compare(batch, 0, wl, ora, log, n, cell["s"])
+    assert wl.state.inv_active == 1
+
+
+@pytest.mark.gpu
+def test_gpu_walk_into_window_matches_oracle(gpu, cell):
+    k, preset, low = _walk_setup(cell, None)
+    print("walk into window", flush=True)
+    batch, wl, ora, n = _run_pair(cell, k, 8, 3, dict(f=1e-3, f_min=-1.0, adaptive=True, min_width=3, **preset),
+                                  1, 1)
+    log, _ = batch.wl_run(1, 1, log=True)
+    assert wl.state.n_walk > 0
+    _compare(batch, 0, wl, ora, log, n, cell["s"])
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_schemes_in_one_batch(gpu, cell):
+    """Fixed f, inverse time and adaptive windows on three walkers of one
+    launch, continued over a second launch."""
+    from cemc_amd.batch import CANONICAL, ChainBatch
+    from cemc_amd.model import CEModel
+    model = CEModel(cell["s"], cell["eci"])
+    ks = (21, 22, 23)
+    batch = ChainBatch([model] * 3, [dict(kind=CANONICAL, temperature=1000.0, occupation=cell["states"][k],
+                                          seed=5, uid=10 + i) for i, k in enumerate(ks)])
+    kws = (dict(), dict(inverse_time=True), dict(adaptive=True, min_width=3))
+    pairs = []
+    for i, (k, kw) in enumerate(zip(ks, kws)):
+        _, wl, ora, _ = _run_pair(cell, k, 5, 10 + i, dict(kw, f_min=1e-3), 0, 1000, batch=batch, chain=i)
+        pairs.append((wl, ora))
+    print("mixed schemes", flush=True)
+    for rep in range(2):
+        log, _ = batch.wl_run(150, 1000, log=True)
+        for i, (wl, ora) in enumerate(pairs):
+            used = []
+            move_begin = int(wl.state.n_moves + wl.state.n_walk)
+            wl.ref_bits = wl_run(ora, wl, 5, 10 + i, 150, 1000, move_begin=move_begin, counters=used)
+            _compare(batch, i, wl, ora, log, used[0], cell["s"])
+
+
+@pytest.mark.gpu
+def test_python_driver_adaptive_windows(gpu, cell, tmp_path):
+    """WangLandau with the CE calculator in adaptive-window mode: eight
+    walkers to f < 1e-5 through several cuts; their averaged ln g matches
+    the enumerated one, and the sub-bin counts are written as CSV."""
+    from cemc_amd.calculator import CE
+    from cemc_amd.lattice import SimpleAtoms
+    from cemc_amd.wang_landau import WangLandau
+    s = cell["s"]
+    occ = cell["states"][100]
+    atoms = SimpleAtoms([s.species[v] for v in occ])
+    CE(atoms, s, eci=dict(cell["eci"]))
+    wl = WangLandau(atoms, Nbins=NB, initial_f=1.0, flatness_criteria=0.8, fmin=1e-5, Emin=EMIN, Emax=EMAX,
+                    check_convergence_every=1000, n_walkers=8, seed=6)
+    out = tmp_path / "subbin.csv"
+    print("python driver, adaptive windows", flush=True)
+    assert wl.run_fast_sampler(maxsteps=3000000, mode="adaptive_windows", minimum_window_width=3,
+                               sub_bin_file=str(out))
+    assert all(wl.scheme_state(w)["has_first"] == 1 for w in range(8))
+    g = cell["g"]
+    known = g > 0
+    est = wl.logdos()
+    assert np.array_equal(~np.isnan(est), known)
+    d = est[known] - np.log(g[known])
+    d -= d.mean()
+    # eight walkers' windows join at different bins; their mean is closer
+    # than one walker's bound
+    assert np.abs(d).max() < 1.0, d
+    rows = out.read_text().splitlines()
+    assert len(rows) == NB and all(r.count(",") == 10 for r in rows)
+    assert sum(int(v) for r in rows for v in r.split(",") if v) == int(wl.sub_bin_distribution(0).sum())
+    wl.close()
+
+
+# ------------------------------------------------------- shared histogram
+@pytest.mark.gpu
+@pytest.mark.parametrize("inverse_time", [False, True])
+def test_gpu_shared_histogram_one_walker_matches_oracle(gpu, cell, inverse_time):
+    """One walker on a shared histogram makes the same moves, visits and f
+    updates as the oracle's own-histogram walker (nothing races)."""
+    print("shared histogram, one walker, inverse time", inverse_time, flush=True)
+    batch, wl, ora, n = _run_pair(cell, 31, 2, 4, dict(inverse_time=inverse_time, f_min=1e-3), 400, 1000)
+    log, _ = batch.wl_run(400, 1000, log=True, shared=True)
+    got = batch.wl_get(0)
+    st = wl.state
+    bits = np.unpackbits(log[0], bitorder="little")[:n]
+    ref = np.unpackbits(np.asarray(wl.ref_bits, dtype=np.uint8), bitorder="little")[:n]
+    assert np.array_equal(bits, ref)
+    assert np.array_equal(got["hist"], wl.hist) and np.array_equal(got["known"], wl.known)
+    assert np.array_equal(got["logdos"], wl.logdos)
+    assert got["f"] == st.f and got["converged"] == st.converged and got["n_flat"] == st.n_flat
+    assert got["n_moves"] == st.n_moves and got["current_bin"] == st.cur_bin
+    sch = batch.wl_scheme(0)
+    assert (sch["inv_active"], sch["steps"], sch["prev_conv"]) == (st.inv_active, st.steps, st.prev_conv)
+    assert np.array_equal(batch.occupation(0), ora.get_occupation())
+
+
+@pytest.mark.gpu
+def test_gpu_shared_histogram_with_fixed_ln_g_adds_the_walkers_visits(gpu, cell):
+    """f = 0 and never flat: ln g stays as given, so each walker's moves are
+    its own oracle walker's and the shared counts are the sum of theirs."""
+    from cemc_amd.batch import CANONICAL, ChainBatch
+    from cemc_amd.model import CEModel
+    g = cell["g"]
+    lg = np.log(np.where(g > 0, g, 1.0))
+    ks = list(range(40, 48))
+    W = len(ks)
+    batch = ChainBatch([CEModel(cell["s"], cell["eci"])] * W,
+                       [dict(kind=CANONICAL, temperature=1000.0, occupation=cell["states"][k], seed=9, uid=k)
+                        for k in ks])
+    for w in range(W):
+        batch.wl_set(w, NB, EMIN, EMAX, 0.0, f_min=-1.0, flatness=2.0, logdos=lg if w == 0 else None)
+    per, n_checks = 250, 40
+    print("shared histogram, fixed ln g,", W, "walkers", flush=True)
+    log, _ = batch.wl_run(n_checks, per * W, log=True, shared=True)
+    hist = np.zeros(NB, dtype=np.int64)
+    for w, k in enumerate(ks):
+        ora = _updater(cell, k)
+        wl = OracleWL(NB, EMIN, EMAX, 0.0, f_min=-1.0, flatness=2.0, logdos=lg)
+        ref = wl_run(ora, wl, 9, k, n_checks, per)
+        n = n_checks * per
+        assert np.array_equal(np.unpackbits(log[w], bitorder="little")[:n],
+                              np.unpackbits(ref, bitorder="little")[:n]), w
+        assert np.array_equal(batch.occupation(w), ora.get_occupation())
+        assert batch.wl_get(w)["n_accept"] == wl.state.n_accept
+        hist += wl.hist
+    got = batch.wl_get(0)
+    assert np.array_equal(got["hist"], hist)
+    assert np.array_equal(got["logdos"], lg)
+
+
+@pytest.mark.gpu
+def test_gpu_shared_histogram_converges_to_exact_dos(gpu, cell):
+    """Sixteen walkers on one histogram, f from 1 to below 1e-5."""
+    from cemc_amd.calculator import CE
+    from cemc_amd.lattice import SimpleAtoms
+    from cemc_amd.wang_landau import WangLandau
+    s = cell["s"]
+    atoms = SimpleAtoms([s.species[v] for v in cell["states"][100]])
+    CE(atoms, s, eci=dict(cell["eci"]))
+    wl = WangLandau(atoms, Nbins=NB, initial_f=1.0, flatness_criteria=0.8, fmin=1e-5, Emin=EMIN, Emax=EMAX,
+                    check_convergence_every=16000, n_walkers=16, seed=12, shared_histogram=True)
+    print("shared histogram, 16 walkers", flush=True)
+    assert wl.run_fast_sampler(maxsteps=16000 * 400)
+    assert wl.f < 1e-5
+    g = cell["g"]
+    known = g > 0
+    est = wl.logdos()
+    assert np.array_equal(~np.isnan(est), known)
+    d = est[known] - np.log(g[known])
+    d -= d.mean()
+    assert np.abs(d).max() < 0.3, d
+    wl.close()
