@@ -1579,8 +1579,10 @@ void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* h
     bool glds;
     int kbm;
     int nm;
+    bool adaptive;
     bool operator<(const WKey& o) const {
-      return std::tie(periodic, kr, glds, kbm, nm) < std::tie(o.periodic, o.kr, o.glds, o.kbm, o.nm);
+      return std::tie(periodic, kr, glds, kbm, nm, adaptive) <
+             std::tie(o.periodic, o.kr, o.glds, o.kbm, o.nm, o.adaptive);
     }
   };
   std::map<WKey, std::pair<std::vector<int32_t>, uint32_t>> groups;
@@ -1604,6 +1606,7 @@ void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* h
     const int n_rec = k.kr == 0 ? int(m->recs.size()) : 0;
     const int nbins = shared ? 0 : b->wl[c].n_bins;
     const bool adaptive = !shared && b->wl[c].adaptive != 0;
+    k.adaptive = adaptive;
     const uint32_t with_g =
         wl_lds_bytes(sweep_lds_bytes(occ_bytes, m->n_sites, n_rec, int(m->gtab.size())), nbins, adaptive);
     const uint32_t no_g = wl_lds_bytes(sweep_lds_bytes(occ_bytes, m->n_sites, n_rec, 0), nbins, adaptive);
@@ -1656,6 +1659,7 @@ void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* h
     a.accept_log = d_log;
     a.log_words = log_words;
     a.shared = shared ? b->d_wl : nullptr;
+    a.adaptive = kv.first.adaptive;
     a.log_base = shared ? chk * per_walker : 0;
     HIP_CHECK(launch_wl(a, b->stream));
     begin += a.n_chains;

@@ -72,13 +72,15 @@ __device__ __attribute__((always_inline)) inline uint32_t wave_min_u32(uint32_t 
 
 }  // namespace
 
-// kShared: every walker of the launch visits the histogram of `shared` (the
-// group's leader) in global memory, as the reference's OpenMP walkers share
-// one (wang_landau_sampler.cpp:288-298): ln g and counts grow by atomic
-// additions and ln g is read at device scope, so walkers see each other's
-// visits as they land.  Such a launch makes one check's steps and leaves the
-// flatness test, f and convergence to wl_shared_check_kernel.
-template <bool kPeriodic, int kR, bool kGtabLds, int kBM, int kNM = 3, bool kShared = false>
+// kMode 0: own histogram, fixed f or the inverse-time law (the adaptive
+// window code is compiled out); 1: own histogram with adaptive windows
+// possible; 2 (shared): every walker of the launch visits the histogram of
+// `shared` (the group's leader) in global memory, as the reference's OpenMP
+// walkers share one (wang_landau_sampler.cpp:288-298): ln g and counts grow
+// by atomic additions and ln g is read at device scope, so walkers see each
+// other's visits as they land.  A shared launch makes one check's steps and
+// leaves the flatness test, f and convergence to wl_shared_check_kernel.
+template <bool kPeriodic, int kR, bool kGtabLds, int kBM, int kNM = 3, int kMode = 0>
 __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ params,
                                                 ChainState* __restrict__ states,
                                                 const ModelDev* __restrict__ models,
@@ -93,6 +95,7 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
   __shared__ double dphi_tab[kMaxSpecies * kMaxSpecies * kMaxBasis];  // [(old*S+new)*B + d]
   __shared__ int grp_tab[2 * kMaxGroups];
   constexpr bool kRecLds = kR == 0;
+  constexpr bool kShared = kMode == 2;
   const int chain = chain_order[blockIdx.x];
   const int lane = int(threadIdx.x);
   const ChainParams* __restrict__ pp = params + chain;
@@ -102,7 +105,7 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
   const int occ_bytes = kPeriodic ? m.pdims[0] * m.pdims[1] * m.pdims[2] : N;
   const LdsLayout L = lds_layout(occ_bytes, N, kRecLds ? m.n_rec : 0, kGtabLds ? m.gtab_size : 0);
   const int nb = wk->n_bins;
-  const bool adaptive = !kShared && wk->adaptive != 0;
+  const bool adaptive = kMode == 1 && wk->adaptive != 0;
   const WLLayout WL = wl_layout(L.total, kShared ? 0 : nb, adaptive);
   uint8_t* occ = reinterpret_cast<uint8_t*>(lds + L.occ);
   uint16_t* trk = reinterpret_cast<uint16_t*>(lds + L.trk);
@@ -521,8 +524,6 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
       const int n_in = int(check_every - blk0 < 64 ? check_every - blk0 : 64);
       for (int l = 0; l < n_in; ++l) {
         const Move mv = eval_lane(d, l);
-        ++steps;
-        ++since_flat;
         const double raw = wl_raw_bin(mv.e_new, e_min, span, nbd);
         if (adaptive) bin_change += fabs(raw - double(cur));  // only set_overlap reads it
         const int bin = __builtin_amdgcn_readfirstlane(wl_bin(raw, nbd, nb));
@@ -549,7 +550,7 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
             commit(mv);
             cur = bin;
             first = 0;
-            cur_e = mv.e_new;
+            if (adaptive) cur_e = mv.e_new;  // the sub-bin energy
             ++n_acc;
             const int64_t q = log_base + done + l;
             if (acc_log != nullptr && lane == 0 && q < 32 * log_words) atomicOr(acc_log + (q >> 5), 1u << (q & 31));
@@ -561,6 +562,8 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
       done += n_in;
       n_steps += uint64_t(n_in);
     }
+    steps += check_every;  // a check always makes all its steps
+    since_flat += check_every;
     __syncthreads();
     if constexpr (kShared) continue;  // wl_shared_check_kernel tests the shared histogram
     bool flat;

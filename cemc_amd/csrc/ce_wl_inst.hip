@@ -1,7 +1,8 @@
 // Instantiations of the Wang-Landau kernel for one lattice path, chosen by
 // CEMC_WL_PERIODIC (0 or 1) at compile time.  Exposes
-// wl_fn_p<P>(kr, glds, kbm, nm, shared), the kernel's address for
-// hipLaunchKernel (shared: walkers on one shared histogram),
+// wl_fn_p<P>(kr, glds, kbm, nm, mode), the kernel's address for
+// hipLaunchKernel (mode 0: own histogram, 1: with adaptive windows, 2:
+// walkers on one shared histogram),
 // or nullptr for a variant that is not built: records in VGPRs (kR = 2, 4,
 // 8; up to 4 species; nm = 2 or 3 gathered members) or in LDS (kR = 0; up
 // to 8 species), energy tables in LDS or HBM.
@@ -17,29 +18,29 @@ namespace {
 
 constexpr bool P = CEMC_WL_PERIODIC != 0;
 
-template <int KBM, bool G, bool SH>
+template <int KBM, bool G, int MODE>
 const void* wl_by_kr(int kr, int nm) {
   if (nm == 2) {
-    if (kr == 2) return reinterpret_cast<const void*>(&wl_kernel<P, 2, G, KBM, 2, SH>);
-    if (kr == 4) return reinterpret_cast<const void*>(&wl_kernel<P, 4, G, KBM, 2, SH>);
-    if (kr == 8) return reinterpret_cast<const void*>(&wl_kernel<P, 8, G, KBM, 2, SH>);
+    if (kr == 2) return reinterpret_cast<const void*>(&wl_kernel<P, 2, G, KBM, 2, MODE>);
+    if (kr == 4) return reinterpret_cast<const void*>(&wl_kernel<P, 4, G, KBM, 2, MODE>);
+    if (kr == 8) return reinterpret_cast<const void*>(&wl_kernel<P, 8, G, KBM, 2, MODE>);
     return nullptr;
   }
   if (nm != 3) return nullptr;
-  if (kr == 2) return reinterpret_cast<const void*>(&wl_kernel<P, 2, G, KBM, 3, SH>);
-  if (kr == 4) return reinterpret_cast<const void*>(&wl_kernel<P, 4, G, KBM, 3, SH>);
-  if (kr == 8) return reinterpret_cast<const void*>(&wl_kernel<P, 8, G, KBM, 3, SH>);
-  if (kr == 0) return reinterpret_cast<const void*>(&wl_kernel<P, 0, G, KBM, 3, SH>);
+  if (kr == 2) return reinterpret_cast<const void*>(&wl_kernel<P, 2, G, KBM, 3, MODE>);
+  if (kr == 4) return reinterpret_cast<const void*>(&wl_kernel<P, 4, G, KBM, 3, MODE>);
+  if (kr == 8) return reinterpret_cast<const void*>(&wl_kernel<P, 8, G, KBM, 3, MODE>);
+  if (kr == 0) return reinterpret_cast<const void*>(&wl_kernel<P, 0, G, KBM, 3, MODE>);
   return nullptr;
 }
 
-template <bool SH>
+template <int MODE>
 const void* wl_pick(int kr, bool glds, int kbm, int nm) {
-  if (kbm == 1) return glds ? wl_by_kr<1, true, SH>(kr, nm) : wl_by_kr<1, false, SH>(kr, nm);
-  if (kbm == 2) return glds ? wl_by_kr<2, true, SH>(kr, nm) : wl_by_kr<2, false, SH>(kr, nm);
-  if (kbm == 3) return glds ? wl_by_kr<3, true, SH>(kr, nm) : wl_by_kr<3, false, SH>(kr, nm);
+  if (kbm == 1) return glds ? wl_by_kr<1, true, MODE>(kr, nm) : wl_by_kr<1, false, MODE>(kr, nm);
+  if (kbm == 2) return glds ? wl_by_kr<2, true, MODE>(kr, nm) : wl_by_kr<2, false, MODE>(kr, nm);
+  if (kbm == 3) return glds ? wl_by_kr<3, true, MODE>(kr, nm) : wl_by_kr<3, false, MODE>(kr, nm);
   if (kbm == 7 && kr == 0 && nm == 3)
-    return reinterpret_cast<const void*>(glds ? &wl_kernel<P, 0, true, 7, 3, SH> : &wl_kernel<P, 0, false, 7, 3, SH>);
+    return reinterpret_cast<const void*>(glds ? &wl_kernel<P, 0, true, 7, 3, MODE> : &wl_kernel<P, 0, false, 7, 3, MODE>);
   return nullptr;
 }
 
@@ -48,8 +49,11 @@ const void* wl_pick(int kr, bool glds, int kbm, int nm) {
 #define CEMC_WL_FN_NAME2(P_) wl_fn_p##P_
 #define CEMC_WL_FN_NAME(P_) CEMC_WL_FN_NAME2(P_)
 
-const void* CEMC_WL_FN_NAME(CEMC_WL_PERIODIC)(int kr, bool glds, int kbm, int nm, bool shared) {
-  return shared ? wl_pick<true>(kr, glds, kbm, nm) : wl_pick<false>(kr, glds, kbm, nm);
+const void* CEMC_WL_FN_NAME(CEMC_WL_PERIODIC)(int kr, bool glds, int kbm, int nm, int mode) {
+  if (mode == 0) return wl_pick<0>(kr, glds, kbm, nm);
+  if (mode == 1) return wl_pick<1>(kr, glds, kbm, nm);
+  if (mode == 2) return wl_pick<2>(kr, glds, kbm, nm);
+  return nullptr;
 }
 
 }  // namespace cemc
