@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # dispatch, each scaled by what the counter reads for a copy kernel of known
 # size (x2.0 and x1.0).  Measured with rocprofv3 --pmc; summary in
 # profiles/sweep_kernel_sgc_binary_fcc10_1024chains.txt
-TRAFFIC_DEFAULT_LAUNCH = 3665889
+TRAFFIC_DEFAULT_LAUNCH = 3686400
 TRAFFIC_SHAPE = (1, 8, 1)
 
 
