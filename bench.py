@@ -238,17 +238,25 @@ def tempering(n_rep=128, cycles=2000, L=32, seed=9, lds_peak=None, cpu=True):
     autotune.apply(batch, tune_w)
     pt = ReplicaExchange(batch, seed=seed, uid=1 << 40)
     pt.run(2, st.n_sites)  # warm-up and graph build
+    # timed in pieces of at most 200 cycles (every cycle is the same sweep and
+    # exchange), with a progress line after each piece
     t0 = time.perf_counter()
-    kernel_ms = pt.run(cycles, st.n_sites)
+    kernel_ms, done = 0.0, 0
+    while done < cycles:
+        n = min(200, cycles - done)
+        kernel_ms += pt.run(n, st.n_sites)
+        done += n
+        if cycles > 200:
+            log("tempering: %d of %d cycles, %.1f s" % (done, cycles, time.perf_counter() - t0))
     wall = time.perf_counter() - t0
     state = pt.state()
     total = cycles * n_rep * st.n_sites
     acc = state["n_accept"].sum() / max(state["n_try"].sum(), 1)
     bytes_per_move = 10.0 * model.gathers_per_flip()
     out = {"workload": "quaternary FCC 32^3 (32768 sites), <=4-body to the 3rd shell (%d CFs, %d sub-clusters per "
-                       "site), %d replicas 200-2000 K, exchange every sweep, %d of the workload's 2000 sweeps "
-                       "(a rate per cycle: every cycle is the same sweep + exchange)"
-                       % (model.n_cf, model.n_subclusters, n_rep, cycles),
+                       "site), %d replicas 200-2000 K, exchange every sweep, %d of the workload's 2000 sweeps%s"
+                       % (model.n_cf, model.n_subclusters, n_rep, cycles,
+                          "" if cycles >= 2000 else " (a rate per cycle: every cycle is the same sweep + exchange)"),
            "cycles": cycles, "value": total / wall, "unit": "moves/s", "moves": total, "kernel_ms": kernel_ms,
            "wall_s": wall, "bytes_per_move": bytes_per_move, "launch_shape": list(batch.launch_shape(0)),
            "shape": autotune.shape_name(tune_w),

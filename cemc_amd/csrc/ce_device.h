@@ -91,6 +91,10 @@ struct ModelDev {
   const SubRec* recs;
   const FatRec* frecs;        // same order as recs
   const double* gtab;
+  // three species (two slices read per record): the same tables with the two
+  // slices of each code side by side, [block][code][d], so one 16-byte read
+  // fetches both; the other models point this at gtab
+  const double* gtab_il;
   const int32_t* trans;       // n_sites * K
   const uint32_t* col_cells;  // K packed cell offsets (x | y<<8 | z<<16)
   const int8_t* site_group;   // n_sites translation group (-1: none, background only)

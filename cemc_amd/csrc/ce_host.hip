@@ -129,6 +129,8 @@ struct cemc_model {
   SubRec* d_recs = nullptr;
   FatRec* d_frecs = nullptr;
   double* d_gtab = nullptr;
+  std::vector<double> gtab_il;  // three species: gtab with the two read slices interleaved
+  double* d_gtab_il = nullptr;
   int64_t* d_gtab_fx = nullptr;
   uint32_t* d_duals = nullptr;
   int32_t* d_trans = nullptr;
@@ -150,6 +152,7 @@ struct cemc_model {
   }
 
   void build_gtab(const std::vector<double>& J);
+  void build_gtab_il();
   void build_gtab_fx();
   void build_duals();
   int code_bits() const;    // bits per species in an energy-table code
@@ -163,6 +166,7 @@ cemc_model::~cemc_model() {
   dev_free(d_recs);
   dev_free(d_frecs);
   dev_free(d_gtab);
+  dev_free(d_gtab_il);
   dev_free(d_gtab_fx);
   dev_free(d_duals);
   dev_free(d_trans);
@@ -253,6 +257,20 @@ void cemc_model::build_gtab(const std::vector<double>& J) {
       }
     }
   }
+}
+
+// For three species the sweep and Wang-Landau kernels read slices 0 and 1 of
+// every record's block together: the block's entry (d, code) moves from
+// d * 64 + code to 2 * code + d, so the two values of a code are adjacent
+// and 16-byte aligned (blocks start at multiples of 3 x 64 doubles).
+void cemc_model::build_gtab_il() {
+  gtab_il.clear();
+  if (read_slices() != 2) return;
+  const int Pc = 1 << (3 * code_bits()), block = deco_slices() * Pc;
+  gtab_il.assign(gtab.size(), 0.0);
+  for (size_t b = 0; b + size_t(block) <= gtab.size(); b += size_t(block))
+    for (int code = 0; code < Pc; ++code)
+      for (int d = 0; d < 2; ++d) gtab_il[b + 2 * code + d] = gtab[b + size_t(d) * Pc + code];
 }
 
 // Integer copy of the energy tables for the local-field path: a site's field
@@ -395,6 +413,7 @@ void cemc_model::refresh_view() {
   v.recs = d_recs;
   v.frecs = d_frecs;
   v.gtab = d_gtab;
+  v.gtab_il = d_gtab_il ? d_gtab_il : d_gtab;
   v.trans = d_trans;
   v.col_cells = d_cells;
   v.site_group = d_site_group;
@@ -644,6 +663,8 @@ static void build_model(cemc_model* m, const cemc_model_desc* d) {
   m->d_recs = upload(m->recs);
   m->d_frecs = upload(m->frecs);
   m->d_gtab = upload(m->gtab);
+  m->build_gtab_il();
+  if (!m->gtab_il.empty()) m->d_gtab_il = upload(m->gtab_il);
   m->build_duals();
   m->build_gtab_fx();
   m->d_gtab_fx = upload(m->gtab_fx);
@@ -1750,6 +1771,10 @@ int cemc_model_set_ecis(cemc_model* m, const double* eci) {
     m->build_gtab(m->eci);
     HIP_CHECK(hipMemcpy(m->d_gtab, m->gtab.data(), sizeof(double) * std::max<size_t>(m->gtab.size(), 1),
                         hipMemcpyHostToDevice));
+    m->build_gtab_il();
+    if (!m->gtab_il.empty())
+      HIP_CHECK(hipMemcpy(m->d_gtab_il, m->gtab_il.data(), sizeof(double) * m->gtab_il.size(),
+                          hipMemcpyHostToDevice));
     m->build_gtab_fx();
     HIP_CHECK(hipMemcpy(m->d_gtab_fx, m->gtab_fx.data(), sizeof(int64_t) * m->gtab_fx.size(),
                         hipMemcpyHostToDevice));
