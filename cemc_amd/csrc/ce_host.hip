@@ -1207,6 +1207,17 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
   for (int c = 0; c < n_chains; ++c) n_launch += included(c) ? 1 : 0;
   int batch_waves = 1;
   while (batch_waves < 8 && int64_t(n_launch) * batch_waves * 2 <= int64_t(4) * n_cu) batch_waves *= 2;
+  // pools: a chain with far more moves left than the launch's median (a
+  // large job among many small ones) gets up to four waves even when the
+  // device is crowded, so the longest jobs do not set the tail alone
+  int64_t long_moves = -1;
+  if (b->dynamic && nm != nullptr && n_launch > 0) {
+    std::vector<int64_t> left;
+    for (int c = 0; c < n_chains; ++c)
+      if (included(c)) left.push_back((*nm)[c]);
+    std::nth_element(left.begin(), left.begin() + left.size() / 2, left.end());
+    long_moves = std::max<int64_t>(8 * left[left.size() / 2], int64_t(1) << 20);
+  }
   std::vector<Key> keys(n_chains);
   std::vector<bool> done(n_chains, false);
   for (int c = 0; c < n_chains; ++c) {
@@ -1227,7 +1238,8 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
     const uint32_t with_g = sweep_lds_bytes(occ_bytes, trk_sites, 0, int(m->gtab.size()));
     const uint32_t no_g = sweep_lds_bytes(occ_bytes, trk_sites, 0, 0);
     int auto_waves = 1;
-    while (auto_waves < batch_waves && cnt > 128 * auto_waves) auto_waves *= 2;
+    const int chain_waves = long_moves > 0 && (*nm)[c] >= long_moves ? std::max(batch_waves, 4) : batch_waves;
+    while (auto_waves < chain_waves && cnt > 128 * auto_waves) auto_waves *= 2;
     auto pick_kr = [&](int waves) {
       if (single && k.kbm <= 3 && no_g <= limit)
         for (int kr : {2, 4, 8})
