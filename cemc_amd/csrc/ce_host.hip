@@ -2513,7 +2513,8 @@ int cemc_batch_wl_set(cemc_batch* b, int32_t chain, const cemc_wl_desc* desc, co
     require(b && desc, "NULL argument");
     require(!b->dynamic, "Wang-Landau walkers run on fixed batches, not pools");
     require(chain >= 0 && chain < b->n_chains, "chain index out of range");
-    require(b->params[chain].kind == CEMC_MOVE_CANONICAL, "Wang-Landau walkers make canonical swaps");
+    require(b->params[chain].kind == CEMC_MOVE_CANONICAL || b->params[chain].kind == CEMC_MOVE_SGC,
+            "unknown move kind");
     require(b->hooks.empty() || !b->hooks[chain].on, "constraints and bias do not apply to Wang-Landau walkers");
     require(desc->n_bins > 0, "the histogram needs at least one bin");
     require(std::isfinite(desc->e_min) && std::isfinite(desc->e_max) && desc->e_max > desc->e_min,

@@ -93,6 +93,7 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
                                                 int64_t log_base) {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ double dphi_tab[kMaxSpecies * kMaxSpecies * kMaxBasis];  // [(old*S+new)*B + d]
+  __shared__ double dsingle_tab[kMaxSpecies * kMaxSpecies];          // SGC: singlet energy change per (old,new)
   __shared__ int grp_tab[2 * kMaxGroups];
   constexpr bool kRecLds = kR == 0;
   constexpr bool kShared = kMode == 2;
@@ -173,7 +174,12 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
   const uint8_t* gocc = reinterpret_cast<const uint8_t*>(occ_pool + pp->occ_off);
   load_lattice(gocc);
   uint16_t* gtrk = trk_pool + pp->trk_off;
-  for (int i = lane; i < N; i += 64) trk[i] = gtrk[i];
+  // semi-grand-canonical walkers flip one site at a time and have no tracker
+  const bool sgc = pp->kind == 1;
+  const bool has_bkg = m.has_background != 0;
+  const int n_pick = has_bkg ? m.n_movable : N;
+  if (!sgc)
+    for (int i = lane; i < N; i += 64) trk[i] = gtrk[i];
   if constexpr (kRecLds)
     for (int i = lane; i < m.n_rec; i += 64) recs_l[i] = m.frecs[i];
   if constexpr (kGtabLds)
@@ -182,7 +188,13 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
   if (lane < 2 * kMaxGroups) grp_tab[lane] = lane < kMaxGroups ? m.grp_rec_off[lane] : m.grp_rec_cnt[lane - kMaxGroups];
   for (int i = lane; i < c.S * c.S; i += 64) {
     const int o = i / c.S, nw = i - o * c.S;
-    for (int d = 0; d < c.B; ++d) dphi_tab[i * c.B + d] = m.basis[d * c.S + nw] - m.basis[d * c.S + o];
+    double ds = 0.0;
+    for (int d = 0; d < c.B; ++d) {
+      const double dp = m.basis[d * c.S + nw] - m.basis[d * c.S + o];
+      dphi_tab[i * c.B + d] = dp;
+      ds += pp->j1[d] * dp;
+    }
+    dsingle_tab[i] = ds;
   }
   if constexpr (!kShared) {
     for (int i = lane; i < nb; i += 64) {
@@ -341,19 +353,57 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
     mv.e_new = energy + dE;
     return mv;
   };
+  // A semi-grand-canonical flip of `site` to the species `off` + 1 after its
+  // own (the SGC move of the sweep kernel); the singlet ECIs carry the
+  // chemical potentials.
+  auto evaluate_flip = [&](int site, uint32_t off) {
+    Move mv;
+    const Ref ri = ref_of<kPeriodic>(c, site);
+    mv.si = mv.sj = site;
+    mv.ta = mv.tb = -1;
+    mv.old_i = uint32_t(__builtin_amdgcn_readfirstlane(int(occ[ri.p])));
+    mv.new_i = mv.old_i + 1u + off;
+    mv.new_i -= mv.new_i >= uint32_t(c.S) ? uint32_t(c.S) : 0u;
+    double dphi[kBM];
+    double de_single;
+    if constexpr (kBM == 1) {
+      dphi[0] = mv.old_i == 0 ? dphi01 : dphi10;
+      de_single = dsingle_tab[mv.old_i == 0 ? 1 : c.S];
+    } else {
+      const int pair_i = int(mv.old_i) * c.S + int(mv.new_i);
+#pragma unroll
+      for (int d = 0; d < kBM; ++d) dphi[d] = d < c.B ? dphi_tab[pair_i * c.B + d] : 0.0;
+      de_single = dsingle_tab[pair_i];
+    }
+    double part;
+    if constexpr (kR > 0) {
+      int gi_i[kR];
+      table_index<kPeriodic, kR, kBM, false, kNM, kBM == 2>(c, rr, ri, occ, 0u, 0u, gi_i);
+      part = table_sum<kR, kBM, kBM == 2>(gi_i, gtab, dphi);
+    } else {
+      const int gi = multi_group ? int(m.site_group[site]) : 0;
+      part = lds_flip_de<kPeriodic, kBM, false, kBM == 2>(c, recs_l, grp_tab[gi], grp_tab[kMaxGroups + gi], lane, 64,
+                                                          ri, occ, gtab, dphi, 0u, 0u);
+    }
+    const double sum = wave_sum_f64(part);
+    const double dE = kBM == 1 ? fma(dphi[0], sum, de_single) : sum + de_single;
+    mv.e_new = energy + dE;
+    return mv;
+  };
   auto commit = [&](const Move& mv) {
+    const int n_sites_changed = sgc ? 1 : 2;
     if constexpr (kPeriodic) {
       // lanes 0-7 write the images of site i, lanes 8-15 those of site j
       const bool second = lane >= 8;
       const Ref r = ref_of<kPeriodic>(c, second ? mv.sj : mv.si);
       const uint32_t img = image_word<true>(c, r);
       const int idx = image_of(c, img, lane & 7);
-      if (idx >= 0 && lane < 16) occ[idx] = uint8_t(second ? mv.old_i : mv.new_i);
+      if (idx >= 0 && lane < 8 * n_sites_changed) occ[idx] = uint8_t(second ? mv.old_i : mv.new_i);
     } else {
       if (lane == 0) occ[mv.si] = uint8_t(mv.new_i);
-      if (lane == 1) occ[mv.sj] = uint8_t(mv.old_i);
+      if (lane == 1 && !sgc) occ[mv.sj] = uint8_t(mv.old_i);
     }
-    if (lane == 0) {
+    if (lane == 0 && !sgc) {
       trk[mv.ta] = uint16_t(mv.sj);
       trk[mv.tb] = uint16_t(mv.si);
     }
@@ -370,6 +420,15 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
   auto draw = [&]() {
     Draw d;
     d.blk = philox4x64_10(t0 + uint64_t(done) + uint64_t(lane), 0, 0, 0, seed, uid);
+    if (sgc) {
+      // the site (a movable one) and the offset of its new species
+      const int pick = int(bounded64(d.blk.w1, uint64_t(n_pick)));
+      d.v_ta = has_bkg ? m.movable[pick] : pick;
+      d.v_tb = 0;
+      d.v_old = 0u;
+      d.v_new = bounded32(uint32_t(d.blk.w0), uint32_t(c.S - 1));
+      return d;
+    }
     const uint32_t ia = bounded32(uint32_t(d.blk.w0 >> 32), np);
     uint32_t ib = ia + 1u + bounded32(uint32_t(d.blk.w0), np - 1u);
     ib -= ib >= np ? np : 0u;
@@ -380,6 +439,7 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
     return d;
   };
   auto eval_lane = [&](const Draw& d, int l) {
+    if (sgc) return evaluate_flip(__builtin_amdgcn_readlane(d.v_ta, l), __builtin_amdgcn_readlane(d.v_new, l));
     return evaluate(__builtin_amdgcn_readlane(d.v_old, l), __builtin_amdgcn_readlane(d.v_new, l),
                     __builtin_amdgcn_readlane(d.v_ta, l), __builtin_amdgcn_readlane(d.v_tb, l));
   };
@@ -618,7 +678,8 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
   __syncthreads();
   uint8_t* wocc = reinterpret_cast<uint8_t*>(occ_pool + pp->occ_off);
   for (int i = lane; i < N; i += 64) wocc[i] = occ[ref_of<kPeriodic>(c, i).p];
-  for (int i = lane; i < N; i += 64) gtrk[i] = trk[i];
+  if (!sgc)
+    for (int i = lane; i < N; i += 64) gtrk[i] = trk[i];
   if constexpr (!kShared) {
     for (int i = lane; i < nb; i += 64) {
       logdos_pool[boff + i] = lg[i];

@@ -3,8 +3,15 @@ sampler (cpp/src/wang_landau_sampler.cpp, driven from
 cemc/wanglandau/wang_landau_sampler.py:450-469 ``run_fast_sampler``) for an
 atoms object whose calculator is ``cemc_amd.calculator.CE``.
 
-Each walker is one canonical chain of a batch.  A trial move is the
-reference's canonical swap; its energy is binned into ``Nbins`` bins over
+Each walker is one chain of a batch.  A trial move is the reference's
+canonical swap, or with ``ensemble="semi-grand-canonical"`` a flip of one
+site to another species, whose energy carries the chemical-potential term
+E - N sum_d mu_d c1_d (the ensemble of the reference's Python loop,
+wang_landau_sampler.py:268-311; the chemical potentials are given per
+singlet as for SGCMonteCarlo and folded into the singlet ECIs; for two
+species the reference's per-element potentials map onto the singlet's and
+the two energies differ by a constant).  The
+energy is binned into ``Nbins`` bins over
 [Emin, Emax) and the swap is kept with probability
 min(1, g(current) / g(proposed)), where ln g is raised by f at every visit
 (``WangLandauSampler::step``, :232-317).  After every
@@ -41,8 +48,10 @@ split over the walkers, and walker 0 carries the result.  Such runs do not
 repeat exactly when f > 0, since a walker reads ln g as the others' visits
 land.
 
-Not carried over: the Python ``run`` loop with its SGC moves (it reads and
-writes the reference's database) and the database records.
+Not carried over: the Python ``run`` loop itself (it reads and writes the
+reference's database, moves one site at a time from Python and shifts the
+energy by the chemical potentials of the two changed elements only) and the
+database records.
 """
 from __future__ import annotations
 
@@ -50,7 +59,7 @@ import os
 
 import numpy as np
 
-from .batch import CANONICAL, ChainBatch
+from .batch import CANONICAL, SGC, ChainBatch
 
 
 class Histogram:
@@ -92,18 +101,24 @@ class WangLandau:
         a flat histogram in the native sampler) or "inverse_time"
     :param bool shared_histogram: the walkers visit one histogram (their
         steps per check add up to check_convergence_every)
+    :param str ensemble: "canonical" (swaps) or "semi-grand-canonical"
+        (flips; the binned energy is E - N sum_d mu_d c1_d, with chem_pot
+        given per singlet as for SGCMonteCarlo, e.g. {"c1_0": -0.1})
     """
 
     def __init__(self, atoms, Nbins=100, initial_f=2.71, flatness_criteria=0.8, fmin=1e-6, Emin=0.0, Emax=1.0,
                  check_convergence_every=1000, n_walkers=1, seed=None, uid=0, scheme="fixed_f",
-                 ensemble="canonical", shared_histogram=False):
+                 ensemble="canonical", shared_histogram=False, chem_pot=None):
         calc = atoms.get_calculator() if hasattr(atoms, "get_calculator") else getattr(atoms, "calc", None)
         if calc is None or not hasattr(calc, "updater"):
             raise ValueError("the atoms object needs a cemc_amd CE calculator attached")
         if scheme not in ("fixed_f", "inverse_time", "square_root_reduction"):
             raise ValueError("scheme has to be one of ['fixed_f', 'inverse_time', 'square_root_reduction']")
-        if ensemble != "canonical":
-            raise NotImplementedError("the GPU walkers make canonical swaps")
+        if ensemble not in ("canonical", "semi-grand-canonical"):
+            raise ValueError("Ensemble has to one of ['canonical', 'semi-grand-canonical']")
+        sgc = ensemble == "semi-grand-canonical"
+        if sgc and not chem_pot:
+            raise ValueError("the semi-grand-canonical ensemble needs chemical potentials")
         if not Emax > Emin:
             raise ValueError("Emax must exceed Emin")
         if int(Nbins) < 1 or int(n_walkers) < 1 or int(check_convergence_every) < 1:
@@ -123,10 +138,13 @@ class WangLandau:
         self.seed = int.from_bytes(os.urandom(8), "little") if seed is None else int(seed)
         self.uid = int(uid)
         occ = calc.updater.get_occupation()
-        chains = [dict(kind=CANONICAL, temperature=1000.0, occupation=occ, seed=self.seed, uid=self.uid + w)
-                  for w in range(self.n_walkers)]
+        self.ensemble = ensemble
+        self.chem_pot = dict(chem_pot) if sgc else None
+        chains = [dict(kind=SGC if sgc else CANONICAL, temperature=1000.0, occupation=occ, seed=self.seed,
+                       uid=self.uid + w, chem_pot=self.chem_pot) for w in range(self.n_walkers)]
         self._batch = ChainBatch([calc.model] * self.n_walkers, chains)
-        e0 = calc.get_energy()
+        # the walkers' energy: with the chemical-potential term in SGC
+        e0 = float(self._batch.energies()[0]) if sgc else calc.get_energy()
         b0 = Histogram(self.Nbins, self.Emin, self.Emax).get_bin(e0)
         self.current_bin = [b0 if 0 <= b0 < self.Nbins else 0] * self.n_walkers
         for w in range(self.n_walkers):

@@ -324,9 +324,13 @@ CEMC_API int cemc_batch_get_hooks_state(cemc_batch* batch, int32_t chain, int64_
                                         double* bias_energy);
 
 /* ----------------------------------------------------------- Wang-Landau
- * A Wang-Landau walker on a canonical chain of a (fixed) batch: the
- * reference's native sampler (cpp/src/wang_landau_sampler.cpp, bound by
- * cemc/cpp_ext/pywang_landau_sampler.pyx:4-30) with one histogram per walker
+ * A Wang-Landau walker on a chain of a (fixed) batch: the reference's
+ * native sampler (cpp/src/wang_landau_sampler.cpp, bound by
+ * cemc/cpp_ext/pywang_landau_sampler.pyx:4-30) on a canonical chain (its
+ * swaps), or on a semi-grand-canonical chain (its flips, the energy with
+ * the chemical potentials folded into the singlet ECIs: the ensemble of the
+ * reference's Python WangLandau with ensemble "semi-grand-canonical",
+ * cemc/wanglandau/wang_landau_sampler.py:268-311), with one histogram per walker
  * instead of one shared under a lock, so each chain is an independent
  * Wang-Landau run and their ln g can be averaged.  Energies are the
  * chain's total energy (eV). */

@@ -722,6 +722,19 @@ static int wl_propose(ceo_chain* c, const int* present, int n_present, const uin
   const uint64_t ctr[4] = {ctr0, 0, 0, 0};
   uint64_t r[4];
   ceo_philox4x64(ctr, key, r);
+  if (n_present < 0) {
+    /* semi-grand-canonical flip: present[] holds the -n_present movable sites */
+    const ceo_model* m = c->m;
+    const int site = present[bounded64(r[1], (uint64_t)(-n_present))];
+    const int old = c->occ[site];
+    const int nw = (int)((old + 1 + (int)bounded32((uint32_t)r[0], (uint32_t)(m->n_species - 1))) % m->n_species);
+    p->a = old;
+    p->b = nw;
+    p->sa = p->sb = -1;
+    p->sites[0] = p->sites[1] = site;
+    p->u = unit_double(r[3]);
+    return calculate_impl(c, 1, &site, &old, &nw, &p->e_new);
+  }
   uint32_t ia = bounded32((uint32_t)(r[0] >> 32), (uint32_t)n_present);
   uint32_t ib = (ia + 1 + bounded32((uint32_t)r[0], (uint32_t)(n_present - 1))) % (uint32_t)n_present;
   p->a = present[ia];
@@ -736,6 +749,7 @@ static int wl_propose(ceo_chain* c, const int* present, int n_present, const uin
 }
 
 static void wl_keep(ceo_chain* c, const wl_prop* p) {
+  if (p->sa < 0) return; /* a flip: no species tracker */
   c->trk[c->trk_off[p->a] + p->sa] = p->sites[1];
   c->trk[c->trk_off[p->b] + p->sb] = p->sites[0];
 }
@@ -871,19 +885,46 @@ static void wl_window_reset(ceo_chain* c, ceo_wl* w) {
 
 int ceo_wl_run(ceo_chain* c, ceo_wl* w, uint64_t seed, uint64_t chain_uid, uint64_t move_begin,
                int64_t n_checks, int64_t check_every, uint8_t* accept_bits, int64_t bits_cap, int64_t* n_used) {
-  int present[64];
-  const int n_present = canonical_setup(c, present);
+  int present_buf[64];
+  int* present = present_buf;
+  int n_present;
+  int* movable = NULL;
   if (n_used) *n_used = 0;
-  if (n_present < 2) return fail(CEO_EINVAL, "canonical moves need at least two species present");
-  if (w->n_bins < 1 || !(w->e_max > w->e_min)) return fail(CEO_EINVAL, "invalid Wang-Landau histogram");
-  if (w->cur_bin < 0 || w->cur_bin >= w->n_bins) return fail(CEO_EINVAL, "current bin outside the histogram");
-  if (w->adaptive && (w->upper < 1 || w->upper > w->n_bins || w->n_overlap < 0 || w->n_overlap > w->n_bins))
-    return fail(CEO_EINVAL, "invalid adaptive window");
+  if (w->kind == 1) {
+    /* the movable sites, as ceo_run's SGC moves pick them */
+    const ceo_model* m = c->m;
+    movable = (int*)malloc(sizeof(int) * (size_t)(m->n_sites > 0 ? m->n_sites : 1));
+    int n_movable = 0;
+    for (int i = 0; i < m->n_sites; ++i)
+      if (!m->background[i]) movable[n_movable++] = i;
+    present = movable;
+    n_present = -n_movable;
+    c->trk_valid = 0; /* flips change the composition */
+    if (n_movable < 1) {
+      free(movable);
+      return fail(CEO_EINVAL, "no movable site");
+    }
+  } else {
+    n_present = canonical_setup(c, present);
+    if (n_present < 2) return fail(CEO_EINVAL, "canonical moves need at least two species present");
+  }
+  const char* bad = NULL;
+  if (w->n_bins < 1 || !(w->e_max > w->e_min)) bad = "invalid Wang-Landau histogram";
+  else if (w->cur_bin < 0 || w->cur_bin >= w->n_bins) bad = "current bin outside the histogram";
+  else if (w->adaptive && (w->upper < 1 || w->upper > w->n_bins || w->n_overlap < 0 || w->n_overlap > w->n_bins))
+    bad = "invalid adaptive window";
+  if (bad) {
+    free(movable);
+    return fail(CEO_EINVAL, bad);
+  }
   if (!w->adaptive) w->upper = w->n_bins;
   const uint64_t key[2] = {seed, chain_uid};
   int64_t t = 0;
   int rc = CEO_OK;
-  if (w->converged) return CEO_OK; /* a converged walker makes no more moves */
+  if (w->converged) { /* a converged walker makes no more moves */
+    free(movable);
+    return CEO_OK;
+  }
   for (int64_t chk = 0; chk < n_checks && rc == CEO_OK; ++chk) {
     /* WangLandauSampler::step, wang_landau_sampler.cpp:232-317 */
     for (int64_t s = 0; s < check_every; ++s) {
@@ -952,5 +993,6 @@ int ceo_wl_run(ceo_chain* c, ceo_wl* w, uint64_t seed, uint64_t chain_uid, uint6
     }
   }
   if (n_used) *n_used = t;
+  free(movable);
   return rc;
 }

@@ -130,7 +130,13 @@ int ceo_run_many(ceo_chain** chains, int n_chains, int kind, uint64_t seed,
  *                (10 per bin) are kept in this mode only, as in the
  *                reference (Histogram::init_sub_bins is called only there).
  * The scheme fields are reset by the caller when a run starts (the
- * reference builds a new sampler per run_fast_sampler call). */
+ * reference builds a new sampler per run_fast_sampler call).
+ *
+ * kind 1 proposes the semi-grand-canonical flip of ceo_run's SGC moves
+ * (a movable site, one of the other species), so the walker samples the
+ * density of states of the energy with the chemical-potential term, the
+ * ensemble of the reference's Python WangLandau with ensemble
+ * "semi-grand-canonical" (cemc/wanglandau/wang_landau_sampler.py:268-311). */
 typedef struct ceo_wl {
   int32_t n_bins;
   int32_t cur_bin;   /* bin of the walker's state (ignored while is_first) */
@@ -163,6 +169,9 @@ typedef struct ceo_wl {
   int32_t* snap_trk;  /* (n_bins + 1) x n_sites: species index trackers */
   int32_t* snap_bin;  /* n_bins + 1 */
   uint32_t* sub_hist; /* n_bins x 10 */
+  int32_t kind;       /* 0: canonical swaps; 1: semi-grand-canonical flips (energies with
+                         the chemical potentials folded into the singlet ECIs) */
+  int32_t pad;
 } ceo_wl;
 
 /* n_checks rounds of check_every steps, each round followed by the

@@ -102,18 +102,20 @@ class WLState(ctypes.Structure):
                 ("edge_vals", ctypes.POINTER(ctypes.c_double)), ("saved", ctypes.POINTER(ctypes.c_uint8)),
                 ("snap_occ", ctypes.POINTER(ctypes.c_int8)), ("snap_cf", ctypes.POINTER(ctypes.c_double)),
                 ("snap_trk", ctypes.POINTER(ctypes.c_int32)), ("snap_bin", ctypes.POINTER(ctypes.c_int32)),
-                ("sub_hist", ctypes.POINTER(ctypes.c_uint32))]
+                ("sub_hist", ctypes.POINTER(ctypes.c_uint32)), ("kind", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class OracleWL:
     """A Wang-Landau walker of the oracle: histogram arrays owned here.
 
     inverse_time, adaptive and min_width choose the schemes of ce_oracle.h;
-    n_sites and n_cf size the state records of adaptive windows."""
+    n_sites and n_cf size the state records of adaptive windows; kind 1
+    makes semi-grand-canonical flips (the updater's ECIs carry the chemical
+    potentials)."""
 
     def __init__(self, n_bins, e_min, e_max, f, f_min=1e-6, flatness=0.8, cur_bin=0, is_first=True,
                  logdos=None, hist=None, known=None, inverse_time=False, adaptive=False, min_width=10,
-                 n_sites=0, n_cf=0):
+                 n_sites=0, n_cf=0, kind=0):
         nb = int(n_bins)
         self.logdos = np.zeros(nb) if logdos is None else np.array(logdos, dtype=np.float64)
         self.hist = np.zeros(nb, dtype=np.int32) if hist is None else np.array(hist, dtype=np.int32)
@@ -141,7 +143,7 @@ class OracleWL:
                              _dp(self.edge_vals), self.saved.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                              self.snap_occ.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)), _dp(self.snap_cf),
                              self.snap_trk.ctypes.data_as(i32), self.snap_bin.ctypes.data_as(i32),
-                             self.sub_hist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+                             self.sub_hist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), int(kind), 0)
 
     def sub_bins(self):
         """Sub-bin visit counts, (n_bins, 10)."""

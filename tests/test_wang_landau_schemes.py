@@ -559,3 +559,147 @@ def test_gpu_scheme_walkers_on_crystals(gpu, crystal):
                                                                                   st.inv_active)
     assert np.array_equal(sch["sub_bins"].ravel(), wl.sub_hist)
     assert np.array_equal(batch.occupation(0), ora.get_occupation())
+
+
+# -------------------------------------------- semi-grand-canonical walkers
+NB_S, EMIN_S = 38, -0.09275  # 2x2x3 cell, all 4096 states: no level within 1.7 % of a bin edge
+EMAX_S = EMIN_S + NB_S * 0.005
+
+
+@pytest.fixture(scope="module")
+def sgc_cell():
+    """2x2x3 binary cell with every composition (4096 states); the singlet
+    ECI carries a chemical potential of 0.008 eV."""
+    s = FCCSettings((2, 2, 3), ["A", "B"], {2: 2, 3: 1})
+    names = s.cf_names()
+    eci = {n: 0.0 for n in names}
+    c2 = [n for n in names if n.startswith("c2")]
+    c3 = [n for n in names if n.startswith("c3")]
+    eci[c2[0]], eci[c2[1]], eci[c3[0]] = 0.01, -0.004, 0.003
+    eci["c1_0"] = 0.003 - 0.008
+    N = s.n_sites
+    states = [np.array(c, dtype=np.int8) for c in itertools.product([0, 1], repeat=N)]
+    energies = np.array([N * sum(eci[n] * v for n, v in brute_force_cf(s, st, names).items()) for st in states])
+    x = (energies - EMIN_S) * NB_S / (EMAX_S - EMIN_S)
+    assert np.min(np.abs(x - np.round(x))) > 0.015
+    g = np.bincount(np.trunc(x).astype(int), minlength=NB_S).astype(float)
+    return dict(s=s, names=names, eci=eci, states=states, energies=energies, g=g, model=OracleModel(s, names))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_sgc_walker_converges_to_exact_dos(sgc_cell, seed):
+    """Flips over all compositions: ln g of the energy with the chemical
+    potential converges to the enumerated one; every occupied bin becomes
+    known."""
+    c = sgc_cell
+    occ = c["states"][100 + 37 * seed]
+    upd = OracleUpdater(c["model"], occ, brute_force_cf(c["s"], occ, c["names"]), c["eci"])
+    wl = OracleWL(NB_S, EMIN_S, EMAX_S, 1.0, f_min=1e-4, flatness=0.8, kind=1)
+    wl_run(upd, wl, seed, 1, 400, 2000)
+    g = c["g"]
+    known = wl.known != 0
+    assert wl.state.converged == 1 and np.array_equal(known, g > 0)
+    d = (wl.logdos - np.log(np.where(g > 0, g, 1.0)))[known]
+    # ln g spans ln(528) = 6.3
+    assert np.abs(d - d.mean()).max() < 0.35
+
+
+def test_sgc_walker_with_fixed_ln_g_samples_states_uniformly(sgc_cell):
+    """f = 0 and ln g = 0: flips accept everything in range, so visits
+    follow g (every state equally likely), and the composition changes."""
+    c = sgc_cell
+    occ = c["states"][77]
+    upd = OracleUpdater(c["model"], occ, brute_force_cf(c["s"], occ, c["names"]), c["eci"])
+    wl = OracleWL(NB_S, EMIN_S, EMAX_S, 0.0, f_min=-1.0, flatness=2.0, kind=1)
+    n = 100 * 2000
+    wl_run(upd, wl, 5, 2, 100, 2000)
+    p = c["g"] / c["g"].sum()
+    assert wl.hist.sum() == n and wl.state.n_outside == 0
+    assert 0.5 * np.abs(wl.hist / n - p).sum() < 0.03
+    assert upd.get_occupation().sum() != occ.sum() or wl.state.n_accept > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["binary_cell", "ternary_periodic", "ternary_adaptive"])
+def test_gpu_sgc_walkers_match_oracle(gpu, sgc_cell, case):
+    from cemc_amd.batch import SGC, ChainBatch
+    from cemc_amd.lattice import random_occupation, seeded_ecis
+    from cemc_amd.model import CEModel
+    if case == "binary_cell":
+        s, eci, occ = sgc_cell["s"], dict(sgc_cell["eci"]), sgc_cell["states"][100]
+        mu = {"c1_0": 0.0}
+        nb, lo, hi = NB_S, EMIN_S, EMAX_S
+        kw = dict(f_min=1e-3)
+    else:
+        s = FCCSettings(5, ["A", "B", "C"], {2: 3, 3: 2})
+        eci = seeded_ecis(s, 8, 0.01)
+        occ = random_occupation(s.n_sites, 3, 6)
+        mu = {"c1_0": 0.02, "c1_1": -0.01}
+        names = s.cf_names()
+        e_mu = dict(eci)
+        for k, v in mu.items():
+            e_mu[k] -= v
+        e0 = OracleUpdater(OracleModel(s, names), occ, brute_force_cf(s, occ, names), e_mu).get_energy()
+        nb, half = 50, 0.5
+        w = 2.0 * half / nb
+        lo = e0 - half + w / 3.0
+        hi = lo + 2.0 * half
+        kw = dict(f_min=1e-3, flatness=0.6, adaptive=case == "ternary_adaptive", min_width=3, inverse_time=True)
+    names = s.cf_names()
+    e_mu = dict(eci)
+    for k, v in mu.items():
+        e_mu[k] -= v
+    ora = OracleUpdater(OracleModel(s, names), occ, brute_force_cf(s, occ, names), e_mu)
+    wl = OracleWL(nb, lo, hi, 1.0, kind=1, n_sites=s.n_sites, n_cf=len(names), cur_bin=nb // 2,
+                  **{k: v for k, v in kw.items() if k != "f_min"}, f_min=kw["f_min"])
+    used = []
+    ref = wl_run(ora, wl, 4, 9, 30, 1000, counters=used)
+    batch = ChainBatch([CEModel(s, eci)], [dict(kind=SGC, temperature=1000.0, occupation=occ, seed=4, uid=9,
+                                                chem_pot=mu)])
+    assert abs(batch.energies()[0] - OracleUpdater(OracleModel(s, names), occ, brute_force_cf(s, occ, names),
+                                                   e_mu).get_energy()) < 1e-9
+    batch.wl_set(0, nb, lo, hi, 1.0, f_min=kw["f_min"], flatness=wl.state.flatness, current_bin=nb // 2)
+    if case != "binary_cell":
+        batch.wl_set_scheme(0, inverse_time=True, adaptive=kw["adaptive"], min_width=3)
+    print("SGC walker", case, flush=True)
+    log, _ = batch.wl_run(30, 1000, log=True)
+    got = batch.wl_get(0)
+    st = wl.state
+    n = min(used[0], 30 * 1000)
+    assert np.array_equal(np.unpackbits(log[0], bitorder="little")[:n], np.unpackbits(ref, bitorder="little")[:n])
+    assert np.array_equal(got["hist"], wl.hist) and np.array_equal(got["logdos"], wl.logdos)
+    assert got["f"] == st.f and got["current_bin"] == st.cur_bin and got["n_accept"] == st.n_accept
+    assert np.array_equal(batch.occupation(0), ora.get_occupation())
+    assert abs(batch.energies()[0] - ora.get_energy()) <= 1e-10 * s.n_sites
+    if case == "ternary_adaptive":
+        sch = batch.wl_scheme(0)
+        assert np.array_equal(sch["sub_bins"].ravel(), wl.sub_hist) and sch["upper"] == st.upper
+
+
+@pytest.mark.gpu
+def test_python_driver_sgc_converges_to_exact_dos(gpu, sgc_cell):
+    """WangLandau in the semi-grand-canonical ensemble with the CE
+    calculator: eight walkers; the averaged ln g matches the enumerated one
+    of the energy with the chemical potential."""
+    from cemc_amd.calculator import CE
+    from cemc_amd.lattice import SimpleAtoms
+    from cemc_amd.wang_landau import WangLandau
+    c = sgc_cell
+    s = c["s"]
+    eci = dict(c["eci"])
+    eci["c1_0"] += 0.008  # the calculator's own ECI; the walkers fold mu = 0.008 back in
+    atoms = SimpleAtoms([s.species[v] for v in c["states"][300]])
+    CE(atoms, s, eci=eci)
+    wl = WangLandau(atoms, Nbins=NB_S, initial_f=1.0, flatness_criteria=0.8, fmin=1e-4, Emin=EMIN_S, Emax=EMAX_S,
+                    check_convergence_every=2000, n_walkers=8, seed=3, ensemble="semi-grand-canonical",
+                    chem_pot={"c1_0": 0.008})
+    print("python driver, SGC", flush=True)
+    assert wl.run_fast_sampler(maxsteps=800000)
+    g = c["g"]
+    known = g > 0
+    est = wl.logdos()
+    assert np.array_equal(~np.isnan(est), known)
+    d = est[known] - np.log(g[known])
+    d -= d.mean()
+    assert np.abs(d).max() < 0.25, d
+    wl.close()
