@@ -10,6 +10,7 @@
 //               runs in updater_kernel.
 // cemc_batch    device-resident Markov chains advanced by sweep_kernel.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -810,13 +811,23 @@ struct cemc_updater {
   double* d_ring = nullptr;
   double* d_slot_energy = nullptr;
   double* d_eci = nullptr;
-  double* d_scratch = nullptr;
   int32_t* d_changes = nullptr;
   int32_t* d_restore = nullptr;
   int change_cap = 0;
-  // pinned host memory: the energy read back after a call, and staging for
-  // change lists too long for the kernel arguments
+  // work plan of updater_kernel: items and segments per translation group
+  // (host copies of the segments give each change its item range)
+  std::vector<UpdSeg> segs;
+  std::vector<int32_t> grp_seg;  // n_groups + 1
+  UpdItem* d_items = nullptr;
+  UpdSeg* d_segs = nullptr;
+  UpdPlan plan{};
+  uint32_t lds_bytes = 0;
+  // pinned host memory: the energy read back by get_energy, the result a
+  // launch stores itself (coherent and mapped: `d_result` is its device
+  // address), and staging for change lists too long for the kernel arguments
   double* h_energy = nullptr;
+  UpdResult* h_result = nullptr;
+  UpdResult* d_result = nullptr;
   int32_t* h_stage = nullptr;
   size_t stage_cap = 0;  // int32 entries at h_stage
   std::vector<int8_t> occ;  // host mirror of the device occupation
@@ -836,12 +847,14 @@ struct cemc_updater {
   ~cemc_updater() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (h_energy) (void)hipHostFree(h_energy);
+    if (h_result) (void)hipHostFree(h_result);
     if (h_stage) (void)hipHostFree(h_stage);
     dev_free(d_occ);
     dev_free(d_ring);
     dev_free(d_slot_energy);
     dev_free(d_eci);
-    dev_free(d_scratch);
+    dev_free(d_items);
+    dev_free(d_segs);
     dev_free(d_changes);
     dev_free(d_restore);
     if (stream) (void)hipStreamDestroy(stream);
@@ -867,29 +880,137 @@ void updater_stage_copy(cemc_updater* u, int32_t* dst, const int32_t* src, size_
   HIP_CHECK(hipMemcpyAsync(dst, u->h_stage, sizeof(int32_t) * n, hipMemcpyHostToDevice, u->stream));
 }
 
-// One launch for a sequence of changes, queued on the updater's stream
-// without waiting: short sequences travel in the kernel arguments.  A read
-// of the result (updater_energy, get_cf) waits for it.
-void updater_flush(cemc_updater* u, const std::vector<int32_t>& packed) {
-  const int n = int(packed.size() / 4);
+// Work plan of updater_kernel for the updater's model (see UpdItem): for
+// each translation group g and cluster correlation function k with a cluster
+// entry at g, one segment of items (equivalent decoration e, record r of the
+// entry), in the order e-major; LDS passes of at most 4096 items.
+void updater_build_plan(cemc_updater* u) {
+  const cemc_model& m = *u->model;
+  const int ncf = m.n_cf;
+  std::vector<UpdItem> items;
+  u->segs.clear();
+  u->grp_seg.assign(size_t(m.n_groups) + 1, 0);
+  for (int g = 0; g < m.n_groups; ++g) {
+    u->grp_seg[g] = int32_t(u->segs.size());
+    for (int k = 0; k < ncf; ++k) {
+      if (m.cf_kind[k] != CEMC_CF_CLUSTER) continue;
+      const int ent = m.cf_entry[size_t(g) * ncf + k];
+      if (ent < 0) continue;
+      const ClusterEntry& E = m.entries[ent];
+      const int e0 = m.cf_eq_off[k], e1 = m.cf_eq_off[k + 1];
+      UpdSeg sg{};
+      sg.k = k;
+      sg.begin = int32_t(items.size());
+      for (int e = e0; e < e1; ++e) {
+        const uint32_t deco = m.eq_decos[e];
+        for (int r = 0; r < E.rec_count; ++r) {
+          const SubRec rec = m.recs[size_t(E.rec_begin) + r];
+          const int nm = rec_size(rec) - 1;
+          UpdItem it{};
+          for (int q = 0; q < nm; ++q) it.m[q] = m.periodic ? m.col_cells[rec_slot(rec, q)] : rec_slot(rec, q);
+          for (int pos = 0; pos <= nm; ++pos) {
+            const uint32_t d = (deco >> (8 * pos)) & 0xFFu;
+            require(d < 8u, "decoration index out of range");
+            it.aux |= d << (3 * pos);
+          }
+          it.aux |= uint32_t(rec_refpos(rec)) << 12;
+          it.aux |= uint32_t(nm) << 14;
+          items.push_back(it);
+          require(items.size() < (size_t(1) << 31), "too many updater items");
+        }
+      }
+      sg.end = int32_t(items.size());
+      sg.mul = double(unsigned(E.size)) / double(e1 - e0);
+      sg.div = double(E.norm * m.grp_count[g]);
+      u->segs.push_back(sg);
+    }
+  }
+  u->grp_seg[m.n_groups] = int32_t(u->segs.size());
+  std::vector<int32_t> singlets;
+  for (int k = 0; k < ncf; ++k)
+    if (m.cf_kind[k] == CEMC_CF_SINGLET) singlets.push_back(int32_t(k | (m.cf_dec[k] << 16)));
+  require(ncf < (1 << 16), "too many correlation functions for the updater");
+  // the occupation is staged in LDS when it fits beside the rest (lattices
+  // up to ~100k sites); passes of up to 4096 items
+  constexpr uint32_t kLdsCap = 150u * 1024u;
+  const int occ_sites = updater_lds_bytes(ncf, 256, m.n_sites) <= kLdsCap ? m.n_sites : 0;
+  int chunk = 4096;
+  while (chunk > 256 && updater_lds_bytes(ncf, chunk, occ_sites) > kLdsCap) chunk -= 256;
+  u->lds_bytes = updater_lds_bytes(ncf, chunk, occ_sites);
+  require(u->lds_bytes <= kLdsCap, "too many correlation functions for the updater's LDS");
+  u->d_items = upload(items);
+  u->d_segs = upload(u->segs);
+  require(singlets.size() <= size_t(kUpdMaxSinglets), "too many singlet correlation functions");
+  for (size_t j = 0; j < singlets.size(); ++j) u->plan.singlets[j] = singlets[j];
+  u->plan.items = u->d_items;
+  u->plan.segs = u->d_segs;
+  u->plan.trans = m.d_trans;
+  u->plan.n_singlets = int32_t(singlets.size());
+  u->plan.chunk = chunk;
+  u->plan.n_sites = m.n_sites;
+  u->plan.n_cf = ncf;
+  u->plan.n_species = m.n_species;
+  u->plan.n_bfs = m.n_bfs;
+  u->plan.K = m.K;
+  u->plan.periodic = m.periodic ? 1 : 0;
+  for (int i = 0; i < 3; ++i) u->plan.dims[i] = m.dims[i];
+  u->plan.occ_lds = occ_sites > 0 ? 1 : 0;
+  require(m.basis.size() <= size_t(kMaxBasis * kMaxSpecies), "basis table too large");
+  for (size_t i = 0; i < m.basis.size(); ++i) u->plan.phi[i] = m.basis[i];
+}
+
+// One launch for a sequence of changes, queued on the updater's stream:
+// short sequences travel in the kernel arguments.  With `want_result` the
+// kernel stores the final energy in the mapped result, which is marked
+// pending first; the host waits for it with updater_wait_result.
+void updater_flush(cemc_updater* u, const std::vector<int32_t>& packed, bool want_result) {
+  const int n = int(packed.size() / kUpdWords);
   if (n == 0) return;
-  InlineChanges inl{};
+  UpdChanges inl{};
   const int32_t* d_list = nullptr;
-  if (n <= kInlineChanges) {
+  if (n <= kUpdInline) {
     std::memcpy(inl.v, packed.data(), sizeof(int32_t) * packed.size());
   } else {
     if (n > u->change_cap) {
       HIP_CHECK(hipStreamSynchronize(u->stream));
       dev_free(u->d_changes);
       u->change_cap = std::max(n, 2 * u->change_cap);
-      HIP_CHECK(hipMalloc(&u->d_changes, sizeof(int32_t) * 4 * u->change_cap));
+      HIP_CHECK(hipMalloc(&u->d_changes, sizeof(int32_t) * kUpdWords * u->change_cap));
     }
     updater_stage_copy(u, u->d_changes, packed.data(), packed.size());
     d_list = u->d_changes;
   }
-  HIP_CHECK(launch_updater(u->model->d_view, u->d_occ, u->d_ring, u->d_slot_energy, inl, d_list, n, u->d_eci,
-                           u->d_scratch, u->stream));
-  u->slot_epoch[packed[packed.size() - 1]] = u->eci_epoch;
+  if (want_result) __atomic_store_n(&u->h_result->energy_bits, kUpdPending, __ATOMIC_SEQ_CST);
+  HIP_CHECK(launch_updater(u->plan, u->d_occ, u->d_ring, u->d_slot_energy, inl, d_list, n,
+                           u->d_eci, want_result ? u->d_result : nullptr, u->lds_bytes, u->stream));
+  u->slot_epoch[(packed[packed.size() - kUpdWords + 2] >> 16) & 0xFFFF] = u->eci_epoch;
+}
+
+// Waits for the energy of the last flush with a result: spins on the mapped
+// result until it is no longer pending, and every 4096 polls asks the stream
+// whether it failed or finished without storing it.
+double updater_wait_result(cemc_updater* u) {
+  const auto t0 = std::chrono::steady_clock::now();
+  auto done = [&](double* e) {
+    const uint64_t bits = __atomic_load_n(&u->h_result->energy_bits, __ATOMIC_ACQUIRE);
+    if (bits == kUpdPending) return false;
+    std::memcpy(e, &bits, sizeof(double));
+    return true;
+  };
+  double e = 0.0;
+  for (uint32_t spin = 1;; ++spin) {
+    if (done(&e)) return e;
+    if ((spin & 4095u) == 0) {
+      const hipError_t q = hipStreamQuery(u->stream);
+      if (q != hipSuccess && q != hipErrorNotReady) HIP_CHECK(q);
+      if (q == hipSuccess || std::chrono::steady_clock::now() - t0 > std::chrono::seconds(60)) {
+        HIP_CHECK(hipStreamSynchronize(u->stream));
+        if (done(&e)) return e;
+        throw DeviceError("the updater kernel stored no result");
+      }
+    }
+    __builtin_ia32_pause();
+  }
 }
 
 // One change (ce_updater.cpp:313-347): no-op if the symbols are equal, else
@@ -915,11 +1036,20 @@ int updater_stage(cemc_updater* u, int site, int old_id, int new_id, std::vector
   u->current = (u->current + 1) % kMaxHistory;
   if (u->buffer_size < unsigned(kMaxHistory)) u->buffer_size += 1;
   u->changes[dst] = cemc_updater::Change{site, old_id, new_id};
+  // the species the site holds, as the reference reads it from its own
+  // symbols (ce_updater.cpp:335), whatever the caller named as old
+  const int cur = u->occ[site];
   u->occ[site] = int8_t(new_id);
+  const int g = m.site_group[site];
+  const int sb = u->grp_seg[g], se = u->grp_seg[g + 1];
   packed.push_back(site);
-  packed.push_back(new_id);
-  packed.push_back(int32_t(src));
-  packed.push_back(int32_t(dst));
+  packed.push_back(new_id | (cur << 8));
+  packed.push_back(int32_t(src | (dst << 16)));
+  packed.push_back(sb < se ? u->segs[sb].begin : 0);
+  packed.push_back(sb < se ? u->segs[se - 1].end : 0);
+  packed.push_back(sb);
+  packed.push_back(se);
+  packed.push_back(0);
   return CEMC_OK;
 }
 
@@ -1827,9 +1957,13 @@ int cemc_updater_create(cemc_model* model, const int8_t* occupation, const doubl
       u->d_eci = upload(u->eci);
       HIP_CHECK(hipMalloc(&u->d_ring, sizeof(double) * size_t(kMaxHistory) * ncf));
       HIP_CHECK(hipMalloc(&u->d_slot_energy, sizeof(double) * kMaxHistory));
-      HIP_CHECK(hipMalloc(&u->d_scratch, sizeof(double) * 4 * size_t(ncf)));
       HIP_CHECK(hipMalloc(&u->d_restore, sizeof(int32_t) * 2 * kMaxHistory));
       HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&u->h_energy), sizeof(double)));
+      HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&u->h_result), sizeof(UpdResult),
+                              hipHostMallocMapped | hipHostMallocCoherent));
+      std::memset(u->h_result, 0, sizeof(UpdResult));
+      HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&u->d_result), u->h_result, 0));
+      updater_build_plan(u);
       std::vector<double> cf0(ncf);
       if (cf_init) {
         cf0.assign(cf_init, cf_init + ncf);
@@ -1914,7 +2048,7 @@ int cemc_updater_update_cf(cemc_updater* u, int32_t site, int32_t old_id, int32_
     require(u != nullptr, "NULL updater");
     std::vector<int32_t> packed;
     rc = updater_stage(u, site, old_id, new_id, packed);
-    updater_flush(u, packed);
+    updater_flush(u, packed, false);
   });
   return g != CEMC_OK ? g : rc;
 }
@@ -1955,8 +2089,15 @@ int cemc_updater_calculate(cemc_updater* u, int32_t n, const int32_t* site, cons
     } else {
       for (int i = 0; i < n && rc == CEMC_OK; ++i) rc = updater_stage(u, site[i], old_id[i], new_id[i], packed);
     }
-    updater_flush(u, packed);
-    if (rc == CEMC_OK && energy) *energy = updater_energy(u);
+    if (rc == CEMC_OK && energy && !packed.empty()) {
+      // the launch stores the energy in mapped host memory: no copy, no
+      // stream wait
+      updater_flush(u, packed, true);
+      *energy = updater_wait_result(u);
+    } else {
+      updater_flush(u, packed, false);
+      if (rc == CEMC_OK && energy) *energy = updater_energy(u);
+    }
   });
   return g != CEMC_OK ? g : rc;
 }

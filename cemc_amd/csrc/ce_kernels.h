@@ -95,17 +95,75 @@ hipError_t launch_cf(const ModelDev* models, const ChainParams* params, const in
 hipError_t launch_energy(const ModelDev* models, const ChainParams* params, const int8_t* occ_pool,
                          const int32_t* chain_ids, int n_chains, int max_sites, double* partial, double* energy_out,
                          hipStream_t stream);
-// Up to kInlineChanges changes of the updater interface travel in the kernel
-// arguments (4 ints each: site, new species, source slot, destination slot;
-// or, for a restore, site and species pairs), so a call needs no copy to the
-// device; longer sequences are read from `changes` in device memory.
+// Up to kInlineChanges site/species pairs of a restore travel in the kernel
+// arguments, so an undo needs no copy to the device.
 constexpr int kInlineChanges = 8;
 struct InlineChanges {
   int32_t v[4 * kInlineChanges];
 };
-hipError_t launch_updater(const ModelDev* model, int8_t* occ, double* ring, double* slot_energy,
-                          const InlineChanges& inl, const int32_t* changes, int n, const double* eci,
-                          double* scratch, hipStream_t stream);
+
+// Work plan of the updater kernel for one model.  The change of correlation
+// function k at a site of translation group g sums one product per
+// (equivalent decoration, sub-cluster record of the site) pair; those
+// products are the items of segment (g, k), and a change at a site of group
+// g touches the segments [grp_seg[g], grp_seg[g + 1]) (ce_updater.cpp:349-404).
+struct UpdItem {
+  // members other than the reference site, in canonical order: closed-form
+  // periodic lattices: wrapped cell offset x | y<<8 | z<<16; otherwise
+  // translation-table column
+  uint32_t m[3];
+  // decoration of position p in bits 3p..3p+2 (p = 0..3), the reference
+  // site's position << 12, members other than the reference site << 14
+  uint32_t aux;
+};
+struct UpdSeg {
+  int32_t k;
+  int32_t begin, end;  // items
+  int32_t pad;
+  double mul;  // cluster size / equivalent decorations
+  double div;  // normalisation C_p * T_g
+};
+// Everything the kernel needs besides the lattice travels in its arguments,
+// so no load waits on another one at the start.
+constexpr int kUpdMaxSinglets = 8;
+struct UpdPlan {
+  const UpdItem* items;
+  const UpdSeg* segs;
+  const int32_t* trans;     // translation table (lattices without the closed form)
+  int32_t singlets[kUpdMaxSinglets];  // singlet correlation functions: k | decoration << 16
+  int32_t n_singlets;
+  int32_t chunk;            // items per LDS pass
+  int32_t n_sites, n_cf, n_species, n_bfs, K, periodic;
+  int32_t dims[3];
+  int32_t occ_lds;          // 1: the occupation is staged in LDS
+  double phi[kMaxBasis * kMaxSpecies];  // basis functions phi[d * n_species + s]
+};
+// Result of an updater launch in mapped, coherent host memory: the bits of
+// the energy, one 8-byte store at system scope.  The host sets them to
+// kUpdPending (a signalling NaN that no arithmetic result carries) before the
+// launch and waits until they change, so no fence or cache write-back is
+// needed on the device.
+constexpr uint64_t kUpdPending = 0x7FF4C0DE0000D1CEull;
+struct UpdResult {
+  uint64_t energy_bits;
+  uint64_t pad;
+};
+// Up to kUpdInline changes of the updater interface travel in the kernel
+// arguments, kUpdWords ints each: site, new | old << 8, src | dst << 16
+// (history slots), first item, end of items, first segment, end of segments,
+// unused; longer sequences are read from `changes` in device memory.
+constexpr int kUpdInline = 8;
+constexpr int kUpdWords = 8;
+struct UpdChanges {
+  int32_t v[kUpdWords * kUpdInline];
+};
+// Dynamic LDS of the updater kernel: correlation functions, segment sums,
+// ECIs and segments (per correlation function), item products of one pass,
+// and the occupation when staged (`occ_sites` bytes, 0 when not).
+uint32_t updater_lds_bytes(int n_cf, int chunk, int occ_sites);
+hipError_t launch_updater(const UpdPlan& plan, int8_t* occ, double* ring,
+                          double* slot_energy, const UpdChanges& inl, const int32_t* changes, int n,
+                          const double* eci, UpdResult* result, uint32_t lds_bytes, hipStream_t stream);
 // Restores sites[i] = species[i] in order; n <= 2 * kInlineChanges pairs
 // come in `inl` (sites in v[0..n), species in v[n..2n)), more from `buf`
 // (sites then species).

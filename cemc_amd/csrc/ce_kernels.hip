@@ -16,6 +16,10 @@
 //                   updater interface (ce_updater.cpp:313-406, 845-852).
 //  restore_kernel   undo of popped changes (ce_updater.cpp:414-448).
 //  slot_energy_kernel  E = N * sum_k J_k cf_k of one history slot.
+#include <atomic>
+#include <map>
+#include <mutex>
+
 #include "ce_sweep.h"
 #include "ce_wl.h"
 
@@ -202,103 +206,233 @@ __global__ __launch_bounds__(64) void cf_finalize_kernel(const ModelDev* __restr
 }
 
 // ---------------------------------------------------------------------------
-// Updater interface: apply `n` single-site changes in order.  Change c reads
-// history slot src[c], writes slot dst[c] (CFHistoryTracker::get_next,
-// cpp/src/cf_history_tracker.cpp:46-56) and sets the site's species.  After
-// the last change the energy of the final slot is stored.
-__global__ __launch_bounds__(256) void updater_kernel(const ModelDev* __restrict__ model,
-                                                      int8_t* __restrict__ occ, double* __restrict__ ring,
-                                                      double* __restrict__ slot_energy, InlineChanges inl,
+// Updater interface: apply `n` single-site changes in order, one workgroup.
+// Change c reads history slot src, writes slot dst (CFHistoryTracker::get_next,
+// cpp/src/cf_history_tracker.cpp:46-56) and sets the site's species
+// (ce_updater.cpp:313-406).  The correlation functions of the current slot
+// stay in LDS between changes.  Per change every thread forms the products of
+// its plan items (one dependent gather of the members' species) into LDS,
+// then wave w sums the segments w, w + 4, ... in a fixed order, so results do
+// not depend on timing.  After the last change E = N J.cf goes to the slot
+// and, when `result` is given, to mapped host memory, so the host reads it
+// without a copy or a stream wait.
+uint32_t updater_lds_bytes(int n_cf, int chunk, int occ_sites) {
+  const int ncf_pad = (n_cf + 31) / 32 * 32;
+  return uint32_t(3 * sizeof(double) + sizeof(UpdSeg)) * uint32_t(ncf_pad) +
+         uint32_t(sizeof(double)) * uint32_t(chunk) + uint32_t((occ_sites + 15) / 16 * 16);
+}
+
+// kInArgs: the changes travel in the kernel arguments (else in `changes`);
+// kOccLds: the occupation is staged in LDS; kPeriodic: closed-form periodic
+// lattice (else members come from the translation table).  With the occupation in LDS the
+// workgroup exchanges data through LDS only, so its barriers wait for LDS
+// operations alone and global loads and stores stay in flight across them
+// (a full barrier waits for every load and store of the wave, one memory
+// round trip); otherwise the occupation written for a change is read back
+// from global memory by the next change, which needs the full barrier.
+// A change in the translation group of the change before it reuses that
+// change's items and segments (every swap on a one-group lattice).
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima).
+__device__ inline void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+template <bool kInArgs, bool kOccLds, bool kPeriodic>
+__global__ __launch_bounds__(256) void updater_kernel(UpdPlan plan, int8_t* occ, double* __restrict__ ring,
+                                                      double* __restrict__ slot_energy, UpdChanges inl,
                                                       const int32_t* __restrict__ changes, int n,
-                                                      const double* __restrict__ eci,
-                                                      double* __restrict__ scratch) {
-  const ModelDev& m = *model;
+                                                      const double* __restrict__ eci, UpdResult* result) {
+  extern __shared__ double upd_lds[];
+  __shared__ double phi[kMaxBasis * kMaxSpecies];
+  __shared__ double red[4];
   const int tid = int(threadIdx.x);
-  const int wave = tid >> 6;
-  const int S = m.n_species;
-  const int ncf = m.n_cf;
-  const bool in_args = n <= kInlineChanges;
-  int last_dst = -1;
-  for (int c = 0; c < n; ++c) {
-    const int site = in_args ? inl.v[4 * c + 0] : changes[4 * c + 0];
-    const int nw = in_args ? inl.v[4 * c + 1] : changes[4 * c + 1];
-    const int src = in_args ? inl.v[4 * c + 2] : changes[4 * c + 2];
-    const int dst = in_args ? inl.v[4 * c + 3] : changes[4 * c + 3];
-    last_dst = dst;
-    const int old = occ[site];
-    __syncthreads();
-    if (tid == 0) occ[site] = int8_t(nw);
-    __threadfence_block();
-    __syncthreads();
-    const double* cur = ring + int64_t(src) * ncf;
-    double* nxt = ring + int64_t(dst) * ncf;
-    const int g = site_group(m, site);
-    const RefSite ref = make_ref(m, site);
-    for (int k = 0; k < ncf; ++k) {
-      const int kind = m.cf_kind[k];
-      if (kind == 2) {
-        const int ent = m.cf_entry[g * ncf + k];
-        if (ent < 0) continue;
-        const ClusterEntry E = m.entries[ent];
-        const int e0 = m.cf_eq_off[k];
-        const int ne = m.cf_eq_off[k + 1] - e0;
-        const int total = ne * E.rec_count;
-        double part = 0.0;
-        for (int idx = tid; idx < total; idx += 256) {
-          const int e = idx / E.rec_count;
-          const int mm = idx - e * E.rec_count;
-          const SubRec r = m.recs[E.rec_begin + mm];
-          const int nm = rec_size(r) - 1;
-          const int rp = rec_refpos(r);
-          const uint32_t deco = m.eq_decos[e0 + e];
-          double prod = 1.0;
-          int q = 0;
-          for (int pos = 0; pos <= nm; ++pos) {
-            const int d = int((deco >> (8 * pos)) & 0xFFu);
-            if (pos == rp) {
-              prod *= m.basis[d * S + nw] - m.basis[d * S + old];
-            } else {
-              const int32_t nb = m.periodic ? neighbour<true>(m, ref, rec_slot(r, q), m.col_cells)
-                                            : neighbour<false>(m, ref, rec_slot(r, q), m.col_cells);
-              prod *= m.basis[d * S + occ[nb]];
-              ++q;
-            }
-          }
-          part += prod;
-        }
-        part = wave_sum_f64(part);
-        if ((tid & 63) == 0) scratch[4 * k + wave] = part;
-      }
+  const int wave = tid >> 6, lane = tid & 63;
+  const int S = plan.n_species;
+  const int N = plan.n_sites;
+  const int ncf = plan.n_cf;
+  const int ncf_pad = (ncf + 31) / 32 * 32;
+  double* cfl = upd_lds;                                     // correlation functions of the current slot
+  double* acc = cfl + ncf_pad;                               // segment sums over earlier passes
+  double* ecil = acc + ncf_pad;                              // ECIs
+  UpdSeg* segl = reinterpret_cast<UpdSeg*>(ecil + ncf_pad);  // segments of the change
+  double* prod = reinterpret_cast<double*>(segl + ncf_pad);  // item products of one pass
+  int8_t* occ_l = reinterpret_cast<int8_t*>(prod + plan.chunk);  // staged occupation
+  auto word = [&](int c, int w) { return kInArgs ? inl.v[kUpdWords * c + w] : changes[kUpdWords * c + w]; };
+  auto barrier = [&] {
+    if constexpr (kOccLds) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    } else {
+      __threadfence_block();
+      __syncthreads();
     }
-    __threadfence_block();
-    __syncthreads();
+  };
+  auto species_at = [&](int32_t i) -> int {
+    if constexpr (kOccLds) {
+      return occ_l[i];
+    } else {
+      return occ[i];
+    }
+  };
+  // items (m[0..2], aux) and segments (k, begin, end, pad; mul, div) are
+  // held as vectors so that they stay in registers
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+  typedef double f64x2 __attribute__((ext_vector_type(2)));
+  static_assert(sizeof(UpdItem) == 16 && sizeof(UpdSeg) == 32, "updater plan layout");
+  const u32x4* items = reinterpret_cast<const u32x4*>(plan.items);
+  u32x4 it0 = {0u, 0u, 0u, 0u};  // item ib + tid of the current group
+  i32x4 si0 = {0, 0, 0, 0};      // segment sb + tid of the current group
+  f64x2 sd0 = {0.0, 0.0};
+  auto fetch = [&](int ib, int ie, int sb, int se) {
+    if (ib + tid < min(ie, ib + plan.chunk)) it0 = items[ib + tid];
+    if (sb + tid < se) {
+      const i32x4* p = reinterpret_cast<const i32x4*>(plan.segs + sb + tid);
+      si0 = p[0];
+      sd0 = *reinterpret_cast<const f64x2*>(p + 1);
+    }
+  };
+  fetch(word(0, 3), word(0, 4), word(0, 5), word(0, 6));
+  {
+    // ECIs go to LDS now, so that no wait at the end covers the stores made
+    // meanwhile (loads and stores share one counter)
+    const int src0 = word(0, 2) & 0xFFFF;
     for (int k = tid; k < ncf; k += 256) {
-      const int kind = m.cf_kind[k];
-      double v = cur[k];
-      if (kind == 1) {
-        const int d = m.cf_dec[k];
-        v = cur[k] + (m.basis[d * S + nw] - m.basis[d * S + old]) / double(unsigned(m.n_sites));
-      } else if (kind == 2) {
-        const int ent = m.cf_entry[g * ncf + k];
-        if (ent >= 0) {
-          const ClusterEntry E = m.entries[ent];
-          const int ne = m.cf_eq_off[k + 1] - m.cf_eq_off[k];
-          double delta = (scratch[4 * k] + scratch[4 * k + 1]) + (scratch[4 * k + 2] + scratch[4 * k + 3]);
-          delta *= double(unsigned(E.size)) / double(ne);
-          delta /= double(E.norm * m.grp_count[g]);
-          v = cur[k] + delta;
+      cfl[k] = ring[int64_t(src0) * ncf + k];
+      ecil[k] = eci[k];
+    }
+    for (int i = tid; i < plan.n_bfs * S; i += 256) phi[i] = plan.phi[i];
+    if constexpr (kOccLds) {
+      // 4-byte words, then the last N mod 4 bytes (the allocation is aligned)
+      const int nw4 = N >> 2;
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(occ);
+      uint32_t* dst = reinterpret_cast<uint32_t*>(occ_l);
+      for (int i = tid; i < nw4; i += 256) dst[i] = src[i];
+      if (tid < (N & 3)) occ_l[4 * nw4 + tid] = occ[4 * nw4 + tid];
+    }
+  }
+  // The loads above complete before the first barrier in any case.  Waiting
+  // for all of them here, by name, tells the compiler that the item held in
+  // registers has arrived, so the changes that reuse it wait for no memory
+  // operation (a conservative wait there would cover the history-slot stores
+  // of the change before).
+  vm_wait_all();
+  int last_dst = -1;
+  int cur_sb = -1;  // segments held in segl (first segment index)
+  for (int c = 0; c < n; ++c) {
+    const int site = word(c, 0);
+    const int nw = word(c, 1) & 0xFF, old = (word(c, 1) >> 8) & 0xFF;
+    const int dst = (word(c, 2) >> 16) & 0xFFFF;
+    const int ib = word(c, 3), ie = word(c, 4), sb = word(c, 5), se = word(c, 6);
+    last_dst = dst;
+    if (sb != cur_sb) {
+      if (c > 0) {
+        fetch(ib, ie, sb, se);
+        vm_wait_all();
+      }
+      if (sb + tid < se) segl[tid] = UpdSeg{si0.x, si0.y, si0.z, si0.w, sd0.x, sd0.y};
+      for (int j = tid + 256; j < se - sb; j += 256) segl[j] = plan.segs[sb + j];
+      cur_sb = sb;
+      // the staged data and the segments are visible after this barrier
+      // (with the same group the barrier closing the change before suffices)
+      barrier();
+    }
+    // cell coordinates of the site (closed-form periodic lattices)
+    int32_t rx = 0, ry = 0, rz = 0;
+    if constexpr (kPeriodic) {
+      const int32_t l23 = plan.dims[1] * plan.dims[2];
+      rx = site / l23;
+      const int32_t rem = site - rx * l23;
+      ry = rem / plan.dims[2];
+      rz = rem - ry * plan.dims[2];
+    }
+    auto product = [&](const u32x4& it) {
+      const uint32_t aux = it.w;
+      const int nm = int((aux >> 14) & 3u);
+      const int rp = int((aux >> 12) & 3u);
+      double p = 1.0;
+      int q = 0;
+#pragma unroll
+      for (int pos = 0; pos < kMaxClusterSize; ++pos) {
+        if (pos > nm) break;
+        const int d = int((aux >> (3 * pos)) & 7u);
+        if (pos == rp) {
+          p *= phi[d * S + nw] - phi[d * S + old];
+        } else {
+          const uint32_t mq = q == 0 ? it.x : (q == 1 ? it.y : it.z);
+          int32_t nb;
+          if constexpr (kPeriodic) {
+            int32_t x = rx + int32_t(mq & 0xFFu);
+            int32_t y = ry + int32_t((mq >> 8) & 0xFFu);
+            int32_t z = rz + int32_t((mq >> 16) & 0xFFu);
+            x -= (x >= plan.dims[0]) ? plan.dims[0] : 0;
+            y -= (y >= plan.dims[1]) ? plan.dims[1] : 0;
+            z -= (z >= plan.dims[2]) ? plan.dims[2] : 0;
+            nb = (x * plan.dims[1] + y) * plan.dims[2] + z;
+          } else {
+            nb = plan.trans[int64_t(site) * plan.K + mq];
+          }
+          p *= phi[d * S + species_at(nb)];
+          ++q;
         }
       }
-      nxt[k] = v;
+      return p;
+    };
+    for (int cb = ib; cb < ie; cb += plan.chunk) {
+      const int ce = min(ie, cb + plan.chunk);
+      int idx = cb + tid;
+      if (cb == ib) {
+        if (idx < ce) prod[idx - cb] = product(it0);
+        idx += 256;
+      }
+      for (; idx < ce; idx += 256) prod[idx - cb] = product(items[idx]);
+      barrier();
+      const bool last_pass = ce == ie;
+      for (int j = wave; j < se - sb; j += 4) {
+        const UpdSeg g = segl[j];
+        const int lo = max(g.begin, cb), hi = min(g.end, ce);
+        double v = 0.0;
+        for (int i = lo + lane; i < hi; i += 64) v += prod[i - cb];
+        v = wave_sum_f64(v);
+        if (lane == 0) {
+          const double t = (cb == ib ? 0.0 : acc[j]) + v;
+          if (last_pass) {
+            cfl[g.k] += (t * g.mul) / g.div;
+          } else {
+            acc[j] = t;
+          }
+        }
+      }
+      if (!last_pass) barrier();
     }
-    __threadfence_block();
-    __syncthreads();
+    // thread j updates singlet j (fixed indices: the list is read from the
+    // kernel arguments with scalar loads)
+#pragma unroll
+    for (int j = 0; j < kUpdMaxSinglets; ++j) {
+      if (tid == j && j < plan.n_singlets) {
+        const int k = plan.singlets[j] & 0xFFFF, d = plan.singlets[j] >> 16;
+        cfl[k] += (phi[d * S + nw] - phi[d * S + old]) / double(unsigned(N));
+      }
+    }
+    // every gather of this change is done (the pass barrier above)
+    if (tid == 0) {
+      occ[site] = int8_t(nw);
+      if constexpr (kOccLds) occ_l[site] = int8_t(nw);
+    }
+    barrier();
+    for (int k = tid; k < ncf; k += 256) ring[int64_t(dst) * ncf + k] = cfl[k];
   }
-  if (tid == 0 && last_dst >= 0) {
-    const double* cf = ring + int64_t(last_dst) * ncf;
-    double dot = 0.0;
-    for (int k = 0; k < ncf; ++k) dot += eci[k] * cf[k];
-    slot_energy[last_dst] = dot * double(unsigned(m.n_sites));
+  if (last_dst < 0) return;
+  double part = 0.0;
+  for (int k = tid; k < ncf; k += 256) part += ecil[k] * cfl[k];
+  part = wave_sum_f64(part);
+  if (lane == 0) red[wave] = part;
+  barrier();
+  if (tid == 0) {
+    const double e = ((red[0] + red[1]) + (red[2] + red[3])) * double(unsigned(N));
+    slot_energy[last_dst] = e;
+    if (result != nullptr)
+      __hip_atomic_store(&result->energy_bits, uint64_t(__double_as_longlong(e)), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -646,12 +780,38 @@ hipError_t launch_energy(const ModelDev* models, const ChainParams* params, cons
   return hipGetLastError();
 }
 
-hipError_t launch_updater(const ModelDev* model, int8_t* occ, double* ring, double* slot_energy,
-                          const InlineChanges& inl, const int32_t* changes, int n, const double* eci,
-                          double* scratch, hipStream_t stream) {
-  hipLaunchKernelGGL(updater_kernel, dim3(1), dim3(256), 0, stream, model, occ, ring, slot_energy, inl, changes, n,
-                     eci, scratch);
+template <bool kInArgs, bool kOccLds, bool kPeriodic>
+static hipError_t launch_updater_t(const UpdPlan& plan, int8_t* occ, double* ring, double* slot_energy,
+                                   const UpdChanges& inl, const int32_t* changes, int n, const double* eci,
+                                   UpdResult* result, uint32_t lds_bytes, hipStream_t stream) {
+  auto* fn = updater_kernel<kInArgs, kOccLds, kPeriodic>;
+  if (lds_bytes > 64u * 1024u) {
+    static std::mutex mu;
+    static uint32_t limit = 64u * 1024u;
+    std::lock_guard<std::mutex> lock(mu);
+    if (lds_bytes > limit) {
+      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
+      if (e != hipSuccess) return e;
+      limit = lds_bytes;
+    }
+  }
+  hipLaunchKernelGGL(fn, dim3(1), dim3(256), lds_bytes, stream, plan, occ, ring, slot_energy, inl, changes, n, eci,
+                     result);
   return hipGetLastError();
+}
+
+hipError_t launch_updater(const UpdPlan& plan, int8_t* occ, double* ring,
+                          double* slot_energy, const UpdChanges& inl, const int32_t* changes, int n,
+                          const double* eci, UpdResult* result, uint32_t lds_bytes, hipStream_t stream) {
+  const bool a = n <= kUpdInline, o = plan.occ_lds != 0, p = plan.periodic != 0;
+  auto go = [&](auto fn) { return fn(plan, occ, ring, slot_energy, inl, changes, n, eci, result, lds_bytes, stream); };
+  if (a) {
+    if (o) return p ? go(launch_updater_t<true, true, true>) : go(launch_updater_t<true, true, false>);
+    return p ? go(launch_updater_t<true, false, true>) : go(launch_updater_t<true, false, false>);
+  }
+  if (o) return p ? go(launch_updater_t<false, true, true>) : go(launch_updater_t<false, true, false>);
+  return p ? go(launch_updater_t<false, false, true>) : go(launch_updater_t<false, false, false>);
 }
 
 hipError_t launch_restore(int8_t* occ, const InlineChanges& inl, const int32_t* buf, int n, hipStream_t stream) {

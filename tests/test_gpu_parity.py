@@ -188,6 +188,122 @@ def test_updater_matches_oracle_with_equivalent_sites(gpu):
         assert np.max(np.abs(upd.get_cf_array() - ora.get_cf_array())) <= 1e-12
 
 
+def _allowed_species(s):
+    """Species ids each site may hold (its sublattice's elements; none for
+    background sites)."""
+    sp = list(s.species)
+    bkg = set(int(i) for i in s.background_indices)
+    sub = getattr(s, "sublattice_of_site", None)
+    out = []
+    for i in range(s.n_sites):
+        if i in bkg:
+            out.append([])
+        elif sub is None:
+            out.append(list(range(len(sp))))
+        else:
+            out.append([sp.index(e) for e in s.basis_elements[int(sub[i])]])
+    return out
+
+
+def _replay_random_calls(s, occ, steps, seed, cf0=None):
+    """Random updater calls on the GPU and on the oracle: single flips, swaps
+    of unlike sites of one sublattice, and twelve flips in one call (more
+    changes than travel in the kernel arguments); undo or clear after each.
+    Energies, correlation functions and lattices are compared after every
+    call."""
+    from cemc_amd.lattice import SimpleAtoms, seeded_ecis
+    from cemc_amd.updater import PyCEUpdater
+    from oracle.brute_force import brute_force_cf
+    from oracle.oracle import OracleModel, OracleUpdater
+    names = s.cf_names()
+    eci = seeded_ecis(s, 4, 0.01)
+    sp = list(s.species)
+    upd = PyCEUpdater(SimpleAtoms([sp[v] for v in occ]), s, cf0 if cf0 is None else dict(cf0), eci)
+    if cf0 is None:
+        cf0 = brute_force_cf(s, occ, names) if s.n_sites <= 20000 else upd.get_cf()
+    ora = OracleUpdater(OracleModel(s, names), occ, cf0, eci)
+    allowed = _allowed_species(s)
+    movable = [i for i in range(s.n_sites) if len(allowed[i]) > 1]
+    rng = np.random.Generator(np.random.Philox(seed))
+    cur = [int(v) for v in occ]
+
+    def flip(state):
+        i = movable[int(rng.integers(len(movable)))]
+        opts = [v for v in allowed[i] if v != state[i]]
+        new = opts[int(rng.integers(len(opts)))]
+        ch = (i, sp[state[i]], sp[new])
+        state[i] = new
+        return ch
+
+    n_long = 0
+    for step in range(steps):
+        tmp = list(cur)
+        if step % 3 == 0:
+            chs = [flip(tmp) for _ in range(12)]
+            n_long += 1
+        elif step % 3 == 1:
+            i = movable[int(rng.integers(len(movable)))]
+            js = [j for j in movable if allowed[j] == allowed[i] and cur[j] != cur[i]]
+            if not js:
+                chs = [flip(tmp)]
+            else:
+                j = js[int(rng.integers(len(js)))]
+                chs = [(i, sp[cur[i]], sp[cur[j]]), (j, sp[cur[j]], sp[cur[i]])]
+        else:
+            chs = [flip(tmp)]
+        e_gpu = upd.calculate(chs)
+        e_ora = ora.calculate(chs)
+        assert abs(e_gpu - e_ora) / s.n_sites <= 1e-12, step
+        if step % 4 == 1:
+            upd.undo_changes()
+            ora.undo_changes()
+        else:
+            upd.clear_history()
+            ora.clear_history()
+        cur = [sp.index(v) for v in upd.get_symbols()]
+        assert cur == [int(v) for v in ora.get_occupation()], step
+        assert np.max(np.abs(upd.get_cf_array() - ora.get_cf_array())) <= 1e-12, step
+        assert abs(upd.get_energy() - ora.get_energy()) / s.n_sites <= 1e-12, step
+    assert n_long > 0
+    upd.close()
+
+
+def _crystal(kind):
+    from cemc_amd.crystal import HCPSettings, Spacegroup217Settings
+    from cemc_amd.lattice import CrystalSettings
+    if kind == "rocksalt":
+        return CrystalSettings(3, [["Na", "K"], ["Cl", "Br"]], {2: 3, 3: 2}, "rocksalt")
+    if kind == "rocksalt_background":
+        return CrystalSettings(3, [["Na", "K", "Li"], ["Cl"]], {2: 3, 3: 2}, "rocksalt")
+    if kind == "hcp_ternary":
+        return HCPSettings((3, 3, 2), ["Al", "Mg", "Si"], {2: 1.45, 3: 1.05, 4: 1.05})
+    wyckoff = [(0, 0, 0), (0.324, 0.324, 0.324), (0.3582, 0.3582, 0.0393), (0.0954, 0.0954, 0.2725)]
+    return Spacegroup217Settings(1, 10.553, wyckoff, [["Al", "Mg"]] * 4, {2: 4.1, 3: 4.1})
+
+
+@pytest.mark.parametrize("kind", ["rocksalt", "rocksalt_background", "hcp_ternary", "sg217_binary"])
+def test_updater_matches_oracle_on_crystals_and_long_change_lists(gpu, kind):
+    """Crystals whose lattices come from translation tables: two translation
+    groups (rocksalt, with and without a background sublattice), one group
+    of two sites (HCP) and 29 groups (I-43m), so consecutive changes of a
+    call switch groups; change lists of twelve flips are read from device
+    memory."""
+    from cemc_amd.lattice import random_crystal_occupation
+    s = _crystal(kind)
+    print("updater on", kind, s.n_sites, "sites", flush=True)
+    _replay_random_calls(s, random_crystal_occupation(s, 5), 45, 11)
+
+
+def test_updater_on_a_lattice_larger_than_lds(gpu):
+    """FCC 54^3 (157464 sites): the occupation does not fit in LDS beside
+    the rest, so the updater reads it from global memory, with full
+    barriers between the changes of a call."""
+    from cemc_amd.lattice import FCCSettings, random_occupation
+    s = FCCSettings(54, ["A", "B"], {2: 1, 3: 1})
+    print("updater on FCC 54^3", flush=True)
+    _replay_random_calls(s, random_occupation(s.n_sites, 2, 3), 30, 12)
+
+
 @pytest.mark.parametrize("fname,n", [("binary_canonical_plain.npz", 3000), ("ternary_sgc.npz", 3000),
                                      ("quaternary_canonical.npz", 400)])
 def test_dropin_updater_loop_reproduces_reference_chain(gpu, fname, n):
