@@ -91,9 +91,11 @@ struct ModelDev {
   const SubRec* recs;
   const FatRec* frecs;        // same order as recs
   const double* gtab;
-  // three species (two slices read per record): the same tables with the two
-  // slices of each code side by side, [block][code][d], so one 16-byte read
-  // fetches both; the other models point this at gtab
+  // three and four species (two and three slices read per record): the same
+  // tables with the read slices of each code side by side, [block][code][d]
+  // (blocks keep their offsets), so one 16-byte read fetches both values of
+  // three species and the three values of four species share one or two
+  // cache lines; the other models point this at gtab
   const double* gtab_il;
   const int32_t* trans;       // n_sites * K
   const uint32_t* col_cells;  // K packed cell offsets (x | y<<8 | z<<16)

@@ -130,7 +130,7 @@ struct cemc_model {
   SubRec* d_recs = nullptr;
   FatRec* d_frecs = nullptr;
   double* d_gtab = nullptr;
-  std::vector<double> gtab_il;  // three species: gtab with the two read slices interleaved
+  std::vector<double> gtab_il;  // three or four species: gtab with the read slices of a code side by side
   double* d_gtab_il = nullptr;
   int64_t* d_gtab_fx = nullptr;
   uint32_t* d_duals = nullptr;
@@ -264,14 +264,20 @@ void cemc_model::build_gtab(const std::vector<double>& J) {
 // every record's block together: the block's entry (d, code) moves from
 // d * 64 + code to 2 * code + d, so the two values of a code are adjacent
 // and 16-byte aligned (blocks start at multiples of 3 x 64 doubles).
+// The tables with the read slices of each code side by side, for the
+// kernels that read three-species tables (two slices: one 16-byte read) and
+// four-species tables from HBM (three slices: adjacent values in one or two
+// cache lines instead of three).  Blocks keep their size and offsets, so a
+// record's table base is the same in both layouts.
 void cemc_model::build_gtab_il() {
   gtab_il.clear();
-  if (read_slices() != 2) return;
-  const int Pc = 1 << (3 * code_bits()), block = deco_slices() * Pc;
+  const int w = read_slices();
+  if (w != 2 && w != 3) return;
+  const int Pc = 1 << (3 * code_bits()), block = deco_slices() * Pc;  // deco_slices() >= w
   gtab_il.assign(gtab.size(), 0.0);
   for (size_t b = 0; b + size_t(block) <= gtab.size(); b += size_t(block))
     for (int code = 0; code < Pc; ++code)
-      for (int d = 0; d < 2; ++d) gtab_il[b + 2 * code + d] = gtab[b + size_t(d) * Pc + code];
+      for (int d = 0; d < w; ++d) gtab_il[b + size_t(w) * code + d] = gtab[b + size_t(d) * Pc + code];
 }
 
 // Integer copy of the energy tables for the local-field path: a site's field

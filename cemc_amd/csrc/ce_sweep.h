@@ -107,6 +107,16 @@ __device__ __attribute__((always_inline)) inline uint32_t wrapped_delta(const Sw
   return uint32_t(dx) | (uint32_t(dy) << 8) | (uint32_t(dz) << 16);
 }
 
+// Slices per code of the interleaved table a kernel reads (0: the plain
+// table): three-species tables always (a record's two values in one 16-byte
+// read), four-species tables when read from HBM (three adjacent values in
+// one or two cache lines); four-species tables in LDS keep the plain layout,
+// whose two first slices a single ds_read2st64_b64 fetches.
+template <int kBM, bool kGtabLds>
+constexpr int il_width() {
+  return kBM == 2 ? 2 : (kBM == 3 && !kGtabLds ? 3 : 0);
+}
+
 template <int kBM>
 struct TableGeom {
   static constexpr int kBits = kBM == 1 ? 1 : (kBM <= 3 ? 2 : 3);  // code bits per species
@@ -120,10 +130,10 @@ struct TableGeom {
 // kNM: members gathered per record (2 when no record has more than two
 // other members, i.e. the model has no quadruplets; the third field of the
 // code is then 0, which reads the same entry since entries repeat over the
-// fields of unused members).  kIL: the interleaved table of three-species
-// models (ModelDev::gtab_il), where the two slices of a code sit at
-// base + 2 code.
-template <bool kPeriodic, int kR, int kBM, bool kPatch, int kNM = 3, bool kIL = false>
+// fields of unused members).  kIL: slices per code in the interleaved table
+// (ModelDev::gtab_il), where the kIL slices of a code sit at base + kIL code
+// (0: the plain table, slice d of a code at base + d * codes + code).
+template <bool kPeriodic, int kR, int kBM, bool kPatch, int kNM = 3, int kIL = 0>
 __device__ __attribute__((always_inline)) inline void table_index(const SweepCtx& c, const FatRec (&rec)[kR],
                                                                   const Ref& ref, const uint8_t* occ,
                                                                   uint32_t patch_w, uint32_t patch_species,
@@ -165,7 +175,7 @@ __device__ __attribute__((always_inline)) inline void table_index(const SweepCtx
     }
     // unused members need no masking: their table entries repeat over them
     const uint32_t code = s0 | (s1 << kBits) | (s2 << (2 * kBits));
-    gi[k] = int(rec[k].meta & 0xFFFFFFu) + int(kIL ? 2u * code : code);
+    gi[k] = int(rec[k].meta & 0xFFFFFFu) + int(kIL ? uint32_t(kIL) * code : code);
   }
 }
 
@@ -194,18 +204,23 @@ __device__ __attribute__((always_inline)) inline dbl2_t load_pair(GlobalDoublePt
 // Phase 2: energy-table reads.  kBM == 1 (two species): the plain sum of the
 // entries, to be scaled by the single basis difference after the reduction;
 // otherwise the change for basis differences dphi.
-template <int kR, int kBM, bool kIL = false, typename TabPtr>
+template <int kR, int kBM, int kIL = 0, typename TabPtr>
 __device__ __attribute__((always_inline)) inline double table_sum(const int (&gi)[kR], TabPtr gtab,
                                                                   const double (&dphi)[kBM]) {
   constexpr int kCodes = TableGeom<kBM>::kCodes;
-  static_assert(!kIL || kBM == 2, "the interleaved table holds two slices");
+  static_assert(kIL == 0 || kIL == kBM, "the interleaved table holds the read slices");
   double g[kR][kBM];
 #pragma unroll
   for (int k = 0; k < kR; ++k) {
-    if constexpr (kIL) {
+    if constexpr (kIL == 2) {
       const dbl2_t t = load_pair(gtab + gi[k]);
       g[k][0] = t.x;
       g[k][kBM - 1] = t.y;
+    } else if constexpr (kIL == 3) {
+      // three adjacent values (24 bytes): one or two cache lines where the
+      // plain layout touches three
+#pragma unroll
+      for (int d = 0; d < kBM; ++d) g[k][d] = gtab[gi[k] + d];
     } else {
 #pragma unroll
       for (int d = 0; d < kBM; ++d) g[k][d] = gtab[gi[k] + d * kCodes];
@@ -227,7 +242,7 @@ __device__ __attribute__((always_inline)) inline double table_sum(const int (&gi
 
 // Records of one translation group read from LDS in batches of 4 per lane
 // (thread `tid` of `nthreads` working on the chain).
-template <bool kPeriodic, int kBM, bool kPatch, bool kIL = false, typename TabPtr>
+template <bool kPeriodic, int kBM, bool kPatch, int kIL = 0, typename TabPtr>
 __device__ __attribute__((always_inline)) inline double lds_flip_de(const SweepCtx& c, const FatRec* recs, int off,
                                                                     int cnt, int tid, int nthreads, const Ref& ref,
                                                                     const uint8_t* occ, TabPtr gtab,
@@ -382,6 +397,7 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
                                                    HookDev* __restrict__ hooks, int64_t time_limit) {
   static_assert(!kHooks || (kW == 1 && kK == 1), "constraints and bias run one wave, one move at a time");
   static_assert(kNM == 3 || kR > 0, "records read from LDS gather three members");
+  constexpr int kILW = il_width<kBM, kGtabLds>();
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ double dphi_tab[kMaxSpecies * kMaxSpecies * kMaxBasis];  // [(old*S+new)*B + d]
   __shared__ double dsingle_tab[kMaxSpecies * kMaxSpecies];          // singlet energy change per (old,new)
@@ -455,8 +471,8 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
   if constexpr (kRecLds)
     for (int i = tid; i < m.n_rec; i += kThreads) recs_l[i] = m.frecs[i];
   if constexpr (kGtabLds)
-    for (int i = tid; i < m.gtab_size; i += kThreads) gtab_l[i] = (kBM == 2 ? m.gtab_il : m.gtab)[i];
-  const auto gtab = table_ptr<kGtabLds>(gtab_l, kBM == 2 ? m.gtab_il : m.gtab);
+    for (int i = tid; i < m.gtab_size; i += kThreads) gtab_l[i] = (kILW ? m.gtab_il : m.gtab)[i];
+  const auto gtab = table_ptr<kGtabLds>(gtab_l, kILW ? m.gtab_il : m.gtab);
   if (tid < 2 * kMaxGroups) grp_tab[tid] = tid < kMaxGroups ? m.grp_rec_off[tid] : m.grp_rec_cnt[tid - kMaxGroups];
   for (int i = tid; i < c.B * c.S; i += kThreads) phi_tab[i] = m.basis[i];
   for (int i = tid; i < c.S * c.S; i += kThreads) {
@@ -714,9 +730,9 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
       if constexpr (!kCanonical && !kHooks) old_i = occ[ri.p];
       if constexpr (kR > 0) {
         // all member gathers of the move are issued together
-        table_index<kPeriodic, kR, kBM, false, kNM, kBM == 2>(c, rr, ri, occ, 0u, 0u, gi_i);
+        table_index<kPeriodic, kR, kBM, false, kNM, kILW>(c, rr, ri, occ, 0u, 0u, gi_i);
         if constexpr (kCanonical)
-          table_index<kPeriodic, kR, kBM, true, kNM, kBM == 2>(c, rr, rj, occ, patch_j, new_i, gi_j);
+          table_index<kPeriodic, kR, kBM, true, kNM, kILW>(c, rr, rj, occ, patch_j, new_i, gi_j);
       }
       if constexpr (!kCanonical && !kHooks) {
         new_i = old_i + 1u + __builtin_amdgcn_readlane(v_new, l);
@@ -746,20 +762,20 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
       for (int d = 0; d < kBM; ++d) dphj[d] = -dphi[d];
       double part;
       if constexpr (kR > 0) {
-        part = table_sum<kR, kBM, kBM == 2>(gi_i, gtab, dphi);
+        part = table_sum<kR, kBM, kILW>(gi_i, gtab, dphi);
         if constexpr (kCanonical) {
           // second flip of the swap: j goes b -> a and sees site i already at b
           // (ce_updater.cpp:845-852 runs update_cf twice in sequence)
-          const double pj = table_sum<kR, kBM, kBM == 2>(gi_j, gtab, dphj);
+          const double pj = table_sum<kR, kBM, kILW>(gi_j, gtab, dphj);
           part = kBM == 1 ? part - pj : part + pj;
         }
       } else {
         const int gi = multi_group ? int(m.site_group[si]) : 0;
-        part = lds_flip_de<kPeriodic, kBM, false, kBM == 2>(c, recs_l, grp_tab[gi], grp_tab[kMaxGroups + gi], tid, kThreads,
+        part = lds_flip_de<kPeriodic, kBM, false, kILW>(c, recs_l, grp_tab[gi], grp_tab[kMaxGroups + gi], tid, kThreads,
                                                   ri, occ, gtab, dphi, 0u, 0u);
         if constexpr (kCanonical) {
           const int gj = multi_group ? int(m.site_group[sj]) : 0;
-          const double pj = lds_flip_de<kPeriodic, kBM, true, kBM == 2>(c, recs_l, grp_tab[gj], grp_tab[kMaxGroups + gj], tid,
+          const double pj = lds_flip_de<kPeriodic, kBM, true, kILW>(c, recs_l, grp_tab[gj], grp_tab[kMaxGroups + gj], tid,
                                                               kThreads, rj, occ, gtab, dphj, patch_j, new_i);
           part = kBM == 1 ? part - pj : part + pj;
         }
@@ -987,8 +1003,8 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
             for (int d = 0; d < kBM; ++d) dphi[d] = d < c.B ? dphi_tab[pair * c.B + d] : 0.0;
           }
           int gi_i[kR];
-          table_index<true, kR, kBM, false, kNM, kBM == 2>(c, rr, ri, occ, 0u, 0u, gi_i);
-          double pk = table_sum<kR, kBM, kBM == 2>(gi_i, gtab, dphi);
+          table_index<true, kR, kBM, false, kNM, kILW>(c, rr, ri, occ, 0u, 0u, gi_i);
+          double pk = table_sum<kR, kBM, kILW>(gi_i, gtab, dphi);
           if constexpr (kCanonical) {
             Ref rj;
             rj.p = pj_k;
@@ -998,9 +1014,9 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
 #pragma unroll
             for (int d = 0; d < kBM; ++d) dphj[d] = -dphi[d];
             int gi_j[kR];
-            table_index<true, kR, kBM, true, kNM, kBM == 2>(c, rr, rj, occ, __builtin_amdgcn_readlane(patch, lk),
+            table_index<true, kR, kBM, true, kNM, kILW>(c, rr, rj, occ, __builtin_amdgcn_readlane(patch, lk),
                                                   __builtin_amdgcn_readlane(n_v, lk), gi_j);
-            const double pj = table_sum<kR, kBM, kBM == 2>(gi_j, gtab, dphj);
+            const double pj = table_sum<kR, kBM, kILW>(gi_j, gtab, dphj);
             pk = kBM == 1 ? pk - pj : pk + pj;
           }
           part[k] = pk;
@@ -1212,8 +1228,8 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
             for (int d = 0; d < kBM; ++d) dphi[d] = d < c.B ? dphi_tab[pair * c.B + d] : 0.0;
           }
           int gi_i[kR];
-          table_index<true, kR, kBM, false, kNM, kBM == 2>(c, rr, ri, occ, 0u, 0u, gi_i);
-          double pk = table_sum<kR, kBM, kBM == 2>(gi_i, gtab, dphi);
+          table_index<true, kR, kBM, false, kNM, kILW>(c, rr, ri, occ, 0u, 0u, gi_i);
+          double pk = table_sum<kR, kBM, kILW>(gi_i, gtab, dphi);
           if constexpr (kCanonical) {
             Ref rj;
             rj.p = pj_k;
@@ -1223,9 +1239,9 @@ __global__ __launch_bounds__(64 * kW) void sweep_kernel(const ChainParams* __res
 #pragma unroll
             for (int d = 0; d < kBM; ++d) dphj[d] = -dphi[d];
             int gi_j[kR];
-            table_index<true, kR, kBM, true, kNM, kBM == 2>(c, rr, rj, occ, __builtin_amdgcn_readlane(patch, lk),
+            table_index<true, kR, kBM, true, kNM, kILW>(c, rr, rj, occ, __builtin_amdgcn_readlane(patch, lk),
                                                   __builtin_amdgcn_readlane(n_v, lk), gi_j);
-            const double pj = table_sum<kR, kBM, kBM == 2>(gi_j, gtab, dphj);
+            const double pj = table_sum<kR, kBM, kILW>(gi_j, gtab, dphj);
             pk = kBM == 1 ? pk - pj : pk + pj;
           }
           part[k] = pk;

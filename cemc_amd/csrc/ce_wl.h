@@ -91,6 +91,7 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
                                                 int64_t check_every, uint32_t* __restrict__ accept_log,
                                                 int64_t log_words, const WLDev* __restrict__ shared,
                                                 int64_t log_base) {
+  constexpr int kILW = il_width<kBM, kGtabLds>();
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ double dphi_tab[kMaxSpecies * kMaxSpecies * kMaxBasis];  // [(old*S+new)*B + d]
   __shared__ double dsingle_tab[kMaxSpecies * kMaxSpecies];          // SGC: singlet energy change per (old,new)
@@ -183,8 +184,8 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
   if constexpr (kRecLds)
     for (int i = lane; i < m.n_rec; i += 64) recs_l[i] = m.frecs[i];
   if constexpr (kGtabLds)
-    for (int i = lane; i < m.gtab_size; i += 64) gtab_l[i] = (kBM == 2 ? m.gtab_il : m.gtab)[i];
-  const auto gtab = table_ptr<kGtabLds>(gtab_l, kBM == 2 ? m.gtab_il : m.gtab);
+    for (int i = lane; i < m.gtab_size; i += 64) gtab_l[i] = (kILW ? m.gtab_il : m.gtab)[i];
+  const auto gtab = table_ptr<kGtabLds>(gtab_l, kILW ? m.gtab_il : m.gtab);
   if (lane < 2 * kMaxGroups) grp_tab[lane] = lane < kMaxGroups ? m.grp_rec_off[lane] : m.grp_rec_cnt[lane - kMaxGroups];
   for (int i = lane; i < c.S * c.S; i += 64) {
     const int o = i / c.S, nw = i - o * c.S;
@@ -332,19 +333,19 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
     double part;
     if constexpr (kR > 0) {
       int gi_i[kR], gi_j[kR];
-      table_index<kPeriodic, kR, kBM, false, kNM, kBM == 2>(c, rr, ri, occ, 0u, 0u, gi_i);
-      table_index<kPeriodic, kR, kBM, true, kNM, kBM == 2>(c, rr, rj, occ, patch_j, new_i, gi_j);
-      part = table_sum<kR, kBM, kBM == 2>(gi_i, gtab, dphi);
+      table_index<kPeriodic, kR, kBM, false, kNM, kILW>(c, rr, ri, occ, 0u, 0u, gi_i);
+      table_index<kPeriodic, kR, kBM, true, kNM, kILW>(c, rr, rj, occ, patch_j, new_i, gi_j);
+      part = table_sum<kR, kBM, kILW>(gi_i, gtab, dphi);
       // the swap's second flip sees site i already changed
       // (ce_updater.cpp:845-852)
-      const double pj = table_sum<kR, kBM, kBM == 2>(gi_j, gtab, dphj);
+      const double pj = table_sum<kR, kBM, kILW>(gi_j, gtab, dphj);
       part = kBM == 1 ? part - pj : part + pj;
     } else {
       const int gi = multi_group ? int(m.site_group[mv.si]) : 0;
       const int gj = multi_group ? int(m.site_group[mv.sj]) : 0;
-      part = lds_flip_de<kPeriodic, kBM, false, kBM == 2>(c, recs_l, grp_tab[gi], grp_tab[kMaxGroups + gi], lane, 64, ri,
+      part = lds_flip_de<kPeriodic, kBM, false, kILW>(c, recs_l, grp_tab[gi], grp_tab[kMaxGroups + gi], lane, 64, ri,
                                                 occ, gtab, dphi, 0u, 0u);
-      const double pj = lds_flip_de<kPeriodic, kBM, true, kBM == 2>(c, recs_l, grp_tab[gj], grp_tab[kMaxGroups + gj],
+      const double pj = lds_flip_de<kPeriodic, kBM, true, kILW>(c, recs_l, grp_tab[gj], grp_tab[kMaxGroups + gj],
                                                           lane, 64, rj, occ, gtab, dphj, patch_j, new_i);
       part = kBM == 1 ? part - pj : part + pj;
     }
@@ -378,11 +379,11 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
     double part;
     if constexpr (kR > 0) {
       int gi_i[kR];
-      table_index<kPeriodic, kR, kBM, false, kNM, kBM == 2>(c, rr, ri, occ, 0u, 0u, gi_i);
-      part = table_sum<kR, kBM, kBM == 2>(gi_i, gtab, dphi);
+      table_index<kPeriodic, kR, kBM, false, kNM, kILW>(c, rr, ri, occ, 0u, 0u, gi_i);
+      part = table_sum<kR, kBM, kILW>(gi_i, gtab, dphi);
     } else {
       const int gi = multi_group ? int(m.site_group[site]) : 0;
-      part = lds_flip_de<kPeriodic, kBM, false, kBM == 2>(c, recs_l, grp_tab[gi], grp_tab[kMaxGroups + gi], lane, 64,
+      part = lds_flip_de<kPeriodic, kBM, false, kILW>(c, recs_l, grp_tab[gi], grp_tab[kMaxGroups + gi], lane, 64,
                                                           ri, occ, gtab, dphi, 0u, 0u);
     }
     const double sum = wave_sum_f64(part);
