@@ -222,19 +222,19 @@ uint32_t updater_lds_bytes(int n_cf, int chunk, int occ_sites) {
          uint32_t(sizeof(double)) * uint32_t(chunk) + uint32_t((occ_sites + 15) / 16 * 16);
 }
 
-// kInArgs: the changes travel in the kernel arguments (else in `changes`);
-// kOccLds: the occupation is staged in LDS; kPeriodic: closed-form periodic
-// lattice (else members come from the translation table).  With the occupation in LDS the
-// workgroup exchanges data through LDS only, so its barriers wait for LDS
-// operations alone and global loads and stores stay in flight across them
-// (a full barrier waits for every load and store of the wave, one memory
-// round trip); otherwise the occupation written for a change is read back
-// from global memory by the next change, which needs the full barrier.
-// A change in the translation group of the change before it reuses that
-// change's items and segments (every swap on a one-group lattice).
 // s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima).
 __device__ inline void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// kInArgs: the changes travel in the kernel arguments (else in `changes`);
+// kOccLds: the occupation is staged in LDS; kPeriodic: closed-form periodic
+// lattice (else members come from the translation table).  With the
+// occupation in LDS the workgroup exchanges data through LDS only, so its
+// barriers wait for LDS operations alone and global loads and stores stay in
+// flight across them (a full barrier waits for every load and store of the
+// wave, one memory round trip); otherwise the occupation written for a change
+// is read back from global memory by the next change, which needs the full
+// barrier.  A change in the translation group of the change before it reuses
+// that change's items and segments (every swap on a one-group lattice).
 template <bool kInArgs, bool kOccLds, bool kPeriodic>
 __global__ __launch_bounds__(256) void updater_kernel(UpdPlan plan, int8_t* occ, double* __restrict__ ring,
                                                       double* __restrict__ slot_energy, UpdChanges inl,
@@ -404,6 +404,10 @@ __global__ __launch_bounds__(256) void updater_kernel(UpdPlan plan, int8_t* occ,
       }
       if (!last_pass) barrier();
     }
+    // a change without items (a group whose correlation functions are all
+    // singlets) has had no barrier since the history-slot stores of the
+    // change before read `cfl`
+    if (ib == ie) barrier();
     // thread j updates singlet j (fixed indices: the list is read from the
     // kernel arguments with scalar loads)
 #pragma unroll

@@ -205,7 +205,7 @@ def _allowed_species(s):
     return out
 
 
-def _replay_random_calls(s, occ, steps, seed, cf0=None):
+def _replay_random_calls(s, occ, steps, seed, cf0=None, eci=None):
     """Random updater calls on the GPU and on the oracle: single flips, swaps
     of unlike sites of one sublattice, and twelve flips in one call (more
     changes than travel in the kernel arguments); undo or clear after each.
@@ -215,8 +215,9 @@ def _replay_random_calls(s, occ, steps, seed, cf0=None):
     from cemc_amd.updater import PyCEUpdater
     from oracle.brute_force import brute_force_cf
     from oracle.oracle import OracleModel, OracleUpdater
-    names = s.cf_names()
-    eci = seeded_ecis(s, 4, 0.01)
+    if eci is None:
+        eci = seeded_ecis(s, 4, 0.01)
+    names = sorted(eci)
     sp = list(s.species)
     upd = PyCEUpdater(SimpleAtoms([sp[v] for v in occ]), s, cf0 if cf0 is None else dict(cf0), eci)
     if cf0 is None:
@@ -292,6 +293,17 @@ def test_updater_matches_oracle_on_crystals_and_long_change_lists(gpu, kind):
     s = _crystal(kind)
     print("updater on", kind, s.n_sites, "sites", flush=True)
     _replay_random_calls(s, random_crystal_occupation(s, 5), 45, 11)
+
+
+def test_updater_with_singlet_correlation_functions_only(gpu):
+    """ECIs on c0 and the singlets only: a change touches no cluster
+    correlation function (no plan items), in calls of up to twelve
+    changes."""
+    from cemc_amd.lattice import FCCSettings, random_occupation
+    s = FCCSettings(4, ["Al", "Mg", "Si"], {2: 1})
+    print("updater with singlets only", flush=True)
+    _replay_random_calls(s, random_occupation(s.n_sites, 3, 7), 30, 13,
+                         eci={"c0": 0.05, "c1_0": 0.3, "c1_1": -0.2})
 
 
 def test_updater_on_a_lattice_larger_than_lds(gpu):
