@@ -270,7 +270,8 @@ class ChainBatch:
     # ------------------------------------------------------- Wang-Landau
     def wl_set(self, chain, n_bins, e_min, e_max, f, f_min=1e-6, flatness=0.8, current_bin=0, is_first=True,
                converged=False, logdos=None, hist=None, known=None):
-        """Give canonical chain `chain` a Wang-Landau walker (cemc_batch_wl_set)."""
+        """Give canonical or semi-grand-canonical chain `chain` a Wang-Landau walker
+        (cemc_batch_wl_set)."""
         d = _capi.WLDesc(int(n_bins), int(current_bin), int(bool(is_first)), int(bool(converged)), float(e_min),
                          float(e_max), float(f), float(f_min), float(flatness))
         arrs = []

@@ -677,6 +677,42 @@ def test_gpu_sgc_walkers_match_oracle(gpu, sgc_cell, case):
 
 
 @pytest.mark.gpu
+def test_gpu_sgc_walkers_on_a_shared_histogram_add_their_visits(gpu, sgc_cell):
+    """Semi-grand-canonical walkers on one histogram with f = 0 and flatness
+    never reached: each walker flips as its own oracle walker does, and the
+    shared counts are the sum of theirs."""
+    from cemc_amd.batch import SGC, ChainBatch
+    from cemc_amd.model import CEModel
+    c = sgc_cell
+    g = c["g"]
+    lg = np.log(np.where(g > 0, g, 1.0))
+    ks = [100, 700, 1500, 2900]
+    W = len(ks)
+    batch = ChainBatch([CEModel(c["s"], dict(c["eci"]))] * W,
+                       [dict(kind=SGC, temperature=1000.0, occupation=c["states"][k], seed=6, uid=k,
+                             chem_pot={"c1_0": 0.0}) for k in ks])
+    for w in range(W):
+        batch.wl_set(w, NB_S, EMIN_S, EMAX_S, 0.0, f_min=-1.0, flatness=2.0, logdos=lg if w == 0 else None)
+    per, n_checks = 300, 20
+    print("shared histogram, SGC,", W, "walkers", flush=True)
+    log, _ = batch.wl_run(n_checks, per * W, log=True, shared=True)
+    hist = np.zeros(NB_S, dtype=np.int64)
+    for w, k in enumerate(ks):
+        occ = c["states"][k]
+        ora = OracleUpdater(c["model"], occ, brute_force_cf(c["s"], occ, c["names"]), c["eci"])
+        wl = OracleWL(NB_S, EMIN_S, EMAX_S, 0.0, f_min=-1.0, flatness=2.0, logdos=lg, kind=1)
+        ref = wl_run(ora, wl, 6, k, n_checks, per)
+        n = n_checks * per
+        assert np.array_equal(np.unpackbits(log[w], bitorder="little")[:n],
+                              np.unpackbits(ref, bitorder="little")[:n]), w
+        assert np.array_equal(batch.occupation(w), ora.get_occupation())
+        hist += wl.hist
+    got = batch.wl_get(0)
+    assert np.array_equal(got["hist"], hist)
+    assert np.array_equal(got["logdos"], lg)
+
+
+@pytest.mark.gpu
 def test_python_driver_sgc_converges_to_exact_dos(gpu, sgc_cell):
     """WangLandau in the semi-grand-canonical ensemble with the CE
     calculator: eight walkers; the averaged ln g matches the enumerated one
