@@ -19,6 +19,9 @@ from . import _capi
 from .model import CEModel
 
 
+_MAX_UNDO = 999
+
+
 class PyCEUpdater:
     def __init__(self, atoms, bc, corr_func, eci, model=None):
         lib = _capi.load()
@@ -53,6 +56,11 @@ class PyCEUpdater:
         self.handle = h
         self._lib = lib
         self._occ = occ.copy()
+        # changes in the device's history since the last clear, as
+        # (site, old symbol, new symbol): an undo restores exactly these on
+        # the atoms object; None when unknown (after an error), and then the
+        # atoms object is compared with the lattice instead
+        self._pending = []
 
     # --------------------------------------------------------------- helpers
     def _ids(self, changes):
@@ -77,15 +85,43 @@ class PyCEUpdater:
     # ------------------------------------------------------------- interface
     def clear_history(self):
         _capi.check(self._lib.cemc_updater_clear_history(self.handle))
+        self._pending = []
 
     def undo_changes(self):
         _capi.check(self._lib.cemc_updater_undo_changes(self.handle))
-        self._sync_atoms()
+        pending = self._pending
+        # the history holds at most 999 changes (ce_updater's ring of 1000 slots)
+        if pending is None or len(pending) > _MAX_UNDO:
+            self._sync_atoms()
+        else:
+            atoms, occ, sid = self.atoms, self._occ, self.model.sid
+            for i, old, _ in reversed(pending):
+                atoms[i].symbol = old
+                occ[i] = sid[old]
+        self._pending = []
+
+    def _applied(self, changes):
+        """Mirror changes the device made (no-op changes, old == new, are
+        not recorded, ce_updater.cpp:315-318) on the atoms object and in the
+        pending list."""
+        atoms, occ, sid, pending = self.atoms, self._occ, self.model.sid, self._pending
+        for c in changes:
+            if c[1] == c[2]:
+                continue
+            i = int(c[0])
+            atoms[i].symbol = c[2]
+            occ[i] = sid[c[2]]
+            if pending is not None:
+                pending.append((i, c[1], c[2]))
 
     def update_cf(self, system_changes):
         site, old, new = self._ids([system_changes])
         rc = self._lib.cemc_updater_update_cf(self.handle, int(site[0]), int(old[0]), int(new[0]))
-        self._sync_atoms()
+        if rc == _capi.CEMC_OK:
+            self._applied([system_changes])
+        else:
+            self._pending = None
+            self._sync_atoms()
         _capi.check(rc)
 
     def calculate(self, system_changes):
@@ -93,10 +129,12 @@ class PyCEUpdater:
         n = len(changes)
         sid = self.model.sid
         arr = ctypes.c_int32 * n
+        site, old, new = arr(), arr(), arr()
         try:
-            site = arr(*[int(c[0]) for c in changes])
-            old = arr(*[sid[c[1]] for c in changes])
-            new = arr(*[sid[c[2]] for c in changes])
+            for k, c in enumerate(changes):
+                site[k] = int(c[0])
+                old[k] = sid[c[1]]
+                new[k] = sid[c[2]]
         except KeyError as exc:
             raise ValueError("unknown symbol %s" % exc)
         e = ctypes.c_double(0.0)
@@ -104,13 +142,9 @@ class PyCEUpdater:
         if rc == _capi.CEMC_OK:
             # the changes were made as given: set the same symbols in atoms
             # (ce_updater.cpp:338-347) without reading the lattice back
-            atoms, occ = self.atoms, self._occ
-            for c in changes:
-                i = int(c[0])
-                if occ[i] != sid[c[2]]:
-                    atoms[i].symbol = c[2]
-                    occ[i] = sid[c[2]]
+            self._applied(changes)
         else:
+            self._pending = None
             self._sync_atoms()
         _capi.check(rc)
         return e.value
@@ -162,6 +196,7 @@ class PyCEUpdater:
             raise ValueError("Length of the symbols array has to match the length of the atoms object.!")
         rc = self._lib.cemc_updater_set_occupation(self.handle, occ.ctypes.data_as(_capi._i8p))
         _capi.check(rc)
+        self._pending = []  # the history is cleared
         self._sync_atoms()
 
     def set_symbols(self, symbols):

@@ -219,7 +219,8 @@ def _replay_random_calls(s, occ, steps, seed, cf0=None, eci=None):
         eci = seeded_ecis(s, 4, 0.01)
     names = sorted(eci)
     sp = list(s.species)
-    upd = PyCEUpdater(SimpleAtoms([sp[v] for v in occ]), s, cf0 if cf0 is None else dict(cf0), eci)
+    atoms = SimpleAtoms([sp[v] for v in occ])
+    upd = PyCEUpdater(atoms, s, cf0 if cf0 is None else dict(cf0), eci)
     if cf0 is None:
         cf0 = brute_force_cf(s, occ, names) if s.n_sites <= 20000 else upd.get_cf()
     ora = OracleUpdater(OracleModel(s, names), occ, cf0, eci)
@@ -263,6 +264,7 @@ def _replay_random_calls(s, occ, steps, seed, cf0=None, eci=None):
             ora.clear_history()
         cur = [sp.index(v) for v in upd.get_symbols()]
         assert cur == [int(v) for v in ora.get_occupation()], step
+        assert [a.symbol for a in atoms] == upd.get_symbols(), step
         assert np.max(np.abs(upd.get_cf_array() - ora.get_cf_array())) <= 1e-12, step
         assert abs(upd.get_energy() - ora.get_energy()) / s.n_sites <= 1e-12, step
     assert n_long > 0
@@ -293,6 +295,39 @@ def test_updater_matches_oracle_on_crystals_and_long_change_lists(gpu, kind):
     s = _crystal(kind)
     print("updater on", kind, s.n_sites, "sites", flush=True)
     _replay_random_calls(s, random_crystal_occupation(s, 5), 45, 11)
+
+
+def test_undo_after_more_changes_than_the_history_holds(gpu):
+    """1005 single-site changes without a clear: the history keeps the last
+    999 (the ring of 1000 slots), an undo restores the state after the
+    sixth change on the device, in the atoms object and in the oracle."""
+    from cemc_amd.lattice import FCCSettings, SimpleAtoms, random_occupation, seeded_ecis
+    from cemc_amd.updater import PyCEUpdater
+    from oracle.brute_force import brute_force_cf
+    from oracle.oracle import OracleModel, OracleUpdater
+    s = FCCSettings(4, ["Al", "Mg"], {2: 2})
+    names = s.cf_names()
+    eci = seeded_ecis(s, 2, 0.01)
+    occ = random_occupation(s.n_sites, 2, 4)
+    atoms = SimpleAtoms([s.species[v] for v in occ])
+    cf0 = brute_force_cf(s, occ, names)
+    upd = PyCEUpdater(atoms, s, cf0, eci)
+    ora = OracleUpdater(OracleModel(s, names), occ, cf0, eci)
+    cur = [int(v) for v in occ]
+    rng = np.random.Generator(np.random.Philox(3))
+    for _ in range(1005):
+        i = int(rng.integers(s.n_sites))
+        ch = (i, s.species[cur[i]], s.species[1 - cur[i]])
+        cur[i] = 1 - cur[i]
+        upd.update_cf(ch)
+        ora.update_cf(ch)
+    assert upd.history_size() == ora.history_size()
+    upd.undo_changes()
+    ora.undo_changes()
+    assert [int(v) for v in upd.get_occupation()] == [int(v) for v in ora.get_occupation()]
+    assert [a.symbol for a in atoms] == upd.get_symbols()
+    assert np.max(np.abs(upd.get_cf_array() - ora.get_cf_array())) <= 1e-12
+    upd.close()
 
 
 def test_updater_with_singlet_correlation_functions_only(gpu):
