@@ -834,6 +834,11 @@ struct cemc_updater {
   double* h_energy = nullptr;
   UpdResult* h_result = nullptr;
   UpdResult* d_result = nullptr;
+  // the final CFs of the last call that returned an energy (bits, mapped
+  // like the result), valid for history slot `cf_slot` (-1: none)
+  uint64_t* h_cf = nullptr;
+  uint64_t* d_cf = nullptr;
+  int cf_slot = -1;
   int32_t* h_stage = nullptr;
   size_t stage_cap = 0;  // int32 entries at h_stage
   std::vector<int8_t> occ;  // host mirror of the device occupation
@@ -854,6 +859,7 @@ struct cemc_updater {
     if (stream) (void)hipStreamSynchronize(stream);
     if (h_energy) (void)hipHostFree(h_energy);
     if (h_result) (void)hipHostFree(h_result);
+    if (h_cf) (void)hipHostFree(h_cf);
     if (h_stage) (void)hipHostFree(h_stage);
     dev_free(d_occ);
     dev_free(d_ring);
@@ -986,10 +992,16 @@ void updater_flush(cemc_updater* u, const std::vector<int32_t>& packed, bool wan
     updater_stage_copy(u, u->d_changes, packed.data(), packed.size());
     d_list = u->d_changes;
   }
-  if (want_result) __atomic_store_n(&u->h_result->energy_bits, kUpdPending, __ATOMIC_SEQ_CST);
-  HIP_CHECK(launch_updater(u->plan, u->d_occ, u->d_ring, u->d_slot_energy, inl, d_list, n,
-                           u->d_eci, want_result ? u->d_result : nullptr, u->lds_bytes, u->stream));
-  u->slot_epoch[(packed[packed.size() - kUpdWords + 2] >> 16) & 0xFFFF] = u->eci_epoch;
+  const int dst = (packed[packed.size() - kUpdWords + 2] >> 16) & 0xFFFF;
+  if (want_result) {
+    for (int k = 0; k < u->model->n_cf; ++k) __atomic_store_n(&u->h_cf[k], kUpdPending, __ATOMIC_RELAXED);
+    __atomic_store_n(&u->h_result->energy_bits, kUpdPending, __ATOMIC_SEQ_CST);
+  }
+  HIP_CHECK(launch_updater(u->plan, u->d_occ, u->d_ring, u->d_slot_energy, inl, d_list, n, u->d_eci,
+                           want_result ? u->d_result : nullptr, want_result ? u->d_cf : nullptr, u->lds_bytes,
+                           u->stream));
+  u->cf_slot = want_result ? dst : -1;
+  u->slot_epoch[dst] = u->eci_epoch;
 }
 
 // Waits for the energy of the last flush with a result: spins on the mapped
@@ -1059,10 +1071,20 @@ int updater_stage(cemc_updater* u, int site, int old_id, int new_id, std::vector
   return CEMC_OK;
 }
 
-// Energy of the current slot: one copy into pinned memory on the stream
-// (after the queued updates) and one wait.
+// Energy of the current slot: when it is the slot the last call with an
+// energy ended in and the ECIs are unchanged since, that energy from the
+// mapped result; else one copy into pinned memory on the stream (after the
+// queued updates) and one wait.
 double updater_energy(cemc_updater* u) {
   const unsigned slot = u->current_slot();
+  if (u->cf_slot >= 0 && unsigned(u->cf_slot) == slot && u->slot_epoch[slot] == u->eci_epoch) {
+    const uint64_t bits = __atomic_load_n(&u->h_result->energy_bits, __ATOMIC_ACQUIRE);
+    if (bits != kUpdPending) {
+      double e;
+      std::memcpy(&e, &bits, sizeof(double));
+      return e;
+    }
+  }
   if (u->slot_epoch[slot] != u->eci_epoch) {
     HIP_CHECK(launch_slot_energy(u->model->d_view, u->d_ring, u->d_slot_energy, u->d_eci, int(slot), u->stream));
     u->slot_epoch[slot] = u->eci_epoch;
@@ -1072,10 +1094,35 @@ double updater_energy(cemc_updater* u) {
   return *u->h_energy;
 }
 
-// Correlation functions of the current slot (n_cf values).
+// Correlation functions of the current slot (n_cf values): from the mapped
+// copy when the current slot is the one the last call with an energy ended
+// in (each value waited for until the launch has stored it), else one copy
+// from the device.
 void updater_current_cf(cemc_updater* u, double* cf) {
-  HIP_CHECK(hipMemcpyAsync(cf, u->d_ring + size_t(u->current_slot()) * u->model->n_cf,
-                           sizeof(double) * u->model->n_cf, hipMemcpyDeviceToHost, u->stream));
+  const int ncf = u->model->n_cf;
+  if (u->cf_slot >= 0 && unsigned(u->cf_slot) == u->current_slot()) {
+    bool ok = true;
+    for (int k = 0; k < ncf && ok; ++k) {
+      uint64_t bits = __atomic_load_n(&u->h_cf[k], __ATOMIC_ACQUIRE);
+      for (uint32_t spin = 1; bits == kUpdPending; ++spin) {
+        if ((spin & 4095u) == 0) {
+          // the launch has ended (its energy was read): what is still
+          // pending after the stream drains never comes
+          HIP_CHECK(hipStreamSynchronize(u->stream));
+          bits = __atomic_load_n(&u->h_cf[k], __ATOMIC_ACQUIRE);
+          if (bits == kUpdPending) ok = false;
+          break;
+        }
+        __builtin_ia32_pause();
+        bits = __atomic_load_n(&u->h_cf[k], __ATOMIC_ACQUIRE);
+      }
+      if (ok) std::memcpy(cf + k, &bits, sizeof(double));
+    }
+    if (ok) return;
+    u->cf_slot = -1;
+  }
+  HIP_CHECK(hipMemcpyAsync(cf, u->d_ring + size_t(u->current_slot()) * ncf, sizeof(double) * ncf,
+                           hipMemcpyDeviceToHost, u->stream));
   HIP_CHECK(hipStreamSynchronize(u->stream));
 }
 
@@ -1969,6 +2016,9 @@ int cemc_updater_create(cemc_model* model, const int8_t* occupation, const doubl
                               hipHostMallocMapped | hipHostMallocCoherent));
       std::memset(u->h_result, 0, sizeof(UpdResult));
       HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&u->d_result), u->h_result, 0));
+      HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&u->h_cf), sizeof(uint64_t) * size_t(ncf),
+                              hipHostMallocMapped | hipHostMallocCoherent));
+      HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&u->d_cf), u->h_cf, 0));
       updater_build_plan(u);
       std::vector<double> cf0(ncf);
       if (cf_init) {
@@ -2044,6 +2094,7 @@ int cemc_updater_set_occupation(cemc_updater* u, const int8_t* occupation) {
     u->current = 1;
     u->buffer_size = 1;
     u->slot_epoch.assign(kMaxHistory, -1);
+    u->cf_slot = -1;
   });
   return g != CEMC_OK ? g : rc;
 }

@@ -139,7 +139,8 @@ struct UpdPlan {
   double phi[kMaxBasis * kMaxSpecies];  // basis functions phi[d * n_species + s]
 };
 // Result of an updater launch in mapped, coherent host memory: the bits of
-// the energy, one 8-byte store at system scope.  The host sets them to
+// the energy, one 8-byte store at system scope (and, in a second buffer, the
+// bits of the final correlation functions, one store each).  The host sets them to
 // kUpdPending (a signalling NaN that no arithmetic result carries) before the
 // launch and waits until they change, so no fence or cache write-back is
 // needed on the device.
@@ -163,7 +164,8 @@ struct UpdChanges {
 uint32_t updater_lds_bytes(int n_cf, int chunk, int occ_sites);
 hipError_t launch_updater(const UpdPlan& plan, int8_t* occ, double* ring,
                           double* slot_energy, const UpdChanges& inl, const int32_t* changes, int n,
-                          const double* eci, UpdResult* result, uint32_t lds_bytes, hipStream_t stream);
+                          const double* eci, UpdResult* result, uint64_t* result_cf, uint32_t lds_bytes,
+                          hipStream_t stream);
 // Restores sites[i] = species[i] in order; n <= 2 * kInlineChanges pairs
 // come in `inl` (sites in v[0..n), species in v[n..2n)), more from `buf`
 // (sites then species).

@@ -239,7 +239,8 @@ template <bool kInArgs, bool kOccLds, bool kPeriodic>
 __global__ __launch_bounds__(256) void updater_kernel(UpdPlan plan, int8_t* occ, double* __restrict__ ring,
                                                       double* __restrict__ slot_energy, UpdChanges inl,
                                                       const int32_t* __restrict__ changes, int n,
-                                                      const double* __restrict__ eci, UpdResult* result) {
+                                                      const double* __restrict__ eci, UpdResult* result,
+                                                      uint64_t* result_cf) {
   extern __shared__ double upd_lds[];
   __shared__ double phi[kMaxBasis * kMaxSpecies];
   __shared__ double red[4];
@@ -426,6 +427,13 @@ __global__ __launch_bounds__(256) void updater_kernel(UpdPlan plan, int8_t* occ,
     for (int k = tid; k < ncf; k += 256) ring[int64_t(dst) * ncf + k] = cfl[k];
   }
   if (last_dst < 0) return;
+  // the final correlation functions to mapped host memory; the host marks
+  // every value pending before the launch and waits for each, so these
+  // stores need no ordering against the energy
+  if (result_cf != nullptr)
+    for (int k = tid; k < ncf; k += 256)
+      __hip_atomic_store(&result_cf[k], uint64_t(__double_as_longlong(cfl[k])), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
   double part = 0.0;
   for (int k = tid; k < ncf; k += 256) part += ecil[k] * cfl[k];
   part = wave_sum_f64(part);
@@ -787,7 +795,8 @@ hipError_t launch_energy(const ModelDev* models, const ChainParams* params, cons
 template <bool kInArgs, bool kOccLds, bool kPeriodic>
 static hipError_t launch_updater_t(const UpdPlan& plan, int8_t* occ, double* ring, double* slot_energy,
                                    const UpdChanges& inl, const int32_t* changes, int n, const double* eci,
-                                   UpdResult* result, uint32_t lds_bytes, hipStream_t stream) {
+                                   UpdResult* result, uint64_t* result_cf, uint32_t lds_bytes,
+                                   hipStream_t stream) {
   auto* fn = updater_kernel<kInArgs, kOccLds, kPeriodic>;
   if (lds_bytes > 64u * 1024u) {
     static std::mutex mu;
@@ -801,15 +810,18 @@ static hipError_t launch_updater_t(const UpdPlan& plan, int8_t* occ, double* rin
     }
   }
   hipLaunchKernelGGL(fn, dim3(1), dim3(256), lds_bytes, stream, plan, occ, ring, slot_energy, inl, changes, n, eci,
-                     result);
+                     result, result_cf);
   return hipGetLastError();
 }
 
 hipError_t launch_updater(const UpdPlan& plan, int8_t* occ, double* ring,
                           double* slot_energy, const UpdChanges& inl, const int32_t* changes, int n,
-                          const double* eci, UpdResult* result, uint32_t lds_bytes, hipStream_t stream) {
+                          const double* eci, UpdResult* result, uint64_t* result_cf, uint32_t lds_bytes,
+                          hipStream_t stream) {
   const bool a = n <= kUpdInline, o = plan.occ_lds != 0, p = plan.periodic != 0;
-  auto go = [&](auto fn) { return fn(plan, occ, ring, slot_energy, inl, changes, n, eci, result, lds_bytes, stream); };
+  auto go = [&](auto fn) {
+    return fn(plan, occ, ring, slot_energy, inl, changes, n, eci, result, result_cf, lds_bytes, stream);
+  };
   if (a) {
     if (o) return p ? go(launch_updater_t<true, true, true>) : go(launch_updater_t<true, true, false>);
     return p ? go(launch_updater_t<true, false, true>) : go(launch_updater_t<true, false, false>);
