@@ -514,19 +514,26 @@ def test_gpu_shared_histogram_converges_to_exact_dos(gpu, cell):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("crystal", ["hcp_ternary", "sg217_binary"])
+@pytest.mark.parametrize("crystal", ["hcp_ternary", "sg217_binary", "fcc_quaternary_tables_in_hbm"])
 def test_gpu_scheme_walkers_on_crystals(gpu, crystal):
     """Adaptive windows with the inverse-time law on crystals with sites
     not related by translations: HCP (two sites per cell, one translation
     group, ternary) and space group I-43m (29 translation groups, records
-    read from LDS); walkers equal the oracle bit for bit."""
+    read from LDS); and on quaternary FCC 8^3 with clusters up to 4-body to
+    the 3rd shell, whose energy tables (three values per code side by side)
+    are too large for LDS; walkers equal the oracle bit for bit."""
     from cemc_amd.batch import CANONICAL, ChainBatch
     from cemc_amd.crystal import HCPSettings, Spacegroup217Settings
     from cemc_amd.lattice import random_occupation, seeded_ecis
     from cemc_amd.model import CEModel
+    n_checks, per = 40, 800
     if crystal == "hcp_ternary":
         s = HCPSettings((4, 4, 3), ["Al", "Mg", "Si"], {2: 1.45, 3: 1.05})
         n_sp = 3
+    elif crystal == "fcc_quaternary_tables_in_hbm":
+        s = FCCSettings(8, ["A", "B", "C", "D"], {2: 3, 3: 3, 4: 3})
+        n_sp = 4
+        n_checks, per = 8, 400  # 855 correlation functions: the oracle's updates are slow
     else:
         s = Spacegroup217Settings(1, 10.553, [(0, 0, 0), (0.324, 0.324, 0.324), (0.3582, 0.3582, 0.0393),
                                               (0.0954, 0.0954, 0.2725)], [["Al", "Mg"]] * 4, {2: 4.1, 3: 4.1})
@@ -542,16 +549,16 @@ def test_gpu_scheme_walkers_on_crystals(gpu, crystal):
     wl = OracleWL(nb, lo, lo + 2.0 * half, 1.0, f_min=1e-3, flatness=0.6, cur_bin=20, inverse_time=True,
                   adaptive=True, min_width=3, n_sites=s.n_sites, n_cf=len(names))
     used = []
-    ref = wl_run(ora, wl, 3, 2, 40, 800, counters=used)
+    ref = wl_run(ora, wl, 3, 2, n_checks, per, counters=used)
     batch = ChainBatch([CEModel(s, eci)], [dict(kind=CANONICAL, temperature=1000.0, occupation=occ, seed=3, uid=2)])
     batch.wl_set(0, nb, lo, lo + 2.0 * half, 1.0, f_min=1e-3, flatness=0.6, current_bin=20)
     batch.wl_set_scheme(0, inverse_time=True, adaptive=True, min_width=3)
     print("scheme walker on", crystal, flush=True)
-    log, _ = batch.wl_run(40, 800, log=True)
+    log, _ = batch.wl_run(n_checks, per, log=True)
     got = batch.wl_get(0)
     sch = batch.wl_scheme(0)
     st = wl.state
-    n = min(used[0], 40 * 800)
+    n = min(used[0], n_checks * per)
     assert np.array_equal(np.unpackbits(log[0], bitorder="little")[:n], np.unpackbits(ref, bitorder="little")[:n])
     assert np.array_equal(got["hist"], wl.hist) and np.array_equal(got["logdos"], wl.logdos)
     assert got["f"] == st.f and got["current_bin"] == st.cur_bin and got["n_flat"] == st.n_flat
