@@ -116,10 +116,15 @@ CEMC_API int cemc_model_destroy(cemc_model* model);
 CEMC_API int cemc_updater_create(cemc_model* model, const int8_t* occupation, const double* cf_init,
                         const double* eci, cemc_updater** out);
 CEMC_API int cemc_updater_destroy(cemc_updater* upd);
-/* PyCEUpdater.update_cf / CEUpdater::update_cf (ce_updater.cpp:287-406). */
+/* PyCEUpdater.update_cf / CEUpdater::update_cf (ce_updater.cpp:287-406).  A
+ * change with old_id == new_id is skipped (ce_updater.cpp:315-318); otherwise
+ * the update starts from the species the site holds, as the reference reads
+ * it from its own symbols (ce_updater.cpp:335), and old_id is what an undo
+ * restores.  One updater is not safe to call from several threads at once. */
 CEMC_API int cemc_updater_update_cf(cemc_updater* upd, int32_t site, int32_t old_id, int32_t new_id);
 /* PyCEUpdater.calculate / CEUpdater::calculate (ce_updater.cpp:471-550, 831-852).
- * Returns the total energy after the changes. */
+ * Returns the total energy after the changes (same change semantics as
+ * cemc_updater_update_cf). */
 CEMC_API int cemc_updater_calculate(cemc_updater* upd, int32_t n, const int32_t* site, const int32_t* old_id,
                            const int32_t* new_id, double* energy);
 /* PyCEUpdater.undo_changes / CEUpdater::undo_changes (ce_updater.cpp:408-448). */
