@@ -85,6 +85,7 @@ void dev_free(T*& p) {
 
 constexpr int kMaxHistory = 1000;  // CFHistoryTracker::max_history (cf_history_tracker.hpp:56)
 constexpr uint32_t kGtabLdsBudget = 96u * 1024u;
+constexpr uint32_t kUnitLds = 160u * 1024u;  // LDS of one compute unit (gfx950)
 
 // Dynamic LDS a workgroup may use on the current device (160 KB on gfx950).
 uint32_t lds_limit() {
@@ -1467,6 +1468,16 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
     k.spec = (k.kr > 0 && m->spec_ok) ? want_spec : 1;
     if (k.kr > 0) {
       k.glds = with_g <= gtab_budget;
+      // a launch with more chains than can be resident at once with their
+      // tables in LDS runs more of them side by side, hiding each one's
+      // latency, when the tables stay in HBM (measured on 1024 ternary FCC
+      // 12^3 chains, two per unit with tables in LDS: 5.7e8 against 3.4e8
+      // swaps/s at one wave per chain, 6.3e8 against 4.1e8 at two)
+      if (k.glds) {
+        const uint32_t st = sweep_static_lds() + 256u;
+        const int per_unit_lds = int(kUnitLds / (with_g + st)), per_unit_hbm = int(kUnitLds / (no_g + st));
+        if (int64_t(n_launch) > int64_t(n_cu) * per_unit_lds && per_unit_hbm > per_unit_lds) k.glds = false;
+      }
       k.lds = k.glds ? with_g : no_g;
     } else {
       k.waves = 1;

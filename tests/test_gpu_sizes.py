@@ -46,6 +46,36 @@ def _oracle(settings, names, occ, eci, mu=None, cf0=None):
     return OracleUpdater(OracleModel(settings, names), occ, cf, e)
 
 
+def test_crowded_ternary_launch_reads_tables_from_hbm(gpu):
+    """1100 ternary FCC 8^3 chains with the uneven-sizes cluster set: more
+    than can be resident at once with their energy tables in LDS, so the
+    launch reads the tables from HBM (one 16-byte global load per record)
+    and runs more chains per compute unit; chains equal oracle chains."""
+    from cemc_amd.batch import CANONICAL, ChainBatch
+    from cemc_amd.model import CEModel
+    st = FCCSettings(8, ["A", "B", "C"], {2: 4, 3: 3, 4: 2})
+    eci = seeded_ecis(st, 2026, 0.01)
+    names = st.cf_names()
+    n_chains = 1100
+    temps = np.geomspace(300.0, 1500.0, n_chains)
+    occs = [random_occupation(st.n_sites, 3, 7000 + c) for c in range(n_chains)]
+    chains = [dict(kind=CANONICAL, temperature=float(temps[c]), occupation=occs[c], seed=501, uid=c)
+              for c in range(n_chains)]
+    batch = ChainBatch([CEModel(st, eci)] * n_chains, chains)
+    shape = batch.launch_shape(0)
+    print("crowded ternary launch shape", shape, flush=True)
+    assert not shape[2], "tables expected in HBM"
+    n = 2000
+    log = batch.run_logged(n)
+    for c in (0, 549, 1099):
+        ora = _oracle(st, names, occs[c], eci)
+        bits, _, acc = ora.run(0, 501, c, n, temps[c])
+        assert np.array_equal(_bits(log[c], n), _bits(bits, n)), c
+        assert np.array_equal(batch.occupation(c), ora.get_occupation())
+        assert abs(batch.energies()[c] - ora.get_energy()) / st.n_sites <= E_TOL
+        assert batch.stats()["accepted"][c] == acc
+
+
 @pytest.mark.parametrize("shape", ["auto", "4x8", "wide"])
 def test_ternary_16_cube_matches_oracle(gpu, shape):
     from cemc_amd.batch import CANONICAL, ChainBatch
