@@ -1424,6 +1424,11 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
     int auto_waves = 1;
     const int chain_waves = long_moves > 0 && (*nm)[c] >= long_moves ? std::max(batch_waves, 4) : batch_waves;
     while (auto_waves < chain_waves && cnt > 128 * auto_waves) auto_waves *= 2;
+    // at most four records per lane even on a crowded device: eight need
+    // about twice the registers, one wave per SIMD (measured on 1024 ternary
+    // FCC 12^3 chains, 494 records: two waves with four records each 6.3e8
+    // swaps/s, one wave with eight 5.7e8)
+    while (auto_waves < 8 && cnt > 256 * auto_waves) auto_waves *= 2;
     auto pick_kr = [&](int waves) {
       if (single && k.kbm <= 3 && no_g <= limit)
         for (int kr : {2, 4, 8})
