@@ -10,6 +10,7 @@
 //               runs in updater_kernel.
 // cemc_batch    device-resident Markov chains advanced by sweep_kernel.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -86,6 +87,19 @@ void dev_free(T*& p) {
 constexpr int kMaxHistory = 1000;  // CFHistoryTracker::max_history (cf_history_tracker.hpp:56)
 constexpr uint32_t kGtabLdsBudget = 96u * 1024u;
 constexpr uint32_t kUnitLds = 160u * 1024u;  // LDS of one compute unit (gfx950)
+
+// Chains of pool launches in flight, per device.  The job server runs one
+// pool per lane, each launching from its own thread onto the same device: a
+// pool choosing its launch shapes counts the other lanes' chains as sharing
+// the device with its own.
+constexpr int kMaxDevices = 64;
+std::atomic<int64_t> g_pool_chains[kMaxDevices];
+
+int current_device_index() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  return dev >= 0 && dev < kMaxDevices ? dev : 0;
+}
 
 // Dynamic LDS a workgroup may use on the current device (160 KB on gfx950).
 uint32_t lds_limit() {
@@ -1389,8 +1403,11 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
   }
   int n_launch = 0;
   for (int c = 0; c < n_chains; ++c) n_launch += included(c) ? 1 : 0;
+  // chains on the device during this launch: the pool's own and, for a
+  // pool, those the other pools have in flight
+  const int64_t on_device = n_launch + (b->dynamic ? g_pool_chains[current_device_index()].load() : 0);
   int batch_waves = 1;
-  while (batch_waves < 8 && int64_t(n_launch) * batch_waves * 2 <= int64_t(4) * n_cu) batch_waves *= 2;
+  while (batch_waves < 8 && on_device * batch_waves * 2 <= int64_t(4) * n_cu) batch_waves *= 2;
   // pools: a chain with far more moves left than the launch's median (a
   // large job among many small ones) gets up to four waves even when the
   // device is crowded, so the longest jobs do not set the tail alone
@@ -1481,7 +1498,7 @@ void derive_classes(cemc_batch* b, int forced_waves, const std::vector<int64_t>*
       if (k.glds) {
         const uint32_t st = sweep_static_lds() + 256u;
         const int per_unit_lds = int(kUnitLds / (with_g + st)), per_unit_hbm = int(kUnitLds / (no_g + st));
-        if (int64_t(n_launch) > int64_t(n_cu) * per_unit_lds && per_unit_hbm > per_unit_lds) k.glds = false;
+        if (on_device > int64_t(n_cu) * per_unit_lds && per_unit_hbm > per_unit_lds) k.glds = false;
       }
       k.lds = k.glds ? with_g : no_g;
     } else {
@@ -1680,6 +1697,20 @@ void batch_launch(cemc_batch* b, const int64_t* n_moves, int64_t moves, bool tim
   }
   if (b->dynamic) derive_classes(b, b->forced_waves, &nm);
   HIP_CHECK(hipMemcpyAsync(b->d_nmoves, nm.data(), sizeof(int64_t) * nm.size(), hipMemcpyHostToDevice, b->stream));
+  // a pool's chains count as on the device from the launch until the wait
+  // below returns (or throws)
+  struct InFlight {
+    std::atomic<int64_t>* counter = nullptr;
+    int64_t n = 0;
+    ~InFlight() {
+      if (counter) counter->fetch_sub(n);
+    }
+  } in_flight;
+  if (b->dynamic) {
+    for (int c = 0; c < b->n_chains; ++c) in_flight.n += (b->active[c] && nm[c] > 0) ? 1 : 0;
+    in_flight.counter = &g_pool_chains[current_device_index()];
+    in_flight.counter->fetch_add(in_flight.n);
+  }
   if (timed) HIP_CHECK(hipEventRecord(b->ev0, b->stream));
   enqueue_sweeps(b, d_log, log_words, time_limit);
   if (timed) HIP_CHECK(hipEventRecord(b->ev1, b->stream));
