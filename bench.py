@@ -431,7 +431,7 @@ def main():
                     help="run one device copy of this size first (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: each chain lives on one device and this build runs "
+        print("bench.py measures one GPU only: each chain lives on one device and the engine runs "
               "on a single MI355X; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
