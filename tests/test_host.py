@@ -41,7 +41,7 @@ def test_header_declarations_match_binding_and_library_exports():
 
 
 def test_library_has_gfx950_code_object():
-    # the offload bundle of this build names its only target
+    # the library's offload bundle names its only target
     data = open(_capi.library_path(), "rb").read()
     assert b"amdgcn-amd-amdhsa--gfx950" in data
 
