@@ -45,7 +45,9 @@ def _units():
             units.append(("ce_sweep_w%d_p%d" % (w, p), "ce_sweep_inst.hip",
                           ["-DCEMC_SWEEP_WAVES=%d" % w, "-DCEMC_SWEEP_PERIODIC=%d" % p]))
     for p in (0, 1):
-        units.append(("ce_wl_p%d" % p, "ce_wl_inst.hip", ["-DCEMC_WL_PERIODIC=%d" % p]))
+        for sgc in (0, 1):
+            units.append(("ce_wl_p%d_s%d" % (p, sgc), "ce_wl_inst.hip",
+                          ["-DCEMC_WL_PERIODIC=%d" % p, "-DCEMC_WL_SGC=%d" % sgc]))
     return units
 
 

@@ -1843,9 +1843,10 @@ void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* h
     int kbm;
     int nm;
     bool adaptive;
+    bool sgc;
     bool operator<(const WKey& o) const {
-      return std::tie(periodic, kr, glds, kbm, nm, adaptive) <
-             std::tie(o.periodic, o.kr, o.glds, o.kbm, o.nm, o.adaptive);
+      return std::tie(periodic, kr, glds, kbm, nm, adaptive, sgc) <
+             std::tie(o.periodic, o.kr, o.glds, o.kbm, o.nm, o.adaptive, o.sgc);
     }
   };
   std::map<WKey, std::pair<std::vector<int32_t>, uint32_t>> groups;
@@ -1854,6 +1855,7 @@ void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* h
     const cemc_model* m = b->models[b->params[c].model];
     WKey k;
     k.periodic = m->periodic;
+    k.sgc = b->params[c].kind == CEMC_MOVE_SGC;
     k.kbm = m->read_slices();
     const bool single = m->n_groups == 1 && int(m->movable.size()) == m->n_sites;
     const int cnt = m->grp_rec_cnt[0];
@@ -1923,6 +1925,7 @@ void wl_launch(cemc_batch* b, int64_t n_checks, int64_t check_every, uint32_t* h
     a.log_words = log_words;
     a.shared = shared ? b->d_wl : nullptr;
     a.adaptive = kv.first.adaptive;
+    a.sgc = kv.first.sgc;
     a.log_base = shared ? chk * per_walker : 0;
     HIP_CHECK(launch_wl(a, b->stream));
     begin += a.n_chains;

@@ -75,6 +75,7 @@ struct LaunchWL {
   int64_t n_checks, check_every;
   uint32_t* accept_log;  // optional, as for the sweep kernel (bits OR-ed in)
   int64_t log_words;
+  bool sgc = false;               // semi-grand-canonical walkers (all of the launch)
   bool adaptive = false;          // some walker of the launch uses adaptive windows
   const WLDev* shared = nullptr;  // walkers on this leader's shared histogram (one check per launch)
   int64_t log_base = 0;           // accept-log bit of the launch's first counter

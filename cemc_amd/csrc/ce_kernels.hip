@@ -552,11 +552,14 @@ CEMC_DECLARE_SWEEP_FN(8, 1)
 #undef CEMC_DECLARE_SWEEP_FN
 
 const void* field_fn(bool canonical, int kr, int kbm);
-const void* wl_fn_p0(int kr, bool glds, int kbm, int nm, int mode);
-const void* wl_fn_p1(int kr, bool glds, int kbm, int nm, int mode);
+const void* wl_fn_p0s0(int kr, bool glds, int kbm, int nm, int mode);
+const void* wl_fn_p1s0(int kr, bool glds, int kbm, int nm, int mode);
+const void* wl_fn_p0s1(int kr, bool glds, int kbm, int nm, int mode);
+const void* wl_fn_p1s1(int kr, bool glds, int kbm, int nm, int mode);
 
-static const void* wl_fn(bool periodic, int kr, bool glds, int kbm, int nm = 3, int mode = 0) {
-  return periodic ? wl_fn_p1(kr, glds, kbm, nm, mode) : wl_fn_p0(kr, glds, kbm, nm, mode);
+static const void* wl_fn(bool periodic, bool sgc, int kr, bool glds, int kbm, int nm, int mode) {
+  if (sgc) return periodic ? wl_fn_p1s1(kr, glds, kbm, nm, mode) : wl_fn_p0s1(kr, glds, kbm, nm, mode);
+  return periodic ? wl_fn_p1s0(kr, glds, kbm, nm, mode) : wl_fn_p0s0(kr, glds, kbm, nm, mode);
 }
 
 static const void* sweep_fn(bool periodic, bool canonical, int kr, bool glds, int kbm, int waves, int spec,
@@ -617,10 +620,10 @@ hipError_t sweep_set_max_lds(uint32_t bytes) {
       for (int g = 0; g < 2; ++g)
         for (int kbm : {1, 2, 3, 7})
           for (int nm : {2, 3})
-            for (int mode = 0; mode < 3; ++mode) {
+            for (int mode = 0; mode < 6; ++mode) {
               if (kbm == 7 && kr != 0) continue;
               if (nm == 2 && kr == 0) continue;
-              const void* fn = wl_fn(p != 0, kr, g != 0, kbm, nm, mode);
+              const void* fn = wl_fn(p != 0, mode >= 3, kr, g != 0, kbm, nm, mode % 3);
               if (fn == nullptr) return hipErrorInvalidValue;
               hipFuncAttributes attr;
               hipError_t e = hipFuncGetAttributes(&attr, fn);
@@ -685,7 +688,8 @@ hipError_t launch_wl(const LaunchWL& a, hipStream_t stream) {
         ((a.kr == 2 || a.kr == 4 || a.kr == 8) && (a.kbm == 1 || a.kbm == 2 || a.kbm == 3))))
     return hipErrorInvalidValue;
   if (a.nm != 3 && !(a.nm == 2 && a.kr > 0)) return hipErrorInvalidValue;
-  const void* fn = wl_fn(a.periodic, a.kr, a.gtab_lds, a.kbm, a.nm, a.shared != nullptr ? 2 : (a.adaptive ? 1 : 0));
+  const void* fn =
+      wl_fn(a.periodic, a.sgc, a.kr, a.gtab_lds, a.kbm, a.nm, a.shared != nullptr ? 2 : (a.adaptive ? 1 : 0));
   if (fn == nullptr) return hipErrorInvalidValue;
   const ChainParams* params = a.params;
   ChainState* states = a.states;

@@ -80,7 +80,9 @@ __device__ __attribute__((always_inline)) inline uint32_t wave_min_u32(uint32_t 
 // by atomic additions and ln g is read at device scope, so walkers see each
 // other's visits as they land.  A shared launch makes one check's steps and
 // leaves the flatness test, f and convergence to wl_shared_check_kernel.
-template <bool kPeriodic, int kR, bool kGtabLds, int kBM, int kNM = 3, int kMode = 0>
+// kSGC: semi-grand-canonical walkers (single-site flips, no tracker); the
+// host launches canonical and semi-grand-canonical walkers apart.
+template <bool kPeriodic, int kR, bool kGtabLds, int kBM, int kNM = 3, int kMode = 0, bool kSGC = false>
 __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ params,
                                                 ChainState* __restrict__ states,
                                                 const ModelDev* __restrict__ models,
@@ -176,7 +178,7 @@ __global__ __launch_bounds__(64) void wl_kernel(const ChainParams* __restrict__ 
   load_lattice(gocc);
   uint16_t* gtrk = trk_pool + pp->trk_off;
   // semi-grand-canonical walkers flip one site at a time and have no tracker
-  const bool sgc = pp->kind == 1;
+  constexpr bool sgc = kSGC;
   const bool has_bkg = m.has_background != 0;
   const int n_pick = has_bkg ? m.n_movable : N;
   if (!sgc)
