@@ -115,25 +115,35 @@ def dropin_rate(settings, eci, seconds=3.0):
     """The drop-in path on the GPU: cemc_amd.updater.PyCEUpdater driven one
     move at a time by the same fed-stream loop the reference's samplers use
     (one calculate, then clear_history or undo_changes, per move), on the
-    plain configuration (canonical swaps at 800 K)."""
+    plain configuration (canonical swaps at 800 K); with the resident
+    updater kernel (the default) and with one kernel launch per call."""
     from cemc_amd.dropin import run_updater_chain
     from cemc_amd.lattice import SimpleAtoms, random_occupation
     from cemc_amd.updater import PyCEUpdater
-    occ = random_occupation(settings.n_sites, 2, 1)
-    atoms = SimpleAtoms([settings.species[v] for v in occ])
-    upd = PyCEUpdater(atoms, settings, None, dict(eci))
-    n = 200
-    t0 = time.perf_counter()
-    run_updater_chain(upd, atoms, settings.species, 0, 42, 0, n, 800.0)
-    dt = time.perf_counter() - t0
-    n = int(max(200, n * seconds / max(dt, 1e-3)))
-    t0 = time.perf_counter()
-    _, _, _, _, acc = run_updater_chain(upd, atoms, settings.species, 0, 42, 1, n, 800.0)
-    dt = time.perf_counter() - t0
-    upd.close()
-    return {"moves_per_s": n / dt, "moves": n, "accepted": acc, "seconds": dt,
+
+    def rate(resident):
+        occ = random_occupation(settings.n_sites, 2, 1)
+        atoms = SimpleAtoms([settings.species[v] for v in occ])
+        upd = PyCEUpdater(atoms, settings, None, dict(eci))
+        upd.set_resident(resident)
+        n = 200
+        t0 = time.perf_counter()
+        run_updater_chain(upd, atoms, settings.species, 0, 42, 0, n, 800.0)
+        dt = time.perf_counter() - t0
+        n = int(max(200, n * seconds / max(dt, 1e-3)))
+        t0 = time.perf_counter()
+        _, _, _, _, acc = run_updater_chain(upd, atoms, settings.species, 0, 42, 1, n, 800.0)
+        dt = time.perf_counter() - t0
+        upd.close()
+        return n / dt, n, acc, dt
+
+    v, n, acc, dt = rate(True)
+    v1, n1, _, _ = rate(False)
+    return {"moves_per_s": v, "moves": n, "accepted": acc, "seconds": dt,
+            "launch_per_call_moves_per_s": v1, "launch_per_call_moves": n1,
             "workload": "PyCEUpdater on binary FCC 10^3, canonical swaps at 800 K, one calculate + "
-                        "clear_history/undo_changes per move from a Python loop (the reference's per-move protocol)"}
+                        "clear_history/undo_changes per move from a Python loop (the reference's per-move protocol); "
+                        "calls as commands to the resident updater kernel (launch_per_call_*: one launch per call)"}
 
 
 def binary_model():
@@ -511,7 +521,8 @@ def main():
             ref1 = reference_sample("canonical", 3.0, 1)
             if ref1 is not None:
                 out["dropin_updater"]["reference_moves_per_s"] = ref1["moves_per_s"]
-        log("drop-in updater: %.3e moves/s" % out["dropin_updater"]["moves_per_s"])
+        log("drop-in updater: %.3e moves/s (one launch per call: %.3e)" % (
+            out["dropin_updater"]["moves_per_s"], out["dropin_updater"]["launch_per_call_moves_per_s"]))
     if "ternary" in work:
         out["ternary_uneven"] = ternary_uneven(lds_peak=lds_meas, cpu=not args.no_cpu)
         log("ternary uneven: %.3e moves/s (kernel %.1f ms)" % (out["ternary_uneven"]["value"],

@@ -106,6 +106,7 @@ SIGNATURES = [
     ("cemc_updater_add_linear_vib_correction", ctypes.c_int, [_vp, ctypes.c_int32, _i32p, _dp]),
     ("cemc_updater_vib_energy", ctypes.c_int, [_vp, ctypes.c_double, _dp]),
     ("cemc_updater_set_num_threads", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("cemc_updater_set_resident", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("cemc_updater_get_occupation", ctypes.c_int, [_vp, _i8p]),
     ("cemc_updater_history_size", ctypes.c_int, [_vp, _i32p]),
     ("cemc_updater_set_occupation", ctypes.c_int, [_vp, _i8p]),

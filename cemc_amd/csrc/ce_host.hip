@@ -856,6 +856,26 @@ struct cemc_updater {
   int cf_slot = -1;
   int32_t* h_stage = nullptr;
   size_t stage_cap = 0;  // int32 entries at h_stage
+  // resident updater kernel: its command ring (mapped like the result), the
+  // next sequence number, whether a launch may still run, the time of the
+  // last command, and the idle time after which the kernel leaves
+  UpdMailbox* h_mb = nullptr;
+  UpdMailbox* d_mb = nullptr;
+  bool resident = true;
+  bool res_running = false;
+  int32_t res_seq = 0;
+  std::chrono::steady_clock::time_point res_last{};
+  int64_t idle_ticks = 0;
+  // the correlation functions of the last call with a result may still be
+  // landing in h_cf (none is pending any more once all have)
+  bool cf_landing = false;
+  // restores of undone changes not yet sent: they travel in front of the
+  // next command of the resident kernel
+  std::vector<int32_t> rs_sites, rs_species;
+  // energies read back from calls, per history slot, valid while
+  // slot_e_epoch[slot] == eci_epoch
+  std::vector<double> slot_e;
+  std::vector<int64_t> slot_e_epoch;
   std::vector<int8_t> occ;  // host mirror of the device occupation
   std::vector<double> eci;
   std::vector<int64_t> slot_epoch;
@@ -871,7 +891,18 @@ struct cemc_updater {
   bool has_vib = false;
   int num_threads = 1;
   ~cemc_updater() {
+    // a resident kernel leaves on the quit command, or at the latest after
+    // its idle time
+    if (res_running && h_mb && __atomic_load_n(&h_mb->consumed, __ATOMIC_ACQUIRE) >= res_seq - kUpdRing) {
+      UpdCmd& c = h_mb->cmd[res_seq % kUpdRing];
+      c.op = kUpdOpQuit;
+      c.n = 0;
+      c.want = 0;
+      c.nr = 0;
+      __atomic_store_n(&c.seq, res_seq, __ATOMIC_RELEASE);
+    }
     if (stream) (void)hipStreamSynchronize(stream);
+    if (h_mb) (void)hipHostFree(h_mb);
     if (h_energy) (void)hipHostFree(h_energy);
     if (h_result) (void)hipHostFree(h_result);
     if (h_cf) (void)hipHostFree(h_cf);
@@ -891,10 +922,100 @@ struct cemc_updater {
 
 namespace {
 
+// ---- resident updater kernel (updater_resident_kernel)
+// Idle time after which a resident kernel leaves, in milliseconds.
+constexpr double kResidentIdleMs = 2.0;
+
+int32_t updater_consumed(const cemc_updater* u) { return __atomic_load_n(&u->h_mb->consumed, __ATOMIC_ACQUIRE); }
+
+// True once the updater's stream has no work left (a resident kernel has
+// left); raises the stream's error, if any.
+bool updater_stream_idle(cemc_updater* u) {
+  const hipError_t q = hipStreamQuery(u->stream);
+  if (q == hipErrorNotReady) return false;
+  HIP_CHECK(q);
+  return true;
+}
+
+// Launches the resident kernel from the first command no launch has copied.
+void updater_resident_launch(cemc_updater* u) {
+  HIP_CHECK(launch_updater_resident(u->plan, u->d_occ, u->d_ring, u->d_slot_energy, u->d_eci, u->d_result, u->d_cf,
+                                    u->d_mb, updater_consumed(u) + 1, u->idle_ticks, u->lds_bytes, u->stream));
+  u->res_running = true;
+}
+
+// Spins until ready(); every 4096 polls asks the stream whether the resident
+// kernel has left (idle before a command arrived: launched again while
+// commands are outstanding) and gives up after 60 s.
+template <class Ready>
+void updater_resident_wait(cemc_updater* u, Ready ready, const char* what) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t spin = 1; !ready(); ++spin) {
+    if ((spin & 4095u) == 0) {
+      if (updater_stream_idle(u)) {
+        u->res_running = false;
+        if (ready()) return;
+        if (updater_consumed(u) + 1 >= u->res_seq) throw DeviceError(what);
+        updater_resident_launch(u);
+      }
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(60)) throw DeviceError(what);
+    }
+    __builtin_ia32_pause();
+  }
+}
+
+// Queues one command for the resident kernel (launching it when none may
+// run), after the kernel has copied the command that last used the slot:
+// the restores waiting in rs_sites / rs_species (they must fit beside the
+// command's own words), then `words`.
+void updater_post(cemc_updater* u, int32_t op, int32_t n, bool want, const int32_t* words, int n_words) {
+  const int nr = int(u->rs_sites.size());
+  require(2 * nr + n_words <= kUpdCmdWords, "updater command too long");
+  const auto now = std::chrono::steady_clock::now();
+  if (u->res_running && now - u->res_last > std::chrono::duration<double, std::milli>(kResidentIdleMs / 2) &&
+      updater_stream_idle(u))
+    u->res_running = false;
+  if (!u->res_running) updater_resident_launch(u);
+  const int32_t seq = u->res_seq;
+  updater_resident_wait(u, [&] { return updater_consumed(u) >= seq - kUpdRing; },
+                        "the resident updater kernel stopped taking commands");
+  UpdCmd& c = u->h_mb->cmd[seq % kUpdRing];
+  c.op = op;
+  c.n = n;
+  c.want = want ? 1 : 0;
+  c.nr = nr;
+  if (nr > 0) {
+    std::memcpy(c.v, u->rs_sites.data(), sizeof(int32_t) * size_t(nr));
+    std::memcpy(c.v + nr, u->rs_species.data(), sizeof(int32_t) * size_t(nr));
+    u->rs_sites.clear();
+    u->rs_species.clear();
+  }
+  if (n_words > 0) std::memcpy(c.v + 2 * nr, words, sizeof(int32_t) * size_t(n_words));
+  __atomic_store_n(&c.seq, seq, __ATOMIC_RELEASE);
+  u->res_seq = seq + 1;
+  u->res_last = std::chrono::steady_clock::now();
+}
+
+// Ends the resident kernel after the commands queued so far, so that other
+// work can use the updater's stream.
+void updater_park(cemc_updater* u) {
+  if (!u->res_running && u->rs_sites.empty()) return;
+  if (!u->rs_sites.empty()) updater_post(u, kUpdOpRestore, 0, false, nullptr, 0);
+  const int32_t quit = u->res_seq;
+  updater_post(u, kUpdOpQuit, 0, false, nullptr, 0);
+  for (;;) {
+    HIP_CHECK(hipStreamSynchronize(u->stream));
+    if (updater_consumed(u) >= quit) break;
+    updater_resident_launch(u);  // it left idle before the commands arrived
+  }
+  u->res_running = false;
+}
+
 // Copies `n` int32 values into the pinned staging buffer and from there to
 // `dst` on the updater's stream (after the stream's earlier work, which may
 // still read the staging buffer).
 void updater_stage_copy(cemc_updater* u, int32_t* dst, const int32_t* src, size_t n) {
+  updater_park(u);
   HIP_CHECK(hipStreamSynchronize(u->stream));
   if (n > u->stage_cap) {
     if (u->h_stage) (void)hipHostFree(u->h_stage);
@@ -993,6 +1114,36 @@ void updater_build_plan(cemc_updater* u) {
 void updater_flush(cemc_updater* u, const std::vector<int32_t>& packed, bool want_result) {
   const int n = int(packed.size() / kUpdWords);
   if (n == 0) return;
+  const int dst = (packed[packed.size() - kUpdWords + 2] >> 16) & 0xFFFF;
+  if (want_result) {
+    // each value is stored once per call: once none is pending, no store
+    // of the previous call is still on its way
+    if (u->cf_landing) {
+      const int ncf = u->model->n_cf;
+      updater_resident_wait(
+          u,
+          [&] {
+            for (int k = 0; k < ncf; ++k)
+              if (__atomic_load_n(&u->h_cf[k], __ATOMIC_ACQUIRE) == kUpdPending) return false;
+            return true;
+          },
+          "the updater kernel stored no correlation functions");
+      u->cf_landing = false;
+    }
+    for (int k = 0; k < u->model->n_cf; ++k) __atomic_store_n(&u->h_cf[k], kUpdPending, __ATOMIC_RELAXED);
+    __atomic_store_n(&u->h_result->energy_bits, kUpdPending, __ATOMIC_SEQ_CST);
+  }
+  u->cf_slot = want_result ? dst : -1;
+  if (want_result) u->cf_landing = true;
+  u->slot_epoch[dst] = u->eci_epoch;
+  if (u->resident && n <= kUpdInline) {
+    // waiting restores ride in front of the changes when both fit
+    if (2 * int(u->rs_sites.size()) + int(packed.size()) > kUpdCmdWords)
+      updater_post(u, kUpdOpRestore, 0, false, nullptr, 0);
+    updater_post(u, kUpdOpChanges, n, want_result, packed.data(), int(packed.size()));
+    return;
+  }
+  updater_park(u);
   UpdChanges inl{};
   const int32_t* d_list = nullptr;
   if (n <= kUpdInline) {
@@ -1007,16 +1158,9 @@ void updater_flush(cemc_updater* u, const std::vector<int32_t>& packed, bool wan
     updater_stage_copy(u, u->d_changes, packed.data(), packed.size());
     d_list = u->d_changes;
   }
-  const int dst = (packed[packed.size() - kUpdWords + 2] >> 16) & 0xFFFF;
-  if (want_result) {
-    for (int k = 0; k < u->model->n_cf; ++k) __atomic_store_n(&u->h_cf[k], kUpdPending, __ATOMIC_RELAXED);
-    __atomic_store_n(&u->h_result->energy_bits, kUpdPending, __ATOMIC_SEQ_CST);
-  }
   HIP_CHECK(launch_updater(u->plan, u->d_occ, u->d_ring, u->d_slot_energy, inl, d_list, n, u->d_eci,
                            want_result ? u->d_result : nullptr, want_result ? u->d_cf : nullptr, u->lds_bytes,
                            u->stream));
-  u->cf_slot = want_result ? dst : -1;
-  u->slot_epoch[dst] = u->eci_epoch;
 }
 
 // Waits for the energy of the last flush with a result: spins on the mapped
@@ -1028,9 +1172,17 @@ double updater_wait_result(cemc_updater* u) {
     const uint64_t bits = __atomic_load_n(&u->h_result->energy_bits, __ATOMIC_ACQUIRE);
     if (bits == kUpdPending) return false;
     std::memcpy(e, &bits, sizeof(double));
+    if (u->cf_slot >= 0) {
+      u->slot_e[u->cf_slot] = *e;
+      u->slot_e_epoch[u->cf_slot] = u->eci_epoch;
+    }
     return true;
   };
   double e = 0.0;
+  if (u->res_running) {
+    updater_resident_wait(u, [&] { return done(&e); }, "the resident updater kernel stored no result");
+    return e;
+  }
   for (uint32_t spin = 1;; ++spin) {
     if (done(&e)) return e;
     if ((spin & 4095u) == 0) {
@@ -1069,6 +1221,7 @@ int updater_stage(cemc_updater* u, int site, int old_id, int new_id, std::vector
   u->current = (u->current + 1) % kMaxHistory;
   if (u->buffer_size < unsigned(kMaxHistory)) u->buffer_size += 1;
   u->changes[dst] = cemc_updater::Change{site, old_id, new_id};
+  u->slot_e_epoch[dst] = -1;
   // the species the site holds, as the reference reads it from its own
   // symbols (ce_updater.cpp:335), whatever the caller named as old
   const int cur = u->occ[site];
@@ -1092,20 +1245,16 @@ int updater_stage(cemc_updater* u, int site, int old_id, int new_id, std::vector
 // queued updates) and one wait.
 double updater_energy(cemc_updater* u) {
   const unsigned slot = u->current_slot();
-  if (u->cf_slot >= 0 && unsigned(u->cf_slot) == slot && u->slot_epoch[slot] == u->eci_epoch) {
-    const uint64_t bits = __atomic_load_n(&u->h_result->energy_bits, __ATOMIC_ACQUIRE);
-    if (bits != kUpdPending) {
-      double e;
-      std::memcpy(&e, &bits, sizeof(double));
-      return e;
-    }
-  }
+  if (u->slot_e_epoch[slot] == u->eci_epoch) return u->slot_e[slot];
+  updater_park(u);
   if (u->slot_epoch[slot] != u->eci_epoch) {
     HIP_CHECK(launch_slot_energy(u->model->d_view, u->d_ring, u->d_slot_energy, u->d_eci, int(slot), u->stream));
     u->slot_epoch[slot] = u->eci_epoch;
   }
   HIP_CHECK(hipMemcpyAsync(u->h_energy, u->d_slot_energy + slot, sizeof(double), hipMemcpyDeviceToHost, u->stream));
   HIP_CHECK(hipStreamSynchronize(u->stream));
+  u->slot_e[slot] = *u->h_energy;
+  u->slot_e_epoch[slot] = u->eci_epoch;
   return *u->h_energy;
 }
 
@@ -1123,6 +1272,7 @@ void updater_current_cf(cemc_updater* u, double* cf) {
         if ((spin & 4095u) == 0) {
           // the launch has ended (its energy was read): what is still
           // pending after the stream drains never comes
+          updater_park(u);
           HIP_CHECK(hipStreamSynchronize(u->stream));
           bits = __atomic_load_n(&u->h_cf[k], __ATOMIC_ACQUIRE);
           if (bits == kUpdPending) ok = false;
@@ -1136,6 +1286,7 @@ void updater_current_cf(cemc_updater* u, double* cf) {
     if (ok) return;
     u->cf_slot = -1;
   }
+  updater_park(u);
   HIP_CHECK(hipMemcpyAsync(cf, u->d_ring + size_t(u->current_slot()) * ncf, sizeof(double) * ncf,
                            hipMemcpyDeviceToHost, u->stream));
   HIP_CHECK(hipStreamSynchronize(u->stream));
@@ -2056,6 +2207,8 @@ int cemc_updater_create(cemc_model* model, const int8_t* occupation, const doubl
       u->eci.assign(eci, eci + ncf);
       u->changes.assign(kMaxHistory, cemc_updater::Change{0, 0, 0});
       u->slot_epoch.assign(kMaxHistory, -1);
+      u->slot_e.assign(kMaxHistory, 0.0);
+      u->slot_e_epoch.assign(kMaxHistory, -1);
       u->d_occ = upload(u->occ);
       u->d_eci = upload(u->eci);
       HIP_CHECK(hipMalloc(&u->d_ring, sizeof(double) * size_t(kMaxHistory) * ncf));
@@ -2069,6 +2222,13 @@ int cemc_updater_create(cemc_model* model, const int8_t* occupation, const doubl
       HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&u->h_cf), sizeof(uint64_t) * size_t(ncf),
                               hipHostMallocMapped | hipHostMallocCoherent));
       HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&u->d_cf), u->h_cf, 0));
+      HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&u->h_mb), sizeof(UpdMailbox),
+                              hipHostMallocMapped | hipHostMallocCoherent));
+      std::memset(u->h_mb, 0, sizeof(UpdMailbox));
+      u->h_mb->consumed = -1;
+      for (int r = 0; r < kUpdRing; ++r) u->h_mb->cmd[r].seq = -1;
+      HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&u->d_mb), u->h_mb, 0));
+      u->idle_ticks = int64_t(kResidentIdleMs * wall_ticks_per_ms());
       updater_build_plan(u);
       std::vector<double> cf0(ncf);
       if (cf_init) {
@@ -2121,6 +2281,7 @@ int cemc_updater_set_occupation(cemc_updater* u, const int8_t* occupation) {
         rc = CEMC_ERUNTIME;
         return;
       }
+    updater_park(u);
     HIP_CHECK(hipStreamSynchronize(u->stream));
     u->occ.assign(occupation, occupation + N);
     HIP_CHECK(hipMemcpy(u->d_occ, u->occ.data(), size_t(N), hipMemcpyHostToDevice));
@@ -2144,6 +2305,7 @@ int cemc_updater_set_occupation(cemc_updater* u, const int8_t* occupation) {
     u->current = 1;
     u->buffer_size = 1;
     u->slot_epoch.assign(kMaxHistory, -1);
+    u->slot_e_epoch.assign(kMaxHistory, -1);
     u->cf_slot = -1;
   });
   return g != CEMC_OK ? g : rc;
@@ -2223,10 +2385,19 @@ int cemc_updater_undo_changes(cemc_updater* u) {
       species.push_back(ch.old_id);
     }
     if (!sites.empty()) {
-      // restores are queued on the stream behind the updates they undo
+      // restores are queued behind the updates they undo: as a command of
+      // the resident kernel, else on the stream
       std::vector<int32_t> buf(sites);
       buf.insert(buf.end(), species.begin(), species.end());
       const int n = int(sites.size());
+      if (u->resident && n <= kUpdRestoreMax) {
+        // sent with the next command (sites in pop order, applied in order)
+        if (int(u->rs_sites.size()) + n > kUpdRestoreMax) updater_post(u, kUpdOpRestore, 0, false, nullptr, 0);
+        u->rs_sites.insert(u->rs_sites.end(), sites.begin(), sites.end());
+        u->rs_species.insert(u->rs_species.end(), species.begin(), species.end());
+        return;
+      }
+      updater_park(u);
       InlineChanges inl{};
       const int32_t* d_buf = nullptr;
       if (n <= 2 * kInlineChanges) {
@@ -2309,6 +2480,14 @@ int cemc_updater_vib_energy(cemc_updater* u, double temperature, double* energy)
   });
 }
 
+int cemc_updater_set_resident(cemc_updater* u, int32_t on) {
+  return guarded([&] {
+    require(u != nullptr, "NULL updater");
+    if (!on) updater_park(u);
+    u->resident = on != 0;
+  });
+}
+
 int cemc_updater_set_num_threads(cemc_updater* u, int32_t num_threads) {
   return guarded([&] {
     require(u != nullptr, "NULL updater");
@@ -2321,6 +2500,7 @@ int cemc_updater_set_ecis(cemc_updater* u, const double* eci) {
   return guarded([&] {
     require(u && eci, "NULL argument");
     u->eci.assign(eci, eci + u->model->n_cf);
+    updater_park(u);  // a resident kernel holds the ECIs in LDS
     HIP_CHECK(hipStreamSynchronize(u->stream));  // queued updates still read the old ECIs
     HIP_CHECK(hipMemcpy(u->d_eci, u->eci.data(), sizeof(double) * u->eci.size(), hipMemcpyHostToDevice));
     u->eci_epoch += 1;

@@ -159,6 +159,37 @@ constexpr int kUpdWords = 8;
 struct UpdChanges {
   int32_t v[kUpdWords * kUpdInline];
 };
+// Command ring of a resident updater kernel, in mapped, coherent host
+// memory.  The host writes command s into slot s % kUpdRing (header and
+// words), then its sequence number `seq`; it reuses a slot once `consumed`
+// (stored by the kernel after copying a command) has reached the command
+// kUpdRing before.
+constexpr int kUpdRing = 4;
+enum : int32_t { kUpdOpChanges = 1, kUpdOpRestore = 2, kUpdOpQuit = 3 };
+constexpr int kUpdCmdWords = 124;
+struct UpdCmd {
+  int32_t seq;
+  int32_t pad0;
+  // the header and the words: 128 ints, read with one 8-byte load per lane
+  int32_t op;
+  int32_t n;     // changes (kUpdOpChanges, at most kUpdInline)
+  int32_t want;  // 1: store the energy and correlation functions in the mapped result
+  int32_t nr;    // restored sites, applied in order before the changes
+  // nr sites, their nr species, then (kUpdOpChanges) the words of each
+  // change as in UpdChanges
+  int32_t v[kUpdCmdWords];
+  int32_t pad[14];  // whole 64-byte lines
+};
+constexpr int kUpdRestoreMax = kUpdCmdWords / 2;
+struct UpdMailbox {
+  int32_t consumed;  // sequence number of the last command the kernel copied
+  int32_t pad[15];
+  UpdCmd cmd[kUpdRing];
+};
+static_assert(sizeof(UpdCmd) % 64 == 0, "updater commands fill whole cache lines");
+static_assert(sizeof(int32_t) * (4 + kUpdCmdWords) == 64 * sizeof(int64_t),
+              "a command's header and words are one 8-byte load per lane of a wave");
+
 // Dynamic LDS of the updater kernel: correlation functions, segment sums,
 // ECIs and segments (per correlation function), item products of one pass,
 // and the occupation when staged (`occ_sites` bytes, 0 when not).
@@ -167,6 +198,12 @@ hipError_t launch_updater(const UpdPlan& plan, int8_t* occ, double* ring,
                           double* slot_energy, const UpdChanges& inl, const int32_t* changes, int n,
                           const double* eci, UpdResult* result, uint64_t* result_cf, uint32_t lds_bytes,
                           hipStream_t stream);
+// Launches the resident updater kernel (same LDS as launch_updater), which
+// runs the commands from sequence number `next` on and leaves on a quit
+// command or after `idle_ticks` wall-clock ticks without one.
+hipError_t launch_updater_resident(const UpdPlan& plan, int8_t* occ, double* ring, double* slot_energy,
+                                   const double* eci, UpdResult* result, uint64_t* result_cf, UpdMailbox* mb,
+                                   int32_t next, int64_t idle_ticks, uint32_t lds_bytes, hipStream_t stream);
 // Restores sites[i] = species[i] in order; n <= 2 * kInlineChanges pairs
 // come in `inl` (sites in v[0..n), species in v[n..2n)), more from `buf`
 // (sites then species).

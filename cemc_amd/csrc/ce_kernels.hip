@@ -14,6 +14,9 @@
 //  updater_kernel   sequence of single-site changes with per-correlation-
 //                   function deltas written to a history slot, for the
 //                   updater interface (ce_updater.cpp:313-406, 845-852).
+//  updater_resident_kernel  the same calls, and restores, as commands taken
+//                   from a ring in mapped host memory by a workgroup that
+//                   stays resident between calls.
 //  restore_kernel   undo of popped changes (ce_updater.cpp:414-448).
 //  slot_energy_kernel  E = N * sum_k J_k cf_k of one history slot.
 #include <atomic>
@@ -225,38 +228,83 @@ uint32_t updater_lds_bytes(int n_cf, int chunk, int occ_sites) {
 // s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima).
 __device__ inline void vm_wait_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-// kInArgs: the changes travel in the kernel arguments (else in `changes`);
-// kOccLds: the occupation is staged in LDS; kPeriodic: closed-form periodic
-// lattice (else members come from the translation table).  With the
-// occupation in LDS the workgroup exchanges data through LDS only, so its
-// barriers wait for LDS operations alone and global loads and stores stay in
-// flight across them (a full barrier waits for every load and store of the
-// wave, one memory round trip); otherwise the occupation written for a change
-// is read back from global memory by the next change, which needs the full
-// barrier.  A change in the translation group of the change before it reuses
-// that change's items and segments (every swap on a one-group lattice).
-template <bool kInArgs, bool kOccLds, bool kPeriodic>
-__global__ __launch_bounds__(256) void updater_kernel(UpdPlan plan, int8_t* occ, double* __restrict__ ring,
-                                                      double* __restrict__ slot_energy, UpdChanges inl,
-                                                      const int32_t* __restrict__ changes, int n,
-                                                      const double* __restrict__ eci, UpdResult* result,
-                                                      uint64_t* result_cf) {
-  extern __shared__ double upd_lds[];
-  __shared__ double phi[kMaxBasis * kMaxSpecies];
-  __shared__ double red[4];
+// Pointers into the updater's dynamic LDS: correlation functions of the
+// current slot, segment sums over earlier passes, ECIs, segments of the
+// change, item products of one pass, staged occupation.
+struct UpdLds {
+  double* cfl;
+  double* acc;
+  double* ecil;
+  UpdSeg* segl;
+  double* prod;
+  int8_t* occ_l;
+};
+__device__ inline UpdLds upd_lds_layout(const UpdPlan& plan, double* upd_lds) {
+  const int ncf_pad = (plan.n_cf + 31) / 32 * 32;
+  UpdLds L;
+  L.cfl = upd_lds;
+  L.acc = L.cfl + ncf_pad;
+  L.ecil = L.acc + ncf_pad;
+  L.segl = reinterpret_cast<UpdSeg*>(L.ecil + ncf_pad);
+  L.prod = reinterpret_cast<double*>(L.segl + ncf_pad);
+  L.occ_l = reinterpret_cast<int8_t*>(L.prod + plan.chunk);
+  return L;
+}
+
+// Stages what every call reads and no change alters: ECIs, basis functions
+// and (kOccLds) the occupation.
+template <bool kOccLds>
+__device__ inline void updater_stage_fixed(const UpdPlan& plan, const int8_t* occ, const double* __restrict__ eci,
+                                           double* upd_lds, double* phi) {
+  const int tid = int(threadIdx.x);
+  const UpdLds L = upd_lds_layout(plan, upd_lds);
+  for (int k = tid; k < plan.n_cf; k += 256) L.ecil[k] = eci[k];
+  for (int i = tid; i < plan.n_bfs * plan.n_species; i += 256) phi[i] = plan.phi[i];
+  if constexpr (kOccLds) {
+    // 4-byte words, then the last N mod 4 bytes (the allocation is aligned)
+    const int N = plan.n_sites;
+    const int nw4 = N >> 2;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(occ);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(L.occ_l);
+    for (int i = tid; i < nw4; i += 256) dst[i] = src[i];
+    if (tid < (N & 3)) L.occ_l[4 * nw4 + tid] = occ[4 * nw4 + tid];
+  }
+}
+
+// One call of the updater interface: changes c = 0 .. n-1 with words
+// word(c, w), then the energy of the last slot.  kOccLds: the occupation is
+// staged in LDS; kPeriodic: closed-form periodic lattice (else members come
+// from the translation table).  With the occupation in LDS the workgroup
+// exchanges data through LDS only, so its barriers wait for LDS operations
+// alone and global loads and stores stay in flight across them (a full
+// barrier waits for every load and store of the wave, one memory round
+// trip); otherwise the occupation written for a change is read back from
+// global memory by the next change, which needs the full barrier.  A change
+// in the translation group of the change before it reuses that change's
+// items and segments (every swap on a one-group lattice).  kStage: a launch
+// for this call alone stages ECIs, basis functions and occupation and loads
+// the correlation functions of the first change's source slot; a resident
+// kernel has staged the first three once and reloads the correlation
+// functions only when the source slot is not the slot `held` in LDS.
+// Returns the slot held in LDS afterwards (the last change's destination).
+template <bool kOccLds, bool kPeriodic, bool kStage, class Word>
+__device__ __forceinline__ int updater_body(const UpdPlan& plan, int8_t* occ, double* __restrict__ ring,
+                                            double* __restrict__ slot_energy, Word word, int n,
+                                            const double* __restrict__ eci, UpdResult* result,
+                                            uint64_t* result_cf, double* upd_lds, double* phi, double* red,
+                                            int held) {
   const int tid = int(threadIdx.x);
   const int wave = tid >> 6, lane = tid & 63;
   const int S = plan.n_species;
   const int N = plan.n_sites;
   const int ncf = plan.n_cf;
-  const int ncf_pad = (ncf + 31) / 32 * 32;
-  double* cfl = upd_lds;                                     // correlation functions of the current slot
-  double* acc = cfl + ncf_pad;                               // segment sums over earlier passes
-  double* ecil = acc + ncf_pad;                              // ECIs
-  UpdSeg* segl = reinterpret_cast<UpdSeg*>(ecil + ncf_pad);  // segments of the change
-  double* prod = reinterpret_cast<double*>(segl + ncf_pad);  // item products of one pass
-  int8_t* occ_l = reinterpret_cast<int8_t*>(prod + plan.chunk);  // staged occupation
-  auto word = [&](int c, int w) { return kInArgs ? inl.v[kUpdWords * c + w] : changes[kUpdWords * c + w]; };
+  const UpdLds L = upd_lds_layout(plan, upd_lds);
+  double* cfl = L.cfl;
+  double* acc = L.acc;
+  double* ecil = L.ecil;
+  UpdSeg* segl = L.segl;
+  double* prod = L.prod;
+  int8_t* occ_l = L.occ_l;
   auto barrier = [&] {
     if constexpr (kOccLds) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -297,19 +345,9 @@ __global__ __launch_bounds__(256) void updater_kernel(UpdPlan plan, int8_t* occ,
     // ECIs go to LDS now, so that no wait at the end covers the stores made
     // meanwhile (loads and stores share one counter)
     const int src0 = word(0, 2) & 0xFFFF;
-    for (int k = tid; k < ncf; k += 256) {
-      cfl[k] = ring[int64_t(src0) * ncf + k];
-      ecil[k] = eci[k];
-    }
-    for (int i = tid; i < plan.n_bfs * S; i += 256) phi[i] = plan.phi[i];
-    if constexpr (kOccLds) {
-      // 4-byte words, then the last N mod 4 bytes (the allocation is aligned)
-      const int nw4 = N >> 2;
-      const uint32_t* src = reinterpret_cast<const uint32_t*>(occ);
-      uint32_t* dst = reinterpret_cast<uint32_t*>(occ_l);
-      for (int i = tid; i < nw4; i += 256) dst[i] = src[i];
-      if (tid < (N & 3)) occ_l[4 * nw4 + tid] = occ[4 * nw4 + tid];
-    }
+    if (kStage || src0 != held)
+      for (int k = tid; k < ncf; k += 256) cfl[k] = ring[int64_t(src0) * ncf + k];
+    if constexpr (kStage) updater_stage_fixed<kOccLds>(plan, occ, eci, upd_lds, phi);
   }
   // The loads above complete before the first barrier in any case.  Waiting
   // for all of them here, by name, tells the compiler that the item held in
@@ -426,7 +464,7 @@ __global__ __launch_bounds__(256) void updater_kernel(UpdPlan plan, int8_t* occ,
     barrier();
     for (int k = tid; k < ncf; k += 256) ring[int64_t(dst) * ncf + k] = cfl[k];
   }
-  if (last_dst < 0) return;
+  if (last_dst < 0) return held;
   // the final correlation functions to mapped host memory; the host marks
   // every value pending before the launch and waits for each, so these
   // stores need no ordering against the energy
@@ -445,6 +483,106 @@ __global__ __launch_bounds__(256) void updater_kernel(UpdPlan plan, int8_t* occ,
     if (result != nullptr)
       __hip_atomic_store(&result->energy_bits, uint64_t(__double_as_longlong(e)), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  return last_dst;
+}
+
+// A launch for one call: the changes travel in the kernel arguments
+// (kInArgs) or in `changes` in device memory.
+template <bool kInArgs, bool kOccLds, bool kPeriodic>
+__global__ __launch_bounds__(256) void updater_kernel(UpdPlan plan, int8_t* occ, double* __restrict__ ring,
+                                                      double* __restrict__ slot_energy, UpdChanges inl,
+                                                      const int32_t* __restrict__ changes, int n,
+                                                      const double* __restrict__ eci, UpdResult* result,
+                                                      uint64_t* result_cf) {
+  extern __shared__ double upd_lds[];
+  __shared__ double phi[kMaxBasis * kMaxSpecies];
+  __shared__ double red[4];
+  auto word = [&](int c, int w) { return kInArgs ? inl.v[kUpdWords * c + w] : changes[kUpdWords * c + w]; };
+  (void)updater_body<kOccLds, kPeriodic, true>(plan, occ, ring, slot_energy, word, n, eci, result, result_cf,
+                                                upd_lds, phi, red, -1);
+}
+
+// A resident updater: one workgroup that stays on its compute unit and takes
+// the calls of one updater as commands from a ring in mapped host memory, so
+// a call costs no launch.  Wave 0 waits for command `next` (its sequence
+// number in slot next % kUpdRing) with system-scope loads, which reach host
+// memory on every poll, then copies the command to LDS and stores `next` in
+// `consumed`, which frees the slot for the host.  The workgroup leaves, all
+// waves together, on a quit command or when no command has come for
+// `idle_ticks` ticks of the constant-rate wall clock; the host launches it
+// again from the first command it has not copied.  Between commands the
+// ECIs, basis functions, staged occupation and the correlation functions of
+// the last destination slot stay in LDS.
+template <bool kOccLds, bool kPeriodic>
+__global__ __launch_bounds__(256) void updater_resident_kernel(UpdPlan plan, int8_t* occ, double* __restrict__ ring,
+                                                               double* __restrict__ slot_energy,
+                                                               const double* __restrict__ eci, UpdResult* result,
+                                                               uint64_t* result_cf, UpdMailbox* mb, int32_t next,
+                                                               int64_t idle_ticks) {
+  extern __shared__ double upd_lds[];
+  __shared__ double phi[kMaxBasis * kMaxSpecies];
+  __shared__ double red[4];
+  __shared__ int64_t cmd64[64];  // the command's header and words
+  int32_t* cmd = reinterpret_cast<int32_t*>(cmd64);
+  const int tid = int(threadIdx.x);
+  updater_stage_fixed<kOccLds>(plan, occ, eci, upd_lds, phi);
+  int8_t* occ_l = upd_lds_layout(plan, upd_lds).occ_l;
+  int held = -1;
+  for (;; ++next) {
+    if (tid < 64) {
+      UpdCmd* c = &mb->cmd[next % kUpdRing];
+      int got = 0;
+      if (tid == 0) {
+        const uint64_t t0 = uint64_t(wall_clock64());
+        for (;;) {
+          if (__hip_atomic_load(&c->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == next) {
+            got = 1;
+            break;
+          }
+          if (uint64_t(wall_clock64()) - t0 > uint64_t(idle_ticks)) break;
+        }
+      }
+      got = __shfl(got, 0);
+      if (got) {
+        // the host wrote the words before the sequence number, and these
+        // loads start after it was seen: one 8-byte load per lane
+        int64_t* src = reinterpret_cast<int64_t*>(&c->op);
+        cmd64[tid] = __hip_atomic_load(src + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        // every load of the wave has returned before the slot is released
+        vm_wait_all();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (tid == 0) __hip_atomic_store(&mb->consumed, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      } else if (tid == 0) {
+        cmd[0] = 0;
+      }
+    }
+    __syncthreads();
+    const int op = cmd[0];
+    if (op != kUpdOpChanges && op != kUpdOpRestore) return;  // quit, or idle
+    const int n = cmd[1], nr = cmd[3];
+    const int32_t* w = cmd + 4;
+    if (nr > 0) {
+      // restores of undone changes (sites, then species), in order
+      if (tid == 0)
+        for (int i = 0; i < nr; ++i) {
+          const int site = w[i];
+          occ[site] = int8_t(w[nr + i]);
+          if constexpr (kOccLds) occ_l[site] = int8_t(w[nr + i]);
+        }
+      __syncthreads();
+    }
+    if (op == kUpdOpChanges) {
+      const bool want = cmd[2] != 0;
+      const int32_t* cw = w + 2 * nr;
+      auto word = [&](int ci, int k) { return cw[kUpdWords * ci + k]; };
+      held = updater_body<kOccLds, kPeriodic, false>(plan, occ, ring, slot_energy, word, n, eci,
+                                                      want ? result : nullptr, want ? result_cf : nullptr, upd_lds,
+                                                      phi, red, held);
+    }
+    // the command's reads of `cmd` and its global stores are done before
+    // the next command is copied and runs
+    __syncthreads();
   }
 }
 
@@ -832,6 +970,39 @@ hipError_t launch_updater(const UpdPlan& plan, int8_t* occ, double* ring,
   }
   if (o) return p ? go(launch_updater_t<false, true, true>) : go(launch_updater_t<false, true, false>);
   return p ? go(launch_updater_t<false, false, true>) : go(launch_updater_t<false, false, false>);
+}
+
+template <bool kOccLds, bool kPeriodic>
+static hipError_t launch_resident_t(const UpdPlan& plan, int8_t* occ, double* ring, double* slot_energy,
+                                    const double* eci, UpdResult* result, uint64_t* result_cf, UpdMailbox* mb,
+                                    int32_t next, int64_t idle_ticks, uint32_t lds_bytes, hipStream_t stream) {
+  auto* fn = updater_resident_kernel<kOccLds, kPeriodic>;
+  if (lds_bytes > 64u * 1024u) {
+    static std::mutex mu;
+    static uint32_t limit = 64u * 1024u;
+    std::lock_guard<std::mutex> lock(mu);
+    if (lds_bytes > limit) {
+      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
+      if (e != hipSuccess) return e;
+      limit = lds_bytes;
+    }
+  }
+  hipLaunchKernelGGL(fn, dim3(1), dim3(256), lds_bytes, stream, plan, occ, ring, slot_energy, eci, result, result_cf,
+                     mb, next, idle_ticks);
+  return hipGetLastError();
+}
+
+hipError_t launch_updater_resident(const UpdPlan& plan, int8_t* occ, double* ring, double* slot_energy,
+                                   const double* eci, UpdResult* result, uint64_t* result_cf, UpdMailbox* mb,
+                                   int32_t next, int64_t idle_ticks, uint32_t lds_bytes, hipStream_t stream) {
+  if (next < 0 || idle_ticks <= 0) return hipErrorInvalidValue;
+  const bool o = plan.occ_lds != 0, p = plan.periodic != 0;
+  auto go = [&](auto fn) {
+    return fn(plan, occ, ring, slot_energy, eci, result, result_cf, mb, next, idle_ticks, lds_bytes, stream);
+  };
+  if (o) return p ? go(launch_resident_t<true, true>) : go(launch_resident_t<true, false>);
+  return p ? go(launch_resident_t<false, true>) : go(launch_resident_t<false, false>);
 }
 
 hipError_t launch_restore(int8_t* occ, const InlineChanges& inl, const int32_t* buf, int n, hipStream_t stream) {

@@ -214,6 +214,11 @@ class PyCEUpdater:
         _capi.check(self._lib.cemc_updater_history_size(self.handle, ctypes.byref(v)))
         return v.value
 
+    def set_resident(self, on=True):
+        """Calls as commands to a resident GPU kernel (the default) or as
+        one kernel launch each; the results are identical."""
+        _capi.check(self._lib.cemc_updater_set_resident(self.handle, 1 if on else 0))
+
     def set_num_threads(self, num_threads):
         """Stored for interface compatibility; the work is spread over GPU lanes."""
         if int(num_threads) <= 0:

@@ -156,6 +156,15 @@ CEMC_API int cemc_updater_vib_energy(cemc_updater* upd, double temperature, doub
  * accepted and stored (positive values only); the work is spread over GPU
  * lanes, so the count does not change the arithmetic or the results. */
 CEMC_API int cemc_updater_set_num_threads(cemc_updater* upd, int32_t num_threads);
+/* No counterpart in the reference: chooses how the updater's calls reach
+ * the GPU.  on != 0 (the default): calls of up to 8 changes, and undos of
+ * up to 32, are commands to a resident kernel, one workgroup that stays on
+ * a compute unit and polls a command ring in mapped host memory, so a call
+ * costs no kernel launch; it leaves 2 ms after the last command (and before
+ * any other call of this updater that uses its stream), and while it runs
+ * it holds that compute unit and the hardware queue of the updater's
+ * stream.  on == 0: one kernel launch per call.  Results are identical. */
+CEMC_API int cemc_updater_set_resident(cemc_updater* upd, int32_t on);
 /* PyCEUpdater.get_symbols (ce_updater.hpp:111) as species ids; n_sites values. */
 CEMC_API int cemc_updater_get_occupation(cemc_updater* upd, int8_t* occupation);
 /* Number of history entries (CFHistoryTracker::history_size, cf_history_tracker.hpp:48). */

@@ -351,6 +351,99 @@ def test_updater_on_a_lattice_larger_than_lds(gpu):
     _replay_random_calls(s, random_occupation(s.n_sites, 2, 3), 30, 12)
 
 
+@pytest.mark.parametrize("case", ["fcc_binary", "rocksalt", "fcc54_occupation_in_hbm"])
+def test_resident_updater_equals_one_launch_per_call(gpu, case):
+    """The resident kernel (calls as commands taken from mapped host memory
+    by a workgroup that stays on the GPU) and one kernel launch per call give
+    identical energies, correlation functions and lattices over the same
+    random calls: one, two and twelve changes per call (twelve go to a
+    launch), update_cf without a result, undos of up to 40 changes (more than
+    32 go to a launch), get_energy and get_cf after undos, new ECIs and a new
+    occupation midway, and pauses longer than the resident kernel's 2 ms
+    idle time, after which it is launched again."""
+    import time
+    from cemc_amd.lattice import FCCSettings, SimpleAtoms, random_crystal_occupation, random_occupation, seeded_ecis
+    from cemc_amd.updater import PyCEUpdater
+    if case == "fcc_binary":
+        s, steps = FCCSettings(6, ["A", "B"], {2: 3, 3: 2}), 240
+    elif case == "rocksalt":
+        s, steps = _crystal("rocksalt"), 160
+    else:
+        s, steps = FCCSettings(54, ["A", "B"], {2: 1, 3: 1}), 60
+    occ = random_occupation(s.n_sites, 2, 5) if case != "rocksalt" else random_crystal_occupation(s, 5)
+    eci = seeded_ecis(s, 8, 0.01)
+    sp = list(s.species)
+    ups = []
+    for resident in (True, False):
+        u = PyCEUpdater(SimpleAtoms([sp[v] for v in occ]), s, None, dict(eci))
+        u.set_resident(resident)
+        ups.append(u)
+    allowed = _allowed_species(s)
+    movable = [i for i in range(s.n_sites) if len(allowed[i]) > 1]
+    rng = np.random.Generator(np.random.Philox(21))
+    cur = [int(v) for v in occ]
+    print("resident vs launch per call on", case, s.n_sites, "sites", flush=True)
+
+    def flips(k):
+        state = list(cur)
+        out = []
+        for _ in range(k):
+            i = movable[int(rng.integers(len(movable)))]
+            opts = [v for v in allowed[i] if v != state[i]]
+            new = opts[int(rng.integers(len(opts)))]
+            out.append((i, sp[state[i]], sp[new]))
+            state[i] = new
+        return out
+
+    def same(step):
+        e = [u.get_energy() for u in ups]
+        assert e[0] == e[1], step
+        assert np.array_equal(ups[0].get_cf_array(), ups[1].get_cf_array()), step
+        assert np.array_equal(ups[0].get_occupation(), ups[1].get_occupation()), step
+
+    for step in range(steps):
+        kind = step % 8
+        if kind in (0, 1, 2, 3):
+            chs = flips((1, 2, 12, 1)[kind])
+            e = [u.calculate(chs) for u in ups]
+            assert e[0] == e[1], step
+        elif kind == 4:
+            for ch in flips(int(rng.integers(20, 41))):
+                for u in ups:
+                    u.update_cf(ch)
+        elif kind == 5:
+            for ch in flips(5):
+                for u in ups:
+                    u.update_cf(ch)
+        elif kind == 6:
+            time.sleep(0.005)
+        else:
+            for ch in flips(3):
+                e = [u.calculate([ch]) for u in ups]
+                assert e[0] == e[1], step
+        if step % 3 == 0:
+            for u in ups:
+                u.undo_changes()
+        else:
+            for u in ups:
+                u.clear_history()
+        same(step)
+        cur = [int(v) for v in ups[0].get_occupation()]
+        if step == steps // 2:
+            new_eci = {k: v * 1.5 for k, v in seeded_ecis(s, 9, 0.01).items()}
+            for u in ups:
+                u.set_ecis(new_eci)
+            same(step)
+        if step == 3 * steps // 4:
+            occ2 = random_occupation(s.n_sites, 2, 6) if case != "rocksalt" else random_crystal_occupation(s, 6)
+            for u in ups:
+                u.set_occupation(occ2)
+            cur = [int(v) for v in occ2]
+            same(step)
+    for u in ups:
+        u.close()
+
+
 @pytest.mark.parametrize("fname,n", [("binary_canonical_plain.npz", 3000), ("ternary_sgc.npz", 3000),
                                      ("quaternary_canonical.npz", 400)])
 def test_dropin_updater_loop_reproduces_reference_chain(gpu, fname, n):
