@@ -328,6 +328,61 @@ __device__ __forceinline__ int updater_body(const UpdPlan& plan, int8_t* occ, do
   typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
   typedef double f64x2 __attribute__((ext_vector_type(2)));
   static_assert(sizeof(UpdItem) == 16 && sizeof(UpdSeg) == 32, "updater plan layout");
+  // cell coordinates of a site (closed-form periodic lattices)
+  auto cell_of = [&](int32_t site, int32_t& rx, int32_t& ry, int32_t& rz) {
+    rx = ry = rz = 0;
+    if constexpr (kPeriodic) {
+      const int32_t l23 = plan.dims[1] * plan.dims[2];
+      rx = site / l23;
+      const int32_t rem = site - rx * l23;
+      ry = rem / plan.dims[2];
+      rz = rem - ry * plan.dims[2];
+    }
+  };
+  // product of one plan item for a change of `site` from `old` to `nw`,
+  // the members' species read through spec(site index).  Branch-free: the
+  // three member gathers and the basis-function reads are issued together,
+  // then the factors are multiplied in position order (1.0 for positions
+  // past the cluster's size, an exact factor), as a loop over positions
+  // would multiply them.
+  auto product_of = [&](const u32x4& it, int32_t site, int32_t rx, int32_t ry, int32_t rz, int nw, int old,
+                        auto&& spec) {
+    const uint32_t aux = it.w;
+    const int nm = int((aux >> 14) & 3u);
+    const int rp = int((aux >> 12) & 3u);
+    int sp[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const uint32_t mq = q == 0 ? it.x : (q == 1 ? it.y : it.z);
+      int32_t nb = site;
+      if constexpr (kPeriodic) {
+        int32_t x = rx + int32_t(mq & 0xFFu);
+        int32_t y = ry + int32_t((mq >> 8) & 0xFFu);
+        int32_t z = rz + int32_t((mq >> 16) & 0xFFu);
+        x -= (x >= plan.dims[0]) ? plan.dims[0] : 0;
+        y -= (y >= plan.dims[1]) ? plan.dims[1] : 0;
+        z -= (z >= plan.dims[2]) ? plan.dims[2] : 0;
+        if (q < nm) nb = (x * plan.dims[1] + y) * plan.dims[2] + z;
+      } else {
+        if (q < nm) nb = plan.trans[int64_t(site) * plan.K + mq];
+      }
+      sp[q] = spec(nb);
+    }
+    double f[kMaxClusterSize];
+#pragma unroll
+    for (int pos = 0; pos < kMaxClusterSize; ++pos) {
+      const int d = pos > nm ? 0 : int((aux >> (3 * pos)) & 7u);  // positions past the size read phi[0..S)
+      const int q = pos - (pos > rp ? 1 : 0);  // member at this position
+      const int s = q == 0 ? sp[0] : (q == 1 ? sp[1] : sp[2]);
+      const double member = phi[d * S + s];
+      const double ref = phi[d * S + nw] - phi[d * S + old];
+      f[pos] = pos > nm ? 1.0 : (pos == rp ? ref : member);
+    }
+    double p = 1.0;
+#pragma unroll
+    for (int pos = 0; pos < kMaxClusterSize; ++pos) p *= f[pos];
+    return p;
+  };
   const u32x4* items = reinterpret_cast<const u32x4*>(plan.items);
   u32x4 it0 = {0u, 0u, 0u, 0u};  // item ib + tid of the current group
   i32x4 si0 = {0, 0, 0, 0};      // segment sb + tid of the current group
@@ -340,7 +395,26 @@ __device__ __forceinline__ int updater_body(const UpdPlan& plan, int8_t* occ, do
       sd0 = *reinterpret_cast<const f64x2*>(p + 1);
     }
   };
-  fetch(word(0, 3), word(0, 4), word(0, 5), word(0, 6));
+  // A resident call whose changes all lie in one translation group and fit
+  // one pass forms the products of every change at once (the members read
+  // the occupation as it is before their change: the call's earlier changes
+  // override the staged occupation), sums every (change, segment) pair at
+  // once, and then each thread runs the correlation functions it owns
+  // through the changes in order.  The sums are formed in the same order as
+  // change by change, so the results are identical.
+  bool fuse = false;
+  int fm = 0, fseg = 0;
+  const int ib0 = word(0, 3), ie0 = word(0, 4), sb0 = word(0, 5), se0 = word(0, 6);
+  if constexpr (!kStage) {
+    fuse = n >= 2;
+    for (int c = 1; c < n && fuse; ++c)
+      fuse = word(c, 3) == ib0 && word(c, 4) == ie0 && word(c, 5) == sb0 && word(c, 6) == se0;
+    fm = ie0 - ib0;
+    fseg = se0 - sb0;
+    fuse = fuse && fm > 0 && n * (fm + fseg) <= plan.chunk;
+  }
+  fetch(ib0, ie0, sb0, se0);
+  if (fuse && tid >= fm && tid < n * fm) it0 = items[ib0 + tid % fm];
   {
     // ECIs go to LDS now, so that no wait at the end covers the stores made
     // meanwhile (loads and stores share one counter)
@@ -357,7 +431,65 @@ __device__ __forceinline__ int updater_body(const UpdPlan& plan, int8_t* occ, do
   vm_wait_all();
   int last_dst = -1;
   int cur_sb = -1;  // segments held in segl (first segment index)
-  for (int c = 0; c < n; ++c) {
+  if (fuse) {
+    int32_t* segof = reinterpret_cast<int32_t*>(acc);  // segment of each correlation function, or -1
+    double* dlt = prod + n * fm;                        // change of each (change, segment) pair
+    if (sb0 + tid < se0) segl[tid] = UpdSeg{si0.x, si0.y, si0.z, si0.w, sd0.x, sd0.y};
+    for (int j = tid + 256; j < fseg; j += 256) segl[j] = plan.segs[sb0 + j];
+    for (int k = tid; k < ncf; k += 256) segof[k] = -1;
+    barrier();
+    for (int j = tid; j < fseg; j += 256) segof[segl[j].k] = j;
+    for (int g = tid; g < n * fm; g += 256) {
+      const int c = g / fm;
+      const u32x4 it = g == tid ? it0 : items[ib0 + (g - c * fm)];
+      const int site = word(c, 0);
+      const int nw = word(c, 1) & 0xFF, old = (word(c, 1) >> 8) & 0xFF;
+      int32_t rx, ry, rz;
+      cell_of(site, rx, ry, rz);
+      auto spec = [&](int32_t i) -> int {
+        int v = species_at(i);
+        for (int q = 0; q < c; ++q)
+          if (word(q, 0) == i) v = word(q, 1) & 0xFF;
+        return v;
+      };
+      prod[g] = product_of(it, site, rx, ry, rz, nw, old, spec);
+    }
+    barrier();
+    for (int pj = wave; pj < n * fseg; pj += 4) {
+      const int c = pj / fseg;
+      const UpdSeg g = segl[pj - c * fseg];
+      double v = 0.0;
+      for (int i = g.begin + lane; i < g.end; i += 64) v += prod[c * fm + (i - ib0)];
+      v = wave_sum_f64(v);
+      if (lane == 0) dlt[pj] = ((0.0 + v) * g.mul) / g.div;
+    }
+    barrier();
+    // the occupation after every change of the call (all products are formed)
+    if (tid == 0)
+      for (int c = 0; c < n; ++c) {
+        occ[word(c, 0)] = int8_t(word(c, 1) & 0xFF);
+        if constexpr (kOccLds) occ_l[word(c, 0)] = int8_t(word(c, 1) & 0xFF);
+      }
+    for (int k = tid; k < ncf; k += 256) {
+      const int j = segof[k];
+      int sd = -1;  // decoration of a singlet function (fixed indices: scalar loads of the arguments)
+#pragma unroll
+      for (int q = 0; q < kUpdMaxSinglets; ++q)
+        if (q < plan.n_singlets && (plan.singlets[q] & 0xFFFF) == k) sd = plan.singlets[q] >> 16;
+      double v = cfl[k];
+      for (int c = 0; c < n; ++c) {
+        if (j >= 0) v += dlt[c * fseg + j];
+        if (sd >= 0) {
+          const int nw = word(c, 1) & 0xFF, old = (word(c, 1) >> 8) & 0xFF;
+          v += (phi[sd * S + nw] - phi[sd * S + old]) / double(unsigned(N));
+        }
+        ring[int64_t((word(c, 2) >> 16) & 0xFFFF) * ncf + k] = v;
+      }
+      cfl[k] = v;
+    }
+    last_dst = (word(n - 1, 2) >> 16) & 0xFFFF;
+  }
+  for (int c = 0; c < (fuse ? 0 : n); ++c) {
     const int site = word(c, 0);
     const int nw = word(c, 1) & 0xFF, old = (word(c, 1) >> 8) & 0xFF;
     const int dst = (word(c, 2) >> 16) & 0xFFFF;
@@ -375,47 +507,9 @@ __device__ __forceinline__ int updater_body(const UpdPlan& plan, int8_t* occ, do
       // (with the same group the barrier closing the change before suffices)
       barrier();
     }
-    // cell coordinates of the site (closed-form periodic lattices)
-    int32_t rx = 0, ry = 0, rz = 0;
-    if constexpr (kPeriodic) {
-      const int32_t l23 = plan.dims[1] * plan.dims[2];
-      rx = site / l23;
-      const int32_t rem = site - rx * l23;
-      ry = rem / plan.dims[2];
-      rz = rem - ry * plan.dims[2];
-    }
-    auto product = [&](const u32x4& it) {
-      const uint32_t aux = it.w;
-      const int nm = int((aux >> 14) & 3u);
-      const int rp = int((aux >> 12) & 3u);
-      double p = 1.0;
-      int q = 0;
-#pragma unroll
-      for (int pos = 0; pos < kMaxClusterSize; ++pos) {
-        if (pos > nm) break;
-        const int d = int((aux >> (3 * pos)) & 7u);
-        if (pos == rp) {
-          p *= phi[d * S + nw] - phi[d * S + old];
-        } else {
-          const uint32_t mq = q == 0 ? it.x : (q == 1 ? it.y : it.z);
-          int32_t nb;
-          if constexpr (kPeriodic) {
-            int32_t x = rx + int32_t(mq & 0xFFu);
-            int32_t y = ry + int32_t((mq >> 8) & 0xFFu);
-            int32_t z = rz + int32_t((mq >> 16) & 0xFFu);
-            x -= (x >= plan.dims[0]) ? plan.dims[0] : 0;
-            y -= (y >= plan.dims[1]) ? plan.dims[1] : 0;
-            z -= (z >= plan.dims[2]) ? plan.dims[2] : 0;
-            nb = (x * plan.dims[1] + y) * plan.dims[2] + z;
-          } else {
-            nb = plan.trans[int64_t(site) * plan.K + mq];
-          }
-          p *= phi[d * S + species_at(nb)];
-          ++q;
-        }
-      }
-      return p;
-    };
+    int32_t rx, ry, rz;
+    cell_of(site, rx, ry, rz);
+    auto product = [&](const u32x4& it) { return product_of(it, site, rx, ry, rz, nw, old, species_at); };
     for (int cb = ib; cb < ie; cb += plan.chunk) {
       const int ce = min(ie, cb + plan.chunk);
       int idx = cb + tid;
@@ -549,17 +643,17 @@ __global__ __launch_bounds__(256) void updater_resident_kernel(UpdPlan plan, int
         // loads start after it was seen: one 8-byte load per lane
         int64_t* src = reinterpret_cast<int64_t*>(&c->op);
         cmd64[tid] = __hip_atomic_load(src + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        // every load of the wave has returned before the slot is released
-        vm_wait_all();
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (tid == 0) __hip_atomic_store(&mb->consumed, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       } else if (tid == 0) {
         cmd[0] = 0;
       }
     }
     __syncthreads();
     const int op = cmd[0];
-    if (op != kUpdOpChanges && op != kUpdOpRestore) return;  // quit, or idle
+    if (op != kUpdOpChanges && op != kUpdOpRestore) {  // quit, or idle
+      if (op == kUpdOpQuit && tid == 0)
+        __hip_atomic_store(&mb->consumed, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      return;
+    }
     const int n = cmd[1], nr = cmd[3];
     const int32_t* w = cmd + 4;
     if (nr > 0) {
@@ -580,6 +674,10 @@ __global__ __launch_bounds__(256) void updater_resident_kernel(UpdPlan plan, int
                                                       want ? result : nullptr, want ? result_cf : nullptr, upd_lds,
                                                       phi, red, held);
     }
+    // the slot is released once the command has run: a store to host
+    // memory issued before it would hold up the command's first wait on
+    // memory by a round trip over the bus
+    if (tid == 0) __hip_atomic_store(&mb->consumed, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     // the command's reads of `cmd` and its global stores are done before
     // the next command is copied and runs
     __syncthreads();
