@@ -924,7 +924,7 @@ namespace {
 
 // ---- resident updater kernel (updater_resident_kernel)
 // Idle time after which a resident kernel leaves, in milliseconds.
-constexpr double kResidentIdleMs = 2.0;
+constexpr double kResidentIdleMs = 0.5;
 
 int32_t updater_consumed(const cemc_updater* u) { return __atomic_load_n(&u->h_mb->consumed, __ATOMIC_ACQUIRE); }
 

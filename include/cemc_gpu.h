@@ -160,7 +160,7 @@ CEMC_API int cemc_updater_set_num_threads(cemc_updater* upd, int32_t num_threads
  * the GPU.  on != 0 (the default): calls of up to 8 changes, and undos of
  * up to 32, are commands to a resident kernel, one workgroup that stays on
  * a compute unit and polls a command ring in mapped host memory, so a call
- * costs no kernel launch; it leaves 2 ms after the last command (and before
+ * costs no kernel launch; it leaves 0.5 ms after the last command (and before
  * any other call of this updater that uses its stream), and while it runs
  * it holds that compute unit and the hardware queue of the updater's
  * stream.  on == 0: one kernel launch per call.  Results are identical. */

@@ -359,7 +359,7 @@ def test_resident_updater_equals_one_launch_per_call(gpu, case):
     random calls: one, two and twelve changes per call (twelve go to a
     launch), update_cf without a result, undos of up to 40 changes (more than
     32 go to a launch), get_energy and get_cf after undos, new ECIs and a new
-    occupation midway, and pauses longer than the resident kernel's 2 ms
+    occupation midway, and pauses longer than the resident kernel's 0.5 ms
     idle time, after which it is launched again."""
     import time
     from cemc_amd.lattice import FCCSettings, SimpleAtoms, random_crystal_occupation, random_occupation, seeded_ecis
