@@ -271,6 +271,16 @@ __device__ inline void updater_stage_fixed(const UpdPlan& plan, const int8_t* oc
   }
 }
 
+// What a resident kernel keeps between fused calls of one translation
+// group: the group's first segment (-1: nothing held), the call's number of
+// changes, and each thread's plan item.  The segments and their map stay
+// staged in LDS alongside.
+struct UpdHeldGroup {
+  int sb = -1;
+  int n = 0;
+  uint32_t it[4] = {0u, 0u, 0u, 0u};
+};
+
 // One call of the updater interface: changes c = 0 .. n-1 with words
 // word(c, w), then the energy of the last slot.  kOccLds: the occupation is
 // staged in LDS; kPeriodic: closed-form periodic lattice (else members come
@@ -287,12 +297,14 @@ __device__ inline void updater_stage_fixed(const UpdPlan& plan, const int8_t* oc
 // kernel has staged the first three once and reloads the correlation
 // functions only when the source slot is not the slot `held` in LDS.
 // Returns the slot held in LDS afterwards (the last change's destination).
+// `hg` (resident kernels): the group held from the call before; a fused call
+// of the same group and size takes its items and segments from there.
 template <bool kOccLds, bool kPeriodic, bool kStage, class Word>
 __device__ __forceinline__ int updater_body(const UpdPlan& plan, int8_t* occ, double* __restrict__ ring,
                                             double* __restrict__ slot_energy, Word word, int n,
                                             const double* __restrict__ eci, UpdResult* result,
                                             uint64_t* result_cf, double* upd_lds, double* phi, double* red,
-                                            int held) {
+                                            int held, UpdHeldGroup* hg) {
   const int tid = int(threadIdx.x);
   const int wave = tid >> 6, lane = tid & 63;
   const int S = plan.n_species;
@@ -413,8 +425,13 @@ __device__ __forceinline__ int updater_body(const UpdPlan& plan, int8_t* occ, do
     fseg = se0 - sb0;
     fuse = fuse && fm > 0 && n * (fm + fseg) <= plan.chunk;
   }
-  fetch(ib0, ie0, sb0, se0);
-  if (fuse && tid >= fm && tid < n * fm) it0 = items[ib0 + tid % fm];
+  const bool reuse = fuse && hg != nullptr && hg->sb == sb0 && hg->n == n;
+  if (reuse) {
+    it0 = u32x4{hg->it[0], hg->it[1], hg->it[2], hg->it[3]};
+  } else {
+    fetch(ib0, ie0, sb0, se0);
+    if (fuse && tid >= fm && tid < n * fm) it0 = items[ib0 + tid % fm];
+  }
   {
     // ECIs go to LDS now, so that no wait at the end covers the stores made
     // meanwhile (loads and stores share one counter)
@@ -428,17 +445,19 @@ __device__ __forceinline__ int updater_body(const UpdPlan& plan, int8_t* occ, do
   // registers has arrived, so the changes that reuse it wait for no memory
   // operation (a conservative wait there would cover the history-slot stores
   // of the change before).
-  vm_wait_all();
+  if (!reuse) vm_wait_all();
   int last_dst = -1;
   int cur_sb = -1;  // segments held in segl (first segment index)
   if (fuse) {
     int32_t* segof = reinterpret_cast<int32_t*>(acc);  // segment of each correlation function, or -1
     double* dlt = prod + n * fm;                        // change of each (change, segment) pair
-    if (sb0 + tid < se0) segl[tid] = UpdSeg{si0.x, si0.y, si0.z, si0.w, sd0.x, sd0.y};
-    for (int j = tid + 256; j < fseg; j += 256) segl[j] = plan.segs[sb0 + j];
-    for (int k = tid; k < ncf; k += 256) segof[k] = -1;
-    barrier();
-    for (int j = tid; j < fseg; j += 256) segof[segl[j].k] = j;
+    if (!reuse) {
+      if (sb0 + tid < se0) segl[tid] = UpdSeg{si0.x, si0.y, si0.z, si0.w, sd0.x, sd0.y};
+      for (int j = tid + 256; j < fseg; j += 256) segl[j] = plan.segs[sb0 + j];
+      for (int k = tid; k < ncf; k += 256) segof[k] = -1;
+      barrier();
+      for (int j = tid; j < fseg; j += 256) segof[segl[j].k] = j;
+    }
     for (int g = tid; g < n * fm; g += 256) {
       const int c = g / fm;
       const u32x4 it = g == tid ? it0 : items[ib0 + (g - c * fm)];
@@ -488,6 +507,16 @@ __device__ __forceinline__ int updater_body(const UpdPlan& plan, int8_t* occ, do
       cfl[k] = v;
     }
     last_dst = (word(n - 1, 2) >> 16) & 0xFFFF;
+    if (hg != nullptr) {
+      hg->sb = sb0;
+      hg->n = n;
+      hg->it[0] = it0.x;
+      hg->it[1] = it0.y;
+      hg->it[2] = it0.z;
+      hg->it[3] = it0.w;
+    }
+  } else if (hg != nullptr) {
+    hg->sb = -1;  // the calls below restage segl and reuse the map's LDS
   }
   for (int c = 0; c < (fuse ? 0 : n); ++c) {
     const int site = word(c, 0);
@@ -594,7 +623,7 @@ __global__ __launch_bounds__(256) void updater_kernel(UpdPlan plan, int8_t* occ,
   __shared__ double red[4];
   auto word = [&](int c, int w) { return kInArgs ? inl.v[kUpdWords * c + w] : changes[kUpdWords * c + w]; };
   (void)updater_body<kOccLds, kPeriodic, true>(plan, occ, ring, slot_energy, word, n, eci, result, result_cf,
-                                                upd_lds, phi, red, -1);
+                                                upd_lds, phi, red, -1, nullptr);
 }
 
 // A resident updater: one workgroup that stays on its compute unit and takes
@@ -623,6 +652,7 @@ __global__ __launch_bounds__(256) void updater_resident_kernel(UpdPlan plan, int
   updater_stage_fixed<kOccLds>(plan, occ, eci, upd_lds, phi);
   int8_t* occ_l = upd_lds_layout(plan, upd_lds).occ_l;
   int held = -1;
+  UpdHeldGroup hg;
   for (;; ++next) {
     if (tid < 64) {
       UpdCmd* c = &mb->cmd[next % kUpdRing];
@@ -672,7 +702,7 @@ __global__ __launch_bounds__(256) void updater_resident_kernel(UpdPlan plan, int
       auto word = [&](int ci, int k) { return cw[kUpdWords * ci + k]; };
       held = updater_body<kOccLds, kPeriodic, false>(plan, occ, ring, slot_energy, word, n, eci,
                                                       want ? result : nullptr, want ? result_cf : nullptr, upd_lds,
-                                                      phi, red, held);
+                                                      phi, red, held, &hg);
     }
     // the slot is released once the command has run: a store to host
     // memory issued before it would hold up the command's first wait on
