@@ -56,6 +56,11 @@ class PyCEUpdater:
         self.handle = h
         self._lib = lib
         self._occ = occ.copy()
+        # calculate's argument buffers, per number of changes (the library
+        # copies them during the call), and its energy
+        self._bufs = {}
+        self._e = ctypes.c_double(0.0)
+        self._e_ref = ctypes.byref(self._e)
         # changes in the device's history since the last clear, as
         # (site, old symbol, new symbol): an undo restores exactly these on
         # the atoms object; None when unknown (after an error), and then the
@@ -128,8 +133,11 @@ class PyCEUpdater:
         changes = list(system_changes)
         n = len(changes)
         sid = self.model.sid
-        arr = ctypes.c_int32 * n
-        site, old, new = arr(), arr(), arr()
+        bufs = self._bufs.get(n)
+        if bufs is None:
+            arr = ctypes.c_int32 * n
+            bufs = self._bufs[n] = (arr(), arr(), arr())
+        site, old, new = bufs
         try:
             for k, c in enumerate(changes):
                 site[k] = int(c[0])
@@ -137,8 +145,9 @@ class PyCEUpdater:
                 new[k] = sid[c[2]]
         except KeyError as exc:
             raise ValueError("unknown symbol %s" % exc)
-        e = ctypes.c_double(0.0)
-        rc = self._lib.cemc_updater_calculate(self.handle, n, site, old, new, ctypes.byref(e))
+        e = self._e
+        e.value = 0.0
+        rc = self._lib.cemc_updater_calculate(self.handle, n, site, old, new, self._e_ref)
         if rc == _capi.CEMC_OK:
             # the changes were made as given: set the same symbols in atoms
             # (ce_updater.cpp:338-347) without reading the lattice back
