@@ -863,7 +863,7 @@ struct cemc_updater {
   UpdMailbox* d_mb = nullptr;
   bool resident = true;
   bool res_running = false;
-  int32_t res_seq = 0;
+  int64_t res_seq = 0;
   std::chrono::steady_clock::time_point res_last{};
   int64_t idle_ticks = 0;
   // the correlation functions of the last call with a result may still be
@@ -895,11 +895,9 @@ struct cemc_updater {
     // its idle time
     if (res_running && h_mb && __atomic_load_n(&h_mb->consumed, __ATOMIC_ACQUIRE) >= res_seq - kUpdRing) {
       UpdCmd& c = h_mb->cmd[res_seq % kUpdRing];
-      c.op = kUpdOpQuit;
-      c.n = 0;
-      c.want = 0;
-      c.nr = 0;
-      __atomic_store_n(&c.seq, res_seq, __ATOMIC_RELEASE);
+      const uint64_t tag = uint64_t(uint32_t(uint64_t(res_seq))) << 32;
+      for (int i = 0; i < 4 + kUpdCmdWords; ++i)
+        __atomic_store_n(&c.w[i], tag | (i == 0 ? uint64_t(kUpdOpQuit) : 0u), __ATOMIC_RELEASE);
     }
     if (stream) (void)hipStreamSynchronize(stream);
     if (h_mb) (void)hipHostFree(h_mb);
@@ -926,7 +924,7 @@ namespace {
 // Idle time after which a resident kernel leaves, in milliseconds.
 constexpr double kResidentIdleMs = 0.5;
 
-int32_t updater_consumed(const cemc_updater* u) { return __atomic_load_n(&u->h_mb->consumed, __ATOMIC_ACQUIRE); }
+int64_t updater_consumed(const cemc_updater* u) { return __atomic_load_n(&u->h_mb->consumed, __ATOMIC_ACQUIRE); }
 
 // True once the updater's stream has no work left (a resident kernel has
 // left); raises the stream's error, if any.
@@ -976,22 +974,26 @@ void updater_post(cemc_updater* u, int32_t op, int32_t n, bool want, const int32
       updater_stream_idle(u))
     u->res_running = false;
   if (!u->res_running) updater_resident_launch(u);
-  const int32_t seq = u->res_seq;
+  const int64_t seq = u->res_seq;
   updater_resident_wait(u, [&] { return updater_consumed(u) >= seq - kUpdRing; },
                         "the resident updater kernel stopped taking commands");
-  UpdCmd& c = u->h_mb->cmd[seq % kUpdRing];
-  c.op = op;
-  c.n = n;
-  c.want = want ? 1 : 0;
-  c.nr = nr;
-  if (nr > 0) {
-    std::memcpy(c.v, u->rs_sites.data(), sizeof(int32_t) * size_t(nr));
-    std::memcpy(c.v + nr, u->rs_species.data(), sizeof(int32_t) * size_t(nr));
-    u->rs_sites.clear();
-    u->rs_species.clear();
+  uint32_t cw[4 + kUpdCmdWords] = {};
+  cw[0] = uint32_t(op);
+  cw[1] = uint32_t(n);
+  cw[2] = want ? 1u : 0u;
+  cw[3] = uint32_t(nr);
+  for (int i = 0; i < nr; ++i) {
+    cw[4 + i] = uint32_t(u->rs_sites[i]);
+    cw[4 + nr + i] = uint32_t(u->rs_species[i]);
   }
-  if (n_words > 0) std::memcpy(c.v + 2 * nr, words, sizeof(int32_t) * size_t(n_words));
-  __atomic_store_n(&c.seq, seq, __ATOMIC_RELEASE);
+  u->rs_sites.clear();
+  u->rs_species.clear();
+  for (int i = 0; i < n_words; ++i) cw[4 + 2 * nr + i] = uint32_t(words[i]);
+  // every word carries the command's number: the kernel takes the command
+  // once all of them do, whatever order the stores arrive in
+  UpdCmd& c = u->h_mb->cmd[seq % kUpdRing];
+  const uint64_t tag = uint64_t(uint32_t(uint64_t(seq))) << 32;
+  for (int i = 0; i < 4 + kUpdCmdWords; ++i) __atomic_store_n(&c.w[i], tag | cw[i], __ATOMIC_RELEASE);
   u->res_seq = seq + 1;
   u->res_last = std::chrono::steady_clock::now();
 }
@@ -1001,7 +1003,7 @@ void updater_post(cemc_updater* u, int32_t op, int32_t n, bool want, const int32
 void updater_park(cemc_updater* u) {
   if (!u->res_running && u->rs_sites.empty()) return;
   if (!u->rs_sites.empty()) updater_post(u, kUpdOpRestore, 0, false, nullptr, 0);
-  const int32_t quit = u->res_seq;
+  const int64_t quit = u->res_seq;
   updater_post(u, kUpdOpQuit, 0, false, nullptr, 0);
   for (;;) {
     HIP_CHECK(hipStreamSynchronize(u->stream));
@@ -1136,7 +1138,7 @@ void updater_flush(cemc_updater* u, const std::vector<int32_t>& packed, bool wan
   u->cf_slot = want_result ? dst : -1;
   if (want_result) u->cf_landing = true;
   u->slot_epoch[dst] = u->eci_epoch;
-  if (u->resident && n <= kUpdInline) {
+  if (u->resident && kUpdWords * n <= kUpdCmdWords) {
     // waiting restores ride in front of the changes when both fit
     if (2 * int(u->rs_sites.size()) + int(packed.size()) > kUpdCmdWords)
       updater_post(u, kUpdOpRestore, 0, false, nullptr, 0);
@@ -2226,7 +2228,9 @@ int cemc_updater_create(cemc_model* model, const int8_t* occupation, const doubl
                               hipHostMallocMapped | hipHostMallocCoherent));
       std::memset(u->h_mb, 0, sizeof(UpdMailbox));
       u->h_mb->consumed = -1;
-      for (int r = 0; r < kUpdRing; ++r) u->h_mb->cmd[r].seq = -1;
+      // no word carries the number of a command yet (the first is 0)
+      for (int r = 0; r < kUpdRing; ++r)
+        for (int i = 0; i < 4 + kUpdCmdWords; ++i) u->h_mb->cmd[r].w[i] = uint64_t(0xFFFFFFFFu) << 32;
       HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&u->d_mb), u->h_mb, 0));
       u->idle_ticks = int64_t(kResidentIdleMs * wall_ticks_per_ms());
       updater_build_plan(u);

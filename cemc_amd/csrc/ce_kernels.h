@@ -160,35 +160,30 @@ struct UpdChanges {
   int32_t v[kUpdWords * kUpdInline];
 };
 // Command ring of a resident updater kernel, in mapped, coherent host
-// memory.  The host writes command s into slot s % kUpdRing (header and
-// words), then its sequence number `seq`; it reuses a slot once `consumed`
-// (stored by the kernel after copying a command) has reached the command
+// memory.  Word i of command s (i = 0..63: op, n, want, nr, then
+// kUpdCmdWords words) sits in the low half of w[i] of slot s % kUpdRing,
+// with s's low 32 bits in the high half; the kernel's wave loads all 64
+// words at every poll and takes the command once every word carries its
+// number (8-byte words are written and read whole, so a taken command
+// holds no word of an older one).  The host reuses a slot once `consumed`
+// (stored by the kernel after running a command) has reached the command
 // kUpdRing before.
 constexpr int kUpdRing = 4;
 enum : int32_t { kUpdOpChanges = 1, kUpdOpRestore = 2, kUpdOpQuit = 3 };
-constexpr int kUpdCmdWords = 124;
+constexpr int kUpdCmdWords = 60;
+// words: nr sites, their nr species (restored in order before the
+// changes), then (kUpdOpChanges) kUpdWords words per change as in
+// UpdChanges; n changes fit when kUpdWords * n + 2 * nr <= kUpdCmdWords
 struct UpdCmd {
-  int32_t seq;
-  int32_t pad0;
-  // the header and the words: 128 ints, read with one 8-byte load per lane
-  int32_t op;
-  int32_t n;     // changes (kUpdOpChanges, at most kUpdInline)
-  int32_t want;  // 1: store the energy and correlation functions in the mapped result
-  int32_t nr;    // restored sites, applied in order before the changes
-  // nr sites, their nr species, then (kUpdOpChanges) the words of each
-  // change as in UpdChanges
-  int32_t v[kUpdCmdWords];
-  int32_t pad[14];  // whole 64-byte lines
+  uint64_t w[4 + kUpdCmdWords];
 };
 constexpr int kUpdRestoreMax = kUpdCmdWords / 2;
 struct UpdMailbox {
-  int32_t consumed;  // sequence number of the last command the kernel copied
-  int32_t pad[15];
+  int64_t consumed;  // sequence number of the last command the kernel ran
+  int64_t pad[7];
   UpdCmd cmd[kUpdRing];
 };
-static_assert(sizeof(UpdCmd) % 64 == 0, "updater commands fill whole cache lines");
-static_assert(sizeof(int32_t) * (4 + kUpdCmdWords) == 64 * sizeof(int64_t),
-              "a command's header and words are one 8-byte load per lane of a wave");
+static_assert(sizeof(UpdCmd) == 64 * sizeof(uint64_t), "a command is one 8-byte word per lane of a wave");
 
 // Dynamic LDS of the updater kernel: correlation functions, segment sums,
 // ECIs and segments (per correlation function), item products of one pass,
@@ -203,7 +198,7 @@ hipError_t launch_updater(const UpdPlan& plan, int8_t* occ, double* ring,
 // command or after `idle_ticks` wall-clock ticks without one.
 hipError_t launch_updater_resident(const UpdPlan& plan, int8_t* occ, double* ring, double* slot_energy,
                                    const double* eci, UpdResult* result, uint64_t* result_cf, UpdMailbox* mb,
-                                   int32_t next, int64_t idle_ticks, uint32_t lds_bytes, hipStream_t stream);
+                                   int64_t next, int64_t idle_ticks, uint32_t lds_bytes, hipStream_t stream);
 // Restores sites[i] = species[i] in order; n <= 2 * kInlineChanges pairs
 // come in `inl` (sites in v[0..n), species in v[n..2n)), more from `buf`
 // (sites then species).

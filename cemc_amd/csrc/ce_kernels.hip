@@ -628,10 +628,10 @@ __global__ __launch_bounds__(256) void updater_kernel(UpdPlan plan, int8_t* occ,
 
 // A resident updater: one workgroup that stays on its compute unit and takes
 // the calls of one updater as commands from a ring in mapped host memory, so
-// a call costs no launch.  Wave 0 waits for command `next` (its sequence
-// number in slot next % kUpdRing) with system-scope loads, which reach host
-// memory on every poll, then copies the command to LDS and stores `next` in
-// `consumed`, which frees the slot for the host.  The workgroup leaves, all
+// a call costs no launch.  Wave 0 polls slot next % kUpdRing with one
+// system-scope 8-byte load per lane, which reaches host memory every time,
+// until all 64 words carry command `next`'s number, copies the command to
+// LDS, runs it and stores `next` in `consumed`, which frees the slot.  The workgroup leaves, all
 // waves together, on a quit command or when no command has come for
 // `idle_ticks` ticks of the constant-rate wall clock; the host launches it
 // again from the first command it has not copied.  Between commands the
@@ -641,13 +641,12 @@ template <bool kOccLds, bool kPeriodic>
 __global__ __launch_bounds__(256) void updater_resident_kernel(UpdPlan plan, int8_t* occ, double* __restrict__ ring,
                                                                double* __restrict__ slot_energy,
                                                                const double* __restrict__ eci, UpdResult* result,
-                                                               uint64_t* result_cf, UpdMailbox* mb, int32_t next,
+                                                               uint64_t* result_cf, UpdMailbox* mb, int64_t next,
                                                                int64_t idle_ticks) {
   extern __shared__ double upd_lds[];
   __shared__ double phi[kMaxBasis * kMaxSpecies];
   __shared__ double red[4];
-  __shared__ int64_t cmd64[64];  // the command's header and words
-  int32_t* cmd = reinterpret_cast<int32_t*>(cmd64);
+  __shared__ int32_t cmd[4 + kUpdCmdWords];  // op, n, want, nr, words
   const int tid = int(threadIdx.x);
   updater_stage_fixed<kOccLds>(plan, occ, eci, upd_lds, phi);
   int8_t* occ_l = upd_lds_layout(plan, upd_lds).occ_l;
@@ -655,24 +654,22 @@ __global__ __launch_bounds__(256) void updater_resident_kernel(UpdPlan plan, int
   UpdHeldGroup hg;
   for (;; ++next) {
     if (tid < 64) {
-      UpdCmd* c = &mb->cmd[next % kUpdRing];
+      // every lane takes part in every poll and reaches the same verdict
+      uint64_t* w = &mb->cmd[next % kUpdRing].w[tid];
+      const uint32_t tag = uint32_t(uint64_t(next));
+      const uint64_t t0 = uint64_t(wall_clock64());
+      uint64_t v = 0;
       int got = 0;
-      if (tid == 0) {
-        const uint64_t t0 = uint64_t(wall_clock64());
-        for (;;) {
-          if (__hip_atomic_load(&c->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == next) {
-            got = 1;
-            break;
-          }
-          if (uint64_t(wall_clock64()) - t0 > uint64_t(idle_ticks)) break;
+      for (;;) {
+        v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (__all(uint32_t(v >> 32) == tag)) {
+          got = 1;
+          break;
         }
+        if (uint64_t(wall_clock64()) - t0 > uint64_t(idle_ticks)) break;
       }
-      got = __shfl(got, 0);
       if (got) {
-        // the host wrote the words before the sequence number, and these
-        // loads start after it was seen: one 8-byte load per lane
-        int64_t* src = reinterpret_cast<int64_t*>(&c->op);
-        cmd64[tid] = __hip_atomic_load(src + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        cmd[tid] = int32_t(uint32_t(v));
       } else if (tid == 0) {
         cmd[0] = 0;
       }
@@ -1103,7 +1100,7 @@ hipError_t launch_updater(const UpdPlan& plan, int8_t* occ, double* ring,
 template <bool kOccLds, bool kPeriodic>
 static hipError_t launch_resident_t(const UpdPlan& plan, int8_t* occ, double* ring, double* slot_energy,
                                     const double* eci, UpdResult* result, uint64_t* result_cf, UpdMailbox* mb,
-                                    int32_t next, int64_t idle_ticks, uint32_t lds_bytes, hipStream_t stream) {
+                                    int64_t next, int64_t idle_ticks, uint32_t lds_bytes, hipStream_t stream) {
   auto* fn = updater_resident_kernel<kOccLds, kPeriodic>;
   if (lds_bytes > 64u * 1024u) {
     static std::mutex mu;
@@ -1123,7 +1120,7 @@ static hipError_t launch_resident_t(const UpdPlan& plan, int8_t* occ, double* ri
 
 hipError_t launch_updater_resident(const UpdPlan& plan, int8_t* occ, double* ring, double* slot_energy,
                                    const double* eci, UpdResult* result, uint64_t* result_cf, UpdMailbox* mb,
-                                   int32_t next, int64_t idle_ticks, uint32_t lds_bytes, hipStream_t stream) {
+                                   int64_t next, int64_t idle_ticks, uint32_t lds_bytes, hipStream_t stream) {
   if (next < 0 || idle_ticks <= 0) return hipErrorInvalidValue;
   const bool o = plan.occ_lds != 0, p = plan.periodic != 0;
   auto go = [&](auto fn) {
