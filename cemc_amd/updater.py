@@ -89,11 +89,15 @@ class PyCEUpdater:
 
     # ------------------------------------------------------------- interface
     def clear_history(self):
-        _capi.check(self._lib.cemc_updater_clear_history(self.handle))
+        rc = self._lib.cemc_updater_clear_history(self.handle)
+        if rc:
+            _capi.check(rc)
         self._pending = []
 
     def undo_changes(self):
-        _capi.check(self._lib.cemc_updater_undo_changes(self.handle))
+        rc = self._lib.cemc_updater_undo_changes(self.handle)
+        if rc:
+            _capi.check(rc)
         pending = self._pending
         # the history holds at most 999 changes (ce_updater's ring of 1000 slots)
         if pending is None or len(pending) > _MAX_UNDO:
@@ -151,10 +155,18 @@ class PyCEUpdater:
         if rc == _capi.CEMC_OK:
             # the changes were made as given: set the same symbols in atoms
             # (ce_updater.cpp:338-347) without reading the lattice back
-            self._applied(changes)
-        else:
-            self._pending = None
-            self._sync_atoms()
+            # (no-op changes, old == new, are not recorded, :315-318)
+            atoms, occ, pending = self.atoms, self._occ, self._pending
+            for c in changes:
+                if c[1] != c[2]:
+                    i = int(c[0])
+                    atoms[i].symbol = c[2]
+                    occ[i] = sid[c[2]]
+                    if pending is not None:
+                        pending.append((i, c[1], c[2]))
+            return e.value
+        self._pending = None
+        self._sync_atoms()
         _capi.check(rc)
         return e.value
 
