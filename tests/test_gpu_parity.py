@@ -356,9 +356,10 @@ def test_resident_updater_equals_one_launch_per_call(gpu, case):
     """The resident kernel (calls as commands taken from mapped host memory
     by a workgroup that stays on the GPU) and one kernel launch per call give
     identical energies, correlation functions and lattices over the same
-    random calls: one, two and twelve changes per call (twelve go to a
-    launch), update_cf without a result, undos of up to 40 changes (more than
-    32 go to a launch), get_energy and get_cf after undos, new ECIs and a new
+    random calls: one, two, seven, eight and twelve changes per call (seven
+    fill a command, or follow restores that no longer fit beside them; eight
+    and twelve go to a launch), update_cf without a result, undos of up to 40
+    changes (more than 30 go to a launch), get_energy and get_cf after undos, new ECIs and a new
     occupation midway, and pauses longer than the resident kernel's 0.5 ms
     idle time, after which it is launched again."""
     import time
@@ -404,7 +405,7 @@ def test_resident_updater_equals_one_launch_per_call(gpu, case):
     for step in range(steps):
         kind = step % 8
         if kind in (0, 1, 2, 3):
-            chs = flips((1, 2, 12, 1)[kind])
+            chs = flips((1, 2, 12, 7)[kind])
             e = [u.calculate(chs) for u in ups]
             assert e[0] == e[1], step
         elif kind == 4:
@@ -421,6 +422,10 @@ def test_resident_updater_equals_one_launch_per_call(gpu, case):
             for ch in flips(3):
                 e = [u.calculate([ch]) for u in ups]
                 assert e[0] == e[1], step
+            cur = [int(v) for v in ups[0].get_occupation()]
+            chs = flips(8)
+            e = [u.calculate(chs) for u in ups]
+            assert e[0] == e[1], step
         if step % 3 == 0:
             for u in ups:
                 u.undo_changes()
