@@ -825,6 +825,13 @@ NoAllocLayout noalloc_layout(int n_cf, int n_sites) {
 }  // namespace
 
 // ============================================================================
+// Live updaters with the resident kernel switched on.  A resident kernel
+// holds the hardware queue of its stream while it is fed, and a process has
+// few of them, so an updater takes the resident path only while it is the
+// only such updater; with several (replicas driven in turn) each launches
+// per call instead of waiting for another's kernel to idle out.
+std::atomic<int> g_resident_updaters{0};
+
 struct cemc_updater {
   cemc_model* model = nullptr;
   hipStream_t stream = nullptr;
@@ -862,6 +869,7 @@ struct cemc_updater {
   UpdMailbox* h_mb = nullptr;
   UpdMailbox* d_mb = nullptr;
   bool resident = true;
+  bool res_counted = false;  // counted in g_resident_updaters
   bool res_running = false;
   int64_t res_seq = 0;
   std::chrono::steady_clock::time_point res_last{};
@@ -891,6 +899,7 @@ struct cemc_updater {
   bool has_vib = false;
   int num_threads = 1;
   ~cemc_updater() {
+    if (res_counted) g_resident_updaters.fetch_sub(1);
     // a resident kernel leaves on the quit command, or at the latest after
     // its idle time
     if (res_running && h_mb && __atomic_load_n(&h_mb->consumed, __ATOMIC_ACQUIRE) >= res_seq - kUpdRing) {
@@ -996,6 +1005,11 @@ void updater_post(cemc_updater* u, int32_t op, int32_t n, bool want, const int32
   for (int i = 0; i < 4 + kUpdCmdWords; ++i) __atomic_store_n(&c.w[i], tag | cw[i], __ATOMIC_RELEASE);
   u->res_seq = seq + 1;
   u->res_last = std::chrono::steady_clock::now();
+}
+
+// True when this call may go to the resident kernel (see g_resident_updaters).
+bool updater_resident_now(const cemc_updater* u) {
+  return u->resident && g_resident_updaters.load(std::memory_order_relaxed) == 1;
 }
 
 // Ends the resident kernel after the commands queued so far, so that other
@@ -1138,7 +1152,7 @@ void updater_flush(cemc_updater* u, const std::vector<int32_t>& packed, bool wan
   u->cf_slot = want_result ? dst : -1;
   if (want_result) u->cf_landing = true;
   u->slot_epoch[dst] = u->eci_epoch;
-  if (u->resident && kUpdWords * n <= kUpdCmdWords) {
+  if (updater_resident_now(u) && kUpdWords * n <= kUpdCmdWords) {
     // waiting restores ride in front of the changes when both fit
     if (2 * int(u->rs_sites.size()) + int(packed.size()) > kUpdCmdWords)
       updater_post(u, kUpdOpRestore, 0, false, nullptr, 0);
@@ -2263,6 +2277,8 @@ int cemc_updater_create(cemc_model* model, const int8_t* occupation, const doubl
       delete u;
       throw;
     }
+    u->res_counted = true;
+    g_resident_updaters.fetch_add(1);
     *out = u;
   });
 }
@@ -2394,7 +2410,7 @@ int cemc_updater_undo_changes(cemc_updater* u) {
       std::vector<int32_t> buf(sites);
       buf.insert(buf.end(), species.begin(), species.end());
       const int n = int(sites.size());
-      if (u->resident && n <= kUpdRestoreMax) {
+      if (updater_resident_now(u) && n <= kUpdRestoreMax) {
         // sent with the next command (sites in pop order, applied in order)
         if (int(u->rs_sites.size()) + n > kUpdRestoreMax) updater_post(u, kUpdOpRestore, 0, false, nullptr, 0);
         u->rs_sites.insert(u->rs_sites.end(), sites.begin(), sites.end());
@@ -2489,6 +2505,10 @@ int cemc_updater_set_resident(cemc_updater* u, int32_t on) {
     require(u != nullptr, "NULL updater");
     if (!on) updater_park(u);
     u->resident = on != 0;
+    if (u->resident != u->res_counted) {
+      g_resident_updaters.fetch_add(u->resident ? 1 : -1);
+      u->res_counted = u->resident;
+    }
   });
 }
 

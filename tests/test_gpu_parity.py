@@ -361,8 +361,12 @@ def test_resident_updater_equals_one_launch_per_call(gpu, case):
     and twelve go to a launch), update_cf without a result, undos of up to 40
     changes (more than 30 go to a launch), get_energy and get_cf after undos, new ECIs and a new
     occupation midway, and pauses longer than the resident kernel's 0.5 ms
-    idle time, after which it is launched again."""
+    idle time, after which it is launched again.  The resident path serves
+    an updater while it is the only live one with the path switched on, so
+    updaters left over from earlier tests are collected first."""
+    import gc
     import time
+    gc.collect()
     from cemc_amd.lattice import FCCSettings, SimpleAtoms, random_crystal_occupation, random_occupation, seeded_ecis
     from cemc_amd.updater import PyCEUpdater
     if case == "fcc_binary":
