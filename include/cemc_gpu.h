@@ -163,7 +163,9 @@ CEMC_API int cemc_updater_set_num_threads(cemc_updater* upd, int32_t num_threads
  * costs no kernel launch; it leaves 0.5 ms after the last command (and before
  * any other call of this updater that uses its stream), and while it runs
  * it holds that compute unit and the hardware queue of the updater's
- * stream.  on == 0: one kernel launch per call.  Results are identical. */
+ * stream, so it serves an updater only while that is the only live updater
+ * of the process with it on (several, used in turn: one launch per call
+ * each).  on == 0: one kernel launch per call.  Results are identical. */
 CEMC_API int cemc_updater_set_resident(cemc_updater* upd, int32_t on);
 /* PyCEUpdater.get_symbols (ce_updater.hpp:111) as species ids; n_sites values. */
 CEMC_API int cemc_updater_get_occupation(cemc_updater* upd, int8_t* occupation);
