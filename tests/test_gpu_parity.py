@@ -453,6 +453,52 @@ def test_resident_updater_equals_one_launch_per_call(gpu, case):
         u.close()
 
 
+def test_resident_updater_hands_over_when_a_second_updater_appears(gpu):
+    """An updater on the resident path while a second updater with the path
+    on is created (both then launch per call, the first after running the
+    restores that were waiting for its next command) and closed again (the
+    first returns to the resident path) gives the same energies, correlation
+    functions and lattice as an updater with one launch per call throughout."""
+    import gc
+    from cemc_amd.lattice import FCCSettings, SimpleAtoms, random_occupation, seeded_ecis
+    from cemc_amd.updater import PyCEUpdater
+    gc.collect()
+    s = FCCSettings(6, ["A", "B"], {2: 3, 3: 2})
+    eci = seeded_ecis(s, 8, 0.01)
+    occ = random_occupation(s.n_sites, 2, 5)
+    sp = list(s.species)
+    a = PyCEUpdater(SimpleAtoms([sp[v] for v in occ]), s, None, dict(eci))
+    ref = PyCEUpdater(SimpleAtoms([sp[v] for v in occ]), s, None, dict(eci))
+    ref.set_resident(False)
+    other = None
+    rng = np.random.Generator(np.random.Philox(33))
+    print("resident updater handing over", flush=True)
+    for step in range(90):
+        cur = [int(v) for v in a.get_occupation()]
+        i, j = (int(v) for v in rng.choice(s.n_sites, 2, replace=False))
+        if cur[i] != cur[j]:
+            chs = [(i, sp[cur[i]], sp[cur[j]]), (j, sp[cur[j]], sp[cur[i]])]
+        else:
+            chs = [(i, sp[cur[i]], sp[1 - cur[i]])]
+        assert a.calculate(chs) == ref.calculate(chs), step
+        if step % 3 != 2:
+            a.undo_changes()
+            ref.undo_changes()
+        else:
+            a.clear_history()
+            ref.clear_history()
+        if step == 30:
+            other = PyCEUpdater(SimpleAtoms([sp[v] for v in occ]), s, None, dict(eci))
+        if step == 60:
+            other.close()
+        if step % 10 == 0:
+            assert a.get_energy() == ref.get_energy(), step
+            assert np.array_equal(a.get_cf_array(), ref.get_cf_array()), step
+        assert np.array_equal(a.get_occupation(), ref.get_occupation()), step
+    a.close()
+    ref.close()
+
+
 @pytest.mark.parametrize("fname,n", [("binary_canonical_plain.npz", 3000), ("ternary_sgc.npz", 3000),
                                      ("quaternary_canonical.npz", 400)])
 def test_dropin_updater_loop_reproduces_reference_chain(gpu, fname, n):
